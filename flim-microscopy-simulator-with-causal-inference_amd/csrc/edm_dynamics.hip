@@ -1,0 +1,1378 @@
+// Dynamics kernels of the ED-MRDT per-timestep path on gfx950:
+// cell binning by counting sort (storage kept in cell order), the fused
+// force + Ermak-McCammon kernel with incremental coarse-grid density counts,
+// and the reaction pass (binding matching, dissociation, catalysis with Hill
+// feedback, recruitment with the index-order cap, detachment, conversion).
+//
+// Reference: ed_mrdt/dynamics.py (see the per-kernel citations).  Numerics
+// are restated so that FP64 runs with injected draws reproduce the
+// reference's events exactly; see edm_device.h.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+
+#include "edm_device.h"
+#include "edm_internal.h"
+
+namespace edm {
+
+static constexpr int kBlock = 256;
+static inline int nblk(int64_t n, int b = kBlock) { return (int)((n + b - 1) / b); }
+
+// ----------------------------------------------------------------------
+// Shared read-only view of the cell-ordered membrane
+// ----------------------------------------------------------------------
+template <typename Real>
+struct Cells {
+    const Real *x, *y;  // pre-move positions (slot order)
+    const int8_t* sp;
+    const uint8_t* fl;
+    const int32_t* id;
+    const uint32_t* start;  // [ncell + 1]
+    int ncx, ncy;
+    Real csx, csy, Lx, Ly, cut2;
+    bool px, py;
+    // reference pair bookkeeping (by particle index)
+    const uint64_t* pcnt;
+    const uint64_t* poff;
+    const int64_t* npairs;  // device scalar: length of the reference pair list
+    bool cap_active;
+};
+
+template <typename Real>
+__device__ __forceinline__ void cell_of(const Cells<Real>& C, Real x, Real y, int& cx, int& cy) {
+    cx = trunc_cell(x, C.csx, C.ncx);
+    cy = trunc_cell(y, C.csy, C.ncy);
+}
+
+// Neighbour cell k (0..8) of (cx, cy) in the reference's (dcx, dcy)
+// lexicographic order (dynamics.py:244-258); -1 when outside a wall.
+template <typename Real>
+__device__ __forceinline__ int nb_cell(const Cells<Real>& C, int cx, int cy, int k) {
+    int nx = cx + (k / 3) - 1;
+    int ny = cy + (k % 3) - 1;
+    if (C.px) nx = ((nx % C.ncx) + C.ncx) % C.ncx;
+    else if (nx < 0 || nx >= C.ncx) return -1;
+    if (C.py) ny = ((ny % C.ncy) + C.ncy) % C.ncy;
+    else if (ny < 0 || ny >= C.ncy) return -1;
+    return nx * C.ncy + ny;
+}
+
+// Rank of slot `sb` in the pair enumeration of slot `sa` (id[sa] < id[sb]):
+// reference order = neighbour cells in order, each chain in descending
+// particle index (slots inside a cell ascend by index, so walk backwards).
+template <typename Real>
+__device__ int64_t pair_rank(const Cells<Real>& C, int sa, int sb) {
+    const Real xa = C.x[sa], ya = C.y[sa];
+    const int ia = C.id[sa];
+    int cx, cy;
+    cell_of(C, xa, ya, cx, cy);
+    int64_t r = 0;
+    for (int k = 0; k < 9; ++k) {
+        const int c = nb_cell(C, cx, cy, k);
+        if (c < 0) continue;
+        for (int u = (int)C.start[c + 1] - 1; u >= (int)C.start[c]; --u) {
+            if (C.id[u] <= ia) continue;
+            Real dx, dy;
+            const Real d2 = min_image(xa, ya, C.x[u], C.y[u], C.Lx, C.Ly, C.px, C.py, dx, dy);
+            if (d2 < C.cut2) {
+                if (u == sb) return r;
+                ++r;
+            }
+        }
+    }
+    return -1;
+}
+
+// Does the pair (sa, sb) exist in the reference's capped pair list?
+// sa must hold the lower particle index.
+template <typename Real>
+__device__ __forceinline__ bool pair_in_list(const Cells<Real>& C, int sa, int sb) {
+    if (!C.cap_active) return true;
+    const int ia = C.id[sa];
+    const uint64_t off = C.poff[ia], cnt = C.pcnt[ia];
+    const uint64_t np = (uint64_t)*C.npairs;
+    if (off + cnt <= np) return true;
+    if (off >= np) return false;
+    return off + (uint64_t)pair_rank(C, sa, sb) < np;
+}
+
+template <typename Real>
+__device__ __forceinline__ int64_t pair_index(const Cells<Real>& C, int sa, int sb) {
+    return (int64_t)C.poff[C.id[sa]] + pair_rank(C, sa, sb);
+}
+
+// Density of a coarse-grid cell holding k sensor particles, as the
+// reference's sequential `density += inv_area` leaves it (FP64), or k*inv.
+template <typename Real>
+__device__ __forceinline__ Real density_of(int k, Real inv) {
+    if constexpr (sizeof(Real) == 8) {
+        double d = 0.0;
+        for (int i = 0; i < k; ++i) d += inv;
+        return d;
+    } else {
+        return (Real)k * inv;
+    }
+}
+
+// Coarse density grids (feedback rules) as seen by the reaction kernels.
+template <typename Real>
+struct Grids {
+    const int32_t* off;   // [n_grids * ns]
+    const int32_t* dims;  // [n_grids][2]
+    const Real* s;        // [n_grids][2] cell sizes
+    const Real* inv;      // [n_grids] 1/area
+    int32_t* counts;      // read (and direct updates)
+    int32_t* delta;       // deferred updates
+    int ns;
+};
+
+template <typename Real>
+__device__ __forceinline__ int grid_cell(const Grids<Real>& G, int g, Real x, Real y) {
+    const int nx = G.dims[2 * g], ny = G.dims[2 * g + 1];
+    const int gx = trunc_cell(x, G.s[2 * g], nx);
+    const int gy = trunc_cell(y, G.s[2 * g + 1], ny);
+    return gx * ny + gy;
+}
+
+struct FbDev {
+    int sensor, grid, inhibition;
+    double K, n, basal;
+};
+
+__host__ __device__ inline FbDev to_fb(const edm_feedback& f) {
+    return FbDev{f.sensor, f.grid, f.inhibition, f.K_half, f.n_hill, f.basal};
+}
+
+template <typename Real>
+__device__ __forceinline__ Real fb_factor(const Grids<Real>& G, const FbDev& f, Real x, Real y) {
+    if (f.sensor < 0) return (Real)1;
+    const int c = grid_cell(G, f.grid, x, y);
+    const int k = G.counts[G.off[f.grid * G.ns + f.sensor] + c];
+    const Real rho = density_of<Real>(k, G.inv[f.grid]);
+    return hill_factor<Real>(rho, (Real)f.K, (Real)f.n, f.inhibition != 0, (Real)f.basal);
+}
+
+// Update the kept density counts after species s of a particle at (x, y)
+// gains (+1) or loses (-1) one member.
+template <typename Real>
+__device__ __forceinline__ void count_update(const Grids<Real>& G, int n_grids, int s, Real x,
+                                             Real y, int d, bool deferred) {
+    for (int g = 0; g < n_grids; ++g) {
+        const int o = G.off[g * G.ns + s];
+        if (o < 0) continue;
+        const int c = grid_cell(G, g, x, y);
+        atomicAdd(&(deferred ? G.delta : G.counts)[o + c], d);
+    }
+}
+
+// ----------------------------------------------------------------------
+// Cell list: counting sort + per-cell index order + permutation
+// ----------------------------------------------------------------------
+template <typename Real>
+__global__ void k_bin(int n, const Real* __restrict__ x, const Real* __restrict__ y, Real csx,
+                      Real csy, int ncx, int ncy, uint32_t* __restrict__ key,
+                      uint32_t* __restrict__ rank, uint32_t* __restrict__ cnt, int32_t* counts,
+                      int n_counts) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n_counts) counts[s] = 0;  // density counts are rebuilt each step
+    if (s >= n) return;
+    const int cx = trunc_cell(x[s], csx, ncx);
+    const int cy = trunc_cell(y[s], csy, ncy);
+    const uint32_t c = (uint32_t)(cx * ncy + cy);
+    key[s] = c;
+    rank[s] = atomicAdd(&cnt[c], 1u);
+}
+
+__global__ void k_scatter(int n, const uint32_t* __restrict__ key, const uint32_t* __restrict__ rank,
+                          const uint32_t* __restrict__ start, uint32_t* __restrict__ perm) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    perm[start[key[s]] + rank[s]] = (uint32_t)s;
+}
+
+// One thread per cell: order the cell's particles by particle index (the
+// atomic ranks of k_bin are arbitrary) and move every field to its new slot.
+template <typename Real>
+__global__ void k_sort_permute(int ncell, const uint32_t* __restrict__ start, uint32_t* perm,
+                               uint32_t* cnt, DevArrays a) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    cnt[c] = 0;
+    const int lo = (int)start[c], hi = (int)start[c + 1];
+    for (int i = lo + 1; i < hi; ++i) {
+        const uint32_t v = perm[i];
+        const int32_t vid = a.id[v];
+        int j = i - 1;
+        while (j >= lo && a.id[perm[j]] > vid) {
+            perm[j + 1] = perm[j];
+            --j;
+        }
+        perm[j + 1] = v;
+    }
+    const Real* x = (const Real*)a.x;
+    const Real* y = (const Real*)a.y;
+    Real* xt = (Real*)a.xt;
+    Real* yt = (Real*)a.yt;
+    for (int t = lo; t < hi; ++t) {
+        const uint32_t s = perm[t];
+        xt[t] = x[s];
+        yt[t] = y[s];
+        a.sp_t[t] = a.sp[s];
+        a.fl_t[t] = a.fl[s];
+        const int32_t pid = a.id[s];
+        a.id_t[t] = pid;
+        a.bond_t[t] = a.bond[s];
+        a.ctmc_t[t] = a.ctmc[s];
+        a.dye_t[t] = a.dye[s];
+        a.slot_of[pid] = t;
+    }
+}
+
+// Reference pair-list bookkeeping: per particle index, the number of
+// partners j > i with d^2 < cutoff^2 (dynamics.py:237-276; dormant
+// particles included, as in the reference).
+template <typename Real>
+__global__ void k_pair_count(int n, Cells<Real> C, uint64_t* __restrict__ pcnt) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const Real xt = C.x[t], yt = C.y[t];
+    const int it = C.id[t];
+    int cx, cy;
+    cell_of(C, xt, yt, cx, cy);
+    uint64_t cnt = 0;
+    for (int k = 0; k < 9; ++k) {
+        const int c = nb_cell(C, cx, cy, k);
+        if (c < 0) continue;
+        for (int u = (int)C.start[c]; u < (int)C.start[c + 1]; ++u) {
+            if (C.id[u] <= it) continue;
+            Real dx, dy;
+            if (min_image(xt, yt, C.x[u], C.y[u], C.Lx, C.Ly, C.px, C.py, dx, dy) < C.cut2) ++cnt;
+        }
+    }
+    pcnt[it] = cnt;
+}
+
+__global__ void k_pair_total(int n, const uint64_t* pcnt, const uint64_t* poff, int64_t cap,
+                             int64_t* scal) {
+    const uint64_t total = n > 0 ? poff[n - 1] + pcnt[n - 1] : 0;
+    scal[1] = (int64_t)total;
+    scal[0] = (cap > 0 && (int64_t)total > cap) ? cap : (int64_t)total;
+}
+
+// Materialise the reference pair list (validation / parity helper).
+template <typename Real>
+__global__ void k_pair_list(int n, Cells<Real> C, int32_t* __restrict__ out, int64_t cap) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const Real xt = C.x[t], yt = C.y[t];
+    const int it = C.id[t];
+    int cx, cy;
+    cell_of(C, xt, yt, cx, cy);
+    int64_t p = (int64_t)C.poff[it];
+    const int64_t np = *C.npairs;
+    for (int k = 0; k < 9; ++k) {
+        const int c = nb_cell(C, cx, cy, k);
+        if (c < 0) continue;
+        for (int u = (int)C.start[c + 1] - 1; u >= (int)C.start[c]; --u) {
+            if (C.id[u] <= it) continue;
+            Real dx, dy;
+            if (min_image(xt, yt, C.x[u], C.y[u], C.Lx, C.Ly, C.px, C.py, dx, dy) < C.cut2) {
+                if (p < np && p < cap) {
+                    out[2 * p] = it;
+                    out[2 * p + 1] = C.id[u];
+                }
+                ++p;
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------------------------
+// Fused forces + Ermak-McCammon step (dynamics.py:666-816, 323-406, 441-519)
+// ----------------------------------------------------------------------
+template <typename Real>
+struct ForceParams {
+    const Real *sig, *eps, *kh, *Dbase;
+    int ns;
+    bool wca, harm;
+    int dfield;  // 0 regions, 1 sinusoidal
+    int n_regions;
+    const Real *rcx, *rcy, *rr, *rf;
+    Real smooth_w, sin_amp, sin_kx, sin_ky;
+    Real dt, max_disp;
+};
+
+// Pair force over r on the particle the separation (dx, dy) points to;
+// kind 0 = WCA, 1 = harmonic.  Returns false if the pair does not interact.
+template <typename Real>
+__device__ __forceinline__ bool pair_force(const ForceParams<Real>& F, int kind, int si, int su,
+                                           Real dsq, Real& f) {
+    const Real s = F.sig[si * F.ns + su];
+    if (kind == 0) {
+        const Real e = F.eps[si * F.ns + su];
+        if (s <= (Real)0 || e <= (Real)0) return false;
+        const Real rcut2 = s * s * (Real)1.2599210498948732;
+        if (dsq >= rcut2 || dsq < (Real)1e-30) return false;
+        const Real inv_r2 = (Real)1 / dsq;
+        const Real s2 = s * s * inv_r2;
+        const Real s6 = s2 * s2 * s2;
+        const Real s12 = s6 * s6;
+        Real fr = (Real)24 * e * ((Real)2 * s12 - s6) * inv_r2;
+        const Real fmax = (Real)2400 * e / (s * s);
+        if (fr > fmax) fr = fmax;
+        f = fr;
+        return true;
+    }
+    const Real k = F.kh[si * F.ns + su];
+    if (s <= (Real)0 || k <= (Real)0) return false;
+    if (dsq >= s * s || dsq < (Real)1e-30) return false;
+    const Real r = sqrt(dsq);
+    const Real fmag = k * (s - r);
+    f = fmag / r;
+    return true;
+}
+
+template <typename Real>
+__device__ __forceinline__ void dfield_eval(const ForceParams<Real>& F, int s, Real x, Real y,
+                                            Real& D, Real& gx, Real& gy) {
+    const Real base = F.Dbase[s];
+    if (F.dfield == 1) {
+        const Real sx = sin(F.sin_kx * x), cx = cos(F.sin_kx * x);
+        const Real sy = sin(F.sin_ky * y), cy = cos(F.sin_ky * y);
+        D = base * ((Real)1 + F.sin_amp * sx * sy);
+        gx = base * F.sin_amp * F.sin_kx * cx * sy;
+        gy = base * F.sin_amp * F.sin_ky * sx * cy;
+        return;
+    }
+    Real factor = (Real)1, gfx = (Real)0, gfy = (Real)0;
+    for (int r = 0; r < F.n_regions; ++r) {
+        const Real dxc = x - F.rcx[r], dyc = y - F.rcy[r];
+        const Real dsq = dxc * dxc + dyc * dyc;
+        const Real d = dsq > (Real)1e-30 ? sqrt(dsq) : (Real)1e-15;
+        const Real arg = (d - F.rr[r]) / F.smooth_w;
+        Real sig, dsig;
+        if (arg > (Real)100) {
+            sig = (Real)0;
+            dsig = (Real)0;
+        } else if (arg < -(Real)100) {
+            sig = (Real)1;
+            dsig = (Real)0;
+        } else {
+            const Real e = exp(arg);
+            const Real den = (Real)1 + e;
+            sig = (Real)1 / den;
+            dsig = -e / (den * den * F.smooth_w);
+        }
+        const Real local = (Real)1 + (F.rf[r] - (Real)1) * sig;
+        if (sig > (Real)0.01) {
+            factor = local;
+            const Real inv_d = (Real)1 / d;
+            gfx = (F.rf[r] - (Real)1) * dsig * dxc * inv_d;
+            gfy = (F.rf[r] - (Real)1) * dsig * dyc * inv_d;
+            break;
+        }
+    }
+    D = base * factor;
+    gx = base * gfx;
+    gy = base * gfy;
+}
+
+// kOrdered: sum pair forces in the reference's pair order (all partners
+// with a lower particle index ascending, then the particle's own
+// enumeration order), WCA over the whole list first, then harmonic — the
+// order compute_forces_wca / compute_forces_harmonic accumulate in.
+template <typename Real, bool kOrdered>
+__global__ __launch_bounds__(256) void k_force_bd(int n, Cells<Real> C, ForceParams<Real> F,
+                                                  Real* __restrict__ xn, Real* __restrict__ yn,
+                                                  const double* __restrict__ noise_hook, Rng rng,
+                                                  uint32_t step, Grids<Real> G, int n_grids,
+                                                  uint32_t keep_mask) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool take_any = false;
+    Real nxp = 0, nyp = 0;
+    int si = 0;
+    if (t < n) {
+        const Real xt = C.x[t], yt = C.y[t];
+        si = C.sp[t];
+        const bool alive = (C.fl[t] & EDM_FLAG_ALIVE) != 0;
+        nxp = xt;
+        nyp = yt;
+        if (alive) {
+            const int it = C.id[t];
+            int cx, cy;
+            cell_of(C, xt, yt, cx, cy);
+            Real Fx = 0, Fy = 0;
+            for (int kind = 0; kind < 2; ++kind) {
+                if (kind == 0 && !F.wca) continue;
+                if (kind == 1 && !F.harm) continue;
+                if constexpr (kOrdered) {
+                    // partners with lower index, ascending
+                    int last = -1;
+                    for (;;) {
+                        int best = INT_MAX, bslot = -1;
+                        for (int k = 0; k < 9; ++k) {
+                            const int c = nb_cell(C, cx, cy, k);
+                            if (c < 0) continue;
+                            for (int u = (int)C.start[c]; u < (int)C.start[c + 1]; ++u) {
+                                const int iu = C.id[u];
+                                if (iu >= it || iu <= last || iu >= best) continue;
+                                Real dx, dy;
+                                if (min_image(xt, yt, C.x[u], C.y[u], C.Lx, C.Ly, C.px, C.py, dx,
+                                              dy) < C.cut2) {
+                                    best = iu;
+                                    bslot = u;
+                                }
+                            }
+                        }
+                        if (bslot < 0) break;
+                        last = best;
+                        if (!(C.fl[bslot] & EDM_FLAG_ALIVE)) continue;
+                        if (!pair_in_list(C, bslot, t)) continue;
+                        Real dx, dy;
+                        const Real dsq = min_image(xt, yt, C.x[bslot], C.y[bslot], C.Lx, C.Ly,
+                                                   C.px, C.py, dx, dy);
+                        Real f;
+                        if (pair_force(F, kind, C.sp[bslot], si, dsq, f)) {
+                            Fx += f * dx;
+                            Fy += f * dy;
+                        }
+                    }
+                    // partners with higher index, reference enumeration order
+                    for (int k = 0; k < 9; ++k) {
+                        const int c = nb_cell(C, cx, cy, k);
+                        if (c < 0) continue;
+                        for (int u = (int)C.start[c + 1] - 1; u >= (int)C.start[c]; --u) {
+                            if (C.id[u] <= it) continue;
+                            Real dx, dy;
+                            const Real dsq = min_image(xt, yt, C.x[u], C.y[u], C.Lx, C.Ly, C.px,
+                                                       C.py, dx, dy);
+                            if (dsq >= C.cut2) continue;
+                            if (!(C.fl[u] & EDM_FLAG_ALIVE)) continue;
+                            if (!pair_in_list(C, t, u)) continue;
+                            Real f;
+                            if (pair_force(F, kind, si, C.sp[u], dsq, f)) {
+                                Fx += f * dx;
+                                Fy += f * dy;
+                            }
+                        }
+                    }
+                } else {
+                    for (int k = 0; k < 9; ++k) {
+                        const int c = nb_cell(C, cx, cy, k);
+                        if (c < 0) continue;
+                        for (int u = (int)C.start[c]; u < (int)C.start[c + 1]; ++u) {
+                            if (u == t) continue;
+                            Real dx, dy;
+                            const Real dsq = min_image(xt, yt, C.x[u], C.y[u], C.Lx, C.Ly, C.px,
+                                                       C.py, dx, dy);
+                            if (dsq >= C.cut2) continue;
+                            if (!(C.fl[u] & EDM_FLAG_ALIVE)) continue;
+                            if (C.cap_active) {
+                                const bool lower = C.id[u] < it;
+                                if (!(lower ? pair_in_list(C, u, t) : pair_in_list(C, t, u)))
+                                    continue;
+                            }
+                            Real f;
+                            if (pair_force(F, kind, si, C.sp[u], dsq, f)) {
+                                Fx += f * dx;
+                                Fy += f * dy;
+                            }
+                        }
+                    }
+                }
+            }
+            Real D, gx, gy;
+            dfield_eval(F, si, xt, yt, D, gx, gy);
+            Real n0, n1;
+            if (noise_hook) {
+                n0 = (Real)noise_hook[2 * (int64_t)it];
+                n1 = (Real)noise_hook[2 * (int64_t)it + 1];
+            } else {
+                normal2<Real>(rng.draw(step, (uint32_t)it, 0u, kTagNoise), n0, n1);
+            }
+            const Real sn = sqrt((Real)2 * D * F.dt);
+            Real dx = gx * F.dt;
+            Real dy = gy * F.dt;
+            dx += sn * n0;
+            dy += sn * n1;
+            dx += D * Fx * F.dt;
+            dy += D * Fy * F.dt;
+            const Real dr2 = dx * dx + dy * dy;
+            if (dr2 > F.max_disp * F.max_disp) {
+                const Real dr = sqrt(dr2);
+                const Real sc = F.max_disp / dr;
+                dx *= sc;
+                dy *= sc;
+            }
+            Real px_ = xt + dx, py_ = yt + dy;
+            if (C.px) px_ = py_mod(px_, C.Lx);
+            else {
+                if (px_ < (Real)0) px_ = -px_;
+                else if (px_ >= C.Lx) px_ = (Real)2 * C.Lx - px_;
+            }
+            if (C.py) py_ = py_mod(py_, C.Ly);
+            else {
+                if (py_ < (Real)0) py_ = -py_;
+                else if (py_ >= C.Ly) py_ = (Real)2 * C.Ly - py_;
+            }
+            nxp = px_;
+            nyp = py_;
+            take_any = ((keep_mask >> si) & 1u) != 0;
+        }
+        xn[t] = nxp;
+        yn[t] = nyp;
+    }
+    // density counts of the kept (sensor) species at the post-move position
+    for (int g = 0; g < n_grids; ++g) {
+        int o = -1;
+        if (take_any) o = G.off[g * G.ns + si];
+        const bool take = take_any && o >= 0;
+        const int idx = take ? o + grid_cell(G, g, nxp, nyp) : 0;
+        wave_aggregated_add(G.counts, idx, take, 1);
+    }
+}
+
+// ----------------------------------------------------------------------
+// Reaction kernels
+// ----------------------------------------------------------------------
+struct Draws {
+    const double* base;  // the step's hook draws (nullptr = Philox)
+    int64_t off;         // offset of this reaction's arrays
+    int64_t n_pairs;     // length of the per-pair array (BIND, CAT)
+    int n;               // particles
+};
+
+template <typename Real>
+struct Post {
+    Real *xn, *yn;  // post-move positions (writable)
+    int8_t* sp;
+    uint8_t* fl;
+    int32_t* bond;
+    const int32_t* slot_of;
+};
+
+__device__ __forceinline__ bool cas_byte(int8_t* base, int idx, int8_t expect, int8_t val) {
+    const uintptr_t addr = (uintptr_t)(base + idx);
+    uint32_t* w = (uint32_t*)(addr & ~(uintptr_t)3);
+    const int sh = (int)(addr & 3) * 8;
+    uint32_t old = *((volatile uint32_t*)w);
+    for (;;) {
+        if ((int8_t)((old >> sh) & 0xffu) != expect) return false;
+        const uint32_t nw = (old & ~(0xffu << sh)) | ((uint32_t)(uint8_t)val << sh);
+        const uint32_t prev = atomicCAS(w, old, nw);
+        if (prev == old) return true;
+        old = prev;
+    }
+}
+
+// Candidate binding edges A(ra) - B(rb): both alive and free, in the
+// (capped) pair list, within contact after the move, and accepted by their
+// per-pair uniform (dynamics.py:529-606).
+template <typename Real>
+__global__ void k_bind_edges(int n, Cells<Real> C, Post<Real> P, int ra, int rb, Real contact2,
+                             Real prob, Draws dr, Rng rng, uint32_t step, uint32_t rxi,
+                             unsigned long long* ekey, int32_t* ea, int32_t* eb, unsigned int* ne,
+                             unsigned int cap) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    if (C.sp[t] != ra || !(C.fl[t] & EDM_FLAG_ALIVE) || P.bond[t] >= 0) return;
+    const Real xt = C.x[t], yt = C.y[t];
+    const int it = C.id[t];
+    int cx, cy;
+    cell_of(C, xt, yt, cx, cy);
+    for (int k = 0; k < 9; ++k) {
+        const int c = nb_cell(C, cx, cy, k);
+        if (c < 0) continue;
+        for (int u = (int)C.start[c]; u < (int)C.start[c + 1]; ++u) {
+            if (u == t || C.sp[u] != rb) continue;
+            const int iu = C.id[u];
+            if (ra == rb && iu < it) continue;  // A is the lower index of an A+A pair
+            if (!(C.fl[u] & EDM_FLAG_ALIVE) || P.bond[u] >= 0) continue;
+            Real dx, dy;
+            if (min_image(xt, yt, C.x[u], C.y[u], C.Lx, C.Ly, C.px, C.py, dx, dy) >= C.cut2)
+                continue;
+            const bool tlow = it < iu;
+            const int sa = tlow ? t : u, sb = tlow ? u : t;
+            if (!pair_in_list(C, sa, sb)) continue;
+            if (min_image(P.xn[t], P.yn[t], P.xn[u], P.yn[u], C.Lx, C.Ly, C.px, C.py, dx, dy) >
+                contact2)
+                continue;
+            unsigned long long key;
+            double uu;
+            if (dr.base) {
+                const int64_t p = pair_index(C, sa, sb);
+                key = (unsigned long long)p;
+                uu = dr.base[dr.off + p];
+            } else {
+                const uint32_t lo = (uint32_t)min(it, iu), hi = (uint32_t)max(it, iu);
+                key = ((unsigned long long)lo << 32) | hi;
+                uu = (double)Uni<Real>::first(rng.draw(step, lo, hi, (kTagBind << 16) | rxi));
+            }
+            if (uu < (double)prob) {
+                const unsigned int e = atomicAdd(ne, 1u);
+                if (e < cap) {
+                    ekey[e] = key;
+                    ea[e] = t;
+                    eb[e] = u;
+                }
+            }
+        }
+    }
+}
+
+// Greedy matching in key order = the reference's sequential pass: an edge
+// is accepted when it is the smallest live edge at both endpoints; repeat
+// until no live edge is left.  One workgroup.
+template <typename Real>
+__global__ __launch_bounds__(1024) void k_bind_match(Post<Real> P, const int32_t* __restrict__ id,
+                                                     int ra, int rb, int prod,
+                                                     const unsigned long long* ekey,
+                                                     const int32_t* ea, const int32_t* eb,
+                                                     const unsigned int* ne, unsigned int cap,
+                                                     unsigned long long* best,
+                                                     unsigned long long* ctr, Grids<Real> G,
+                                                     int n_grids) {
+    const unsigned int m = min(*ne, cap);
+    volatile int8_t* vsp = P.sp;
+    volatile uint8_t* vfl = P.fl;
+    volatile int32_t* vbd = P.bond;
+    for (unsigned int round = 0; round <= m; ++round) {
+        for (unsigned int e = threadIdx.x; e < m; e += blockDim.x) {
+            const int a = ea[e], b = eb[e];
+            const bool ok = vsp[a] == ra && vsp[b] == rb && (vfl[a] & EDM_FLAG_ALIVE) &&
+                            (vfl[b] & EDM_FLAG_ALIVE) && vbd[a] < 0 && vbd[b] < 0;
+            if (ok) {
+                atomicExch(&best[a], ~0ull);
+                atomicExch(&best[b], ~0ull);
+            }
+        }
+        __threadfence();
+        __syncthreads();
+        for (unsigned int e = threadIdx.x; e < m; e += blockDim.x) {
+            const int a = ea[e], b = eb[e];
+            const bool ok = vsp[a] == ra && vsp[b] == rb && (vfl[a] & EDM_FLAG_ALIVE) &&
+                            (vfl[b] & EDM_FLAG_ALIVE) && vbd[a] < 0 && vbd[b] < 0;
+            if (ok) {
+                atomicMin(&best[a], ekey[e]);
+                atomicMin(&best[b], ekey[e]);
+            }
+        }
+        __threadfence();
+        __syncthreads();
+        int live = 0;
+        for (unsigned int e = threadIdx.x; e < m; e += blockDim.x) {
+            const int a = ea[e], b = eb[e];
+            const bool ok = vsp[a] == ra && vsp[b] == rb && (vfl[a] & EDM_FLAG_ALIVE) &&
+                            (vfl[b] & EDM_FLAG_ALIVE) && vbd[a] < 0 && vbd[b] < 0;
+            if (!ok) continue;
+            const unsigned long long k = ekey[e];
+            if (atomicAdd(&best[a], 0ull) == k && atomicAdd(&best[b], 0ull) == k) {
+                vsp[a] = (int8_t)prod;
+                vfl[b] = vfl[b] & ~EDM_FLAG_ALIVE;
+                vbd[a] = id[b];
+                vbd[b] = id[a];
+                atomicAdd(&ctr[0], 1ull);
+                count_update(G, n_grids, ra, P.xn[a], P.yn[a], -1, false);
+                count_update(G, n_grids, prod, P.xn[a], P.yn[a], +1, false);
+                count_update(G, n_grids, rb, P.xn[b], P.yn[b], -1, false);
+            } else {
+                live = 1;
+            }
+        }
+        __threadfence();
+        if (!__syncthreads_or(live)) break;
+    }
+}
+
+// dynamics.py:610-654 apply_dissociation
+template <typename Real>
+__global__ void k_dissoc(int n, const int32_t* __restrict__ id, Post<Real> P, int prod, int ra,
+                         int rb, Real prob, Draws dr, Rng rng, uint32_t step, uint32_t rxi,
+                         unsigned long long* ctr, Grids<Real> G, int n_grids) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool hit = false;
+    if (t < n && (P.fl[t] & EDM_FLAG_ALIVE) && P.sp[t] == prod && P.bond[t] >= 0) {
+        const int it = id[t];
+        const double u = dr.base ? dr.base[dr.off + dr.n_pairs + it]
+                                 : (double)Uni<Real>::first(
+                                       rng.draw(step, (uint32_t)it, 0u, (kTagDissoc << 16) | rxi));
+        if (u < (double)prob) {
+            const int s = P.slot_of[P.bond[t]];
+            P.sp[t] = (int8_t)ra;
+            P.sp[s] = (int8_t)rb;
+            P.fl[s] = P.fl[s] | EDM_FLAG_ALIVE;
+            P.xn[s] = P.xn[t];
+            P.yn[s] = P.yn[t];
+            P.bond[t] = -1;
+            P.bond[s] = -1;
+            count_update(G, n_grids, prod, P.xn[t], P.yn[t], -1, false);
+            count_update(G, n_grids, ra, P.xn[t], P.yn[t], +1, false);
+            count_update(G, n_grids, rb, P.xn[t], P.yn[t], +1, false);
+            hit = true;
+        }
+    }
+    wave_count(&ctr[1], hit);
+}
+
+// dynamics.py:1094-1112 update_adaptation_field, reading the sensor density
+template <typename Real>
+__global__ void k_adapt(Grids<Real> G, int grid, int species, double* w, double rate, int ncells) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    const double rho = (double)density_of<Real>(G.counts[G.off[grid * G.ns + species] + c],
+                                                G.inv[grid]);
+    w[c] += rate * (rho - w[c]);
+}
+
+// dynamics.py:826-1090 catalysis: enzyme-centred sweep of the pair list;
+// the substrate flips once however many of its pairs succeed (byte CAS),
+// which is the outcome of the reference's sequential loop.
+template <typename Real>
+__global__ void k_cat(int n, Cells<Real> C, Post<Real> P, int enz, int sub, int prod, Real contact2,
+                      Real prob, FbDev fb, const double* w, int wnx, int wny, Real Kw, Real nw,
+                      Draws dr, Rng rng, uint32_t step, uint32_t rxi, unsigned long long* ctr,
+                      Grids<Real> G, int n_grids, bool deferred) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int hits = 0;
+    if (t < n && C.sp[t] == enz && (C.fl[t] & EDM_FLAG_ALIVE)) {
+        const Real xt = C.x[t], yt = C.y[t];
+        const int it = C.id[t];
+        int cx, cy;
+        cell_of(C, xt, yt, cx, cy);
+        for (int k = 0; k < 9; ++k) {
+            const int c = nb_cell(C, cx, cy, k);
+            if (c < 0) continue;
+            for (int u = (int)C.start[c]; u < (int)C.start[c + 1]; ++u) {
+                if (u == t) continue;
+                if (P.sp[u] != sub || !(C.fl[u] & EDM_FLAG_ALIVE)) continue;
+                Real dx, dy;
+                if (min_image(xt, yt, C.x[u], C.y[u], C.Lx, C.Ly, C.px, C.py, dx, dy) >= C.cut2)
+                    continue;
+                const int iu = C.id[u];
+                const bool tlow = it < iu;
+                const int sa = tlow ? t : u, sb = tlow ? u : t;
+                if (!pair_in_list(C, sa, sb)) continue;
+                if (min_image(P.xn[t], P.yn[t], P.xn[u], P.yn[u], C.Lx, C.Ly, C.px, C.py, dx,
+                              dy) > contact2)
+                    continue;
+                Real pe = prob;
+                const Real sx = P.xn[u], sy = P.yn[u];
+                if (fb.sensor >= 0) {
+                    const Real hf = fb_factor(G, fb, sx, sy);
+                    if (w) {
+                        const int gnx = G.dims[2 * fb.grid], gny = G.dims[2 * fb.grid + 1];
+                        int gx = trunc_cell(sx, G.s[2 * fb.grid], gnx);
+                        int gy = trunc_cell(sy, G.s[2 * fb.grid + 1], gny);
+                        gx = clamp_cell(gx, wnx);
+                        gy = clamp_cell(gy, wny);
+                        Real af = (Real)1;
+                        if (Kw > (Real)0) {
+                            const Real wr = pow((Real)w[gx * wny + gy] / Kw, nw);
+                            af = (Real)1 / ((Real)1 + wr);
+                        }
+                        pe = prob * hf * af;
+                    } else {
+                        pe = prob * hf;
+                    }
+                }
+                double uu;
+                if (dr.base) {
+                    uu = dr.base[dr.off + pair_index(C, sa, sb)];
+                } else {
+                    const uint32_t lo = (uint32_t)min(it, iu), hi = (uint32_t)max(it, iu);
+                    uu = (double)Uni<Real>::first(rng.draw(step, lo, hi, (kTagCat << 16) | rxi));
+                }
+                if (uu < (double)pe) {
+                    if (cas_byte(P.sp, u, (int8_t)sub, (int8_t)prod)) {
+                        ++hits;
+                        count_update(G, n_grids, sub, sx, sy, -1, deferred);
+                        count_update(G, n_grids, prod, sx, sy, +1, deferred);
+                    }
+                }
+            }
+        }
+    }
+    unsigned long long v = hits;
+    // wavefront reduction then one atomic
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+    if (lane_id() == 0 && v) atomicAdd(&ctr[2], v);
+}
+
+// Recruitment candidates (dynamics.py:1230-1458): dormant particles of the
+// species whose acceptance test passes at their candidate position.
+template <typename Real>
+__global__ void k_recruit_flag(int n, const int32_t* __restrict__ id, Post<Real> P, int target,
+                               Real prob, Real Lx, Real Ly, FbDev f1, FbDev f2, Draws dr, Rng rng,
+                               uint32_t step, uint32_t rxi, Grids<Real> G, int32_t* cid,
+                               int32_t* cslot, double* cxo, double* cyo, unsigned int* nc) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    if ((P.fl[t] & EDM_FLAG_ALIVE) || P.sp[t] != target) return;
+    const int it = id[t];
+    Real ux, uy;
+    double ua;
+    if (dr.base) {
+        ua = dr.base[dr.off + it];
+        ux = (Real)dr.base[dr.off + dr.n + 2 * (int64_t)it];
+        uy = (Real)dr.base[dr.off + dr.n + 2 * (int64_t)it + 1];
+    } else {
+        const U4 r = rng.draw(step, (uint32_t)it, 0u, (kTagRecruitAcc << 16) | rxi);
+        ua = (double)Uni<Real>::first(r);
+        const U4 q = rng.draw(step, (uint32_t)it, 1u, (kTagRecruitPos << 16) | rxi);
+        ux = Uni<Real>::first(q);
+        uy = Uni<Real>::second(q);
+    }
+    const Real cx = ux * Lx, cy = uy * Ly;
+    Real pe = prob;
+    if (f1.sensor >= 0 && f2.sensor >= 0) {
+        const Real h1 = fb_factor(G, f1, cx, cy);
+        const Real h2 = fb_factor(G, f2, cx, cy);
+        pe = prob * h1 * h2;
+    } else if (f1.sensor >= 0) {
+        pe = prob * fb_factor(G, f1, cx, cy);
+    }
+    if (ua < (double)pe) {
+        const unsigned int k = atomicAdd(nc, 1u);
+        cid[k] = it;
+        cslot[k] = t;
+        cxo[k] = (double)cx;
+        cyo[k] = (double)cy;
+    }
+}
+
+// Accept the max_recruits candidates with the lowest particle index (the
+// reference stops its index-order scan after max_recruits), one workgroup.
+template <typename Real>
+__global__ __launch_bounds__(1024) void k_recruit_apply(Post<Real> P, int target, int maxr, int n,
+                                                        const int32_t* cid, const int32_t* cslot,
+                                                        const double* cxo, const double* cyo,
+                                                        const unsigned int* nc,
+                                                        unsigned long long* ctr, Grids<Real> G,
+                                                        int n_grids) {
+    __shared__ int s_cnt;
+    const int m = (int)*nc;
+    int thr = INT_MAX;
+    if (m > maxr) {
+        int lo = 0, hi = n - 1;
+        while (lo < hi) {
+            const int mid = lo + (hi - lo) / 2;
+            if (threadIdx.x == 0) s_cnt = 0;
+            __syncthreads();
+            int local = 0;
+            for (int k = threadIdx.x; k < m; k += blockDim.x) local += cid[k] <= mid;
+            if (local) atomicAdd(&s_cnt, local);
+            __syncthreads();
+            const int cnt = s_cnt;
+            __syncthreads();
+            if (cnt >= maxr) hi = mid;
+            else lo = mid + 1;
+        }
+        thr = lo;
+    }
+    int acc = 0;
+    for (int k = threadIdx.x; k < m; k += blockDim.x) {
+        if (cid[k] > thr || maxr <= 0) continue;
+        const int t = cslot[k];
+        P.fl[t] = P.fl[t] | EDM_FLAG_ALIVE;
+        P.xn[t] = (Real)cxo[k];
+        P.yn[t] = (Real)cyo[k];
+        count_update(G, n_grids, target, P.xn[t], P.yn[t], +1, false);
+        ++acc;
+    }
+    if (acc) atomicAdd(&ctr[3], (unsigned long long)acc);
+}
+
+// dynamics.py:1462-1555 detachment (deferred when the species is its own sensor)
+template <typename Real>
+__global__ void k_detach(int n, const int32_t* __restrict__ id, Post<Real> P, int target, Real prob,
+                         FbDev fb, Draws dr, Rng rng, uint32_t step, uint32_t rxi,
+                         unsigned long long* ctr, Grids<Real> G, int n_grids, bool deferred) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool hit = false;
+    if (t < n && (P.fl[t] & EDM_FLAG_ALIVE) && P.sp[t] == target) {
+        const int it = id[t];
+        const Real x = P.xn[t], y = P.yn[t];
+        const Real pe = fb.sensor >= 0 ? prob * fb_factor(G, fb, x, y) : prob;
+        const double u = dr.base ? dr.base[dr.off + it]
+                                 : (double)Uni<Real>::first(
+                                       rng.draw(step, (uint32_t)it, 0u, (kTagDetach << 16) | rxi));
+        if (u < (double)pe) {
+            P.fl[t] = P.fl[t] & ~EDM_FLAG_ALIVE;
+            count_update(G, n_grids, target, x, y, -1, deferred);
+            hit = true;
+        }
+    }
+    wave_count(&ctr[4], hit);
+}
+
+// dynamics.py:1566-1657 conversion
+template <typename Real>
+__global__ void k_convert(int n, const int32_t* __restrict__ id, Post<Real> P, int from, int to,
+                          Real prob, FbDev fb, Draws dr, Rng rng, uint32_t step, uint32_t rxi,
+                          unsigned long long* ctr, Grids<Real> G, int n_grids, bool deferred) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool hit = false;
+    if (t < n && (P.fl[t] & EDM_FLAG_ALIVE) && P.sp[t] == from) {
+        const int it = id[t];
+        const Real x = P.xn[t], y = P.yn[t];
+        const Real pe = fb.sensor >= 0 ? prob * fb_factor(G, fb, x, y) : prob;
+        const double u = dr.base ? dr.base[dr.off + it]
+                                 : (double)Uni<Real>::first(
+                                       rng.draw(step, (uint32_t)it, 0u, (kTagConvert << 16) | rxi));
+        if (u < (double)pe) {
+            P.sp[t] = (int8_t)to;
+            count_update(G, n_grids, from, x, y, -1, deferred);
+            count_update(G, n_grids, to, x, y, +1, deferred);
+            hit = true;
+        }
+    }
+    wave_count(&ctr[5], hit);
+}
+
+__global__ void k_fold(int n, int32_t* counts, int32_t* delta) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    counts[i] += delta[i];
+    delta[i] = 0;
+}
+
+template <typename Real>
+__global__ void k_density_out(Grids<Real> G, int grid, int species, double* out, int ncells) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    out[c] = (double)density_of<Real>(G.counts[G.off[grid * G.ns + species] + c], G.inv[grid]);
+}
+
+// ----------------------------------------------------------------------
+// Host side
+// ----------------------------------------------------------------------
+template <typename Real>
+static Cells<Real> make_cells(edm_membrane* m) {
+    Cells<Real> C;
+    C.x = (const Real*)m->a.x;
+    C.y = (const Real*)m->a.y;
+    C.sp = m->a.sp;
+    C.fl = m->a.fl;
+    C.id = m->a.id;
+    C.start = m->cell_start;
+    C.ncx = m->p.ncx;
+    C.ncy = m->p.ncy;
+    C.csx = (Real)m->p.csx;
+    C.csy = (Real)m->p.csy;
+    C.Lx = (Real)m->p.Lx;
+    C.Ly = (Real)m->p.Ly;
+    C.cut2 = (Real)m->p.cutoff * (Real)m->p.cutoff;
+    C.px = m->p.periodic_x != 0;
+    C.py = m->p.periodic_y != 0;
+    C.pcnt = m->pair_cnt;
+    C.poff = m->pair_off;
+    C.npairs = m->d_scal;
+    C.cap_active = m->need_pair_offsets && m->n_pairs < m->n_pairs_found;
+    return C;
+}
+
+template <typename Real>
+static Grids<Real> make_grids(edm_membrane* m) {
+    Grids<Real> G;
+    G.off = m->d_count_off;
+    G.dims = m->d_grid_dims;
+    G.s = (const Real*)m->d_grid_s;
+    G.inv = (const Real*)m->d_grid_s + 2 * m->grids.size();
+    G.counts = m->counts;
+    G.delta = m->delta;
+    G.ns = m->p.n_species;
+    return G;
+}
+
+template <typename Real>
+static Post<Real> make_post(edm_membrane* m) {
+    Post<Real> P;
+    P.xn = (Real*)m->a.xn;
+    P.yn = (Real*)m->a.yn;
+    P.sp = m->a.sp;
+    P.fl = m->a.fl;
+    P.bond = m->a.bond;
+    P.slot_of = m->a.slot_of;
+    return P;
+}
+
+template <typename T>
+static int dalloc(T** p, size_t count) {
+    EDM_HIP(hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T) + 16));
+    return 0;
+}
+
+template <typename Real>
+static int upload_real(void** dst, const double* src, size_t n) {
+    std::vector<Real> tmp(std::max<size_t>(n, 1));
+    for (size_t i = 0; i < n; ++i) tmp[i] = (Real)src[i];
+    EDM_HIP(hipMalloc(dst, tmp.size() * sizeof(Real)));
+    EDM_HIP(hipMemcpy(*dst, tmp.data(), tmp.size() * sizeof(Real), hipMemcpyHostToDevice));
+    return 0;
+}
+
+template <typename Real>
+static int create_t(edm_membrane* m) {
+    const edm_params& p = m->p;
+    const int n = m->n, ns = p.n_species;
+    const size_t rb = sizeof(Real);
+    void** rbufs[6] = {&m->a.x, &m->a.y, &m->a.xn, &m->a.yn, &m->a.xt, &m->a.yt};
+    for (auto b : rbufs) EDM_HIP(hipMalloc(b, (size_t)n * rb + 16));
+    if (dalloc(&m->a.sp, n + 4) || dalloc(&m->a.sp_t, n + 4) || dalloc(&m->a.fl, n + 4) ||
+        dalloc(&m->a.fl_t, n + 4) || dalloc(&m->a.id, n) || dalloc(&m->a.id_t, n) ||
+        dalloc(&m->a.bond, n) || dalloc(&m->a.bond_t, n) || dalloc(&m->a.ctmc, n + 4) ||
+        dalloc(&m->a.ctmc_t, n + 4) || dalloc(&m->a.dye, n + 4) || dalloc(&m->a.dye_t, n + 4) ||
+        dalloc(&m->a.slot_of, n))
+        return EDM_EHIP;
+    m->ncell = p.ncx * p.ncy;
+    if (dalloc(&m->key, n) || dalloc(&m->rank, n) || dalloc(&m->perm, n) ||
+        dalloc(&m->cell_cnt, m->ncell + 1) || dalloc(&m->cell_start, m->ncell + 1))
+        return EDM_EHIP;
+    EDM_HIP(hipMemset(m->cell_cnt, 0, (m->ncell + 1) * sizeof(uint32_t)));
+    // scan temp storage: the larger of the cell scan and the pair-count scan
+    size_t b1 = 0, b2 = 0, b3 = 0;
+    EDM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b1, m->cell_cnt, m->cell_start,
+                                             m->ncell + 1, m->stream));
+    uint64_t* dummy64 = nullptr;
+    EDM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, dummy64, dummy64, n, m->stream));
+    int64_t* dummyi = nullptr;
+    EDM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b3, dummyi, dummyi, n, m->stream));
+    m->cub_tmp_bytes = std::max(b1, std::max(b2, b3)) + 256;
+    EDM_HIP(hipMalloc(&m->cub_tmp, m->cub_tmp_bytes));
+    if (dalloc(&m->pair_cnt, n) || dalloc(&m->pair_off, n) || dalloc(&m->d_scal, 16)) return EDM_EHIP;
+    EDM_HIP(hipMemset(m->d_scal, 0, 16 * sizeof(int64_t)));
+    // parameters
+    int rc = 0;
+    rc |= upload_real<Real>(&m->sig, p.sigma, (size_t)ns * ns);
+    rc |= upload_real<Real>(&m->eps, p.epsilon, (size_t)ns * ns);
+    rc |= upload_real<Real>(&m->kh, p.k_harm, (size_t)ns * ns);
+    rc |= upload_real<Real>(&m->Dbase, p.D_base, ns);
+    std::vector<double> z(1, 0.0);
+    rc |= upload_real<Real>(&m->rcx, p.n_regions ? p.rcx : z.data(), p.n_regions);
+    rc |= upload_real<Real>(&m->rcy, p.n_regions ? p.rcy : z.data(), p.n_regions);
+    rc |= upload_real<Real>(&m->rr, p.n_regions ? p.rr : z.data(), p.n_regions);
+    rc |= upload_real<Real>(&m->rf, p.n_regions ? p.rf : z.data(), p.n_regions);
+    if (rc) return EDM_EHIP;
+    // density grids: keep counts for every (grid, sensor) read by a feedback
+    const int ng = (int)m->grids.size();
+    m->count_off.assign((size_t)std::max(ng, 1) * ns, -1);
+    auto keep = [&](int g, int s) {
+        if (g < 0 || s < 0) return;
+        m->count_off[(size_t)g * ns + s] = 0;
+    };
+    for (auto& r : m->rx) {
+        keep(r.fb1.grid, r.fb1.sensor);
+        keep(r.fb2.grid, r.fb2.sensor);
+    }
+    if (p.adapt_grid >= 0) keep(p.adapt_grid, p.adapt_species);
+    int total = 0;
+    for (int g = 0; g < ng; ++g)
+        for (int s = 0; s < ns; ++s)
+            if (m->count_off[(size_t)g * ns + s] == 0) {
+                m->count_off[(size_t)g * ns + s] = total;
+                total += m->grids[g].nx * m->grids[g].ny;
+                m->sensor_mask_all |= 1u << s;
+            }
+    m->n_count = total;
+    if (dalloc(&m->counts, total) || dalloc(&m->delta, total) ||
+        dalloc(&m->d_count_off, m->count_off.size()) || dalloc(&m->d_grid_dims, 2 * std::max(ng, 1)))
+        return EDM_EHIP;
+    EDM_HIP(hipMemset(m->counts, 0, (size_t)std::max(total, 1) * 4));
+    EDM_HIP(hipMemset(m->delta, 0, (size_t)std::max(total, 1) * 4));
+    EDM_HIP(hipMemcpy(m->d_count_off, m->count_off.data(), m->count_off.size() * 4,
+                      hipMemcpyHostToDevice));
+    std::vector<int32_t> dims(2 * std::max(ng, 1), 1);
+    std::vector<double> gs(3 * std::max(ng, 1), 1.0);
+    for (int g = 0; g < ng; ++g) {
+        dims[2 * g] = m->grids[g].nx;
+        dims[2 * g + 1] = m->grids[g].ny;
+        gs[2 * g] = m->grids[g].sx;
+        gs[2 * g + 1] = m->grids[g].sy;
+        const double area = m->grids[g].sx * m->grids[g].sy;
+        gs[2 * ng + g] = area > 0.0 ? 1.0 / area : 0.0;
+    }
+    EDM_HIP(hipMemcpy(m->d_grid_dims, dims.data(), dims.size() * 4, hipMemcpyHostToDevice));
+    if (upload_real<Real>(&m->d_grid_s, gs.data(), gs.size())) return EDM_EHIP;
+    if (p.adapt_grid >= 0) {
+        const auto& g = m->grids[p.adapt_grid];
+        if (dalloc(&m->w_adapt, (size_t)g.nx * g.ny)) return EDM_EHIP;
+        EDM_HIP(hipMemset(m->w_adapt, 0, (size_t)g.nx * g.ny * sizeof(double)));
+    }
+    // reaction scratch
+    m->edge_cap = (size_t)std::max(n, 1024);
+    if (dalloc(&m->edge_key, m->edge_cap) || dalloc(&m->edge_a, m->edge_cap) ||
+        dalloc(&m->edge_b, m->edge_cap) || dalloc(&m->n_edges, 4) || dalloc(&m->best, n) ||
+        dalloc(&m->cand_id, n) || dalloc(&m->cand_slot, n) || dalloc(&m->cand_x, n) ||
+        dalloc(&m->cand_y, n) || dalloc(&m->n_cand, 4) || dalloc(&m->d_ctr, 8))
+        return EDM_EHIP;
+    EDM_HIP(hipMemset(m->d_ctr, 0, 8 * sizeof(unsigned long long)));
+    EDM_HIP(hipMemset(m->n_edges, 0, 16));
+    EDM_HIP(hipMemset(m->n_cand, 0, 16));
+    return 0;
+}
+
+int dyn_create(edm_membrane* m) { return m->fp64 ? create_t<double>(m) : create_t<float>(m); }
+
+int64_t hook_draws_per_step(const edm_membrane* m, int64_t n_pairs) {
+    int64_t total = 2 * (int64_t)m->n;
+    for (const auto& r : m->rx) {
+        switch (r.kind) {
+            case EDM_RX_BIND: total += n_pairs + m->n; break;
+            case EDM_RX_CAT: total += n_pairs; break;
+            case EDM_RX_RECRUIT: total += 3 * (int64_t)m->n; break;
+            case EDM_RX_DETACH:
+            case EDM_RX_CONVERT: total += m->n; break;
+            default: break;
+        }
+    }
+    return total;
+}
+
+template <typename Real>
+static int begin_t(edm_membrane* m) {
+    const int n = m->n;
+    hipStream_t st = m->stream;
+    // bin on the current positions, counting sort, order cells by index
+    hipLaunchKernelGGL(k_bin<Real>, dim3(nblk(std::max(n, m->n_count))), dim3(kBlock), 0, st, n,
+                       (const Real*)m->a.x, (const Real*)m->a.y, (Real)m->p.csx, (Real)m->p.csy,
+                       m->p.ncx, m->p.ncy, m->key, m->rank, m->cell_cnt, m->counts, m->n_count);
+    size_t tb = m->cub_tmp_bytes;
+    EDM_HIP(hipcub::DeviceScan::ExclusiveSum(m->cub_tmp, tb, m->cell_cnt, m->cell_start,
+                                             m->ncell + 1, st));
+    hipLaunchKernelGGL(k_scatter, dim3(nblk(n)), dim3(kBlock), 0, st, n, m->key, m->rank,
+                       m->cell_start, m->perm);
+    hipLaunchKernelGGL(k_sort_permute<Real>, dim3(nblk(m->ncell)), dim3(kBlock), 0, st, m->ncell,
+                       m->cell_start, m->perm, m->cell_cnt, m->a);
+    std::swap(m->a.x, m->a.xt);
+    std::swap(m->a.y, m->a.yt);
+    std::swap(m->a.sp, m->a.sp_t);
+    std::swap(m->a.fl, m->a.fl_t);
+    std::swap(m->a.id, m->a.id_t);
+    std::swap(m->a.bond, m->a.bond_t);
+    std::swap(m->a.ctmc, m->a.ctmc_t);
+    std::swap(m->a.dye, m->a.dye_t);
+    if (m->need_pair_offsets) {
+        Cells<Real> C = make_cells<Real>(m);
+        hipLaunchKernelGGL(k_pair_count<Real>, dim3(nblk(n)), dim3(kBlock), 0, st, n, C,
+                           m->pair_cnt);
+        tb = m->cub_tmp_bytes;
+        EDM_HIP(hipcub::DeviceScan::ExclusiveSum(m->cub_tmp, tb, m->pair_cnt,
+                                                 m->pair_off, n, st));
+        hipLaunchKernelGGL(k_pair_total, dim3(1), dim3(1), 0, st, n, m->pair_cnt,
+                           (const uint64_t*)m->pair_off, (int64_t)m->p.pair_cap, m->d_scal);
+        int64_t sc[2];
+        EDM_HIP(hipMemcpyAsync(sc, m->d_scal, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        EDM_HIP(hipStreamSynchronize(st));
+        m->n_pairs = sc[0];
+        m->n_pairs_found = sc[1];
+    }
+    EDM_HIP(hipGetLastError());
+    return 0;
+}
+
+int dyn_step_begin(edm_membrane* m) { return m->fp64 ? begin_t<double>(m) : begin_t<float>(m); }
+
+template <typename Real>
+static int finish_t(edm_membrane* m, const double* draws, bool hook) {
+    const int n = m->n;
+    hipStream_t st = m->stream;
+    const edm_params& p = m->p;
+    Cells<Real> C = make_cells<Real>(m);
+    Grids<Real> G = make_grids<Real>(m);
+    Post<Real> P = make_post<Real>(m);
+    const int ng = (int)m->grids.size();
+    Rng rng{(uint32_t)(p.seed & 0xffffffffu), (uint32_t)(p.seed >> 32)};
+    const uint32_t step = m->step;
+
+    ForceParams<Real> F;
+    F.sig = (const Real*)m->sig;
+    F.eps = (const Real*)m->eps;
+    F.kh = (const Real*)m->kh;
+    F.Dbase = (const Real*)m->Dbase;
+    F.ns = p.n_species;
+    F.wca = p.use_wca != 0;
+    F.harm = p.use_harmonic != 0;
+    F.dfield = p.dfield;
+    F.n_regions = p.n_regions;
+    F.rcx = (const Real*)m->rcx;
+    F.rcy = (const Real*)m->rcy;
+    F.rr = (const Real*)m->rr;
+    F.rf = (const Real*)m->rf;
+    F.smooth_w = (Real)p.smooth_w;
+    F.sin_amp = (Real)p.sin_amp;
+    F.sin_kx = (Real)p.sin_kx;
+    F.sin_ky = (Real)p.sin_ky;
+    F.dt = (Real)p.dt;
+    F.max_disp = (Real)p.max_disp;
+    const double* noise = hook ? draws : nullptr;
+    if (m->time_force) EDM_HIP(hipEventRecord(m->kev[0], st));
+    if (hook && m->fp64)
+        hipLaunchKernelGGL((k_force_bd<Real, true>), dim3(nblk(n)), dim3(kBlock), 0, st, n, C, F,
+                           (Real*)m->a.xn, (Real*)m->a.yn, noise, rng, step, G, ng,
+                           m->sensor_mask_all);
+    else
+        hipLaunchKernelGGL((k_force_bd<Real, false>), dim3(nblk(n)), dim3(kBlock), 0, st, n, C, F,
+                           (Real*)m->a.xn, (Real*)m->a.yn, noise, rng, step, G, ng,
+                           m->sensor_mask_all);
+    if (m->time_force) EDM_HIP(hipEventRecord(m->kev[1], st));
+
+    int64_t off = 2 * (int64_t)n;
+    const int64_t npairs = m->n_pairs;
+    for (size_t ri = 0; ri < m->rx.size(); ++ri) {
+        const edm_reaction& r = m->rx[ri];
+        Draws dr{hook ? draws : nullptr, off, npairs, n};
+        const uint32_t rxi = (uint32_t)ri;
+        const FbDev f1 = to_fb(r.fb1), f2 = to_fb(r.fb2);
+        switch (r.kind) {
+            case EDM_RX_BIND: {
+                EDM_HIP(hipMemsetAsync(m->n_edges, 0, sizeof(unsigned int), st));
+                const Real c2 = (Real)r.contact * (Real)r.contact;
+                hipLaunchKernelGGL(k_bind_edges<Real>, dim3(nblk(n)), dim3(kBlock), 0, st, n, C, P,
+                                   r.a, r.b, c2, (Real)r.prob, dr, rng, step, rxi, m->edge_key,
+                                   m->edge_a, m->edge_b, m->n_edges, (unsigned int)m->edge_cap);
+                hipLaunchKernelGGL(k_bind_match<Real>, dim3(1), dim3(1024), 0, st, P,
+                                   (const int32_t*)m->a.id, r.a, r.b, r.c,
+                                   (const unsigned long long*)m->edge_key,
+                                   (const int32_t*)m->edge_a, (const int32_t*)m->edge_b,
+                                   (const unsigned int*)m->n_edges, (unsigned int)m->edge_cap,
+                                   m->best, m->d_ctr, G, ng);
+                hipLaunchKernelGGL(k_dissoc<Real>, dim3(nblk(n)), dim3(kBlock), 0, st, n,
+                                   (const int32_t*)m->a.id, P, r.c, r.a, r.b, (Real)r.prob2, dr,
+                                   rng, step, rxi, m->d_ctr, G, ng);
+                off += npairs + n;
+                break;
+            }
+            case EDM_RX_ADAPT: {
+                const auto& g = m->grids[p.adapt_grid];
+                hipLaunchKernelGGL(k_adapt<Real>, dim3(nblk(g.nx * g.ny)), dim3(kBlock), 0, st, G,
+                                   p.adapt_grid, p.adapt_species, m->w_adapt, p.dt / p.adapt_tau,
+                                   g.nx * g.ny);
+                break;
+            }
+            case EDM_RX_CAT: {
+                const bool defer = f1.sensor >= 0 && (f1.sensor == r.b || f1.sensor == r.c);
+                const Real c2 = (Real)r.contact * (Real)r.contact;
+                int wnx = 1, wny = 1;
+                const double* w = nullptr;
+                if (r.use_adapt && p.adapt_grid >= 0) {
+                    w = m->w_adapt;
+                    wnx = m->grids[p.adapt_grid].nx;
+                    wny = m->grids[p.adapt_grid].ny;
+                }
+                hipLaunchKernelGGL(k_cat<Real>, dim3(nblk(n)), dim3(kBlock), 0, st, n, C, P, r.a,
+                                   r.b, r.c, c2, (Real)r.prob, f1, w, wnx, wny, (Real)p.adapt_K,
+                                   (Real)p.adapt_n, dr, rng, step, rxi, m->d_ctr, G, ng, defer);
+                if (defer && m->n_count)
+                    hipLaunchKernelGGL(k_fold, dim3(nblk(m->n_count)), dim3(kBlock), 0, st,
+                                       m->n_count, m->counts, m->delta);
+                off += npairs;
+                break;
+            }
+            case EDM_RX_RECRUIT: {
+                if (r.prob > 0.0) {
+                    EDM_HIP(hipMemsetAsync(m->n_cand, 0, sizeof(unsigned int), st));
+                    hipLaunchKernelGGL(k_recruit_flag<Real>, dim3(nblk(n)), dim3(kBlock), 0, st, n,
+                                       (const int32_t*)m->a.id, P, r.a, (Real)r.prob, (Real)p.Lx,
+                                       (Real)p.Ly, f1, f2, dr, rng, step, rxi, G, m->cand_id,
+                                       m->cand_slot, m->cand_x, m->cand_y, m->n_cand);
+                    hipLaunchKernelGGL(k_recruit_apply<Real>, dim3(1), dim3(1024), 0, st, P, r.a,
+                                       r.max_recruits, n, (const int32_t*)m->cand_id,
+                                       (const int32_t*)m->cand_slot, (const double*)m->cand_x,
+                                       (const double*)m->cand_y, (const unsigned int*)m->n_cand,
+                                       m->d_ctr, G, ng);
+                }
+                off += 3 * (int64_t)n;
+                break;
+            }
+            case EDM_RX_DETACH: {
+                if (r.prob > 0.0) {
+                    const bool defer = f1.sensor >= 0 && f1.sensor == r.a;
+                    hipLaunchKernelGGL(k_detach<Real>, dim3(nblk(n)), dim3(kBlock), 0, st, n,
+                                       (const int32_t*)m->a.id, P, r.a, (Real)r.prob, f1, dr, rng,
+                                       step, rxi, m->d_ctr, G, ng, defer);
+                    if (defer && m->n_count)
+                        hipLaunchKernelGGL(k_fold, dim3(nblk(m->n_count)), dim3(kBlock), 0, st,
+                                           m->n_count, m->counts, m->delta);
+                }
+                off += n;
+                break;
+            }
+            case EDM_RX_CONVERT: {
+                if (r.prob > 0.0) {
+                    const bool defer =
+                        f1.sensor >= 0 && (f1.sensor == r.a || f1.sensor == r.b);
+                    hipLaunchKernelGGL(k_convert<Real>, dim3(nblk(n)), dim3(kBlock), 0, st, n,
+                                       (const int32_t*)m->a.id, P, r.a, r.b, (Real)r.prob, f1, dr,
+                                       rng, step, rxi, m->d_ctr, G, ng, defer);
+                    if (defer && m->n_count)
+                        hipLaunchKernelGGL(k_fold, dim3(nblk(m->n_count)), dim3(kBlock), 0, st,
+                                           m->n_count, m->counts, m->delta);
+                }
+                off += n;
+                break;
+            }
+            default: break;
+        }
+    }
+    // post-move positions become the current positions
+    std::swap(m->a.x, m->a.xn);
+    std::swap(m->a.y, m->a.yn);
+    m->step += 1;
+    EDM_HIP(hipGetLastError());
+    return 0;
+}
+
+int dyn_step_finish(edm_membrane* m, const double* d_draws_dev, bool hook) {
+    return m->fp64 ? finish_t<double>(m, d_draws_dev, hook) : finish_t<float>(m, d_draws_dev, hook);
+}
+
+template <typename Real>
+static int pairs_t(edm_membrane* m, int32_t* host_pairs, int64_t cap, int64_t* n_out) {
+    // The pair list of the last begin: positions (x, y) were swapped to the
+    // post-move buffer by finish, so this must run between begin and finish.
+    if (!m->need_pair_offsets) return EDM_ESTATE;
+    const int64_t np = std::min(m->n_pairs, cap);
+    int32_t* d = nullptr;
+    EDM_HIP(hipMalloc(&d, (size_t)std::max<int64_t>(np, 1) * 2 * sizeof(int32_t)));
+    Cells<Real> C = make_cells<Real>(m);
+    hipLaunchKernelGGL(k_pair_list<Real>, dim3(nblk(m->n)), dim3(kBlock), 0, m->stream, m->n, C, d,
+                       np);
+    EDM_HIP(hipMemcpyAsync(host_pairs, d, (size_t)np * 2 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                           m->stream));
+    EDM_HIP(hipStreamSynchronize(m->stream));
+    EDM_HIP(hipFree(d));
+    *n_out = m->n_pairs;
+    return 0;
+}
+
+int dyn_pairs(edm_membrane* m, int32_t* pairs, int64_t cap, int64_t* n_out) {
+    if (!m->begun) return EDM_ESTATE;
+    return m->fp64 ? pairs_t<double>(m, pairs, cap, n_out) : pairs_t<float>(m, pairs, cap, n_out);
+}
+
+template <typename Real>
+static int density_t(edm_membrane* m, int32_t species, int32_t grid, double* out) {
+    if (grid < 0 || grid >= (int)m->grids.size() || species < 0 || species >= m->p.n_species)
+        return EDM_EINVAL;
+    if (m->count_off[(size_t)grid * m->p.n_species + species] < 0) return EDM_EINVAL;
+    const int nc = m->grids[grid].nx * m->grids[grid].ny;
+    double* d = nullptr;
+    EDM_HIP(hipMalloc(&d, (size_t)nc * sizeof(double)));
+    Grids<Real> G = make_grids<Real>(m);
+    hipLaunchKernelGGL(k_density_out<Real>, dim3(nblk(nc)), dim3(kBlock), 0, m->stream, G, grid,
+                       species, d, nc);
+    EDM_HIP(hipMemcpyAsync(out, d, (size_t)nc * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+    EDM_HIP(hipStreamSynchronize(m->stream));
+    EDM_HIP(hipFree(d));
+    return 0;
+}
+
+int dyn_density(edm_membrane* m, int32_t species, int32_t grid, double* out) {
+    return m->fp64 ? density_t<double>(m, species, grid, out)
+                   : density_t<float>(m, species, grid, out);
+}
+
+}  // namespace edm

@@ -1,0 +1,171 @@
+// Internal declarations of libedm_hip.so: the membrane handle and the
+// kernel-parameter blocks shared by the dynamics and photon translation
+// units.  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "edm_hip.h"
+
+#define EDM_FLAG_ALIVE 1u
+#define EDM_FLAG_DYE 2u
+#define EDM_FLAG_BLEACHED 4u
+
+namespace edm {
+
+void set_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define EDM_HIP(call)                                           \
+    do {                                                        \
+        hipError_t _e = (call);                                 \
+        if (_e != hipSuccess) return ::edm::hip_fail(_e, #call); \
+    } while (0)
+
+// Device buffers of one membrane.  Particle arrays are in slot order (cell
+// order of the last cell-list build); `id` maps slot -> particle index and
+// `slot_of` maps back.  Positions: (x, y) are the current positions between
+// steps and the pre-move positions during a step; (xn, yn) receive the
+// post-move positions and are swapped in at the end of the step.
+struct DevArrays {
+    void *x = nullptr, *y = nullptr, *xn = nullptr, *yn = nullptr, *xt = nullptr, *yt = nullptr;
+    int8_t *sp = nullptr, *sp_t = nullptr;
+    uint8_t *fl = nullptr, *fl_t = nullptr;
+    int32_t *id = nullptr, *id_t = nullptr;
+    int32_t *bond = nullptr, *bond_t = nullptr;
+    int8_t *ctmc = nullptr, *ctmc_t = nullptr;
+    int8_t *dye = nullptr, *dye_t = nullptr;
+    int32_t* slot_of = nullptr;
+};
+
+}  // namespace edm
+
+struct edm_membrane {
+    edm_params p{};
+    std::vector<edm_grid> grids;
+    std::vector<edm_reaction> rx;
+    std::vector<int64_t> draw_off;  // per reaction: offset of its draws in a hook step
+    int64_t draws_noise = 0;        // 2n
+    hipStream_t stream = nullptr;
+    int device = 0;
+    int n = 0, ncell = 0;
+    bool fp64 = true;
+    size_t rs = 8;
+    edm::DevArrays a;
+
+    // cell list
+    uint32_t *key = nullptr, *rank = nullptr, *cell_cnt = nullptr, *cell_start = nullptr,
+             *perm = nullptr;
+    void* cub_tmp = nullptr;
+    size_t cub_tmp_bytes = 0;
+
+    // parameters (Real arrays)
+    void *sig = nullptr, *eps = nullptr, *kh = nullptr, *Dbase = nullptr;
+    void *rcx = nullptr, *rcy = nullptr, *rr = nullptr, *rf = nullptr;
+
+    // density counts: counts[count_off[g * ns + s] + cell]; -1 = not kept
+    std::vector<int32_t> count_off;
+    int32_t* d_count_off = nullptr;
+    int32_t* d_grid_dims = nullptr;  // [n_grids][2]
+    void* d_grid_s = nullptr;        // Real [n_grids][2]
+    int32_t *counts = nullptr, *delta = nullptr;
+    int n_count = 0;
+    uint32_t sensor_mask_all = 0;  // species kept on any grid
+
+    // reference pair bookkeeping (parity runs and the faithful pair cap)
+    uint64_t* pair_cnt = nullptr;  // by particle index
+    uint64_t* pair_off = nullptr;  // exclusive scan of pair_cnt
+    int64_t n_pairs_found = 0, n_pairs = 0;
+    bool need_pair_offsets = false;
+
+    // step state
+    uint32_t step = 0;
+    bool begun = false;
+    unsigned long long* d_ctr = nullptr;  // 8 counters
+
+    // hook draws staging
+    double* d_draws = nullptr;
+    size_t draws_cap = 0;
+
+    // reaction scratch
+    unsigned long long* edge_key = nullptr;
+    int32_t *edge_a = nullptr, *edge_b = nullptr;
+    unsigned int* n_edges = nullptr;
+    size_t edge_cap = 0;
+    unsigned long long* best = nullptr;  // per slot
+    int32_t *cand_id = nullptr, *cand_slot = nullptr;
+    double *cand_x = nullptr, *cand_y = nullptr;
+    unsigned int* n_cand = nullptr;
+    double* w_adapt = nullptr;
+
+    // photophysics
+    bool has_pp = false;
+    edm_photo_params pp{};
+    double* d_pp = nullptr;  // packed photophysics tables (see edm_photons.hip)
+    double* k_fret = nullptr;
+    int32_t* acc = nullptr;      // by slot: acceptor particle index or -1
+    int32_t* ph_nd = nullptr;    // by particle index
+    int32_t* ph_nf = nullptr;
+    int64_t* ph_cnt = nullptr;   // by particle index, then scanned in place
+    int64_t* ph_off = nullptr;
+    int32_t* ph_pid = nullptr;
+    double* ph_t = nullptr;
+    int8_t* ph_ch = nullptr;
+    uint8_t* ph_fret = nullptr;
+    int64_t ph_cap = 0, n_ph = 0;
+
+    // imaging
+    bool has_ip = false;
+    edm_image_params ip{};
+    uint32_t* flim = nullptr;
+    size_t flim_elems = 0;
+    double *pcx = nullptr, *pcy = nullptr;
+    // hook-run photon staging
+    int64_t* det_flag = nullptr;
+    int64_t* det_pos = nullptr;
+    uint32_t *sv_key = nullptr, *sv_key2 = nullptr, *sv_val = nullptr, *sv_val2 = nullptr;
+    int64_t n_surv = 0, n_asg = 0;
+    int32_t *asg_px = nullptr, *asg_py = nullptr;
+    int64_t* asg_flag = nullptr;
+    int64_t* asg_pos = nullptr;
+    double* asg_t = nullptr;
+    int8_t* asg_ch = nullptr;
+    int32_t *sv_px = nullptr, *sv_py = nullptr;
+    uint32_t* pix_tot = nullptr;
+    int64_t* d_scal = nullptr;  // device scalars for reductions
+
+    // timing
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> kev;
+    bool time_force = false;
+    double force_ms_acc = 0.0;
+};
+
+namespace edm {
+// dynamics translation unit
+int dyn_create(edm_membrane* m);
+int dyn_step_begin(edm_membrane* m);
+int dyn_step_finish(edm_membrane* m, const double* d_draws_dev, bool hook);
+int dyn_pairs(edm_membrane* m, int32_t* pairs, int64_t cap, int64_t* n_out);
+int dyn_density(edm_membrane* m, int32_t species, int32_t grid, double* out);
+int64_t hook_draws_per_step(const edm_membrane* m, int64_t n_pairs);
+// photon translation unit
+int ph_alloc(edm_membrane* m);
+int ph_fret(edm_membrane* m);
+int ph_hook(edm_membrane* m, const double* u_pois, const double* u_trans, const double* u_arr,
+            int64_t n_arr, int64_t* n_photons);
+int ph_fused_image(edm_membrane* m, int64_t* n_emitted, int64_t* n_detected);
+int img_alloc(edm_membrane* m);
+int img_detect(edm_membrane* m, const double* u_qe, int64_t n, int64_t* n_surv);
+int img_assign(edm_membrane* m, const double* u_pix, int64_t n, int64_t* n_asg);
+int img_bin(edm_membrane* m, const double* jitter, int64_t n);
+int frame_totals(edm_membrane* m, uint32_t* totals);
+int frame_add(edm_membrane* m, const int64_t* flat, int64_t n);
+int frame_noise(edm_membrane* m, int64_t* n_dark, int64_t* n_ap);
+int frame_reset(edm_membrane* m);
+int stage_draws(edm_membrane* m, const double* host, int64_t n);
+}  // namespace edm

@@ -1,0 +1,864 @@
+// Photophysics and virtual-microscope kernels of the ED-MRDT path on gfx950:
+// bond / proximity FRET rates, the PSS photon kernel as count -> scan ->
+// emit (so each photon's arrival draw index is the reference's sequential
+// index), the emitter-centred PSF pixel draw, IRF jitter + TCSPC binning,
+// and the frame-end detector noise.  A fused per-emitter kernel runs the
+// whole photon path with Philox draws and no photon list.
+//
+// Reference: ed_mrdt/photophysics.py and ed_mrdt/microscope.py.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "edm_device.h"
+#include "edm_internal.h"
+
+namespace edm {
+
+static constexpr int kB = 256;
+static inline int nb(int64_t n, int b = kB) { return (int)std::max<int64_t>(1, (n + b - 1) / b); }
+
+template <typename T>
+static int dalloc(T** p, size_t count) {
+    EDM_HIP(hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T) + 16));
+    return 0;
+}
+
+// ----------------------------------------------------------------------
+// FRET rates (photophysics.py:1021-1114)
+// ----------------------------------------------------------------------
+__global__ void k_fret_zero(int n, double* k_fret, int32_t* acc) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    k_fret[t] = 0.0;
+    acc[t] = -1;
+}
+
+// photophysics.py:529-588, one bond-mode FRET map
+__global__ void k_fret_bond(int n, const edm_photo_params* pp, int j, const uint8_t* fl,
+                            const int8_t* dye, const int32_t* bond, const int32_t* slot_of,
+                            double* k_fret, int32_t* acc) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    if (!(fl[t] & EDM_FLAG_DYE) || (fl[t] & EDM_FLAG_BLEACHED)) return;
+    if (dye[t] != pp->fm_donor[j]) return;
+    const int partner = bond[t];
+    if (partner < 0) return;
+    if (pp->fm_intra[j]) {
+        k_fret[t] = pp->fm_kfret[j];
+        acc[t] = -1;
+    } else {
+        const int s = slot_of[partner];
+        if ((fl[s] & EDM_FLAG_DYE) && dye[s] == pp->fm_acceptor[j]) {
+            k_fret[t] = pp->fm_kfret[j];
+            acc[t] = partner;
+        }
+    }
+}
+
+// photophysics.py:592-683, brute-force proximity search (as the reference)
+template <typename Real>
+__global__ void k_fret_prox(int n, const edm_photo_params* pp, int j, const Real* x, const Real* y,
+                            const int8_t* sp, const uint8_t* fl, const int8_t* dye, Real Lx,
+                            Real Ly, bool px, bool py, double* k_fret, int32_t* acc) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint8_t f = fl[t];
+    if (!(f & EDM_FLAG_DYE) || !(f & EDM_FLAG_ALIVE) || (f & EDM_FLAG_BLEACHED)) return;
+    if (dye[t] != pp->fm_donor[j]) return;
+    const Real r2 = (Real)pp->fm_radius[j] * (Real)pp->fm_radius[j];
+    const int target = pp->fm_target[j];
+    const Real xi = x[t], yi = y[t];
+    bool found = false;
+    for (int u = 0; u < n && !found; ++u) {
+        if (u == t) continue;
+        if (!(fl[u] & EDM_FLAG_ALIVE) || sp[u] != target) continue;
+        Real dx = x[u] - xi, dy = y[u] - yi;
+        if (px) {
+            if (dx > Lx / (Real)2) dx -= Lx;
+            else if (dx < -Lx / (Real)2) dx += Lx;
+        }
+        if (py) {
+            if (dy > Ly / (Real)2) dy -= Ly;
+            else if (dy < -Ly / (Real)2) dy += Ly;
+        }
+        if (dx * dx + dy * dy < r2) found = true;
+    }
+    k_fret[t] = found ? pp->fm_k_hi[j] : pp->fm_k_lo[j];
+    acc[t] = -1;
+}
+
+// ----------------------------------------------------------------------
+// PSS kernel (photophysics.py:339-483)
+// ----------------------------------------------------------------------
+struct PssOut {
+    int nd, nf;      // donor photons, acceptor photons actually emitted
+    double tau;      // donor arrival-time scale
+    int ty;          // dye type
+};
+
+// Evaluate one emitter: photon counts and the CTMC transition.  Returns
+// false when the particle does not take part (no dye, dormant, bleached).
+__device__ __forceinline__ bool pss_eval(const edm_photo_params* pp, uint8_t& fl, int8_t& ctmc,
+                                         int8_t dye, double kf, int32_t acc_id, double u_p,
+                                         double u_t, PssOut& o) {
+    if (!(fl & EDM_FLAG_DYE) || !(fl & EDM_FLAG_ALIVE) || (fl & EDM_FLAG_BLEACHED)) return false;
+    const int ty = dye;
+    if (ty < 0 || ty >= pp->n_fluoro) return false;
+    const int s0 = pp->gs[ty], s1 = pp->es[ty], t1 = pp->t1[ty], bl = pp->bl[ty];
+    const int cur = ctmc;
+    int pss;
+    if (cur == s0 || cur == s1) pss = 0;
+    else if (t1 >= 0 && cur == t1) pss = 1;
+    else if (bl >= 0 && cur == bl) {
+        fl = fl | EDM_FLAG_BLEACHED;
+        return false;
+    } else pss = 0;
+    const bool has_fret = kf > 0.0;
+    double md, mf, tau, T00, T01;
+    if (has_fret) {
+        const double tD = pp->tau[ty];
+        tau = tD / (1.0 + kf * tD);
+        const double kb = pp->k_bound[ty];
+        double frac = 0.0;
+        if (kb > 0.0) {
+            frac = kf / kb;
+            if (frac > 1.0) frac = 1.0;
+        }
+        md = pp->mean_ph[ty][pss] * (1.0 - frac) + pp->mean_ph_f[ty][pss] * frac;
+        mf = pp->mean_fr[ty][pss] * (1.0 - frac) + pp->mean_fr_f[ty][pss] * frac;
+        T00 = pp->T_step[ty][pss * 3 + 0] * (1.0 - frac) + pp->T_step_f[ty][pss * 3 + 0] * frac;
+        T01 = pp->T_step[ty][pss * 3 + 1] * (1.0 - frac) + pp->T_step_f[ty][pss * 3 + 1] * frac;
+    } else {
+        md = pp->mean_ph[ty][pss];
+        mf = pp->mean_fr[ty][pss];
+        tau = pp->tau[ty];
+        T00 = pp->T_step[ty][pss * 3 + 0];
+        T01 = pp->T_step[ty][pss * 3 + 1];
+    }
+    const int nd = poisson_lcg(md, u_p);
+    int nfr = 0;
+    if (mf > 0.0 && has_fret) nfr = poisson_lcg(mf, u_p * 0.5 + 0.25);
+    o.nd = nd;
+    o.nf = (nfr > 0 && acc_id >= 0) ? nfr : 0;
+    o.tau = tau;
+    o.ty = ty;
+    if (u_t < T00) ctmc = (int8_t)s0;
+    else if (u_t < T00 + T01) ctmc = (int8_t)(t1 >= 0 ? t1 : s0);
+    else {
+        ctmc = (int8_t)(bl >= 0 ? bl : s0);
+        fl = fl | EDM_FLAG_BLEACHED;
+    }
+    return true;
+}
+
+__global__ void k_pss_count(int n, const edm_photo_params* pp, const int32_t* id, uint8_t* fl,
+                            const int8_t* dye, int8_t* ctmc, const double* k_fret,
+                            const int32_t* acc, const double* u_pois, const double* u_trans,
+                            int32_t* nd_out, int32_t* nf_out, int64_t* cnt) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int it = id[t];
+    uint8_t f = fl[t];
+    int8_t c = ctmc[t];
+    PssOut o{0, 0, 0.0, 0};
+    const bool on = pss_eval(pp, f, c, dye[t], k_fret[t], acc[t], u_pois[it], u_trans[it], o);
+    fl[t] = f;
+    ctmc[t] = c;
+    nd_out[it] = on ? o.nd : 0;
+    nf_out[it] = on ? o.nf : 0;
+    cnt[it] = on ? (int64_t)(o.nd + o.nf) : 0;
+}
+
+__global__ void k_pss_emit(int n, const edm_photo_params* pp, const int32_t* id,
+                           const int8_t* dye, const int32_t* slot_of, const double* k_fret,
+                           const int32_t* acc, const int32_t* nd_in, const int32_t* nf_in,
+                           const int64_t* off, const double* u_arr, int64_t n_arr,
+                           int64_t max_photons, int32_t* o_pid, double* o_t, int8_t* o_ch,
+                           uint8_t* o_fret) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int it = id[t];
+    const int nd = nd_in[it], nf = nf_in[it];
+    if (nd + nf == 0) return;
+    const int ty = dye[t];
+    const double kf = k_fret[t];
+    const double tau = kf > 0.0 ? pp->tau[ty] / (1.0 + kf * pp->tau[ty]) : pp->tau[ty];
+    const int64_t base = off[it];
+    for (int k = 0; k < nd; ++k) {
+        const int64_t pos = base + k;
+        if (pos >= max_photons) break;
+        double ta;
+        if (pos < n_arr) {
+            double u = u_arr[pos];
+            if (u < 1e-30) u = 1e-30;
+            ta = -tau * log(u);
+        } else {
+            ta = tau;
+        }
+        o_pid[pos] = it;
+        o_t[pos] = ta;
+        o_ch[pos] = (int8_t)ty;
+        o_fret[pos] = 0;
+    }
+    if (nf > 0) {
+        const int a = acc[t];
+        const int at = dye[slot_of[a]];
+        const double tau_a = (at >= 0 && at < pp->n_fluoro) ? pp->tau[at] : tau;
+        for (int k = 0; k < nf; ++k) {
+            const int64_t pos = base + nd + k;
+            if (pos >= max_photons) break;
+            double ta;
+            if (pos < n_arr) {
+                double u = u_arr[pos];
+                if (u < 1e-30) u = 1e-30;
+                ta = -tau_a * log(u);
+            } else {
+                ta = tau_a;
+            }
+            o_pid[pos] = a;
+            o_t[pos] = ta;
+            o_ch[pos] = (int8_t)at;
+            o_fret[pos] = 1;
+        }
+    }
+}
+
+// ----------------------------------------------------------------------
+// Emitter-centred PSF pixel draw (microscope.py:523-631)
+// ----------------------------------------------------------------------
+__device__ __forceinline__ bool psf_pick(const edm_image_params* ip, const double* pcx,
+                                         const double* pcy, double x, double y, int ch, double u,
+                                         int& opx, int& opy) {
+    if (ch < 0) ch = 0;
+    if (ch >= ip->n_sigma) ch = 0;
+    double sigma = ip->sigma_ch[ch];
+    if (sigma <= 0.0) sigma = ip->sigma_ch[0];
+    const int R = ip->R, nx = ip->nx, ny = ip->ny;
+    const double fx = (x - ip->offx) / ip->pixel - 0.5;
+    const double fy = (y - ip->offy) / ip->pixel - 0.5;
+    const int i0 = (int)floor(fx + 0.5);
+    const int j0 = (int)floor(fy + 0.5);
+    if (i0 < -R || i0 >= nx + R) return false;
+    if (j0 < -R || j0 >= ny + R) return false;
+    const double inv2 = 0.5 / (sigma * sigma);
+    double total = 0.0;
+    int first_i = -1, first_j = -1;
+    for (int di = -R; di <= R; ++di) {
+        const int ii = i0 + di;
+        if (ii < 0 || ii >= nx) continue;
+        const double dx = x - pcx[ii];
+        for (int dj = -R; dj <= R; ++dj) {
+            const int jj = j0 + dj;
+            if (jj < 0 || jj >= ny) continue;
+            const double dy = y - pcy[jj];
+            total += exp(-(dx * dx + dy * dy) * inv2);
+            if (first_i < 0) {
+                first_i = ii;
+                first_j = jj;
+            }
+        }
+    }
+    if (first_i < 0 || total <= 0.0) return false;
+    const double uu = u * total;
+    double cum = 0.0;
+    for (int di = -R; di <= R; ++di) {
+        const int ii = i0 + di;
+        if (ii < 0 || ii >= nx) continue;
+        const double dx = x - pcx[ii];
+        for (int dj = -R; dj <= R; ++dj) {
+            const int jj = j0 + dj;
+            if (jj < 0 || jj >= ny) continue;
+            const double dy = y - pcy[jj];
+            cum += exp(-(dx * dx + dy * dy) * inv2);
+            if (cum >= uu) {
+                opx = ii;
+                opy = jj;
+                return true;
+            }
+        }
+    }
+    opx = first_i;
+    opy = first_j;
+    return true;
+}
+
+__device__ __forceinline__ void tcspc_add(const edm_image_params* ip, uint32_t* flim, int ch,
+                                          int px, int py, double t) {
+    if (ch < 0 || ch >= ip->n_ch) return;
+    if (px < 0 || px >= ip->nx || py < 0 || py >= ip->ny) return;
+    if (t < 0.0) t = 0.0;
+    const long long b = (long long)(t / ip->bin_w);
+    if (b >= ip->n_bins) return;
+    atomicAdd(&flim[(((size_t)ch * ip->nx + px) * ip->ny + py) * ip->n_bins + b], 1u);
+}
+
+// ----------------------------------------------------------------------
+// Fused per-emitter photon path with Philox draws
+// ----------------------------------------------------------------------
+template <typename Real>
+__global__ void k_pss_image(int n, const edm_photo_params* pp, const edm_image_params* ip,
+                            const double* pcx, const double* pcy, const Real* x, const Real* y,
+                            const int32_t* id, uint8_t* fl, const int8_t* dye, int8_t* ctmc,
+                            const double* k_fret, const int32_t* acc, const int32_t* slot_of,
+                            uint32_t* flim, Rng rng, uint32_t step, unsigned long long* ctr) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int emitted = 0, detected = 0;
+    if (t < n) {
+        const int it = id[t];
+        uint8_t f = fl[t];
+        int8_t c = ctmc[t];
+        const U4 r0 = rng.draw(step, (uint32_t)it, 0u, kTagPhotonPois);
+        const double up = u01d(r0.x, r0.y), ut = u01d(r0.z, r0.w);
+        PssOut o{0, 0, 0.0, 0};
+        const int aid = acc[t];
+        if (pss_eval(pp, f, c, dye[t], k_fret[t], aid, up, ut, o)) {
+            emitted = o.nd + o.nf;
+            const double qe = ip->qe;
+            const double sirf = ip->sigma_irf;
+            for (int k = 0; k < o.nd + o.nf; ++k) {
+                const bool donor = k < o.nd;
+                int es = t, ch = o.ty;
+                double tau = o.tau;
+                if (!donor) {
+                    es = slot_of[aid];
+                    ch = dye[es];
+                    tau = (ch >= 0 && ch < pp->n_fluoro) ? pp->tau[ch] : o.tau;
+                }
+                const U4 ra = rng.draw(step, (uint32_t)it, (uint32_t)k, kTagArrival);
+                double ua = u01d(ra.x, ra.y);
+                if (ua < 1e-30) ua = 1e-30;
+                const double ta = -tau * log(ua);
+                if (u01d(ra.z, ra.w) >= qe) continue;  // detection efficiency
+                const U4 rp = rng.draw(step, (uint32_t)it, (uint32_t)k, kTagPixel);
+                int px_, py_;
+                if (!psf_pick(ip, pcx, pcy, (double)x[es], (double)y[es], ch, u01d(rp.x, rp.y), px_,
+                              py_))
+                    continue;
+                double z0, z1;
+                normal2<double>(rng.draw(step, (uint32_t)it, (uint32_t)k, kTagJitter), z0, z1);
+                tcspc_add(ip, flim, ch, px_, py_, ta + sirf * z0);
+                ++detected;
+            }
+        }
+        fl[t] = f;
+        ctmc[t] = c;
+    }
+    unsigned long long e = emitted, d = detected;
+    for (int o = 32; o > 0; o >>= 1) {
+        e += __shfl_down(e, o);
+        d += __shfl_down(d, o);
+    }
+    if (lane_id() == 0) {
+        if (e) atomicAdd(&ctr[6], e);
+        if (d) atomicAdd(&ctr[7], d);
+    }
+}
+
+// ----------------------------------------------------------------------
+// Hook-run imaging kernels (microscope.py:1069-1157)
+// ----------------------------------------------------------------------
+__global__ void k_det_flag(int64_t n, const double* u_qe, double qe, int64_t* flag) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    flag[p] = u_qe[p] < qe ? 1 : 0;
+}
+
+__global__ void k_det_compact(int64_t n, const int64_t* flag, const int64_t* pos,
+                              const int32_t* pid, uint32_t* key, uint32_t* val) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || !flag[p]) return;
+    key[pos[p]] = (uint32_t)pid[p];
+    val[pos[p]] = (uint32_t)p;
+}
+
+template <typename Real>
+__global__ void k_assign(int64_t n, const edm_image_params* ip, const double* pcx,
+                         const double* pcy, const Real* x, const Real* y, const int32_t* slot_of,
+                         const uint32_t* sv_val, const int32_t* pid, const int8_t* ch,
+                         const double* u_pix, int64_t* aflag, int32_t* spx, int32_t* spy) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const uint32_t p = sv_val[q];
+    const int s = slot_of[pid[p]];
+    int a, b;
+    const bool ok = psf_pick(ip, pcx, pcy, (double)x[s], (double)y[s], ch[p], u_pix[q], a, b);
+    aflag[q] = ok ? 1 : 0;
+    spx[q] = a;
+    spy[q] = b;
+}
+
+__global__ void k_assign_compact(int64_t n, const int64_t* aflag, const int64_t* apos,
+                                 const uint32_t* sv_val, const int32_t* spx, const int32_t* spy,
+                                 const double* ph_t, const int8_t* ph_ch, int32_t* apx,
+                                 int32_t* apy, double* at, int8_t* ach) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n || !aflag[q]) return;
+    const int64_t o = apos[q];
+    const uint32_t p = sv_val[q];
+    apx[o] = spx[q];
+    apy[o] = spy[q];
+    at[o] = ph_t[p];
+    ach[o] = ph_ch[p];
+}
+
+__global__ void k_bin_tcspc(int64_t n, const edm_image_params* ip, uint32_t* flim,
+                            const int32_t* apx, const int32_t* apy, const double* at,
+                            const int8_t* ach, const double* jitter) {
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    tcspc_add(ip, flim, ach[a], apx[a], apy[a], at[a] + jitter[a]);
+}
+
+// ----------------------------------------------------------------------
+// Frame end (microscope.py:859-920)
+// ----------------------------------------------------------------------
+__global__ void k_pixel_totals(int64_t npix, int nb, const uint32_t* flim, uint32_t* tot) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= npix) return;
+    uint32_t s = 0;
+    const uint32_t* row = flim + q * nb;
+    for (int b = 0; b < nb; ++b) s += row[b];
+    tot[q] = s;
+}
+
+__global__ void k_add_counts(int64_t n, const int64_t* idx, uint32_t* flim, int64_t nelem) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t k = idx[i];
+    if (k >= 0 && k < nelem) atomicAdd(&flim[k], 1u);
+}
+
+// Poisson by sequential inversion (small means: dark counts per pixel).
+__device__ __forceinline__ int poisson_inv(double lam, double u) {
+    double p = exp(-lam), F = p;
+    int k = 0;
+    while (u > F && k < 1000) {
+        ++k;
+        p *= lam / k;
+        F += p;
+    }
+    return k;
+}
+
+// Binomial by inversion (Kachitvichyanukul & Schmeiser 1988, BINV) for
+// small n*p, normal approximation otherwise.
+__device__ __forceinline__ int binomial_draw(int nn, double p, double u, double z) {
+    if (nn <= 0 || p <= 0.0) return 0;
+    const double mean = nn * p;
+    if (mean > 30.0) {
+        const double v = mean + sqrt(mean * (1.0 - p)) * z + 0.5;
+        long long iv = (long long)v;
+        if (iv < 0) iv = 0;
+        if (iv > nn) iv = nn;
+        return (int)iv;
+    }
+    const double q = 1.0 - p, s = p / q, a = (nn + 1) * s;
+    double r = pow(q, (double)nn);
+    int k = 0;
+    while (u > r && k < nn) {
+        u -= r;
+        ++k;
+        r *= (a / k - s);
+    }
+    return k;
+}
+
+__global__ void k_frame_noise(int64_t npix, int nb, uint32_t* flim, double lam, double p_ap,
+                              Rng rng, uint32_t frame, unsigned long long* out) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long nd = 0, na = 0;
+    if (q < npix) {
+        uint32_t* row = flim + q * nb;
+        const uint32_t lo = (uint32_t)q, hi = (uint32_t)(q >> 32);
+        if (lam >= 1e-10) {
+            const U4 r = rng.draw(frame, lo, hi, kTagDark);
+            const int k = poisson_inv(lam, u01d(r.x, r.y));
+            for (int i = 0; i < k; ++i) {
+                const U4 rb = rng.draw(frame, lo, hi + 0x10000u * (uint32_t)(i + 1), kTagDark);
+                row[(int)(u01d(rb.x, rb.y) * nb) % nb] += 1u;
+            }
+            nd = (unsigned long long)k;
+        }
+        if (p_ap > 0.0) {
+            uint32_t tot = 0;
+            for (int b = 0; b < nb; ++b) tot += row[b];
+            if (tot) {
+                const U4 r = rng.draw(frame, lo, hi, kTagAfterpulse);
+                double z0, z1;
+                normal2<double>(r, z0, z1);
+                const int k = binomial_draw((int)tot, p_ap, u01d(r.x, r.z), z0);
+                for (int i = 0; i < k; ++i) {
+                    const U4 rb =
+                        rng.draw(frame, lo, hi + 0x10000u * (uint32_t)(i + 1), kTagAfterpulse);
+                    row[(int)(u01d(rb.x, rb.y) * nb) % nb] += 1u;
+                }
+                na = (unsigned long long)k;
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        nd += __shfl_down(nd, o);
+        na += __shfl_down(na, o);
+    }
+    if (lane_id() == 0) {
+        if (nd) atomicAdd(&out[0], nd);
+        if (na) atomicAdd(&out[1], na);
+    }
+}
+
+__global__ void k_fill_u32(int64_t n, uint32_t* p, uint32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// ----------------------------------------------------------------------
+// Host side
+// ----------------------------------------------------------------------
+int ph_alloc(edm_membrane* m) {
+    const int n = m->n;
+    if (!m->d_pp) EDM_HIP(hipMalloc((void**)&m->d_pp, sizeof(edm_photo_params)));
+    EDM_HIP(hipMemcpy(m->d_pp, &m->pp, sizeof(edm_photo_params), hipMemcpyHostToDevice));
+    if (!m->k_fret) {
+        if (dalloc(&m->k_fret, n) || dalloc(&m->acc, n) || dalloc(&m->ph_nd, n) ||
+            dalloc(&m->ph_nf, n) || dalloc(&m->ph_cnt, n) || dalloc(&m->ph_off, n))
+            return EDM_EHIP;
+        hipLaunchKernelGGL(k_fret_zero, dim3(nb(n)), dim3(kB), 0, m->stream, n, m->k_fret, m->acc);
+    }
+    return 0;
+}
+
+static int ensure_photon_cap(edm_membrane* m, int64_t need) {
+    if (need <= m->ph_cap) return 0;
+    int64_t cap = std::max<int64_t>(need, 1024);
+    cap = cap + cap / 2;
+    hipFree(m->ph_pid);
+    hipFree(m->ph_t);
+    hipFree(m->ph_ch);
+    hipFree(m->ph_fret);
+    hipFree(m->det_flag);
+    hipFree(m->det_pos);
+    hipFree(m->sv_key);
+    hipFree(m->sv_key2);
+    hipFree(m->sv_val);
+    hipFree(m->sv_val2);
+    hipFree(m->asg_flag);
+    hipFree(m->asg_pos);
+    hipFree(m->asg_px);
+    hipFree(m->asg_py);
+    hipFree(m->asg_t);
+    hipFree(m->asg_ch);
+    hipFree(m->sv_px);
+    hipFree(m->sv_py);
+    if (dalloc(&m->ph_pid, cap) || dalloc(&m->ph_t, cap) || dalloc(&m->ph_ch, cap) ||
+        dalloc(&m->ph_fret, cap) || dalloc(&m->det_flag, cap) || dalloc(&m->det_pos, cap) ||
+        dalloc(&m->sv_key, cap) || dalloc(&m->sv_key2, cap) || dalloc(&m->sv_val, cap) ||
+        dalloc(&m->sv_val2, cap) || dalloc(&m->asg_flag, cap) || dalloc(&m->asg_pos, cap) ||
+        dalloc(&m->asg_px, cap) || dalloc(&m->asg_py, cap) || dalloc(&m->asg_t, cap) ||
+        dalloc(&m->asg_ch, cap) || dalloc(&m->sv_px, cap) || dalloc(&m->sv_py, cap))
+        return EDM_EHIP;
+    m->ph_cap = cap;
+    return 0;
+}
+
+static int scan_i64(edm_membrane* m, const int64_t* in, int64_t* out, int64_t n) {
+    size_t need = 0;
+    EDM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, in, out, (int)n, m->stream));
+    if (need > m->cub_tmp_bytes) {
+        EDM_HIP(hipStreamSynchronize(m->stream));
+        EDM_HIP(hipFree(m->cub_tmp));
+        m->cub_tmp_bytes = need + 256;
+        EDM_HIP(hipMalloc(&m->cub_tmp, m->cub_tmp_bytes));
+    }
+    size_t tb = m->cub_tmp_bytes;
+    EDM_HIP(hipcub::DeviceScan::ExclusiveSum(m->cub_tmp, tb, in, out, (int)n, m->stream));
+    return 0;
+}
+
+// total = out[n-1] + in[n-1] read back to the host
+static int scan_total(edm_membrane* m, const int64_t* in, const int64_t* out, int64_t n,
+                      int64_t* total) {
+    if (n <= 0) {
+        *total = 0;
+        return 0;
+    }
+    int64_t a = 0, b = 0;
+    EDM_HIP(hipMemcpyAsync(&a, in + n - 1, 8, hipMemcpyDeviceToHost, m->stream));
+    EDM_HIP(hipMemcpyAsync(&b, out + n - 1, 8, hipMemcpyDeviceToHost, m->stream));
+    EDM_HIP(hipStreamSynchronize(m->stream));
+    *total = a + b;
+    return 0;
+}
+
+int ph_fret(edm_membrane* m) {
+    const int n = m->n;
+    hipStream_t st = m->stream;
+    hipLaunchKernelGGL(k_fret_zero, dim3(nb(n)), dim3(kB), 0, st, n, m->k_fret, m->acc);
+    const edm_photo_params* dpp = (const edm_photo_params*)m->d_pp;
+    for (int j = 0; j < m->pp.n_fret_maps; ++j) {
+        if (m->pp.fm_proximity[j] || m->pp.fm_skip[j]) continue;
+        hipLaunchKernelGGL(k_fret_bond, dim3(nb(n)), dim3(kB), 0, st, n, dpp, j, m->a.fl,
+                           m->a.dye, m->a.bond, m->a.slot_of, m->k_fret, m->acc);
+    }
+    for (int j = 0; j < m->pp.n_fret_maps; ++j) {
+        if (!m->pp.fm_proximity[j] || m->pp.fm_skip[j]) continue;
+        if (m->fp64)
+            hipLaunchKernelGGL(k_fret_prox<double>, dim3(nb(n)), dim3(kB), 0, st, n, dpp, j,
+                               (const double*)m->a.x, (const double*)m->a.y, m->a.sp, m->a.fl,
+                               m->a.dye, m->p.Lx, m->p.Ly, m->p.periodic_x != 0,
+                               m->p.periodic_y != 0, m->k_fret, m->acc);
+        else
+            hipLaunchKernelGGL(k_fret_prox<float>, dim3(nb(n)), dim3(kB), 0, st, n, dpp, j,
+                               (const float*)m->a.x, (const float*)m->a.y, m->a.sp, m->a.fl,
+                               m->a.dye, (float)m->p.Lx, (float)m->p.Ly, m->p.periodic_x != 0,
+                               m->p.periodic_y != 0, m->k_fret, m->acc);
+    }
+    EDM_HIP(hipGetLastError());
+    return 0;
+}
+
+int ph_hook(edm_membrane* m, const double* u_pois, const double* u_trans, const double* u_arr,
+            int64_t n_arr, int64_t* n_photons) {
+    const int n = m->n;
+    hipStream_t st = m->stream;
+    if (int rc = ph_fret(m)) return rc;
+    // stage the three draw arrays contiguously
+    const int64_t tot = 2 * (int64_t)n + n_arr;
+    if (int rc = stage_draws(m, nullptr, tot)) return rc;
+    EDM_HIP(hipMemcpyAsync(m->d_draws, u_pois, (size_t)n * 8, hipMemcpyHostToDevice, st));
+    EDM_HIP(hipMemcpyAsync(m->d_draws + n, u_trans, (size_t)n * 8, hipMemcpyHostToDevice, st));
+    if (n_arr > 0)
+        EDM_HIP(hipMemcpyAsync(m->d_draws + 2 * (int64_t)n, u_arr, (size_t)n_arr * 8,
+                               hipMemcpyHostToDevice, st));
+    const edm_photo_params* dpp = (const edm_photo_params*)m->d_pp;
+    hipLaunchKernelGGL(k_pss_count, dim3(nb(n)), dim3(kB), 0, st, n, dpp,
+                       (const int32_t*)m->a.id, m->a.fl, (const int8_t*)m->a.dye, m->a.ctmc,
+                       (const double*)m->k_fret, (const int32_t*)m->acc,
+                       (const double*)m->d_draws, (const double*)(m->d_draws + n), m->ph_nd,
+                       m->ph_nf, m->ph_cnt);
+    if (int rc = scan_i64(m, m->ph_cnt, m->ph_off, n)) return rc;
+    int64_t total = 0;
+    if (int rc = scan_total(m, m->ph_cnt, m->ph_off, n, &total)) return rc;
+    const int64_t np = std::min<int64_t>(total, m->pp.max_photons);
+    if (int rc = ensure_photon_cap(m, np)) return rc;
+    hipLaunchKernelGGL(k_pss_emit, dim3(nb(n)), dim3(kB), 0, st, n, dpp, (const int32_t*)m->a.id,
+                       (const int8_t*)m->a.dye, (const int32_t*)m->a.slot_of,
+                       (const double*)m->k_fret, (const int32_t*)m->acc,
+                       (const int32_t*)m->ph_nd, (const int32_t*)m->ph_nf,
+                       (const int64_t*)m->ph_off, (const double*)(m->d_draws + 2 * (int64_t)n),
+                       n_arr, (int64_t)m->pp.max_photons, m->ph_pid, m->ph_t, m->ph_ch,
+                       m->ph_fret);
+    EDM_HIP(hipGetLastError());
+    EDM_HIP(hipStreamSynchronize(st));
+    m->n_ph = np;
+    *n_photons = np;
+    return 0;
+}
+
+static const edm_image_params* dev_ip(edm_membrane* m) {
+    return (const edm_image_params*)(m->pcx + m->ip.nx + m->ip.ny);
+}
+
+int ph_fused_image(edm_membrane* m, int64_t* n_emitted, int64_t* n_detected) {
+    if (!m->has_ip) return EDM_ESTATE;
+    const int n = m->n;
+    hipStream_t st = m->stream;
+    if (int rc = ph_fret(m)) return rc;
+    const edm_photo_params* dpp = (const edm_photo_params*)m->d_pp;
+    Rng rng{(uint32_t)(m->p.seed & 0xffffffffu) ^ 0x5bd1e995u, (uint32_t)(m->p.seed >> 32)};
+    // the photon step uses the count of completed dynamics steps as its counter
+    const uint32_t step = m->step;
+    const edm_image_params* d_ip = dev_ip(m);
+    if (m->fp64)
+        hipLaunchKernelGGL(k_pss_image<double>, dim3(nb(n)), dim3(kB), 0, st, n, dpp, d_ip,
+                           (const double*)m->pcx, (const double*)m->pcy, (const double*)m->a.x,
+                           (const double*)m->a.y, (const int32_t*)m->a.id, m->a.fl,
+                           (const int8_t*)m->a.dye, m->a.ctmc, (const double*)m->k_fret,
+                           (const int32_t*)m->acc, (const int32_t*)m->a.slot_of, m->flim, rng,
+                           step, m->d_ctr);
+    else
+        hipLaunchKernelGGL(k_pss_image<float>, dim3(nb(n)), dim3(kB), 0, st, n, dpp, d_ip,
+                           (const double*)m->pcx, (const double*)m->pcy, (const float*)m->a.x,
+                           (const float*)m->a.y, (const int32_t*)m->a.id, m->a.fl,
+                           (const int8_t*)m->a.dye, m->a.ctmc, (const double*)m->k_fret,
+                           (const int32_t*)m->acc, (const int32_t*)m->a.slot_of, m->flim, rng,
+                           step, m->d_ctr);
+    EDM_HIP(hipGetLastError());
+    if (n_emitted || n_detected) {
+        unsigned long long c[2];
+        EDM_HIP(hipMemcpyAsync(c, m->d_ctr + 6, 16, hipMemcpyDeviceToHost, st));
+        EDM_HIP(hipStreamSynchronize(st));
+        if (n_emitted) *n_emitted = (int64_t)c[0];
+        if (n_detected) *n_detected = (int64_t)c[1];
+    }
+    return 0;
+}
+
+// Imaging buffers: flim stack, pixel centres and the parameter block live
+// in one allocation after the pixel-centre arrays.
+int img_alloc(edm_membrane* m) {
+    const edm_image_params& ip = m->ip;
+    const size_t elems = (size_t)ip.n_ch * ip.nx * ip.ny * ip.n_bins;
+    if (elems != m->flim_elems) {
+        hipFree(m->flim);
+        hipFree(m->pix_tot);
+        m->flim = nullptr;
+        if (dalloc(&m->flim, elems) || dalloc(&m->pix_tot, (size_t)ip.n_ch * ip.nx * ip.ny))
+            return EDM_EHIP;
+        m->flim_elems = elems;
+        if (int rc = frame_reset(m)) return rc;
+    }
+    hipFree(m->pcx);
+    const size_t ip_words = (sizeof(edm_image_params) + 7) / 8;
+    if (dalloc(&m->pcx, (size_t)ip.nx + ip.ny + ip_words)) return EDM_EHIP;
+    m->pcy = m->pcx + ip.nx;
+    std::vector<double> host((size_t)ip.nx + ip.ny + ip_words, 0.0);
+    for (int i = 0; i < ip.nx; ++i) host[i] = ((double)i + 0.5) * ip.pixel + ip.offx;
+    for (int j = 0; j < ip.ny; ++j) host[ip.nx + j] = ((double)j + 0.5) * ip.pixel + ip.offy;
+    memcpy(host.data() + ip.nx + ip.ny, &ip, sizeof(edm_image_params));
+    EDM_HIP(hipMemcpy(m->pcx, host.data(), host.size() * 8, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int img_detect(edm_membrane* m, const double* u_qe, int64_t n, int64_t* n_surv) {
+    hipStream_t st = m->stream;
+    if (n != m->n_ph) return EDM_EINVAL;
+    if (n == 0) {
+        m->n_surv = 0;
+        *n_surv = 0;
+        return 0;
+    }
+    if (int rc = stage_draws(m, u_qe, n)) return rc;
+    hipLaunchKernelGGL(k_det_flag, dim3(nb(n)), dim3(kB), 0, st, n, (const double*)m->d_draws,
+                       m->ip.qe, m->det_flag);
+    if (int rc = scan_i64(m, m->det_flag, m->det_pos, n)) return rc;
+    int64_t ns = 0;
+    if (int rc = scan_total(m, m->det_flag, m->det_pos, n, &ns)) return rc;
+    hipLaunchKernelGGL(k_det_compact, dim3(nb(n)), dim3(kB), 0, st, n,
+                       (const int64_t*)m->det_flag, (const int64_t*)m->det_pos,
+                       (const int32_t*)m->ph_pid, m->sv_key, m->sv_val);
+    if (ns > 0) {
+        size_t need = 0;
+        EDM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, need, m->sv_key, m->sv_key2, m->sv_val,
+                                                   m->sv_val2, (int)ns, 0, 32, st));
+        if (need > m->cub_tmp_bytes) {
+            EDM_HIP(hipStreamSynchronize(st));
+            EDM_HIP(hipFree(m->cub_tmp));
+            m->cub_tmp_bytes = need + 256;
+            EDM_HIP(hipMalloc(&m->cub_tmp, m->cub_tmp_bytes));
+        }
+        size_t tb = m->cub_tmp_bytes;
+        EDM_HIP(hipcub::DeviceRadixSort::SortPairs(m->cub_tmp, tb, m->sv_key, m->sv_key2,
+                                                   m->sv_val, m->sv_val2, (int)ns, 0, 32, st));
+    }
+    EDM_HIP(hipGetLastError());
+    EDM_HIP(hipStreamSynchronize(st));
+    m->n_surv = ns;
+    *n_surv = ns;
+    return 0;
+}
+
+int img_assign(edm_membrane* m, const double* u_pix, int64_t n, int64_t* n_asg) {
+    hipStream_t st = m->stream;
+    if (n != m->n_surv) return EDM_EINVAL;
+    if (n == 0) {
+        m->n_asg = 0;
+        *n_asg = 0;
+        return 0;
+    }
+    if (int rc = stage_draws(m, u_pix, n)) return rc;
+    if (m->fp64)
+        hipLaunchKernelGGL(k_assign<double>, dim3(nb(n)), dim3(kB), 0, st, n, dev_ip(m),
+                           (const double*)m->pcx, (const double*)m->pcy, (const double*)m->a.x,
+                           (const double*)m->a.y, (const int32_t*)m->a.slot_of,
+                           (const uint32_t*)m->sv_val2, (const int32_t*)m->ph_pid,
+                           (const int8_t*)m->ph_ch, (const double*)m->d_draws, m->asg_flag,
+                           m->sv_px, m->sv_py);
+    else
+        hipLaunchKernelGGL(k_assign<float>, dim3(nb(n)), dim3(kB), 0, st, n, dev_ip(m),
+                           (const double*)m->pcx, (const double*)m->pcy, (const float*)m->a.x,
+                           (const float*)m->a.y, (const int32_t*)m->a.slot_of,
+                           (const uint32_t*)m->sv_val2, (const int32_t*)m->ph_pid,
+                           (const int8_t*)m->ph_ch, (const double*)m->d_draws, m->asg_flag,
+                           m->sv_px, m->sv_py);
+    if (int rc = scan_i64(m, m->asg_flag, m->asg_pos, n)) return rc;
+    int64_t na = 0;
+    if (int rc = scan_total(m, m->asg_flag, m->asg_pos, n, &na)) return rc;
+    hipLaunchKernelGGL(k_assign_compact, dim3(nb(n)), dim3(kB), 0, st, n,
+                       (const int64_t*)m->asg_flag, (const int64_t*)m->asg_pos,
+                       (const uint32_t*)m->sv_val2, (const int32_t*)m->sv_px,
+                       (const int32_t*)m->sv_py, (const double*)m->ph_t, (const int8_t*)m->ph_ch,
+                       m->asg_px, m->asg_py, m->asg_t, m->asg_ch);
+    EDM_HIP(hipGetLastError());
+    EDM_HIP(hipStreamSynchronize(st));
+    m->n_asg = na;
+    *n_asg = na;
+    return 0;
+}
+
+int img_bin(edm_membrane* m, const double* jitter, int64_t n) {
+    hipStream_t st = m->stream;
+    if (n != m->n_asg) return EDM_EINVAL;
+    if (n == 0) return 0;
+    if (int rc = stage_draws(m, jitter, n)) return rc;
+    hipLaunchKernelGGL(k_bin_tcspc, dim3(nb(n)), dim3(kB), 0, st, n, dev_ip(m), m->flim,
+                       (const int32_t*)m->asg_px, (const int32_t*)m->asg_py,
+                       (const double*)m->asg_t, (const int8_t*)m->asg_ch,
+                       (const double*)m->d_draws);
+    EDM_HIP(hipGetLastError());
+    EDM_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+int frame_totals(edm_membrane* m, uint32_t* totals) {
+    const int64_t npix = (int64_t)m->ip.n_ch * m->ip.nx * m->ip.ny;
+    hipLaunchKernelGGL(k_pixel_totals, dim3(nb(npix)), dim3(kB), 0, m->stream, npix, m->ip.n_bins,
+                       (const uint32_t*)m->flim, m->pix_tot);
+    EDM_HIP(hipMemcpyAsync(totals, m->pix_tot, (size_t)npix * 4, hipMemcpyDeviceToHost, m->stream));
+    EDM_HIP(hipStreamSynchronize(m->stream));
+    return 0;
+}
+
+int frame_add(edm_membrane* m, const int64_t* flat, int64_t n) {
+    if (n <= 0) return 0;
+    int64_t* d = nullptr;
+    EDM_HIP(hipMalloc(&d, (size_t)n * 8));
+    EDM_HIP(hipMemcpyAsync(d, flat, (size_t)n * 8, hipMemcpyHostToDevice, m->stream));
+    hipLaunchKernelGGL(k_add_counts, dim3(nb(n)), dim3(kB), 0, m->stream, n, (const int64_t*)d,
+                       m->flim, (int64_t)m->flim_elems);
+    EDM_HIP(hipStreamSynchronize(m->stream));
+    EDM_HIP(hipFree(d));
+    return 0;
+}
+
+int frame_noise(edm_membrane* m, int64_t* n_dark, int64_t* n_ap) {
+    const int64_t npix = (int64_t)m->ip.n_ch * m->ip.nx * m->ip.ny;
+    Rng rng{(uint32_t)(m->p.seed & 0xffffffffu) ^ 0x2545F491u, (uint32_t)(m->p.seed >> 32)};
+    unsigned long long* out = (unsigned long long*)(m->d_scal + 8);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, m->stream, (int64_t)4, (uint32_t*)out, 0u);
+    hipLaunchKernelGGL(k_frame_noise, dim3(nb(npix)), dim3(kB), 0, m->stream, npix, m->ip.n_bins,
+                       m->flim, m->ip.dark_lambda, m->ip.p_ap, rng, m->step, out);
+    EDM_HIP(hipGetLastError());
+    if (n_dark || n_ap) {
+        unsigned long long c[2];
+        EDM_HIP(hipMemcpyAsync(c, out, 16, hipMemcpyDeviceToHost, m->stream));
+        EDM_HIP(hipStreamSynchronize(m->stream));
+        if (n_dark) *n_dark = (int64_t)c[0];
+        if (n_ap) *n_ap = (int64_t)c[1];
+    }
+    return 0;
+}
+
+int frame_reset(edm_membrane* m) {
+    if (!m->flim) return 0;
+    const int64_t n = (int64_t)m->flim_elems;
+    hipLaunchKernelGGL(k_fill_u32, dim3(std::min(nb(n), 8192)), dim3(kB), 0, m->stream, n, m->flim,
+                       0u);
+    EDM_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace edm

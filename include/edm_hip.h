@@ -1,0 +1,225 @@
+/*
+ * edm_hip.h — C ABI of the MI355X (gfx950) implementation of the ED-MRDT
+ * per-timestep particle path (libedm_hip.so).
+ *
+ * The reference has no native boundary of its own: its Python engines call
+ * numba-compiled kernels on numpy arrays.  These entry points replace that
+ * Python -> numba boundary.  Each comment names the reference code the entry
+ * point stands in for (paths relative to the reference repository).
+ *
+ * Conventions
+ *   - Plain C types only: host pointers, sizes, POD structs.  A membrane's
+ *     state lives in device memory owned by its handle; host arrays are
+ *     copied in/out explicitly.
+ *   - Every function returns 0 on success or a negative EDM_E* code; the
+ *     message of the last error on the calling thread is edm_last_error().
+ *   - Particle arrays passed in or out are in particle-index order (the
+ *     reference's order).  Inside the handle they are kept in cell order.
+ *   - "hook" draws are the reference's own numpy random arrays, passed in
+ *     the order and sizes the reference draws them; without hook draws the
+ *     library uses its counter-based Philox streams (seed, step, particle).
+ */
+#ifndef EDM_HIP_H
+#define EDM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EDM_OK 0
+#define EDM_EINVAL -1   /* bad argument or unsupported configuration      */
+#define EDM_EHIP -2     /* a HIP runtime call failed                       */
+#define EDM_ESTATE -3   /* call out of order (e.g. finish before begin)    */
+#define EDM_ECAPACITY -4 /* a device buffer bound was exceeded              */
+
+/* Reaction kinds, in the order DynamicsEngine.step runs them
+ * (dynamics.py:2205-2465). */
+#define EDM_RX_BIND 1     /* binding + dissociation (dynamics.py:529-654)   */
+#define EDM_RX_CAT 2      /* catalysis (dynamics.py:826-1090)               */
+#define EDM_RX_RECRUIT 3  /* recruitment (dynamics.py:1230-1458)            */
+#define EDM_RX_DETACH 4   /* detachment (dynamics.py:1462-1555)             */
+#define EDM_RX_CONVERT 5  /* conversion (dynamics.py:1566-1657)             */
+#define EDM_RX_ADAPT 6    /* adaptation field update (dynamics.py:1094-1112)*/
+
+/* One Hill feedback factor read from a coarse density grid
+ * (dynamics.py:1955-1971 _build_fb_info). sensor < 0: no factor. */
+typedef struct edm_feedback {
+    int32_t sensor;      /* species whose density is read                  */
+    int32_t grid;        /* index into edm_params.grids                    */
+    int32_t inhibition;  /* 1: 1/(1+r^n); 0: basal + (1-basal) r^n/(1+r^n) */
+    int32_t pad;
+    double K_half, n_hill, basal;
+} edm_feedback;
+
+typedef struct edm_grid {
+    int32_t nx, ny;
+    double sx, sy;
+} edm_grid;
+
+typedef struct edm_reaction {
+    int32_t kind;          /* EDM_RX_*                                      */
+    int32_t a, b, c;       /* BIND: ra, rb, product; CAT: enzyme, substrate,
+                              product; RECRUIT/DETACH: species; CONVERT:
+                              from, to                                      */
+    double contact;        /* BIND/CAT: effective contact radius (um)       */
+    double prob;           /* per-step probability: P_bind, P_cat, P_on,
+                              P_off, P_convert                              */
+    double prob2;          /* BIND: P_dissoc                                */
+    int32_t max_recruits;  /* RECRUIT cap per step (index order)            */
+    int32_t use_adapt;     /* CAT: multiply the adaptation factor            */
+    edm_feedback fb1, fb2; /* RECRUIT may use both (dual feedback)          */
+} edm_reaction;
+
+typedef struct edm_params {
+    int32_t n;              /* particles, alive + dormant                    */
+    int32_t n_species;
+    int32_t fp64;           /* 1: FP64 state and arithmetic; 0: FP32         */
+    int32_t periodic_x, periodic_y;
+    int32_t ncx, ncy;       /* cell list (dynamics.py:109-114)               */
+    int32_t use_wca, use_harmonic;
+    int32_t dfield;         /* 0: circular regions (n_regions may be 0);
+                               1: sinusoidal D(x)                            */
+    int32_t n_regions;
+    int32_t n_grids;
+    int32_t n_reactions;
+    int32_t pair_cap;       /* reference pair-list cap min(30N, 5e5); 0 = none */
+    double Lx, Ly, dt;
+    double cutoff, csx, csy;
+    double max_disp;        /* displacement cap (dynamics.py:1826-1864)      */
+    double smooth_w;        /* region sigmoid width (um)                     */
+    double sin_amp, sin_kx, sin_ky;
+    double adapt_tau, adapt_K, adapt_n; /* adaptation (dynamics.py:2068-2096) */
+    int32_t adapt_grid, adapt_species;  /* -1 = no adaptation                */
+    uint64_t seed;
+    const double *sigma, *epsilon, *k_harm; /* [n_species * n_species]      */
+    const double *D_base;                   /* [n_species]                  */
+    const double *rcx, *rcy, *rr, *rf;      /* [n_regions]                  */
+    const edm_grid *grids;                  /* [n_grids]                    */
+    const edm_reaction *reactions;          /* [n_reactions], run order     */
+} edm_params;
+
+/* Host-side particle state in particle-index order (particles.py:130-139). */
+typedef struct edm_host_state {
+    double *positions;     /* [n, 2] row-major                               */
+    int32_t *species_id;
+    int32_t *bond_partner;
+    uint8_t *is_alive, *has_dye, *is_bleached;
+    int32_t *dye_type_id;
+    int8_t *ctmc_state;
+} edm_host_state;
+
+/* Photophysics tables (photophysics.py:796-887), FP64, per fluorophore t. */
+typedef struct edm_photo_params {
+    int32_t n_fluoro;
+    int32_t n_fret_maps;
+    double T_step[8][9], T_step_f[8][9];  /* [t][3x3] row-major             */
+    double mean_ph[8][2], mean_ph_f[8][2], mean_fr[8][2], mean_fr_f[8][2];
+    double tau[8], k_bound[8];
+    int32_t gs[8], es[8], t1[8], bl[8];
+    /* bond-mode FRET maps (photophysics.py:1044-1070), in config order    */
+    int32_t fm_donor[8], fm_acceptor[8], fm_intra[8], fm_proximity[8];
+    int32_t fm_target[8];
+    double fm_kfret[8];     /* (R0/r)^6 / tau_D  for bond mode               */
+    double fm_k_hi[8], fm_k_lo[8], fm_radius[8]; /* proximity mode          */
+    int32_t fm_skip[8];     /* 1: map skipped by the reference (tau_D <= 0)  */
+    int32_t max_photons;    /* photophysics.py:879-887                       */
+} edm_photo_params;
+
+/* Virtual microscope constants (microscope.py:73-391, 650-712, 819-838). */
+typedef struct edm_image_params {
+    int32_t nx, ny, n_bins, n_ch;
+    int32_t R;              /* PSF window half-width in pixels              */
+    int32_t n_sigma;
+    double pixel, offx, offy, bin_w, sigma_irf;
+    double qe, dark_lambda, p_ap;
+    double sigma_ch[8];
+} edm_image_params;
+
+typedef struct edm_membrane edm_membrane;
+
+const char *edm_last_error(void);
+int edm_device_count(int *count);
+
+/* ---- membrane lifetime (ParticleState + DynamicsEngine.__init__) ------ */
+int edm_create(const edm_params *p, edm_membrane **out);
+int edm_destroy(edm_membrane *m);
+int edm_upload_state(edm_membrane *m, const edm_host_state *s);
+int edm_download_state(edm_membrane *m, edm_host_state *s);
+int edm_set_photophysics(edm_membrane *m, const edm_photo_params *pp);
+int edm_set_imaging(edm_membrane *m, const edm_image_params *ip);
+
+/* ---- DynamicsEngine.step (dynamics.py:2150-2465) -----------------------
+ * Plain runs: edm_run(m, k) advances k steps with Philox draws, no host
+ * synchronisation.  Parity runs split a step in two so the host can draw
+ * the reference's arrays once the pair count is known:
+ *   edm_step_begin   D(x), cell list and pair count on pre-move positions
+ *                    (dynamics.py:2159-2172); *n_pairs is the length of the
+ *                    reference's (possibly capped) pair list
+ *   edm_step_finish  forces, Ermak-McCammon step and every reaction, taking
+ *                    `draws` = the concatenation of the step's numpy arrays
+ *                    in the reference's draw order (NULL = Philox)          */
+int edm_step_begin(edm_membrane *m, int64_t *n_pairs);
+int edm_step_finish(edm_membrane *m, const double *draws, int64_t n_draws);
+int edm_run(edm_membrane *m, int32_t n_steps);
+
+/* Event counters: bindings, dissociations, catalytic, recruited, detached,
+ * converted (dynamics.py:2068-2071, 2147-2148), cumulative. */
+int edm_counters(edm_membrane *m, int64_t out[6]);
+/* Reference pair list in reference order (CellList.get_neighbor_pairs,
+ * dynamics.py:220-278) for the cell list of the last edm_step_begin. */
+int edm_pairs(edm_membrane *m, int32_t *pairs, int64_t cap, int64_t *n_pairs);
+/* Density field of a sensor on a grid (compute_density_field,
+ * dynamics.py:1668-1703) for the current post-step state. */
+int edm_density(edm_membrane *m, int32_t species, int32_t grid, double *out);
+
+/* ---- PhotonSimulator.evolve_one_bd_step (photophysics.py:1116-1197) ----
+ * Hook run: update FRET rates, then the PSS kernel with the reference's
+ * rng_poisson[n], rng_transition[n] and rng_arrival[n_arr]; the photon
+ * batch stays on the device, *n_photons is its length.                   */
+int edm_photons(edm_membrane *m, const double *u_pois, const double *u_trans,
+                const double *u_arr, int64_t n_arr, int64_t *n_photons);
+int edm_photon_batch(edm_membrane *m, int32_t *particle_idx, double *arrival_time,
+                     int8_t *channel, uint8_t *is_fret, int64_t cap);
+int edm_fret_rates(edm_membrane *m, double *k_fret, int32_t *acceptor_idx);
+
+/* ---- VirtualMicroscope.process_photons (microscope.py:1069-1157) -------
+ * Hook run, three calls with the reference's draws:
+ *   edm_image_detect  QE filter with rng.random(n_photons); stable sort by
+ *                     particle index; *n_surviving returned
+ *   edm_image_assign  emitter-centric PSF pixel draw with rng.random(n_surv)
+ *   edm_image_bin     IRF jitter rng.normal(0, sigma_irf, n_assigned) and
+ *                     TCSPC binning into the frame histogram               */
+int edm_image_detect(edm_membrane *m, const double *u_qe, int64_t n, int64_t *n_surviving);
+int edm_image_assign(edm_membrane *m, const double *u_pix, int64_t n, int64_t *n_assigned);
+int edm_image_bin(edm_membrane *m, const double *jitter, int64_t n);
+
+/* Plain run: PSS emission, QE, PSF pixel draw, IRF jitter and TCSPC binning
+ * fused per emitter with Philox draws.  *n_emitted / *n_detected may be NULL. */
+int edm_photons_image(edm_membrane *m, int64_t *n_emitted, int64_t *n_detected);
+
+/* ---- VirtualMicroscope.finalize_frame (microscope.py:1159-1203) --------
+ * Hook run: per-pixel totals for the afterpulse draws and explicit counts
+ * (flat [ch, x, y, bin] indices) from the host's numpy draws.
+ * Plain run: edm_frame_noise adds Poisson dark counts and binomial
+ * afterpulses on the device.                                             */
+int edm_frame_pixel_totals(edm_membrane *m, uint32_t *totals);
+int edm_frame_add_counts(edm_membrane *m, const int64_t *flat_idx, int64_t n);
+int edm_frame_noise(edm_membrane *m, int64_t *n_dark, int64_t *n_afterpulse);
+int edm_frame_download(edm_membrane *m, uint32_t *flim, int64_t n_elems);
+int edm_frame_reset(edm_membrane *m);
+
+/* ---- timing (HIP events on the handle's stream) ------------------------ */
+int edm_synchronize(edm_membrane *m);
+/* Runs n_steps plain steps (+ fused imaging every step when image != 0),
+ * returning total elapsed ms and the summed duration of the force/BD
+ * kernel launches measured with events around each launch. */
+int edm_time_steps(edm_membrane *m, int32_t n_steps, int32_t image, double *total_ms,
+                   double *force_ms, double *breakdown_ms /* [8] or NULL */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EDM_HIP_H */

@@ -1,0 +1,129 @@
+"""GPU parity: the HIP path against the oracle on the same seeded inputs,
+with the reference's numpy draws injected on both sides (rng="numpy").
+
+Bar: particle species / alive / bond / CTMC / bleach arrays, reaction event
+counters, photon batches and TCSPC stacks bit-exact; positions within 1e-9
+relative (FP64).
+"""
+
+import numpy as np
+import pytest
+
+import reference_model as rm
+from ed_mrdt import workloads
+from ed_mrdt.dynamics import DynamicsEngine
+from ed_mrdt.microscope import VirtualMicroscope
+from ed_mrdt.particles import ParticleState
+from ed_mrdt.photophysics import PhotonSimulator
+from parity_util import compare_states, host_arrays, oracle_arrays
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(cfg, pair_cap=True):
+    st = ParticleState(cfg)
+    eng = DynamicsEngine(cfg, st, precision="fp64", rng="numpy", pair_cap=pair_cap)
+    ost = rm.OracleState(cfg)
+    odyn = rm.OracleDynamics(cfg, ost, pair_cap=pair_cap)
+    return st, eng, ost, odyn
+
+
+def test_initial_state_matches_oracle(oracle_lib):
+    cfg = workloads.binding_fret()
+    st = ParticleState(cfg)
+    ost = rm.OracleState(cfg)
+    compare_states(host_arrays(st), oracle_arrays(ost), pos_rtol=0.0)
+
+
+def test_pair_list_reference_order(oracle_lib):
+    cfg = workloads.plain_membrane()
+    st, eng, ost, odyn = _pair(cfg)
+    for _ in range(3):
+        eng.step()
+        odyn.step()
+    n = eng.step_begin()
+    gp = eng.pairs()
+    op = odyn.build_pairs()
+    assert n == op.shape[0]
+    np.testing.assert_array_equal(gp, op)
+    eng.step_finish(eng._numpy_draws(n))
+
+
+@pytest.mark.parametrize("n_steps", [40])
+def test_plain_membrane_dynamics_fp64(oracle_lib, n_steps):
+    cfg = workloads.plain_membrane()
+    st, eng, ost, odyn = _pair(cfg)
+    worst = 0.0
+    for k in range(n_steps):
+        eng.step()
+        odyn.step()
+        if (k + 1) % 10 == 0:
+            worst = max(worst, compare_states(host_arrays(st), oracle_arrays(ost),
+                                              label=f"step {k + 1}: "))
+            oc = odyn.counters()
+            assert eng.n_catalytic == oc["n_catalytic"]
+            assert eng.n_recruited == oc["n_recruited"]
+            assert eng.n_detached == oc["n_detached"]
+            assert eng.n_converted == oc["n_converted"]
+    print(f"plain membrane {n_steps} steps: worst relative position error {worst:.3e}")
+
+
+@pytest.mark.parametrize("variant", [dict(), dict(intramolecular=False),
+                                     dict(reflective=True, harmonic=True)])
+def test_binding_fret_full_path_fp64(oracle_lib, variant):
+    cfg = workloads.binding_fret(**variant)
+    st, eng, ost, odyn = _pair(cfg)
+    duty = min(1.0, cfg.scan.dwell_time / cfg.scan.frame_interval)
+    ph = PhotonSimulator(cfg, st, duty)
+    mic = VirtualMicroscope(cfg, st)
+    oph = rm.OraclePhotons(cfg, ost, duty)
+    omic = rm.OracleMicroscope(cfg, ost)
+    tot_ph = tot_det = 0
+    for k in range(25):
+        eng.step()
+        odyn.step()
+        b = ph.evolve_one_bd_step()
+        ob = oph.evolve_one_bd_step()
+        assert b.n_photons == ob["n_photons"], f"step {k}: photon count"
+        if b.n_photons:
+            np.testing.assert_array_equal(b.particle_idx, ob["particle_idx"])
+            np.testing.assert_array_equal(b.channel, ob["channel"])
+            np.testing.assert_array_equal(b.is_fret, ob["is_fret"].astype(bool))
+            np.testing.assert_allclose(b.arrival_time, ob["arrival_time"], rtol=1e-12, atol=0)
+        nd = mic.process_photons(b)
+        ond = omic.process_photons(ob)
+        assert nd == ond, f"step {k}: detected photons"
+        tot_ph += b.n_photons
+        tot_det += nd
+    compare_states(host_arrays(st), oracle_arrays(ost), label="final: ")
+    oc = odyn.counters()
+    assert (eng.n_bindings, eng.n_dissociations) == (oc["n_bindings"], oc["n_dissociations"])
+    fr = mic.finalize_frame()
+    ofr = omic.finalize_frame()
+    assert fr.n_dark_counts == ofr["n_dark"] and fr.n_afterpulses == ofr["n_ap"]
+    np.testing.assert_array_equal(fr.flim_stack, ofr["flim_stack"])
+    assert tot_ph > 0 and tot_det > 0
+    print(f"binding/FRET {variant}: photons {tot_ph}, detected {tot_det}, "
+          f"bindings {eng.n_bindings}, counts {fr.total_counts}")
+
+
+def test_fp32_philox_large_runs():
+    cfg = workloads.large_membrane(side=20.0, n_alive=40_000, pixels=128)
+    st = ParticleState(cfg)
+    eng = DynamicsEngine(cfg, st, precision="fp32", rng="philox", pair_cap=False)
+    ph = PhotonSimulator(cfg, st, min(1.0, cfg.scan.dwell_time / cfg.scan.frame_interval))
+    mic = VirtualMicroscope(cfg, st)
+    n0 = int(st.is_alive.sum())
+    for _ in range(50):
+        eng.step()
+        b = ph.evolve_one_bd_step()
+        mic.process_photons(b)
+    pos = st.positions
+    assert np.all(np.isfinite(pos))
+    assert np.all((pos >= 0) & (pos <= 20.0))
+    n1 = int(st.is_alive.sum())
+    assert abs(n1 - n0) < 0.2 * n0
+    fr = mic.finalize_frame()
+    assert fr.total_counts > 0
+    print("fp32 philox: catalytic", eng.n_catalytic, "detached", eng.n_detached,
+          "recruited", eng.n_recruited, "counts", fr.total_counts)
