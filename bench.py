@@ -1,0 +1,185 @@
+"""Benchmark of the per-timestep particle path on one MI355X.
+
+Workload: the 1,000,000-particle membrane (100 x 100 um periodic box, FP32
+positions, harmonic repulsion k = 1e7 kT/um^2 with r_c = 10 nm, the plain
+PIP3/PTEN reaction set with Hill feedback on a 200 x 200 density grid,
+PSS photon emission from PIP3-borne mTFP1, 1024 x 1024 px x 256-bin TCSPC
+imaging with frames finalised on the device every 100 steps).
+
+One step = cell binning + forces + Ermak-McCammon move + all reactions +
+photon emission, detection, PSF pixel draw and TCSPC binning.  The state
+is resident in HBM before the timed region starts.  ``value`` counts alive
+particles per step (particle-steps/s).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+Prints one JSON line.  The CPU baseline is the oracle (a C restatement of
+the reference hot path, bd_step threaded like the reference's prange) timed
+on a bounded sample of the same workload on this machine's host cores.
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "flim-microscopy-simulator-with-causal-inference_amd")
+sys.path.insert(0, PKG)
+
+METRIC = "particle-steps/s @1M; ensemble jobs/s + p99 latency, batched vs plain"
+HBM_PEAK_GBPS = 8000.0
+# algorithmic bytes of one force/BD launch per particle (FP32): read x, y
+# (8), species + flags (2), particle index (4); write x', y' (8)
+FORCE_BYTES_PER_PARTICLE = 22
+
+
+def gpu_run(steps: int, warmup: int, n_alive: int):
+    import numpy as np
+    from ed_mrdt import _hip, workloads
+    from ed_mrdt.dynamics import DynamicsEngine
+    from ed_mrdt.microscope import VirtualMicroscope
+    from ed_mrdt.particles import ParticleState
+    from ed_mrdt.photophysics import PhotonSimulator
+
+    side = 100.0 * (n_alive / 1e6) ** 0.5
+    cfg = workloads.large_membrane(side=side, n_alive=n_alive, pixels=1024)
+    st = ParticleState(cfg)
+    eng = DynamicsEngine(cfg, st, precision="fp32", rng="philox", pair_cap=False)
+    duty = min(1.0, cfg.scan.dwell_time / cfg.scan.frame_interval)
+    PhotonSimulator(cfg, st, duty)
+    mic = VirtualMicroscope(cfg, st)
+    L = _hip.lib()
+    h = eng.handle
+    frame_every = int(round(cfg.scan.frame_interval / cfg.bd_timestep))
+    tot, fms = ctypes.c_double(0), ctypes.c_double(0)
+
+    def run(k):
+        """k steps in chunks ending on frame boundaries; returns (ms, force_ms)."""
+        done = 0
+        ms = fm = 0.0
+        while done < k:
+            chunk = min(frame_every, k - done)
+            _hip.check(L.edm_time_steps(h, chunk, 1, ctypes.byref(tot), ctypes.byref(fms), None))
+            ms += tot.value
+            fm += fms.value
+            done += chunk
+            if done % frame_every == 0:
+                t0 = time.perf_counter()
+                _hip.check(L.edm_frame_noise(h, None, None))
+                _hip.check(L.edm_frame_reset(h))
+                _hip.check(L.edm_synchronize(h))
+                ms += (time.perf_counter() - t0) * 1e3
+        return ms, fm
+
+    if warmup:
+        run(warmup)
+    ph0 = np.zeros(2, np.int64)
+    _hip.check(L.edm_photon_counters(h, _hip.ptr(ph0)))
+    st.mark_stale()
+    alive0 = int(np.sum(st.is_alive))
+    _hip.check(L.edm_synchronize(h))
+    t0 = time.perf_counter()
+    ms, force_ms = run(steps)
+    _hip.check(L.edm_synchronize(h))
+    wall = time.perf_counter() - t0
+    st.mark_stale()
+    alive1 = int(np.sum(st.is_alive))
+    ph1 = np.zeros(2, np.int64)
+    _hip.check(L.edm_photon_counters(h, _hip.ptr(ph1)))
+    alive = 0.5 * (alive0 + alive1)
+    return dict(cfg=cfg, N=st.N, alive=alive, wall=wall, event_ms=ms, force_ms=force_ms,
+                steps=steps, emitted=int(ph1[0] - ph0[0]), detected=int(ph1[1] - ph0[1]))
+
+
+def cpu_baseline(n_alive: int, budget_s: float = 20.0):
+    """Oracle steps (dynamics + photophysics) on the 1M workload until the
+    budget is spent; returns alive particle-steps/s and the sample size."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import reference_model as rm
+    from ed_mrdt import workloads
+
+    side = 100.0 * (n_alive / 1e6) ** 0.5
+    cfg = workloads.large_membrane(side=side, n_alive=n_alive, pixels=1024)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    threads = max(1, min(threads, 16))
+    st = rm.OracleState(cfg)
+    dyn = rm.OracleDynamics(cfg, st, pair_cap=False, n_threads=threads)
+    duty = min(1.0, cfg.scan.dwell_time / cfg.scan.frame_interval)
+    ph = rm.OraclePhotons(cfg, st, duty)
+    dyn.step()
+    ph.evolve_one_bd_step()
+    steps = 0
+    ps = 0.0
+    t0 = time.perf_counter()
+    while True:
+        a = int(np.sum(st.is_alive))
+        dyn.step()
+        ph.evolve_one_bd_step()
+        steps += 1
+        ps += a
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 200:
+            break
+    return dict(value=ps / el, steps=steps, seconds=el, threads=threads)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not "
+                                   "part of it (run with --gpus 1)"}))
+        sys.exit(2)
+    r = gpu_run(a.steps, a.warmup, a.n)
+    value = r["alive"] * r["steps"] / r["wall"]
+    force_avg_s = r["force_ms"] / r["steps"] * 1e-3
+    achieved = FORCE_BYTES_PER_PARTICLE * r["N"] / force_avg_s / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": r["wall"] * 1e3 / r["steps"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "1M-particle PIP3/PTEN membrane, 100x100 um, harmonic k=1e7 kT/um^2, "
+                               "200x200 density grid, 1024x1024x256 TCSPC frame every 100 steps",
+                   "particles": r["N"], "alive_particles": int(r["alive"]),
+                   "step": "bin+force+BD+reactions+PSS photons+PSF/TCSPC imaging"},
+        "roofline": {"bound": "hbm", "kernel": "k_force_bd<float>",
+                     "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "avg_launch_us": force_avg_s * 1e6,
+                     "bytes_per_launch": FORCE_BYTES_PER_PARTICLE * r["N"]},
+        "photons_per_s": r["emitted"] / max(r["wall"], 1e-9),
+        "detected_per_s": r["detected"] / max(r["wall"], 1e-9),
+    }
+    if not a.no_cpu_baseline:
+        c = cpu_baseline(a.n, a.cpu_budget)
+        out["cpu_baseline"] = {"value": c["value"], "unit": "particle-steps/s", "cores": c["threads"],
+                               "kind": "port",
+                               "sample": f"{c['steps']} oracle steps (dynamics + photophysics, "
+                                         f"no pair cap) of the same 1M workload in {c['seconds']:.1f} s"}
+        out["speedup_vs_cpu"] = value / c["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

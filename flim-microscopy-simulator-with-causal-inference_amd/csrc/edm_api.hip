@@ -126,7 +126,8 @@ int edm_destroy(edm_membrane* m) {
                     m->ph_nd, m->ph_nf, m->ph_cnt, m->ph_off, m->ph_pid, m->ph_t, m->ph_ch,
                     m->ph_fret, m->flim, m->pcx, m->det_flag, m->det_pos, m->sv_key, m->sv_key2,
                     m->sv_val, m->sv_val2, m->asg_px, m->asg_py, m->asg_flag, m->asg_pos,
-                    m->asg_t, m->asg_ch, m->sv_px, m->sv_py, m->pix_tot, m->d_scal};
+                    m->asg_t, m->asg_ch, m->sv_px, m->sv_py, m->pix_tot, m->d_scal,
+                    m->lists, m->list_cnt, m->dye_ids};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (auto e : m->kev) hipEventDestroy(e);
@@ -177,6 +178,16 @@ int edm_upload_state(edm_membrane* m, const edm_host_state* s) {
     EDM_HIP(hipMemcpy(m->a.bond, bond.data(), n * 4, hipMemcpyHostToDevice));
     EDM_HIP(hipMemcpy(m->a.ctmc, ctmc.data(), n, hipMemcpyHostToDevice));
     EDM_HIP(hipMemcpy(m->a.dye, dye.data(), n, hipMemcpyHostToDevice));
+    // dye carriers never change: the photon kernels run over this list
+    std::vector<int32_t> dyes;
+    for (int i = 0; i < n; ++i)
+        if (fl[i] & EDM_FLAG_DYE) dyes.push_back(i);
+    if (m->dye_ids) EDM_HIP(hipFree(m->dye_ids));
+    m->dye_ids = nullptr;
+    EDM_HIP(hipMalloc(&m->dye_ids, std::max<size_t>(dyes.size(), 1) * sizeof(int32_t)));
+    if (!dyes.empty())
+        EDM_HIP(hipMemcpy(m->dye_ids, dyes.data(), dyes.size() * 4, hipMemcpyHostToDevice));
+    m->n_dye = (int)dyes.size();
     return 0;
 }
 
@@ -292,6 +303,16 @@ int edm_counters(edm_membrane* m, int64_t out[6]) {
     return 0;
 }
 
+int edm_photon_counters(edm_membrane* m, int64_t out[2]) {
+    if (!m || !out) return invalid("edm_photon_counters: null argument");
+    unsigned long long c[8];
+    EDM_HIP(hipMemcpyAsync(c, m->d_ctr, sizeof(c), hipMemcpyDeviceToHost, m->stream));
+    EDM_HIP(hipStreamSynchronize(m->stream));
+    out[0] = (int64_t)c[6];
+    out[1] = (int64_t)c[7];
+    return 0;
+}
+
 int edm_pairs(edm_membrane* m, int32_t* pairs, int64_t cap, int64_t* n_pairs) {
     if (!m || !pairs || !n_pairs) return invalid("edm_pairs: null argument");
     return dyn_pairs(m, pairs, cap, n_pairs);
@@ -335,16 +356,21 @@ int edm_fret_rates(edm_membrane* m, double* k_fret, int32_t* acceptor_idx) {
         return EDM_ESTATE;
     }
     if (int rc = ph_fret(m)) return rc;
-    const int n = m->n;
-    std::vector<double> k(n);
-    std::vector<int32_t> a(n), id(n);
+    const int n = m->n, nd = m->n_dye;
+    if (k_fret)
+        for (int i = 0; i < n; ++i) k_fret[i] = 0.0;
+    if (acceptor_idx)
+        for (int i = 0; i < n; ++i) acceptor_idx[i] = -1;
+    if (nd == 0) return 0;
+    std::vector<double> k(nd);
+    std::vector<int32_t> a(nd), id(nd);
     EDM_HIP(hipStreamSynchronize(m->stream));
-    EDM_HIP(hipMemcpy(k.data(), m->k_fret, n * 8, hipMemcpyDeviceToHost));
-    EDM_HIP(hipMemcpy(a.data(), m->acc, n * 4, hipMemcpyDeviceToHost));
-    EDM_HIP(hipMemcpy(id.data(), m->a.id, n * 4, hipMemcpyDeviceToHost));
-    for (int t = 0; t < n; ++t) {
-        if (k_fret) k_fret[id[t]] = k[t];
-        if (acceptor_idx) acceptor_idx[id[t]] = a[t];
+    EDM_HIP(hipMemcpy(k.data(), m->k_fret, nd * 8, hipMemcpyDeviceToHost));
+    EDM_HIP(hipMemcpy(a.data(), m->acc, nd * 4, hipMemcpyDeviceToHost));
+    EDM_HIP(hipMemcpy(id.data(), m->dye_ids, nd * 4, hipMemcpyDeviceToHost));
+    for (int q = 0; q < nd; ++q) {
+        if (k_fret) k_fret[id[q]] = k[q];
+        if (acceptor_idx) acceptor_idx[id[q]] = a[q];
     }
     return 0;
 }

@@ -194,41 +194,43 @@ __global__ void k_scatter(int n, const uint32_t* __restrict__ key, const uint32_
 }
 
 // One thread per cell: order the cell's particles by particle index (the
-// atomic ranks of k_bin are arbitrary) and move every field to its new slot.
-template <typename Real>
-__global__ void k_sort_permute(int ncell, const uint32_t* __restrict__ start, uint32_t* perm,
-                               uint32_t* cnt, DevArrays a) {
+// atomic ranks of k_bin are arbitrary) and clear the cell counter.
+__global__ void k_sort_cells(int ncell, const uint32_t* __restrict__ start, uint32_t* perm,
+                             uint32_t* cnt, const int32_t* __restrict__ id) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
     cnt[c] = 0;
     const int lo = (int)start[c], hi = (int)start[c + 1];
+    if (hi - lo < 2) return;
     for (int i = lo + 1; i < hi; ++i) {
         const uint32_t v = perm[i];
-        const int32_t vid = a.id[v];
+        const int32_t vid = id[v];
         int j = i - 1;
-        while (j >= lo && a.id[perm[j]] > vid) {
+        while (j >= lo && id[perm[j]] > vid) {
             perm[j + 1] = perm[j];
             --j;
         }
         perm[j + 1] = v;
     }
-    const Real* x = (const Real*)a.x;
-    const Real* y = (const Real*)a.y;
-    Real* xt = (Real*)a.xt;
-    Real* yt = (Real*)a.yt;
-    for (int t = lo; t < hi; ++t) {
-        const uint32_t s = perm[t];
-        xt[t] = x[s];
-        yt[t] = y[s];
-        a.sp_t[t] = a.sp[s];
-        a.fl_t[t] = a.fl[s];
-        const int32_t pid = a.id[s];
-        a.id_t[t] = pid;
-        a.bond_t[t] = a.bond[s];
-        a.ctmc_t[t] = a.ctmc[s];
-        a.dye_t[t] = a.dye[s];
-        a.slot_of[pid] = t;
-    }
+}
+
+// One thread per new slot: gather every field from its old slot (the
+// permutation is close to the identity, so reads stay nearly coalesced).
+template <typename Real>
+__global__ void k_gather(int n, const uint32_t* __restrict__ perm, DevArrays a) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t s = perm[t];
+    ((Real*)a.xt)[t] = ((const Real*)a.x)[s];
+    ((Real*)a.yt)[t] = ((const Real*)a.y)[s];
+    a.sp_t[t] = a.sp[s];
+    a.fl_t[t] = a.fl[s];
+    const int32_t pid = a.id[s];
+    a.id_t[t] = pid;
+    a.bond_t[t] = a.bond[s];
+    a.ctmc_t[t] = a.ctmc[s];
+    a.dye_t[t] = a.dye[s];
+    a.slot_of[pid] = t;
 }
 
 // Reference pair-list bookkeeping: per particle index, the number of
@@ -380,6 +382,66 @@ __device__ __forceinline__ void dfield_eval(const ForceParams<Real>& F, int s, R
     gy = base * gfy;
 }
 
+// Ermak-McCammon move of one alive particle (dynamics.py:359-406): Ito
+// drift grad(D) dt, noise sqrt(2 D dt) xi, force drift D F dt, displacement
+// cap on |dr|, then periodic wrap (Python modulo) or one reflection.
+template <typename Real>
+__device__ __forceinline__ void move_particle(const Cells<Real>& C, const ForceParams<Real>& F,
+                                              int si, int it, Real xt, Real yt, Real Fx, Real Fy,
+                                              const double* __restrict__ noise_hook, Rng rng,
+                                              uint32_t step, Real& nxp, Real& nyp) {
+    Real D, gx, gy;
+    dfield_eval(F, si, xt, yt, D, gx, gy);
+    Real n0, n1;
+    if (noise_hook) {
+        n0 = (Real)noise_hook[2 * (int64_t)it];
+        n1 = (Real)noise_hook[2 * (int64_t)it + 1];
+    } else {
+        normal2<Real>(rng.draw(step, (uint32_t)it, 0u, kTagNoise), n0, n1);
+    }
+    const Real sn = sqrt((Real)2 * D * F.dt);
+    Real dx = gx * F.dt;
+    Real dy = gy * F.dt;
+    dx += sn * n0;
+    dy += sn * n1;
+    dx += D * Fx * F.dt;
+    dy += D * Fy * F.dt;
+    const Real dr2 = dx * dx + dy * dy;
+    if (dr2 > F.max_disp * F.max_disp) {
+        const Real dr = sqrt(dr2);
+        const Real sc = F.max_disp / dr;
+        dx *= sc;
+        dy *= sc;
+    }
+    Real px_ = xt + dx, py_ = yt + dy;
+    if (C.px) px_ = py_mod(px_, C.Lx);
+    else {
+        if (px_ < (Real)0) px_ = -px_;
+        else if (px_ >= C.Lx) px_ = (Real)2 * C.Lx - px_;
+    }
+    if (C.py) py_ = py_mod(py_, C.Ly);
+    else {
+        if (py_ < (Real)0) py_ = -py_;
+        else if (py_ >= C.Ly) py_ = (Real)2 * C.Ly - py_;
+    }
+    nxp = px_;
+    nyp = py_;
+}
+
+// Density counts of the kept (sensor) species at the post-move position;
+// every lane of the wavefront must call this together.
+template <typename Real>
+__device__ __forceinline__ void add_density(const Grids<Real>& G, int n_grids, bool take_any,
+                                            int si, Real x, Real y) {
+    for (int g = 0; g < n_grids; ++g) {
+        int o = -1;
+        if (take_any) o = G.off[g * G.ns + si];
+        const bool take = take_any && o >= 0;
+        const int idx = take ? o + grid_cell(G, g, x, y) : 0;
+        wave_aggregated_add(G.counts, idx, take, 1);
+    }
+}
+
 // kOrdered: sum pair forces in the reference's pair order (all partners
 // with a lower particle index ascending, then the particle's own
 // enumeration order), WCA over the whole list first, then harmonic — the
@@ -484,54 +546,165 @@ __global__ __launch_bounds__(256) void k_force_bd(int n, Cells<Real> C, ForcePar
                     }
                 }
             }
-            Real D, gx, gy;
-            dfield_eval(F, si, xt, yt, D, gx, gy);
-            Real n0, n1;
-            if (noise_hook) {
-                n0 = (Real)noise_hook[2 * (int64_t)it];
-                n1 = (Real)noise_hook[2 * (int64_t)it + 1];
-            } else {
-                normal2<Real>(rng.draw(step, (uint32_t)it, 0u, kTagNoise), n0, n1);
-            }
-            const Real sn = sqrt((Real)2 * D * F.dt);
-            Real dx = gx * F.dt;
-            Real dy = gy * F.dt;
-            dx += sn * n0;
-            dy += sn * n1;
-            dx += D * Fx * F.dt;
-            dy += D * Fy * F.dt;
-            const Real dr2 = dx * dx + dy * dy;
-            if (dr2 > F.max_disp * F.max_disp) {
-                const Real dr = sqrt(dr2);
-                const Real sc = F.max_disp / dr;
-                dx *= sc;
-                dy *= sc;
-            }
-            Real px_ = xt + dx, py_ = yt + dy;
-            if (C.px) px_ = py_mod(px_, C.Lx);
-            else {
-                if (px_ < (Real)0) px_ = -px_;
-                else if (px_ >= C.Lx) px_ = (Real)2 * C.Lx - px_;
-            }
-            if (C.py) py_ = py_mod(py_, C.Ly);
-            else {
-                if (py_ < (Real)0) py_ = -py_;
-                else if (py_ >= C.Ly) py_ = (Real)2 * C.Ly - py_;
-            }
-            nxp = px_;
-            nyp = py_;
+            move_particle(C, F, si, it, xt, yt, Fx, Fy, noise_hook, rng, step, nxp, nyp);
             take_any = ((keep_mask >> si) & 1u) != 0;
         }
         xn[t] = nxp;
         yn[t] = nyp;
     }
-    // density counts of the kept (sensor) species at the post-move position
-    for (int g = 0; g < n_grids; ++g) {
-        int o = -1;
-        if (take_any) o = G.off[g * G.ns + si];
-        const bool take = take_any && o >= 0;
-        const int idx = take ? o + grid_cell(G, g, nxp, nyp) : 0;
-        wave_aggregated_add(G.counts, idx, take, 1);
+    add_density(G, n_grids, take_any, si, nxp, nyp);
+}
+
+// LDS-tiled forces + move for throughput runs (any pair summation order).
+// A workgroup owns a strip of kTileRows cells of one cell column; the three
+// neighbouring columns' rows (strip +/- 1 row) are staged in LDS with
+// coalesced loads, so every particle's 3x3 neighbourhood is three
+// contiguous LDS ranges.  Blocks that the dispatcher sends to the same XCD
+// (blockIdx % 8) take adjacent strips, so neighbouring strips share that
+// XCD's L2.  A strip whose neighbourhood exceeds the LDS capacity falls
+// back to global-memory neighbour walks.
+static constexpr int kTileRows = 64;
+
+template <typename Real>
+__global__ __launch_bounds__(256) void k_force_tile(Cells<Real> C, ForceParams<Real> F,
+                                                    Real* __restrict__ xn, Real* __restrict__ yn,
+                                                    const double* __restrict__ noise_hook,
+                                                    Rng rng, uint32_t step, Grids<Real> G,
+                                                    int n_grids, uint32_t keep_mask, int ntile_y,
+                                                    int ntiles) {
+    constexpr int kCap = sizeof(Real) == 8 ? 2048 : 4096;
+    constexpr int kCells = 3 * (kTileRows + 2);
+    __shared__ Real s_x[kCap], s_y[kCap];
+    __shared__ int8_t s_sp[kCap];
+    __shared__ uint8_t s_fl[kCap];
+    __shared__ int s_off[kCells + 1];
+    __shared__ int s_gst[kCells];
+
+    const int per = (ntiles + 7) / 8;
+    const int tile = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    if (tile >= ntiles) return;  // uniform across the block
+    const int cx = tile / ntile_y;
+    const int cy0 = (tile % ntile_y) * kTileRows;
+    const int rows = min(kTileRows, C.ncy - cy0);
+    const int nrow = rows + 2;
+    const int ncells = 3 * nrow;
+    for (int k = threadIdx.x; k < ncells; k += blockDim.x) {
+        const int col = k / nrow, r = k % nrow;
+        int gx = cx + col - 1, gy = cy0 + r - 1;
+        bool ok = true;
+        if (C.px) gx = (gx + C.ncx) % C.ncx;
+        else if (gx < 0 || gx >= C.ncx) ok = false;
+        if (C.py) gy = (gy + C.ncy) % C.ncy;
+        else if (gy < 0 || gy >= C.ncy) ok = false;
+        int gs = 0, cnt = 0;
+        if (ok) {
+            const int c = gx * C.ncy + gy;
+            gs = (int)C.start[c];
+            cnt = (int)C.start[c + 1] - gs;
+        }
+        s_gst[k] = gs;
+        s_off[k] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int k = 0; k < ncells; ++k) {
+            const int c = s_off[k];
+            s_off[k] = acc;
+            acc += c;
+        }
+        s_off[ncells] = acc;
+    }
+    __syncthreads();
+    const int total = s_off[ncells];
+    const bool staged = total <= kCap;
+    if (staged) {
+        for (int i = threadIdx.x; i < total; i += blockDim.x) {
+            int lo = 0, hi = ncells - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_off[mid] <= i) lo = mid;
+                else hi = mid - 1;
+            }
+            const int g = s_gst[lo] + (i - s_off[lo]);
+            s_x[i] = C.x[g];
+            s_y[i] = C.y[g];
+            s_sp[i] = C.sp[g];
+            s_fl[i] = C.fl[g];
+        }
+    }
+    __syncthreads();
+    // own particles: the middle column's rows 1..rows (contiguous in memory)
+    const int own_begin = s_off[nrow + 1], own_end = s_off[nrow + rows + 1];
+    const int gbase = s_gst[nrow + 1];
+    for (int i0 = own_begin; i0 < own_end; i0 += blockDim.x) {
+        const int i = i0 + (int)threadIdx.x;
+        bool take_any = false;
+        Real nxp = 0, nyp = 0;
+        int si = 0;
+        if (i < own_end) {
+            const int t = gbase + (i - own_begin);
+            const Real xt = C.x[t], yt = C.y[t];
+            si = C.sp[t];
+            nxp = xt;
+            nyp = yt;
+            if (C.fl[t] & EDM_FLAG_ALIVE) {
+                Real Fx = 0, Fy = 0;
+                int cxx, cyy;
+                cell_of(C, xt, yt, cxx, cyy);
+                if (staged) {
+                    const int r = cyy - cy0 + 1;  // staged row of this particle
+                    for (int col = 0; col < 3; ++col) {
+                        const int k0 = col * nrow + r - 1;
+                        const int ub = s_off[k0 + 3];
+                        for (int u = s_off[k0]; u < ub; ++u) {
+                            if (u == i) continue;
+                            Real dx, dy;
+                            const Real dsq = min_image(xt, yt, s_x[u], s_y[u], C.Lx, C.Ly, C.px,
+                                                       C.py, dx, dy);
+                            if (dsq >= C.cut2 || !(s_fl[u] & EDM_FLAG_ALIVE)) continue;
+                            const int su = s_sp[u];
+                            Real f;
+                            if (F.wca && pair_force(F, 0, si, su, dsq, f)) {
+                                Fx += f * dx;
+                                Fy += f * dy;
+                            }
+                            if (F.harm && pair_force(F, 1, si, su, dsq, f)) {
+                                Fx += f * dx;
+                                Fy += f * dy;
+                            }
+                        }
+                    }
+                } else {
+                    for (int k = 0; k < 9; ++k) {
+                        const int c = nb_cell(C, cxx, cyy, k);
+                        if (c < 0) continue;
+                        for (int u = (int)C.start[c]; u < (int)C.start[c + 1]; ++u) {
+                            if (u == t) continue;
+                            Real dx, dy;
+                            const Real dsq = min_image(xt, yt, C.x[u], C.y[u], C.Lx, C.Ly, C.px,
+                                                       C.py, dx, dy);
+                            if (dsq >= C.cut2 || !(C.fl[u] & EDM_FLAG_ALIVE)) continue;
+                            const int su = C.sp[u];
+                            Real f;
+                            if (F.wca && pair_force(F, 0, si, su, dsq, f)) {
+                                Fx += f * dx;
+                                Fy += f * dy;
+                            }
+                            if (F.harm && pair_force(F, 1, si, su, dsq, f)) {
+                                Fx += f * dx;
+                                Fy += f * dy;
+                            }
+                        }
+                    }
+                }
+                move_particle(C, F, si, C.id[t], xt, yt, Fx, Fy, noise_hook, rng, step, nxp, nyp);
+                take_any = ((keep_mask >> si) & 1u) != 0;
+            }
+            xn[t] = nxp;
+            yn[t] = nyp;
+        }
+        add_density(G, n_grids, take_any, si, nxp, nyp);
     }
 }
 
@@ -727,17 +900,55 @@ __global__ void k_adapt(Grids<Real> G, int grid, int species, double* w, double 
     w[c] += rate * (rho - w[c]);
 }
 
-// dynamics.py:826-1090 catalysis: enzyme-centred sweep of the pair list;
-// the substrate flips once however many of its pairs succeed (byte CAS),
-// which is the outcome of the reference's sequential loop.
+// Compact the slots of alive particles of up to 8 species into per-species
+// lists (wavefront ballot + one atomic per wavefront and list).
+struct ListSpec {
+    int n_lists;
+    int species[8];
+};
+
+__global__ void k_compact(int n, const int8_t* __restrict__ sp, const uint8_t* __restrict__ fl,
+                          ListSpec spec, int32_t* __restrict__ lists, unsigned int* counts) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = t < n;
+    const int s = in ? sp[t] : -1;
+    const bool alive = in && (fl[t] & EDM_FLAG_ALIVE);
+    const int lane = lane_id();
+    for (int l = 0; l < spec.n_lists; ++l) {
+        const bool take = alive && s == spec.species[l];
+        const uint64_t m = __ballot(take);
+        if (!m) continue;
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        unsigned int base = 0;
+        if (lane == leader) base = atomicAdd(&counts[l], (unsigned int)__popcll(m));
+        base = __shfl(base, leader);
+        if (take) {
+            const uint64_t below = m & ((1ull << lane) - 1ull);
+            lists[(int64_t)l * n + base + __popcll(below)] = t;
+        }
+    }
+}
+
+__global__ void k_zero_u32(int n, unsigned int* p) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// dynamics.py:826-1090 catalysis: enzyme-centred sweep of the pair list
+// over the compacted enzyme slots; the substrate flips once however many of
+// its pairs succeed (byte CAS), which is the outcome of the reference's
+// sequential loop.
 template <typename Real>
-__global__ void k_cat(int n, Cells<Real> C, Post<Real> P, int enz, int sub, int prod, Real contact2,
+__global__ void k_cat(const int32_t* __restrict__ list, const unsigned int* list_n, Cells<Real> C,
+                      Post<Real> P, int enz, int sub, int prod, Real contact2,
                       Real prob, FbDev fb, const double* w, int wnx, int wny, Real Kw, Real nw,
                       Draws dr, Rng rng, uint32_t step, uint32_t rxi, unsigned long long* ctr,
                       Grids<Real> G, int n_grids, bool deferred) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     int hits = 0;
-    if (t < n && C.sp[t] == enz && (C.fl[t] & EDM_FLAG_ALIVE)) {
+    const int nl = (int)*list_n;
+    for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < nl; li += gridDim.x * blockDim.x) {
+    const int t = list[li];
+    if (C.sp[t] == enz && (C.fl[t] & EDM_FLAG_ALIVE)) {
         const Real xt = C.x[t], yt = C.y[t];
         const int it = C.id[t];
         int cx, cy;
@@ -794,6 +1005,7 @@ __global__ void k_cat(int n, Cells<Real> C, Post<Real> P, int enz, int sub, int 
                 }
             }
         }
+    }
     }
     unsigned long long v = hits;
     // wavefront reduction then one atomic
@@ -1101,6 +1313,47 @@ static int create_t(edm_membrane* m) {
         if (dalloc(&m->w_adapt, (size_t)g.nx * g.ny)) return EDM_EHIP;
         EDM_HIP(hipMemset(m->w_adapt, 0, (size_t)g.nx * g.ny * sizeof(double)));
     }
+    // enzyme lists: one compaction per run of consecutive catalytic passes
+    // when no enzyme of the run is a substrate or product of the run (the
+    // enzyme sets then cannot change inside it); else one per pass.
+    const size_t nr = m->rx.size();
+    m->cat_list.assign(nr, -1);
+    m->compact_n.assign(nr, 0);
+    m->compact_species.assign(nr, std::vector<int>());
+    int max_lists = 1;
+    for (size_t i = 0; i < nr;) {
+        if (m->rx[i].kind != EDM_RX_CAT) {
+            ++i;
+            continue;
+        }
+        size_t j = i;
+        while (j < nr && m->rx[j].kind == EDM_RX_CAT) ++j;
+        std::vector<int> enz;
+        bool shared = true;
+        for (size_t k = i; k < j; ++k) {
+            if (std::find(enz.begin(), enz.end(), m->rx[k].a) == enz.end()) enz.push_back(m->rx[k].a);
+        }
+        for (size_t k = i; k < j; ++k)
+            for (int e : enz)
+                if (m->rx[k].b == e || m->rx[k].c == e) shared = false;
+        if (enz.size() > 8) shared = false;
+        if (shared) {
+            m->compact_species[i] = enz;
+            m->compact_n[i] = (int)enz.size();
+            for (size_t k = i; k < j; ++k)
+                m->cat_list[k] = (int)(std::find(enz.begin(), enz.end(), m->rx[k].a) - enz.begin());
+            max_lists = std::max(max_lists, (int)enz.size());
+        } else {
+            for (size_t k = i; k < j; ++k) {
+                m->compact_species[k] = {m->rx[k].a};
+                m->compact_n[k] = 1;
+                m->cat_list[k] = 0;
+            }
+        }
+        i = j;
+    }
+    m->n_lists_alloc = max_lists;
+    if (dalloc(&m->lists, (size_t)n * max_lists) || dalloc(&m->list_cnt, 8)) return EDM_EHIP;
     // reaction scratch
     m->edge_cap = (size_t)std::max(n, 1024);
     if (dalloc(&m->edge_key, m->edge_cap) || dalloc(&m->edge_a, m->edge_cap) ||
@@ -1144,8 +1397,10 @@ static int begin_t(edm_membrane* m) {
                                              m->ncell + 1, st));
     hipLaunchKernelGGL(k_scatter, dim3(nblk(n)), dim3(kBlock), 0, st, n, m->key, m->rank,
                        m->cell_start, m->perm);
-    hipLaunchKernelGGL(k_sort_permute<Real>, dim3(nblk(m->ncell)), dim3(kBlock), 0, st, m->ncell,
-                       m->cell_start, m->perm, m->cell_cnt, m->a);
+    hipLaunchKernelGGL(k_sort_cells, dim3(nblk(m->ncell)), dim3(kBlock), 0, st, m->ncell,
+                       m->cell_start, m->perm, m->cell_cnt, (const int32_t*)m->a.id);
+    hipLaunchKernelGGL(k_gather<Real>, dim3(nblk(n)), dim3(kBlock), 0, st, n,
+                       (const uint32_t*)m->perm, m->a);
     std::swap(m->a.x, m->a.xt);
     std::swap(m->a.y, m->a.yt);
     std::swap(m->a.sp, m->a.sp_t);
@@ -1209,14 +1464,21 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
     F.max_disp = (Real)p.max_disp;
     const double* noise = hook ? draws : nullptr;
     if (m->time_force) EDM_HIP(hipEventRecord(m->kev[0], st));
-    if (hook && m->fp64)
+    if (hook && m->fp64 && p.exact_force_order)
         hipLaunchKernelGGL((k_force_bd<Real, true>), dim3(nblk(n)), dim3(kBlock), 0, st, n, C, F,
                            (Real*)m->a.xn, (Real*)m->a.yn, noise, rng, step, G, ng,
                            m->sensor_mask_all);
-    else
+    else if (C.cap_active)
         hipLaunchKernelGGL((k_force_bd<Real, false>), dim3(nblk(n)), dim3(kBlock), 0, st, n, C, F,
                            (Real*)m->a.xn, (Real*)m->a.yn, noise, rng, step, G, ng,
                            m->sensor_mask_all);
+    else {
+        const int ntile_y = (p.ncy + kTileRows - 1) / kTileRows;
+        const int ntiles = p.ncx * ntile_y;
+        hipLaunchKernelGGL((k_force_tile<Real>), dim3(8 * ((ntiles + 7) / 8)), dim3(256), 0, st, C,
+                           F, (Real*)m->a.xn, (Real*)m->a.yn, noise, rng, step, G, ng,
+                           m->sensor_mask_all, ntile_y, ntiles);
+    }
     if (m->time_force) EDM_HIP(hipEventRecord(m->kev[1], st));
 
     int64_t off = 2 * (int64_t)n;
@@ -1228,7 +1490,7 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
         const FbDev f1 = to_fb(r.fb1), f2 = to_fb(r.fb2);
         switch (r.kind) {
             case EDM_RX_BIND: {
-                EDM_HIP(hipMemsetAsync(m->n_edges, 0, sizeof(unsigned int), st));
+                hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, 1, m->n_edges);
                 const Real c2 = (Real)r.contact * (Real)r.contact;
                 hipLaunchKernelGGL(k_bind_edges<Real>, dim3(nblk(n)), dim3(kBlock), 0, st, n, C, P,
                                    r.a, r.b, c2, (Real)r.prob, dr, rng, step, rxi, m->edge_key,
@@ -1262,7 +1524,19 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
                     wnx = m->grids[p.adapt_grid].nx;
                     wny = m->grids[p.adapt_grid].ny;
                 }
-                hipLaunchKernelGGL(k_cat<Real>, dim3(nblk(n)), dim3(kBlock), 0, st, n, C, P, r.a,
+                if (m->compact_n[ri] > 0) {
+                    ListSpec spec{};
+                    spec.n_lists = m->compact_n[ri];
+                    for (int l = 0; l < spec.n_lists; ++l) spec.species[l] = m->compact_species[ri][l];
+                    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, 8, m->list_cnt);
+                    hipLaunchKernelGGL(k_compact, dim3(nblk(n)), dim3(kBlock), 0, st, n,
+                                       (const int8_t*)m->a.sp, (const uint8_t*)m->a.fl, spec,
+                                       m->lists, m->list_cnt);
+                }
+                const int li = m->cat_list[ri];
+                hipLaunchKernelGGL(k_cat<Real>, dim3(std::min(nblk(n), 2048)), dim3(kBlock), 0, st,
+                                   (const int32_t*)(m->lists + (int64_t)li * n),
+                                   (const unsigned int*)(m->list_cnt + li), C, P, r.a,
                                    r.b, r.c, c2, (Real)r.prob, f1, w, wnx, wny, (Real)p.adapt_K,
                                    (Real)p.adapt_n, dr, rng, step, rxi, m->d_ctr, G, ng, defer);
                 if (defer && m->n_count)
@@ -1273,7 +1547,7 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
             }
             case EDM_RX_RECRUIT: {
                 if (r.prob > 0.0) {
-                    EDM_HIP(hipMemsetAsync(m->n_cand, 0, sizeof(unsigned int), st));
+                    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, 1, m->n_cand);
                     hipLaunchKernelGGL(k_recruit_flag<Real>, dim3(nblk(n)), dim3(kBlock), 0, st, n,
                                        (const int32_t*)m->a.id, P, r.a, (Real)r.prob, (Real)p.Lx,
                                        (Real)p.Ly, f1, f2, dr, rng, step, rxi, G, m->cand_id,

@@ -91,6 +91,14 @@ struct edm_membrane {
     double* d_draws = nullptr;
     size_t draws_cap = 0;
 
+    // compacted per-species slot lists (enzymes of the catalytic passes)
+    int32_t* lists = nullptr;
+    unsigned int* list_cnt = nullptr;
+    int n_lists_alloc = 0;
+    std::vector<int> cat_list;          // per reaction: list index used by CAT
+    std::vector<int> compact_n;         // per reaction: lists built before it (0 = none)
+    std::vector<std::vector<int>> compact_species;
+
     // reaction scratch
     unsigned long long* edge_key = nullptr;
     int32_t *edge_a = nullptr, *edge_b = nullptr;
@@ -106,8 +114,10 @@ struct edm_membrane {
     bool has_pp = false;
     edm_photo_params pp{};
     double* d_pp = nullptr;  // packed photophysics tables (see edm_photons.hip)
-    double* k_fret = nullptr;
-    int32_t* acc = nullptr;      // by slot: acceptor particle index or -1
+    int32_t* dye_ids = nullptr;  // particle indices carrying a dye (fixed at upload)
+    int n_dye = 0;
+    double* k_fret = nullptr;    // by position in dye_ids
+    int32_t* acc = nullptr;      // by position in dye_ids: acceptor particle index or -1
     int32_t* ph_nd = nullptr;    // by particle index
     int32_t* ph_nf = nullptr;
     int64_t* ph_cnt = nullptr;   // by particle index, then scanned in place

@@ -37,34 +37,37 @@ __global__ void k_fret_zero(int n, double* k_fret, int32_t* acc) {
 }
 
 // photophysics.py:529-588, one bond-mode FRET map
-__global__ void k_fret_bond(int n, const edm_photo_params* pp, int j, const uint8_t* fl,
-                            const int8_t* dye, const int32_t* bond, const int32_t* slot_of,
-                            double* k_fret, int32_t* acc) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+__global__ void k_fret_bond(int n_dye, const int32_t* dye_ids, const edm_photo_params* pp, int j,
+                            const uint8_t* fl, const int8_t* dye, const int32_t* bond,
+                            const int32_t* slot_of, double* k_fret, int32_t* acc) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n_dye) return;
+    const int t = slot_of[dye_ids[q]];
     if (!(fl[t] & EDM_FLAG_DYE) || (fl[t] & EDM_FLAG_BLEACHED)) return;
     if (dye[t] != pp->fm_donor[j]) return;
     const int partner = bond[t];
     if (partner < 0) return;
     if (pp->fm_intra[j]) {
-        k_fret[t] = pp->fm_kfret[j];
-        acc[t] = -1;
+        k_fret[q] = pp->fm_kfret[j];
+        acc[q] = -1;
     } else {
         const int s = slot_of[partner];
         if ((fl[s] & EDM_FLAG_DYE) && dye[s] == pp->fm_acceptor[j]) {
-            k_fret[t] = pp->fm_kfret[j];
-            acc[t] = partner;
+            k_fret[q] = pp->fm_kfret[j];
+            acc[q] = partner;
         }
     }
 }
 
 // photophysics.py:592-683, brute-force proximity search (as the reference)
 template <typename Real>
-__global__ void k_fret_prox(int n, const edm_photo_params* pp, int j, const Real* x, const Real* y,
+__global__ void k_fret_prox(int n_dye, const int32_t* dye_ids, const int32_t* slot_of, int n,
+                            const edm_photo_params* pp, int j, const Real* x, const Real* y,
                             const int8_t* sp, const uint8_t* fl, const int8_t* dye, Real Lx,
                             Real Ly, bool px, bool py, double* k_fret, int32_t* acc) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n_dye) return;
+    const int t = slot_of[dye_ids[q]];
     const uint8_t f = fl[t];
     if (!(f & EDM_FLAG_DYE) || !(f & EDM_FLAG_ALIVE) || (f & EDM_FLAG_BLEACHED)) return;
     if (dye[t] != pp->fm_donor[j]) return;
@@ -86,8 +89,8 @@ __global__ void k_fret_prox(int n, const edm_photo_params* pp, int j, const Real
         }
         if (dx * dx + dy * dy < r2) found = true;
     }
-    k_fret[t] = found ? pp->fm_k_hi[j] : pp->fm_k_lo[j];
-    acc[t] = -1;
+    k_fret[q] = found ? pp->fm_k_hi[j] : pp->fm_k_lo[j];
+    acc[q] = -1;
 }
 
 // ----------------------------------------------------------------------
@@ -154,17 +157,19 @@ __device__ __forceinline__ bool pss_eval(const edm_photo_params* pp, uint8_t& fl
     return true;
 }
 
-__global__ void k_pss_count(int n, const edm_photo_params* pp, const int32_t* id, uint8_t* fl,
-                            const int8_t* dye, int8_t* ctmc, const double* k_fret,
-                            const int32_t* acc, const double* u_pois, const double* u_trans,
-                            int32_t* nd_out, int32_t* nf_out, int64_t* cnt) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int it = id[t];
+__global__ void k_pss_count(int n_dye, const int32_t* dye_ids, const int32_t* slot_of,
+                            const edm_photo_params* pp, uint8_t* fl, const int8_t* dye,
+                            int8_t* ctmc, const double* k_fret, const int32_t* acc,
+                            const double* u_pois, const double* u_trans, int32_t* nd_out,
+                            int32_t* nf_out, int64_t* cnt) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n_dye) return;
+    const int it = dye_ids[q];
+    const int t = slot_of[it];
     uint8_t f = fl[t];
     int8_t c = ctmc[t];
     PssOut o{0, 0, 0.0, 0};
-    const bool on = pss_eval(pp, f, c, dye[t], k_fret[t], acc[t], u_pois[it], u_trans[it], o);
+    const bool on = pss_eval(pp, f, c, dye[t], k_fret[q], acc[q], u_pois[it], u_trans[it], o);
     fl[t] = f;
     ctmc[t] = c;
     nd_out[it] = on ? o.nd : 0;
@@ -172,19 +177,20 @@ __global__ void k_pss_count(int n, const edm_photo_params* pp, const int32_t* id
     cnt[it] = on ? (int64_t)(o.nd + o.nf) : 0;
 }
 
-__global__ void k_pss_emit(int n, const edm_photo_params* pp, const int32_t* id,
+__global__ void k_pss_emit(int n_dye, const int32_t* dye_ids, const edm_photo_params* pp,
                            const int8_t* dye, const int32_t* slot_of, const double* k_fret,
                            const int32_t* acc, const int32_t* nd_in, const int32_t* nf_in,
                            const int64_t* off, const double* u_arr, int64_t n_arr,
                            int64_t max_photons, int32_t* o_pid, double* o_t, int8_t* o_ch,
                            uint8_t* o_fret) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int it = id[t];
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n_dye) return;
+    const int it = dye_ids[q];
+    const int t = slot_of[it];
     const int nd = nd_in[it], nf = nf_in[it];
     if (nd + nf == 0) return;
     const int ty = dye[t];
-    const double kf = k_fret[t];
+    const double kf = k_fret[q];
     const double tau = kf > 0.0 ? pp->tau[ty] / (1.0 + kf * pp->tau[ty]) : pp->tau[ty];
     const int64_t base = off[it];
     for (int k = 0; k < nd; ++k) {
@@ -204,7 +210,7 @@ __global__ void k_pss_emit(int n, const edm_photo_params* pp, const int32_t* id,
         o_fret[pos] = 0;
     }
     if (nf > 0) {
-        const int a = acc[t];
+        const int a = acc[q];
         const int at = dye[slot_of[a]];
         const double tau_a = (at >= 0 && at < pp->n_fluoro) ? pp->tau[at] : tau;
         for (int k = 0; k < nf; ++k) {
@@ -299,22 +305,24 @@ __device__ __forceinline__ void tcspc_add(const edm_image_params* ip, uint32_t* 
 // Fused per-emitter photon path with Philox draws
 // ----------------------------------------------------------------------
 template <typename Real>
-__global__ void k_pss_image(int n, const edm_photo_params* pp, const edm_image_params* ip,
-                            const double* pcx, const double* pcy, const Real* x, const Real* y,
-                            const int32_t* id, uint8_t* fl, const int8_t* dye, int8_t* ctmc,
-                            const double* k_fret, const int32_t* acc, const int32_t* slot_of,
-                            uint32_t* flim, Rng rng, uint32_t step, unsigned long long* ctr) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_pss_image(int n_dye, const int32_t* dye_ids, const edm_photo_params* pp,
+                            const edm_image_params* ip, const double* pcx, const double* pcy,
+                            const Real* x, const Real* y, uint8_t* fl, const int8_t* dye,
+                            int8_t* ctmc, const double* k_fret, const int32_t* acc,
+                            const int32_t* slot_of, uint32_t* flim, Rng rng, uint32_t step,
+                            unsigned long long* ctr) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
     int emitted = 0, detected = 0;
-    if (t < n) {
-        const int it = id[t];
+    if (q < n_dye) {
+        const int it = dye_ids[q];
+        const int t = slot_of[it];
         uint8_t f = fl[t];
         int8_t c = ctmc[t];
         const U4 r0 = rng.draw(step, (uint32_t)it, 0u, kTagPhotonPois);
         const double up = u01d(r0.x, r0.y), ut = u01d(r0.z, r0.w);
         PssOut o{0, 0, 0.0, 0};
-        const int aid = acc[t];
-        if (pss_eval(pp, f, c, dye[t], k_fret[t], aid, up, ut, o)) {
+        const int aid = acc[q];
+        if (pss_eval(pp, f, c, dye[t], k_fret[q], aid, up, ut, o)) {
             emitted = o.nd + o.nf;
             const double qe = ip->qe;
             const double sirf = ip->sigma_irf;
@@ -526,6 +534,10 @@ int ph_alloc(edm_membrane* m) {
         if (dalloc(&m->k_fret, n) || dalloc(&m->acc, n) || dalloc(&m->ph_nd, n) ||
             dalloc(&m->ph_nf, n) || dalloc(&m->ph_cnt, n) || dalloc(&m->ph_off, n))
             return EDM_EHIP;
+        // only dye carriers are ever written: everything else stays zero
+        EDM_HIP(hipMemset(m->ph_nd, 0, (size_t)n * 4));
+        EDM_HIP(hipMemset(m->ph_nf, 0, (size_t)n * 4));
+        EDM_HIP(hipMemset(m->ph_cnt, 0, (size_t)n * 8));
         hipLaunchKernelGGL(k_fret_zero, dim3(nb(n)), dim3(kB), 0, m->stream, n, m->k_fret, m->acc);
     }
     return 0;
@@ -594,24 +606,28 @@ static int scan_total(edm_membrane* m, const int64_t* in, const int64_t* out, in
 }
 
 int ph_fret(edm_membrane* m) {
-    const int n = m->n;
+    const int n = m->n, nd = m->n_dye;
     hipStream_t st = m->stream;
-    hipLaunchKernelGGL(k_fret_zero, dim3(nb(n)), dim3(kB), 0, st, n, m->k_fret, m->acc);
+    if (nd == 0) return 0;
+    hipLaunchKernelGGL(k_fret_zero, dim3(nb(nd)), dim3(kB), 0, st, nd, m->k_fret, m->acc);
     const edm_photo_params* dpp = (const edm_photo_params*)m->d_pp;
     for (int j = 0; j < m->pp.n_fret_maps; ++j) {
         if (m->pp.fm_proximity[j] || m->pp.fm_skip[j]) continue;
-        hipLaunchKernelGGL(k_fret_bond, dim3(nb(n)), dim3(kB), 0, st, n, dpp, j, m->a.fl,
-                           m->a.dye, m->a.bond, m->a.slot_of, m->k_fret, m->acc);
+        hipLaunchKernelGGL(k_fret_bond, dim3(nb(nd)), dim3(kB), 0, st, nd,
+                           (const int32_t*)m->dye_ids, dpp, j, m->a.fl, m->a.dye, m->a.bond,
+                           m->a.slot_of, m->k_fret, m->acc);
     }
     for (int j = 0; j < m->pp.n_fret_maps; ++j) {
         if (!m->pp.fm_proximity[j] || m->pp.fm_skip[j]) continue;
         if (m->fp64)
-            hipLaunchKernelGGL(k_fret_prox<double>, dim3(nb(n)), dim3(kB), 0, st, n, dpp, j,
+            hipLaunchKernelGGL(k_fret_prox<double>, dim3(nb(nd)), dim3(kB), 0, st, nd,
+                               (const int32_t*)m->dye_ids, (const int32_t*)m->a.slot_of, n, dpp, j,
                                (const double*)m->a.x, (const double*)m->a.y, m->a.sp, m->a.fl,
                                m->a.dye, m->p.Lx, m->p.Ly, m->p.periodic_x != 0,
                                m->p.periodic_y != 0, m->k_fret, m->acc);
         else
-            hipLaunchKernelGGL(k_fret_prox<float>, dim3(nb(n)), dim3(kB), 0, st, n, dpp, j,
+            hipLaunchKernelGGL(k_fret_prox<float>, dim3(nb(nd)), dim3(kB), 0, st, nd,
+                               (const int32_t*)m->dye_ids, (const int32_t*)m->a.slot_of, n, dpp, j,
                                (const float*)m->a.x, (const float*)m->a.y, m->a.sp, m->a.fl,
                                m->a.dye, (float)m->p.Lx, (float)m->p.Ly, m->p.periodic_x != 0,
                                m->p.periodic_y != 0, m->k_fret, m->acc);
@@ -634,8 +650,9 @@ int ph_hook(edm_membrane* m, const double* u_pois, const double* u_trans, const 
         EDM_HIP(hipMemcpyAsync(m->d_draws + 2 * (int64_t)n, u_arr, (size_t)n_arr * 8,
                                hipMemcpyHostToDevice, st));
     const edm_photo_params* dpp = (const edm_photo_params*)m->d_pp;
-    hipLaunchKernelGGL(k_pss_count, dim3(nb(n)), dim3(kB), 0, st, n, dpp,
-                       (const int32_t*)m->a.id, m->a.fl, (const int8_t*)m->a.dye, m->a.ctmc,
+    hipLaunchKernelGGL(k_pss_count, dim3(nb(m->n_dye)), dim3(kB), 0, st, m->n_dye,
+                       (const int32_t*)m->dye_ids, (const int32_t*)m->a.slot_of, dpp, m->a.fl,
+                       (const int8_t*)m->a.dye, m->a.ctmc,
                        (const double*)m->k_fret, (const int32_t*)m->acc,
                        (const double*)m->d_draws, (const double*)(m->d_draws + n), m->ph_nd,
                        m->ph_nf, m->ph_cnt);
@@ -644,8 +661,8 @@ int ph_hook(edm_membrane* m, const double* u_pois, const double* u_trans, const 
     if (int rc = scan_total(m, m->ph_cnt, m->ph_off, n, &total)) return rc;
     const int64_t np = std::min<int64_t>(total, m->pp.max_photons);
     if (int rc = ensure_photon_cap(m, np)) return rc;
-    hipLaunchKernelGGL(k_pss_emit, dim3(nb(n)), dim3(kB), 0, st, n, dpp, (const int32_t*)m->a.id,
-                       (const int8_t*)m->a.dye, (const int32_t*)m->a.slot_of,
+    hipLaunchKernelGGL(k_pss_emit, dim3(nb(m->n_dye)), dim3(kB), 0, st, m->n_dye,
+                       (const int32_t*)m->dye_ids, dpp, (const int8_t*)m->a.dye, (const int32_t*)m->a.slot_of,
                        (const double*)m->k_fret, (const int32_t*)m->acc,
                        (const int32_t*)m->ph_nd, (const int32_t*)m->ph_nf,
                        (const int64_t*)m->ph_off, (const double*)(m->d_draws + 2 * (int64_t)n),
@@ -673,16 +690,18 @@ int ph_fused_image(edm_membrane* m, int64_t* n_emitted, int64_t* n_detected) {
     const uint32_t step = m->step;
     const edm_image_params* d_ip = dev_ip(m);
     if (m->fp64)
-        hipLaunchKernelGGL(k_pss_image<double>, dim3(nb(n)), dim3(kB), 0, st, n, dpp, d_ip,
+        hipLaunchKernelGGL(k_pss_image<double>, dim3(nb(m->n_dye)), dim3(kB), 0, st, m->n_dye,
+                           (const int32_t*)m->dye_ids, dpp, d_ip,
                            (const double*)m->pcx, (const double*)m->pcy, (const double*)m->a.x,
-                           (const double*)m->a.y, (const int32_t*)m->a.id, m->a.fl,
+                           (const double*)m->a.y, m->a.fl,
                            (const int8_t*)m->a.dye, m->a.ctmc, (const double*)m->k_fret,
                            (const int32_t*)m->acc, (const int32_t*)m->a.slot_of, m->flim, rng,
                            step, m->d_ctr);
     else
-        hipLaunchKernelGGL(k_pss_image<float>, dim3(nb(n)), dim3(kB), 0, st, n, dpp, d_ip,
+        hipLaunchKernelGGL(k_pss_image<float>, dim3(nb(m->n_dye)), dim3(kB), 0, st, m->n_dye,
+                           (const int32_t*)m->dye_ids, dpp, d_ip,
                            (const double*)m->pcx, (const double*)m->pcy, (const float*)m->a.x,
-                           (const float*)m->a.y, (const int32_t*)m->a.id, m->a.fl,
+                           (const float*)m->a.y, m->a.fl,
                            (const int8_t*)m->a.dye, m->a.ctmc, (const double*)m->k_fret,
                            (const int32_t*)m->acc, (const int32_t*)m->a.slot_of, m->flim, rng,
                            step, m->d_ctr);

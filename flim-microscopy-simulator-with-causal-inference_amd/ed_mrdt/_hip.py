@@ -48,7 +48,7 @@ class Params(ctypes.Structure):
                 ("periodic_x", c_int32), ("periodic_y", c_int32), ("ncx", c_int32),
                 ("ncy", c_int32), ("use_wca", c_int32), ("use_harmonic", c_int32),
                 ("dfield", c_int32), ("n_regions", c_int32), ("n_grids", c_int32),
-                ("n_reactions", c_int32), ("pair_cap", c_int32),
+                ("n_reactions", c_int32), ("pair_cap", c_int32), ("exact_force_order", c_int32),
                 ("Lx", c_double), ("Ly", c_double), ("dt", c_double), ("cutoff", c_double),
                 ("csx", c_double), ("csy", c_double), ("max_disp", c_double),
                 ("smooth_w", c_double), ("sin_amp", c_double), ("sin_kx", c_double),
@@ -103,6 +103,7 @@ _SIGS = {
     "edm_step_finish": [c_void_p, c_void_p, c_int64],
     "edm_run": [c_void_p, c_int32],
     "edm_counters": [c_void_p, c_void_p],
+    "edm_photon_counters": [c_void_p, c_void_p],
     "edm_pairs": [c_void_p, c_void_p, c_int64, ctypes.POINTER(c_int64)],
     "edm_density": [c_void_p, c_int32, c_int32, c_void_p],
     "edm_photons": [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, ctypes.POINTER(c_int64)],

@@ -221,10 +221,14 @@ class DynamicsEngine:
       rng        "numpy" (default, reference streams) or "philox"
       pair_cap   True: keep the reference's min(30 N, 500000) pair-list cap
                  (default); False: no cap (every pair within the cutoff)
+      exact_force_order
+                 True (default): with numpy draws in FP64, sum pair forces in
+                 the reference's pair order (bit-identical positions); False:
+                 the LDS-tiled force kernel (positions agree to ~1e-15)
     """
 
     def __init__(self, config: SimulationConfig, state: ParticleState, precision: str = None,
-                 rng: str = None, pair_cap: bool = True):
+                 rng: str = None, pair_cap: bool = True, exact_force_order: bool = True):
         self.config = config
         self.state = state
         precision = precision or os.environ.get("EDM_PRECISION", "fp64")
@@ -274,6 +278,7 @@ class DynamicsEngine:
         p.n_grids = len(self.plan.grids)
         p.n_reactions = len(self.plan.reactions)
         p.pair_cap = min(N * 30, 500000) if pair_cap else 0
+        p.exact_force_order = 1 if exact_force_order else 0
         p.Lx, p.Ly, p.dt = self.Lx, self.Ly, self.dt
         p.cutoff, p.csx, p.csy = c["cutoff"], c["csx"], c["csy"]
         p.max_disp = c["max_disp"]

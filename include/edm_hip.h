@@ -86,6 +86,9 @@ typedef struct edm_params {
     int32_t n_grids;
     int32_t n_reactions;
     int32_t pair_cap;       /* reference pair-list cap min(30N, 5e5); 0 = none */
+    int32_t exact_force_order; /* 1: with hook draws in FP64, sum pair forces in
+                               the reference's pair order (bit-exact runs);
+                               0: always the LDS-tiled kernel              */
     double Lx, Ly, dt;
     double cutoff, csx, csy;
     double max_disp;        /* displacement cap (dynamics.py:1826-1864)      */
@@ -168,6 +171,8 @@ int edm_run(edm_membrane *m, int32_t n_steps);
 /* Event counters: bindings, dissociations, catalytic, recruited, detached,
  * converted (dynamics.py:2068-2071, 2147-2148), cumulative. */
 int edm_counters(edm_membrane *m, int64_t out[6]);
+/* Cumulative photons emitted and detected by edm_photons_image. */
+int edm_photon_counters(edm_membrane *m, int64_t out[2]);
 /* Reference pair list in reference order (CellList.get_neighbor_pairs,
  * dynamics.py:220-278) for the cell list of the last edm_step_begin. */
 int edm_pairs(edm_membrane *m, int32_t *pairs, int64_t cap, int64_t *n_pairs);

@@ -20,9 +20,10 @@ from parity_util import compare_states, host_arrays, oracle_arrays
 pytestmark = pytest.mark.gpu
 
 
-def _pair(cfg, pair_cap=True):
+def _pair(cfg, pair_cap=True, exact=True):
     st = ParticleState(cfg)
-    eng = DynamicsEngine(cfg, st, precision="fp64", rng="numpy", pair_cap=pair_cap)
+    eng = DynamicsEngine(cfg, st, precision="fp64", rng="numpy", pair_cap=pair_cap,
+                         exact_force_order=exact)
     ost = rm.OracleState(cfg)
     odyn = rm.OracleDynamics(cfg, ost, pair_cap=pair_cap)
     return st, eng, ost, odyn
@@ -49,10 +50,26 @@ def test_pair_list_reference_order(oracle_lib):
     eng.step_finish(eng._numpy_draws(n))
 
 
-@pytest.mark.parametrize("n_steps", [40])
-def test_plain_membrane_dynamics_fp64(oracle_lib, n_steps):
+@pytest.mark.parametrize("exact", [True, False])
+def test_dense_harmonic_tiles_fp64(oracle_lib, exact):
+    """1M-workload density (100 / um^2, harmonic) on a 14 x 14 um box: 84
+    cell rows, so the LDS-tiled kernel spans two strips per column."""
+    cfg = workloads.large_membrane(side=14.0, n_alive=19_600, pixels=64)
+    st, eng, ost, odyn = _pair(cfg, exact=exact)
+    for k in range(20):
+        eng.step()
+        odyn.step()
+    worst = compare_states(host_arrays(st), oracle_arrays(ost), label=f"exact={exact}: ")
+    oc = odyn.counters()
+    assert (eng.n_catalytic, eng.n_detached, eng.n_converted) == (
+        oc["n_catalytic"], oc["n_detached"], oc["n_converted"])
+    print(f"dense harmonic exact={exact}: worst relative position error {worst:.3e}")
+
+
+@pytest.mark.parametrize("n_steps,exact", [(40, True), (40, False)])
+def test_plain_membrane_dynamics_fp64(oracle_lib, n_steps, exact):
     cfg = workloads.plain_membrane()
-    st, eng, ost, odyn = _pair(cfg)
+    st, eng, ost, odyn = _pair(cfg, exact=exact)
     worst = 0.0
     for k in range(n_steps):
         eng.step()
@@ -69,10 +86,13 @@ def test_plain_membrane_dynamics_fp64(oracle_lib, n_steps):
 
 
 @pytest.mark.parametrize("variant", [dict(), dict(intramolecular=False),
-                                     dict(reflective=True, harmonic=True)])
+                                     dict(reflective=True, harmonic=True),
+                                     dict(intramolecular=False, exact=False)])
 def test_binding_fret_full_path_fp64(oracle_lib, variant):
+    variant = dict(variant)
+    exact = variant.pop("exact", True)
     cfg = workloads.binding_fret(**variant)
-    st, eng, ost, odyn = _pair(cfg)
+    st, eng, ost, odyn = _pair(cfg, exact=exact)
     duty = min(1.0, cfg.scan.dwell_time / cfg.scan.frame_interval)
     ph = PhotonSimulator(cfg, st, duty)
     mic = VirtualMicroscope(cfg, st)
