@@ -1,0 +1,223 @@
+// Tiled catalysis pass (dynamics.py:826-1090), included by edm_dynamics.hip
+// after the reaction helpers.  Each workgroup stages its strip's
+// neighbourhood (pre-move and post-move positions, species, flags), lists
+// the strip's own alive enzymes in LDS and tests every enzyme against the
+// substrates of its 3x3 cells, one wavefront per enzyme.  A pair qualifies
+// as in the reference: in the (capped) pair list built on pre-move
+// positions (d^2 < cutoff^2), within the effective contact radius after the
+// move, and accepted by its uniform against P_cat x Hill(rho at the substrate) [x adaptation].  The
+// substrate flips with a byte CAS on the global species array, so however
+// many of its pairs succeed it converts once, which is the outcome of the
+// reference's sequential loop; counts and density updates follow the CAS.
+
+// Up to kCatFuse catalytic reactions that run back to back with the same
+// substrate and product and without density feedback or adaptation act as
+// one: a substrate converts if any pair of any of them accepts (each pair
+// keeps its own uniform per reaction), which is the outcome of running them
+// one after another, so they share one launch.  Member j has enzyme enz[j].
+static constexpr int kCatFuse = 4;
+
+template <typename Real>
+struct CatArgs {
+    int n_enz;
+    int enz[kCatFuse];
+    Real contact2[kCatFuse], prob[kCatFuse];
+    uint32_t rxi[kCatFuse];
+    int64_t doff[kCatFuse];  // offset of the member's pair uniforms in hook draws
+    int sub, prod;
+    FbDev fb;                // member 0 only (a fused run has none)
+    const double* w;         // adaptation field, member 0 only
+    int wnx, wny;
+    Real Kw, nw;
+    Draws dr;
+    Rng rng;
+    uint32_t step;
+    bool deferred;
+};
+
+template <typename T>
+__device__ __forceinline__ T cat_pick(const T (&a)[kCatFuse], int j) {
+    T v = a[0];
+#pragma unroll
+    for (int k = 1; k < kCatFuse; ++k)
+        if (j == k) v = a[k];
+    return v;
+}
+
+template <typename Real>
+__device__ __forceinline__ int cat_try(const Cells<Real>& C, const Post<Real>& P,
+                                       const CatArgs<Real>& A, int j, const Grids<Real>& G,
+                                       int n_grids, int t, int it, int u, Real xnt, Real ynt,
+                                       Real xnu, Real ynu) {
+    Real dx, dy;
+    if (min_image(xnt, ynt, xnu, ynu, C.Lx, C.Ly, C.px, C.py, dx, dy) > cat_pick(A.contact2, j))
+        return 0;
+    const int iu = C.id[u];
+    const bool tlow = it < iu;
+    const int sa = tlow ? t : u, sb = tlow ? u : t;
+    if (!pair_in_list(C, sa, sb)) return 0;
+    const Real prob = cat_pick(A.prob, j);
+    Real pe = prob;
+    if (A.fb.sensor >= 0) {
+        const Real hf = fb_factor(G, A.fb, xnu, ynu);
+        if (A.w) {
+            int gx = trunc_cell(xnu, (Real)A.fb.gsx, A.fb.gnx);
+            int gy = trunc_cell(ynu, (Real)A.fb.gsy, A.fb.gny);
+            gx = clamp_cell(gx, A.wnx);
+            gy = clamp_cell(gy, A.wny);
+            Real af = (Real)1;
+            if (A.Kw > (Real)0) {
+                const Real wr = pow((Real)A.w[gx * A.wny + gy] / A.Kw, A.nw);
+                af = (Real)1 / ((Real)1 + wr);
+            }
+            pe = prob * hf * af;
+        } else {
+            pe = prob * hf;
+        }
+    }
+    double uu;
+    if (A.dr.base) {
+        uu = A.dr.base[cat_pick(A.doff, j) + pair_index(C, sa, sb)];
+    } else {
+        const uint32_t lo = (uint32_t)min(it, iu), hi = (uint32_t)max(it, iu);
+        uu = (double)Uni<Real>::first(
+            A.rng.draw(A.step, lo, hi, (kTagCat << 16) | cat_pick(A.rxi, j)));
+    }
+    if (uu < (double)pe && cas_byte(P.sp, u, (int8_t)A.sub, (int8_t)A.prod)) {
+        count_update(G, n_grids, A.sub, xnu, ynu, -1, A.deferred);
+        count_update(G, n_grids, A.prod, xnu, ynu, +1, A.deferred);
+        return 1;
+    }
+    return 0;
+}
+
+template <typename Real>
+__global__ __launch_bounds__(256) void k_cat_tile(Cells<Real> C, Post<Real> P, CatArgs<Real> A,
+                                                  unsigned long long* ctr, Grids<Real> G,
+                                                  int n_grids, int ntile_y, int ntiles) {
+    constexpr int kCap = 1024;
+    __shared__ Real s_x[kCap], s_y[kCap], s_xn[kCap], s_yn[kCap];
+    __shared__ int8_t s_sp[kCap];
+    __shared__ uint8_t s_fl[kCap];
+    __shared__ int s_list[kCap];
+    __shared__ int s_off[kStripCells + 1];
+    __shared__ int s_gst[kStripCells];
+    __shared__ int s_wsum[4];
+    __shared__ int s_nlist;
+
+    const int tile = xcd_tile(ntiles);
+    if (tile >= ntiles) return;  // uniform across the block
+    const StripGeo S = strip_geo(C, tile, ntile_y);
+    if (threadIdx.x == 0) s_nlist = 0;
+    const int total = strip_cells(C, S, s_gst, s_off, s_wsum);
+    const bool staged = total <= kCap;
+    if (staged) {
+        // all loads first, then the LDS stores: one memory round trip
+        constexpr int kPer = kCap / 256;
+        Real vx[kPer], vy[kPer], vxn[kPer], vyn[kPer];
+        int8_t vs[kPer];
+        uint8_t vf[kPer];
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int o = (int)threadIdx.x + 256 * j;
+            vx[j] = vy[j] = vxn[j] = vyn[j] = (Real)0;
+            vs[j] = 0;
+            vf[j] = 0;
+            if (o < total) {
+                const int g = strip_src(S, s_off, s_gst, o);
+                vx[j] = C.x[g];
+                vy[j] = C.y[g];
+                vxn[j] = P.xn[g];
+                vyn[j] = P.yn[g];
+                vs[j] = P.sp[g];
+                vf[j] = C.fl[g];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int o = (int)threadIdx.x + 256 * j;
+            if (o < total) {
+                s_x[o] = vx[j];
+                s_y[o] = vy[j];
+                s_xn[o] = vxn[j];
+                s_yn[o] = vyn[j];
+                s_sp[o] = vs[j];
+                s_fl[o] = vf[j];
+            }
+        }
+    }
+    __syncthreads();
+    const int own_begin = s_off[S.nrow + 1], own_end = s_off[S.nrow + S.rows + 1];
+    const int gbase = s_gst[S.nrow + 1];
+    // the strip's own alive enzymes as (member << 28 | own index) entries
+    for (int i = own_begin + (int)threadIdx.x; i < own_end; i += blockDim.x) {
+        const int t = gbase + (i - own_begin);
+        const int s = staged ? s_sp[i] : P.sp[t];
+        const uint8_t f = staged ? s_fl[i] : C.fl[t];
+        if (!(f & EDM_FLAG_ALIVE)) continue;
+#pragma unroll
+        for (int j = 0; j < kCatFuse; ++j)
+            if (j < A.n_enz && s == A.enz[j]) {
+                const int e = atomicAdd(&s_nlist, 1);
+                if (e < kCap) s_list[e] = (j << 28) | (i - own_begin);
+            }
+    }
+    __syncthreads();
+    // more enzymes than list slots (only in pathologically dense strips):
+    // walk the own range directly, once per member
+    const bool use_list = s_nlist <= kCap;
+    const int nown = own_end - own_begin;
+    const int nl = use_list ? s_nlist : nown * A.n_enz;
+    // one wavefront per enzyme, its lanes over the enzyme's candidates, so
+    // that the per-candidate work (Hill lookups, draws, CAS) runs side by
+    // side instead of in one thread's chain
+    const int wave = (int)(threadIdx.x >> 6), lane = lane_id();
+    const int nwave = (int)(blockDim.x >> 6);
+    int hits = 0;
+    for (int e = wave; e < nl; e += nwave) {
+        const int ent = use_list ? s_list[e] : ((e / nown) << 28) | (e % nown);
+        const int j = ent >> 28;
+        const int i = own_begin + (ent & 0x0fffffff);
+        const int t = gbase + (i - own_begin);
+        if (!use_list) {
+            const int s = staged ? s_sp[i] : P.sp[t];
+            const uint8_t f = staged ? s_fl[i] : C.fl[t];
+            if (s != cat_pick(A.enz, j) || !(f & EDM_FLAG_ALIVE)) continue;  // wave-uniform
+        }
+        const int it = C.id[t];
+        if (staged) {
+            const Real xt = s_x[i], yt = s_y[i], xnt = s_xn[i], ynt = s_yn[i];
+            const int r = strip_row(C, S, yt);
+            const int b0 = s_off[r - 1], l0 = s_off[r + 2] - b0;
+            const int b1 = s_off[S.nrow + r - 1], l1 = s_off[S.nrow + r + 2] - b1;
+            const int b2 = s_off[2 * S.nrow + r - 1], l2 = s_off[2 * S.nrow + r + 2] - b2;
+            for (int c = lane; c < l0 + l1 + l2; c += 64) {
+                const int u = c < l0 ? b0 + c : (c < l0 + l1 ? b1 + (c - l0) : b2 + (c - l0 - l1));
+                if (u == i || s_sp[u] != A.sub || !(s_fl[u] & EDM_FLAG_ALIVE)) continue;
+                Real dx, dy;
+                if (min_image(xt, yt, s_x[u], s_y[u], C.Lx, C.Ly, C.px, C.py, dx, dy) >= C.cut2)
+                    continue;
+                hits += cat_try(C, P, A, j, G, n_grids, t, it, strip_src(S, s_off, s_gst, u),
+                                xnt, ynt, s_xn[u], s_yn[u]);
+            }
+        } else if (lane < 9) {
+            const Real xt = C.x[t], yt = C.y[t], xnt = P.xn[t], ynt = P.yn[t];
+            int cx, cy;
+            cell_of(C, xt, yt, cx, cy);
+            const int c = nb_cell(C, cx, cy, lane);
+            if (c >= 0)
+                for (int u = (int)C.start[c]; u < (int)C.start[c + 1]; ++u) {
+                    if (u == t || P.sp[u] != A.sub || !(C.fl[u] & EDM_FLAG_ALIVE)) continue;
+                    Real dx, dy;
+                    if (min_image(xt, yt, C.x[u], C.y[u], C.Lx, C.Ly, C.px, C.py, dx, dy) >=
+                        C.cut2)
+                        continue;
+                    hits += cat_try(C, P, A, j, G, n_grids, t, it, u, xnt, ynt, P.xn[u],
+                                    P.yn[u]);
+                }
+        }
+    }
+    unsigned long long v = hits;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+    if (lane_id() == 0 && v) atomicAdd(&ctr[2], v);
+}

@@ -137,22 +137,48 @@ __device__ __forceinline__ int grid_cell(const Grids<Real>& G, int g, Real x, Re
     return gx * ny + gy;
 }
 
+// A feedback rule with its grid resolved on the host: the kept-count offset
+// of (grid, sensor), the grid's shape, its cell size and 1 / cell area, so
+// that a lookup is one count load.
 struct FbDev {
     int sensor, grid, inhibition;
     double K, n, basal;
+    int cbase, gnx, gny;
+    double gsx, gsy, ginv;
 };
 
-__host__ __device__ inline FbDev to_fb(const edm_feedback& f) {
-    return FbDev{f.sensor, f.grid, f.inhibition, f.K_half, f.n_hill, f.basal};
+static FbDev make_fb(const edm_membrane* m, const edm_feedback& f) {
+    FbDev d{f.sensor, f.grid, f.inhibition, f.K_half, f.n_hill, f.basal, 0, 1, 1, 1.0, 1.0, 0.0};
+    if (f.sensor >= 0 && f.grid >= 0) {
+        const edm_grid& g = m->grids[f.grid];
+        d.cbase = m->count_off[(size_t)f.grid * m->p.n_species + f.sensor];
+        d.gnx = g.nx;
+        d.gny = g.ny;
+        d.gsx = g.sx;
+        d.gsy = g.sy;
+        const double area = g.sx * g.sy;
+        d.ginv = area > 0.0 ? 1.0 / area : 0.0;
+    }
+    return d;
+}
+
+// Grid cell of (x, y) on the rule's grid (int() truncation, clamped).
+template <typename Real>
+__device__ __forceinline__ int fb_cell(const FbDev& f, Real x, Real y) {
+    return trunc_cell(x, (Real)f.gsx, f.gnx) * f.gny + trunc_cell(y, (Real)f.gsy, f.gny);
+}
+
+// Hill factor of a grid cell holding k sensor particles.
+template <typename Real>
+__device__ __forceinline__ Real fb_hill(const FbDev& f, int k) {
+    const Real rho = density_of<Real>(k, (Real)f.ginv);
+    return hill_factor<Real>(rho, (Real)f.K, (Real)f.n, f.inhibition != 0, (Real)f.basal);
 }
 
 template <typename Real>
 __device__ __forceinline__ Real fb_factor(const Grids<Real>& G, const FbDev& f, Real x, Real y) {
     if (f.sensor < 0) return (Real)1;
-    const int c = grid_cell(G, f.grid, x, y);
-    const int k = G.counts[G.off[f.grid * G.ns + f.sensor] + c];
-    const Real rho = density_of<Real>(k, G.inv[f.grid]);
-    return hill_factor<Real>(rho, (Real)f.K, (Real)f.n, f.inhibition != 0, (Real)f.basal);
+    return fb_hill<Real>(f, G.counts[f.cbase + fb_cell(f, x, y)]);
 }
 
 // Update the kept density counts after species s of a particle at (x, y)
@@ -178,12 +204,32 @@ __global__ void k_bin(int n, const Real* __restrict__ x, const Real* __restrict_
                       int n_counts) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < n_counts) counts[s] = 0;  // density counts are rebuilt each step
-    if (s >= n) return;
-    const int cx = trunc_cell(x[s], csx, ncx);
-    const int cy = trunc_cell(y[s], csy, ncy);
-    const uint32_t c = (uint32_t)(cx * ncy + cy);
-    key[s] = c;
-    rank[s] = atomicAdd(&cnt[c], 1u);
+    const bool live = s < n;          // live lanes are a prefix of the wavefront
+    uint32_t c = 0xffffffffu;
+    if (live) {
+        const int cx = trunc_cell(x[s], csx, ncx);
+        const int cy = trunc_cell(y[s], csy, ncy);
+        c = (uint32_t)(cx * ncy + cy);
+        key[s] = c;
+    }
+    // Slots are in the previous step's cell order, so equal keys mostly sit
+    // in adjacent lanes: one atomic per run of equal keys, ranks inside the
+    // run by lane.
+    const int lane = lane_id();
+    const uint32_t prev = __shfl_up(c, 1);
+    const bool head = live && (lane == 0 || prev != c);
+    const uint64_t heads = __ballot(head);
+    const int nlive = __popcll(__ballot(live));
+    const uint64_t upto = heads & ((2ull << lane) - 1ull);
+    const int rs = upto ? 63 - __clzll(upto) : lane;
+    uint32_t base = 0;
+    if (head) {
+        const uint64_t after = heads & ~((2ull << lane) - 1ull);
+        const int nxt = after ? min(__ffsll((unsigned long long)after) - 1, nlive) : nlive;
+        base = atomicAdd(&cnt[c], (uint32_t)(nxt - lane));
+    }
+    base = __shfl(base, rs);
+    if (live) rank[s] = base + (uint32_t)(lane - rs);
 }
 
 __global__ void k_scatter(int n, const uint32_t* __restrict__ key, const uint32_t* __restrict__ rank,
@@ -194,14 +240,44 @@ __global__ void k_scatter(int n, const uint32_t* __restrict__ key, const uint32_
 }
 
 // One thread per cell: order the cell's particles by particle index (the
-// atomic ranks of k_bin are arbitrary) and clear the cell counter.
+// atomic ranks of k_bin are arbitrary) and clear the cell counter.  Cells of
+// up to 8 particles are sorted in registers (all loads issued together);
+// larger ones by insertion in place.
 __global__ void k_sort_cells(int ncell, const uint32_t* __restrict__ start, uint32_t* perm,
                              uint32_t* cnt, const int32_t* __restrict__ id) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
     cnt[c] = 0;
     const int lo = (int)start[c], hi = (int)start[c + 1];
-    if (hi - lo < 2) return;
+    const int m = hi - lo;
+    if (m < 2) return;
+    constexpr int K = 8;
+    if (m <= K) {
+        uint32_t v[K];
+        int32_t k[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) v[j] = j < m ? perm[lo + j] : 0u;
+#pragma unroll
+        for (int j = 0; j < K; ++j) k[j] = j < m ? id[v[j]] : 0x7fffffff;
+#pragma unroll
+        for (int r = 0; r < K; ++r) {
+#pragma unroll
+            for (int j = r & 1; j + 1 < K; j += 2) {
+                if (k[j] > k[j + 1]) {
+                    const int32_t tk = k[j];
+                    k[j] = k[j + 1];
+                    k[j + 1] = tk;
+                    const uint32_t tv = v[j];
+                    v[j] = v[j + 1];
+                    v[j + 1] = tv;
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+            if (j < m) perm[lo + j] = v[j];
+        return;
+    }
     for (int i = lo + 1; i < hi; ++i) {
         const uint32_t v = perm[i];
         const int32_t vid = id[v];
@@ -305,6 +381,7 @@ struct ForceParams {
     const Real *rcx, *rcy, *rr, *rf;
     Real smooth_w, sin_amp, sin_kx, sin_ky;
     Real dt, max_disp;
+    Real rint2;  // no species pair interacts at d^2 >= rint2
 };
 
 // Pair force over r on the particle the separation (dx, dy) points to;
@@ -555,158 +632,7 @@ __global__ __launch_bounds__(256) void k_force_bd(int n, Cells<Real> C, ForcePar
     add_density(G, n_grids, take_any, si, nxp, nyp);
 }
 
-// LDS-tiled forces + move for throughput runs (any pair summation order).
-// A workgroup owns a strip of kTileRows cells of one cell column; the three
-// neighbouring columns' rows (strip +/- 1 row) are staged in LDS with
-// coalesced loads, so every particle's 3x3 neighbourhood is three
-// contiguous LDS ranges.  Blocks that the dispatcher sends to the same XCD
-// (blockIdx % 8) take adjacent strips, so neighbouring strips share that
-// XCD's L2.  A strip whose neighbourhood exceeds the LDS capacity falls
-// back to global-memory neighbour walks.
-static constexpr int kTileRows = 64;
-
-template <typename Real>
-__global__ __launch_bounds__(256) void k_force_tile(Cells<Real> C, ForceParams<Real> F,
-                                                    Real* __restrict__ xn, Real* __restrict__ yn,
-                                                    const double* __restrict__ noise_hook,
-                                                    Rng rng, uint32_t step, Grids<Real> G,
-                                                    int n_grids, uint32_t keep_mask, int ntile_y,
-                                                    int ntiles) {
-    constexpr int kCap = sizeof(Real) == 8 ? 2048 : 4096;
-    constexpr int kCells = 3 * (kTileRows + 2);
-    __shared__ Real s_x[kCap], s_y[kCap];
-    __shared__ int8_t s_sp[kCap];
-    __shared__ uint8_t s_fl[kCap];
-    __shared__ int s_off[kCells + 1];
-    __shared__ int s_gst[kCells];
-
-    const int per = (ntiles + 7) / 8;
-    const int tile = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
-    if (tile >= ntiles) return;  // uniform across the block
-    const int cx = tile / ntile_y;
-    const int cy0 = (tile % ntile_y) * kTileRows;
-    const int rows = min(kTileRows, C.ncy - cy0);
-    const int nrow = rows + 2;
-    const int ncells = 3 * nrow;
-    for (int k = threadIdx.x; k < ncells; k += blockDim.x) {
-        const int col = k / nrow, r = k % nrow;
-        int gx = cx + col - 1, gy = cy0 + r - 1;
-        bool ok = true;
-        if (C.px) gx = (gx + C.ncx) % C.ncx;
-        else if (gx < 0 || gx >= C.ncx) ok = false;
-        if (C.py) gy = (gy + C.ncy) % C.ncy;
-        else if (gy < 0 || gy >= C.ncy) ok = false;
-        int gs = 0, cnt = 0;
-        if (ok) {
-            const int c = gx * C.ncy + gy;
-            gs = (int)C.start[c];
-            cnt = (int)C.start[c + 1] - gs;
-        }
-        s_gst[k] = gs;
-        s_off[k] = cnt;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int k = 0; k < ncells; ++k) {
-            const int c = s_off[k];
-            s_off[k] = acc;
-            acc += c;
-        }
-        s_off[ncells] = acc;
-    }
-    __syncthreads();
-    const int total = s_off[ncells];
-    const bool staged = total <= kCap;
-    if (staged) {
-        for (int i = threadIdx.x; i < total; i += blockDim.x) {
-            int lo = 0, hi = ncells - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (s_off[mid] <= i) lo = mid;
-                else hi = mid - 1;
-            }
-            const int g = s_gst[lo] + (i - s_off[lo]);
-            s_x[i] = C.x[g];
-            s_y[i] = C.y[g];
-            s_sp[i] = C.sp[g];
-            s_fl[i] = C.fl[g];
-        }
-    }
-    __syncthreads();
-    // own particles: the middle column's rows 1..rows (contiguous in memory)
-    const int own_begin = s_off[nrow + 1], own_end = s_off[nrow + rows + 1];
-    const int gbase = s_gst[nrow + 1];
-    for (int i0 = own_begin; i0 < own_end; i0 += blockDim.x) {
-        const int i = i0 + (int)threadIdx.x;
-        bool take_any = false;
-        Real nxp = 0, nyp = 0;
-        int si = 0;
-        if (i < own_end) {
-            const int t = gbase + (i - own_begin);
-            const Real xt = C.x[t], yt = C.y[t];
-            si = C.sp[t];
-            nxp = xt;
-            nyp = yt;
-            if (C.fl[t] & EDM_FLAG_ALIVE) {
-                Real Fx = 0, Fy = 0;
-                int cxx, cyy;
-                cell_of(C, xt, yt, cxx, cyy);
-                if (staged) {
-                    const int r = cyy - cy0 + 1;  // staged row of this particle
-                    for (int col = 0; col < 3; ++col) {
-                        const int k0 = col * nrow + r - 1;
-                        const int ub = s_off[k0 + 3];
-                        for (int u = s_off[k0]; u < ub; ++u) {
-                            if (u == i) continue;
-                            Real dx, dy;
-                            const Real dsq = min_image(xt, yt, s_x[u], s_y[u], C.Lx, C.Ly, C.px,
-                                                       C.py, dx, dy);
-                            if (dsq >= C.cut2 || !(s_fl[u] & EDM_FLAG_ALIVE)) continue;
-                            const int su = s_sp[u];
-                            Real f;
-                            if (F.wca && pair_force(F, 0, si, su, dsq, f)) {
-                                Fx += f * dx;
-                                Fy += f * dy;
-                            }
-                            if (F.harm && pair_force(F, 1, si, su, dsq, f)) {
-                                Fx += f * dx;
-                                Fy += f * dy;
-                            }
-                        }
-                    }
-                } else {
-                    for (int k = 0; k < 9; ++k) {
-                        const int c = nb_cell(C, cxx, cyy, k);
-                        if (c < 0) continue;
-                        for (int u = (int)C.start[c]; u < (int)C.start[c + 1]; ++u) {
-                            if (u == t) continue;
-                            Real dx, dy;
-                            const Real dsq = min_image(xt, yt, C.x[u], C.y[u], C.Lx, C.Ly, C.px,
-                                                       C.py, dx, dy);
-                            if (dsq >= C.cut2 || !(C.fl[u] & EDM_FLAG_ALIVE)) continue;
-                            const int su = C.sp[u];
-                            Real f;
-                            if (F.wca && pair_force(F, 0, si, su, dsq, f)) {
-                                Fx += f * dx;
-                                Fy += f * dy;
-                            }
-                            if (F.harm && pair_force(F, 1, si, su, dsq, f)) {
-                                Fx += f * dx;
-                                Fy += f * dy;
-                            }
-                        }
-                    }
-                }
-                move_particle(C, F, si, C.id[t], xt, yt, Fx, Fy, noise_hook, rng, step, nxp, nyp);
-                take_any = ((keep_mask >> si) & 1u) != 0;
-            }
-            xn[t] = nxp;
-            yn[t] = nyp;
-        }
-        add_density(G, n_grids, take_any, si, nxp, nyp);
-    }
-}
+#include "edm_tiles.h"
 
 // ----------------------------------------------------------------------
 // Reaction kernels
@@ -858,6 +784,8 @@ __global__ __launch_bounds__(1024) void k_bind_match(Post<Real> P, const int32_t
         __threadfence();
         if (!__syncthreads_or(live)) break;
     }
+    __syncthreads();
+    if (threadIdx.x == 0) *const_cast<unsigned int*>(ne) = 0u;  // ready for the next pass
 }
 
 // dynamics.py:610-654 apply_dissociation
@@ -900,160 +828,12 @@ __global__ void k_adapt(Grids<Real> G, int grid, int species, double* w, double 
     w[c] += rate * (rho - w[c]);
 }
 
-// Compact the slots of alive particles of up to 8 species into per-species
-// lists (wavefront ballot + one atomic per wavefront and list).
-struct ListSpec {
-    int n_lists;
-    int species[8];
-};
-
-__global__ void k_compact(int n, const int8_t* __restrict__ sp, const uint8_t* __restrict__ fl,
-                          ListSpec spec, int32_t* __restrict__ lists, unsigned int* counts) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool in = t < n;
-    const int s = in ? sp[t] : -1;
-    const bool alive = in && (fl[t] & EDM_FLAG_ALIVE);
-    const int lane = lane_id();
-    for (int l = 0; l < spec.n_lists; ++l) {
-        const bool take = alive && s == spec.species[l];
-        const uint64_t m = __ballot(take);
-        if (!m) continue;
-        const int leader = __ffsll((unsigned long long)m) - 1;
-        unsigned int base = 0;
-        if (lane == leader) base = atomicAdd(&counts[l], (unsigned int)__popcll(m));
-        base = __shfl(base, leader);
-        if (take) {
-            const uint64_t below = m & ((1ull << lane) - 1ull);
-            lists[(int64_t)l * n + base + __popcll(below)] = t;
-        }
-    }
-}
-
 __global__ void k_zero_u32(int n, unsigned int* p) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
 }
 
-// dynamics.py:826-1090 catalysis: enzyme-centred sweep of the pair list
-// over the compacted enzyme slots; the substrate flips once however many of
-// its pairs succeed (byte CAS), which is the outcome of the reference's
-// sequential loop.
-template <typename Real>
-__global__ void k_cat(const int32_t* __restrict__ list, const unsigned int* list_n, Cells<Real> C,
-                      Post<Real> P, int enz, int sub, int prod, Real contact2,
-                      Real prob, FbDev fb, const double* w, int wnx, int wny, Real Kw, Real nw,
-                      Draws dr, Rng rng, uint32_t step, uint32_t rxi, unsigned long long* ctr,
-                      Grids<Real> G, int n_grids, bool deferred) {
-    int hits = 0;
-    const int nl = (int)*list_n;
-    for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < nl; li += gridDim.x * blockDim.x) {
-    const int t = list[li];
-    if (C.sp[t] == enz && (C.fl[t] & EDM_FLAG_ALIVE)) {
-        const Real xt = C.x[t], yt = C.y[t];
-        const int it = C.id[t];
-        int cx, cy;
-        cell_of(C, xt, yt, cx, cy);
-        for (int k = 0; k < 9; ++k) {
-            const int c = nb_cell(C, cx, cy, k);
-            if (c < 0) continue;
-            for (int u = (int)C.start[c]; u < (int)C.start[c + 1]; ++u) {
-                if (u == t) continue;
-                if (P.sp[u] != sub || !(C.fl[u] & EDM_FLAG_ALIVE)) continue;
-                Real dx, dy;
-                if (min_image(xt, yt, C.x[u], C.y[u], C.Lx, C.Ly, C.px, C.py, dx, dy) >= C.cut2)
-                    continue;
-                const int iu = C.id[u];
-                const bool tlow = it < iu;
-                const int sa = tlow ? t : u, sb = tlow ? u : t;
-                if (!pair_in_list(C, sa, sb)) continue;
-                if (min_image(P.xn[t], P.yn[t], P.xn[u], P.yn[u], C.Lx, C.Ly, C.px, C.py, dx,
-                              dy) > contact2)
-                    continue;
-                Real pe = prob;
-                const Real sx = P.xn[u], sy = P.yn[u];
-                if (fb.sensor >= 0) {
-                    const Real hf = fb_factor(G, fb, sx, sy);
-                    if (w) {
-                        const int gnx = G.dims[2 * fb.grid], gny = G.dims[2 * fb.grid + 1];
-                        int gx = trunc_cell(sx, G.s[2 * fb.grid], gnx);
-                        int gy = trunc_cell(sy, G.s[2 * fb.grid + 1], gny);
-                        gx = clamp_cell(gx, wnx);
-                        gy = clamp_cell(gy, wny);
-                        Real af = (Real)1;
-                        if (Kw > (Real)0) {
-                            const Real wr = pow((Real)w[gx * wny + gy] / Kw, nw);
-                            af = (Real)1 / ((Real)1 + wr);
-                        }
-                        pe = prob * hf * af;
-                    } else {
-                        pe = prob * hf;
-                    }
-                }
-                double uu;
-                if (dr.base) {
-                    uu = dr.base[dr.off + pair_index(C, sa, sb)];
-                } else {
-                    const uint32_t lo = (uint32_t)min(it, iu), hi = (uint32_t)max(it, iu);
-                    uu = (double)Uni<Real>::first(rng.draw(step, lo, hi, (kTagCat << 16) | rxi));
-                }
-                if (uu < (double)pe) {
-                    if (cas_byte(P.sp, u, (int8_t)sub, (int8_t)prod)) {
-                        ++hits;
-                        count_update(G, n_grids, sub, sx, sy, -1, deferred);
-                        count_update(G, n_grids, prod, sx, sy, +1, deferred);
-                    }
-                }
-            }
-        }
-    }
-    }
-    unsigned long long v = hits;
-    // wavefront reduction then one atomic
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
-    if (lane_id() == 0 && v) atomicAdd(&ctr[2], v);
-}
-
-// Recruitment candidates (dynamics.py:1230-1458): dormant particles of the
-// species whose acceptance test passes at their candidate position.
-template <typename Real>
-__global__ void k_recruit_flag(int n, const int32_t* __restrict__ id, Post<Real> P, int target,
-                               Real prob, Real Lx, Real Ly, FbDev f1, FbDev f2, Draws dr, Rng rng,
-                               uint32_t step, uint32_t rxi, Grids<Real> G, int32_t* cid,
-                               int32_t* cslot, double* cxo, double* cyo, unsigned int* nc) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    if ((P.fl[t] & EDM_FLAG_ALIVE) || P.sp[t] != target) return;
-    const int it = id[t];
-    Real ux, uy;
-    double ua;
-    if (dr.base) {
-        ua = dr.base[dr.off + it];
-        ux = (Real)dr.base[dr.off + dr.n + 2 * (int64_t)it];
-        uy = (Real)dr.base[dr.off + dr.n + 2 * (int64_t)it + 1];
-    } else {
-        const U4 r = rng.draw(step, (uint32_t)it, 0u, (kTagRecruitAcc << 16) | rxi);
-        ua = (double)Uni<Real>::first(r);
-        const U4 q = rng.draw(step, (uint32_t)it, 1u, (kTagRecruitPos << 16) | rxi);
-        ux = Uni<Real>::first(q);
-        uy = Uni<Real>::second(q);
-    }
-    const Real cx = ux * Lx, cy = uy * Ly;
-    Real pe = prob;
-    if (f1.sensor >= 0 && f2.sensor >= 0) {
-        const Real h1 = fb_factor(G, f1, cx, cy);
-        const Real h2 = fb_factor(G, f2, cx, cy);
-        pe = prob * h1 * h2;
-    } else if (f1.sensor >= 0) {
-        pe = prob * fb_factor(G, f1, cx, cy);
-    }
-    if (ua < (double)pe) {
-        const unsigned int k = atomicAdd(nc, 1u);
-        cid[k] = it;
-        cslot[k] = t;
-        cxo[k] = (double)cx;
-        cyo[k] = (double)cy;
-    }
-}
+#include "edm_cat_tile.h"
 
 // Accept the max_recruits candidates with the lowest particle index (the
 // reference stops its index-order scan after max_recruits), one workgroup.
@@ -1095,54 +875,11 @@ __global__ __launch_bounds__(1024) void k_recruit_apply(Post<Real> P, int target
         ++acc;
     }
     if (acc) atomicAdd(&ctr[3], (unsigned long long)acc);
+    __syncthreads();
+    if (threadIdx.x == 0) *const_cast<unsigned int*>(nc) = 0u;  // ready for the next pass
 }
 
-// dynamics.py:1462-1555 detachment (deferred when the species is its own sensor)
-template <typename Real>
-__global__ void k_detach(int n, const int32_t* __restrict__ id, Post<Real> P, int target, Real prob,
-                         FbDev fb, Draws dr, Rng rng, uint32_t step, uint32_t rxi,
-                         unsigned long long* ctr, Grids<Real> G, int n_grids, bool deferred) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    bool hit = false;
-    if (t < n && (P.fl[t] & EDM_FLAG_ALIVE) && P.sp[t] == target) {
-        const int it = id[t];
-        const Real x = P.xn[t], y = P.yn[t];
-        const Real pe = fb.sensor >= 0 ? prob * fb_factor(G, fb, x, y) : prob;
-        const double u = dr.base ? dr.base[dr.off + it]
-                                 : (double)Uni<Real>::first(
-                                       rng.draw(step, (uint32_t)it, 0u, (kTagDetach << 16) | rxi));
-        if (u < (double)pe) {
-            P.fl[t] = P.fl[t] & ~EDM_FLAG_ALIVE;
-            count_update(G, n_grids, target, x, y, -1, deferred);
-            hit = true;
-        }
-    }
-    wave_count(&ctr[4], hit);
-}
-
-// dynamics.py:1566-1657 conversion
-template <typename Real>
-__global__ void k_convert(int n, const int32_t* __restrict__ id, Post<Real> P, int from, int to,
-                          Real prob, FbDev fb, Draws dr, Rng rng, uint32_t step, uint32_t rxi,
-                          unsigned long long* ctr, Grids<Real> G, int n_grids, bool deferred) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    bool hit = false;
-    if (t < n && (P.fl[t] & EDM_FLAG_ALIVE) && P.sp[t] == from) {
-        const int it = id[t];
-        const Real x = P.xn[t], y = P.yn[t];
-        const Real pe = fb.sensor >= 0 ? prob * fb_factor(G, fb, x, y) : prob;
-        const double u = dr.base ? dr.base[dr.off + it]
-                                 : (double)Uni<Real>::first(
-                                       rng.draw(step, (uint32_t)it, 0u, (kTagConvert << 16) | rxi));
-        if (u < (double)pe) {
-            P.sp[t] = (int8_t)to;
-            count_update(G, n_grids, from, x, y, -1, deferred);
-            count_update(G, n_grids, to, x, y, +1, deferred);
-            hit = true;
-        }
-    }
-    wave_count(&ctr[5], hit);
-}
+#include "edm_passes.h"
 
 __global__ void k_fold(int n, int32_t* counts, int32_t* delta) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1313,47 +1050,93 @@ static int create_t(edm_membrane* m) {
         if (dalloc(&m->w_adapt, (size_t)g.nx * g.ny)) return EDM_EHIP;
         EDM_HIP(hipMemset(m->w_adapt, 0, (size_t)g.nx * g.ny * sizeof(double)));
     }
-    // enzyme lists: one compaction per run of consecutive catalytic passes
-    // when no enzyme of the run is a substrate or product of the run (the
-    // enzyme sets then cannot change inside it); else one per pass.
-    const size_t nr = m->rx.size();
-    m->cat_list.assign(nr, -1);
-    m->compact_n.assign(nr, 0);
-    m->compact_species.assign(nr, std::vector<int>());
-    int max_lists = 1;
-    for (size_t i = 0; i < nr;) {
-        if (m->rx[i].kind != EDM_RX_CAT) {
-            ++i;
-            continue;
+    // Fused per-particle passes.  Counts exist only for kept (sensor)
+    // species; a pass "writes" the kept species whose counts it changes and
+    // "reads" its feedback sensors.  A run is cut before a pass that reads
+    // what an earlier pass of the run writes, after a pass that reads what
+    // it writes itself (deferred, folded after the run) and after every
+    // recruitment pass (its cap needs all candidates).  Passes with zero
+    // probability change nothing and are skipped.
+    {
+        const size_t nr = m->rx.size();
+        m->group_start.assign(nr, -1);
+        m->pass_deferred.assign(nr, 0);
+        m->pass_groups.clear();
+        auto kept = [&](int s) { return s >= 0 && ((m->sensor_mask_all >> s) & 1u); };
+        std::vector<int> cur;
+        uint32_t wmask = 0;
+        auto close = [&]() {
+            if (!cur.empty()) {
+                m->group_start[cur.front()] = (int)m->pass_groups.size();
+                m->pass_groups.push_back(cur);
+            }
+            cur.clear();
+            wmask = 0;
+        };
+        for (size_t i = 0; i < nr; ++i) {
+            const edm_reaction& r = m->rx[i];
+            const bool per_particle =
+                r.kind == EDM_RX_DETACH || r.kind == EDM_RX_CONVERT || r.kind == EDM_RX_RECRUIT;
+            if (!per_particle) {
+                close();
+                continue;
+            }
+            if (r.prob <= 0.0) continue;
+            uint32_t rmask = 0, own = 0;
+            if (r.fb1.sensor >= 0) rmask |= 1u << r.fb1.sensor;
+            if (r.fb2.sensor >= 0) rmask |= 1u << r.fb2.sensor;
+            if (r.kind == EDM_RX_DETACH && kept(r.a)) own |= 1u << r.a;
+            if (r.kind == EDM_RX_CONVERT) {
+                if (kept(r.a)) own |= 1u << r.a;
+                if (kept(r.b)) own |= 1u << r.b;
+            }
+            if (!cur.empty() && ((rmask & wmask) || cur.size() >= (size_t)kMaxPasses)) close();
+            m->pass_deferred[i] = (rmask & own) ? 1 : 0;
+            cur.push_back((int)i);
+            wmask |= own;
+            if (m->pass_deferred[i] || r.kind == EDM_RX_RECRUIT) close();
         }
-        size_t j = i;
-        while (j < nr && m->rx[j].kind == EDM_RX_CAT) ++j;
-        std::vector<int> enz;
-        bool shared = true;
-        for (size_t k = i; k < j; ++k) {
-            if (std::find(enz.begin(), enz.end(), m->rx[k].a) == enz.end()) enz.push_back(m->rx[k].a);
-        }
-        for (size_t k = i; k < j; ++k)
-            for (int e : enz)
-                if (m->rx[k].b == e || m->rx[k].c == e) shared = false;
-        if (enz.size() > 8) shared = false;
-        if (shared) {
-            m->compact_species[i] = enz;
-            m->compact_n[i] = (int)enz.size();
-            for (size_t k = i; k < j; ++k)
-                m->cat_list[k] = (int)(std::find(enz.begin(), enz.end(), m->rx[k].a) - enz.begin());
-            max_lists = std::max(max_lists, (int)enz.size());
-        } else {
-            for (size_t k = i; k < j; ++k) {
-                m->compact_species[k] = {m->rx[k].a};
-                m->compact_n[k] = 1;
-                m->cat_list[k] = 0;
+        close();
+    }
+    // Species the passes act on, closed under every reaction that turns one
+    // species into another: a particle outside the set is never touched by a
+    // pass and never enters the set, so the passes can run over a list of
+    // particle indices fixed at upload.
+    {
+        uint32_t S = 0;
+        for (auto& g : m->pass_groups)
+            for (int i : g) {
+                const edm_reaction& r = m->rx[i];
+                S |= 1u << r.a;
+                if (r.kind == EDM_RX_CONVERT) S |= 1u << r.b;
+            }
+        for (bool grow = S != 0; grow;) {
+            grow = false;
+            for (auto& r : m->rx) {
+                uint32_t T = 0;
+                if (r.kind == EDM_RX_CAT) T = (1u << r.b) | (1u << r.c);
+                else if (r.kind == EDM_RX_BIND) T = (1u << r.a) | (1u << r.b) | (1u << r.c);
+                else if (r.kind == EDM_RX_CONVERT) T = (1u << r.a) | (1u << r.b);
+                if ((T & S) && (T & ~S)) {
+                    S |= T;
+                    grow = true;
+                }
             }
         }
-        i = j;
+        m->pass_species = S;
     }
-    m->n_lists_alloc = max_lists;
-    if (dalloc(&m->lists, (size_t)n * max_lists) || dalloc(&m->list_cnt, 8)) return EDM_EHIP;
+    // largest squared range of any pair potential (the tiled force kernel
+    // skips candidates beyond it before looking up the species tables)
+    {
+        double r2 = 0.0;
+        for (int k = 0; k < ns * ns; ++k) {
+            const double sg = p.sigma[k];
+            if (!(sg > 0.0)) continue;
+            if (p.use_wca && p.epsilon[k] > 0.0) r2 = std::max(r2, sg * sg * 1.2599210498948732);
+            if (p.use_harmonic && p.k_harm[k] > 0.0) r2 = std::max(r2, sg * sg);
+        }
+        m->rint2 = r2 * (1.0 + 1e-6);
+    }
     // reaction scratch
     m->edge_cap = (size_t)std::max(n, 1024);
     if (dalloc(&m->edge_key, m->edge_cap) || dalloc(&m->edge_a, m->edge_cap) ||
@@ -1462,6 +1245,7 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
     F.sin_ky = (Real)p.sin_ky;
     F.dt = (Real)p.dt;
     F.max_disp = (Real)p.max_disp;
+    F.rint2 = (Real)m->rint2;
     const double* noise = hook ? draws : nullptr;
     if (m->time_force) EDM_HIP(hipEventRecord(m->kev[0], st));
     if (hook && m->fp64 && p.exact_force_order)
@@ -1483,14 +1267,30 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
 
     int64_t off = 2 * (int64_t)n;
     const int64_t npairs = m->n_pairs;
+    // offsets of every reaction's arrays in the step's hook draws
+    std::vector<int64_t> draw_off(m->rx.size());
+    {
+        int64_t o = 2 * (int64_t)n;
+        for (size_t ri = 0; ri < m->rx.size(); ++ri) {
+            draw_off[ri] = o;
+            switch (m->rx[ri].kind) {
+                case EDM_RX_BIND: o += npairs + n; break;
+                case EDM_RX_CAT: o += npairs; break;
+                case EDM_RX_RECRUIT: o += 3 * (int64_t)n; break;
+                case EDM_RX_DETACH:
+                case EDM_RX_CONVERT: o += n; break;
+                default: break;
+            }
+        }
+    }
+    int cat_skip = 0;
     for (size_t ri = 0; ri < m->rx.size(); ++ri) {
         const edm_reaction& r = m->rx[ri];
         Draws dr{hook ? draws : nullptr, off, npairs, n};
         const uint32_t rxi = (uint32_t)ri;
-        const FbDev f1 = to_fb(r.fb1), f2 = to_fb(r.fb2);
+        const FbDev f1 = make_fb(m, r.fb1), f2 = make_fb(m, r.fb2);
         switch (r.kind) {
             case EDM_RX_BIND: {
-                hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, 1, m->n_edges);
                 const Real c2 = (Real)r.contact * (Real)r.contact;
                 hipLaunchKernelGGL(k_bind_edges<Real>, dim3(nblk(n)), dim3(kBlock), 0, st, n, C, P,
                                    r.a, r.b, c2, (Real)r.prob, dr, rng, step, rxi, m->edge_key,
@@ -1515,8 +1315,21 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
                 break;
             }
             case EDM_RX_CAT: {
+                if (cat_skip > 0) {  // member of the fused run launched before
+                    --cat_skip;
+                    off += npairs;
+                    break;
+                }
                 const bool defer = f1.sensor >= 0 && (f1.sensor == r.b || f1.sensor == r.c);
-                const Real c2 = (Real)r.contact * (Real)r.contact;
+                const bool plain = f1.sensor < 0 && !r.use_adapt;
+                int nf = 1;
+                while (plain && nf < kCatFuse && ri + nf < m->rx.size()) {
+                    const edm_reaction& r2 = m->rx[ri + nf];
+                    if (r2.kind != EDM_RX_CAT || r2.b != r.b || r2.c != r.c ||
+                        r2.fb1.sensor >= 0 || r2.use_adapt)
+                        break;
+                    ++nf;
+                }
                 int wnx = 1, wny = 1;
                 const double* w = nullptr;
                 if (r.use_adapt && p.adapt_grid >= 0) {
@@ -1524,68 +1337,88 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
                     wnx = m->grids[p.adapt_grid].nx;
                     wny = m->grids[p.adapt_grid].ny;
                 }
-                if (m->compact_n[ri] > 0) {
-                    ListSpec spec{};
-                    spec.n_lists = m->compact_n[ri];
-                    for (int l = 0; l < spec.n_lists; ++l) spec.species[l] = m->compact_species[ri][l];
-                    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, 8, m->list_cnt);
-                    hipLaunchKernelGGL(k_compact, dim3(nblk(n)), dim3(kBlock), 0, st, n,
-                                       (const int8_t*)m->a.sp, (const uint8_t*)m->a.fl, spec,
-                                       m->lists, m->list_cnt);
+                CatArgs<Real> A{};
+                A.n_enz = nf;
+                for (int j = 0; j < nf; ++j) {
+                    const edm_reaction& rj = m->rx[ri + j];
+                    A.enz[j] = rj.a;
+                    A.contact2[j] = (Real)rj.contact * (Real)rj.contact;
+                    A.prob[j] = (Real)rj.prob;
+                    A.rxi[j] = (uint32_t)(ri + j);
+                    A.doff[j] = draw_off[ri + j];
                 }
-                const int li = m->cat_list[ri];
-                hipLaunchKernelGGL(k_cat<Real>, dim3(std::min(nblk(n), 2048)), dim3(kBlock), 0, st,
-                                   (const int32_t*)(m->lists + (int64_t)li * n),
-                                   (const unsigned int*)(m->list_cnt + li), C, P, r.a,
-                                   r.b, r.c, c2, (Real)r.prob, f1, w, wnx, wny, (Real)p.adapt_K,
-                                   (Real)p.adapt_n, dr, rng, step, rxi, m->d_ctr, G, ng, defer);
+                A.sub = r.b;
+                A.prod = r.c;
+                A.fb = f1;
+                A.w = w;
+                A.wnx = wnx;
+                A.wny = wny;
+                A.Kw = (Real)p.adapt_K;
+                A.nw = (Real)p.adapt_n;
+                A.dr = dr;
+                A.rng = rng;
+                A.step = step;
+                A.deferred = defer;
+                const int ntile_y = (p.ncy + kTileRows - 1) / kTileRows;
+                const int ntiles = p.ncx * ntile_y;
+                hipLaunchKernelGGL((k_cat_tile<Real>), dim3(8 * ((ntiles + 7) / 8)), dim3(256), 0,
+                                   st, C, P, A, m->d_ctr, G, ng, ntile_y, ntiles);
                 if (defer && m->n_count)
                     hipLaunchKernelGGL(k_fold, dim3(nblk(m->n_count)), dim3(kBlock), 0, st,
                                        m->n_count, m->counts, m->delta);
+                cat_skip = nf - 1;
                 off += npairs;
                 break;
             }
-            case EDM_RX_RECRUIT: {
-                if (r.prob > 0.0) {
-                    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, 1, m->n_cand);
-                    hipLaunchKernelGGL(k_recruit_flag<Real>, dim3(nblk(n)), dim3(kBlock), 0, st, n,
-                                       (const int32_t*)m->a.id, P, r.a, (Real)r.prob, (Real)p.Lx,
-                                       (Real)p.Ly, f1, f2, dr, rng, step, rxi, G, m->cand_id,
-                                       m->cand_slot, m->cand_x, m->cand_y, m->n_cand);
-                    hipLaunchKernelGGL(k_recruit_apply<Real>, dim3(1), dim3(1024), 0, st, P, r.a,
-                                       r.max_recruits, n, (const int32_t*)m->cand_id,
-                                       (const int32_t*)m->cand_slot, (const double*)m->cand_x,
-                                       (const double*)m->cand_y, (const unsigned int*)m->n_cand,
-                                       m->d_ctr, G, ng);
-                }
-                off += 3 * (int64_t)n;
-                break;
-            }
-            case EDM_RX_DETACH: {
-                if (r.prob > 0.0) {
-                    const bool defer = f1.sensor >= 0 && f1.sensor == r.a;
-                    hipLaunchKernelGGL(k_detach<Real>, dim3(nblk(n)), dim3(kBlock), 0, st, n,
-                                       (const int32_t*)m->a.id, P, r.a, (Real)r.prob, f1, dr, rng,
-                                       step, rxi, m->d_ctr, G, ng, defer);
-                    if (defer && m->n_count)
-                        hipLaunchKernelGGL(k_fold, dim3(nblk(m->n_count)), dim3(kBlock), 0, st,
-                                           m->n_count, m->counts, m->delta);
-                }
-                off += n;
-                break;
-            }
+            case EDM_RX_RECRUIT:
+            case EDM_RX_DETACH:
             case EDM_RX_CONVERT: {
-                if (r.prob > 0.0) {
-                    const bool defer =
-                        f1.sensor >= 0 && (f1.sensor == r.a || f1.sensor == r.b);
-                    hipLaunchKernelGGL(k_convert<Real>, dim3(nblk(n)), dim3(kBlock), 0, st, n,
-                                       (const int32_t*)m->a.id, P, r.a, r.b, (Real)r.prob, f1, dr,
-                                       rng, step, rxi, m->d_ctr, G, ng, defer);
-                    if (defer && m->n_count)
+                if (m->group_start[ri] >= 0) {
+                    const std::vector<int>& g = m->pass_groups[m->group_start[ri]];
+                    PassGroupDev grp{};
+                    grp.n = (int)g.size();
+                    bool defer_any = false;
+                    for (int k = 0; k < grp.n; ++k) {
+                        const edm_reaction& q = m->rx[g[k]];
+                        PassDev& d = grp.p[k];
+                        d.kind = q.kind;
+                        d.a = q.a;
+                        d.b = q.b;
+                        d.rxi = g[k];
+                        d.prob = q.prob;
+                        d.f1 = make_fb(m, q.fb1);
+                        d.f2 = make_fb(m, q.fb2);
+                        d.draw_off = draw_off[g[k]];
+                        d.deferred = m->pass_deferred[g[k]];
+                        defer_any = defer_any || d.deferred;
+                        uint32_t reach = 1u << q.a;
+                        for (int j = k - 1; j >= 0; --j) {
+                            const edm_reaction& qj = m->rx[g[j]];
+                            if (qj.kind == EDM_RX_CONVERT && ((reach >> qj.b) & 1u))
+                                reach |= 1u << qj.a;
+                        }
+                        d.reach = reach;
+                        grp.species |= reach;
+                    }
+                    const int nt = m->pass_ids ? m->n_pass : n;
+                    hipLaunchKernelGGL(k_passes<Real>, dim3(nblk(std::max(nt, 1))), dim3(kBlock), 0,
+                                       st, nt, n, (const int32_t*)m->pass_ids,
+                                       (const int32_t*)m->a.id, P, grp, hook ? draws : nullptr,
+                                       rng, step, (Real)p.Lx, (Real)p.Ly, m->d_ctr, G, ng,
+                                       m->cand_id, m->cand_slot, m->cand_x, m->cand_y, m->n_cand);
+                    const edm_reaction& last = m->rx[g.back()];
+                    if (last.kind == EDM_RX_RECRUIT)
+                        hipLaunchKernelGGL(k_recruit_apply<Real>, dim3(1), dim3(1024), 0, st, P,
+                                           last.a, last.max_recruits, n,
+                                           (const int32_t*)m->cand_id,
+                                           (const int32_t*)m->cand_slot, (const double*)m->cand_x,
+                                           (const double*)m->cand_y,
+                                           (const unsigned int*)m->n_cand, m->d_ctr, G, ng);
+                    if (defer_any && m->n_count)
                         hipLaunchKernelGGL(k_fold, dim3(nblk(m->n_count)), dim3(kBlock), 0, st,
                                            m->n_count, m->counts, m->delta);
                 }
-                off += n;
+                off += (r.kind == EDM_RX_RECRUIT) ? 3 * (int64_t)n : (int64_t)n;
                 break;
             }
             default: break;

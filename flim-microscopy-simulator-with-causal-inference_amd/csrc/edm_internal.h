@@ -65,6 +65,7 @@ struct edm_membrane {
 
     // parameters (Real arrays)
     void *sig = nullptr, *eps = nullptr, *kh = nullptr, *Dbase = nullptr;
+    double rint2 = 0.0;  // largest squared interaction range of any species pair
     void *rcx = nullptr, *rcy = nullptr, *rr = nullptr, *rf = nullptr;
 
     // density counts: counts[count_off[g * ns + s] + cell]; -1 = not kept
@@ -91,13 +92,14 @@ struct edm_membrane {
     double* d_draws = nullptr;
     size_t draws_cap = 0;
 
-    // compacted per-species slot lists (enzymes of the catalytic passes)
-    int32_t* lists = nullptr;
-    unsigned int* list_cnt = nullptr;
-    int n_lists_alloc = 0;
-    std::vector<int> cat_list;          // per reaction: list index used by CAT
-    std::vector<int> compact_n;         // per reaction: lists built before it (0 = none)
-    std::vector<std::vector<int>> compact_species;
+    // fused per-particle passes: runs of detach / convert / recruit-flag
+    std::vector<std::vector<int>> pass_groups;
+    uint32_t pass_species = 0;    // species closed under every transformation that
+                                  // touches a pass species (bit per species)
+    int32_t* pass_ids = nullptr;  // particle indices of those species (fixed at upload)
+    int n_pass = 0;               // 0 with pass_ids null: passes run over every slot
+    std::vector<int> group_start;    // per reaction: group starting here, else -1
+    std::vector<int> pass_deferred;  // per reaction: reads a count it writes
 
     // reaction scratch
     unsigned long long* edge_key = nullptr;

@@ -1,0 +1,150 @@
+// Fused per-particle reaction passes (included by edm_dynamics.hip).
+//
+// Detachment (dynamics.py:1462-1555), conversion (dynamics.py:1566-1657)
+// and recruitment-candidate flagging (dynamics.py:1230-1458) each look at
+// one particle at a time.  A run of such passes can share one kernel in
+// which every thread applies the passes to its own particle in the
+// reference's order, provided no pass reads a density count that an
+// earlier pass of the run changes (the run is split there on the host).
+// A recruitment pass ends its run: the index-order cap needs all
+// candidates before any is accepted (k_recruit_apply).  With a particle
+// list (the indices of the species the passes can touch) a thread takes one
+// listed particle instead of one slot.
+
+struct PassDev {
+    int kind, a, b, rxi;
+    double prob;
+    FbDev f1, f2;
+    int64_t draw_off;
+    int deferred;
+    uint32_t reach;  // species a particle may hold before the run and still
+                     // reach this pass as species a (through earlier conversions)
+};
+
+static constexpr int kMaxPasses = 8;
+
+struct PassGroupDev {
+    int n;
+    uint32_t species;  // union of the passes' reach masks
+    PassDev p[kMaxPasses];
+};
+
+template <typename Real>
+__global__ void k_passes(int nt, int n, const int32_t* __restrict__ list,
+                         const int32_t* __restrict__ id, Post<Real> P, PassGroupDev grp,
+                         const double* __restrict__ draws, Rng rng, uint32_t step, Real Lx,
+                         Real Ly, unsigned long long* ctr, Grids<Real> G, int n_grids,
+                         int32_t* cid, int32_t* cslot, double* cxo, double* cyo,
+                         unsigned int* nc) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    int det_hits = 0, conv_hits = 0;
+    int t = -1;
+    int s0 = 0;
+    if (q < nt) {
+        t = list ? P.slot_of[list[q]] : q;
+        s0 = P.sp[t];
+    }
+    if (t >= 0 && ((grp.species >> s0) & 1u)) {
+        uint8_t f = P.fl[t];
+        const uint8_t f0 = f;
+        int8_t s = (int8_t)s0;
+        const int it = id[t];
+        const Real x = P.xn[t], y = P.yn[t];
+        // Phase 1: every pass this particle can reach gets its uniform, its
+        // position (the particle's, or a recruitment's drawn site) and its
+        // density counts up front, so the loads of all passes are in flight
+        // together.  No pass of a run reads a count that another pass of the
+        // run changes, so reading them early sees the same values.
+        double u[kMaxPasses];
+        Real px[kMaxPasses], py[kMaxPasses];
+        int k1[kMaxPasses], k2[kMaxPasses];
+#pragma unroll
+        for (int k = 0; k < kMaxPasses; ++k) {
+            u[k] = 1.0;
+            px[k] = x;
+            py[k] = y;
+            k1[k] = 0;
+            k2[k] = 0;
+            if (k >= grp.n) continue;
+            const PassDev& d = grp.p[k];
+            if (!((d.reach >> s0) & 1u)) continue;
+            if (d.kind == EDM_RX_RECRUIT) {
+                Real ux, uy;
+                if (draws) {
+                    u[k] = draws[d.draw_off + it];
+                    ux = (Real)draws[d.draw_off + n + 2 * (int64_t)it];
+                    uy = (Real)draws[d.draw_off + n + 2 * (int64_t)it + 1];
+                } else {
+                    const U4 r = rng.draw(step, (uint32_t)it, 0u, (kTagRecruitAcc << 16) | d.rxi);
+                    u[k] = (double)Uni<Real>::first(r);
+                    const U4 w = rng.draw(step, (uint32_t)it, 1u, (kTagRecruitPos << 16) | d.rxi);
+                    ux = Uni<Real>::first(w);
+                    uy = Uni<Real>::second(w);
+                }
+                px[k] = ux * Lx;
+                py[k] = uy * Ly;
+                if (d.f2.sensor >= 0) k2[k] = G.counts[d.f2.cbase + fb_cell(d.f2, px[k], py[k])];
+            } else {
+                const uint32_t tag = d.kind == EDM_RX_DETACH ? kTagDetach : kTagConvert;
+                u[k] = draws ? draws[d.draw_off + it]
+                             : (double)Uni<Real>::first(
+                                   rng.draw(step, (uint32_t)it, 0u, (tag << 16) | d.rxi));
+            }
+            if (d.f1.sensor >= 0) k1[k] = G.counts[d.f1.cbase + fb_cell(d.f1, px[k], py[k])];
+        }
+        // Phase 2: the passes in the reference's order.
+#pragma unroll
+        for (int k = 0; k < kMaxPasses; ++k) {
+            if (k >= grp.n) continue;
+            const PassDev& d = grp.p[k];
+            if (d.kind == EDM_RX_DETACH) {
+                if (!(f & EDM_FLAG_ALIVE) || s != d.a) continue;
+                const Real pe =
+                    d.f1.sensor >= 0 ? (Real)d.prob * fb_hill<Real>(d.f1, k1[k]) : (Real)d.prob;
+                if (u[k] < (double)pe) {
+                    f = f & ~EDM_FLAG_ALIVE;
+                    count_update(G, n_grids, d.a, x, y, -1, d.deferred != 0);
+                    ++det_hits;
+                }
+            } else if (d.kind == EDM_RX_CONVERT) {
+                if (!(f & EDM_FLAG_ALIVE) || s != d.a) continue;
+                const Real pe =
+                    d.f1.sensor >= 0 ? (Real)d.prob * fb_hill<Real>(d.f1, k1[k]) : (Real)d.prob;
+                if (u[k] < (double)pe) {
+                    s = (int8_t)d.b;
+                    count_update(G, n_grids, d.a, x, y, -1, d.deferred != 0);
+                    count_update(G, n_grids, d.b, x, y, +1, d.deferred != 0);
+                    ++conv_hits;
+                }
+            } else if (d.kind == EDM_RX_RECRUIT) {
+                if ((f & EDM_FLAG_ALIVE) || s != d.a) continue;
+                Real pe = (Real)d.prob;
+                if (d.f1.sensor >= 0 && d.f2.sensor >= 0) {
+                    const Real h1 = fb_hill<Real>(d.f1, k1[k]);
+                    const Real h2 = fb_hill<Real>(d.f2, k2[k]);
+                    pe = (Real)d.prob * h1 * h2;
+                } else if (d.f1.sensor >= 0) {
+                    pe = (Real)d.prob * fb_hill<Real>(d.f1, k1[k]);
+                }
+                if (u[k] < (double)pe) {
+                    const unsigned int c = atomicAdd(nc, 1u);
+                    cid[c] = it;
+                    cslot[c] = t;
+                    cxo[c] = (double)px[k];
+                    cyo[c] = (double)py[k];
+                }
+            }
+        }
+        if (f != f0) P.fl[t] = f;
+        if (s != s0) P.sp[t] = s;
+    }
+    unsigned long long dh = det_hits, ch = conv_hits;
+    for (int o = 32; o > 0; o >>= 1) {
+        dh += __shfl_down(dh, o);
+        ch += __shfl_down(ch, o);
+    }
+    if (lane_id() == 0) {
+        if (dh) atomicAdd(&ctr[4], dh);
+        if (ch) atomicAdd(&ctr[5], ch);
+    }
+}

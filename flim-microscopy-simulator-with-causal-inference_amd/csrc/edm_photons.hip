@@ -304,13 +304,104 @@ __device__ __forceinline__ void tcspc_add(const edm_image_params* ip, uint32_t* 
 // ----------------------------------------------------------------------
 // Fused per-emitter photon path with Philox draws
 // ----------------------------------------------------------------------
+// Bond-mode FRET rate of one donor, all maps in config order (later maps
+// overwrite earlier ones, as update_fret_rates does; photophysics.py:529-588).
+__device__ __forceinline__ void fret_inline(const edm_photo_params* pp, int t, const uint8_t* fl,
+                                            const int8_t* dye, const int32_t* bond,
+                                            const int32_t* slot_of, double& kf, int& acc) {
+    kf = 0.0;
+    acc = -1;
+    if (!(fl[t] & EDM_FLAG_DYE) || (fl[t] & EDM_FLAG_BLEACHED)) return;
+    const int partner = bond[t];
+    if (partner < 0) return;
+    for (int j = 0; j < pp->n_fret_maps; ++j) {
+        if (pp->fm_proximity[j] || pp->fm_skip[j] || dye[t] != pp->fm_donor[j]) continue;
+        if (pp->fm_intra[j]) {
+            kf = pp->fm_kfret[j];
+            acc = -1;
+        } else {
+            const int s = slot_of[partner];
+            if ((fl[s] & EDM_FLAG_DYE) && dye[s] == pp->fm_acceptor[j]) {
+                kf = pp->fm_kfret[j];
+                acc = partner;
+            }
+        }
+    }
+}
+
+// Pixel draw from the emitter's (2R+1)^2 Gaussian window factorised as
+// exp(-dx^2 k) exp(-dy^2 k): the same discrete distribution as the
+// reference's joint CDF walk (di-major), evaluated with 2(2R+1) exps in
+// the working precision.  Falls back to psf_pick for very wide windows.
+template <typename Real>
+__device__ __forceinline__ bool psf_pick_sep(const edm_image_params* ip, const double* pcx,
+                                             const double* pcy, Real x, Real y, int ch, Real u,
+                                             int& opx, int& opy) {
+    constexpr int kMax = 31;
+    const int R = ip->R, nx = ip->nx, ny = ip->ny;
+    if (2 * R + 1 > kMax) return psf_pick(ip, pcx, pcy, (double)x, (double)y, ch, (double)u, opx, opy);
+    if (ch < 0) ch = 0;
+    if (ch >= ip->n_sigma) ch = 0;
+    Real sigma = (Real)ip->sigma_ch[ch];
+    if (sigma <= (Real)0) sigma = (Real)ip->sigma_ch[0];
+    const Real pix = (Real)ip->pixel;
+    const int i0 = (int)floor((x - (Real)ip->offx) / pix - (Real)0.5 + (Real)0.5);
+    const int j0 = (int)floor((y - (Real)ip->offy) / pix - (Real)0.5 + (Real)0.5);
+    if (i0 < -R || i0 >= nx + R || j0 < -R || j0 >= ny + R) return false;
+    const Real inv2 = (Real)0.5 / (sigma * sigma);
+    Real wx[kMax], wy[kMax];
+    Real Wx = 0, Wy = 0;
+    for (int d = -R; d <= R; ++d) {
+        const int ii = i0 + d, jj = j0 + d;
+        Real a = 0, b = 0;
+        if (ii >= 0 && ii < nx) {
+            const Real dx = x - (Real)pcx[ii];
+            a = exp(-dx * dx * inv2);
+        }
+        if (jj >= 0 && jj < ny) {
+            const Real dy = y - (Real)pcy[jj];
+            b = exp(-dy * dy * inv2);
+        }
+        wx[d + R] = a;
+        wy[d + R] = b;
+        Wx += a;
+        Wy += b;
+    }
+    const Real total = Wx * Wy;
+    if (!(total > (Real)0)) return false;
+    Real uu = u * total;
+    int si = -1;
+    for (int d = 0; d < 2 * R + 1; ++d) {
+        if (wx[d] <= (Real)0) continue;
+        const Real row = wx[d] * Wy;
+        si = d;
+        if (uu <= row) break;
+        uu -= row;
+    }
+    int sj = -1;
+    for (int d = 0; d < 2 * R + 1; ++d) {
+        if (wy[d] <= (Real)0) continue;
+        const Real cell = wx[si] * wy[d];
+        sj = d;
+        if (uu <= cell) break;
+        uu -= cell;
+    }
+    if (si < 0 || sj < 0) return false;
+    opx = i0 + si - R;
+    opy = j0 + sj - R;
+    return true;
+}
+
+// Fused per-emitter photon path with Philox draws: FRET rate, PSS counts
+// and CTMC transition, then for every photon its arrival time, detection,
+// PSF pixel, IRF jitter and TCSPC bin.  Only dye carriers are launched.
 template <typename Real>
 __global__ void k_pss_image(int n_dye, const int32_t* dye_ids, const edm_photo_params* pp,
                             const edm_image_params* ip, const double* pcx, const double* pcy,
                             const Real* x, const Real* y, uint8_t* fl, const int8_t* dye,
-                            int8_t* ctmc, const double* k_fret, const int32_t* acc,
-                            const int32_t* slot_of, uint32_t* flim, Rng rng, uint32_t step,
-                            unsigned long long* ctr) {
+                            int8_t* ctmc, const int32_t* bond, const double* k_fret,
+                            const int32_t* acc, const int32_t* slot_of, uint32_t* flim, Rng rng,
+                            uint32_t step, unsigned long long* ctr) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     int emitted = 0, detected = 0;
     if (q < n_dye) {
@@ -318,36 +409,43 @@ __global__ void k_pss_image(int n_dye, const int32_t* dye_ids, const edm_photo_p
         const int t = slot_of[it];
         uint8_t f = fl[t];
         int8_t c = ctmc[t];
+        double kf;
+        int aid;
+        if (k_fret) {
+            kf = k_fret[q];
+            aid = acc[q];
+        } else {
+            fret_inline(pp, t, fl, dye, bond, slot_of, kf, aid);
+        }
         const U4 r0 = rng.draw(step, (uint32_t)it, 0u, kTagPhotonPois);
         const double up = u01d(r0.x, r0.y), ut = u01d(r0.z, r0.w);
         PssOut o{0, 0, 0.0, 0};
-        const int aid = acc[q];
-        if (pss_eval(pp, f, c, dye[t], k_fret[q], aid, up, ut, o)) {
+        if (pss_eval(pp, f, c, dye[t], kf, aid, up, ut, o)) {
             emitted = o.nd + o.nf;
-            const double qe = ip->qe;
-            const double sirf = ip->sigma_irf;
+            const Real qe = (Real)ip->qe;
+            const Real sirf = (Real)ip->sigma_irf;
             for (int k = 0; k < o.nd + o.nf; ++k) {
                 const bool donor = k < o.nd;
                 int es = t, ch = o.ty;
-                double tau = o.tau;
+                Real tau = (Real)o.tau;
                 if (!donor) {
                     es = slot_of[aid];
                     ch = dye[es];
-                    tau = (ch >= 0 && ch < pp->n_fluoro) ? pp->tau[ch] : o.tau;
+                    tau = (ch >= 0 && ch < pp->n_fluoro) ? (Real)pp->tau[ch] : (Real)o.tau;
                 }
                 const U4 ra = rng.draw(step, (uint32_t)it, (uint32_t)k, kTagArrival);
-                double ua = u01d(ra.x, ra.y);
-                if (ua < 1e-30) ua = 1e-30;
-                const double ta = -tau * log(ua);
-                if (u01d(ra.z, ra.w) >= qe) continue;  // detection efficiency
+                if (Uni<Real>::second(ra) >= qe) continue;  // detection efficiency
+                Real ua = Uni<Real>::first(ra);
+                if (ua < (Real)1e-30) ua = (Real)1e-30;
+                const Real ta = -tau * log(ua);
                 const U4 rp = rng.draw(step, (uint32_t)it, (uint32_t)k, kTagPixel);
                 int px_, py_;
-                if (!psf_pick(ip, pcx, pcy, (double)x[es], (double)y[es], ch, u01d(rp.x, rp.y), px_,
-                              py_))
+                if (!psf_pick_sep<Real>(ip, pcx, pcy, x[es], y[es], ch, Uni<Real>::first(rp), px_,
+                                        py_))
                     continue;
-                double z0, z1;
-                normal2<double>(rng.draw(step, (uint32_t)it, (uint32_t)k, kTagJitter), z0, z1);
-                tcspc_add(ip, flim, ch, px_, py_, ta + sirf * z0);
+                Real z0, z1;
+                normal2<Real>(rng.draw(step, (uint32_t)it, (uint32_t)k, kTagJitter), z0, z1);
+                tcspc_add(ip, flim, ch, px_, py_, (double)(ta + sirf * z0));
                 ++detected;
             }
         }
@@ -681,9 +779,21 @@ static const edm_image_params* dev_ip(edm_membrane* m) {
 
 int ph_fused_image(edm_membrane* m, int64_t* n_emitted, int64_t* n_detected) {
     if (!m->has_ip) return EDM_ESTATE;
-    const int n = m->n;
     hipStream_t st = m->stream;
-    if (int rc = ph_fret(m)) return rc;
+    if (m->n_dye == 0) {
+        if (n_emitted) *n_emitted = 0;
+        if (n_detected) *n_detected = 0;
+        return 0;
+    }
+    // proximity FRET needs the global search kernel; bond FRET is computed
+    // inside the fused kernel
+    bool prox = false;
+    for (int j = 0; j < m->pp.n_fret_maps; ++j)
+        if (m->pp.fm_proximity[j] && !m->pp.fm_skip[j]) prox = true;
+    if (prox)
+        if (int rc = ph_fret(m)) return rc;
+    const double* kf = prox ? (const double*)m->k_fret : nullptr;
+    const int32_t* ac = prox ? (const int32_t*)m->acc : nullptr;
     const edm_photo_params* dpp = (const edm_photo_params*)m->d_pp;
     Rng rng{(uint32_t)(m->p.seed & 0xffffffffu) ^ 0x5bd1e995u, (uint32_t)(m->p.seed >> 32)};
     // the photon step uses the count of completed dynamics steps as its counter
@@ -691,20 +801,18 @@ int ph_fused_image(edm_membrane* m, int64_t* n_emitted, int64_t* n_detected) {
     const edm_image_params* d_ip = dev_ip(m);
     if (m->fp64)
         hipLaunchKernelGGL(k_pss_image<double>, dim3(nb(m->n_dye)), dim3(kB), 0, st, m->n_dye,
-                           (const int32_t*)m->dye_ids, dpp, d_ip,
-                           (const double*)m->pcx, (const double*)m->pcy, (const double*)m->a.x,
-                           (const double*)m->a.y, m->a.fl,
-                           (const int8_t*)m->a.dye, m->a.ctmc, (const double*)m->k_fret,
-                           (const int32_t*)m->acc, (const int32_t*)m->a.slot_of, m->flim, rng,
-                           step, m->d_ctr);
+                           (const int32_t*)m->dye_ids, dpp, d_ip, (const double*)m->pcx,
+                           (const double*)m->pcy, (const double*)m->a.x, (const double*)m->a.y,
+                           m->a.fl, (const int8_t*)m->a.dye, m->a.ctmc,
+                           (const int32_t*)m->a.bond, kf, ac, (const int32_t*)m->a.slot_of,
+                           m->flim, rng, step, m->d_ctr);
     else
         hipLaunchKernelGGL(k_pss_image<float>, dim3(nb(m->n_dye)), dim3(kB), 0, st, m->n_dye,
-                           (const int32_t*)m->dye_ids, dpp, d_ip,
-                           (const double*)m->pcx, (const double*)m->pcy, (const float*)m->a.x,
-                           (const float*)m->a.y, m->a.fl,
-                           (const int8_t*)m->a.dye, m->a.ctmc, (const double*)m->k_fret,
-                           (const int32_t*)m->acc, (const int32_t*)m->a.slot_of, m->flim, rng,
-                           step, m->d_ctr);
+                           (const int32_t*)m->dye_ids, dpp, d_ip, (const double*)m->pcx,
+                           (const double*)m->pcy, (const float*)m->a.x, (const float*)m->a.y,
+                           m->a.fl, (const int8_t*)m->a.dye, m->a.ctmc,
+                           (const int32_t*)m->a.bond, kf, ac, (const int32_t*)m->a.slot_of,
+                           m->flim, rng, step, m->d_ctr);
     EDM_HIP(hipGetLastError());
     if (n_emitted || n_detected) {
         unsigned long long c[2];
