@@ -66,7 +66,7 @@ def test_dense_harmonic_tiles_fp64(oracle_lib, exact):
     print(f"dense harmonic exact={exact}: worst relative position error {worst:.3e}")
 
 
-@pytest.mark.parametrize("n_steps,exact", [(40, True), (40, False)])
+@pytest.mark.parametrize("n_steps,exact", [(100, True), (40, False)])
 def test_plain_membrane_dynamics_fp64(oracle_lib, n_steps, exact):
     cfg = workloads.plain_membrane()
     st, eng, ost, odyn = _pair(cfg, exact=exact)
