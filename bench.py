@@ -38,6 +38,38 @@ HBM_PEAK_GBPS = 8000.0
 FORCE_BYTES_PER_PARTICLE = 22
 
 
+def copy_probe(mib: int) -> int:
+    """One device-to-device copy of ``mib`` MiB before the run: the kernel
+    of known traffic (read mib, write mib) that scales the FETCH_SIZE and
+    WRITE_SIZE counters of a rocprofv3 --pmc pass."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed in the copy probe")
+    hip.hipMemset(src, 1, ctypes.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    if hip.hipMemcpy(dst, src, ctypes.c_size_t(n), 3) != 0:  # device to device
+        raise RuntimeError("hipMemcpy failed in the copy probe")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    return n
+
+
+def traffic_from_profile():
+    """HBM bytes per force launch measured by rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE (scaled by the same counters on a copy of known size), as
+    recorded in profiles/k_force_tile_1M_counters.json; None if absent."""
+    path = os.path.join(ROOT, "profiles", "k_force_tile_1M_counters.json")
+    if not os.path.exists(path):
+        return None
+    with open(path, "r", encoding="utf-8") as f:
+        c = json.load(f)
+    return c["fetch_kib"] * 1024.0 * c["fetch_scale"] + c["write_kib"] * 1024.0 * c["write_scale"]
+
+
 def gpu_run(steps: int, warmup: int, n_alive: int):
     import numpy as np
     from ed_mrdt import _hip, workloads
@@ -137,12 +169,17 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="copy this many MiB device-to-device first (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not "
                                    "part of it (run with --gpus 1)"}))
         sys.exit(2)
+    if a.copy_probe_mib:
+        copy_probe(a.copy_probe_mib)
     r = gpu_run(a.steps, a.warmup, a.n)
+    traffic = traffic_from_profile()
     value = r["alive"] * r["steps"] / r["wall"]
     force_avg_s = r["force_ms"] / r["steps"] * 1e-3
     achieved = FORCE_BYTES_PER_PARTICLE * r["N"] / force_avg_s / 1e9
@@ -165,7 +202,7 @@ def main():
                    "step": "bin+force+BD+reactions+PSS photons+PSF/TCSPC imaging"},
         "roofline": {"bound": "hbm", "kernel": "k_force_tile<float>",
                      "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
                      "avg_launch_us": force_avg_s * 1e6,
                      "bytes_per_launch": FORCE_BYTES_PER_PARTICLE * r["N"]},
         "photons_per_s": r["emitted"] / max(r["wall"], 1e-9),
