@@ -12,9 +12,11 @@
  * this library, and only as the checker or the timed CPU baseline.  The
  * product path (the HIP library) never calls it.
  *
- * Pinning (which check fixes each function) is listed in oracle/README.md;
- * functions marked "parity unpinned" there are checked only against this
- * restatement's own reading of the reference source.
+ * Pinning (which check fixes each function) is listed in oracle/README.md.
+ * parity unpinned: orc_pss_kernel's photon order and arrival-array
+ * consumption (no reference test fixes them; checked only against this
+ * restatement's reading of photophysics.py:339-483 and frozen by the golden
+ * vectors of tests/golden/).
  *
  * Build: oracle/build_oracle.py (gcc -O2 -ffp-contract=off -fopenmp).
  */

@@ -17,6 +17,12 @@ package (the product's own host code is compared against them in tests).
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use
 this module.
+
+parity unpinned: the compositions (OracleDynamics.step, the photon step,
+process_photons / finalize_frame) follow the reference source text; no
+reference test or fixture fixes their multi-step output.  They are frozen by
+the golden vectors of tests/golden/ (oracle/README.md lists the pinning of
+every kernel they call).
 """
 
 from __future__ import annotations
