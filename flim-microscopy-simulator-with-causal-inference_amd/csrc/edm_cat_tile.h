@@ -79,7 +79,8 @@ __device__ __forceinline__ int cat_try(const Cells<Real>& C, const Post<Real>& P
     if (A.dr.base) {
         uu = A.dr.base[cat_pick(A.doff, j) + pair_index(C, sa, sb)];
     } else {
-        const uint32_t lo = (uint32_t)min(it, iu), hi = (uint32_t)max(it, iu);
+        const uint32_t lo = (uint32_t)(min(it, iu) - C.kbase);
+        const uint32_t hi = (uint32_t)(max(it, iu) - C.kbase);
         uu = (double)Uni<Real>::first(
             A.rng.draw(A.step, lo, hi, (kTagCat << 16) | cat_pick(A.rxi, j)));
     }
@@ -91,10 +92,11 @@ __device__ __forceinline__ int cat_try(const Cells<Real>& C, const Post<Real>& P
     return 0;
 }
 
+// Catalysis over one strip; every thread of the 256-wide block must call it.
 template <typename Real>
-__global__ __launch_bounds__(256) void k_cat_tile(Cells<Real> C, Post<Real> P, CatArgs<Real> A,
-                                                  unsigned long long* ctr, Grids<Real> G,
-                                                  int n_grids, int ntile_y, int ntiles) {
+__device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>& P,
+                                          const CatArgs<Real>& A, unsigned long long* ctr,
+                                          const Grids<Real>& G, int n_grids, const StripGeo& S) {
     constexpr int kCap = 1024;
     __shared__ Real s_x[kCap], s_y[kCap], s_xn[kCap], s_yn[kCap];
     __shared__ int8_t s_sp[kCap];
@@ -105,9 +107,6 @@ __global__ __launch_bounds__(256) void k_cat_tile(Cells<Real> C, Post<Real> P, C
     __shared__ int s_wsum[4];
     __shared__ int s_nlist;
 
-    const int tile = xcd_tile(ntiles);
-    if (tile >= ntiles) return;  // uniform across the block
-    const StripGeo S = strip_geo(C, tile, ntile_y);
     if (threadIdx.x == 0) s_nlist = 0;
     const int total = strip_cells(C, S, s_gst, s_off, s_wsum);
     const bool staged = total <= kCap;
@@ -220,4 +219,13 @@ __global__ __launch_bounds__(256) void k_cat_tile(Cells<Real> C, Post<Real> P, C
     unsigned long long v = hits;
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
     if (lane_id() == 0 && v) atomicAdd(&ctr[2], v);
+}
+
+template <typename Real>
+__global__ __launch_bounds__(256) void k_cat_tile(Cells<Real> C, Post<Real> P, CatArgs<Real> A,
+                                                  unsigned long long* ctr, Grids<Real> G,
+                                                  int n_grids, int ntile_y, int ntiles) {
+    const int tile = xcd_tile(ntiles);
+    if (tile >= ntiles) return;  // uniform across the block
+    cat_strip(C, P, A, ctr, G, n_grids, strip_geo(C, tile, ntile_y));
 }

@@ -15,6 +15,7 @@
 
 #include "edm_device.h"
 #include "edm_internal.h"
+#include "edm_ensemble.h"
 
 namespace edm {
 
@@ -39,6 +40,8 @@ struct Cells {
     const uint64_t* poff;
     const int64_t* npairs;  // device scalar: length of the reference pair list
     bool cap_active;
+    int kbase;  // first particle index of this membrane (0 unless batched); draw keys
+                // use the index relative to it
 };
 
 template <typename Real>
@@ -160,6 +163,13 @@ static FbDev make_fb(const edm_membrane* m, const edm_feedback& f) {
         d.ginv = area > 0.0 ? 1.0 / area : 0.0;
     }
     return d;
+}
+
+// A rule without its grid resolved (the batched ensemble binds the grid of
+// each replica on the device).
+static FbDev to_fb_model(const edm_feedback& f) {
+    return FbDev{f.sensor, f.grid, f.inhibition, f.K_half, f.n_hill, f.basal, 0, 1, 1, 1.0, 1.0,
+                 0.0};
 }
 
 // Grid cell of (x, y) on the rule's grid (int() truncation, clamped).
@@ -837,18 +847,22 @@ __global__ void k_zero_u32(int n, unsigned int* p) {
 
 // Accept the max_recruits candidates with the lowest particle index (the
 // reference stops its index-order scan after max_recruits), one workgroup.
+// Index-order recruitment cap over one membrane's candidates (particle
+// indices id_lo..id_hi): a binary search finds the smallest index threshold
+// that admits max_recruits candidates, then the admitted ones come alive at
+// their drawn sites.  Every thread of the block must call it.
 template <typename Real>
-__global__ __launch_bounds__(1024) void k_recruit_apply(Post<Real> P, int target, int maxr, int n,
-                                                        const int32_t* cid, const int32_t* cslot,
-                                                        const double* cxo, const double* cyo,
-                                                        const unsigned int* nc,
-                                                        unsigned long long* ctr, Grids<Real> G,
-                                                        int n_grids) {
+__device__ __forceinline__ void recruit_apply_block(const Post<Real>& P, int target, int maxr,
+                                                    int id_lo, int id_hi, const int32_t* cid,
+                                                    const int32_t* cslot, const double* cxo,
+                                                    const double* cyo, unsigned int* nc,
+                                                    unsigned long long* ctr, const Grids<Real>& G,
+                                                    int n_grids) {
     __shared__ int s_cnt;
     const int m = (int)*nc;
     int thr = INT_MAX;
     if (m > maxr) {
-        int lo = 0, hi = n - 1;
+        int lo = id_lo, hi = id_hi;
         while (lo < hi) {
             const int mid = lo + (hi - lo) / 2;
             if (threadIdx.x == 0) s_cnt = 0;
@@ -876,7 +890,17 @@ __global__ __launch_bounds__(1024) void k_recruit_apply(Post<Real> P, int target
     }
     if (acc) atomicAdd(&ctr[3], (unsigned long long)acc);
     __syncthreads();
-    if (threadIdx.x == 0) *const_cast<unsigned int*>(nc) = 0u;  // ready for the next pass
+    if (threadIdx.x == 0) *nc = 0u;  // ready for the next pass
+}
+
+template <typename Real>
+__global__ __launch_bounds__(1024) void k_recruit_apply(Post<Real> P, int target, int maxr, int n,
+                                                        const int32_t* cid, const int32_t* cslot,
+                                                        const double* cxo, const double* cyo,
+                                                        unsigned int* nc,
+                                                        unsigned long long* ctr, Grids<Real> G,
+                                                        int n_grids) {
+    recruit_apply_block(P, target, maxr, 0, n - 1, cid, cslot, cxo, cyo, nc, ctr, G, n_grids);
 }
 
 #include "edm_passes.h"
@@ -920,6 +944,7 @@ static Cells<Real> make_cells(edm_membrane* m) {
     C.poff = m->pair_off;
     C.npairs = m->d_scal;
     C.cap_active = m->need_pair_offsets && m->n_pairs < m->n_pairs_found;
+    C.kbase = 0;
     return C;
 }
 
@@ -961,6 +986,57 @@ static int upload_real(void** dst, const double* src, size_t n) {
     EDM_HIP(hipMalloc(dst, tmp.size() * sizeof(Real)));
     EDM_HIP(hipMemcpy(*dst, tmp.data(), tmp.size() * sizeof(Real), hipMemcpyHostToDevice));
     return 0;
+}
+
+// Fused per-particle passes.  Counts exist only for kept (sensor) species;
+// a pass "writes" the kept species whose counts it changes and "reads" its
+// feedback sensors.  A run is cut before a pass that reads what an earlier
+// pass of the run writes, after a pass that reads what it writes itself
+// (deferred, folded after the run) and after every recruitment pass (its
+// cap needs all candidates).  Passes with zero probability change nothing
+// and are skipped.
+void plan_passes(const std::vector<edm_reaction>& rx, uint32_t sensor_mask,
+                 std::vector<std::vector<int>>& groups, std::vector<int>& group_start,
+                 std::vector<int>& deferred) {
+    const size_t nr = rx.size();
+    group_start.assign(nr, -1);
+    deferred.assign(nr, 0);
+    groups.clear();
+    auto kept = [&](int s) { return s >= 0 && ((sensor_mask >> s) & 1u); };
+    std::vector<int> cur;
+    uint32_t wmask = 0;
+    auto close = [&]() {
+        if (!cur.empty()) {
+            group_start[cur.front()] = (int)groups.size();
+            groups.push_back(cur);
+        }
+        cur.clear();
+        wmask = 0;
+    };
+    for (size_t i = 0; i < nr; ++i) {
+        const edm_reaction& r = rx[i];
+        const bool per_particle =
+            r.kind == EDM_RX_DETACH || r.kind == EDM_RX_CONVERT || r.kind == EDM_RX_RECRUIT;
+        if (!per_particle) {
+            close();
+            continue;
+        }
+        if (r.prob <= 0.0) continue;
+        uint32_t rmask = 0, own = 0;
+        if (r.fb1.sensor >= 0) rmask |= 1u << r.fb1.sensor;
+        if (r.fb2.sensor >= 0) rmask |= 1u << r.fb2.sensor;
+        if (r.kind == EDM_RX_DETACH && kept(r.a)) own |= 1u << r.a;
+        if (r.kind == EDM_RX_CONVERT) {
+            if (kept(r.a)) own |= 1u << r.a;
+            if (kept(r.b)) own |= 1u << r.b;
+        }
+        if (!cur.empty() && ((rmask & wmask) || cur.size() >= (size_t)kMaxPasses)) close();
+        deferred[i] = (rmask & own) ? 1 : 0;
+        cur.push_back((int)i);
+        wmask |= own;
+        if (deferred[i] || r.kind == EDM_RX_RECRUIT) close();
+    }
+    close();
 }
 
 template <typename Real>
@@ -1050,54 +1126,7 @@ static int create_t(edm_membrane* m) {
         if (dalloc(&m->w_adapt, (size_t)g.nx * g.ny)) return EDM_EHIP;
         EDM_HIP(hipMemset(m->w_adapt, 0, (size_t)g.nx * g.ny * sizeof(double)));
     }
-    // Fused per-particle passes.  Counts exist only for kept (sensor)
-    // species; a pass "writes" the kept species whose counts it changes and
-    // "reads" its feedback sensors.  A run is cut before a pass that reads
-    // what an earlier pass of the run writes, after a pass that reads what
-    // it writes itself (deferred, folded after the run) and after every
-    // recruitment pass (its cap needs all candidates).  Passes with zero
-    // probability change nothing and are skipped.
-    {
-        const size_t nr = m->rx.size();
-        m->group_start.assign(nr, -1);
-        m->pass_deferred.assign(nr, 0);
-        m->pass_groups.clear();
-        auto kept = [&](int s) { return s >= 0 && ((m->sensor_mask_all >> s) & 1u); };
-        std::vector<int> cur;
-        uint32_t wmask = 0;
-        auto close = [&]() {
-            if (!cur.empty()) {
-                m->group_start[cur.front()] = (int)m->pass_groups.size();
-                m->pass_groups.push_back(cur);
-            }
-            cur.clear();
-            wmask = 0;
-        };
-        for (size_t i = 0; i < nr; ++i) {
-            const edm_reaction& r = m->rx[i];
-            const bool per_particle =
-                r.kind == EDM_RX_DETACH || r.kind == EDM_RX_CONVERT || r.kind == EDM_RX_RECRUIT;
-            if (!per_particle) {
-                close();
-                continue;
-            }
-            if (r.prob <= 0.0) continue;
-            uint32_t rmask = 0, own = 0;
-            if (r.fb1.sensor >= 0) rmask |= 1u << r.fb1.sensor;
-            if (r.fb2.sensor >= 0) rmask |= 1u << r.fb2.sensor;
-            if (r.kind == EDM_RX_DETACH && kept(r.a)) own |= 1u << r.a;
-            if (r.kind == EDM_RX_CONVERT) {
-                if (kept(r.a)) own |= 1u << r.a;
-                if (kept(r.b)) own |= 1u << r.b;
-            }
-            if (!cur.empty() && ((rmask & wmask) || cur.size() >= (size_t)kMaxPasses)) close();
-            m->pass_deferred[i] = (rmask & own) ? 1 : 0;
-            cur.push_back((int)i);
-            wmask |= own;
-            if (m->pass_deferred[i] || r.kind == EDM_RX_RECRUIT) close();
-        }
-        close();
-    }
+    plan_passes(m->rx, m->sensor_mask_all, m->pass_groups, m->group_start, m->pass_deferred);
     // Species the passes act on, closed under every reaction that turns one
     // species into another: a particle outside the set is never touched by a
     // pass and never enters the set, so the passes can run over a list of
@@ -1412,8 +1441,8 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
                                            last.a, last.max_recruits, n,
                                            (const int32_t*)m->cand_id,
                                            (const int32_t*)m->cand_slot, (const double*)m->cand_x,
-                                           (const double*)m->cand_y,
-                                           (const unsigned int*)m->n_cand, m->d_ctr, G, ng);
+                                           (const double*)m->cand_y, m->n_cand, m->d_ctr, G,
+                                           ng);
                     if (defer_any && m->n_count)
                         hipLaunchKernelGGL(k_fold, dim3(nblk(m->n_count)), dim3(kBlock), 0, st,
                                            m->n_count, m->counts, m->delta);
@@ -1481,5 +1510,7 @@ int dyn_density(edm_membrane* m, int32_t species, int32_t grid, double* out) {
     return m->fp64 ? density_t<double>(m, species, grid, out)
                    : density_t<float>(m, species, grid, out);
 }
+
+#include "edm_ens_dyn.h"
 
 }  // namespace edm

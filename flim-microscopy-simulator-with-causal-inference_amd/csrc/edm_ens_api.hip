@@ -1,0 +1,369 @@
+// C-ABI entry points of the batched ensemble (edm_ens_* in edm_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "edm_ensemble.h"
+
+using namespace edm;
+
+static int ens_invalid(const std::string& msg) {
+    set_error(msg);
+    return EDM_EINVAL;
+}
+
+static void ens_free(edm_ensemble* e) {
+    void* bufs[] = {e->a.x, e->a.y, e->a.xn, e->a.yn, e->a.xt, e->a.yt, e->a.sp, e->a.sp_t,
+                    e->a.fl, e->a.fl_t, e->a.id, e->a.id_t, e->a.ctmc, e->a.ctmc_t, e->a.dye,
+                    e->a.dye_t, e->rep, e->rep_t, e->key, e->rank, e->perm, e->cell_cnt,
+                    e->cell_start, e->cub_tmp, e->counts, e->delta, e->cand_id, e->cand_slot,
+                    e->cand_x, e->cand_y, e->n_cand, e->ctr, e->frame_out, e->sig, e->eps, e->kh,
+                    e->Dbase, e->d_pp, e->d_seg, e->d_tiles};
+    for (void* b : bufs)
+        if (b) hipFree(b);
+    for (auto& g : e->seg)
+        if (g.flim) hipFree(g.flim);
+    if (e->stream) hipStreamDestroy(e->stream);
+    delete e;
+}
+
+extern "C" {
+
+int edm_ens_create(const edm_params* p, const edm_photo_params* pp, const edm_image_params* ip,
+                   int64_t capacity, int32_t max_replicas, edm_ensemble** out) {
+    if (!p || !out) return ens_invalid("edm_ens_create: null argument");
+    *out = nullptr;
+    if (capacity <= 0 || capacity >= INT32_MAX / 2)
+        return ens_invalid("edm_ens_create: particle capacity out of range");
+    if (max_replicas <= 0 || max_replicas > 65535)
+        return ens_invalid("edm_ens_create: 1..65535 replicas");
+    if (p->n_species <= 0 || p->n_species > kEnsMaxSpecies)
+        return ens_invalid("edm_ens_create: 1..32 species supported");
+    if (p->n_grids < 0 || p->n_grids > kEnsMaxGrids || (p->n_grids > 0 && !p->grids))
+        return ens_invalid("edm_ens_create: at most 4 density grids");
+    if (p->n_reactions < 0 || p->n_reactions > kEnsMaxRx || (p->n_reactions > 0 && !p->reactions))
+        return ens_invalid("edm_ens_create: at most 16 reactions");
+    if (!p->sigma || !p->epsilon || !p->k_harm || !p->D_base)
+        return ens_invalid("edm_ens_create: missing parameter arrays");
+    if (p->n_regions != 0)
+        return ens_invalid("edm_ens_create: circular D(x) regions are per-membrane; not batched");
+    if (p->adapt_grid >= 0)
+        return ens_invalid("edm_ens_create: the adaptation field is not batched");
+    for (int i = 0; i < p->n_reactions; ++i) {
+        const edm_reaction& r = p->reactions[i];
+        if (r.kind == EDM_RX_BIND || r.kind == EDM_RX_ADAPT)
+            return ens_invalid("edm_ens_create: binding reactions are not batched");
+        if (r.kind == EDM_RX_CAT && (r.a == r.b || r.a == r.c))
+            return ens_invalid("edm_ens_create: catalysis needs enzyme distinct from substrate "
+                               "and product");
+        if (r.kind < EDM_RX_BIND || r.kind > EDM_RX_ADAPT)
+            return ens_invalid("edm_ens_create: unknown reaction kind");
+        const edm_feedback* fbs[2] = {&r.fb1, &r.fb2};
+        for (auto f : fbs)
+            if (f->sensor >= 0 && (f->grid < 0 || f->grid >= p->n_grids || f->sensor >= p->n_species))
+                return ens_invalid("edm_ens_create: feedback refers to a missing grid or species");
+    }
+    if (pp) {
+        for (int j = 0; j < pp->n_fret_maps; ++j)
+            if (!pp->fm_skip[j] && (pp->fm_proximity[j] || !pp->fm_intra[j]))
+                return ens_invalid("edm_ens_create: only intramolecular bond FRET is batched");
+    }
+    edm_ensemble* e = new edm_ensemble();
+    e->p = *p;
+    e->fp64 = p->fp64 != 0;
+    e->rs = e->fp64 ? 8 : 4;
+    e->cap = capacity;
+    e->rmax = max_replicas;
+    e->grids.assign(p->grids, p->grids + p->n_grids);
+    e->rx.assign(p->reactions, p->reactions + p->n_reactions);
+    hipError_t he = hipGetDevice(&e->device);
+    if (he == hipSuccess) he = hipStreamCreate(&e->stream);
+    if (he != hipSuccess) {
+        delete e;
+        return hip_fail(he, "edm_ens_create: stream");
+    }
+    int rc = ens_create(e);
+    e->p.grids = nullptr;
+    e->p.reactions = nullptr;
+    e->p.sigma = e->p.epsilon = e->p.k_harm = e->p.D_base = nullptr;
+    e->p.rcx = e->p.rcy = e->p.rr = e->p.rf = nullptr;
+    if (!rc && pp) {
+        e->pp = *pp;
+        e->has_pp = true;
+        he = hipMalloc(&e->d_pp, sizeof(edm_photo_params));
+        if (he == hipSuccess)
+            he = hipMemcpy(e->d_pp, pp, sizeof(edm_photo_params), hipMemcpyHostToDevice);
+        if (he != hipSuccess) rc = hip_fail(he, "edm_ens_create: photophysics");
+    }
+    if (!rc && ip) {
+        if (ip->n_bins <= 0 || ip->n_ch <= 0 || ip->n_sigma <= 0 || ip->n_sigma > 8)
+            rc = ens_invalid("edm_ens_create: bad imaging constants");
+        e->ip = *ip;
+    }
+    if (rc) {
+        ens_free(e);
+        return rc;
+    }
+    *out = e;
+    return 0;
+}
+
+int edm_ens_destroy(edm_ensemble* e) {
+    if (!e) return 0;
+    hipStreamSynchronize(e->stream);
+    ens_free(e);
+    return 0;
+}
+
+int edm_ens_add(edm_ensemble* e, const edm_replica* r, const edm_host_state* s, int32_t* slot) {
+    if (!e || !r || !s || !slot || !s->positions || !s->species_id || !s->is_alive)
+        return ens_invalid("edm_ens_add: null argument");
+    *slot = -1;
+    if (r->n <= 0 || r->ncx < 3 || r->ncy < 3 || !(r->Lx > 0.0) || !(r->Ly > 0.0))
+        return ens_invalid("edm_ens_add: bad box or cell grid");
+    if (r->n_grids != (int)e->grids.size() || r->n_reactions != (int)e->rx.size())
+        return ens_invalid("edm_ens_add: the replica does not match the model's grids/reactions");
+    if (e->n_total + r->n > e->cap) {
+        set_error("edm_ens_add: particle pool full");
+        return EDM_ECAPACITY;
+    }
+    if (e->next_id + r->n >= INT32_MAX) {
+        set_error("edm_ens_add: particle index space exhausted");
+        return EDM_ECAPACITY;
+    }
+    const bool imaging = e->has_pp && e->ip.n_bins > 0;
+    if (imaging && (r->img_nx <= 0 || r->img_ny <= 0 || r->img_R < 0 || 2 * r->img_R + 1 > 31))
+        return ens_invalid("edm_ens_add: bad image geometry (PSF window at most 31 pixels)");
+    int k = -1;
+    for (int q = 0; q < e->rmax && k < 0; ++q)
+        if (!e->seg[q].active &&
+            std::find(e->draining.begin(), e->draining.end(), q) == e->draining.end())
+            k = q;
+    if (k < 0) {
+        set_error("edm_ens_add: no free replica slot");
+        return EDM_ECAPACITY;
+    }
+    const int n = r->n;
+    const int ns = e->p.n_species;
+    for (int i = 0; i < n; ++i)
+        if (s->species_id[i] < 0 || s->species_id[i] >= ns)
+            return ens_invalid("edm_ens_add: species id out of range");
+    EnsSeg g{};
+    g.active = 1;
+    g.kbase = (int32_t)e->next_id;
+    g.n = n;
+    g.ncx = r->ncx;
+    g.ncy = r->ncy;
+    g.px = e->p.periodic_x;
+    g.py = e->p.periodic_y;
+    g.Lx = r->Lx;
+    g.Ly = r->Ly;
+    g.csx = r->csx;
+    g.csy = r->csy;
+    g.cutoff = r->cutoff;
+    g.sin_kx = r->sin_kx;
+    g.sin_ky = r->sin_ky;
+    g.seed_lo = (uint32_t)(r->seed & 0xffffffffu);
+    g.seed_hi = (uint32_t)(r->seed >> 32);
+    g.step0 = e->gstep;
+    g.n_grids = r->n_grids;
+    for (int q = 0; q < r->n_grids; ++q) {
+        g.gdims[2 * q] = r->grids[q].nx;
+        g.gdims[2 * q + 1] = r->grids[q].ny;
+        g.gs_d[2 * q] = r->grids[q].sx;
+        g.gs_d[2 * q + 1] = r->grids[q].sy;
+        g.gs_f[2 * q] = (float)r->grids[q].sx;
+        g.gs_f[2 * q + 1] = (float)r->grids[q].sy;
+        const double area = r->grids[q].sx * r->grids[q].sy;
+        g.ginv_d[q] = area > 0.0 ? 1.0 / area : 0.0;
+        g.ginv_f[q] = (float)g.ginv_d[q];
+    }
+    for (int q = 0; q < r->n_reactions; ++q) {
+        g.prob[q] = r->prob[q];
+        g.maxr[q] = r->max_recruits[q];
+    }
+    if (imaging) {
+        g.ip = e->ip;
+        g.ip.nx = r->img_nx;
+        g.ip.ny = r->img_ny;
+        g.ip.R = r->img_R;
+        g.ip.pixel = r->img_pixel;
+        g.ip.offx = r->img_offx;
+        g.ip.offy = r->img_offy;
+        g.flim_elems = (int64_t)g.ip.n_ch * g.ip.nx * g.ip.ny * g.ip.n_bins;
+        EDM_HIP(hipMalloc(&g.flim, (size_t)g.flim_elems * sizeof(uint32_t)));
+        EDM_HIP(hipMemsetAsync(g.flim, 0, (size_t)g.flim_elems * sizeof(uint32_t), e->stream));
+    }
+    // particles, appended after every pooled one (particle-index order)
+    const int64_t beg = e->n_total;
+    std::vector<double> x(n), y(n);
+    std::vector<int8_t> sp(n), ctmc(n, 0), dye(n, -1);
+    std::vector<uint8_t> fl(n);
+    std::vector<int32_t> id(n);
+    std::vector<uint16_t> rep(n, (uint16_t)k);
+    for (int i = 0; i < n; ++i) {
+        x[i] = s->positions[2 * i];
+        y[i] = s->positions[2 * i + 1];
+        sp[i] = (int8_t)s->species_id[i];
+        uint8_t f = s->is_alive[i] ? EDM_FLAG_ALIVE : 0;
+        if (s->has_dye && s->has_dye[i]) f |= EDM_FLAG_DYE;
+        if (s->is_bleached && s->is_bleached[i]) f |= EDM_FLAG_BLEACHED;
+        fl[i] = f;
+        id[i] = g.kbase + i;
+        if (s->ctmc_state) ctmc[i] = s->ctmc_state[i];
+        if (s->dye_type_id) dye[i] = (int8_t)s->dye_type_id[i];
+    }
+    hipStream_t st = e->stream;
+    EDM_HIP(hipStreamSynchronize(st));
+    if (e->fp64) {
+        EDM_HIP(hipMemcpy((double*)e->a.x + beg, x.data(), n * 8, hipMemcpyHostToDevice));
+        EDM_HIP(hipMemcpy((double*)e->a.y + beg, y.data(), n * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> xf(x.begin(), x.end()), yf(y.begin(), y.end());
+        EDM_HIP(hipMemcpy((float*)e->a.x + beg, xf.data(), n * 4, hipMemcpyHostToDevice));
+        EDM_HIP(hipMemcpy((float*)e->a.y + beg, yf.data(), n * 4, hipMemcpyHostToDevice));
+    }
+    EDM_HIP(hipMemcpy(e->a.sp + beg, sp.data(), n, hipMemcpyHostToDevice));
+    EDM_HIP(hipMemcpy(e->a.fl + beg, fl.data(), n, hipMemcpyHostToDevice));
+    EDM_HIP(hipMemcpy(e->a.id + beg, id.data(), n * 4, hipMemcpyHostToDevice));
+    EDM_HIP(hipMemcpy(e->a.ctmc + beg, ctmc.data(), n, hipMemcpyHostToDevice));
+    EDM_HIP(hipMemcpy(e->a.dye + beg, dye.data(), n, hipMemcpyHostToDevice));
+    EDM_HIP(hipMemcpy(e->rep + beg, rep.data(), n * 2, hipMemcpyHostToDevice));
+    EDM_HIP(hipMemset(e->ctr + 8 * (size_t)k, 0, 8 * sizeof(unsigned long long)));
+    e->seg[k] = g;
+    e->fresh_beg[k] = beg;
+    e->n_total += n;
+    e->next_id += n;
+    e->layout_dirty = true;
+    *slot = k;
+    return 0;
+}
+
+int edm_ens_remove(edm_ensemble* e, int32_t slot) {
+    if (!e || slot < 0 || slot >= e->rmax || !e->seg[slot].active)
+        return ens_invalid("edm_ens_remove: no such replica");
+    EDM_HIP(hipStreamSynchronize(e->stream));
+    EnsSeg& g = e->seg[slot];
+    if (g.flim) EDM_HIP(hipFree(g.flim));
+    g.flim = nullptr;
+    g.active = 0;
+    e->n_retired += g.n;
+    e->draining.push_back(slot);
+    e->layout_dirty = true;
+    return 0;
+}
+
+int edm_ens_run(edm_ensemble* e, int32_t n_steps, int32_t image) {
+    if (!e || n_steps < 0) return ens_invalid("edm_ens_run: bad argument");
+    for (int k = 0; k < n_steps; ++k)
+        if (int rc = ens_step(e, image != 0)) return rc;
+    return 0;
+}
+
+int edm_ens_frame(edm_ensemble* e, int32_t slot, uint32_t* flim, int64_t n_elems, int64_t* n_dark,
+                  int64_t* n_ap) {
+    if (!e || slot < 0 || slot >= e->rmax || !e->seg[slot].active || !e->seg[slot].flim)
+        return ens_invalid("edm_ens_frame: no imaged replica in that slot");
+    EnsSeg& g = e->seg[slot];
+    if (int rc = ens_frame_noise(e, slot, n_dark, n_ap)) return rc;
+    if (flim) {
+        if (n_elems < g.flim_elems) return ens_invalid("edm_ens_frame: output too small");
+        EDM_HIP(hipMemcpyAsync(flim, g.flim, (size_t)g.flim_elems * 4, hipMemcpyDeviceToHost,
+                               e->stream));
+    }
+    if (int rc = ens_frame_reset(e, slot)) return rc;
+    EDM_HIP(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+int edm_ens_download(edm_ensemble* e, int32_t slot, edm_host_state* s) {
+    if (!e || !s || slot < 0 || slot >= e->rmax || !e->seg[slot].active)
+        return ens_invalid("edm_ens_download: no such replica");
+    const EnsSeg& g = e->seg[slot];
+    hipStream_t st = e->stream;
+    EDM_HIP(hipStreamSynchronize(st));
+    int64_t beg = e->fresh_beg[slot];
+    if (beg < 0) {
+        uint32_t se[2];
+        EDM_HIP(hipMemcpy(&se[0], e->cell_start + g.coff, 4, hipMemcpyDeviceToHost));
+        EDM_HIP(hipMemcpy(&se[1], e->cell_start + g.coff + (int64_t)g.ncx * g.ncy, 4,
+                          hipMemcpyDeviceToHost));
+        if ((int64_t)(se[1] - se[0]) != g.n) {
+            set_error("edm_ens_download: replica range inconsistent with its size");
+            return EDM_ESTATE;
+        }
+        beg = se[0];
+    }
+    const int n = g.n;
+    std::vector<double> x(n), y(n);
+    std::vector<int8_t> sp(n), ctmc(n), dye(n);
+    std::vector<uint8_t> fl(n);
+    std::vector<int32_t> id(n);
+    if (e->fp64) {
+        EDM_HIP(hipMemcpy(x.data(), (const double*)e->a.x + beg, n * 8, hipMemcpyDeviceToHost));
+        EDM_HIP(hipMemcpy(y.data(), (const double*)e->a.y + beg, n * 8, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> xf(n), yf(n);
+        EDM_HIP(hipMemcpy(xf.data(), (const float*)e->a.x + beg, n * 4, hipMemcpyDeviceToHost));
+        EDM_HIP(hipMemcpy(yf.data(), (const float*)e->a.y + beg, n * 4, hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i) {
+            x[i] = xf[i];
+            y[i] = yf[i];
+        }
+    }
+    EDM_HIP(hipMemcpy(sp.data(), e->a.sp + beg, n, hipMemcpyDeviceToHost));
+    EDM_HIP(hipMemcpy(fl.data(), e->a.fl + beg, n, hipMemcpyDeviceToHost));
+    EDM_HIP(hipMemcpy(id.data(), e->a.id + beg, n * 4, hipMemcpyDeviceToHost));
+    EDM_HIP(hipMemcpy(ctmc.data(), e->a.ctmc + beg, n, hipMemcpyDeviceToHost));
+    EDM_HIP(hipMemcpy(dye.data(), e->a.dye + beg, n, hipMemcpyDeviceToHost));
+    for (int t = 0; t < n; ++t) {
+        const int i = id[t] - g.kbase;
+        if (i < 0 || i >= n) {
+            set_error("edm_ens_download: particle index outside the replica");
+            return EDM_ESTATE;
+        }
+        s->positions[2 * i] = x[t];
+        s->positions[2 * i + 1] = y[t];
+        s->species_id[i] = sp[t];
+        s->is_alive[i] = (fl[t] & EDM_FLAG_ALIVE) ? 1 : 0;
+        if (s->has_dye) s->has_dye[i] = (fl[t] & EDM_FLAG_DYE) ? 1 : 0;
+        if (s->is_bleached) s->is_bleached[i] = (fl[t] & EDM_FLAG_BLEACHED) ? 1 : 0;
+        if (s->ctmc_state) s->ctmc_state[i] = ctmc[t];
+        if (s->dye_type_id) s->dye_type_id[i] = dye[t];
+        if (s->bond_partner) s->bond_partner[i] = -1;
+    }
+    return 0;
+}
+
+int edm_ens_counters(edm_ensemble* e, int32_t slot, int64_t out[8]) {
+    if (!e || !out || slot < 0 || slot >= e->rmax || !e->seg[slot].active)
+        return ens_invalid("edm_ens_counters: no such replica");
+    unsigned long long c[8];
+    EDM_HIP(hipMemcpyAsync(c, e->ctr + 8 * (size_t)slot, sizeof(c), hipMemcpyDeviceToHost,
+                           e->stream));
+    EDM_HIP(hipStreamSynchronize(e->stream));
+    for (int i = 0; i < 8; ++i) out[i] = (int64_t)c[i];
+    return 0;
+}
+
+int edm_ens_info(edm_ensemble* e, int64_t out[4]) {
+    if (!e || !out) return ens_invalid("edm_ens_info: null argument");
+    int64_t live = 0;
+    for (auto& g : e->seg) live += g.active ? 1 : 0;
+    out[0] = e->n_total;
+    out[1] = e->n_cells;
+    out[2] = live;
+    out[3] = e->cap;
+    return 0;
+}
+
+int edm_ens_synchronize(edm_ensemble* e) {
+    if (!e) return ens_invalid("edm_ens_synchronize: null handle");
+    EDM_HIP(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,574 @@
+// Batched ensemble: dynamics kernels and the step (included at the end of
+// edm_dynamics.hip, inside namespace edm).  Every kernel runs the single-
+// membrane bodies (force_strip, cat_strip, passes_particle,
+// recruit_apply_block) on views of one replica built from its EnsSeg entry,
+// so a replica computes exactly what it computes alone.
+
+template <typename Real>
+__device__ __forceinline__ Cells<Real> ens_cells(const EnsSeg& g, const Real* x, const Real* y,
+                                                 const int8_t* sp, const uint8_t* fl,
+                                                 const int32_t* id, const uint32_t* start) {
+    Cells<Real> C;
+    C.x = x;
+    C.y = y;
+    C.sp = sp;
+    C.fl = fl;
+    C.id = id;
+    C.start = start + g.coff;
+    C.ncx = g.ncx;
+    C.ncy = g.ncy;
+    C.csx = (Real)g.csx;
+    C.csy = (Real)g.csy;
+    C.Lx = (Real)g.Lx;
+    C.Ly = (Real)g.Ly;
+    C.cut2 = (Real)g.cutoff * (Real)g.cutoff;
+    C.px = g.px != 0;
+    C.py = g.py != 0;
+    C.pcnt = nullptr;
+    C.poff = nullptr;
+    C.npairs = nullptr;
+    C.cap_active = false;
+    C.kbase = g.kbase;
+    return C;
+}
+
+template <typename Real>
+__device__ __forceinline__ Grids<Real> ens_grids(const EnsSeg& g, int32_t* counts, int32_t* delta,
+                                                 int ns) {
+    Grids<Real> G;
+    G.off = g.goff;
+    G.dims = g.gdims;
+    if constexpr (sizeof(Real) == 8) {
+        G.s = (const Real*)g.gs_d;
+        G.inv = (const Real*)g.ginv_d;
+    } else {
+        G.s = (const Real*)g.gs_f;
+        G.inv = (const Real*)g.ginv_f;
+    }
+    G.counts = counts;
+    G.delta = delta;
+    G.ns = ns;
+    return G;
+}
+
+// A model feedback rule bound to replica g's grid.
+__device__ __forceinline__ FbDev ens_fb(const FbDev& f, const EnsSeg& g, int ns) {
+    FbDev d = f;
+    if (f.sensor >= 0 && f.grid >= 0) {
+        d.cbase = g.goff[f.grid * ns + f.sensor];
+        d.gnx = g.gdims[2 * f.grid];
+        d.gny = g.gdims[2 * f.grid + 1];
+        d.gsx = g.gs_d[2 * f.grid];
+        d.gsy = g.gs_d[2 * f.grid + 1];
+        d.ginv = g.ginv_d[f.grid];
+    }
+    return d;
+}
+
+__device__ __forceinline__ Rng ens_rng(const EnsSeg& g) { return Rng{g.seed_lo, g.seed_hi}; }
+
+template <typename Real>
+__device__ __forceinline__ StripGeo ens_strip(const EnsSeg& g, const EnsTile& T) {
+    StripGeo S;
+    S.tile = 0;
+    S.cx = T.cx;
+    S.cy0 = T.cy0;
+    S.rows = min(kTileRows, g.ncy - T.cy0);
+    S.nrow = S.rows + 2;
+    S.ncells = 3 * S.nrow;
+    return S;
+}
+
+// Cell key of every pooled particle (its replica's cell, offset by the
+// replica's first key; the sentinel for removed replicas), run-aggregated
+// ranks as in k_bin, and the per-step reset of the density counts.
+template <typename Real>
+__global__ void k_ens_bin(int n, const Real* __restrict__ x, const Real* __restrict__ y,
+                          const uint16_t* __restrict__ rep, const EnsSeg* __restrict__ segs,
+                          uint32_t sentinel, uint32_t* __restrict__ key,
+                          uint32_t* __restrict__ rank, uint32_t* __restrict__ cnt,
+                          int32_t* counts, int n_counts) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n_counts) counts[s] = 0;
+    const bool live = s < n;
+    uint32_t c = 0xffffffffu;
+    if (live) {
+        const EnsSeg& g = segs[rep[s]];
+        if (g.active) {
+            const int cx = trunc_cell(x[s], (Real)g.csx, g.ncx);
+            const int cy = trunc_cell(y[s], (Real)g.csy, g.ncy);
+            c = (uint32_t)(g.coff + cx * g.ncy + cy);
+        } else {
+            c = sentinel;
+        }
+        key[s] = c;
+    }
+    const int lane = lane_id();
+    const uint32_t prev = __shfl_up(c, 1);
+    const bool head = live && (lane == 0 || prev != c);
+    const uint64_t heads = __ballot(head);
+    const int nlive = __popcll(__ballot(live));
+    const uint64_t upto = heads & ((2ull << lane) - 1ull);
+    const int rs = upto ? 63 - __clzll(upto) : lane;
+    uint32_t base = 0;
+    if (head) {
+        const uint64_t after = heads & ~((2ull << lane) - 1ull);
+        const int nxt = after ? min(__ffsll((unsigned long long)after) - 1, nlive) : nlive;
+        base = atomicAdd(&cnt[c], (uint32_t)(nxt - lane));
+    }
+    base = __shfl(base, rs);
+    if (live) rank[s] = base + (uint32_t)(lane - rs);
+}
+
+template <typename Real>
+__global__ void k_ens_gather(int n, const uint32_t* __restrict__ perm, DevArrays a,
+                             const uint16_t* __restrict__ rep, uint16_t* __restrict__ rep_t) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t s = perm[t];
+    ((Real*)a.xt)[t] = ((const Real*)a.x)[s];
+    ((Real*)a.yt)[t] = ((const Real*)a.y)[s];
+    a.sp_t[t] = a.sp[s];
+    a.fl_t[t] = a.fl[s];
+    a.id_t[t] = a.id[s];
+    a.ctmc_t[t] = a.ctmc[s];
+    a.dye_t[t] = a.dye[s];
+    rep_t[t] = rep[s];
+}
+
+template <typename Real>
+struct EnsPool {
+    const Real *x, *y;
+    Real *xn, *yn;
+    int8_t* sp;
+    uint8_t* fl;
+    const int32_t* id;
+    const uint32_t* start;
+    int32_t *counts, *delta;
+    int ns, n_grids;
+};
+
+template <typename Real>
+__device__ __forceinline__ Post<Real> ens_post(const EnsPool<Real>& Q) {
+    Post<Real> P;
+    P.xn = Q.xn;
+    P.yn = Q.yn;
+    P.sp = Q.sp;
+    P.fl = Q.fl;
+    P.bond = nullptr;
+    P.slot_of = nullptr;
+    return P;
+}
+
+template <typename Real>
+__global__ __launch_bounds__(256) void k_ens_force(const EnsTile* __restrict__ tiles, int ntiles,
+                                                   const EnsSeg* __restrict__ segs,
+                                                   EnsPool<Real> Q, ForceParams<Real> F,
+                                                   uint32_t keep_mask, uint32_t gstep) {
+    const int tile = xcd_tile(ntiles);
+    if (tile >= ntiles) return;
+    const EnsTile T = tiles[tile];
+    const EnsSeg& g = segs[T.r];
+    const Cells<Real> C = ens_cells<Real>(g, Q.x, Q.y, Q.sp, Q.fl, Q.id, Q.start);
+    ForceParams<Real> Fr = F;
+    Fr.sin_kx = (Real)g.sin_kx;
+    Fr.sin_ky = (Real)g.sin_ky;
+    const Grids<Real> G = ens_grids<Real>(g, Q.counts, Q.delta, Q.ns);
+    force_strip(C, Fr, Q.xn, Q.yn, (const double*)nullptr, ens_rng(g), gstep - g.step0, G,
+                Q.n_grids, keep_mask, ens_strip<Real>(g, T));
+}
+
+// Catalysis for one run of catalytic reactions (CatArgs with the model's
+// enzymes, substrate, product and contact radii; probabilities, feedback
+// grids, draws and step per replica).
+template <typename Real>
+__global__ __launch_bounds__(256) void k_ens_cat(const EnsTile* __restrict__ tiles, int ntiles,
+                                                 const EnsSeg* __restrict__ segs, EnsPool<Real> Q,
+                                                 CatArgs<Real> A, unsigned long long* ctr,
+                                                 uint32_t gstep) {
+    const int tile = xcd_tile(ntiles);
+    if (tile >= ntiles) return;
+    const EnsTile T = tiles[tile];
+    const EnsSeg& g = segs[T.r];
+    const Cells<Real> C = ens_cells<Real>(g, Q.x, Q.y, Q.sp, Q.fl, Q.id, Q.start);
+    CatArgs<Real> Ar = A;
+    for (int j = 0; j < kCatFuse; ++j)
+        if (j < A.n_enz) Ar.prob[j] = (Real)g.prob[A.rxi[j]];
+    Ar.fb = ens_fb(A.fb, g, Q.ns);
+    Ar.rng = ens_rng(g);
+    Ar.step = gstep - g.step0;
+    const Grids<Real> G = ens_grids<Real>(g, Q.counts, Q.delta, Q.ns);
+    cat_strip(C, ens_post(Q), Ar, ctr + 8 * T.r, G, Q.n_grids, ens_strip<Real>(g, T));
+}
+
+// A run of per-particle passes over the strip's own particles; the run's
+// probabilities and feedback grids are the replica's.
+template <typename Real>
+__global__ __launch_bounds__(256) void k_ens_passes(const EnsTile* __restrict__ tiles,
+                                                    int ntiles, const EnsSeg* __restrict__ segs,
+                                                    EnsPool<Real> Q, PassGroupDev grp,
+                                                    unsigned long long* ctr, int32_t* cid,
+                                                    int32_t* cslot, double* cxo, double* cyo,
+                                                    unsigned int* nc, uint32_t gstep) {
+    __shared__ PassGroupDev s_grp;
+    const int tile = xcd_tile(ntiles);
+    if (tile >= ntiles) return;
+    const EnsTile T = tiles[tile];
+    const EnsSeg& g = segs[T.r];
+    if (threadIdx.x == 0) {
+        s_grp = grp;
+        for (int k = 0; k < grp.n; ++k) {
+            s_grp.p[k].prob = g.prob[grp.p[k].rxi];
+            s_grp.p[k].f1 = ens_fb(grp.p[k].f1, g, Q.ns);
+            s_grp.p[k].f2 = ens_fb(grp.p[k].f2, g, Q.ns);
+        }
+    }
+    __syncthreads();
+    const StripGeo S = ens_strip<Real>(g, T);
+    const uint32_t* st = Q.start + g.coff;
+    const int c0 = S.cx * g.ncy + S.cy0;
+    const int b = (int)st[c0], e = (int)st[c0 + S.rows];
+    // this replica's candidate region is its own slot range
+    const int pbeg = (int)st[0];
+    const Grids<Real> G = ens_grids<Real>(g, Q.counts, Q.delta, Q.ns);
+    const Post<Real> P = ens_post(Q);
+    const Rng rng = ens_rng(g);
+    const uint32_t step = gstep - g.step0;
+    int det = 0, conv = 0;
+    for (int t0 = b; t0 < e; t0 += blockDim.x) {
+        const int t = t0 + (int)threadIdx.x;
+        if (t < e) {
+            const int s0 = Q.sp[t];
+            if ((s_grp.species >> s0) & 1u)
+                passes_particle(t, s0, g.n, Q.id, g.kbase, P, s_grp, (const double*)nullptr, rng,
+                                step, (Real)g.Lx, (Real)g.Ly, G, Q.n_grids, cid + pbeg,
+                                cslot + pbeg, cxo + pbeg, cyo + pbeg, nc + T.r, det, conv);
+        }
+    }
+    passes_count<Real>(det, conv, ctr + 8 * T.r);
+}
+
+// Index-order recruitment cap of one recruitment reaction, one block per
+// replica slot.
+template <typename Real>
+__global__ __launch_bounds__(1024) void k_ens_recruit(const EnsSeg* __restrict__ segs,
+                                                      EnsPool<Real> Q, int rxi, int target,
+                                                      const int32_t* cid, const int32_t* cslot,
+                                                      const double* cxo, const double* cyo,
+                                                      unsigned int* nc, unsigned long long* ctr) {
+    const int r = blockIdx.x;
+    const EnsSeg& g = segs[r];
+    if (!g.active) return;
+    const int pbeg = (int)Q.start[g.coff];
+    const Grids<Real> G = ens_grids<Real>(g, Q.counts, Q.delta, Q.ns);
+    recruit_apply_block(ens_post(Q), target, g.maxr[rxi], g.kbase, g.kbase + g.n - 1, cid + pbeg,
+                        cslot + pbeg, cxo + pbeg, cyo + pbeg, nc + r, ctr + 8 * r, G,
+                        Q.n_grids);
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+template <typename Real>
+static int ens_create_t(edm_ensemble* e) {
+    const edm_params& p = e->p;
+    const int ns = p.n_species;
+    const int64_t cap = e->cap;
+    const size_t rb = sizeof(Real);
+    void** rbufs[6] = {&e->a.x, &e->a.y, &e->a.xn, &e->a.yn, &e->a.xt, &e->a.yt};
+    for (auto b : rbufs) EDM_HIP(hipMalloc(b, (size_t)cap * rb + 16));
+    if (dalloc(&e->a.sp, cap + 4) || dalloc(&e->a.sp_t, cap + 4) || dalloc(&e->a.fl, cap + 4) ||
+        dalloc(&e->a.fl_t, cap + 4) || dalloc(&e->a.id, cap) || dalloc(&e->a.id_t, cap) ||
+        dalloc(&e->a.ctmc, cap + 4) || dalloc(&e->a.ctmc_t, cap + 4) ||
+        dalloc(&e->a.dye, cap + 4) || dalloc(&e->a.dye_t, cap + 4) || dalloc(&e->rep, cap) ||
+        dalloc(&e->rep_t, cap) || dalloc(&e->key, cap) || dalloc(&e->rank, cap) ||
+        dalloc(&e->perm, cap) || dalloc(&e->cand_id, cap) || dalloc(&e->cand_slot, cap) ||
+        dalloc(&e->cand_x, cap) || dalloc(&e->cand_y, cap) || dalloc(&e->n_cand, e->rmax) ||
+        dalloc(&e->ctr, 8 * (size_t)e->rmax) || dalloc(&e->frame_out, 4) ||
+        dalloc(&e->d_seg, e->rmax))
+        return EDM_EHIP;
+    EDM_HIP(hipMemset(e->n_cand, 0, (size_t)e->rmax * sizeof(unsigned int)));
+    EDM_HIP(hipMemset(e->ctr, 0, (size_t)e->rmax * 8 * sizeof(unsigned long long)));
+    int rc = 0;
+    rc |= upload_real<Real>(&e->sig, p.sigma, (size_t)ns * ns);
+    rc |= upload_real<Real>(&e->eps, p.epsilon, (size_t)ns * ns);
+    rc |= upload_real<Real>(&e->kh, p.k_harm, (size_t)ns * ns);
+    rc |= upload_real<Real>(&e->Dbase, p.D_base, ns);
+    if (rc) return EDM_EHIP;
+    double r2 = 0.0;
+    for (int k = 0; k < ns * ns; ++k) {
+        const double sg = p.sigma[k];
+        if (!(sg > 0.0)) continue;
+        if (p.use_wca && p.epsilon[k] > 0.0) r2 = std::max(r2, sg * sg * 1.2599210498948732);
+        if (p.use_harmonic && p.k_harm[k] > 0.0) r2 = std::max(r2, sg * sg);
+    }
+    e->rint2 = r2 * (1.0 + 1e-6);
+    // kept (grid, sensor) pairs, in grid-major order as for one membrane
+    const int ng = (int)e->grids.size();
+    std::vector<int> keep((size_t)std::max(ng, 1) * ns, 0);
+    for (auto& r : e->rx) {
+        if (r.fb1.sensor >= 0) keep[(size_t)r.fb1.grid * ns + r.fb1.sensor] = 1;
+        if (r.fb2.sensor >= 0) keep[(size_t)r.fb2.grid * ns + r.fb2.sensor] = 1;
+    }
+    e->kept.clear();
+    for (int g = 0; g < ng; ++g)
+        for (int s = 0; s < ns; ++s)
+            if (keep[(size_t)g * ns + s]) {
+                e->kept.emplace_back(g, s);
+                e->sensor_mask_all |= 1u << s;
+            }
+    plan_passes(e->rx, e->sensor_mask_all, e->pass_groups, e->group_start, e->pass_deferred);
+    e->seg.assign(e->rmax, EnsSeg{});
+    e->fresh_beg.assign(e->rmax, -1);
+    EDM_HIP(hipMemset(e->d_seg, 0, (size_t)e->rmax * sizeof(EnsSeg)));
+    return 0;
+}
+
+int ens_create(edm_ensemble* e) { return e->fp64 ? ens_create_t<double>(e) : ens_create_t<float>(e); }
+
+// Offsets of the live replicas in the cell-key space and the count pool,
+// the strip table, and buffer growth.  Runs between steps only.
+int ens_layout(edm_ensemble* e) {
+    const int ns = e->p.n_species;
+    int64_t coff = 0, cnt = 0;
+    e->tiles.clear();
+    for (int r = 0; r < e->rmax; ++r) {
+        EnsSeg& g = e->seg[r];
+        if (!g.active) continue;
+        g.coff = (int32_t)coff;
+        coff += (int64_t)g.ncx * g.ncy;
+        for (int k = 0; k < kEnsMaxGrids * kEnsMaxSpecies; ++k) g.goff[k] = -1;
+        for (auto& kp : e->kept) {
+            g.goff[kp.first * ns + kp.second] = (int32_t)cnt;
+            cnt += (int64_t)g.gdims[2 * kp.first] * g.gdims[2 * kp.first + 1];
+        }
+        const int ty = (g.ncy + kTileRows - 1) / kTileRows;
+        for (int cx = 0; cx < g.ncx; ++cx)
+            for (int t = 0; t < ty; ++t) e->tiles.push_back(EnsTile{r, cx, t * kTileRows, 0});
+    }
+    if (coff >= (int64_t)INT32_MAX - 2 || cnt >= (int64_t)INT32_MAX) {
+        set_error("edm_ensemble: too many cells for 32-bit keys");
+        return EDM_ECAPACITY;
+    }
+    e->n_cells = coff;
+    e->n_counts = cnt;
+    if (coff + 2 > e->cell_cap) {
+        hipFree(e->cell_cnt);
+        hipFree(e->cell_start);
+        hipFree(e->cub_tmp);
+        e->cell_cap = std::max<int64_t>(coff + 2, e->cell_cap * 3 / 2);
+        if (dalloc(&e->cell_cnt, e->cell_cap) || dalloc(&e->cell_start, e->cell_cap))
+            return EDM_EHIP;
+        EDM_HIP(hipMemset(e->cell_cnt, 0, (size_t)e->cell_cap * sizeof(uint32_t)));
+        size_t b = 0;
+        EDM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b, e->cell_cnt, e->cell_start,
+                                                 (int)e->cell_cap, e->stream));
+        e->cub_bytes = b + 256;
+        EDM_HIP(hipMalloc(&e->cub_tmp, e->cub_bytes));
+    }
+    if (cnt > e->count_cap) {
+        hipFree(e->counts);
+        hipFree(e->delta);
+        e->count_cap = std::max<int64_t>(cnt, e->count_cap * 3 / 2);
+        if (dalloc(&e->counts, e->count_cap) || dalloc(&e->delta, e->count_cap)) return EDM_EHIP;
+        EDM_HIP(hipMemset(e->delta, 0, (size_t)e->count_cap * sizeof(int32_t)));
+        EDM_HIP(hipMemset(e->counts, 0, (size_t)e->count_cap * sizeof(int32_t)));
+    }
+    if ((int64_t)e->tiles.size() > e->tile_cap) {
+        hipFree(e->d_tiles);
+        e->tile_cap = std::max<int64_t>((int64_t)e->tiles.size(), e->tile_cap * 3 / 2);
+        if (dalloc(&e->d_tiles, e->tile_cap)) return EDM_EHIP;
+    }
+    if (!e->tiles.empty())
+        EDM_HIP(hipMemcpyAsync(e->d_tiles, e->tiles.data(), e->tiles.size() * sizeof(EnsTile),
+                               hipMemcpyHostToDevice, e->stream));
+    EDM_HIP(hipMemcpyAsync(e->d_seg, e->seg.data(), (size_t)e->rmax * sizeof(EnsSeg),
+                           hipMemcpyHostToDevice, e->stream));
+    EDM_HIP(hipStreamSynchronize(e->stream));
+    e->layout_dirty = false;
+    return 0;
+}
+
+template <typename Real>
+static int ens_step_t(edm_ensemble* e) {
+    if (e->layout_dirty)
+        if (int rc = ens_layout(e)) return rc;
+    hipStream_t st = e->stream;
+    const edm_params& p = e->p;
+    const int n = (int)e->n_total;
+    const int ntiles = (int)e->tiles.size();
+    const uint32_t sentinel = (uint32_t)e->n_cells;
+    if (n > 0) {
+        const int64_t ncnt = e->n_cells + 2;  // live keys, sentinel, scan end
+        hipLaunchKernelGGL(k_ens_bin<Real>, dim3(nblk(std::max<int64_t>(n, e->n_counts))),
+                           dim3(kBlock), 0, st, n, (const Real*)e->a.x, (const Real*)e->a.y,
+                           (const uint16_t*)e->rep, (const EnsSeg*)e->d_seg, sentinel, e->key,
+                           e->rank, e->cell_cnt, e->counts, (int)e->n_counts);
+        size_t tb = e->cub_bytes;
+        EDM_HIP(hipcub::DeviceScan::ExclusiveSum(e->cub_tmp, tb, e->cell_cnt, e->cell_start,
+                                                 (int)ncnt, st));
+        hipLaunchKernelGGL(k_scatter, dim3(nblk(n)), dim3(kBlock), 0, st, n, e->key, e->rank,
+                           e->cell_start, e->perm);
+        hipLaunchKernelGGL(k_sort_cells, dim3(nblk(ncnt - 1)), dim3(kBlock), 0, st,
+                           (int)(ncnt - 1), e->cell_start, e->perm, e->cell_cnt,
+                           (const int32_t*)e->a.id);
+        hipLaunchKernelGGL(k_ens_gather<Real>, dim3(nblk(n)), dim3(kBlock), 0, st, n,
+                           (const uint32_t*)e->perm, e->a, (const uint16_t*)e->rep, e->rep_t);
+        std::swap(e->a.x, e->a.xt);
+        std::swap(e->a.y, e->a.yt);
+        std::swap(e->a.sp, e->a.sp_t);
+        std::swap(e->a.fl, e->a.fl_t);
+        std::swap(e->a.id, e->a.id_t);
+        std::swap(e->a.ctmc, e->a.ctmc_t);
+        std::swap(e->a.dye, e->a.dye_t);
+        std::swap(e->rep, e->rep_t);
+    }
+    // particles of removed replicas now sit after every live one; their
+    // slots may be reused, and every replica's particles are cell-sorted
+    e->n_total -= e->n_retired;
+    e->n_retired = 0;
+    e->draining.clear();
+    std::fill(e->fresh_beg.begin(), e->fresh_beg.end(), (int64_t)-1);
+    if (ntiles == 0) {
+        e->gstep += 1;
+        return 0;
+    }
+    EnsPool<Real> Q;
+    Q.x = (const Real*)e->a.x;
+    Q.y = (const Real*)e->a.y;
+    Q.xn = (Real*)e->a.xn;
+    Q.yn = (Real*)e->a.yn;
+    Q.sp = e->a.sp;
+    Q.fl = e->a.fl;
+    Q.id = e->a.id;
+    Q.start = e->cell_start;
+    Q.counts = e->counts;
+    Q.delta = e->delta;
+    Q.ns = p.n_species;
+    Q.n_grids = (int)e->grids.size();
+
+    ForceParams<Real> F;
+    F.sig = (const Real*)e->sig;
+    F.eps = (const Real*)e->eps;
+    F.kh = (const Real*)e->kh;
+    F.Dbase = (const Real*)e->Dbase;
+    F.ns = p.n_species;
+    F.wca = p.use_wca != 0;
+    F.harm = p.use_harmonic != 0;
+    F.dfield = p.dfield;
+    F.n_regions = 0;
+    F.rcx = F.rcy = F.rr = F.rf = nullptr;
+    F.smooth_w = (Real)p.smooth_w;
+    F.sin_amp = (Real)p.sin_amp;
+    F.sin_kx = F.sin_ky = (Real)0;
+    F.dt = (Real)p.dt;
+    F.max_disp = (Real)p.max_disp;
+    F.rint2 = (Real)e->rint2;
+    const dim3 tgrid(8 * ((ntiles + 7) / 8));
+    const uint32_t gstep = e->gstep;
+    hipLaunchKernelGGL((k_ens_force<Real>), tgrid, dim3(256), 0, st,
+                       (const EnsTile*)e->d_tiles, ntiles, (const EnsSeg*)e->d_seg, Q, F,
+                       e->sensor_mask_all, gstep);
+
+    int cat_skip = 0;
+    for (size_t ri = 0; ri < e->rx.size(); ++ri) {
+        const edm_reaction& r = e->rx[ri];
+        if (r.kind == EDM_RX_CAT) {
+            if (cat_skip > 0) {
+                --cat_skip;
+                continue;
+            }
+            const FbDev f1 = to_fb_model(r.fb1);
+            const bool defer = f1.sensor >= 0 && (f1.sensor == r.b || f1.sensor == r.c);
+            const bool plain = f1.sensor < 0 && !r.use_adapt;
+            int nf = 1;
+            while (plain && nf < kCatFuse && ri + nf < e->rx.size()) {
+                const edm_reaction& r2 = e->rx[ri + nf];
+                if (r2.kind != EDM_RX_CAT || r2.b != r.b || r2.c != r.c || r2.fb1.sensor >= 0 ||
+                    r2.use_adapt)
+                    break;
+                ++nf;
+            }
+            CatArgs<Real> A{};
+            A.n_enz = nf;
+            for (int j = 0; j < nf; ++j) {
+                const edm_reaction& rj = e->rx[ri + j];
+                A.enz[j] = rj.a;
+                A.contact2[j] = (Real)rj.contact * (Real)rj.contact;
+                A.prob[j] = (Real)0;  // per replica
+                A.rxi[j] = (uint32_t)(ri + j);
+                A.doff[j] = 0;
+            }
+            A.sub = r.b;
+            A.prod = r.c;
+            A.fb = f1;
+            A.w = nullptr;
+            A.wnx = A.wny = 1;
+            A.Kw = (Real)p.adapt_K;
+            A.nw = (Real)p.adapt_n;
+            A.dr = Draws{nullptr, 0, 0, 0};
+            A.deferred = defer;
+            hipLaunchKernelGGL((k_ens_cat<Real>), tgrid, dim3(256), 0, st,
+                               (const EnsTile*)e->d_tiles, ntiles, (const EnsSeg*)e->d_seg, Q, A,
+                               e->ctr, gstep);
+            if (defer && e->n_counts)
+                hipLaunchKernelGGL(k_fold, dim3(nblk(e->n_counts)), dim3(kBlock), 0, st,
+                                   (int)e->n_counts, e->counts, e->delta);
+            cat_skip = nf - 1;
+        } else if (r.kind == EDM_RX_RECRUIT || r.kind == EDM_RX_DETACH ||
+                   r.kind == EDM_RX_CONVERT) {
+            if (e->group_start[ri] < 0) continue;
+            const std::vector<int>& grp_ix = e->pass_groups[e->group_start[ri]];
+            PassGroupDev grp{};
+            grp.n = (int)grp_ix.size();
+            bool defer_any = false;
+            for (int k = 0; k < grp.n; ++k) {
+                const edm_reaction& q = e->rx[grp_ix[k]];
+                PassDev& d = grp.p[k];
+                d.kind = q.kind;
+                d.a = q.a;
+                d.b = q.b;
+                d.rxi = grp_ix[k];
+                d.prob = 0.0;  // per replica
+                d.f1 = to_fb_model(q.fb1);
+                d.f2 = to_fb_model(q.fb2);
+                d.draw_off = 0;
+                d.deferred = e->pass_deferred[grp_ix[k]];
+                defer_any = defer_any || d.deferred;
+                uint32_t reach = 1u << q.a;
+                for (int j = k - 1; j >= 0; --j) {
+                    const edm_reaction& qj = e->rx[grp_ix[j]];
+                    if (qj.kind == EDM_RX_CONVERT && ((reach >> qj.b) & 1u)) reach |= 1u << qj.a;
+                }
+                d.reach = reach;
+                grp.species |= reach;
+            }
+            hipLaunchKernelGGL((k_ens_passes<Real>), tgrid, dim3(256), 0, st,
+                               (const EnsTile*)e->d_tiles, ntiles, (const EnsSeg*)e->d_seg, Q,
+                               grp, e->ctr, e->cand_id, e->cand_slot, e->cand_x, e->cand_y,
+                               e->n_cand, gstep);
+            const edm_reaction& last = e->rx[grp_ix.back()];
+            if (last.kind == EDM_RX_RECRUIT)
+                hipLaunchKernelGGL((k_ens_recruit<Real>), dim3(e->rmax), dim3(1024), 0, st,
+                                   (const EnsSeg*)e->d_seg, Q, grp_ix.back(), last.a,
+                                   (const int32_t*)e->cand_id, (const int32_t*)e->cand_slot,
+                                   (const double*)e->cand_x, (const double*)e->cand_y, e->n_cand,
+                                   e->ctr);
+            if (defer_any && e->n_counts)
+                hipLaunchKernelGGL(k_fold, dim3(nblk(e->n_counts)), dim3(kBlock), 0, st,
+                                   (int)e->n_counts, e->counts, e->delta);
+        }
+    }
+    std::swap(e->a.x, e->a.xn);
+    std::swap(e->a.y, e->a.yn);
+    e->gstep += 1;
+    EDM_HIP(hipGetLastError());
+    return 0;
+}
+
+int ens_step(edm_ensemble* e, bool image) {
+    if (int rc = e->fp64 ? ens_step_t<double>(e) : ens_step_t<float>(e)) return rc;
+    if (image && e->has_pp) return ens_photons(e);
+    return 0;
+}

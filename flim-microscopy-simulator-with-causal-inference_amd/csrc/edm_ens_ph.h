@@ -1,0 +1,81 @@
+// Batched ensemble: photon emission + imaging and frame end (included at
+// the end of edm_photons.hip, inside namespace edm).  Each strip's own dye
+// carriers emit with their replica's draws, image geometry and TCSPC stack.
+
+template <typename Real>
+__global__ __launch_bounds__(256) void k_ens_photons(const EnsTile* __restrict__ tiles, int ntiles,
+                                                     const EnsSeg* __restrict__ segs,
+                                                     const edm_photo_params* pp, const Real* x,
+                                                     const Real* y, uint8_t* fl, const int8_t* dye,
+                                                     int8_t* ctmc, const int32_t* __restrict__ id,
+                                                     const uint32_t* __restrict__ start,
+                                                     unsigned long long* ctr, uint32_t gstep) {
+    const int tile = blockIdx.x;
+    if (tile >= ntiles) return;
+    const EnsTile T = tiles[tile];
+    const EnsSeg& g = segs[T.r];
+    const int rows = min(kTileRows, g.ncy - T.cy0);
+    const int c0 = g.coff + T.cx * g.ncy + T.cy0;
+    const int b = (int)start[c0], e = (int)start[c0 + rows];
+    const Rng rng{g.seed_lo ^ 0x5bd1e995u, g.seed_hi};
+    const uint32_t step = gstep - g.step0;
+    int emitted = 0, detected = 0;
+    for (int t0 = b; t0 < e; t0 += blockDim.x) {
+        const int t = t0 + (int)threadIdx.x;
+        if (t < e && (fl[t] & EDM_FLAG_DYE))
+            emit_image<Real>(pp, &g.ip, t, (uint32_t)(id[t] - g.kbase), x, y, fl, dye, ctmc,
+                             nullptr, nullptr, true, 0.0, -1, g.flim, rng, step, emitted,
+                             detected);
+    }
+    photon_count(emitted, detected, ctr + 8 * T.r);
+}
+
+int ens_photons(edm_ensemble* e) {
+    const int ntiles = (int)e->tiles.size();
+    if (ntiles == 0 || !e->has_pp) return 0;
+    if (e->fp64)
+        hipLaunchKernelGGL(k_ens_photons<double>, dim3(ntiles), dim3(256), 0, e->stream,
+                           (const EnsTile*)e->d_tiles, ntiles, (const EnsSeg*)e->d_seg,
+                           (const edm_photo_params*)e->d_pp, (const double*)e->a.x,
+                           (const double*)e->a.y, e->a.fl, (const int8_t*)e->a.dye, e->a.ctmc,
+                           (const int32_t*)e->a.id, (const uint32_t*)e->cell_start, e->ctr,
+                           e->gstep);
+    else
+        hipLaunchKernelGGL(k_ens_photons<float>, dim3(ntiles), dim3(256), 0, e->stream,
+                           (const EnsTile*)e->d_tiles, ntiles, (const EnsSeg*)e->d_seg,
+                           (const edm_photo_params*)e->d_pp, (const float*)e->a.x,
+                           (const float*)e->a.y, e->a.fl, (const int8_t*)e->a.dye, e->a.ctmc,
+                           (const int32_t*)e->a.id, (const uint32_t*)e->cell_start, e->ctr,
+                           e->gstep);
+    EDM_HIP(hipGetLastError());
+    return 0;
+}
+
+int ens_frame_noise(edm_ensemble* e, int slot, int64_t* n_dark, int64_t* n_ap) {
+    const EnsSeg& g = e->seg[slot];
+    const edm_image_params& ip = g.ip;
+    const int64_t npix = (int64_t)ip.n_ch * ip.nx * ip.ny;
+    Rng rng{g.seed_lo ^ 0x2545F491u, g.seed_hi};
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, e->stream, (int64_t)4,
+                       (uint32_t*)e->frame_out, 0u);
+    hipLaunchKernelGGL(k_frame_noise, dim3(nb(npix)), dim3(kB), 0, e->stream, npix, ip.n_bins,
+                       g.flim, ip.dark_lambda, ip.p_ap, rng, e->gstep - g.step0, e->frame_out);
+    EDM_HIP(hipGetLastError());
+    if (n_dark || n_ap) {
+        unsigned long long c[2];
+        EDM_HIP(hipMemcpyAsync(c, e->frame_out, 16, hipMemcpyDeviceToHost, e->stream));
+        EDM_HIP(hipStreamSynchronize(e->stream));
+        if (n_dark) *n_dark = (int64_t)c[0];
+        if (n_ap) *n_ap = (int64_t)c[1];
+    }
+    return 0;
+}
+
+int ens_frame_reset(edm_ensemble* e, int slot) {
+    const EnsSeg& g = e->seg[slot];
+    if (!g.flim) return 0;
+    hipLaunchKernelGGL(k_fill_u32, dim3(std::min(nb(g.flim_elems), 8192)), dim3(kB), 0, e->stream,
+                       g.flim_elems, g.flim, 0u);
+    EDM_HIP(hipGetLastError());
+    return 0;
+}

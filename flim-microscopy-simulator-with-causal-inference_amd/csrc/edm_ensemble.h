@@ -1,0 +1,119 @@
+// Internal declarations of the batched ensemble (many independent
+// membranes of one model stepped by the same launches).  Not part of the C
+// ABI; the entry points are the edm_ens_* functions of edm_hip.h.
+//
+// Layout.  All replicas share one particle pool and one cell-key space.
+// Replica r owns the cell keys [coff_r, coff_r + ncx_r * ncy_r); the per-step
+// counting sort by key therefore keeps every replica's particles contiguous
+// and cell-ordered, exactly as a single membrane's are.  Particles of a
+// removed replica get the sentinel key (after every live cell) and fall off
+// the end of the pool at the next sort.  Every per-replica quantity (box,
+// cell grid, density grids, reaction probabilities, seed, image geometry,
+// TCSPC stack, counters) lives in the replica's EnsSeg entry; every kernel
+// that works on strips of cells reads its replica from the strip table, so
+// a replica's results do not depend on which other replicas share the
+// launch.
+#pragma once
+
+#include "edm_internal.h"
+
+static constexpr int kEnsMaxGrids = 4;
+static constexpr int kEnsMaxRx = 16;
+static constexpr int kEnsMaxSpecies = 32;
+
+// One replica slot, as the kernels read it.
+struct EnsSeg {
+    int32_t active;  // 0: free slot
+    int32_t kbase;   // first particle index of the replica
+    int32_t n;       // particles (alive + dormant)
+    int32_t coff;    // first cell key
+    int32_t ncx, ncy;
+    int32_t px, py;
+    double Lx, Ly, csx, csy, cutoff, sin_kx, sin_ky;
+    uint32_t seed_lo, seed_hi;
+    uint32_t step0;  // ensemble step at admission: draws use step - step0
+    int32_t n_grids;
+    int32_t gdims[2 * kEnsMaxGrids];  // [g][nx, ny]
+    float gs_f[2 * kEnsMaxGrids];     // [g][sx, sy]
+    double gs_d[2 * kEnsMaxGrids];
+    float ginv_f[kEnsMaxGrids];       // 1 / cell area
+    double ginv_d[kEnsMaxGrids];
+    int32_t goff[kEnsMaxGrids * kEnsMaxSpecies];  // count offset of (grid, species) or -1
+    double prob[kEnsMaxRx];
+    int32_t maxr[kEnsMaxRx];
+    edm_image_params ip;  // the model's imaging constants with this replica's geometry
+    uint32_t* flim;       // this replica's TCSPC stack
+    int64_t flim_elems;
+};
+
+// Strip tile of one replica.
+struct EnsTile {
+    int32_t r, cx, cy0, pad;
+};
+
+struct edm_ensemble {
+    edm_params p{};  // the model: species, potentials, reactions, D field, dt
+    std::vector<edm_grid> grids;
+    std::vector<edm_reaction> rx;
+    bool fp64 = false;
+    size_t rs = 4;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    int64_t cap = 0;  // particle capacity of the pool
+    int rmax = 0;
+
+    edm::DevArrays a;  // pool (bond and slot_of unused)
+    uint16_t *rep = nullptr, *rep_t = nullptr;
+    uint32_t *key = nullptr, *rank = nullptr, *perm = nullptr;
+    uint32_t *cell_cnt = nullptr, *cell_start = nullptr;
+    int64_t cell_cap = 0;
+    void* cub_tmp = nullptr;
+    size_t cub_bytes = 0;
+    int32_t *counts = nullptr, *delta = nullptr;
+    int64_t count_cap = 0;
+    int32_t *cand_id = nullptr, *cand_slot = nullptr;
+    double *cand_x = nullptr, *cand_y = nullptr;
+    unsigned int* n_cand = nullptr;        // [rmax]
+    unsigned long long* ctr = nullptr;     // [rmax][8]
+    unsigned long long* frame_out = nullptr;  // [2] dark counts, afterpulses of one frame
+    void *sig = nullptr, *eps = nullptr, *kh = nullptr, *Dbase = nullptr;
+    double rint2 = 0.0;
+    double max_disp_model = 0.0;
+    edm_photo_params pp{};
+    bool has_pp = false;
+    double* d_pp = nullptr;
+    edm_image_params ip{};  // model imaging constants
+
+    std::vector<EnsSeg> seg;  // host mirror, [rmax]
+    EnsSeg* d_seg = nullptr;
+    std::vector<EnsTile> tiles;
+    EnsTile* d_tiles = nullptr;
+    int64_t tile_cap = 0;
+    int64_t n_total = 0;  // particles in the pool (live replicas first after a sort)
+    int64_t n_cells = 0;  // live cell keys; key n_cells is the sentinel
+    int64_t n_counts = 0;
+    uint32_t gstep = 0;
+    bool layout_dirty = true;
+    int64_t n_retired = 0;  // particles of removed replicas still in the pool
+    int64_t next_id = 0;    // particle indices handed out so far
+    std::vector<int64_t> fresh_beg;  // per slot: pool position before its first sort, or -1
+    std::vector<int> draining;       // removed slots whose particles are still pooled
+
+    // model-level reaction planning (as for one membrane)
+    uint32_t sensor_mask_all = 0;
+    std::vector<std::pair<int, int>> kept;  // (grid, species) with density counts
+    std::vector<std::vector<int>> pass_groups;
+    std::vector<int> group_start, pass_deferred;
+};
+
+namespace edm {
+int ens_create(edm_ensemble* e);
+int ens_layout(edm_ensemble* e);
+int ens_step(edm_ensemble* e, bool image);
+int ens_photons(edm_ensemble* e);
+int ens_frame_noise(edm_ensemble* e, int slot, int64_t* n_dark, int64_t* n_ap);
+int ens_frame_reset(edm_ensemble* e, int slot);
+void plan_passes(const std::vector<edm_reaction>& rx, uint32_t sensor_mask,
+                 std::vector<std::vector<int>>& groups, std::vector<int>& group_start,
+                 std::vector<int>& deferred);
+}  // namespace edm

@@ -15,6 +15,9 @@
 #define EDM_FLAG_DYE 2u
 #define EDM_FLAG_BLEACHED 4u
 
+// Cell rows per strip tile of the tiled dynamics kernels.
+static constexpr int kTileRows = 64;
+
 namespace edm {
 
 void set_error(const std::string& msg);

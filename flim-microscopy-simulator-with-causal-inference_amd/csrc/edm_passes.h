@@ -29,26 +29,24 @@ struct PassGroupDev {
     PassDev p[kMaxPasses];
 };
 
+// Applies a run of passes to the particle in slot t (species s0, known to
+// be in the run's species set).  Draw keys use the particle index minus
+// kbase; hook draws are indexed by the particle index.
 template <typename Real>
-__global__ void k_passes(int nt, int n, const int32_t* __restrict__ list,
-                         const int32_t* __restrict__ id, Post<Real> P, PassGroupDev grp,
-                         const double* __restrict__ draws, Rng rng, uint32_t step, Real Lx,
-                         Real Ly, unsigned long long* ctr, Grids<Real> G, int n_grids,
-                         int32_t* cid, int32_t* cslot, double* cxo, double* cyo,
-                         unsigned int* nc) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    int det_hits = 0, conv_hits = 0;
-    int t = -1;
-    int s0 = 0;
-    if (q < nt) {
-        t = list ? P.slot_of[list[q]] : q;
-        s0 = P.sp[t];
-    }
-    if (t >= 0 && ((grp.species >> s0) & 1u)) {
+__device__ __forceinline__ void passes_particle(int t, int s0, int n, const int32_t* __restrict__ id,
+                                                int kbase, const Post<Real>& P,
+                                                const PassGroupDev& grp,
+                                                const double* __restrict__ draws, Rng rng,
+                                                uint32_t step, Real Lx, Real Ly,
+                                                const Grids<Real>& G, int n_grids, int32_t* cid,
+                                                int32_t* cslot, double* cxo, double* cyo,
+                                                unsigned int* nc, int& det_hits, int& conv_hits) {
+    {
         uint8_t f = P.fl[t];
         const uint8_t f0 = f;
         int8_t s = (int8_t)s0;
         const int it = id[t];
+        const uint32_t key = (uint32_t)(it - kbase);
         const Real x = P.xn[t], y = P.yn[t];
         // Phase 1: every pass this particle can reach gets its uniform, its
         // position (the particle's, or a recruitment's drawn site) and its
@@ -75,9 +73,9 @@ __global__ void k_passes(int nt, int n, const int32_t* __restrict__ list,
                     ux = (Real)draws[d.draw_off + n + 2 * (int64_t)it];
                     uy = (Real)draws[d.draw_off + n + 2 * (int64_t)it + 1];
                 } else {
-                    const U4 r = rng.draw(step, (uint32_t)it, 0u, (kTagRecruitAcc << 16) | d.rxi);
+                    const U4 r = rng.draw(step, key, 0u, (kTagRecruitAcc << 16) | d.rxi);
                     u[k] = (double)Uni<Real>::first(r);
-                    const U4 w = rng.draw(step, (uint32_t)it, 1u, (kTagRecruitPos << 16) | d.rxi);
+                    const U4 w = rng.draw(step, key, 1u, (kTagRecruitPos << 16) | d.rxi);
                     ux = Uni<Real>::first(w);
                     uy = Uni<Real>::second(w);
                 }
@@ -88,7 +86,7 @@ __global__ void k_passes(int nt, int n, const int32_t* __restrict__ list,
                 const uint32_t tag = d.kind == EDM_RX_DETACH ? kTagDetach : kTagConvert;
                 u[k] = draws ? draws[d.draw_off + it]
                              : (double)Uni<Real>::first(
-                                   rng.draw(step, (uint32_t)it, 0u, (tag << 16) | d.rxi));
+                                   rng.draw(step, key, 0u, (tag << 16) | d.rxi));
             }
             if (d.f1.sensor >= 0) k1[k] = G.counts[d.f1.cbase + fb_cell(d.f1, px[k], py[k])];
         }
@@ -138,6 +136,10 @@ __global__ void k_passes(int nt, int n, const int32_t* __restrict__ list,
         if (f != f0) P.fl[t] = f;
         if (s != s0) P.sp[t] = s;
     }
+}
+
+template <typename Real>
+__device__ __forceinline__ void passes_count(int det_hits, int conv_hits, unsigned long long* ctr) {
     unsigned long long dh = det_hits, ch = conv_hits;
     for (int o = 32; o > 0; o >>= 1) {
         dh += __shfl_down(dh, o);
@@ -147,4 +149,23 @@ __global__ void k_passes(int nt, int n, const int32_t* __restrict__ list,
         if (dh) atomicAdd(&ctr[4], dh);
         if (ch) atomicAdd(&ctr[5], ch);
     }
+}
+
+template <typename Real>
+__global__ void k_passes(int nt, int n, const int32_t* __restrict__ list,
+                         const int32_t* __restrict__ id, Post<Real> P, PassGroupDev grp,
+                         const double* __restrict__ draws, Rng rng, uint32_t step, Real Lx,
+                         Real Ly, unsigned long long* ctr, Grids<Real> G, int n_grids,
+                         int32_t* cid, int32_t* cslot, double* cxo, double* cyo,
+                         unsigned int* nc) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    int det_hits = 0, conv_hits = 0;
+    if (q < nt) {
+        const int t = list ? P.slot_of[list[q]] : q;
+        const int s0 = P.sp[t];
+        if ((grp.species >> s0) & 1u)
+            passes_particle(t, s0, n, id, 0, P, grp, draws, rng, step, Lx, Ly, G, n_grids, cid,
+                            cslot, cxo, cyo, nc, det_hits, conv_hits);
+    }
+    passes_count<Real>(det_hits, conv_hits, ctr);
 }

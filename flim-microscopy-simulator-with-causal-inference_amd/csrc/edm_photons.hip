@@ -14,6 +14,7 @@
 
 #include "edm_device.h"
 #include "edm_internal.h"
+#include "edm_ensemble.h"
 
 namespace edm {
 
@@ -319,7 +320,7 @@ __device__ __forceinline__ void fret_inline(const edm_photo_params* pp, int t, c
         if (pp->fm_intra[j]) {
             kf = pp->fm_kfret[j];
             acc = -1;
-        } else {
+        } else if (slot_of) {
             const int s = slot_of[partner];
             if ((fl[s] & EDM_FLAG_DYE) && dye[s] == pp->fm_acceptor[j]) {
                 kf = pp->fm_kfret[j];
@@ -332,14 +333,13 @@ __device__ __forceinline__ void fret_inline(const edm_photo_params* pp, int t, c
 // Pixel draw from the emitter's (2R+1)^2 Gaussian window factorised as
 // exp(-dx^2 k) exp(-dy^2 k): the same discrete distribution as the
 // reference's joint CDF walk (di-major), evaluated with 2(2R+1) exps in
-// the working precision.  Falls back to psf_pick for very wide windows.
+// the working precision.  Windows wider than 31 pixels are rejected when
+// the imaging parameters are set.
 template <typename Real>
-__device__ __forceinline__ bool psf_pick_sep(const edm_image_params* ip, const double* pcx,
-                                             const double* pcy, Real x, Real y, int ch, Real u,
-                                             int& opx, int& opy) {
+__device__ __forceinline__ bool psf_pick_sep(const edm_image_params* ip, Real x, Real y, int ch,
+                                             Real u, int& opx, int& opy) {
     constexpr int kMax = 31;
     const int R = ip->R, nx = ip->nx, ny = ip->ny;
-    if (2 * R + 1 > kMax) return psf_pick(ip, pcx, pcy, (double)x, (double)y, ch, (double)u, opx, opy);
     if (ch < 0) ch = 0;
     if (ch >= ip->n_sigma) ch = 0;
     Real sigma = (Real)ip->sigma_ch[ch];
@@ -348,18 +348,20 @@ __device__ __forceinline__ bool psf_pick_sep(const edm_image_params* ip, const d
     const int i0 = (int)floor((x - (Real)ip->offx) / pix - (Real)0.5 + (Real)0.5);
     const int j0 = (int)floor((y - (Real)ip->offy) / pix - (Real)0.5 + (Real)0.5);
     if (i0 < -R || i0 >= nx + R || j0 < -R || j0 >= ny + R) return false;
+    if (2 * R + 1 > kMax) return false;  // rejected on the host
     const Real inv2 = (Real)0.5 / (sigma * sigma);
     Real wx[kMax], wy[kMax];
     Real Wx = 0, Wy = 0;
     for (int d = -R; d <= R; ++d) {
         const int ii = i0 + d, jj = j0 + d;
         Real a = 0, b = 0;
+        // pixel centres (i + 0.5) * pixel + offset, as the scanner computes them
         if (ii >= 0 && ii < nx) {
-            const Real dx = x - (Real)pcx[ii];
+            const Real dx = x - (Real)(((double)ii + 0.5) * ip->pixel + ip->offx);
             a = exp(-dx * dx * inv2);
         }
         if (jj >= 0 && jj < ny) {
-            const Real dy = y - (Real)pcy[jj];
+            const Real dy = y - (Real)(((double)jj + 0.5) * ip->pixel + ip->offy);
             b = exp(-dy * dy * inv2);
         }
         wx[d + R] = a;
@@ -392,66 +394,65 @@ __device__ __forceinline__ bool psf_pick_sep(const edm_image_params* ip, const d
     return true;
 }
 
-// Fused per-emitter photon path with Philox draws: FRET rate, PSS counts
-// and CTMC transition, then for every photon its arrival time, detection,
-// PSF pixel, IRF jitter and TCSPC bin.  Only dye carriers are launched.
+// One emitter of the fused photon path with Philox draws: FRET rate, PSS
+// counts and CTMC transition, then for every photon its arrival time,
+// detection, PSF pixel, IRF jitter and TCSPC bin.  `key` is the particle's
+// draw key; FRET comes from (kf_in, acc_in) when have_kf, else from the
+// bond maps (acceptor lookups need slot_of).
 template <typename Real>
-__global__ void k_pss_image(int n_dye, const int32_t* dye_ids, const edm_photo_params* pp,
-                            const edm_image_params* ip, const double* pcx, const double* pcy,
-                            const Real* x, const Real* y, uint8_t* fl, const int8_t* dye,
-                            int8_t* ctmc, const int32_t* bond, const double* k_fret,
-                            const int32_t* acc, const int32_t* slot_of, uint32_t* flim, Rng rng,
-                            uint32_t step, unsigned long long* ctr) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    int emitted = 0, detected = 0;
-    if (q < n_dye) {
-        const int it = dye_ids[q];
-        const int t = slot_of[it];
-        uint8_t f = fl[t];
-        int8_t c = ctmc[t];
-        double kf;
-        int aid;
-        if (k_fret) {
-            kf = k_fret[q];
-            aid = acc[q];
-        } else {
-            fret_inline(pp, t, fl, dye, bond, slot_of, kf, aid);
-        }
-        const U4 r0 = rng.draw(step, (uint32_t)it, 0u, kTagPhotonPois);
-        const double up = u01d(r0.x, r0.y), ut = u01d(r0.z, r0.w);
-        PssOut o{0, 0, 0.0, 0};
-        if (pss_eval(pp, f, c, dye[t], kf, aid, up, ut, o)) {
-            emitted = o.nd + o.nf;
-            const Real qe = (Real)ip->qe;
-            const Real sirf = (Real)ip->sigma_irf;
-            for (int k = 0; k < o.nd + o.nf; ++k) {
-                const bool donor = k < o.nd;
-                int es = t, ch = o.ty;
-                Real tau = (Real)o.tau;
-                if (!donor) {
-                    es = slot_of[aid];
-                    ch = dye[es];
-                    tau = (ch >= 0 && ch < pp->n_fluoro) ? (Real)pp->tau[ch] : (Real)o.tau;
-                }
-                const U4 ra = rng.draw(step, (uint32_t)it, (uint32_t)k, kTagArrival);
-                if (Uni<Real>::second(ra) >= qe) continue;  // detection efficiency
-                Real ua = Uni<Real>::first(ra);
-                if (ua < (Real)1e-30) ua = (Real)1e-30;
-                const Real ta = -tau * log(ua);
-                const U4 rp = rng.draw(step, (uint32_t)it, (uint32_t)k, kTagPixel);
-                int px_, py_;
-                if (!psf_pick_sep<Real>(ip, pcx, pcy, x[es], y[es], ch, Uni<Real>::first(rp), px_,
-                                        py_))
-                    continue;
-                Real z0, z1;
-                normal2<Real>(rng.draw(step, (uint32_t)it, (uint32_t)k, kTagJitter), z0, z1);
-                tcspc_add(ip, flim, ch, px_, py_, (double)(ta + sirf * z0));
-                ++detected;
-            }
-        }
-        fl[t] = f;
-        ctmc[t] = c;
+__device__ __forceinline__ void emit_image(const edm_photo_params* pp, const edm_image_params* ip,
+                                           int t, uint32_t key, const Real* x, const Real* y,
+                                           uint8_t* fl, const int8_t* dye, int8_t* ctmc,
+                                           const int32_t* bond, const int32_t* slot_of,
+                                           bool have_kf, double kf_in, int acc_in,
+                                           uint32_t* flim, Rng rng, uint32_t step, int& emitted,
+                                           int& detected) {
+    uint8_t f = fl[t];
+    int8_t c = ctmc[t];
+    double kf;
+    int aid;
+    if (have_kf) {
+        kf = kf_in;
+        aid = acc_in;
+    } else {
+        fret_inline(pp, t, fl, dye, bond, slot_of, kf, aid);
     }
+    const U4 r0 = rng.draw(step, key, 0u, kTagPhotonPois);
+    const double up = u01d(r0.x, r0.y), ut = u01d(r0.z, r0.w);
+    PssOut o{0, 0, 0.0, 0};
+    if (pss_eval(pp, f, c, dye[t], kf, aid, up, ut, o)) {
+        emitted += o.nd + o.nf;
+        const Real qe = (Real)ip->qe;
+        const Real sirf = (Real)ip->sigma_irf;
+        for (int k = 0; k < o.nd + o.nf; ++k) {
+            const bool donor = k < o.nd;
+            int es = t, ch = o.ty;
+            Real tau = (Real)o.tau;
+            if (!donor) {
+                es = slot_of[aid];
+                ch = dye[es];
+                tau = (ch >= 0 && ch < pp->n_fluoro) ? (Real)pp->tau[ch] : (Real)o.tau;
+            }
+            const U4 ra = rng.draw(step, key, (uint32_t)k, kTagArrival);
+            if (Uni<Real>::second(ra) >= qe) continue;  // detection efficiency
+            Real ua = Uni<Real>::first(ra);
+            if (ua < (Real)1e-30) ua = (Real)1e-30;
+            const Real ta = -tau * log(ua);
+            const U4 rp = rng.draw(step, key, (uint32_t)k, kTagPixel);
+            int px_, py_;
+            if (!psf_pick_sep<Real>(ip, x[es], y[es], ch, Uni<Real>::first(rp), px_, py_))
+                continue;
+            Real z0, z1;
+            normal2<Real>(rng.draw(step, key, (uint32_t)k, kTagJitter), z0, z1);
+            tcspc_add(ip, flim, ch, px_, py_, (double)(ta + sirf * z0));
+            ++detected;
+        }
+    }
+    fl[t] = f;
+    ctmc[t] = c;
+}
+
+__device__ __forceinline__ void photon_count(int emitted, int detected, unsigned long long* ctr) {
     unsigned long long e = emitted, d = detected;
     for (int o = 32; o > 0; o >>= 1) {
         e += __shfl_down(e, o);
@@ -461,6 +462,24 @@ __global__ void k_pss_image(int n_dye, const int32_t* dye_ids, const edm_photo_p
         if (e) atomicAdd(&ctr[6], e);
         if (d) atomicAdd(&ctr[7], d);
     }
+}
+
+// Single membrane: one thread per dye carrier.
+template <typename Real>
+__global__ void k_pss_image(int n_dye, const int32_t* dye_ids, const edm_photo_params* pp,
+                            const edm_image_params* ip, const Real* x, const Real* y, uint8_t* fl,
+                            const int8_t* dye, int8_t* ctmc, const int32_t* bond,
+                            const double* k_fret, const int32_t* acc, const int32_t* slot_of,
+                            uint32_t* flim, Rng rng, uint32_t step, unsigned long long* ctr) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    int emitted = 0, detected = 0;
+    if (q < n_dye) {
+        const int it = dye_ids[q];
+        emit_image<Real>(pp, ip, slot_of[it], (uint32_t)it, x, y, fl, dye, ctmc, bond, slot_of,
+                         k_fret != nullptr, k_fret ? k_fret[q] : 0.0, acc ? acc[q] : -1, flim,
+                         rng, step, emitted, detected);
+    }
+    photon_count(emitted, detected, ctr);
 }
 
 // ----------------------------------------------------------------------
@@ -779,6 +798,10 @@ static const edm_image_params* dev_ip(edm_membrane* m) {
 
 int ph_fused_image(edm_membrane* m, int64_t* n_emitted, int64_t* n_detected) {
     if (!m->has_ip) return EDM_ESTATE;
+    if (2 * m->ip.R + 1 > 31) {
+        set_error("edm_photons_image: PSF window wider than 31 pixels (use the hook path)");
+        return EDM_EINVAL;
+    }
     hipStream_t st = m->stream;
     if (m->n_dye == 0) {
         if (n_emitted) *n_emitted = 0;
@@ -801,15 +824,15 @@ int ph_fused_image(edm_membrane* m, int64_t* n_emitted, int64_t* n_detected) {
     const edm_image_params* d_ip = dev_ip(m);
     if (m->fp64)
         hipLaunchKernelGGL(k_pss_image<double>, dim3(nb(m->n_dye)), dim3(kB), 0, st, m->n_dye,
-                           (const int32_t*)m->dye_ids, dpp, d_ip, (const double*)m->pcx,
-                           (const double*)m->pcy, (const double*)m->a.x, (const double*)m->a.y,
+                           (const int32_t*)m->dye_ids, dpp, d_ip, (const double*)m->a.x,
+                           (const double*)m->a.y,
                            m->a.fl, (const int8_t*)m->a.dye, m->a.ctmc,
                            (const int32_t*)m->a.bond, kf, ac, (const int32_t*)m->a.slot_of,
                            m->flim, rng, step, m->d_ctr);
     else
         hipLaunchKernelGGL(k_pss_image<float>, dim3(nb(m->n_dye)), dim3(kB), 0, st, m->n_dye,
-                           (const int32_t*)m->dye_ids, dpp, d_ip, (const double*)m->pcx,
-                           (const double*)m->pcy, (const float*)m->a.x, (const float*)m->a.y,
+                           (const int32_t*)m->dye_ids, dpp, d_ip, (const float*)m->a.x,
+                           (const float*)m->a.y,
                            m->a.fl, (const int8_t*)m->a.dye, m->a.ctmc,
                            (const int32_t*)m->a.bond, kf, ac, (const int32_t*)m->a.slot_of,
                            m->flim, rng, step, m->d_ctr);
@@ -987,5 +1010,7 @@ int frame_reset(edm_membrane* m) {
     EDM_HIP(hipGetLastError());
     return 0;
 }
+
+#include "edm_ens_ph.h"
 
 }  // namespace edm

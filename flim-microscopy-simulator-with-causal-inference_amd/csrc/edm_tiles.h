@@ -13,7 +13,6 @@
 // global-memory neighbour walks instead.
 #pragma once
 
-static constexpr int kTileRows = 64;
 static constexpr int kStripCells = 3 * (kTileRows + 2);
 
 struct StripGeo {
@@ -99,14 +98,14 @@ __device__ __forceinline__ int strip_row(const Cells<Real>& C, const StripGeo& S
     return trunc_cell(y, C.csy, C.ncy) - S.cy0 + 1;
 }
 
-// Forces + Ermak-McCammon move over the strip (any pair summation order).
+// Forces + Ermak-McCammon move over one strip (any pair summation order).
+// Every thread of the 256-wide block must call it.
 template <typename Real>
-__global__ __launch_bounds__(256) void k_force_tile(Cells<Real> C, ForceParams<Real> F,
-                                                    Real* __restrict__ xn, Real* __restrict__ yn,
-                                                    const double* __restrict__ noise_hook,
-                                                    Rng rng, uint32_t step, Grids<Real> G,
-                                                    int n_grids, uint32_t keep_mask, int ntile_y,
-                                                    int ntiles) {
+__device__ __forceinline__ void force_strip(const Cells<Real>& C, const ForceParams<Real>& F,
+                                            Real* __restrict__ xn, Real* __restrict__ yn,
+                                            const double* __restrict__ noise_hook, Rng rng,
+                                            uint32_t step, const Grids<Real>& G, int n_grids,
+                                            uint32_t keep_mask, const StripGeo& S) {
     constexpr int kCap = 1024;
     __shared__ Real s_x[kCap], s_y[kCap];
     __shared__ int8_t s_sp[kCap];
@@ -115,9 +114,6 @@ __global__ __launch_bounds__(256) void k_force_tile(Cells<Real> C, ForceParams<R
     __shared__ int s_gst[kStripCells];
     __shared__ int s_wsum[4];
 
-    const int tile = xcd_tile(ntiles);
-    if (tile >= ntiles) return;  // uniform across the block
-    const StripGeo S = strip_geo(C, tile, ntile_y);
     const int total = strip_cells(C, S, s_gst, s_off, s_wsum);
     const bool staged = total <= kCap;
     if (staged) {
@@ -230,7 +226,8 @@ __global__ __launch_bounds__(256) void k_force_tile(Cells<Real> C, ForceParams<R
                         }
                     }
                 }
-                move_particle(C, F, si, C.id[t], xt, yt, Fx, Fy, noise_hook, rng, step, nxp, nyp);
+                move_particle(C, F, si, C.id[t] - C.kbase, xt, yt, Fx, Fy, noise_hook, rng, step,
+                              nxp, nyp);
                 take_any = ((keep_mask >> si) & 1u) != 0;
             }
             xn[t] = nxp;
@@ -238,4 +235,17 @@ __global__ __launch_bounds__(256) void k_force_tile(Cells<Real> C, ForceParams<R
         }
         add_density(G, n_grids, take_any, si, nxp, nyp);
     }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(256) void k_force_tile(Cells<Real> C, ForceParams<Real> F,
+                                                    Real* __restrict__ xn, Real* __restrict__ yn,
+                                                    const double* __restrict__ noise_hook,
+                                                    Rng rng, uint32_t step, Grids<Real> G,
+                                                    int n_grids, uint32_t keep_mask, int ntile_y,
+                                                    int ntiles) {
+    const int tile = xcd_tile(ntiles);
+    if (tile >= ntiles) return;  // uniform across the block
+    force_strip(C, F, xn, yn, noise_hook, rng, step, G, n_grids, keep_mask,
+                strip_geo(C, tile, ntile_y));
 }

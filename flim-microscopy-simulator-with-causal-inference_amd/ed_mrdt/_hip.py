@@ -90,6 +90,16 @@ class ImageParams(ctypes.Structure):
                 ("sigma_ch", D8)]
 
 
+class Replica(ctypes.Structure):
+    _fields_ = [("n", c_int32), ("ncx", c_int32), ("ncy", c_int32), ("n_grids", c_int32),
+                ("Lx", c_double), ("Ly", c_double), ("csx", c_double), ("csy", c_double),
+                ("cutoff", c_double), ("sin_kx", c_double), ("sin_ky", c_double),
+                ("seed", ctypes.c_uint64), ("grids", Grid * 4), ("prob", c_double * 16),
+                ("max_recruits", c_int32 * 16), ("img_nx", c_int32), ("img_ny", c_int32),
+                ("img_R", c_int32), ("n_reactions", c_int32), ("img_pixel", c_double),
+                ("img_offx", c_double), ("img_offy", c_double)]
+
+
 # function name -> (argtypes) ; every function returns int32 status
 _SIGS = {
     "edm_device_count": [ctypes.POINTER(ctypes.c_int)],
@@ -121,6 +131,19 @@ _SIGS = {
     "edm_synchronize": [c_void_p],
     "edm_time_steps": [c_void_p, c_int32, c_int32, ctypes.POINTER(c_double),
                        ctypes.POINTER(c_double), c_void_p],
+    "edm_ens_create": [ctypes.POINTER(Params), ctypes.POINTER(PhotoParams),
+                       ctypes.POINTER(ImageParams), c_int64, c_int32, ctypes.POINTER(c_void_p)],
+    "edm_ens_destroy": [c_void_p],
+    "edm_ens_add": [c_void_p, ctypes.POINTER(Replica), ctypes.POINTER(HostState),
+                    ctypes.POINTER(c_int32)],
+    "edm_ens_remove": [c_void_p, c_int32],
+    "edm_ens_run": [c_void_p, c_int32, c_int32],
+    "edm_ens_frame": [c_void_p, c_int32, c_void_p, c_int64, ctypes.POINTER(c_int64),
+                      ctypes.POINTER(c_int64)],
+    "edm_ens_download": [c_void_p, c_int32, ctypes.POINTER(HostState)],
+    "edm_ens_counters": [c_void_p, c_int32, c_void_p],
+    "edm_ens_info": [c_void_p, c_void_p],
+    "edm_ens_synchronize": [c_void_p],
 }
 
 EXPORTS = tuple(_SIGS) + ("edm_last_error",)

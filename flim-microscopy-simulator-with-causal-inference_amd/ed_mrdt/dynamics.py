@@ -213,6 +213,74 @@ class _Plan:
                prob=min(1.0, cv.rate * dt), fb1=feedback(cv.feedback_rule), label="convert")
 
 
+class BuiltParams:
+    """An edm_params block with the host arrays its pointers refer to."""
+
+    def __init__(self, params, keep):
+        self.params = params
+        self.keep = keep
+
+
+def build_params(config: SimulationConfig, c: Dict, plan: "_Plan", n: int,
+                 precision: str = "fp64", pair_cap: bool = True,
+                 exact_force_order: bool = True) -> BuiltParams:
+    """The C parameter block of one membrane (edm_params): geometry, the
+    species tables, the D field, the reaction plan and the seed."""
+    df = config.diffusion_field
+    circles = [r for r in df.regions if r.get("shape") == "circle"]
+    keep: List = []
+
+    def arr(v):
+        a = np.ascontiguousarray(np.asarray(v, dtype=np.float64))
+        keep.append(a)
+        return a.ctypes.data
+
+    Lx, Ly = c["Lx"], c["Ly"]
+    grids = (_hip.Grid * max(1, len(plan.grids)))()
+    for i, (gx, gy, sx, sy) in enumerate(plan.grids):
+        grids[i] = _hip.Grid(nx=gx, ny=gy, sx=sx, sy=sy)
+    rxs = (_hip.Reaction * max(1, len(plan.reactions)))(*plan.reactions)
+    keep += [grids, rxs]
+    p = _hip.Params()
+    p.n = n
+    p.n_species = len(config.species)
+    p.fp64 = 1 if precision == "fp64" else 0
+    periodic_x, periodic_y = config.membrane.periodic
+    p.periodic_x = 1 if periodic_x else 0
+    p.periodic_y = 1 if periodic_y else 0
+    p.ncx, p.ncy = c["nx"], c["ny"]
+    p.use_wca = 1 if c["use_wca"] else 0
+    p.use_harmonic = 1 if c["use_harmonic"] else 0
+    p.dfield = 1 if df.type == "sinusoidal" else 0
+    p.n_regions = 0 if p.dfield else len(circles)
+    p.n_grids = len(plan.grids)
+    p.n_reactions = len(plan.reactions)
+    p.pair_cap = min(n * 30, 500000) if pair_cap else 0
+    p.exact_force_order = 1 if exact_force_order else 0
+    p.Lx, p.Ly, p.dt = Lx, Ly, c["dt"]
+    p.cutoff, p.csx, p.csy = c["cutoff"], c["csx"], c["csy"]
+    p.max_disp = c["max_disp"]
+    p.smooth_w = config.region_smoothing_width
+    if p.dfield:
+        lx, ly = df.wavelength
+        p.sin_amp = float(df.amplitude)
+        p.sin_kx = 2.0 * math.pi / (lx if lx > 0 else Lx)
+        p.sin_ky = 2.0 * math.pi / (ly if ly > 0 else Ly)
+    tau_w, K_w, n_w = plan.adapt
+    p.adapt_tau, p.adapt_K, p.adapt_n = tau_w, K_w, n_w
+    p.adapt_grid, p.adapt_species = plan.adapt_grid, plan.adapt_species
+    p.seed = int(config.random_seed) & 0xFFFFFFFFFFFFFFFF
+    p.sigma, p.epsilon, p.k_harm = arr(c["sigma"]), arr(c["epsilon"]), arr(c["k_harm"])
+    p.D_base = arr([s.diffusion_coefficient for s in config.species])
+    p.rcx = arr([r["center"][0] for r in circles] or [0.0])
+    p.rcy = arr([r["center"][1] for r in circles] or [0.0])
+    p.rr = arr([r["radius"] for r in circles] or [0.0])
+    p.rf = arr([r.get("D_factor", 1.0) for r in circles] or [0.0])
+    p.grids = ctypes.cast(grids, ctypes.c_void_p)
+    p.reactions = ctypes.cast(rxs, ctypes.c_void_p)
+    return BuiltParams(p, keep)
+
+
 class DynamicsEngine:
     """GPU DynamicsEngine with the reference's interface.
 
@@ -251,55 +319,9 @@ class DynamicsEngine:
         self._use_wca, self._use_harmonic = c["use_wca"], c["use_harmonic"]
         self.plan = _Plan(config, c)
         N = state.N
-        df = config.diffusion_field
-        circles = [r for r in df.regions if r.get("shape") == "circle"]
-        self._keep = []  # arrays whose pointers the library reads during create
-
-        def arr(v):
-            a = np.ascontiguousarray(np.asarray(v, dtype=np.float64))
-            self._keep.append(a)
-            return a.ctypes.data
-
-        grids = (_hip.Grid * max(1, len(self.plan.grids)))()
-        for i, (gx, gy, sx, sy) in enumerate(self.plan.grids):
-            grids[i] = _hip.Grid(nx=gx, ny=gy, sx=sx, sy=sy)
-        rxs = (_hip.Reaction * max(1, len(self.plan.reactions)))(*self.plan.reactions)
-        p = _hip.Params()
-        p.n = N
-        p.n_species = len(config.species)
-        p.fp64 = 1 if precision == "fp64" else 0
-        p.periodic_x = 1 if self.periodic_x else 0
-        p.periodic_y = 1 if self.periodic_y else 0
-        p.ncx, p.ncy = c["nx"], c["ny"]
-        p.use_wca = 1 if c["use_wca"] else 0
-        p.use_harmonic = 1 if c["use_harmonic"] else 0
-        p.dfield = 1 if df.type == "sinusoidal" else 0
-        p.n_regions = 0 if p.dfield else len(circles)
-        p.n_grids = len(self.plan.grids)
-        p.n_reactions = len(self.plan.reactions)
-        p.pair_cap = min(N * 30, 500000) if pair_cap else 0
-        p.exact_force_order = 1 if exact_force_order else 0
-        p.Lx, p.Ly, p.dt = self.Lx, self.Ly, self.dt
-        p.cutoff, p.csx, p.csy = c["cutoff"], c["csx"], c["csy"]
-        p.max_disp = c["max_disp"]
-        p.smooth_w = config.region_smoothing_width
-        if p.dfield:
-            lx, ly = df.wavelength
-            p.sin_amp = float(df.amplitude)
-            p.sin_kx = 2.0 * math.pi / (lx if lx > 0 else self.Lx)
-            p.sin_ky = 2.0 * math.pi / (ly if ly > 0 else self.Ly)
-        tau_w, K_w, n_w = self.plan.adapt
-        p.adapt_tau, p.adapt_K, p.adapt_n = tau_w, K_w, n_w
-        p.adapt_grid, p.adapt_species = self.plan.adapt_grid, self.plan.adapt_species
-        p.seed = int(config.random_seed) & 0xFFFFFFFFFFFFFFFF
-        p.sigma, p.epsilon, p.k_harm = arr(c["sigma"]), arr(c["epsilon"]), arr(c["k_harm"])
-        p.D_base = arr([s.diffusion_coefficient for s in config.species])
-        p.rcx = arr([r["center"][0] for r in circles] or [0.0])
-        p.rcy = arr([r["center"][1] for r in circles] or [0.0])
-        p.rr = arr([r["radius"] for r in circles] or [0.0])
-        p.rf = arr([r.get("D_factor", 1.0) for r in circles] or [0.0])
-        p.grids = ctypes.cast(grids, ctypes.c_void_p)
-        p.reactions = ctypes.cast(rxs, ctypes.c_void_p)
+        built = build_params(config, c, self.plan, N, precision, pair_cap, exact_force_order)
+        p = built.params
+        self._keep = built.keep
         h = ctypes.c_void_p()
         check(lib().edm_create(ctypes.byref(p), ctypes.byref(h)), "edm_create")
         self._h = h

@@ -136,6 +136,31 @@ class FLIMFrame:
         return self.flim_stack.shape
 
 
+def image_params(config: SimulationConfig, psf: "PSF" = None, scanner: "ConfocalScanner" = None,
+                 tcspc: "TCSPCAccumulator" = None,
+                 detector: "DetectorModel" = None) -> _hip.ImageParams:
+    """edm_image_params of a configuration (microscope.py:73-391, 650-838)."""
+    psf = psf or PSF(config)
+    sc = scanner or ConfocalScanner(config, psf)
+    tc = tcspc or TCSPCAccumulator(config)
+    de = detector or DetectorModel(config)
+    ip = _hip.ImageParams()
+    ip.nx, ip.ny, ip.n_bins, ip.n_ch = sc.nx, sc.ny, tc.n_bins, tc.n_channels
+    ip.R = sc.psf_range_px
+    n_sig = max((max(psf.sigma_um.keys()) + 1) if psf.sigma_um else 1, 2)
+    if n_sig > 8:
+        raise ConfigError("the GPU path supports at most 8 channels")
+    ip.n_sigma = n_sig
+    for ch, s in psf.sigma_um.items():
+        ip.sigma_ch[ch] = s
+    ip.pixel, ip.offx, ip.offy = sc.pixel_size_um, sc.offset_x, sc.offset_y
+    ip.bin_w, ip.sigma_irf = tc.bin_width, tc.sigma_irf
+    ip.qe = de.det_efficiency
+    ip.dark_lambda = de.dark_count_rate * de.dwell_time * 1
+    ip.p_ap = de.afterpulsing_prob
+    return ip
+
+
 class VirtualMicroscope:
     """GPU VirtualMicroscope with the reference's interface (microscope.py:1021)."""
 
@@ -151,21 +176,7 @@ class VirtualMicroscope:
             raise RuntimeError("VirtualMicroscope needs the state on the GPU: build the "
                                "DynamicsEngine first (as SimulationPipeline does)")
         self._dev = dev
-        sc, tc, de = self.scanner, self.tcspc, self.detector
-        ip = _hip.ImageParams()
-        ip.nx, ip.ny, ip.n_bins, ip.n_ch = sc.nx, sc.ny, tc.n_bins, tc.n_channels
-        ip.R = sc.psf_range_px
-        n_sig = max((max(self.psf.sigma_um.keys()) + 1) if self.psf.sigma_um else 1, 2)
-        if n_sig > 8:
-            raise ConfigError("the GPU path supports at most 8 channels")
-        ip.n_sigma = n_sig
-        for ch, s in self.psf.sigma_um.items():
-            ip.sigma_ch[ch] = s
-        ip.pixel, ip.offx, ip.offy = sc.pixel_size_um, sc.offset_x, sc.offset_y
-        ip.bin_w, ip.sigma_irf = tc.bin_width, tc.sigma_irf
-        ip.qe = de.det_efficiency
-        ip.dark_lambda = de.dark_count_rate * de.dwell_time * 1
-        ip.p_ap = de.afterpulsing_prob
+        ip = image_params(config, self.psf, self.scanner, self.tcspc, self.detector)
         self._ip = ip
         check(lib().edm_set_imaging(dev.handle, ctypes.byref(ip)), "edm_set_imaging")
         self._rng = np.random.default_rng(config.random_seed + 2000)

@@ -192,25 +192,46 @@ def build_pss_matrices(rm: RateMatrix, p_exc: float, T_laser: float, n_pulses: i
             "tau_eff": 1.0 / kS1 if kS1 > 0 else 4.1e-9}
 
 
+def photo_params(config: SimulationConfig, scan_duty_fraction: float = 1.0,
+                 n_labeled: int = 0) -> Optional[_hip.PhotoParams]:
+    """edm_photo_params of a configuration, computed on the host (None when
+    the configuration has no fluorophores)."""
+    sim = PhotonSimulator.__new__(PhotonSimulator)
+    sim._init_tables(config, scan_duty_fraction)
+    return sim._build_pp(n_labeled) if sim._active else None
+
+
 class PhotonSimulator:
     """GPU PhotonSimulator with the reference's interface (photophysics.py:690)."""
 
     def __init__(self, config: SimulationConfig, state: ParticleState, scan_duty_fraction: float = 1.0):
-        self.config = config
         self.state = state
-        self.rate_matrices = [RateMatrix(f) for f in config.fluorophores]
-        self.n_fluoro_types = len(self.rate_matrices)
-        if self.n_fluoro_types == 0:
-            self._active = False
+        self._init_tables(config, scan_duty_fraction)
+        if not self._active:
             return
-        self._active = True
-        if self.n_fluoro_types > 8 or len(config.fret_pairs) > 8:
-            raise ConfigError("the GPU path supports at most 8 fluorophores and 8 FRET pairs")
         dev = state._device
         if dev is None:
             raise RuntimeError("PhotonSimulator needs the state on the GPU: build the "
                                "DynamicsEngine first (as SimulationPipeline does)")
         self._dev = dev
+        self._upload(int(np.sum(state._a["has_dye"])))
+        self._rng = np.random.default_rng(config.random_seed + 1000)
+        self.total_donor_photons = 0
+        self.total_acceptor_photons = 0
+        self.total_fret_events = 0
+        self.total_bleached = 0
+
+    def _init_tables(self, config: SimulationConfig, scan_duty_fraction: float) -> None:
+        """Host-side tables (photophysics.py:690-887): rate matrices,
+        excitation probabilities, FRET maps and the PSS matrices."""
+        self.config = config
+        self.rate_matrices = [RateMatrix(f) for f in config.fluorophores]
+        self.n_fluoro_types = len(self.rate_matrices)
+        self._active = self.n_fluoro_types > 0
+        if not self._active:
+            return
+        if self.n_fluoro_types > 8 or len(config.fret_pairs) > 8:
+            raise ConfigError("the GPU path supports at most 8 fluorophores and 8 FRET pairs")
         nf = self.n_fluoro_types
         self.scan_duty_fraction = max(0.0, min(1.0, scan_duty_fraction))
         self.laser_period = config.laser.period
@@ -230,15 +251,6 @@ class PhotonSimulator:
             self.donor_channel = 0
             self.acceptor_channel = 1 if nf > 1 else 0
         self._precompute_pss_matrices()
-        n_lab = int(np.sum(state._a["has_dye"]))
-        mm = max(float(self._mean_ph_all[t, 0]) for t in range(nf))
-        self._max_photons = max(1000, int(n_lab * mm * 3))
-        self._upload()
-        self._rng = np.random.default_rng(config.random_seed + 1000)
-        self.total_donor_photons = 0
-        self.total_acceptor_photons = 0
-        self.total_fret_events = 0
-        self.total_bleached = 0
 
     # ---- host precompute (photophysics.py:796-991) ------------------------
     def _compute_excitation_prob(self, fluo: FluorophoreType) -> float:
@@ -303,7 +315,16 @@ class PhotonSimulator:
             self._T_step_fret_all[t], self._mean_ph_fret_all[t] = b["T_step"], b["mean_ph"]
             self._mean_fret_fret_all[t], self._tau_fret_all[t] = b["mean_fret"], b["tau_eff"]
 
-    def _upload(self) -> None:
+    def _upload(self, n_labeled: int) -> None:
+        pp = self._build_pp(n_labeled)
+        self._pp = pp
+        check(lib().edm_set_photophysics(self._dev.handle, ctypes.byref(pp)), "edm_set_photophysics")
+
+    def _build_pp(self, n_labeled: int) -> _hip.PhotoParams:
+        """The edm_photo_params block (photon-buffer bound from the number of
+        labelled particles, photophysics.py:879-887)."""
+        mm = max(float(self._mean_ph_all[t, 0]) for t in range(self.n_fluoro_types))
+        self._max_photons = max(1000, int(n_labeled * mm * 3))
         pp = _hip.PhotoParams()
         nf = self.n_fluoro_types
         pp.n_fluoro = nf
@@ -350,8 +371,7 @@ class PhotonSimulator:
                     pp.fm_kfret[j] = q * q * q * q * q * q / tD
             pp.fm_skip[j] = skip
         pp.max_photons = int(self._max_photons)
-        self._pp = pp
-        check(lib().edm_set_photophysics(self._dev.handle, ctypes.byref(pp)), "edm_set_photophysics")
+        return pp
 
     # ---- per step (photophysics.py:1021-1197) --------------------------------
     def update_fret_rates(self) -> None:

@@ -224,6 +224,56 @@ int edm_synchronize(edm_membrane *m);
 int edm_time_steps(edm_membrane *m, int32_t n_steps, int32_t image, double *total_ms,
                    double *force_ms, double *breakdown_ms /* [8] or NULL */);
 
+/* ---- Batched ensemble ---------------------------------------------------
+ * Many independent membranes of one model advanced by the same launches
+ * (the reference runs replicas one after another: one SimulationPipeline
+ * per replica, pipeline.py:489-738).  The model is an edm_params (species,
+ * potentials, reaction network with feedback rules, D-field type, dt,
+ * displacement cap; n, box and cell fields ignored) plus the photophysics
+ * tables and imaging constants; each replica brings its own box, particle
+ * state, seed, reaction probabilities, recruitment caps, density-grid
+ * shapes and image geometry.  Philox draws only (keys relative to each
+ * replica's first particle and admission step), so a replica evolves
+ * exactly as it does alone.  Not batched: binding, circular D(x) regions,
+ * the adaptation field, proximity or intermolecular FRET.               */
+typedef struct edm_replica {
+    int32_t n;                  /* particles (alive + dormant)              */
+    int32_t ncx, ncy;           /* cell grid (CellList, dynamics.py:109-114) */
+    int32_t n_grids;            /* must equal the model's                   */
+    double Lx, Ly, csx, csy, cutoff;
+    double sin_kx, sin_ky;      /* sinusoidal D(x) wave numbers             */
+    uint64_t seed;
+    edm_grid grids[4];          /* this replica's density grids             */
+    double prob[16];            /* per model reaction, in model order       */
+    int32_t max_recruits[16];
+    int32_t img_nx, img_ny, img_R, n_reactions;
+    double img_pixel, img_offx, img_offy;
+} edm_replica;
+
+typedef struct edm_ensemble edm_ensemble;
+
+/* capacity: particles in the shared pool; max_replicas: resident replicas */
+int edm_ens_create(const edm_params *model, const edm_photo_params *pp,
+                   const edm_image_params *ip, int64_t capacity, int32_t max_replicas,
+                   edm_ensemble **out);
+int edm_ens_destroy(edm_ensemble *e);
+/* admit a replica; its state is in particle-index order; *slot receives its id */
+int edm_ens_add(edm_ensemble *e, const edm_replica *r, const edm_host_state *s, int32_t *slot);
+int edm_ens_remove(edm_ensemble *e, int32_t slot);
+/* n_steps steps of every resident replica (+ photons and TCSPC if image) */
+int edm_ens_run(edm_ensemble *e, int32_t n_steps, int32_t image);
+/* frame end of one replica: dark counts and afterpulses, copy (flim may be
+ * NULL) and reset of its TCSPC stack */
+int edm_ens_frame(edm_ensemble *e, int32_t slot, uint32_t *flim, int64_t n_elems,
+                  int64_t *n_dark, int64_t *n_afterpulse);
+int edm_ens_download(edm_ensemble *e, int32_t slot, edm_host_state *s);
+/* the replica's counters: bindings, dissociations, catalytic, recruited,
+ * detached, converted, photons emitted, photons detected */
+int edm_ens_counters(edm_ensemble *e, int32_t slot, int64_t out[8]);
+/* pooled particles, live cells, resident replicas, particle capacity */
+int edm_ens_info(edm_ensemble *e, int64_t out[4]);
+int edm_ens_synchronize(edm_ensemble *e);
+
 #ifdef __cplusplus
 }
 #endif
