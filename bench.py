@@ -128,6 +128,91 @@ def gpu_run(steps: int, warmup: int, n_alive: int):
                 steps=steps, emitted=int(ph1[0] - ph0[0]), detected=int(ph1[1] - ph0[1]))
 
 
+def _progress(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def ensemble_bench(n_rep: int, steps: int, plain_every: int):
+    """Batched ensemble of the 512-replica workload (box sides U[5,30] um,
+    rates jittered, seeds 0..511, 64 x 64 px FLIM every 50 steps) against the
+    plain path (one replica per launch sequence) on every plain_every-th
+    replica.  Both run every step with imaging, end every frame on the
+    device and copy each replica's final frame to the host."""
+    import numpy as np
+    from ed_mrdt import workloads
+    from ed_mrdt.ensemble import EnsembleEngine
+    from ed_mrdt.particles import ParticleState
+    from ed_mrdt.serving import Job, serve_plain
+
+    cfgs = [workloads.ensemble_replica(r) for r in range(n_rep)]
+    states = [ParticleState(c) for c in cfgs]
+    alive = sum(int(np.sum(s.is_alive)) for s in states)
+    cap = sum(s.N for s in states) + 1024
+    eng = EnsembleEngine(cfgs[0], cap, max_replicas=n_rep)
+    slots = [eng.add(c, s) for c, s in zip(cfgs, states)]
+    _progress(f"ensemble: {n_rep} replicas, {cap - 1024} particles admitted")
+    fe = max(1, int(round(cfgs[0].scan.frame_interval / cfgs[0].bd_timestep)))
+    eng.run(1)  # first step: layout, buffers
+    eng.synchronize()
+    t0 = time.perf_counter()
+    done = 1
+    while done < steps:
+        k = min(fe - done % fe, steps - done)
+        eng.run(k)
+        done += k
+        if done < steps and done % fe == 0:
+            eng.end_frames(slots)
+    counts = 0
+    for sl in slots:
+        counts += eng.frame(sl, download=True).total_counts
+    eng.synchronize()
+    t_b = time.perf_counter() - t0
+    eng.close()
+    _progress(f"ensemble: batched {steps} steps in {t_b:.2f} s")
+    sample = [Job(i, c, steps) for i, c in enumerate(cfgs[::plain_every])]
+    t1 = time.perf_counter()
+    serve_plain(sample, realtime=False, progress=_progress)
+    t_p = time.perf_counter() - t1
+    _progress(f"ensemble: plain {len(sample)} replicas in {t_p:.2f} s")
+    b_jobs = n_rep / t_b * (steps - 1) / steps  # the timed region ran steps - 1 steps
+    p_jobs = len(sample) / t_p
+    return {"replicas": n_rep, "steps": steps, "particles": cap - 1024, "alive": alive,
+            "batched_s": t_b, "batched_jobs_per_s": b_jobs,
+            "batched_particle_steps_per_s": alive * (steps - 1) / t_b,
+            "plain_sample_replicas": len(sample), "plain_s": t_p, "plain_jobs_per_s": p_jobs,
+            "batched_over_plain": b_jobs / p_jobs, "final_frame_counts": int(counts)}
+
+
+def served_bench(n_jobs: int, rate: float, budget_gib: float):
+    """Served stream: Poisson arrivals at `rate` jobs/s of the first n_jobs
+    jobs of the served workload (box side logU[5,50] um, 1k-250k particles
+    clamped to 100 / um^2, 500-5000 steps, 64-256 px), continuous batching
+    under an HBM budget vs the same arrivals served one job at a time."""
+    import numpy as np
+    from ed_mrdt import workloads
+    from ed_mrdt.serving import Job, Server, latency_stats, n_particles, serve_plain
+
+    g = np.random.default_rng(2026)
+    arr = np.concatenate([[0.0], np.cumsum(g.exponential(1.0 / rate, n_jobs - 1))])
+    jobs = []
+    for j in range(n_jobs):
+        cfg, steps = workloads.served_job(j)
+        jobs.append(Job(j, cfg, steps, arrival=float(arr[j])))
+    cap = sum(n_particles(jb.config) for jb in jobs) + 1024
+    srv = Server(jobs[0].config, int(budget_gib * (1 << 30)), cap, max_jobs=n_jobs, chunk=100)
+    res = srv.serve(jobs, realtime=True, progress=_progress)
+    srv.engine.close()
+    b = latency_stats(res)
+    _progress(f"served: batched {n_jobs} jobs, {b['jobs_per_s']:.2f} jobs/s")
+    p = latency_stats(serve_plain(jobs, realtime=True, progress=_progress))
+    _progress(f"served: plain {n_jobs} jobs, {p['jobs_per_s']:.2f} jobs/s")
+    work = sum(n_particles(jb.config) * jb.steps for jb in jobs)
+    return {"jobs": n_jobs, "arrival_rate_per_s": rate, "hbm_budget_gib": budget_gib,
+            "particle_steps": work, "batched": b, "plain": p,
+            "peak_resident_jobs": srv.peak_resident,
+            "batched_over_plain": b["jobs_per_s"] / p["jobs_per_s"] if p["jobs_per_s"] else None}
+
+
 def cpu_baseline(n_alive: int, budget_s: float = 20.0):
     """Oracle steps (dynamics + photophysics) on the 1M workload until the
     budget is spent; returns alive particle-steps/s and the sample size."""
@@ -169,6 +254,11 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-replicas", type=int, default=512)
+    ap.add_argument("--ensemble-steps", type=int, default=5000)
+    ap.add_argument("--plain-every", type=int, default=32)
+    ap.add_argument("--served-jobs", type=int, default=48)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="copy this many MiB device-to-device first (counter calibration)")
     a = ap.parse_args()
@@ -179,6 +269,7 @@ def main():
     if a.copy_probe_mib:
         copy_probe(a.copy_probe_mib)
     r = gpu_run(a.steps, a.warmup, a.n)
+    _progress(f"1M membrane: {r['steps']} steps in {r['wall']:.3f} s")
     traffic = traffic_from_profile()
     value = r["alive"] * r["steps"] / r["wall"]
     force_avg_s = r["force_ms"] / r["steps"] * 1e-3
@@ -208,6 +299,9 @@ def main():
         "photons_per_s": r["emitted"] / max(r["wall"], 1e-9),
         "detected_per_s": r["detected"] / max(r["wall"], 1e-9),
     }
+    if not a.no_ensemble:
+        out["ensemble"] = ensemble_bench(a.ensemble_replicas, a.ensemble_steps, a.plain_every)
+        out["served"] = served_bench(a.served_jobs, 50.0, 96.0)
     if not a.no_cpu_baseline:
         c = cpu_baseline(a.n, a.cpu_budget)
         out["cpu_baseline"] = {"value": c["value"], "unit": "particle-steps/s", "cores": c["threads"],

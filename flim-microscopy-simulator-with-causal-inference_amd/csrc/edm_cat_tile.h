@@ -46,9 +46,9 @@ __device__ __forceinline__ T cat_pick(const T (&a)[kCatFuse], int j) {
 
 template <typename Real>
 __device__ __forceinline__ int cat_try(const Cells<Real>& C, const Post<Real>& P,
-                                       const CatArgs<Real>& A, int j, const Grids<Real>& G,
-                                       int n_grids, int t, int it, int u, Real xnt, Real ynt,
-                                       Real xnu, Real ynu) {
+                                       const CatArgs<Real>& A, uint32_t step, int j,
+                                       const Grids<Real>& G, int n_grids, int t, int it, int u,
+                                       Real xnt, Real ynt, Real xnu, Real ynu) {
     Real dx, dy;
     if (min_image(xnt, ynt, xnu, ynu, C.Lx, C.Ly, C.px, C.py, dx, dy) > cat_pick(A.contact2, j))
         return 0;
@@ -82,7 +82,7 @@ __device__ __forceinline__ int cat_try(const Cells<Real>& C, const Post<Real>& P
         const uint32_t lo = (uint32_t)(min(it, iu) - C.kbase);
         const uint32_t hi = (uint32_t)(max(it, iu) - C.kbase);
         uu = (double)Uni<Real>::first(
-            A.rng.draw(A.step, lo, hi, (kTagCat << 16) | cat_pick(A.rxi, j)));
+            A.rng.draw(step, lo, hi, (kTagCat << 16) | cat_pick(A.rxi, j)));
     }
     if (uu < (double)pe && cas_byte(P.sp, u, (int8_t)A.sub, (int8_t)A.prod)) {
         count_update(G, n_grids, A.sub, xnu, ynu, -1, A.deferred);
@@ -95,8 +95,9 @@ __device__ __forceinline__ int cat_try(const Cells<Real>& C, const Post<Real>& P
 // Catalysis over one strip; every thread of the 256-wide block must call it.
 template <typename Real>
 __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>& P,
-                                          const CatArgs<Real>& A, unsigned long long* ctr,
-                                          const Grids<Real>& G, int n_grids, const StripGeo& S) {
+                                          const CatArgs<Real>& A, uint32_t step,
+                                          unsigned long long* ctr, const Grids<Real>& G,
+                                          int n_grids, const StripGeo& S) {
     constexpr int kCap = 1024;
     __shared__ Real s_x[kCap], s_y[kCap], s_xn[kCap], s_yn[kCap];
     __shared__ int8_t s_sp[kCap];
@@ -196,8 +197,8 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
                 Real dx, dy;
                 if (min_image(xt, yt, s_x[u], s_y[u], C.Lx, C.Ly, C.px, C.py, dx, dy) >= C.cut2)
                     continue;
-                hits += cat_try(C, P, A, j, G, n_grids, t, it, strip_src(S, s_off, s_gst, u),
-                                xnt, ynt, s_xn[u], s_yn[u]);
+                hits += cat_try(C, P, A, step, j, G, n_grids, t, it,
+                                strip_src(S, s_off, s_gst, u), xnt, ynt, s_xn[u], s_yn[u]);
             }
         } else if (lane < 9) {
             const Real xt = C.x[t], yt = C.y[t], xnt = P.xn[t], ynt = P.yn[t];
@@ -211,7 +212,7 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
                     if (min_image(xt, yt, C.x[u], C.y[u], C.Lx, C.Ly, C.px, C.py, dx, dy) >=
                         C.cut2)
                         continue;
-                    hits += cat_try(C, P, A, j, G, n_grids, t, it, u, xnt, ynt, P.xn[u],
+                    hits += cat_try(C, P, A, step, j, G, n_grids, t, it, u, xnt, ynt, P.xn[u],
                                     P.yn[u]);
                 }
         }
@@ -225,7 +226,12 @@ template <typename Real>
 __global__ __launch_bounds__(256) void k_cat_tile(Cells<Real> C, Post<Real> P, CatArgs<Real> A,
                                                   unsigned long long* ctr, Grids<Real> G,
                                                   int n_grids, int ntile_y, int ntiles) {
+    // the parameter block is indexed with per-lane values: keep it in LDS
+    // rather than in per-lane scratch
+    __shared__ CatArgs<Real> sA;
     const int tile = xcd_tile(ntiles);
     if (tile >= ntiles) return;  // uniform across the block
-    cat_strip(C, P, A, ctr, G, n_grids, strip_geo(C, tile, ntile_y));
+    if (threadIdx.x == 0) sA = A;
+    __syncthreads();
+    cat_strip(C, P, sA, A.step, ctr, G, n_grids, strip_geo(C, tile, ntile_y));
 }

@@ -21,7 +21,8 @@ static void ens_free(edm_ensemble* e) {
                     e->a.dye_t, e->rep, e->rep_t, e->key, e->rank, e->perm, e->cell_cnt,
                     e->cell_start, e->cub_tmp, e->counts, e->delta, e->cand_id, e->cand_slot,
                     e->cand_x, e->cand_y, e->n_cand, e->ctr, e->frame_out, e->sig, e->eps, e->kh,
-                    e->Dbase, e->d_pp, e->d_seg, e->d_tiles};
+                    e->Dbase, e->d_pp, e->d_seg, e->d_tiles, e->noise_ctr, e->d_slot_list,
+                    e->d_cats, e->d_pgrp};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (auto& g : e->seg)
@@ -233,6 +234,7 @@ int edm_ens_add(edm_ensemble* e, const edm_replica* r, const edm_host_state* s, 
     EDM_HIP(hipMemcpy(e->a.dye + beg, dye.data(), n, hipMemcpyHostToDevice));
     EDM_HIP(hipMemcpy(e->rep + beg, rep.data(), n * 2, hipMemcpyHostToDevice));
     EDM_HIP(hipMemset(e->ctr + 8 * (size_t)k, 0, 8 * sizeof(unsigned long long)));
+    EDM_HIP(hipMemset(e->noise_ctr + 2 * (size_t)k, 0, 2 * sizeof(unsigned long long)));
     e->seg[k] = g;
     e->fresh_beg[k] = beg;
     e->n_total += n;
@@ -276,6 +278,26 @@ int edm_ens_frame(edm_ensemble* e, int32_t slot, uint32_t* flim, int64_t n_elems
     }
     if (int rc = ens_frame_reset(e, slot)) return rc;
     EDM_HIP(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+int edm_ens_frame_end(edm_ensemble* e, const int32_t* slots, int32_t n_slots) {
+    if (!e || (n_slots > 0 && !slots)) return ens_invalid("edm_ens_frame_end: null argument");
+    for (int i = 0; i < n_slots; ++i)
+        if (slots[i] < 0 || slots[i] >= e->rmax || !e->seg[slots[i]].active)
+            return ens_invalid("edm_ens_frame_end: no such replica");
+    return ens_frame_end(e, slots, n_slots);
+}
+
+int edm_ens_noise_counters(edm_ensemble* e, int32_t slot, int64_t out[2]) {
+    if (!e || !out || slot < 0 || slot >= e->rmax || !e->seg[slot].active)
+        return ens_invalid("edm_ens_noise_counters: no such replica");
+    unsigned long long c[2];
+    EDM_HIP(hipMemcpyAsync(c, e->noise_ctr + 2 * (size_t)slot, sizeof(c), hipMemcpyDeviceToHost,
+                           e->stream));
+    EDM_HIP(hipStreamSynchronize(e->stream));
+    out[0] = (int64_t)c[0];
+    out[1] = (int64_t)c[1];
     return 0;
 }
 

@@ -178,52 +178,41 @@ __global__ __launch_bounds__(256) void k_ens_force(const EnsTile* __restrict__ t
                 Q.n_grids, keep_mask, ens_strip<Real>(g, T));
 }
 
-// Catalysis for one run of catalytic reactions (CatArgs with the model's
-// enzymes, substrate, product and contact radii; probabilities, feedback
-// grids, draws and step per replica).
+// Catalysis for one run of catalytic reactions; the run's parameter block
+// for each replica (probabilities, feedback grid, seed) is precomputed.
 template <typename Real>
 __global__ __launch_bounds__(256) void k_ens_cat(const EnsTile* __restrict__ tiles, int ntiles,
                                                  const EnsSeg* __restrict__ segs, EnsPool<Real> Q,
-                                                 CatArgs<Real> A, unsigned long long* ctr,
+                                                 const CatArgs<Real>* __restrict__ cats,
+                                                 int n_runs, int ci, unsigned long long* ctr,
                                                  uint32_t gstep) {
     const int tile = xcd_tile(ntiles);
     if (tile >= ntiles) return;
     const EnsTile T = tiles[tile];
     const EnsSeg& g = segs[T.r];
     const Cells<Real> C = ens_cells<Real>(g, Q.x, Q.y, Q.sp, Q.fl, Q.id, Q.start);
-    CatArgs<Real> Ar = A;
-    for (int j = 0; j < kCatFuse; ++j)
-        if (j < A.n_enz) Ar.prob[j] = (Real)g.prob[A.rxi[j]];
-    Ar.fb = ens_fb(A.fb, g, Q.ns);
-    Ar.rng = ens_rng(g);
-    Ar.step = gstep - g.step0;
+    const CatArgs<Real>& A = cats[(size_t)T.r * n_runs + ci];
     const Grids<Real> G = ens_grids<Real>(g, Q.counts, Q.delta, Q.ns);
-    cat_strip(C, ens_post(Q), Ar, ctr + 8 * T.r, G, Q.n_grids, ens_strip<Real>(g, T));
+    cat_strip(C, ens_post(Q), A, gstep - g.step0, ctr + 8 * T.r, G, Q.n_grids,
+              ens_strip<Real>(g, T));
 }
 
-// A run of per-particle passes over the strip's own particles; the run's
-// probabilities and feedback grids are the replica's.
+// A run of per-particle passes over the strip's own particles, with the
+// run's parameter block of the strip's replica.
 template <typename Real>
 __global__ __launch_bounds__(256) void k_ens_passes(const EnsTile* __restrict__ tiles,
                                                     int ntiles, const EnsSeg* __restrict__ segs,
-                                                    EnsPool<Real> Q, PassGroupDev grp,
+                                                    EnsPool<Real> Q,
+                                                    const PassGroupDev* __restrict__ pg,
+                                                    int n_groups, int gi,
                                                     unsigned long long* ctr, int32_t* cid,
                                                     int32_t* cslot, double* cxo, double* cyo,
                                                     unsigned int* nc, uint32_t gstep) {
-    __shared__ PassGroupDev s_grp;
     const int tile = xcd_tile(ntiles);
     if (tile >= ntiles) return;
     const EnsTile T = tiles[tile];
     const EnsSeg& g = segs[T.r];
-    if (threadIdx.x == 0) {
-        s_grp = grp;
-        for (int k = 0; k < grp.n; ++k) {
-            s_grp.p[k].prob = g.prob[grp.p[k].rxi];
-            s_grp.p[k].f1 = ens_fb(grp.p[k].f1, g, Q.ns);
-            s_grp.p[k].f2 = ens_fb(grp.p[k].f2, g, Q.ns);
-        }
-    }
-    __syncthreads();
+    const PassGroupDev& grp = pg[(size_t)T.r * n_groups + gi];
     const StripGeo S = ens_strip<Real>(g, T);
     const uint32_t* st = Q.start + g.coff;
     const int c0 = S.cx * g.ncy + S.cy0;
@@ -234,13 +223,14 @@ __global__ __launch_bounds__(256) void k_ens_passes(const EnsTile* __restrict__ 
     const Post<Real> P = ens_post(Q);
     const Rng rng = ens_rng(g);
     const uint32_t step = gstep - g.step0;
+    const uint32_t species = grp.species;
     int det = 0, conv = 0;
     for (int t0 = b; t0 < e; t0 += blockDim.x) {
         const int t = t0 + (int)threadIdx.x;
         if (t < e) {
             const int s0 = Q.sp[t];
-            if ((s_grp.species >> s0) & 1u)
-                passes_particle(t, s0, g.n, Q.id, g.kbase, P, s_grp, (const double*)nullptr, rng,
+            if ((species >> s0) & 1u)
+                passes_particle(t, s0, g.n, Q.id, g.kbase, P, grp, (const double*)nullptr, rng,
                                 step, (Real)g.Lx, (Real)g.Ly, G, Q.n_grids, cid + pbeg,
                                 cslot + pbeg, cxo + pbeg, cyo + pbeg, nc + T.r, det, conv);
         }
@@ -270,6 +260,108 @@ __global__ __launch_bounds__(1024) void k_ens_recruit(const EnsSeg* __restrict__
 // Host side
 // ---------------------------------------------------------------------------
 
+// A model feedback rule bound to replica g's grid (host side of ens_fb).
+static FbDev ens_fb_host(const edm_feedback& f, const EnsSeg& g, int ns) {
+    FbDev d = to_fb_model(f);
+    if (f.sensor >= 0 && f.grid >= 0) {
+        d.cbase = g.goff[f.grid * ns + f.sensor];
+        d.gnx = g.gdims[2 * f.grid];
+        d.gny = g.gdims[2 * f.grid + 1];
+        d.gsx = g.gs_d[2 * f.grid];
+        d.gsy = g.gs_d[2 * f.grid + 1];
+        d.ginv = g.ginv_d[f.grid];
+    }
+    return d;
+}
+
+// Pass group gi of the model with replica g's probabilities and grids.
+static PassGroupDev ens_group_host(const edm_ensemble* e, int gi, const EnsSeg& g) {
+    const std::vector<int>& ix = e->pass_groups[gi];
+    const int ns = e->p.n_species;
+    PassGroupDev grp{};
+    grp.n = (int)ix.size();
+    for (int k = 0; k < grp.n; ++k) {
+        const edm_reaction& q = e->rx[ix[k]];
+        PassDev& d = grp.p[k];
+        d.kind = q.kind;
+        d.a = q.a;
+        d.b = q.b;
+        d.rxi = ix[k];
+        d.prob = g.prob[ix[k]];
+        d.f1 = ens_fb_host(q.fb1, g, ns);
+        d.f2 = ens_fb_host(q.fb2, g, ns);
+        d.draw_off = 0;
+        d.deferred = e->pass_deferred[ix[k]];
+        uint32_t reach = 1u << q.a;
+        for (int j = k - 1; j >= 0; --j) {
+            const edm_reaction& qj = e->rx[ix[j]];
+            if (qj.kind == EDM_RX_CONVERT && ((reach >> qj.b) & 1u)) reach |= 1u << qj.a;
+        }
+        d.reach = reach;
+        grp.species |= reach;
+    }
+    return grp;
+}
+
+// Catalysis run (start, nf) of the model with replica g's probabilities,
+// feedback grid and seed (the step is set by the kernel).
+template <typename Real>
+static CatArgs<Real> ens_cat_host(const edm_ensemble* e, int start, int nf, const EnsSeg& g) {
+    const edm_reaction& r = e->rx[start];
+    CatArgs<Real> A{};
+    A.n_enz = nf;
+    for (int j = 0; j < nf; ++j) {
+        const edm_reaction& rj = e->rx[start + j];
+        A.enz[j] = rj.a;
+        A.contact2[j] = (Real)rj.contact * (Real)rj.contact;
+        A.prob[j] = (Real)g.prob[start + j];
+        A.rxi[j] = (uint32_t)(start + j);
+        A.doff[j] = 0;
+    }
+    A.sub = r.b;
+    A.prod = r.c;
+    A.fb = ens_fb_host(r.fb1, g, e->p.n_species);
+    A.w = nullptr;
+    A.wnx = A.wny = 1;
+    A.Kw = (Real)e->p.adapt_K;
+    A.nw = (Real)e->p.adapt_n;
+    A.dr = Draws{nullptr, 0, 0, 0};
+    A.rng = Rng{g.seed_lo, g.seed_hi};
+    A.step = 0;
+    A.deferred = r.fb1.sensor >= 0 && (r.fb1.sensor == r.b || r.fb1.sensor == r.c);
+    return A;
+}
+
+template <typename Real>
+static int ens_tables_t(edm_ensemble* e) {
+    const size_t nr = e->cat_runs.size(), ng = e->pass_groups.size();
+    std::vector<CatArgs<Real>> cats(std::max<size_t>(1, (size_t)e->rmax * nr));
+    std::vector<PassGroupDev> pg(std::max<size_t>(1, (size_t)e->rmax * ng));
+    for (int r = 0; r < e->rmax; ++r) {
+        const EnsSeg& g = e->seg[r];
+        if (!g.active) continue;
+        for (size_t c = 0; c < nr; ++c)
+            cats[(size_t)r * nr + c] =
+                ens_cat_host<Real>(e, e->cat_runs[c].first, e->cat_runs[c].second, g);
+        for (size_t k = 0; k < ng; ++k) pg[(size_t)r * ng + k] = ens_group_host(e, (int)k, g);
+    }
+    const size_t cb = cats.size() * sizeof(CatArgs<Real>), pb = pg.size() * sizeof(PassGroupDev);
+    if (cb > e->cats_bytes) {
+        hipFree(e->d_cats);
+        EDM_HIP(hipMalloc(&e->d_cats, cb));
+        e->cats_bytes = cb;
+    }
+    if (pb > e->pgrp_bytes) {
+        hipFree(e->d_pgrp);
+        EDM_HIP(hipMalloc(&e->d_pgrp, pb));
+        e->pgrp_bytes = pb;
+    }
+    EDM_HIP(hipMemcpyAsync(e->d_cats, cats.data(), cb, hipMemcpyHostToDevice, e->stream));
+    EDM_HIP(hipMemcpyAsync(e->d_pgrp, pg.data(), pb, hipMemcpyHostToDevice, e->stream));
+    EDM_HIP(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
 template <typename Real>
 static int ens_create_t(edm_ensemble* e) {
     const edm_params& p = e->p;
@@ -286,10 +378,12 @@ static int ens_create_t(edm_ensemble* e) {
         dalloc(&e->perm, cap) || dalloc(&e->cand_id, cap) || dalloc(&e->cand_slot, cap) ||
         dalloc(&e->cand_x, cap) || dalloc(&e->cand_y, cap) || dalloc(&e->n_cand, e->rmax) ||
         dalloc(&e->ctr, 8 * (size_t)e->rmax) || dalloc(&e->frame_out, 4) ||
+        dalloc(&e->noise_ctr, 2 * (size_t)e->rmax) ||
         dalloc(&e->d_seg, e->rmax))
         return EDM_EHIP;
     EDM_HIP(hipMemset(e->n_cand, 0, (size_t)e->rmax * sizeof(unsigned int)));
     EDM_HIP(hipMemset(e->ctr, 0, (size_t)e->rmax * 8 * sizeof(unsigned long long)));
+    EDM_HIP(hipMemset(e->noise_ctr, 0, (size_t)e->rmax * 2 * sizeof(unsigned long long)));
     int rc = 0;
     rc |= upload_real<Real>(&e->sig, p.sigma, (size_t)ns * ns);
     rc |= upload_real<Real>(&e->eps, p.epsilon, (size_t)ns * ns);
@@ -319,6 +413,29 @@ static int ens_create_t(edm_ensemble* e) {
                 e->sensor_mask_all |= 1u << s;
             }
     plan_passes(e->rx, e->sensor_mask_all, e->pass_groups, e->group_start, e->pass_deferred);
+    // catalysis runs: back-to-back reactions with one substrate and product and
+    // no feedback share a launch (as for one membrane)
+    e->cat_runs.clear();
+    e->cat_run_of.assign(e->rx.size(), -1);
+    for (size_t ri = 0; ri < e->rx.size();) {
+        const edm_reaction& r = e->rx[ri];
+        if (r.kind != EDM_RX_CAT) {
+            ++ri;
+            continue;
+        }
+        const bool plain = r.fb1.sensor < 0 && !r.use_adapt;
+        int nf = 1;
+        while (plain && nf < kCatFuse && ri + nf < e->rx.size()) {
+            const edm_reaction& r2 = e->rx[ri + nf];
+            if (r2.kind != EDM_RX_CAT || r2.b != r.b || r2.c != r.c || r2.fb1.sensor >= 0 ||
+                r2.use_adapt)
+                break;
+            ++nf;
+        }
+        e->cat_run_of[ri] = (int)e->cat_runs.size();
+        e->cat_runs.emplace_back((int)ri, nf);
+        ri += nf;
+    }
     e->seg.assign(e->rmax, EnsSeg{});
     e->fresh_beg.assign(e->rmax, -1);
     EDM_HIP(hipMemset(e->d_seg, 0, (size_t)e->rmax * sizeof(EnsSeg)));
@@ -386,6 +503,7 @@ int ens_layout(edm_ensemble* e) {
     EDM_HIP(hipMemcpyAsync(e->d_seg, e->seg.data(), (size_t)e->rmax * sizeof(EnsSeg),
                            hipMemcpyHostToDevice, e->stream));
     EDM_HIP(hipStreamSynchronize(e->stream));
+    if (int rc = e->fp64 ? ens_tables_t<double>(e) : ens_tables_t<float>(e)) return rc;
     e->layout_dirty = false;
     return 0;
 }
@@ -410,9 +528,14 @@ static int ens_step_t(edm_ensemble* e) {
                                                  (int)ncnt, st));
         hipLaunchKernelGGL(k_scatter, dim3(nblk(n)), dim3(kBlock), 0, st, n, e->key, e->rank,
                            e->cell_start, e->perm);
-        hipLaunchKernelGGL(k_sort_cells, dim3(nblk(ncnt - 1)), dim3(kBlock), 0, st,
-                           (int)(ncnt - 1), e->cell_start, e->perm, e->cell_cnt,
-                           (const int32_t*)e->a.id);
+        // live cells only: the sentinel bucket holds every particle of the
+        // removed replicas, in no particular order, and is just cleared
+        if (e->n_cells > 0)
+            hipLaunchKernelGGL(k_sort_cells, dim3(nblk(e->n_cells)), dim3(kBlock), 0, st,
+                               (int)e->n_cells, e->cell_start, e->perm, e->cell_cnt,
+                               (const int32_t*)e->a.id);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, 1,
+                           (unsigned int*)(e->cell_cnt + e->n_cells));
         hipLaunchKernelGGL(k_ens_gather<Real>, dim3(nblk(n)), dim3(kBlock), 0, st, n,
                            (const uint32_t*)e->perm, e->a, (const uint16_t*)e->rep, e->rep_t);
         std::swap(e->a.x, e->a.xt);
@@ -471,82 +594,33 @@ static int ens_step_t(edm_ensemble* e) {
                        (const EnsTile*)e->d_tiles, ntiles, (const EnsSeg*)e->d_seg, Q, F,
                        e->sensor_mask_all, gstep);
 
-    int cat_skip = 0;
     for (size_t ri = 0; ri < e->rx.size(); ++ri) {
         const edm_reaction& r = e->rx[ri];
         if (r.kind == EDM_RX_CAT) {
-            if (cat_skip > 0) {
-                --cat_skip;
-                continue;
-            }
-            const FbDev f1 = to_fb_model(r.fb1);
-            const bool defer = f1.sensor >= 0 && (f1.sensor == r.b || f1.sensor == r.c);
-            const bool plain = f1.sensor < 0 && !r.use_adapt;
-            int nf = 1;
-            while (plain && nf < kCatFuse && ri + nf < e->rx.size()) {
-                const edm_reaction& r2 = e->rx[ri + nf];
-                if (r2.kind != EDM_RX_CAT || r2.b != r.b || r2.c != r.c || r2.fb1.sensor >= 0 ||
-                    r2.use_adapt)
-                    break;
-                ++nf;
-            }
-            CatArgs<Real> A{};
-            A.n_enz = nf;
-            for (int j = 0; j < nf; ++j) {
-                const edm_reaction& rj = e->rx[ri + j];
-                A.enz[j] = rj.a;
-                A.contact2[j] = (Real)rj.contact * (Real)rj.contact;
-                A.prob[j] = (Real)0;  // per replica
-                A.rxi[j] = (uint32_t)(ri + j);
-                A.doff[j] = 0;
-            }
-            A.sub = r.b;
-            A.prod = r.c;
-            A.fb = f1;
-            A.w = nullptr;
-            A.wnx = A.wny = 1;
-            A.Kw = (Real)p.adapt_K;
-            A.nw = (Real)p.adapt_n;
-            A.dr = Draws{nullptr, 0, 0, 0};
-            A.deferred = defer;
+            const int ci = e->cat_run_of[ri];
+            if (ci < 0) continue;  // member of a run launched before
+            const int start = e->cat_runs[ci].first;
             hipLaunchKernelGGL((k_ens_cat<Real>), tgrid, dim3(256), 0, st,
-                               (const EnsTile*)e->d_tiles, ntiles, (const EnsSeg*)e->d_seg, Q, A,
+                               (const EnsTile*)e->d_tiles, ntiles, (const EnsSeg*)e->d_seg, Q,
+                               (const CatArgs<Real>*)e->d_cats, (int)e->cat_runs.size(), ci,
                                e->ctr, gstep);
+            const edm_reaction& r0 = e->rx[start];
+            const bool defer =
+                r0.fb1.sensor >= 0 && (r0.fb1.sensor == r0.b || r0.fb1.sensor == r0.c);
             if (defer && e->n_counts)
                 hipLaunchKernelGGL(k_fold, dim3(nblk(e->n_counts)), dim3(kBlock), 0, st,
                                    (int)e->n_counts, e->counts, e->delta);
-            cat_skip = nf - 1;
         } else if (r.kind == EDM_RX_RECRUIT || r.kind == EDM_RX_DETACH ||
                    r.kind == EDM_RX_CONVERT) {
             if (e->group_start[ri] < 0) continue;
-            const std::vector<int>& grp_ix = e->pass_groups[e->group_start[ri]];
-            PassGroupDev grp{};
-            grp.n = (int)grp_ix.size();
+            const int gi = e->group_start[ri];
+            const std::vector<int>& grp_ix = e->pass_groups[gi];
             bool defer_any = false;
-            for (int k = 0; k < grp.n; ++k) {
-                const edm_reaction& q = e->rx[grp_ix[k]];
-                PassDev& d = grp.p[k];
-                d.kind = q.kind;
-                d.a = q.a;
-                d.b = q.b;
-                d.rxi = grp_ix[k];
-                d.prob = 0.0;  // per replica
-                d.f1 = to_fb_model(q.fb1);
-                d.f2 = to_fb_model(q.fb2);
-                d.draw_off = 0;
-                d.deferred = e->pass_deferred[grp_ix[k]];
-                defer_any = defer_any || d.deferred;
-                uint32_t reach = 1u << q.a;
-                for (int j = k - 1; j >= 0; --j) {
-                    const edm_reaction& qj = e->rx[grp_ix[j]];
-                    if (qj.kind == EDM_RX_CONVERT && ((reach >> qj.b) & 1u)) reach |= 1u << qj.a;
-                }
-                d.reach = reach;
-                grp.species |= reach;
-            }
+            for (int k : grp_ix) defer_any = defer_any || e->pass_deferred[k];
             hipLaunchKernelGGL((k_ens_passes<Real>), tgrid, dim3(256), 0, st,
                                (const EnsTile*)e->d_tiles, ntiles, (const EnsSeg*)e->d_seg, Q,
-                               grp, e->ctr, e->cand_id, e->cand_slot, e->cand_x, e->cand_y,
+                               (const PassGroupDev*)e->d_pgrp, (int)e->pass_groups.size(), gi,
+                               e->ctr, e->cand_id, e->cand_slot, e->cand_x, e->cand_y,
                                e->n_cand, gstep);
             const edm_reaction& last = e->rx[grp_ix.back()];
             if (last.kind == EDM_RX_RECRUIT)

@@ -79,3 +79,55 @@ int ens_frame_reset(edm_ensemble* e, int slot) {
     EDM_HIP(hipGetLastError());
     return 0;
 }
+
+// Frame end of several replicas at once (blockIdx.y = position in the slot
+// list): detector noise into each listed stack, counted per replica.
+__global__ void k_ens_frame_noise(const EnsSeg* __restrict__ segs, const int32_t* __restrict__ slots,
+                                  unsigned long long* noise_ctr, uint32_t gstep) {
+    const int r = slots[blockIdx.y];
+    const EnsSeg& g = segs[r];
+    const int64_t npix = (int64_t)g.ip.n_ch * g.ip.nx * g.ip.ny;
+    const Rng rng{g.seed_lo ^ 0x2545F491u, g.seed_hi};
+    const uint32_t frame = gstep - g.step0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t end = (npix + 63) / 64 * 64;  // whole wavefronts take part in the counts
+    for (int64_t q0 = (int64_t)blockIdx.x * blockDim.x; q0 < end; q0 += stride) {
+        const int64_t q = q0 + threadIdx.x;
+        unsigned long long nd = 0, na = 0;
+        if (q < npix && g.flim)
+            frame_noise_pixel(q, g.ip.n_bins, g.flim, g.ip.dark_lambda, g.ip.p_ap, rng, frame, nd,
+                              na);
+        noise_count(nd, na, noise_ctr + 2 * r);
+    }
+}
+
+__global__ void k_ens_frame_reset(const EnsSeg* __restrict__ segs,
+                                  const int32_t* __restrict__ slots) {
+    const EnsSeg& g = segs[slots[blockIdx.y]];
+    if (!g.flim) return;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < g.flim_elems;
+         i += (int64_t)gridDim.x * blockDim.x)
+        g.flim[i] = 0u;
+}
+
+int ens_frame_end(edm_ensemble* e, const int32_t* slots, int32_t n) {
+    if (n <= 0) return 0;
+    if (e->layout_dirty)
+        if (int rc = ens_layout(e)) return rc;
+    if ((size_t)n > e->slot_list_cap) {
+        hipFree(e->d_slot_list);
+        e->slot_list_cap = (size_t)std::max(n, e->rmax);
+        EDM_HIP(hipMalloc(&e->d_slot_list, e->slot_list_cap * sizeof(int32_t)));
+    }
+    EDM_HIP(hipMemcpyAsync(e->d_slot_list, slots, (size_t)n * sizeof(int32_t),
+                           hipMemcpyHostToDevice, e->stream));
+    hipLaunchKernelGGL(k_ens_frame_noise, dim3(32, n), dim3(kB), 0, e->stream,
+                       (const EnsSeg*)e->d_seg, (const int32_t*)e->d_slot_list, e->noise_ctr,
+                       e->gstep);
+    hipLaunchKernelGGL(k_ens_frame_reset, dim3(64, n), dim3(kB), 0, e->stream,
+                       (const EnsSeg*)e->d_seg, (const int32_t*)e->d_slot_list);
+    EDM_HIP(hipGetLastError());
+    // the slot list must stay valid until the kernels have read it
+    EDM_HIP(hipStreamSynchronize(e->stream));
+    return 0;
+}

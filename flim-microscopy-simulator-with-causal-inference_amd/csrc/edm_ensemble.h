@@ -76,6 +76,9 @@ struct edm_ensemble {
     unsigned int* n_cand = nullptr;        // [rmax]
     unsigned long long* ctr = nullptr;     // [rmax][8]
     unsigned long long* frame_out = nullptr;  // [2] dark counts, afterpulses of one frame
+    unsigned long long* noise_ctr = nullptr;  // [rmax][2] dark counts, afterpulses (frame_end)
+    int32_t* d_slot_list = nullptr;
+    size_t slot_list_cap = 0;
     void *sig = nullptr, *eps = nullptr, *kh = nullptr, *Dbase = nullptr;
     double rint2 = 0.0;
     double max_disp_model = 0.0;
@@ -104,6 +107,14 @@ struct edm_ensemble {
     std::vector<std::pair<int, int>> kept;  // (grid, species) with density counts
     std::vector<std::vector<int>> pass_groups;
     std::vector<int> group_start, pass_deferred;
+    std::vector<std::pair<int, int>> cat_runs;  // (first reaction, reactions) per launch
+    std::vector<int> cat_run_of;                // per reaction: run it starts, else -1
+    // per replica slot and run / group: the kernels' parameter blocks with the
+    // replica's probabilities, feedback grids and seed (rebuilt by ens_layout)
+    void* d_cats = nullptr;
+    size_t cats_bytes = 0;
+    void* d_pgrp = nullptr;
+    size_t pgrp_bytes = 0;
 };
 
 namespace edm {
@@ -113,6 +124,7 @@ int ens_step(edm_ensemble* e, bool image);
 int ens_photons(edm_ensemble* e);
 int ens_frame_noise(edm_ensemble* e, int slot, int64_t* n_dark, int64_t* n_ap);
 int ens_frame_reset(edm_ensemble* e, int slot);
+int ens_frame_end(edm_ensemble* e, const int32_t* slots, int32_t n);
 void plan_passes(const std::vector<edm_reaction>& rx, uint32_t sensor_mask,
                  std::vector<std::vector<int>>& groups, std::vector<int>& group_start,
                  std::vector<int>& deferred);

@@ -591,39 +591,43 @@ __device__ __forceinline__ int binomial_draw(int nn, double p, double u, double 
     return k;
 }
 
-__global__ void k_frame_noise(int64_t npix, int nb, uint32_t* flim, double lam, double p_ap,
-                              Rng rng, uint32_t frame, unsigned long long* out) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long nd = 0, na = 0;
-    if (q < npix) {
-        uint32_t* row = flim + q * nb;
-        const uint32_t lo = (uint32_t)q, hi = (uint32_t)(q >> 32);
-        if (lam >= 1e-10) {
-            const U4 r = rng.draw(frame, lo, hi, kTagDark);
-            const int k = poisson_inv(lam, u01d(r.x, r.y));
+// Dark counts (Poisson, uniform bins) and afterpulses (binomial of the
+// pixel's counts, uniform bins) of one pixel histogram.
+__device__ __forceinline__ void frame_noise_pixel(int64_t q, int nb, uint32_t* flim, double lam,
+                                                  double p_ap, Rng rng, uint32_t frame,
+                                                  unsigned long long& nd,
+                                                  unsigned long long& na) {
+    uint32_t* row = flim + q * nb;
+    const uint32_t lo = (uint32_t)q, hi = (uint32_t)(q >> 32);
+    if (lam >= 1e-10) {
+        const U4 r = rng.draw(frame, lo, hi, kTagDark);
+        const int k = poisson_inv(lam, u01d(r.x, r.y));
+        for (int i = 0; i < k; ++i) {
+            const U4 rb = rng.draw(frame, lo, hi + 0x10000u * (uint32_t)(i + 1), kTagDark);
+            row[(int)(u01d(rb.x, rb.y) * nb) % nb] += 1u;
+        }
+        nd += (unsigned long long)k;
+    }
+    if (p_ap > 0.0) {
+        uint32_t tot = 0;
+        for (int b = 0; b < nb; ++b) tot += row[b];
+        if (tot) {
+            const U4 r = rng.draw(frame, lo, hi, kTagAfterpulse);
+            double z0, z1;
+            normal2<double>(r, z0, z1);
+            const int k = binomial_draw((int)tot, p_ap, u01d(r.x, r.z), z0);
             for (int i = 0; i < k; ++i) {
-                const U4 rb = rng.draw(frame, lo, hi + 0x10000u * (uint32_t)(i + 1), kTagDark);
+                const U4 rb =
+                    rng.draw(frame, lo, hi + 0x10000u * (uint32_t)(i + 1), kTagAfterpulse);
                 row[(int)(u01d(rb.x, rb.y) * nb) % nb] += 1u;
             }
-            nd = (unsigned long long)k;
-        }
-        if (p_ap > 0.0) {
-            uint32_t tot = 0;
-            for (int b = 0; b < nb; ++b) tot += row[b];
-            if (tot) {
-                const U4 r = rng.draw(frame, lo, hi, kTagAfterpulse);
-                double z0, z1;
-                normal2<double>(r, z0, z1);
-                const int k = binomial_draw((int)tot, p_ap, u01d(r.x, r.z), z0);
-                for (int i = 0; i < k; ++i) {
-                    const U4 rb =
-                        rng.draw(frame, lo, hi + 0x10000u * (uint32_t)(i + 1), kTagAfterpulse);
-                    row[(int)(u01d(rb.x, rb.y) * nb) % nb] += 1u;
-                }
-                na = (unsigned long long)k;
-            }
+            na += (unsigned long long)k;
         }
     }
+}
+
+__device__ __forceinline__ void noise_count(unsigned long long nd, unsigned long long na,
+                                            unsigned long long* out) {
     for (int o = 32; o > 0; o >>= 1) {
         nd += __shfl_down(nd, o);
         na += __shfl_down(na, o);
@@ -632,6 +636,14 @@ __global__ void k_frame_noise(int64_t npix, int nb, uint32_t* flim, double lam, 
         if (nd) atomicAdd(&out[0], nd);
         if (na) atomicAdd(&out[1], na);
     }
+}
+
+__global__ void k_frame_noise(int64_t npix, int nb, uint32_t* flim, double lam, double p_ap,
+                              Rng rng, uint32_t frame, unsigned long long* out) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long nd = 0, na = 0;
+    if (q < npix) frame_noise_pixel(q, nb, flim, lam, p_ap, rng, frame, nd, na);
+    noise_count(nd, na, out);
 }
 
 __global__ void k_fill_u32(int64_t n, uint32_t* p, uint32_t v) {

@@ -140,6 +140,8 @@ _SIGS = {
     "edm_ens_run": [c_void_p, c_int32, c_int32],
     "edm_ens_frame": [c_void_p, c_int32, c_void_p, c_int64, ctypes.POINTER(c_int64),
                       ctypes.POINTER(c_int64)],
+    "edm_ens_frame_end": [c_void_p, c_void_p, c_int32],
+    "edm_ens_noise_counters": [c_void_p, c_int32, c_void_p],
     "edm_ens_download": [c_void_p, c_int32, ctypes.POINTER(HostState)],
     "edm_ens_counters": [c_void_p, c_int32, c_void_p],
     "edm_ens_info": [c_void_p, c_void_p],

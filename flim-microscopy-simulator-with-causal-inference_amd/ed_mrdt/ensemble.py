@@ -74,6 +74,10 @@ class EnsembleEngine:
                                    int(capacity), int(max_replicas), ctypes.byref(h)),
               "edm_ens_create")
         self._h = h
+        self.max_replicas = int(max_replicas)
+        self.capacity = int(capacity)
+        self._draining = 0  # removed replicas whose slots free up at the next step
+        self._pooled = 0    # particles in the pool, removed replicas' included
         self._n_ch = ip.n_ch if ip is not None else 0
         self._n_bins = ip.n_bins if ip is not None else 0
         self.replicas: Dict[int, Dict] = {}
@@ -124,19 +128,34 @@ class EnsembleEngine:
         check(lib().edm_ens_add(self._h, ctypes.byref(r), ctypes.byref(hs), ctypes.byref(slot)),
               "edm_ens_add")
         k = int(slot.value)
+        self._pooled += st.N
         self.replicas[k] = dict(config=config, N=st.N, shape=shape, frame_index=0,
                                 frame_every=max(1, int(round(config.scan.frame_interval /
                                                              config.bd_timestep))))
         return k
 
     def remove(self, slot: int) -> None:
+        """Remove a replica; its slot and pool space are reusable after the
+        next step."""
         check(lib().edm_ens_remove(self._h, int(slot)), "edm_ens_remove")
-        self.replicas.pop(slot, None)
+        info = self.replicas.pop(slot, None)
+        self._draining += 1
+        self._retired = getattr(self, "_retired", 0) + (info["N"] if info else 0)
+
+    def free_slots(self) -> int:
+        return self.max_replicas - len(self.replicas) - self._draining
+
+    def free_particles(self) -> int:
+        return self.capacity - self._pooled
 
     # ---- stepping ------------------------------------------------------------
     def run(self, n_steps: int) -> None:
         """Advance every resident replica by n_steps (asynchronous)."""
         check(lib().edm_ens_run(self._h, int(n_steps), 1 if self.image else 0), "edm_ens_run")
+        if n_steps > 0:
+            self._draining = 0
+            self._pooled -= getattr(self, "_retired", 0)
+            self._retired = 0
 
     def synchronize(self) -> None:
         check(lib().edm_ens_synchronize(self._h), "edm_ens_synchronize")
@@ -161,6 +180,22 @@ class EnsembleEngine:
         return FLIMFrame.from_stack(out, frame_index=info["frame_index"] - 1,
                                     n_signal=c["photons_detected"], n_dark=int(nd.value),
                                     n_ap=int(na.value))
+
+    def end_frames(self, slots) -> None:
+        """End the current frame of several replicas at once (detector noise
+        and reset on the device, nothing copied out)."""
+        sl = np.ascontiguousarray(np.asarray(list(slots), dtype=np.int32))
+        if sl.size == 0:
+            return
+        check(lib().edm_ens_frame_end(self._h, ptr(sl), int(sl.size)), "edm_ens_frame_end")
+        for k in sl.tolist():
+            self.replicas[k]["frame_index"] += 1
+
+    def noise_counters(self, slot: int) -> Dict[str, int]:
+        """Dark counts and afterpulses added by end_frames so far."""
+        out = np.zeros(2, dtype=np.int64)
+        check(lib().edm_ens_noise_counters(self._h, int(slot), ptr(out)), "edm_ens_noise_counters")
+        return dict(n_dark=int(out[0]), n_afterpulses=int(out[1]))
 
     def state(self, slot: int) -> Dict[str, np.ndarray]:
         """The replica's particle arrays in particle-index order."""

@@ -266,6 +266,10 @@ int edm_ens_run(edm_ensemble *e, int32_t n_steps, int32_t image);
  * NULL) and reset of its TCSPC stack */
 int edm_ens_frame(edm_ensemble *e, int32_t slot, uint32_t *flim, int64_t n_elems,
                   int64_t *n_dark, int64_t *n_afterpulse);
+/* frame end of several replicas at once: detector noise and reset, no copy;
+ * the noise counts accumulate per replica (edm_ens_noise_counters) */
+int edm_ens_frame_end(edm_ensemble *e, const int32_t *slots, int32_t n_slots);
+int edm_ens_noise_counters(edm_ensemble *e, int32_t slot, int64_t out[2]);
 int edm_ens_download(edm_ensemble *e, int32_t slot, edm_host_state *s);
 /* the replica's counters: bindings, dissociations, catalytic, recruited,
  * detached, converted, photons emitted, photons detected */
