@@ -127,7 +127,7 @@ int edm_destroy(edm_membrane* m) {
                     m->ph_fret, m->flim, m->pcx, m->det_flag, m->det_pos, m->sv_key, m->sv_key2,
                     m->sv_val, m->sv_val2, m->asg_px, m->asg_py, m->asg_flag, m->asg_pos,
                     m->asg_t, m->asg_ch, m->sv_px, m->sv_py, m->pix_tot, m->d_scal,
-                    m->dye_ids, m->pass_ids};
+                    m->dye_ids, m->wl_pass, m->wl_dye, m->wl_cnt};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (auto e : m->kev) hipEventDestroy(e);
@@ -188,20 +188,21 @@ int edm_upload_state(edm_membrane* m, const edm_host_state* s) {
     if (!dyes.empty())
         EDM_HIP(hipMemcpy(m->dye_ids, dyes.data(), dyes.size() * 4, hipMemcpyHostToDevice));
     m->n_dye = (int)dyes.size();
-    // particles of the species the per-particle passes can touch; a list only
-    // pays when it is much shorter than the state
-    std::vector<int32_t> pass;
+    // particles of the species the per-particle passes can touch; the
+    // gather of every step lists their slots (and the dye carriers' slots)
+    int n_pass = 0;
     if (m->pass_species)
         for (int i = 0; i < n; ++i)
-            if ((m->pass_species >> sp[i]) & 1u) pass.push_back(i);
-    if (m->pass_ids) EDM_HIP(hipFree(m->pass_ids));
-    m->pass_ids = nullptr;
-    m->n_pass = 0;
-    if (!pass.empty() && 16 * pass.size() <= (size_t)n) {
-        EDM_HIP(hipMalloc(&m->pass_ids, pass.size() * sizeof(int32_t)));
-        EDM_HIP(hipMemcpy(m->pass_ids, pass.data(), pass.size() * 4, hipMemcpyHostToDevice));
-        m->n_pass = (int)pass.size();
-    }
+            if ((m->pass_species >> sp[i]) & 1u) ++n_pass;
+    m->n_pass = n_pass;
+    for (int32_t** l : {&m->wl_pass, &m->wl_dye})
+        if (*l) EDM_HIP(hipFree(*l));
+    m->wl_pass = m->wl_dye = nullptr;
+    EDM_HIP(hipMalloc(&m->wl_pass, (size_t)std::max(n_pass, 1) * sizeof(int32_t)));
+    EDM_HIP(hipMalloc(&m->wl_dye, (size_t)std::max(m->n_dye, 1) * sizeof(int32_t)));
+    if (!m->wl_cnt) EDM_HIP(hipMalloc(&m->wl_cnt, 2 * sizeof(uint32_t)));
+    EDM_HIP(hipMemset(m->wl_cnt, 0, 2 * sizeof(uint32_t)));
+    m->wl_valid = false;
     return 0;
 }
 

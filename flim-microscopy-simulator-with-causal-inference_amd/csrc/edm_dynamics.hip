@@ -211,9 +211,10 @@ template <typename Real>
 __global__ void k_bin(int n, const Real* __restrict__ x, const Real* __restrict__ y, Real csx,
                       Real csy, int ncx, int ncy, uint32_t* __restrict__ key,
                       uint32_t* __restrict__ rank, uint32_t* __restrict__ cnt, int32_t* counts,
-                      int n_counts) {
+                      int n_counts, uint32_t* wl_cnt) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < n_counts) counts[s] = 0;  // density counts are rebuilt each step
+    if (wl_cnt && s < 2) wl_cnt[s] = 0;  // work lists are rebuilt by the gather
     const bool live = s < n;          // live lanes are a prefix of the wavefront
     uint32_t c = 0xffffffffu;
     if (live) {
@@ -317,6 +318,93 @@ __global__ void k_gather(int n, const uint32_t* __restrict__ perm, DevArrays a) 
     a.ctmc_t[t] = a.ctmc[s];
     a.dye_t[t] = a.dye[s];
     a.slot_of[pid] = t;
+}
+
+// Work lists, rebuilt after every sort: the slots of the particles the
+// per-particle passes can touch and the slots of the dye carriers, so that
+// those kernels run one listed slot per thread.  A block lists its
+// kWlSlots slots in LDS and appends each list with one atomic;
+// the order of the blocks' runs in a list is unspecified and every consumer
+// is independent of it.  The counters are cleared by the binning kernel.
+static constexpr int kWlPer = 8;  // slots per thread
+static constexpr int kWlSlots = kWlPer * 256;
+
+struct WlRun {
+    int32_t* list;
+    uint32_t* cnt;  // entries appended this step
+    uint32_t off;   // first entry of this list region
+    uint32_t cap;   // entries the region holds
+};
+
+// Lists the members of [beg, end) of both lists and appends them to their
+// regions.  Row k of the block holds slots beg + k * 256 + thread (coalesced
+// loads); a member's place in the block's list is its (row, wave, lane)
+// rank.  Every thread of the block (256) calls it.
+template <typename PredP, typename PredD>
+__device__ __forceinline__ void wl_build_block(int beg, int end, const PredP& in_pass,
+                                               const PredD& in_dye, WlRun Rp, WlRun Rd) {
+    constexpr int kW = 4;  // wavefronts per block
+    __shared__ int32_t s_list[2][kWlSlots];
+    __shared__ int s_cnt[2][kWlPer * kW];  // members per (row, wave), then their offsets
+    __shared__ uint32_t s_base[2];
+    const int lane = lane_id(), w = (int)(threadIdx.x >> 6);
+    uint64_t bal[2][kWlPer];
+#pragma unroll
+    for (int k = 0; k < kWlPer; ++k) {
+        const int t = beg + k * 256 + (int)threadIdx.x;
+        const bool live = t < end;
+        bal[0][k] = __ballot(live && in_pass(t));
+        bal[1][k] = __ballot(live && in_dye(t));
+        if (lane == 0) {
+            s_cnt[0][k * kW + w] = __popcll(bal[0][k]);
+            s_cnt[1][k * kW + w] = __popcll(bal[1][k]);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {  // exclusive prefix of the counts, list by list
+        int run = 0;
+        for (int i = 0; i < kWlPer * kW; ++i) {
+            const int c = s_cnt[threadIdx.x][i];
+            s_cnt[threadIdx.x][i] = run;
+            run += c;
+        }
+        s_base[threadIdx.x] = run;  // the block's members, for now
+    }
+    __syncthreads();
+    const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int k = 0; k < kWlPer; ++k) {
+        const int t = beg + k * 256 + (int)threadIdx.x;
+#pragma unroll
+        for (int L = 0; L < 2; ++L)
+            if ((bal[L][k] >> lane) & 1ull)
+                s_list[L][s_cnt[L][k * kW + w] + __popcll(bal[L][k] & below)] = t;
+    }
+    __syncthreads();
+    const int mp = (int)s_base[0], md = (int)s_base[1];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s_base[0] = mp ? atomicAdd(Rp.cnt, (uint32_t)mp) : 0u;
+        s_base[1] = md ? atomicAdd(Rd.cnt, (uint32_t)md) : 0u;
+    }
+    __syncthreads();
+    for (int i = (int)threadIdx.x; i < mp; i += 256) {
+        const uint32_t k = s_base[0] + (uint32_t)i;
+        if (k < Rp.cap) Rp.list[Rp.off + k] = s_list[0][i];
+    }
+    for (int i = (int)threadIdx.x; i < md; i += 256) {
+        const uint32_t k = s_base[1] + (uint32_t)i;
+        if (k < Rd.cap) Rd.list[Rd.off + k] = s_list[1][i];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_wl_build(int n, const int8_t* __restrict__ sp,
+                                                  const uint8_t* __restrict__ fl,
+                                                  uint32_t pass_species, WlRun Rp, WlRun Rd) {
+    const int beg = (int)blockIdx.x * kWlSlots;
+    wl_build_block(
+        beg, min(n, beg + kWlSlots), [&](int t) { return ((pass_species >> sp[t]) & 1u) != 0u; },
+        [&](int t) { return (fl[t] & EDM_FLAG_DYE) != 0; }, Rp, Rd);
 }
 
 // Reference pair-list bookkeeping: per particle index, the number of
@@ -1039,6 +1127,35 @@ void plan_passes(const std::vector<edm_reaction>& rx, uint32_t sensor_mask,
     close();
 }
 
+// Species the passes act on, closed under every reaction that turns one
+// species into another: a particle outside the set is never touched by a
+// pass and never enters the set, so the number of particles in the set is
+// fixed and the passes run over a list of them.
+uint32_t pass_species_of(const std::vector<edm_reaction>& rx,
+                         const std::vector<std::vector<int>>& groups) {
+    uint32_t S = 0;
+    for (auto& g : groups)
+        for (int i : g) {
+            const edm_reaction& r = rx[i];
+            S |= 1u << r.a;
+            if (r.kind == EDM_RX_CONVERT) S |= 1u << r.b;
+        }
+    for (bool grow = S != 0; grow;) {
+        grow = false;
+        for (auto& r : rx) {
+            uint32_t T = 0;
+            if (r.kind == EDM_RX_CAT) T = (1u << r.b) | (1u << r.c);
+            else if (r.kind == EDM_RX_BIND) T = (1u << r.a) | (1u << r.b) | (1u << r.c);
+            else if (r.kind == EDM_RX_CONVERT) T = (1u << r.a) | (1u << r.b);
+            if ((T & S) && (T & ~S)) {
+                S |= T;
+                grow = true;
+            }
+        }
+    }
+    return S;
+}
+
 template <typename Real>
 static int create_t(edm_membrane* m) {
     const edm_params& p = m->p;
@@ -1127,33 +1244,7 @@ static int create_t(edm_membrane* m) {
         EDM_HIP(hipMemset(m->w_adapt, 0, (size_t)g.nx * g.ny * sizeof(double)));
     }
     plan_passes(m->rx, m->sensor_mask_all, m->pass_groups, m->group_start, m->pass_deferred);
-    // Species the passes act on, closed under every reaction that turns one
-    // species into another: a particle outside the set is never touched by a
-    // pass and never enters the set, so the passes can run over a list of
-    // particle indices fixed at upload.
-    {
-        uint32_t S = 0;
-        for (auto& g : m->pass_groups)
-            for (int i : g) {
-                const edm_reaction& r = m->rx[i];
-                S |= 1u << r.a;
-                if (r.kind == EDM_RX_CONVERT) S |= 1u << r.b;
-            }
-        for (bool grow = S != 0; grow;) {
-            grow = false;
-            for (auto& r : m->rx) {
-                uint32_t T = 0;
-                if (r.kind == EDM_RX_CAT) T = (1u << r.b) | (1u << r.c);
-                else if (r.kind == EDM_RX_BIND) T = (1u << r.a) | (1u << r.b) | (1u << r.c);
-                else if (r.kind == EDM_RX_CONVERT) T = (1u << r.a) | (1u << r.b);
-                if ((T & S) && (T & ~S)) {
-                    S |= T;
-                    grow = true;
-                }
-            }
-        }
-        m->pass_species = S;
-    }
+    m->pass_species = pass_species_of(m->rx, m->pass_groups);
     // largest squared range of any pair potential (the tiled force kernel
     // skips candidates beyond it before looking up the species tables)
     {
@@ -1201,9 +1292,10 @@ static int begin_t(edm_membrane* m) {
     const int n = m->n;
     hipStream_t st = m->stream;
     // bin on the current positions, counting sort, order cells by index
-    hipLaunchKernelGGL(k_bin<Real>, dim3(nblk(std::max(n, m->n_count))), dim3(kBlock), 0, st, n,
-                       (const Real*)m->a.x, (const Real*)m->a.y, (Real)m->p.csx, (Real)m->p.csy,
-                       m->p.ncx, m->p.ncy, m->key, m->rank, m->cell_cnt, m->counts, m->n_count);
+    hipLaunchKernelGGL(k_bin<Real>, dim3(nblk(std::max({n, m->n_count, 2}))), dim3(kBlock), 0,
+                       st, n, (const Real*)m->a.x, (const Real*)m->a.y, (Real)m->p.csx,
+                       (Real)m->p.csy, m->p.ncx, m->p.ncy, m->key, m->rank, m->cell_cnt,
+                       m->counts, m->n_count, m->wl_cnt);
     size_t tb = m->cub_tmp_bytes;
     EDM_HIP(hipcub::DeviceScan::ExclusiveSum(m->cub_tmp, tb, m->cell_cnt, m->cell_start,
                                              m->ncell + 1, st));
@@ -1213,6 +1305,14 @@ static int begin_t(edm_membrane* m) {
                        m->cell_start, m->perm, m->cell_cnt, (const int32_t*)m->a.id);
     hipLaunchKernelGGL(k_gather<Real>, dim3(nblk(n)), dim3(kBlock), 0, st, n,
                        (const uint32_t*)m->perm, m->a);
+    if (m->wl_cnt && n > 0) {
+        // the gather wrote the new order into the _t arrays (swapped below)
+        hipLaunchKernelGGL(k_wl_build, dim3(nblk(n, kWlSlots)), dim3(256), 0, st, n,
+                           (const int8_t*)m->a.sp_t, (const uint8_t*)m->a.fl_t, m->pass_species,
+                           WlRun{m->wl_pass, m->wl_cnt, 0u, (uint32_t)m->n_pass},
+                           WlRun{m->wl_dye, m->wl_cnt + 1, 0u, (uint32_t)m->n_dye});
+        m->wl_valid = true;
+    }
     std::swap(m->a.x, m->a.xt);
     std::swap(m->a.y, m->a.yt);
     std::swap(m->a.sp, m->a.sp_t);
@@ -1429,9 +1529,11 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
                         d.reach = reach;
                         grp.species |= reach;
                     }
-                    const int nt = m->pass_ids ? m->n_pass : n;
+                    const bool wl = m->wl_valid;
+                    const int nt = wl ? m->n_pass : n;
                     hipLaunchKernelGGL(k_passes<Real>, dim3(nblk(std::max(nt, 1))), dim3(kBlock), 0,
-                                       st, nt, n, (const int32_t*)m->pass_ids,
+                                       st, nt, n, wl ? (const int32_t*)m->wl_pass : nullptr,
+                                       wl ? (const uint32_t*)m->wl_cnt : nullptr,
                                        (const int32_t*)m->a.id, P, grp, hook ? draws : nullptr,
                                        rng, step, (Real)p.Lx, (Real)p.Ly, m->d_ctr, G, ng,
                                        m->cand_id, m->cand_slot, m->cand_x, m->cand_y, m->n_cand);

@@ -22,7 +22,8 @@ static void ens_free(edm_ensemble* e) {
                     e->cell_start, e->cub_tmp, e->counts, e->delta, e->cand_id, e->cand_slot,
                     e->cand_x, e->cand_y, e->n_cand, e->ctr, e->frame_out, e->sig, e->eps, e->kh,
                     e->Dbase, e->d_pp, e->d_seg, e->d_tiles, e->noise_ctr, e->d_slot_list,
-                    e->d_cats, e->d_pgrp};
+                    e->d_cats, e->d_pgrp, e->wl_pass, e->wl_dye, e->wl_rcnt,
+                    e->wl_wrep_pass, e->wl_wrep_dye, e->d_wl_chunks};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (auto& g : e->seg)
@@ -237,6 +238,13 @@ int edm_ens_add(edm_ensemble* e, const edm_replica* r, const edm_host_state* s, 
     EDM_HIP(hipMemset(e->noise_ctr + 2 * (size_t)k, 0, 2 * sizeof(unsigned long long)));
     e->seg[k] = g;
     e->fresh_beg[k] = beg;
+    int64_t npass = 0, ndye = 0;
+    for (int i = 0; i < n; ++i) {
+        npass += (e->pass_species >> sp[i]) & 1u;
+        ndye += (fl[i] & EDM_FLAG_DYE) ? 1 : 0;
+    }
+    e->seg_npass[k] = npass;
+    e->seg_ndye[k] = ndye;
     e->n_total += n;
     e->next_id += n;
     e->layout_dirty = true;

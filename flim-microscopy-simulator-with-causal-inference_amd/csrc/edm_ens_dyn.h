@@ -87,9 +87,10 @@ __global__ void k_ens_bin(int n, const Real* __restrict__ x, const Real* __restr
                           const uint16_t* __restrict__ rep, const EnsSeg* __restrict__ segs,
                           uint32_t sentinel, uint32_t* __restrict__ key,
                           uint32_t* __restrict__ rank, uint32_t* __restrict__ cnt,
-                          int32_t* counts, int n_counts) {
+                          int32_t* counts, int n_counts, uint32_t* wl_rcnt, int n_rcnt) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < n_counts) counts[s] = 0;
+    if (s < n_rcnt) wl_rcnt[s] = 0;  // work lists are rebuilt by the gather
     const bool live = s < n;
     uint32_t c = 0xffffffffu;
     if (live) {
@@ -134,6 +135,23 @@ __global__ void k_ens_gather(int n, const uint32_t* __restrict__ perm, DevArrays
     a.ctmc_t[t] = a.ctmc[s];
     a.dye_t[t] = a.dye[s];
     rep_t[t] = rep[s];
+}
+
+// Work lists of the live replicas: block x lists the pass-species slots and
+// the dye slots of chunk x (one replica's slots) into that replica's regions.
+__global__ __launch_bounds__(256) void k_ens_wl_build(const EnsChunk* __restrict__ chunks,
+                                                      const EnsSeg* __restrict__ segs,
+                                                      const int8_t* __restrict__ sp,
+                                                      const uint8_t* __restrict__ fl,
+                                                      uint32_t pass_species, int32_t* wl_pass,
+                                                      int32_t* wl_dye, uint32_t* rcnt) {
+    const EnsChunk C = chunks[blockIdx.x];
+    const EnsSeg& g = segs[C.r];
+    wl_build_block(
+        C.beg, C.end, [&](int t) { return ((pass_species >> sp[t]) & 1u) != 0u; },
+        [&](int t) { return (fl[t] & EDM_FLAG_DYE) != 0; },
+        WlRun{wl_pass, rcnt + 2 * C.r, (uint32_t)g.wl_off[0], (uint32_t)g.wl_n[0]},
+        WlRun{wl_dye, rcnt + 2 * C.r + 1, (uint32_t)g.wl_off[1], (uint32_t)g.wl_n[1]});
 }
 
 template <typename Real>
@@ -200,42 +218,38 @@ __global__ __launch_bounds__(256) void k_ens_cat(const EnsTile* __restrict__ til
 // A run of per-particle passes over the strip's own particles, with the
 // run's parameter block of the strip's replica.
 template <typename Real>
-__global__ __launch_bounds__(256) void k_ens_passes(const EnsTile* __restrict__ tiles,
-                                                    int ntiles, const EnsSeg* __restrict__ segs,
+__global__ __launch_bounds__(256) void k_ens_passes(int n_waves, const int32_t* __restrict__ list,
+                                                    const int32_t* __restrict__ wrep,
+                                                    const uint32_t* __restrict__ rcnt,
+                                                    const EnsSeg* __restrict__ segs,
                                                     EnsPool<Real> Q,
                                                     const PassGroupDev* __restrict__ pg,
                                                     int n_groups, int gi,
                                                     unsigned long long* ctr, int32_t* cid,
                                                     int32_t* cslot, double* cxo, double* cyo,
                                                     unsigned int* nc, uint32_t gstep) {
-    const int tile = xcd_tile(ntiles);
-    if (tile >= ntiles) return;
-    const EnsTile T = tiles[tile];
-    const EnsSeg& g = segs[T.r];
-    const PassGroupDev& grp = pg[(size_t)T.r * n_groups + gi];
-    const StripGeo S = ens_strip<Real>(g, T);
-    const uint32_t* st = Q.start + g.coff;
-    const int c0 = S.cx * g.ncy + S.cy0;
-    const int b = (int)st[c0], e = (int)st[c0 + S.rows];
-    // this replica's candidate region is its own slot range
-    const int pbeg = (int)st[0];
-    const Grids<Real> G = ens_grids<Real>(g, Q.counts, Q.delta, Q.ns);
-    const Post<Real> P = ens_post(Q);
-    const Rng rng = ens_rng(g);
-    const uint32_t step = gstep - g.step0;
-    const uint32_t species = grp.species;
+    if ((int)blockIdx.x * (int)(blockDim.x >> 6) +
+            __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) >= n_waves)
+        return;  // whole waves
+    const EnsWave W = ens_wave(wrep, segs, 0);
+    const EnsSeg& g = segs[W.r];
+    const PassGroupDev& grp = pg[(size_t)W.r * n_groups + gi];
+    const int j = W.j + lane_id();
     int det = 0, conv = 0;
-    for (int t0 = b; t0 < e; t0 += blockDim.x) {
-        const int t = t0 + (int)threadIdx.x;
-        if (t < e) {
-            const int s0 = Q.sp[t];
-            if ((species >> s0) & 1u)
-                passes_particle(t, s0, g.n, Q.id, g.kbase, P, grp, (const double*)nullptr, rng,
-                                step, (Real)g.Lx, (Real)g.Ly, G, Q.n_grids, cid + pbeg,
-                                cslot + pbeg, cxo + pbeg, cyo + pbeg, nc + T.r, det, conv);
+    if (j < min(g.wl_n[0], (int)rcnt[2 * W.r])) {
+        const int t = list[g.wl_off[0] + j];
+        const int s0 = Q.sp[t];
+        if ((grp.species >> s0) & 1u) {
+            const Grids<Real> G = ens_grids<Real>(g, Q.counts, Q.delta, Q.ns);
+            // this replica's candidate region is its own slot range
+            const int pbeg = (int)Q.start[g.coff];
+            passes_particle(t, s0, g.n, Q.id, g.kbase, ens_post(Q), grp, (const double*)nullptr,
+                            ens_rng(g), gstep - g.step0, (Real)g.Lx, (Real)g.Ly, G, Q.n_grids,
+                            cid + pbeg, cslot + pbeg, cxo + pbeg, cyo + pbeg, nc + W.r, det,
+                            conv);
         }
     }
-    passes_count<Real>(det, conv, ctr + 8 * T.r);
+    passes_count<Real>(det, conv, ctr + 8 * W.r);
 }
 
 // Index-order recruitment cap of one recruitment reaction, one block per
@@ -381,6 +395,13 @@ static int ens_create_t(edm_ensemble* e) {
         dalloc(&e->noise_ctr, 2 * (size_t)e->rmax) ||
         dalloc(&e->d_seg, e->rmax))
         return EDM_EHIP;
+    // each replica's region of a list is padded to whole waves
+    e->wl_cap = cap + 64 * (int64_t)e->rmax;
+    if (dalloc(&e->wl_pass, e->wl_cap) || dalloc(&e->wl_dye, e->wl_cap) ||
+        dalloc(&e->wl_wrep_pass, e->wl_cap / 64) || dalloc(&e->wl_wrep_dye, e->wl_cap / 64) ||
+        dalloc(&e->wl_rcnt, 2 * (size_t)e->rmax))
+        return EDM_EHIP;
+    EDM_HIP(hipMemset(e->wl_rcnt, 0, 2 * (size_t)e->rmax * sizeof(uint32_t)));
     EDM_HIP(hipMemset(e->n_cand, 0, (size_t)e->rmax * sizeof(unsigned int)));
     EDM_HIP(hipMemset(e->ctr, 0, (size_t)e->rmax * 8 * sizeof(unsigned long long)));
     EDM_HIP(hipMemset(e->noise_ctr, 0, (size_t)e->rmax * 2 * sizeof(unsigned long long)));
@@ -413,6 +434,9 @@ static int ens_create_t(edm_ensemble* e) {
                 e->sensor_mask_all |= 1u << s;
             }
     plan_passes(e->rx, e->sensor_mask_all, e->pass_groups, e->group_start, e->pass_deferred);
+    e->pass_species = pass_species_of(e->rx, e->pass_groups);
+    e->seg_npass.assign(e->rmax, 0);
+    e->seg_ndye.assign(e->rmax, 0);
     // catalysis runs: back-to-back reactions with one substrate and product and
     // no feedback share a launch (as for one membrane)
     e->cat_runs.clear();
@@ -450,9 +474,16 @@ int ens_layout(edm_ensemble* e) {
     const int ns = e->p.n_species;
     int64_t coff = 0, cnt = 0;
     e->tiles.clear();
+    std::vector<int32_t> wrep[2];
     for (int r = 0; r < e->rmax; ++r) {
         EnsSeg& g = e->seg[r];
         if (!g.active) continue;
+        const int64_t nl[2] = {e->seg_npass[r], e->seg_ndye[r]};
+        for (int L = 0; L < 2; ++L) {
+            g.wl_off[L] = (int32_t)(64 * wrep[L].size());
+            g.wl_n[L] = (int32_t)nl[L];
+            wrep[L].insert(wrep[L].end(), (size_t)((nl[L] + 63) / 64), (int32_t)r);
+        }
         g.coff = (int32_t)coff;
         coff += (int64_t)g.ncx * g.ncy;
         for (int k = 0; k < kEnsMaxGrids * kEnsMaxSpecies; ++k) g.goff[k] = -1;
@@ -500,6 +531,39 @@ int ens_layout(edm_ensemble* e) {
     if (!e->tiles.empty())
         EDM_HIP(hipMemcpyAsync(e->d_tiles, e->tiles.data(), e->tiles.size() * sizeof(EnsTile),
                                hipMemcpyHostToDevice, e->stream));
+    // chunks of the live replicas' slot ranges for the list build: after the
+    // sort the live replicas occupy consecutive slot ranges in replica order,
+    // whose lengths (alive and dormant particles) do not change
+    e->wl_chunks.clear();
+    for (int64_t r = 0, beg = 0; r < e->rmax; ++r) {
+        if (!e->seg[r].active) continue;
+        const int64_t end = beg + e->seg[r].n;
+        if (e->seg_npass[r] + e->seg_ndye[r] > 0)
+            for (int64_t b = beg; b < end; b += kWlSlots)
+                e->wl_chunks.push_back(EnsChunk{(int32_t)r, (int32_t)b,
+                                                (int32_t)std::min<int64_t>(end, b + kWlSlots), 0});
+        beg = end;
+    }
+    if ((int64_t)e->wl_chunks.size() > e->wl_chunk_cap) {
+        hipFree(e->d_wl_chunks);
+        e->wl_chunk_cap = std::max<int64_t>((int64_t)e->wl_chunks.size(), e->wl_chunk_cap * 3 / 2);
+        if (dalloc(&e->d_wl_chunks, e->wl_chunk_cap)) return EDM_EHIP;
+    }
+    if (!e->wl_chunks.empty())
+        EDM_HIP(hipMemcpyAsync(e->d_wl_chunks, e->wl_chunks.data(),
+                               e->wl_chunks.size() * sizeof(EnsChunk), hipMemcpyHostToDevice,
+                               e->stream));
+    e->wl_npass = 64 * (int64_t)wrep[0].size();
+    e->wl_ndye = 64 * (int64_t)wrep[1].size();
+    if (e->wl_npass > e->wl_cap || e->wl_ndye > e->wl_cap) {
+        set_error("edm_ensemble: work lists exceed their capacity");
+        return EDM_ECAPACITY;
+    }
+    int32_t* wdev[2] = {e->wl_wrep_pass, e->wl_wrep_dye};
+    for (int L = 0; L < 2; ++L)
+        if (!wrep[L].empty())
+            EDM_HIP(hipMemcpyAsync(wdev[L], wrep[L].data(), wrep[L].size() * sizeof(int32_t),
+                                   hipMemcpyHostToDevice, e->stream));
     EDM_HIP(hipMemcpyAsync(e->d_seg, e->seg.data(), (size_t)e->rmax * sizeof(EnsSeg),
                            hipMemcpyHostToDevice, e->stream));
     EDM_HIP(hipStreamSynchronize(e->stream));
@@ -519,10 +583,12 @@ static int ens_step_t(edm_ensemble* e) {
     const uint32_t sentinel = (uint32_t)e->n_cells;
     if (n > 0) {
         const int64_t ncnt = e->n_cells + 2;  // live keys, sentinel, scan end
-        hipLaunchKernelGGL(k_ens_bin<Real>, dim3(nblk(std::max<int64_t>(n, e->n_counts))),
+        hipLaunchKernelGGL(k_ens_bin<Real>,
+                           dim3(nblk(std::max<int64_t>({n, e->n_counts, 2 * (int64_t)e->rmax}))),
                            dim3(kBlock), 0, st, n, (const Real*)e->a.x, (const Real*)e->a.y,
                            (const uint16_t*)e->rep, (const EnsSeg*)e->d_seg, sentinel, e->key,
-                           e->rank, e->cell_cnt, e->counts, (int)e->n_counts);
+                           e->rank, e->cell_cnt, e->counts, (int)e->n_counts, e->wl_rcnt,
+                           2 * e->rmax);
         size_t tb = e->cub_bytes;
         EDM_HIP(hipcub::DeviceScan::ExclusiveSum(e->cub_tmp, tb, e->cell_cnt, e->cell_start,
                                                  (int)ncnt, st));
@@ -538,6 +604,12 @@ static int ens_step_t(edm_ensemble* e) {
                            (unsigned int*)(e->cell_cnt + e->n_cells));
         hipLaunchKernelGGL(k_ens_gather<Real>, dim3(nblk(n)), dim3(kBlock), 0, st, n,
                            (const uint32_t*)e->perm, e->a, (const uint16_t*)e->rep, e->rep_t);
+        // the gather wrote the new order into the _t arrays (swapped below)
+        if (!e->wl_chunks.empty())
+            hipLaunchKernelGGL(k_ens_wl_build, dim3((unsigned)e->wl_chunks.size()), dim3(256), 0,
+                               st, (const EnsChunk*)e->d_wl_chunks, (const EnsSeg*)e->d_seg,
+                               (const int8_t*)e->a.sp_t, (const uint8_t*)e->a.fl_t,
+                               e->pass_species, e->wl_pass, e->wl_dye, e->wl_rcnt);
         std::swap(e->a.x, e->a.xt);
         std::swap(e->a.y, e->a.yt);
         std::swap(e->a.sp, e->a.sp_t);
@@ -617,11 +689,13 @@ static int ens_step_t(edm_ensemble* e) {
             const std::vector<int>& grp_ix = e->pass_groups[gi];
             bool defer_any = false;
             for (int k : grp_ix) defer_any = defer_any || e->pass_deferred[k];
-            hipLaunchKernelGGL((k_ens_passes<Real>), tgrid, dim3(256), 0, st,
-                               (const EnsTile*)e->d_tiles, ntiles, (const EnsSeg*)e->d_seg, Q,
-                               (const PassGroupDev*)e->d_pgrp, (int)e->pass_groups.size(), gi,
-                               e->ctr, e->cand_id, e->cand_slot, e->cand_x, e->cand_y,
-                               e->n_cand, gstep);
+            if (e->wl_npass > 0)
+                hipLaunchKernelGGL((k_ens_passes<Real>), dim3(nblk(e->wl_npass)), dim3(256), 0,
+                                   st, (int)(e->wl_npass / 64), (const int32_t*)e->wl_pass,
+                                   (const int32_t*)e->wl_wrep_pass, (const uint32_t*)e->wl_rcnt,
+                                   (const EnsSeg*)e->d_seg, Q, (const PassGroupDev*)e->d_pgrp,
+                                   (int)e->pass_groups.size(), gi, e->ctr, e->cand_id,
+                                   e->cand_slot, e->cand_x, e->cand_y, e->n_cand, gstep);
             const edm_reaction& last = e->rx[grp_ix.back()];
             if (last.kind == EDM_RX_RECRUIT)
                 hipLaunchKernelGGL((k_ens_recruit<Real>), dim3(e->rmax), dim3(1024), 0, st,

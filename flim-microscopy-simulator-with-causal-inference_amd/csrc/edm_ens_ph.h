@@ -1,52 +1,51 @@
 // Batched ensemble: photon emission + imaging and frame end (included at
-// the end of edm_photons.hip, inside namespace edm).  Each strip's own dye
-// carriers emit with their replica's draws, image geometry and TCSPC stack.
+// the end of edm_photons.hip, inside namespace edm).  The dye carriers of
+// the step's work list emit with their replica's draws, image geometry and
+// TCSPC stack.
 
 template <typename Real>
-__global__ __launch_bounds__(256) void k_ens_photons(const EnsTile* __restrict__ tiles, int ntiles,
+__global__ __launch_bounds__(256) void k_ens_photons(int n_waves, const int32_t* __restrict__ list,
+                                                     const int32_t* __restrict__ wrep,
+                                                     const uint32_t* __restrict__ rcnt,
                                                      const EnsSeg* __restrict__ segs,
                                                      const edm_photo_params* pp, const Real* x,
                                                      const Real* y, uint8_t* fl, const int8_t* dye,
                                                      int8_t* ctmc, const int32_t* __restrict__ id,
-                                                     const uint32_t* __restrict__ start,
                                                      unsigned long long* ctr, uint32_t gstep) {
-    const int tile = blockIdx.x;
-    if (tile >= ntiles) return;
-    const EnsTile T = tiles[tile];
-    const EnsSeg& g = segs[T.r];
-    const int rows = min(kTileRows, g.ncy - T.cy0);
-    const int c0 = g.coff + T.cx * g.ncy + T.cy0;
-    const int b = (int)start[c0], e = (int)start[c0 + rows];
-    const Rng rng{g.seed_lo ^ 0x5bd1e995u, g.seed_hi};
-    const uint32_t step = gstep - g.step0;
+    if ((int)blockIdx.x * (int)(blockDim.x >> 6) +
+            __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) >= n_waves)
+        return;  // whole waves
+    const EnsWave W = ens_wave(wrep, segs, 1);
+    const EnsSeg& g = segs[W.r];
+    const int j = W.j + lane_id();
     int emitted = 0, detected = 0;
-    for (int t0 = b; t0 < e; t0 += blockDim.x) {
-        const int t = t0 + (int)threadIdx.x;
-        if (t < e && (fl[t] & EDM_FLAG_DYE))
-            emit_image<Real>(pp, &g.ip, t, (uint32_t)(id[t] - g.kbase), x, y, fl, dye, ctmc,
-                             nullptr, nullptr, true, 0.0, -1, g.flim, rng, step, emitted,
-                             detected);
+    if (j < min(g.wl_n[1], (int)rcnt[2 * W.r + 1])) {
+        const int t = list[g.wl_off[1] + j];
+        const Rng rng{g.seed_lo ^ 0x5bd1e995u, g.seed_hi};
+        emit_image<Real>(pp, &g.ip, t, (uint32_t)(id[t] - g.kbase), x, y, fl, dye, ctmc, nullptr,
+                         nullptr, true, 0.0, -1, g.flim, rng, gstep - g.step0, emitted,
+                         detected);
     }
-    photon_count(emitted, detected, ctr + 8 * T.r);
+    photon_count(emitted, detected, ctr + 8 * W.r);
 }
 
 int ens_photons(edm_ensemble* e) {
-    const int ntiles = (int)e->tiles.size();
-    if (ntiles == 0 || !e->has_pp) return 0;
+    const int nt = (int)e->wl_ndye;
+    if (nt == 0 || !e->has_pp) return 0;
     if (e->fp64)
-        hipLaunchKernelGGL(k_ens_photons<double>, dim3(ntiles), dim3(256), 0, e->stream,
-                           (const EnsTile*)e->d_tiles, ntiles, (const EnsSeg*)e->d_seg,
+        hipLaunchKernelGGL(k_ens_photons<double>, dim3(nb(nt)), dim3(256), 0, e->stream, nt / 64,
+                           (const int32_t*)e->wl_dye, (const int32_t*)e->wl_wrep_dye,
+                           (const uint32_t*)e->wl_rcnt, (const EnsSeg*)e->d_seg,
                            (const edm_photo_params*)e->d_pp, (const double*)e->a.x,
                            (const double*)e->a.y, e->a.fl, (const int8_t*)e->a.dye, e->a.ctmc,
-                           (const int32_t*)e->a.id, (const uint32_t*)e->cell_start, e->ctr,
-                           e->gstep);
+                           (const int32_t*)e->a.id, e->ctr, e->gstep);
     else
-        hipLaunchKernelGGL(k_ens_photons<float>, dim3(ntiles), dim3(256), 0, e->stream,
-                           (const EnsTile*)e->d_tiles, ntiles, (const EnsSeg*)e->d_seg,
+        hipLaunchKernelGGL(k_ens_photons<float>, dim3(nb(nt)), dim3(256), 0, e->stream, nt / 64,
+                           (const int32_t*)e->wl_dye, (const int32_t*)e->wl_wrep_dye,
+                           (const uint32_t*)e->wl_rcnt, (const EnsSeg*)e->d_seg,
                            (const edm_photo_params*)e->d_pp, (const float*)e->a.x,
                            (const float*)e->a.y, e->a.fl, (const int8_t*)e->a.dye, e->a.ctmc,
-                           (const int32_t*)e->a.id, (const uint32_t*)e->cell_start, e->ctr,
-                           e->gstep);
+                           (const int32_t*)e->a.id, e->ctr, e->gstep);
     EDM_HIP(hipGetLastError());
     return 0;
 }

@@ -44,6 +44,31 @@ struct EnsSeg {
     edm_image_params ip;  // the model's imaging constants with this replica's geometry
     uint32_t* flim;       // this replica's TCSPC stack
     int64_t flim_elems;
+    // work lists: this replica's region of the pass list [0] and of the dye
+    // list [1] starts at entry wl_off (a multiple of the wavefront size, so
+    // that every wave of a list launch belongs to one replica) and holds
+    // wl_n entries
+    int32_t wl_off[2];
+    int32_t wl_n[2];
+};
+
+// The wave's replica and its first entry of list L: every wave of a list
+// launch lies inside one replica's region.
+struct EnsWave {
+    int r, j;  // replica; entry index of lane 0 within the replica's region
+};
+
+__device__ __forceinline__ EnsWave ens_wave(const int32_t* __restrict__ wrep,
+                                            const EnsSeg* __restrict__ segs, int L) {
+    const int w = (int)blockIdx.x * (int)(blockDim.x >> 6) +
+                  __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int r = wrep[w];
+    return EnsWave{r, 64 * w - segs[r].wl_off[L]};
+}
+
+// A run of one live replica's slots [beg, end) (the work-list build's unit).
+struct EnsChunk {
+    int32_t r, beg, end, pad;
 };
 
 // Strip tile of one replica.
@@ -115,6 +140,19 @@ struct edm_ensemble {
     size_t cats_bytes = 0;
     void* d_pgrp = nullptr;
     size_t pgrp_bytes = 0;
+    // work lists rebuilt by every step's gather over the live replicas: the
+    // slots of pass-species particles and of dye carriers, in one region per
+    // replica padded to whole waves
+    uint32_t pass_species = 0;
+    int32_t *wl_pass = nullptr, *wl_dye = nullptr;  // [wl_cap]
+    int64_t wl_cap = 0;
+    uint32_t* wl_rcnt = nullptr;                     // [rmax][2] entries appended this step
+    int32_t *wl_wrep_pass = nullptr, *wl_wrep_dye = nullptr;  // replica of each wave
+    std::vector<int64_t> seg_npass, seg_ndye;  // per slot: entries it contributes
+    int64_t wl_npass = 0, wl_ndye = 0;         // padded list lengths (launch sizes)
+    std::vector<EnsChunk> wl_chunks;           // the list build's blocks
+    EnsChunk* d_wl_chunks = nullptr;
+    int64_t wl_chunk_cap = 0;
 };
 
 namespace edm {
@@ -128,4 +166,6 @@ int ens_frame_end(edm_ensemble* e, const int32_t* slots, int32_t n);
 void plan_passes(const std::vector<edm_reaction>& rx, uint32_t sensor_mask,
                  std::vector<std::vector<int>>& groups, std::vector<int>& group_start,
                  std::vector<int>& deferred);
+uint32_t pass_species_of(const std::vector<edm_reaction>& rx,
+                         const std::vector<std::vector<int>>& groups);
 }  // namespace edm

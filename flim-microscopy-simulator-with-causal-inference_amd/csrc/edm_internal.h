@@ -99,8 +99,14 @@ struct edm_membrane {
     std::vector<std::vector<int>> pass_groups;
     uint32_t pass_species = 0;    // species closed under every transformation that
                                   // touches a pass species (bit per species)
-    int32_t* pass_ids = nullptr;  // particle indices of those species (fixed at upload)
-    int n_pass = 0;               // 0 with pass_ids null: passes run over every slot
+    int n_pass = 0;               // particles of those species (fixed at upload)
+    // Work lists rebuilt by every step's gather: the slots of pass-species
+    // particles and the slots of dye carriers (valid once a step has run
+    // since the last upload)
+    int32_t* wl_pass = nullptr;  // [max(n_pass, 1)]
+    int32_t* wl_dye = nullptr;   // [max(n_dye, 1)]
+    uint32_t* wl_cnt = nullptr;  // [2] entries of this step's lists
+    bool wl_valid = false;
     std::vector<int> group_start;    // per reaction: group starting here, else -1
     std::vector<int> pass_deferred;  // per reaction: reads a count it writes
 
