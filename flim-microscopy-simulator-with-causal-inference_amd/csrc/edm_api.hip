@@ -190,6 +190,9 @@ int edm_upload_state(edm_membrane* m, const edm_host_state* s) {
     m->n_dye = (int)dyes.size();
     // particles of the species the per-particle passes can touch; the
     // gather of every step lists their slots (and the dye carriers' slots)
+    m->alive_at_upload.assign(m->p.n_species, 0);
+    for (int i = 0; i < n; ++i)
+        if (fl[i] & EDM_FLAG_ALIVE) ++m->alive_at_upload[sp[i]];
     int n_pass = 0;
     if (m->pass_species)
         for (int i = 0; i < n; ++i)
@@ -204,6 +207,33 @@ int edm_upload_state(edm_membrane* m, const edm_host_state* s) {
     EDM_HIP(hipMemset(m->wl_cnt, 0, 2 * sizeof(uint32_t)));
     m->wl_valid = false;
     return 0;
+}
+
+int edm_set_tuning(edm_membrane* m, int32_t key, int64_t value) {
+    if (!m) return invalid("edm_set_tuning: null membrane");
+    switch (key) {
+        case EDM_TUNE_TILE_ROWS:
+            if (value < 1 || value > kTileRows)
+                return invalid("edm_set_tuning: tile rows must be in 1..64");
+            m->tile_rows = (int)value;
+            return 0;
+        case EDM_TUNE_CAT_SIDE:
+            if (value < -1 || value > 1)
+                return invalid("edm_set_tuning: catalysis side must be -1, 0 or 1");
+            m->cat_side = (int)value;
+            return 0;
+        default:
+            return invalid("edm_set_tuning: unknown key");
+    }
+}
+
+int edm_get_tuning(edm_membrane* m, int32_t key, int64_t* value) {
+    if (!m || !value) return invalid("edm_get_tuning: null argument");
+    switch (key) {
+        case EDM_TUNE_TILE_ROWS: *value = m->tile_rows; return 0;
+        case EDM_TUNE_CAT_SIDE: *value = m->cat_side; return 0;
+        default: return invalid("edm_get_tuning: unknown key");
+    }
 }
 
 int edm_download_state(edm_membrane* m, edm_host_state* s) {

@@ -2,7 +2,11 @@
 // after the reaction helpers.  Each workgroup stages its strip's
 // neighbourhood (pre-move and post-move positions, species, flags), lists
 // the strip's own alive enzymes in LDS and tests every enzyme against the
-// substrates of its 3x3 cells, one wavefront per enzyme.  A pair qualifies
+// substrates of its 3x3 cells, one wavefront per enzyme.  When substrates
+// are the rarer side the walk starts from them instead (own alive
+// substrates, one wavefront each, against the enzymes of their 3x3 cells):
+// the set of pairs, each pair's uniform and hence the outcome are the same
+// either way, and every launch uses one side for all strips.  A pair qualifies
 // as in the reference: in the (capped) pair list built on pre-move
 // positions (d^2 < cutoff^2), within the effective contact radius after the
 // move, and accepted by its uniform against P_cat x Hill(rho at the substrate) [x adaptation].  The
@@ -33,6 +37,7 @@ struct CatArgs {
     Rng rng;
     uint32_t step;
     bool deferred;
+    bool by_substrate;  // walk the pairs from the substrates' side (the rarer one)
 };
 
 template <typename T>
@@ -149,12 +154,21 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
     __syncthreads();
     const int own_begin = s_off[S.nrow + 1], own_end = s_off[S.nrow + S.rows + 1];
     const int gbase = s_gst[S.nrow + 1];
-    // the strip's own alive enzymes as (member << 28 | own index) entries
+    // the strip's own alive enzymes as (member << 28 | own index) entries, or
+    // its own alive substrates as (own index) entries
+    const bool by_sub = A.by_substrate;
     for (int i = own_begin + (int)threadIdx.x; i < own_end; i += blockDim.x) {
         const int t = gbase + (i - own_begin);
         const int s = staged ? s_sp[i] : P.sp[t];
         const uint8_t f = staged ? s_fl[i] : C.fl[t];
         if (!(f & EDM_FLAG_ALIVE)) continue;
+        if (by_sub) {
+            if (s == A.sub) {
+                const int e = atomicAdd(&s_nlist, 1);
+                if (e < kCap) s_list[e] = i - own_begin;
+            }
+            continue;
+        }
 #pragma unroll
         for (int j = 0; j < kCatFuse; ++j)
             if (j < A.n_enz && s == A.enz[j]) {
@@ -163,18 +177,79 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
             }
     }
     __syncthreads();
-    // more enzymes than list slots (only in pathologically dense strips):
+    // more entries than list slots (only in pathologically dense strips):
     // walk the own range directly, once per member
     const bool use_list = s_nlist <= kCap;
     const int nown = own_end - own_begin;
-    const int nl = use_list ? s_nlist : nown * A.n_enz;
+    const int nl = use_list ? s_nlist : (by_sub ? nown : nown * A.n_enz);
     // one wavefront per enzyme, its lanes over the enzyme's candidates, so
     // that the per-candidate work (Hill lookups, draws, CAS) runs side by
     // side instead of in one thread's chain
     const int wave = (int)(threadIdx.x >> 6), lane = lane_id();
     const int nwave = (int)(blockDim.x >> 6);
     int hits = 0;
-    for (int e = wave; e < nl; e += nwave) {
+    if (by_sub) {
+        for (int e = wave; e < nl; e += nwave) {
+            const int i = own_begin + (use_list ? s_list[e] : e);
+            const int t = gbase + (i - own_begin);  // the substrate
+            if (!use_list) {
+                const int s = staged ? s_sp[i] : P.sp[t];
+                const uint8_t f = staged ? s_fl[i] : C.fl[t];
+                if (s != A.sub || !(f & EDM_FLAG_ALIVE)) continue;  // wave-uniform
+            }
+            if (staged) {
+                const Real xt = s_x[i], yt = s_y[i], xnt = s_xn[i], ynt = s_yn[i];
+                const int r = strip_row(C, S, yt);
+                const int b0 = s_off[r - 1], l0 = s_off[r + 2] - b0;
+                const int b1 = s_off[S.nrow + r - 1], l1 = s_off[S.nrow + r + 2] - b1;
+                const int b2 = s_off[2 * S.nrow + r - 1], l2 = s_off[2 * S.nrow + r + 2] - b2;
+                for (int c = lane; c < l0 + l1 + l2; c += 64) {
+                    const int u =
+                        c < l0 ? b0 + c : (c < l0 + l1 ? b1 + (c - l0) : b2 + (c - l0 - l1));
+                    if (u == i || !(s_fl[u] & EDM_FLAG_ALIVE)) continue;
+                    const int su = s_sp[u];
+                    bool enz = false;
+#pragma unroll
+                    for (int j = 0; j < kCatFuse; ++j) enz |= j < A.n_enz && su == A.enz[j];
+                    if (!enz) continue;
+                    Real dx, dy;
+                    if (min_image(s_x[u], s_y[u], xt, yt, C.Lx, C.Ly, C.px, C.py, dx, dy) >=
+                        C.cut2)
+                        continue;
+                    const int tu = strip_src(S, s_off, s_gst, u);  // the enzyme
+                    const int iu = C.id[tu];
+                    for (int j = 0; j < A.n_enz; ++j)
+                        if (su == cat_pick(A.enz, j))
+                            hits += cat_try(C, P, A, step, j, G, n_grids, tu, iu, t, s_xn[u],
+                                            s_yn[u], xnt, ynt);
+                }
+            } else if (lane < 9) {
+                const Real xt = C.x[t], yt = C.y[t], xnt = P.xn[t], ynt = P.yn[t];
+                int cx, cy;
+                cell_of(C, xt, yt, cx, cy);
+                const int c = nb_cell(C, cx, cy, lane);
+                if (c >= 0)
+                    for (int u = (int)C.start[c]; u < (int)C.start[c + 1]; ++u) {
+                        if (u == t || !(C.fl[u] & EDM_FLAG_ALIVE)) continue;
+                        const int su = P.sp[u];
+                        bool enz = false;
+#pragma unroll
+                        for (int j = 0; j < kCatFuse; ++j) enz |= j < A.n_enz && su == A.enz[j];
+                        if (!enz) continue;
+                        Real dx, dy;
+                        if (min_image(C.x[u], C.y[u], xt, yt, C.Lx, C.Ly, C.px, C.py, dx, dy) >=
+                            C.cut2)
+                            continue;
+                        const int iu = C.id[u];
+                        for (int j = 0; j < A.n_enz; ++j)
+                            if (su == cat_pick(A.enz, j))
+                                hits += cat_try(C, P, A, step, j, G, n_grids, u, iu, t, P.xn[u],
+                                                P.yn[u], xnt, ynt);
+                    }
+            }
+        }
+    }
+    for (int e = wave; e < (by_sub ? 0 : nl); e += nwave) {
         const int ent = use_list ? s_list[e] : ((e / nown) << 28) | (e % nown);
         const int j = ent >> 28;
         const int i = own_begin + (ent & 0x0fffffff);

@@ -42,6 +42,7 @@ struct Cells {
     bool cap_active;
     int kbase;  // first particle index of this membrane (0 unless batched); draw keys
                 // use the index relative to it
+    int tile_rows;  // cell rows per strip of the tiled kernels (1..kTileRows)
 };
 
 template <typename Real>
@@ -163,6 +164,16 @@ static FbDev make_fb(const edm_membrane* m, const edm_feedback& f) {
         d.ginv = area > 0.0 ? 1.0 / area : 0.0;
     }
     return d;
+}
+
+// Catalysis walks its pairs from the rarer side: the substrates when fewer of
+// them than of the (fused) enzymes were alive at upload.  Either side gives
+// the same pairs and outcome.
+static bool cat_by_substrate(const std::vector<int64_t>& alive, const int* enz, int nf, int sub) {
+    if (alive.empty()) return false;
+    int64_t ne = 0;
+    for (int j = 0; j < nf; ++j) ne += alive[enz[j]];
+    return alive[sub] < ne;
 }
 
 // A rule without its grid resolved (the batched ensemble binds the grid of
@@ -1033,6 +1044,7 @@ static Cells<Real> make_cells(edm_membrane* m) {
     C.npairs = m->d_scal;
     C.cap_active = m->need_pair_offsets && m->n_pairs < m->n_pairs_found;
     C.kbase = 0;
+    C.tile_rows = m->tile_rows;
     return C;
 }
 
@@ -1386,7 +1398,7 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
                            (Real*)m->a.xn, (Real*)m->a.yn, noise, rng, step, G, ng,
                            m->sensor_mask_all);
     else {
-        const int ntile_y = (p.ncy + kTileRows - 1) / kTileRows;
+        const int ntile_y = (p.ncy + m->tile_rows - 1) / m->tile_rows;
         const int ntiles = p.ncx * ntile_y;
         hipLaunchKernelGGL((k_force_tile<Real>), dim3(8 * ((ntiles + 7) / 8)), dim3(256), 0, st, C,
                            F, (Real*)m->a.xn, (Real*)m->a.yn, noise, rng, step, G, ng,
@@ -1488,7 +1500,10 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
                 A.rng = rng;
                 A.step = step;
                 A.deferred = defer;
-                const int ntile_y = (p.ncy + kTileRows - 1) / kTileRows;
+                A.by_substrate = m->cat_side >= 0
+                                     ? m->cat_side == 1
+                                     : cat_by_substrate(m->alive_at_upload, A.enz, nf, r.b);
+                const int ntile_y = (p.ncy + m->tile_rows - 1) / m->tile_rows;
                 const int ntiles = p.ncx * ntile_y;
                 hipLaunchKernelGGL((k_cat_tile<Real>), dim3(8 * ((ntiles + 7) / 8)), dim3(256), 0,
                                    st, C, P, A, m->d_ctr, G, ng, ntile_y, ntiles);

@@ -245,6 +245,9 @@ int edm_ens_add(edm_ensemble* e, const edm_replica* r, const edm_host_state* s, 
     }
     e->seg_npass[k] = npass;
     e->seg_ndye[k] = ndye;
+    e->seg_alive[k].assign(ns, 0);
+    for (int i = 0; i < n; ++i)
+        if (fl[i] & EDM_FLAG_ALIVE) ++e->seg_alive[k][sp[i]];
     e->n_total += n;
     e->next_id += n;
     e->layout_dirty = true;

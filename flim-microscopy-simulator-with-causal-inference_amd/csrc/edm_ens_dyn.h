@@ -29,6 +29,7 @@ __device__ __forceinline__ Cells<Real> ens_cells(const EnsSeg& g, const Real* x,
     C.npairs = nullptr;
     C.cap_active = false;
     C.kbase = g.kbase;
+    C.tile_rows = kTileRows;
     return C;
 }
 
@@ -343,6 +344,7 @@ static CatArgs<Real> ens_cat_host(const edm_ensemble* e, int start, int nf, cons
     A.rng = Rng{g.seed_lo, g.seed_hi};
     A.step = 0;
     A.deferred = r.fb1.sensor >= 0 && (r.fb1.sensor == r.b || r.fb1.sensor == r.c);
+    A.by_substrate = cat_by_substrate(e->seg_alive[&g - e->seg.data()], A.enz, nf, r.b);
     return A;
 }
 
@@ -437,6 +439,7 @@ static int ens_create_t(edm_ensemble* e) {
     e->pass_species = pass_species_of(e->rx, e->pass_groups);
     e->seg_npass.assign(e->rmax, 0);
     e->seg_ndye.assign(e->rmax, 0);
+    e->seg_alive.assign(e->rmax, {});
     // catalysis runs: back-to-back reactions with one substrate and product and
     // no feedback share a launch (as for one membrane)
     e->cat_runs.clear();

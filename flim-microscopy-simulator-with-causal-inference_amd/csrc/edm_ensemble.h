@@ -149,6 +149,7 @@ struct edm_ensemble {
     uint32_t* wl_rcnt = nullptr;                     // [rmax][2] entries appended this step
     int32_t *wl_wrep_pass = nullptr, *wl_wrep_dye = nullptr;  // replica of each wave
     std::vector<int64_t> seg_npass, seg_ndye;  // per slot: entries it contributes
+    std::vector<std::vector<int64_t>> seg_alive;  // per slot: alive per species at admission
     int64_t wl_npass = 0, wl_ndye = 0;         // padded list lengths (launch sizes)
     std::vector<EnsChunk> wl_chunks;           // the list build's blocks
     EnsChunk* d_wl_chunks = nullptr;

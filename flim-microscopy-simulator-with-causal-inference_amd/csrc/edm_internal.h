@@ -100,6 +100,10 @@ struct edm_membrane {
     uint32_t pass_species = 0;    // species closed under every transformation that
                                   // touches a pass species (bit per species)
     int n_pass = 0;               // particles of those species (fixed at upload)
+    std::vector<int64_t> alive_at_upload;  // alive particles per species at upload
+    // tuning (every value gives the same results)
+    int tile_rows = kTileRows;  // cell rows per strip of the tiled force / catalysis kernels
+    int cat_side = -1;          // catalysis walk: -1 the rarer side, 0 enzymes, 1 substrates
     // Work lists rebuilt by every step's gather: the slots of pass-species
     // particles and the slots of dye carriers (valid once a step has run
     // since the last upload)

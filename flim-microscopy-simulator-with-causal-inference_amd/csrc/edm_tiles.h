@@ -1,8 +1,8 @@
 // LDS strip staging shared by the tiled dynamics kernels (included by
 // edm_dynamics.hip; relies on its Cells / ForceParams / Grids helpers).
 //
-// A workgroup of 256 threads owns a strip of kTileRows cells of one cell
-// column.  The strip's 3 x (rows + 2) neighbourhood cells (columns cx-1..cx+1,
+// A workgroup of 256 threads owns a strip of C.tile_rows (at most
+// kTileRows) cells of one cell column.  The strip's 3 x (rows + 2) neighbourhood cells (columns cx-1..cx+1,
 // rows cy0-1..cy0+rows, wrapped or clipped at the box edges) are staged in
 // LDS, one particle per thread, so that consecutive threads read consecutive
 // slots (strip_src).  Inside the strip every particle's 3x3
@@ -29,8 +29,8 @@ __device__ __forceinline__ StripGeo strip_geo(const Cells<Real>& C, int tile, in
     StripGeo S;
     S.tile = tile;
     S.cx = tile / ntile_y;
-    S.cy0 = (tile % ntile_y) * kTileRows;
-    S.rows = min(kTileRows, C.ncy - S.cy0);
+    S.cy0 = (tile % ntile_y) * C.tile_rows;
+    S.rows = min(C.tile_rows, C.ncy - S.cy0);
     S.nrow = S.rows + 2;
     S.ncells = 3 * S.nrow;
     return S;

@@ -100,6 +100,10 @@ class Replica(ctypes.Structure):
                 ("img_offx", c_double), ("img_offy", c_double)]
 
 
+# tuning keys (edm_set_tuning)
+TUNE_TILE_ROWS = 1
+TUNE_CAT_SIDE = 2
+
 # function name -> (argtypes) ; every function returns int32 status
 _SIGS = {
     "edm_device_count": [ctypes.POINTER(ctypes.c_int)],
@@ -114,6 +118,8 @@ _SIGS = {
     "edm_run": [c_void_p, c_int32],
     "edm_counters": [c_void_p, c_void_p],
     "edm_photon_counters": [c_void_p, c_void_p],
+    "edm_set_tuning": [c_void_p, c_int32, c_int64],
+    "edm_get_tuning": [c_void_p, c_int32, ctypes.POINTER(c_int64)],
     "edm_pairs": [c_void_p, c_void_p, c_int64, ctypes.POINTER(c_int64)],
     "edm_density": [c_void_p, c_int32, c_int32, c_void_p],
     "edm_photons": [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, ctypes.POINTER(c_int64)],

@@ -437,6 +437,26 @@ class DynamicsEngine:
         check(lib().edm_density(self._h, sid, g, ptr(out)), "edm_density")
         return out.reshape(nxg, nyg)
 
+    # ---- tuning (no reference counterpart) ----------------------------------
+    def set_tuning(self, tile_rows: Optional[int] = None, cat_side: Optional[int] = None) -> None:
+        """Work-decomposition knobs that leave every result unchanged: cell
+        rows per strip of the tiled kernels (1..64) and the side catalysis
+        walks its pairs from (0 enzymes, 1 substrates, -1 the rarer one)."""
+        if tile_rows is not None:
+            check(lib().edm_set_tuning(self._h, _hip.TUNE_TILE_ROWS, int(tile_rows)),
+                  "edm_set_tuning")
+        if cat_side is not None:
+            check(lib().edm_set_tuning(self._h, _hip.TUNE_CAT_SIDE, int(cat_side)),
+                  "edm_set_tuning")
+
+    def tuning(self) -> Dict[str, int]:
+        out = {}
+        for name, key in (("tile_rows", _hip.TUNE_TILE_ROWS), ("cat_side", _hip.TUNE_CAT_SIDE)):
+            v = ctypes.c_int64(0)
+            check(lib().edm_get_tuning(self._h, key, ctypes.byref(v)), "edm_get_tuning")
+            out[name] = int(v.value)
+        return out
+
     # ---- counters (dynamics.py:2068-2071, 2147-2148) ----------------------
     def _counters(self):
         if self._ctr_cache is None:

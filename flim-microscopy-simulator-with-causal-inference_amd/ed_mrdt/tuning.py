@@ -1,0 +1,60 @@
+"""Autotuning of the kernels' work decomposition.
+
+The knobs -- cell rows per strip of the tiled force and catalysis kernels,
+and the side (enzymes, substrates, or per run the side with fewer alive
+particles) catalysis walks its pairs from -- leave
+every result bit-identical, so the fastest setting can be found by timing.
+`autotune` builds a scratch engine from the configuration (its own initial
+state, Philox draws), times each candidate over a few steps and returns the
+fastest settings; `DynamicsEngine.set_tuning(**settings)` applies them.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import Dict, Iterable
+
+from . import _hip
+from .config import SimulationConfig
+
+TILE_ROWS = (8, 16, 32, 64)
+CAT_SIDES = (-1, 0, 1)
+
+
+def autotune(config: SimulationConfig, precision: str = "fp32", steps: int = 20,
+             warmup: int = 3, tile_rows: Iterable[int] = TILE_ROWS,
+             cat_sides: Iterable[int] = CAT_SIDES) -> Dict:
+    """Fastest tile_rows and cat_side for this configuration; `times_ms`
+    holds the measured milliseconds per step of every candidate."""
+    from .dynamics import DynamicsEngine
+    from .particles import ParticleState
+
+    L = _hip.lib()
+    st = ParticleState(config)
+    eng = DynamicsEngine(config, st, precision=precision, rng="philox", pair_cap=False)
+    tot, fms = ctypes.c_double(0), ctypes.c_double(0)
+
+    def per_step() -> float:
+        _hip.check(L.edm_time_steps(eng.handle, int(warmup), 0, ctypes.byref(tot),
+                                    ctypes.byref(fms), None), "edm_time_steps")
+        _hip.check(L.edm_time_steps(eng.handle, int(steps), 0, ctypes.byref(tot),
+                                    ctypes.byref(fms), None), "edm_time_steps")
+        return tot.value / steps
+
+    times: Dict[str, float] = {}
+    try:
+        sides = list(cat_sides)
+        for s in sides:
+            eng.set_tuning(cat_side=s)
+            times[f"cat_side={s}"] = per_step()
+        best_side = min(sides, key=lambda s: times[f"cat_side={s}"])
+        eng.set_tuning(cat_side=best_side)
+        rows = list(tile_rows)
+        for r in rows:
+            eng.set_tuning(tile_rows=r)
+            times[f"tile_rows={r}"] = per_step()
+        best_rows = min(rows, key=lambda r: times[f"tile_rows={r}"])
+    finally:
+        st.mark_stale()
+        eng.close()
+    return dict(tile_rows=best_rows, cat_side=best_side, times_ms=times)

@@ -173,6 +173,19 @@ int edm_run(edm_membrane *m, int32_t n_steps);
 int edm_counters(edm_membrane *m, int64_t out[6]);
 /* Cumulative photons emitted and detected by edm_photons_image. */
 int edm_photon_counters(edm_membrane *m, int64_t out[2]);
+
+/* ---- Tuning (no reference counterpart) ----------------------------------
+ * Knobs of the kernels' work decomposition; every value gives bit-identical
+ * results, so an autotuner may time them on the live membrane.
+ *   EDM_TUNE_TILE_ROWS  cell rows per strip of the tiled force and catalysis
+ *                       kernels, 1..64 (default 64)
+ *   EDM_TUNE_CAT_SIDE   catalysis walks its pairs from the enzymes (0) or
+ *                       the substrates (1); -1 (default): the side with fewer
+ *                       alive particles at upload                        */
+#define EDM_TUNE_TILE_ROWS 1
+#define EDM_TUNE_CAT_SIDE 2
+int edm_set_tuning(edm_membrane *m, int32_t key, int64_t value);
+int edm_get_tuning(edm_membrane *m, int32_t key, int64_t *value);
 /* Reference pair list in reference order (CellList.get_neighbor_pairs,
  * dynamics.py:220-278) for the cell list of the last edm_step_begin. */
 int edm_pairs(edm_membrane *m, int32_t *pairs, int64_t cap, int64_t *n_pairs);

@@ -85,6 +85,24 @@ def test_plain_membrane_dynamics_fp64(oracle_lib, n_steps, exact):
     print(f"plain membrane {n_steps} steps: worst relative position error {worst:.3e}")
 
 
+@pytest.mark.parametrize("cat_side,tile_rows", [(0, 64), (1, 8)])
+def test_tuned_membrane_matches_oracle(oracle_lib, cat_side, tile_rows):
+    """Catalysis walked from either side, tiled kernels with short strips:
+    still the oracle's events and positions (FP64, injected draws)."""
+    cfg = workloads.plain_membrane()
+    st, eng, ost, odyn = _pair(cfg, exact=False)
+    eng.set_tuning(tile_rows=tile_rows, cat_side=cat_side)
+    assert eng.tuning() == dict(tile_rows=tile_rows, cat_side=cat_side)
+    for k in range(100):
+        eng.step()
+        odyn.step()
+    compare_states(host_arrays(st), oracle_arrays(ost), label="step 100: ")
+    oc = odyn.counters()
+    assert eng.n_catalytic == oc["n_catalytic"] and oc["n_catalytic"] > 0
+    assert (eng.n_recruited, eng.n_detached, eng.n_converted) == (
+        oc["n_recruited"], oc["n_detached"], oc["n_converted"])
+
+
 @pytest.mark.parametrize("variant", [dict(), dict(intramolecular=False),
                                      dict(reflective=True, harmonic=True),
                                      dict(intramolecular=False, exact=False)])
