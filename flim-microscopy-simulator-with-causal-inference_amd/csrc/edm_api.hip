@@ -127,7 +127,7 @@ int edm_destroy(edm_membrane* m) {
                     m->ph_fret, m->flim, m->pcx, m->det_flag, m->det_pos, m->sv_key, m->sv_key2,
                     m->sv_val, m->sv_val2, m->asg_px, m->asg_py, m->asg_flag, m->asg_pos,
                     m->asg_t, m->asg_ch, m->sv_px, m->sv_py, m->pix_tot, m->d_scal,
-                    m->dye_ids, m->wl_pass, m->wl_dye, m->wl_cnt};
+                    m->dye_ids, m->wl, m->wl_cnt};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (auto e : m->kev) hipEventDestroy(e);
@@ -198,13 +198,17 @@ int edm_upload_state(edm_membrane* m, const edm_host_state* s) {
         for (int i = 0; i < n; ++i)
             if ((m->pass_species >> sp[i]) & 1u) ++n_pass;
     m->n_pass = n_pass;
-    for (int32_t** l : {&m->wl_pass, &m->wl_dye})
-        if (*l) EDM_HIP(hipFree(*l));
-    m->wl_pass = m->wl_dye = nullptr;
-    EDM_HIP(hipMalloc(&m->wl_pass, (size_t)std::max(n_pass, 1) * sizeof(int32_t)));
-    EDM_HIP(hipMalloc(&m->wl_dye, (size_t)std::max(m->n_dye, 1) * sizeof(int32_t)));
-    if (!m->wl_cnt) EDM_HIP(hipMalloc(&m->wl_cnt, 2 * sizeof(uint32_t)));
-    EDM_HIP(hipMemset(m->wl_cnt, 0, 2 * sizeof(uint32_t)));
+    const int nb = std::max(1, (n + kWlSlots - 1) / kWlSlots);
+    if (nb != m->wl_nblk) {
+        if (m->wl) EDM_HIP(hipFree(m->wl));
+        if (m->wl_cnt) EDM_HIP(hipFree(m->wl_cnt));
+        m->wl = nullptr;
+        m->wl_cnt = nullptr;
+        EDM_HIP(hipMalloc(&m->wl, (size_t)kWlMaxLists * nb * kWlSlots * sizeof(int32_t)));
+        EDM_HIP(hipMalloc(&m->wl_cnt, (size_t)kWlMaxLists * nb * sizeof(uint32_t)));
+        m->wl_nblk = nb;
+    }
+    EDM_HIP(hipMemset(m->wl_cnt, 0, (size_t)kWlMaxLists * nb * sizeof(uint32_t)));
     m->wl_valid = false;
     return 0;
 }
@@ -222,6 +226,15 @@ int edm_set_tuning(edm_membrane* m, int32_t key, int64_t value) {
                 return invalid("edm_set_tuning: catalysis side must be -1, 0 or 1");
             m->cat_side = (int)value;
             return 0;
+        case EDM_TUNE_CAT_WALK:
+            if (value < 0 || value > 1) return invalid("edm_set_tuning: catalysis walk must be 0 or 1");
+            m->cat_walk = (int)value;
+            return 0;
+        case EDM_TUNE_CAT_LANES:
+            if (value < 1 || value > 64 || (value & (value - 1)))
+                return invalid("edm_set_tuning: catalysis lanes must be a power of two in 1..64");
+            m->cat_lanes = (int)value;
+            return 0;
         default:
             return invalid("edm_set_tuning: unknown key");
     }
@@ -232,6 +245,8 @@ int edm_get_tuning(edm_membrane* m, int32_t key, int64_t* value) {
     switch (key) {
         case EDM_TUNE_TILE_ROWS: *value = m->tile_rows; return 0;
         case EDM_TUNE_CAT_SIDE: *value = m->cat_side; return 0;
+        case EDM_TUNE_CAT_WALK: *value = m->cat_walk; return 0;
+        case EDM_TUNE_CAT_LANES: *value = m->cat_lanes; return 0;
         default: return invalid("edm_get_tuning: unknown key");
     }
 }

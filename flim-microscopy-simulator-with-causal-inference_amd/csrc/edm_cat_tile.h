@@ -38,6 +38,7 @@ struct CatArgs {
     uint32_t step;
     bool deferred;
     bool by_substrate;  // walk the pairs from the substrates' side (the rarer one)
+    int group_shift;    // log2 of the lanes that walk one entity's candidates (0..6)
 };
 
 template <typename T>
@@ -182,11 +183,13 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
     const bool use_list = s_nlist <= kCap;
     const int nown = own_end - own_begin;
     const int nl = use_list ? s_nlist : (by_sub ? nown : nown * A.n_enz);
-    // one wavefront per enzyme, its lanes over the enzyme's candidates, so
-    // that the per-candidate work (Hill lookups, draws, CAS) runs side by
-    // side instead of in one thread's chain
-    const int wave = (int)(threadIdx.x >> 6), lane = lane_id();
-    const int nwave = (int)(blockDim.x >> 6);
+    // a group of 2^group_shift lanes per entity (a wavefront at most), its
+    // lanes over the entity's candidates: wide groups run the per-candidate
+    // work (Hill lookups, draws, CAS) side by side, narrow ones keep more
+    // lanes busy when few candidates qualify
+    const int gs = A.group_shift, gsize = 1 << gs;
+    const int wave = (int)(threadIdx.x >> gs), lane = (int)threadIdx.x & (gsize - 1);
+    const int nwave = (int)(blockDim.x >> gs);
     int hits = 0;
     if (by_sub) {
         for (int e = wave; e < nl; e += nwave) {
@@ -195,7 +198,7 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
             if (!use_list) {
                 const int s = staged ? s_sp[i] : P.sp[t];
                 const uint8_t f = staged ? s_fl[i] : C.fl[t];
-                if (s != A.sub || !(f & EDM_FLAG_ALIVE)) continue;  // wave-uniform
+                if (s != A.sub || !(f & EDM_FLAG_ALIVE)) continue;  // group-uniform
             }
             if (staged) {
                 const Real xt = s_x[i], yt = s_y[i], xnt = s_xn[i], ynt = s_yn[i];
@@ -203,7 +206,7 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
                 const int b0 = s_off[r - 1], l0 = s_off[r + 2] - b0;
                 const int b1 = s_off[S.nrow + r - 1], l1 = s_off[S.nrow + r + 2] - b1;
                 const int b2 = s_off[2 * S.nrow + r - 1], l2 = s_off[2 * S.nrow + r + 2] - b2;
-                for (int c = lane; c < l0 + l1 + l2; c += 64) {
+                for (int c = lane; c < l0 + l1 + l2; c += gsize) {
                     const int u =
                         c < l0 ? b0 + c : (c < l0 + l1 ? b1 + (c - l0) : b2 + (c - l0 - l1));
                     if (u == i || !(s_fl[u] & EDM_FLAG_ALIVE)) continue;
@@ -223,12 +226,13 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
                             hits += cat_try(C, P, A, step, j, G, n_grids, tu, iu, t, s_xn[u],
                                             s_yn[u], xnt, ynt);
                 }
-            } else if (lane < 9) {
+            } else {
                 const Real xt = C.x[t], yt = C.y[t], xnt = P.xn[t], ynt = P.yn[t];
                 int cx, cy;
                 cell_of(C, xt, yt, cx, cy);
-                const int c = nb_cell(C, cx, cy, lane);
-                if (c >= 0)
+                for (int k = lane; k < 9; k += gsize) {
+                    const int c = nb_cell(C, cx, cy, k);
+                    if (c < 0) continue;
                     for (int u = (int)C.start[c]; u < (int)C.start[c + 1]; ++u) {
                         if (u == t || !(C.fl[u] & EDM_FLAG_ALIVE)) continue;
                         const int su = P.sp[u];
@@ -246,6 +250,7 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
                                 hits += cat_try(C, P, A, step, j, G, n_grids, u, iu, t, P.xn[u],
                                                 P.yn[u], xnt, ynt);
                     }
+                }
             }
         }
     }
@@ -257,7 +262,7 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
         if (!use_list) {
             const int s = staged ? s_sp[i] : P.sp[t];
             const uint8_t f = staged ? s_fl[i] : C.fl[t];
-            if (s != cat_pick(A.enz, j) || !(f & EDM_FLAG_ALIVE)) continue;  // wave-uniform
+            if (s != cat_pick(A.enz, j) || !(f & EDM_FLAG_ALIVE)) continue;  // group-uniform
         }
         const int it = C.id[t];
         if (staged) {
@@ -266,7 +271,7 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
             const int b0 = s_off[r - 1], l0 = s_off[r + 2] - b0;
             const int b1 = s_off[S.nrow + r - 1], l1 = s_off[S.nrow + r + 2] - b1;
             const int b2 = s_off[2 * S.nrow + r - 1], l2 = s_off[2 * S.nrow + r + 2] - b2;
-            for (int c = lane; c < l0 + l1 + l2; c += 64) {
+            for (int c = lane; c < l0 + l1 + l2; c += gsize) {
                 const int u = c < l0 ? b0 + c : (c < l0 + l1 ? b1 + (c - l0) : b2 + (c - l0 - l1));
                 if (u == i || s_sp[u] != A.sub || !(s_fl[u] & EDM_FLAG_ALIVE)) continue;
                 Real dx, dy;
@@ -275,12 +280,13 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
                 hits += cat_try(C, P, A, step, j, G, n_grids, t, it,
                                 strip_src(S, s_off, s_gst, u), xnt, ynt, s_xn[u], s_yn[u]);
             }
-        } else if (lane < 9) {
+        } else {
             const Real xt = C.x[t], yt = C.y[t], xnt = P.xn[t], ynt = P.yn[t];
             int cx, cy;
             cell_of(C, xt, yt, cx, cy);
-            const int c = nb_cell(C, cx, cy, lane);
-            if (c >= 0)
+            for (int k = lane; k < 9; k += gsize) {
+                const int c = nb_cell(C, cx, cy, k);
+                if (c < 0) continue;
                 for (int u = (int)C.start[c]; u < (int)C.start[c + 1]; ++u) {
                     if (u == t || P.sp[u] != A.sub || !(C.fl[u] & EDM_FLAG_ALIVE)) continue;
                     Real dx, dy;
@@ -290,6 +296,7 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
                     hits += cat_try(C, P, A, step, j, G, n_grids, t, it, u, xnt, ynt, P.xn[u],
                                     P.yn[u]);
                 }
+            }
         }
     }
     unsigned long long v = hits;
@@ -309,4 +316,81 @@ __global__ __launch_bounds__(256) void k_cat_tile(Cells<Real> C, Post<Real> P, C
     if (threadIdx.x == 0) sA = A;
     __syncthreads();
     cat_strip(C, P, sA, A.step, ctr, G, n_grids, strip_geo(C, tile, ntile_y));
+}
+
+// Catalysis over a work list of its walk entities, one listed slot per
+// thread: an alive enzyme against the substrates of its 3x3 cells, or (by
+// substrate) an alive substrate against the enzymes of its 3x3 cells, read
+// from global memory.  Used when no enzyme species is the run's substrate
+// or product, so that no entity changes identity during the run: species
+// read as the run goes then agree with the run's start.
+template <typename Real>
+__device__ __forceinline__ int cat_walk(const Cells<Real>& C, const Post<Real>& P,
+                                        const CatArgs<Real>& A, const Grids<Real>& G,
+                                        int n_grids, int t) {
+    if (!(C.fl[t] & EDM_FLAG_ALIVE)) return 0;
+    const int st = P.sp[t];
+    const bool by_sub = A.by_substrate;
+    if (by_sub ? st != A.sub : true) {
+        bool enz = false;
+#pragma unroll
+        for (int j = 0; j < kCatFuse; ++j) enz |= j < A.n_enz && st == A.enz[j];
+        if (by_sub || !enz) return 0;
+    }
+    const Real xt = C.x[t], yt = C.y[t], xnt = P.xn[t], ynt = P.yn[t];
+    const int it = C.id[t];
+    int cx, cy;
+    cell_of(C, xt, yt, cx, cy);
+    int hits = 0;
+    for (int k = 0; k < 9; ++k) {
+        const int c = nb_cell(C, cx, cy, k);
+        if (c < 0) continue;
+        const int ue = (int)C.start[c + 1];
+        for (int u = (int)C.start[c]; u < ue; ++u) {
+            if (u == t || !(C.fl[u] & EDM_FLAG_ALIVE)) continue;
+            const int su = P.sp[u];
+            Real dx, dy;
+            if (by_sub) {  // u a candidate enzyme of the substrate t
+                bool enz = false;
+#pragma unroll
+                for (int j = 0; j < kCatFuse; ++j) enz |= j < A.n_enz && su == A.enz[j];
+                if (!enz) continue;
+                if (min_image(C.x[u], C.y[u], xt, yt, C.Lx, C.Ly, C.px, C.py, dx, dy) >= C.cut2)
+                    continue;
+                const int iu = C.id[u];
+                const Real xnu = P.xn[u], ynu = P.yn[u];
+                for (int j = 0; j < A.n_enz; ++j)
+                    if (su == cat_pick(A.enz, j))
+                        hits += cat_try(C, P, A, A.step, j, G, n_grids, u, iu, t, xnu, ynu, xnt,
+                                        ynt);
+            } else {  // u a candidate substrate of the enzyme t
+                if (su != A.sub) continue;
+                if (min_image(xt, yt, C.x[u], C.y[u], C.Lx, C.Ly, C.px, C.py, dx, dy) >= C.cut2)
+                    continue;
+                const Real xnu = P.xn[u], ynu = P.yn[u];
+                for (int j = 0; j < A.n_enz; ++j)
+                    if (st == cat_pick(A.enz, j))
+                        hits += cat_try(C, P, A, A.step, j, G, n_grids, t, it, u, xnt, ynt, xnu,
+                                        ynu);
+            }
+        }
+    }
+    return hits;
+}
+
+template <typename Real>
+__global__ __launch_bounds__(256) void k_cat_list(Cells<Real> C, Post<Real> P, CatArgs<Real> A,
+                                                  WlView V, unsigned long long* ctr,
+                                                  Grids<Real> G, int n_grids) {
+    const int m = wl_slice(V);
+    if (m == 0) return;  // uniform: an empty slice
+    // the parameter block is indexed with per-lane values: keep it in LDS
+    // rather than in per-lane scratch
+    __shared__ CatArgs<Real> sA;
+    if (threadIdx.x == 0) sA = A;
+    __syncthreads();
+    const int hits = (int)threadIdx.x < m ? cat_walk(C, P, sA, G, n_grids, wl_slot(V)) : 0;
+    unsigned long long v = hits;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+    if (lane_id() == 0 && v) atomicAdd(&ctr[2], v);
 }

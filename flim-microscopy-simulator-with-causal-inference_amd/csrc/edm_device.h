@@ -137,15 +137,19 @@ template <typename Real>
 __device__ __forceinline__ Real min_image(Real x1, Real y1, Real x2, Real y2, Real Lx,
                                           Real Ly, bool px, bool py, Real& dx,
                                           Real& dy) {
+    // single image shift by +-L beyond L/2, written as selects so that the
+    // wavefront does not branch per candidate
     dx = x1 - x2;
     dy = y1 - y2;
     if (px) {
-        if (dx > (Real)0.5 * Lx) dx -= Lx;
-        else if (dx < -(Real)0.5 * Lx) dx += Lx;
+        const Real h = (Real)0.5 * Lx;
+        const Real lo = dx + Lx, hi = dx - Lx;
+        dx = dx > h ? hi : (dx < -h ? lo : dx);
     }
     if (py) {
-        if (dy > (Real)0.5 * Ly) dy -= Ly;
-        else if (dy < -(Real)0.5 * Ly) dy += Ly;
+        const Real h = (Real)0.5 * Ly;
+        const Real lo = dy + Ly, hi = dy - Ly;
+        dy = dy > h ? hi : (dy < -h ? lo : dy);
     }
     return dx * dx + dy * dy;
 }

@@ -222,10 +222,9 @@ template <typename Real>
 __global__ void k_bin(int n, const Real* __restrict__ x, const Real* __restrict__ y, Real csx,
                       Real csy, int ncx, int ncy, uint32_t* __restrict__ key,
                       uint32_t* __restrict__ rank, uint32_t* __restrict__ cnt, int32_t* counts,
-                      int n_counts, uint32_t* wl_cnt) {
+                      int n_counts) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < n_counts) counts[s] = 0;  // density counts are rebuilt each step
-    if (wl_cnt && s < 2) wl_cnt[s] = 0;  // work lists are rebuilt by the gather
     const bool live = s < n;          // live lanes are a prefix of the wavefront
     uint32_t c = 0xffffffffu;
     if (live) {
@@ -332,14 +331,75 @@ __global__ void k_gather(int n, const uint32_t* __restrict__ perm, DevArrays a) 
 }
 
 // Work lists, rebuilt after every sort: the slots of the particles the
-// per-particle passes can touch and the slots of the dye carriers, so that
-// those kernels run one listed slot per thread.  A block lists its
-// kWlSlots slots in LDS and appends each list with one atomic;
-// the order of the blocks' runs in a list is unspecified and every consumer
-// is independent of it.  The counters are cleared by the binning kernel.
-static constexpr int kWlPer = 8;  // slots per thread
-static constexpr int kWlSlots = kWlPer * 256;
+// per-particle passes can touch, of the dye carriers and of the entities a
+// catalysis walk starts from, so that those kernels run one listed slot per
+// thread.  The ensemble builds its lists by replica (wl_build_block below,
+// one atomic per chunk and list on the replica's counter); one membrane's
+// lists need no atomics: block b of the build lists its kWlSlots slots into
+// its own region [b * kWlSlots, ...) of every list, in slot order, and
+// records how many (WlView).
+struct WlBuild {
+    WlSpec spec[kWlMaxLists];
+    int32_t* list[kWlMaxLists];  // region b at b * kWlSlots
+    uint32_t* cnt[kWlMaxLists];  // [blocks]
+    int n_lists;
+};
 
+__global__ __launch_bounds__(256) void k_wl_build(int n, const int8_t* __restrict__ sp,
+                                                  const uint8_t* __restrict__ fl, WlBuild W) {
+    constexpr int kW = 4;  // wavefronts per block
+    __shared__ int s_off[kWlMaxLists][kWlPer * kW];  // members per (row, wave), then offsets
+    const int beg = (int)blockIdx.x * kWlSlots, end = min(n, beg + kWlSlots);
+    const int lane = lane_id(), w = (int)(threadIdx.x >> 6);
+    // row k of the block holds slots beg + k * 256 + thread (coalesced loads)
+    int8_t vs[kWlPer];
+    uint8_t vf[kWlPer];
+#pragma unroll
+    for (int k = 0; k < kWlPer; ++k) {
+        const int t = beg + k * 256 + (int)threadIdx.x;
+        vs[k] = t < end ? sp[t] : (int8_t)0;
+        vf[k] = t < end ? fl[t] : (uint8_t)0;
+    }
+    uint64_t bal[kWlMaxLists][kWlPer];
+#pragma unroll
+    for (int L = 0; L < kWlMaxLists; ++L) {
+        if (L >= W.n_lists) break;
+        const WlSpec S = W.spec[L];
+#pragma unroll
+        for (int k = 0; k < kWlPer; ++k) {
+            const bool live = beg + k * 256 + (int)threadIdx.x < end;
+            bal[L][k] = __ballot(live && ((S.species >> vs[k]) & 1u) &&
+                                 (vf[k] & S.flags) == S.flags);
+            if (lane == 0) s_off[L][k * kW + w] = __popcll(bal[L][k]);
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < W.n_lists) {  // exclusive prefix, one thread per list
+        const int L = (int)threadIdx.x;
+        int run = 0;
+        for (int i = 0; i < kWlPer * kW; ++i) {
+            const int c = s_off[L][i];
+            s_off[L][i] = run;
+            run += c;
+        }
+        W.cnt[L][blockIdx.x] = (uint32_t)run;
+    }
+    __syncthreads();
+    const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int L = 0; L < kWlMaxLists; ++L) {
+        if (L >= W.n_lists) break;
+        int32_t* out = W.list[L] + (size_t)blockIdx.x * kWlSlots;
+#pragma unroll
+        for (int k = 0; k < kWlPer; ++k)
+            if ((bal[L][k] >> lane) & 1ull)
+                out[s_off[L][k * kW + w] + __popcll(bal[L][k] & below)] =
+                    beg + k * 256 + (int)threadIdx.x;
+    }
+}
+
+// Ensemble list build (one replica's chunk): the chunk's members of both
+// lists, appended to the replica's regions with one atomic per list.
 struct WlRun {
     int32_t* list;
     uint32_t* cnt;  // entries appended this step
@@ -347,10 +407,9 @@ struct WlRun {
     uint32_t cap;   // entries the region holds
 };
 
-// Lists the members of [beg, end) of both lists and appends them to their
-// regions.  Row k of the block holds slots beg + k * 256 + thread (coalesced
-// loads); a member's place in the block's list is its (row, wave, lane)
-// rank.  Every thread of the block (256) calls it.
+// Row k of the block holds slots beg + k * 256 + thread (coalesced loads); a
+// member's place in the block's list is its (row, wave, lane) rank.  Every
+// thread of the block (256) calls it.
 template <typename PredP, typename PredD>
 __device__ __forceinline__ void wl_build_block(int beg, int end, const PredP& in_pass,
                                                const PredD& in_dye, WlRun Rp, WlRun Rd) {
@@ -409,14 +468,6 @@ __device__ __forceinline__ void wl_build_block(int beg, int end, const PredP& in
     }
 }
 
-__global__ __launch_bounds__(256) void k_wl_build(int n, const int8_t* __restrict__ sp,
-                                                  const uint8_t* __restrict__ fl,
-                                                  uint32_t pass_species, WlRun Rp, WlRun Rd) {
-    const int beg = (int)blockIdx.x * kWlSlots;
-    wl_build_block(
-        beg, min(n, beg + kWlSlots), [&](int t) { return ((pass_species >> sp[t]) & 1u) != 0u; },
-        [&](int t) { return (fl[t] & EDM_FLAG_DYE) != 0; }, Rp, Rd);
-}
 
 // Reference pair-list bookkeeping: per particle index, the number of
 // partners j > i with d^2 < cutoff^2 (dynamics.py:237-276; dormant
@@ -504,12 +555,16 @@ __device__ __forceinline__ bool pair_force(const ForceParams<Real>& F, int kind,
         if (s <= (Real)0 || e <= (Real)0) return false;
         const Real rcut2 = s * s * (Real)1.2599210498948732;
         if (dsq >= rcut2 || dsq < (Real)1e-30) return false;
-        const Real inv_r2 = (Real)1 / dsq;
+        // FP32: hardware reciprocals (about 1 ulp, inside the FP32 bound);
+        // FP64: the reference's expressions, correctly rounded
+        constexpr bool f32 = sizeof(Real) == 4;
+        const Real inv_r2 = f32 ? (Real)__builtin_amdgcn_rcpf((float)dsq) : (Real)1 / dsq;
         const Real s2 = s * s * inv_r2;
         const Real s6 = s2 * s2 * s2;
         const Real s12 = s6 * s6;
         Real fr = (Real)24 * e * ((Real)2 * s12 - s6) * inv_r2;
-        const Real fmax = (Real)2400 * e / (s * s);
+        const Real fmax = f32 ? (Real)2400 * e * (Real)__builtin_amdgcn_rcpf((float)(s * s))
+                              : (Real)2400 * e / (s * s);
         if (fr > fmax) fr = fmax;
         f = fr;
         return true;
@@ -517,9 +572,14 @@ __device__ __forceinline__ bool pair_force(const ForceParams<Real>& F, int kind,
     const Real k = F.kh[si * F.ns + su];
     if (s <= (Real)0 || k <= (Real)0) return false;
     if (dsq >= s * s || dsq < (Real)1e-30) return false;
-    const Real r = sqrt(dsq);
-    const Real fmag = k * (s - r);
-    f = fmag / r;
+    if constexpr (sizeof(Real) == 4) {
+        // k (s - r) / r = k (s / r - 1) with the hardware reciprocal square root
+        f = k * (s * (Real)__builtin_amdgcn_rsqf((float)dsq) - (Real)1);
+    } else {
+        const Real r = sqrt(dsq);
+        const Real fmag = k * (s - r);
+        f = fmag / r;
+    }
     return true;
 }
 
@@ -1299,15 +1359,100 @@ int64_t hook_draws_per_step(const edm_membrane* m, int64_t n_pairs) {
     return total;
 }
 
+// Reactions ri .. ri + len - 1 share one catalysis launch: up to kCatFuse
+// back-to-back catalytic reactions with one substrate and product and
+// without feedback or adaptation.
+static int cat_run_len(const std::vector<edm_reaction>& rx, size_t ri) {
+    const edm_reaction& r = rx[ri];
+    const bool plain = r.fb1.sensor < 0 && !r.use_adapt;
+    int nf = 1;
+    while (plain && nf < kCatFuse && ri + nf < rx.size()) {
+        const edm_reaction& r2 = rx[ri + nf];
+        if (r2.kind != EDM_RX_CAT || r2.b != r.b || r2.c != r.c || r2.fb1.sensor >= 0 ||
+            r2.use_adapt)
+            break;
+        ++nf;
+    }
+    return nf;
+}
+
+static uint32_t sbit(int s) { return s >= 0 ? 1u << s : 0u; }
+
+// This step's catalysis walks.  A run walks a work list of its entities
+// (alive enzymes, or alive substrates) unless an enzyme species is also its
+// substrate or product (an entity could then change identity during the
+// run; the strip kernel keeps the species snapshot of the run's start).  The
+// list is built with the others after the sort, or -- when an earlier
+// reaction of the step can change the species or alive flag of its
+// entities -- just before the run.
+static void plan_cat_walks(edm_membrane* m) {
+    m->cat_plan.clear();
+    int next = kWlCat0;
+    for (size_t ri = 0; ri < m->rx.size();) {
+        const edm_reaction& r = m->rx[ri];
+        if (r.kind != EDM_RX_CAT) {
+            ++ri;
+            continue;
+        }
+        CatPlan c{};
+        c.ri = (int)ri;
+        c.nf = cat_run_len(m->rx, ri);
+        int enz[kCatFuse];
+        uint32_t E = 0;
+        for (int j = 0; j < c.nf; ++j) {
+            enz[j] = m->rx[ri + j].a;
+            E |= sbit(enz[j]);
+        }
+        c.by_sub = m->cat_side >= 0 ? m->cat_side == 1
+                                    : cat_by_substrate(m->alive_at_upload, enz, c.nf, r.b);
+        c.list = -1;
+        if (m->cat_walk == 1 && !(E & (sbit(r.b) | sbit(r.c))) && next < kWlMaxLists) {
+            c.list = next++;
+            c.spec = WlSpec{c.by_sub ? sbit(r.b) : E, (uint32_t)EDM_FLAG_ALIVE};
+            for (size_t k = 0; k < ri; ++k) {
+                const edm_reaction& q = m->rx[k];
+                uint32_t touched = 0;
+                switch (q.kind) {
+                    case EDM_RX_BIND: touched = sbit(q.a) | sbit(q.b) | sbit(q.c); break;
+                    case EDM_RX_CAT: touched = sbit(q.b) | sbit(q.c); break;
+                    case EDM_RX_RECRUIT:
+                    case EDM_RX_DETACH: touched = sbit(q.a); break;
+                    case EDM_RX_CONVERT: touched = sbit(q.a) | sbit(q.b); break;
+                    default: break;
+                }
+                if (touched & c.spec.species) c.late = true;
+            }
+        }
+        m->cat_plan.push_back(c);
+        ri += c.nf;
+    }
+}
+
+
+// Builds the listed work lists from the current arrays (sp, fl: the slot
+// order of this step).
+static void wl_build(edm_membrane* m, const int8_t* sp, const uint8_t* fl, const int* lists,
+                     const WlSpec* specs, int nl) {
+    if (nl == 0 || m->n == 0 || !m->wl) return;
+    WlBuild W{};
+    W.n_lists = nl;
+    for (int i = 0; i < nl; ++i) {
+        W.spec[i] = specs[i];
+        W.list[i] = m->wl + (size_t)lists[i] * m->wl_nblk * kWlSlots;
+        W.cnt[i] = m->wl_cnt + (size_t)lists[i] * m->wl_nblk;
+        m->wl_built |= 1u << lists[i];
+    }
+    hipLaunchKernelGGL(k_wl_build, dim3(m->wl_nblk), dim3(256), 0, m->stream, m->n, sp, fl, W);
+}
+
 template <typename Real>
 static int begin_t(edm_membrane* m) {
     const int n = m->n;
     hipStream_t st = m->stream;
     // bin on the current positions, counting sort, order cells by index
-    hipLaunchKernelGGL(k_bin<Real>, dim3(nblk(std::max({n, m->n_count, 2}))), dim3(kBlock), 0,
-                       st, n, (const Real*)m->a.x, (const Real*)m->a.y, (Real)m->p.csx,
-                       (Real)m->p.csy, m->p.ncx, m->p.ncy, m->key, m->rank, m->cell_cnt,
-                       m->counts, m->n_count, m->wl_cnt);
+    hipLaunchKernelGGL(k_bin<Real>, dim3(nblk(std::max(n, m->n_count))), dim3(kBlock), 0, st, n,
+                       (const Real*)m->a.x, (const Real*)m->a.y, (Real)m->p.csx, (Real)m->p.csy,
+                       m->p.ncx, m->p.ncy, m->key, m->rank, m->cell_cnt, m->counts, m->n_count);
     size_t tb = m->cub_tmp_bytes;
     EDM_HIP(hipcub::DeviceScan::ExclusiveSum(m->cub_tmp, tb, m->cell_cnt, m->cell_start,
                                              m->ncell + 1, st));
@@ -1317,12 +1462,29 @@ static int begin_t(edm_membrane* m) {
                        m->cell_start, m->perm, m->cell_cnt, (const int32_t*)m->a.id);
     hipLaunchKernelGGL(k_gather<Real>, dim3(nblk(n)), dim3(kBlock), 0, st, n,
                        (const uint32_t*)m->perm, m->a);
-    if (m->wl_cnt && n > 0) {
-        // the gather wrote the new order into the _t arrays (swapped below)
-        hipLaunchKernelGGL(k_wl_build, dim3(nblk(n, kWlSlots)), dim3(256), 0, st, n,
-                           (const int8_t*)m->a.sp_t, (const uint8_t*)m->a.fl_t, m->pass_species,
-                           WlRun{m->wl_pass, m->wl_cnt, 0u, (uint32_t)m->n_pass},
-                           WlRun{m->wl_dye, m->wl_cnt + 1, 0u, (uint32_t)m->n_dye});
+    if (m->wl) {
+        // work lists of this step's slot order (the gather wrote it into the
+        // _t arrays, swapped below): pass species, dye carriers and the
+        // catalysis walks that no earlier reaction of the step can change
+        plan_cat_walks(m);
+        int lists[kWlMaxLists];
+        WlSpec specs[kWlMaxLists];
+        int nl = 0;
+        if (m->pass_species) {
+            lists[nl] = kWlPass;
+            specs[nl++] = WlSpec{m->pass_species, 0u};
+        }
+        if (m->n_dye > 0) {
+            lists[nl] = kWlDye;
+            specs[nl++] = WlSpec{0xffffffffu, (uint32_t)EDM_FLAG_DYE};
+        }
+        for (const CatPlan& c : m->cat_plan)
+            if (c.list >= 0 && !c.late) {
+                lists[nl] = c.list;
+                specs[nl++] = c.spec;
+            }
+        m->wl_built = 0;
+        wl_build(m, (const int8_t*)m->a.sp_t, (const uint8_t*)m->a.fl_t, lists, specs, nl);
         m->wl_valid = true;
     }
     std::swap(m->a.x, m->a.xt);
@@ -1462,15 +1624,10 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
                     break;
                 }
                 const bool defer = f1.sensor >= 0 && (f1.sensor == r.b || f1.sensor == r.c);
-                const bool plain = f1.sensor < 0 && !r.use_adapt;
-                int nf = 1;
-                while (plain && nf < kCatFuse && ri + nf < m->rx.size()) {
-                    const edm_reaction& r2 = m->rx[ri + nf];
-                    if (r2.kind != EDM_RX_CAT || r2.b != r.b || r2.c != r.c ||
-                        r2.fb1.sensor >= 0 || r2.use_adapt)
-                        break;
-                    ++nf;
-                }
+                const CatPlan* cp = nullptr;
+                for (const CatPlan& c : m->cat_plan)
+                    if (c.ri == (int)ri) cp = &c;
+                const int nf = cp ? cp->nf : cat_run_len(m->rx, ri);
                 int wnx = 1, wny = 1;
                 const double* w = nullptr;
                 if (r.use_adapt && p.adapt_grid >= 0) {
@@ -1500,13 +1657,30 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
                 A.rng = rng;
                 A.step = step;
                 A.deferred = defer;
-                A.by_substrate = m->cat_side >= 0
-                                     ? m->cat_side == 1
-                                     : cat_by_substrate(m->alive_at_upload, A.enz, nf, r.b);
-                const int ntile_y = (p.ncy + m->tile_rows - 1) / m->tile_rows;
-                const int ntiles = p.ncx * ntile_y;
-                hipLaunchKernelGGL((k_cat_tile<Real>), dim3(8 * ((ntiles + 7) / 8)), dim3(256), 0,
-                                   st, C, P, A, m->d_ctr, G, ng, ntile_y, ntiles);
+                A.group_shift = __builtin_ctz((unsigned)m->cat_lanes);
+                A.by_substrate = cp ? cp->by_sub
+                                    : m->cat_side >= 0
+                                          ? m->cat_side == 1
+                                          : cat_by_substrate(m->alive_at_upload, A.enz, nf, r.b);
+                bool walked = false;
+                if (cp && cp->list >= 0 && m->wl_valid) {
+                    if (cp->late)  // entities as they are now
+                        wl_build(m, (const int8_t*)m->a.sp, (const uint8_t*)m->a.fl, &cp->list,
+                                 &cp->spec, 1);
+                    const WlView V = wl_view(m, cp->list);
+                    if (V.list) {
+                        hipLaunchKernelGGL((k_cat_list<Real>), dim3(m->wl_nblk, kWlPer),
+                                           dim3(256), 0, st, C, P, A, V, m->d_ctr, G, ng);
+                        walked = true;
+                    }
+                }
+                if (!walked) {
+                    const int ntile_y = (p.ncy + m->tile_rows - 1) / m->tile_rows;
+                    const int ntiles = p.ncx * ntile_y;
+                    hipLaunchKernelGGL((k_cat_tile<Real>), dim3(8 * ((ntiles + 7) / 8)),
+                                       dim3(256), 0, st, C, P, A, m->d_ctr, G, ng, ntile_y,
+                                       ntiles);
+                }
                 if (defer && m->n_count)
                     hipLaunchKernelGGL(k_fold, dim3(nblk(m->n_count)), dim3(kBlock), 0, st,
                                        m->n_count, m->counts, m->delta);
@@ -1544,11 +1718,10 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
                         d.reach = reach;
                         grp.species |= reach;
                     }
-                    const bool wl = m->wl_valid;
-                    const int nt = wl ? m->n_pass : n;
-                    hipLaunchKernelGGL(k_passes<Real>, dim3(nblk(std::max(nt, 1))), dim3(kBlock), 0,
-                                       st, nt, n, wl ? (const int32_t*)m->wl_pass : nullptr,
-                                       wl ? (const uint32_t*)m->wl_cnt : nullptr,
+                    const WlView V = wl_view(m, kWlPass);
+                    hipLaunchKernelGGL(k_passes<Real>,
+                                       V.list ? dim3(m->wl_nblk, kWlPer) : dim3(nblk(n)),
+                                       dim3(kBlock), 0, st, n, V,
                                        (const int32_t*)m->a.id, P, grp, hook ? draws : nullptr,
                                        rng, step, (Real)p.Lx, (Real)p.Ly, m->d_ctr, G, ng,
                                        m->cand_id, m->cand_slot, m->cand_x, m->cand_y, m->n_cand);

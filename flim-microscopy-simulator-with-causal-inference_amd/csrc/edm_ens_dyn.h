@@ -345,6 +345,7 @@ static CatArgs<Real> ens_cat_host(const edm_ensemble* e, int start, int nf, cons
     A.step = 0;
     A.deferred = r.fb1.sensor >= 0 && (r.fb1.sensor == r.b || r.fb1.sensor == r.c);
     A.by_substrate = cat_by_substrate(e->seg_alive[&g - e->seg.data()], A.enz, nf, r.b);
+    A.group_shift = 6;
     return A;
 }
 

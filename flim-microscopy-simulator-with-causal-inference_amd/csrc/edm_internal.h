@@ -17,6 +17,46 @@
 
 // Cell rows per strip tile of the tiled dynamics kernels.
 static constexpr int kTileRows = 64;
+// work lists: kWlPer slots per thread of a 256-thread build block; lists 0
+// (pass species), 1 (dye carriers), 2.. (catalysis walk entities)
+static constexpr int kWlPer = 8;
+static constexpr int kWlSlots = kWlPer * 256;
+static constexpr int kWlMaxLists = 6;
+static constexpr int kWlPass = 0, kWlDye = 1, kWlCat0 = 2;
+
+// Work-list membership: species bit set and every flag of `flags` set.
+struct WlSpec {
+    uint32_t species;
+    uint32_t flags;
+};
+
+// One list as its consumers read it: a (blocks, kWlPer) grid of 256-thread
+// blocks; block (b, y) takes entries y * 256 .. y * 256 + 255 of region b.
+struct WlView {
+    const int32_t* list;
+    const uint32_t* cnt;
+};
+
+// Entries of this block's slice (0: the block has no work; uniform).
+__device__ __forceinline__ int wl_slice(const WlView& V) {
+    const int c = (int)V.cnt[blockIdx.x] - (int)(blockIdx.y * blockDim.x);
+    return c > 0 ? min(c, (int)blockDim.x) : 0;
+}
+
+__device__ __forceinline__ int wl_slot(const WlView& V) {
+    return V.list[(size_t)blockIdx.x * kWlSlots + blockIdx.y * blockDim.x + threadIdx.x];
+}
+
+// How one catalysis run (reactions ri .. ri + nf - 1, one launch) walks its
+// pairs this step.
+struct CatPlan {
+    int ri, nf;
+    bool by_sub;  // from the substrates' side
+    int list;     // work list of its walk entities, or -1: strip kernel
+    bool late;    // an earlier reaction of the step may change its entities:
+                  // the list is built just before the run
+    WlSpec spec;
+};
 
 namespace edm {
 
@@ -104,13 +144,16 @@ struct edm_membrane {
     // tuning (every value gives the same results)
     int tile_rows = kTileRows;  // cell rows per strip of the tiled force / catalysis kernels
     int cat_side = -1;          // catalysis walk: -1 the rarer side, 0 enzymes, 1 substrates
-    // Work lists rebuilt by every step's gather: the slots of pass-species
-    // particles and the slots of dye carriers (valid once a step has run
-    // since the last upload)
-    int32_t* wl_pass = nullptr;  // [max(n_pass, 1)]
-    int32_t* wl_dye = nullptr;   // [max(n_dye, 1)]
-    uint32_t* wl_cnt = nullptr;  // [2] entries of this step's lists
+    int cat_walk = 0;           // 1: catalysis runs walk work lists where they can; 0: strips
+    int cat_lanes = 16;         // lanes per entity of the strip walk (1, 2, 4, .., 64)
+    // Work lists rebuilt after every sort (valid once a step has run since
+    // the last upload): kWlMaxLists lists of wl_nblk regions of kWlSlots
+    int32_t* wl = nullptr;       // [kWlMaxLists][wl_nblk * kWlSlots]
+    uint32_t* wl_cnt = nullptr;  // [kWlMaxLists][wl_nblk]
+    int wl_nblk = 0;
     bool wl_valid = false;
+    uint32_t wl_built = 0;  // lists built since the last sort (bit per list)
+    std::vector<CatPlan> cat_plan;  // this step's catalysis runs
     std::vector<int> group_start;    // per reaction: group starting here, else -1
     std::vector<int> pass_deferred;  // per reaction: reads a count it writes
 
@@ -169,6 +212,12 @@ struct edm_membrane {
     bool time_force = false;
     double force_ms_acc = 0.0;
 };
+
+static inline WlView wl_view(const edm_membrane* m, int L) {
+    if (!m->wl_valid || !((m->wl_built >> L) & 1u)) return WlView{nullptr, nullptr};
+    return WlView{m->wl + (size_t)L * m->wl_nblk * kWlSlots,
+                  m->wl_cnt + (size_t)L * m->wl_nblk};
+}
 
 namespace edm {
 // dynamics translation unit

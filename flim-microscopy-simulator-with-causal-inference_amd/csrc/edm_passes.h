@@ -7,8 +7,8 @@
 // reference's order, provided no pass reads a density count that an
 // earlier pass of the run changes (the run is split there on the host).
 // A recruitment pass ends its run: the index-order cap needs all
-// candidates before any is accepted (k_recruit_apply).  With a work list
-// (the slots of the species the passes can touch) a thread takes one listed
+// candidates before any is accepted (k_recruit_apply).  With the step's
+// work list of the species the passes can touch a thread takes one listed
 // slot instead of one slot of the whole state.
 
 struct PassDev {
@@ -151,18 +151,27 @@ __device__ __forceinline__ void passes_count(int det_hits, int conv_hits, unsign
     }
 }
 
+// One listed slot per thread (the step's pass-species work list), or one
+// slot per thread without a list.
 template <typename Real>
-__global__ void k_passes(int nt, int n, const int32_t* __restrict__ list,
-                         const uint32_t* __restrict__ lcnt, const int32_t* __restrict__ id,
-                         Post<Real> P, PassGroupDev grp, const double* __restrict__ draws, Rng rng,
-                         uint32_t step, Real Lx, Real Ly, unsigned long long* ctr, Grids<Real> G,
-                         int n_grids, int32_t* cid, int32_t* cslot, double* cxo, double* cyo,
-                         unsigned int* nc) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_passes(int n, WlView V, const int32_t* __restrict__ id,
+                                                Post<Real> P, PassGroupDev grp,
+                                                const double* __restrict__ draws, Rng rng,
+                                                uint32_t step, Real Lx, Real Ly,
+                                                unsigned long long* ctr, Grids<Real> G,
+                                                int n_grids, int32_t* cid, int32_t* cslot,
+                                                double* cxo, double* cyo, unsigned int* nc) {
+    int t = -1;
+    if (V.list) {
+        const int m = wl_slice(V);
+        if (m == 0) return;  // uniform: an empty slice
+        if ((int)threadIdx.x < m) t = wl_slot(V);
+    } else {
+        const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+        if (q < n) t = q;
+    }
     int det_hits = 0, conv_hits = 0;
-    if (list) nt = min(nt, (int)*lcnt);
-    if (q < nt) {
-        const int t = list ? list[q] : q;
+    if (t >= 0) {
         const int s0 = P.sp[t];
         if ((grp.species >> s0) & 1u)
             passes_particle(t, s0, n, id, 0, P, grp, draws, rng, step, Lx, Ly, G, n_grids, cid,

@@ -466,8 +466,7 @@ __device__ __forceinline__ void photon_count(int emitted, int detected, unsigned
 
 // Single membrane: one thread per dye carrier.
 template <typename Real>
-__global__ void k_pss_image(int n_dye, const int32_t* dye_ids, const int32_t* dye_slots,
-                            const uint32_t* n_slots, const int32_t* id,
+__global__ void k_pss_image(int n_dye, const int32_t* dye_ids, WlView V, const int32_t* id,
                             const edm_photo_params* pp, const edm_image_params* ip, const Real* x,
                             const Real* y, uint8_t* fl, const int8_t* dye, int8_t* ctmc,
                             const int32_t* bond, const double* k_fret, const int32_t* acc,
@@ -475,10 +474,12 @@ __global__ void k_pss_image(int n_dye, const int32_t* dye_ids, const int32_t* dy
                             unsigned long long* ctr) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     int emitted = 0, detected = 0;
-    if (dye_slots) {
+    if (V.list) {
         // this step's dye work list (slots); FRET from the bond maps
-        if (q < min(n_dye, (int)*n_slots)) {
-            const int t = dye_slots[q];
+        const int m = wl_slice(V);
+        if (m == 0) return;  // uniform: an empty slice
+        if ((int)threadIdx.x < m) {
+            const int t = wl_slot(V);
             emit_image<Real>(pp, ip, t, (uint32_t)id[t], x, y, fl, dye, ctmc, bond, slot_of,
                              false, 0.0, -1, flim, rng, step, emitted, detected);
         }
@@ -844,19 +845,20 @@ int ph_fused_image(edm_membrane* m, int64_t* n_emitted, int64_t* n_detected) {
     const uint32_t step = m->step;
     const edm_image_params* d_ip = dev_ip(m);
     // after a step the dye carriers' slots are listed in cell order
-    const int32_t* wl = (!prox && m->wl_valid) ? (const int32_t*)m->wl_dye : nullptr;
+    const WlView V = prox ? WlView{nullptr, nullptr} : wl_view(m, kWlDye);
+    const dim3 grid = V.list ? dim3(m->wl_nblk, kWlPer) : dim3(nb(m->n_dye));
     if (m->fp64)
-        hipLaunchKernelGGL(k_pss_image<double>, dim3(nb(m->n_dye)), dim3(kB), 0, st, m->n_dye,
-                           (const int32_t*)m->dye_ids, wl, (const uint32_t*)m->wl_cnt + 1,
-                           (const int32_t*)m->a.id, dpp, d_ip, (const double*)m->a.x,
+        hipLaunchKernelGGL(k_pss_image<double>, grid, dim3(kB), 0, st, m->n_dye,
+                           (const int32_t*)m->dye_ids, V, (const int32_t*)m->a.id, dpp, d_ip,
+                           (const double*)m->a.x,
                            (const double*)m->a.y,
                            m->a.fl, (const int8_t*)m->a.dye, m->a.ctmc,
                            (const int32_t*)m->a.bond, kf, ac, (const int32_t*)m->a.slot_of,
                            m->flim, rng, step, m->d_ctr);
     else
-        hipLaunchKernelGGL(k_pss_image<float>, dim3(nb(m->n_dye)), dim3(kB), 0, st, m->n_dye,
-                           (const int32_t*)m->dye_ids, wl, (const uint32_t*)m->wl_cnt + 1,
-                           (const int32_t*)m->a.id, dpp, d_ip, (const float*)m->a.x,
+        hipLaunchKernelGGL(k_pss_image<float>, grid, dim3(kB), 0, st, m->n_dye,
+                           (const int32_t*)m->dye_ids, V, (const int32_t*)m->a.id, dpp, d_ip,
+                           (const float*)m->a.x,
                            (const float*)m->a.y,
                            m->a.fl, (const int8_t*)m->a.dye, m->a.ctmc,
                            (const int32_t*)m->a.bond, kf, ac, (const int32_t*)m->a.slot_of,

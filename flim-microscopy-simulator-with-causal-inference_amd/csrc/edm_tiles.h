@@ -107,13 +107,26 @@ __device__ __forceinline__ void force_strip(const Cells<Real>& C, const ForcePar
                                             uint32_t step, const Grids<Real>& G, int n_grids,
                                             uint32_t keep_mask, const StripGeo& S) {
     constexpr int kCap = 1024;
+    constexpr int kTab = 32 * 32;  // species-pair tables (at most 32 species)
     __shared__ Real s_x[kCap], s_y[kCap];
     __shared__ int8_t s_sp[kCap];
     __shared__ uint8_t s_fl[kCap];
     __shared__ int s_off[kStripCells + 1];
     __shared__ int s_gst[kStripCells];
     __shared__ int s_wsum[4];
+    __shared__ Real s_sig[kTab], s_eps[kTab], s_kh[kTab];
 
+    // the species-pair tables in LDS (read per candidate pair); the
+    // barriers of strip_cells and of the staging below order them
+    for (int k = (int)threadIdx.x; k < F.ns * F.ns; k += (int)blockDim.x) {
+        s_sig[k] = F.sig[k];
+        s_eps[k] = F.eps[k];
+        s_kh[k] = F.kh[k];
+    }
+    ForceParams<Real> FL = F;
+    FL.sig = s_sig;
+    FL.eps = s_eps;
+    FL.kh = s_kh;
     const int total = strip_cells(C, S, s_gst, s_off, s_wsum);
     const bool staged = total <= kCap;
     if (staged) {
@@ -183,19 +196,20 @@ __device__ __forceinline__ void force_strip(const Cells<Real>& C, const ForcePar
                     for (int col = 0; col < 3; ++col) {
                         const int k0 = col * S.nrow + r - 1;
                         const int ub = s_off[k0 + 3];
+                        // (the particle itself is among the candidates: at d^2 = 0 no
+                        // potential acts)
                         for (int u = s_off[k0]; u < ub; ++u) {
-                            if (u == i) continue;
                             Real dx, dy;
                             const Real dsq = min_image(xt, yt, s_x[u], s_y[u], C.Lx, C.Ly, C.px,
                                                        C.py, dx, dy);
                             if (dsq >= rl2 || !(s_fl[u] & EDM_FLAG_ALIVE)) continue;
                             const int su = s_sp[u];
                             Real f;
-                            if (F.wca && pair_force(F, 0, si, su, dsq, f)) {
+                            if (F.wca && pair_force(FL, 0, si, su, dsq, f)) {
                                 Fx += f * dx;
                                 Fy += f * dy;
                             }
-                            if (F.harm && pair_force(F, 1, si, su, dsq, f)) {
+                            if (F.harm && pair_force(FL, 1, si, su, dsq, f)) {
                                 Fx += f * dx;
                                 Fy += f * dy;
                             }

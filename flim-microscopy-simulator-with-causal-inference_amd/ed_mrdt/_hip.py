@@ -103,6 +103,8 @@ class Replica(ctypes.Structure):
 # tuning keys (edm_set_tuning)
 TUNE_TILE_ROWS = 1
 TUNE_CAT_SIDE = 2
+TUNE_CAT_WALK = 3
+TUNE_CAT_LANES = 4
 
 # function name -> (argtypes) ; every function returns int32 status
 _SIGS = {

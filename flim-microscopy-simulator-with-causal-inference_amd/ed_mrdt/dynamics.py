@@ -438,20 +438,25 @@ class DynamicsEngine:
         return out.reshape(nxg, nyg)
 
     # ---- tuning (no reference counterpart) ----------------------------------
-    def set_tuning(self, tile_rows: Optional[int] = None, cat_side: Optional[int] = None) -> None:
+    _TUNE = (("tile_rows", _hip.TUNE_TILE_ROWS), ("cat_side", _hip.TUNE_CAT_SIDE),
+             ("cat_walk", _hip.TUNE_CAT_WALK), ("cat_lanes", _hip.TUNE_CAT_LANES))
+
+    def set_tuning(self, tile_rows: Optional[int] = None, cat_side: Optional[int] = None,
+                   cat_walk: Optional[int] = None, cat_lanes: Optional[int] = None) -> None:
         """Work-decomposition knobs that leave every result unchanged: cell
-        rows per strip of the tiled kernels (1..64) and the side catalysis
-        walks its pairs from (0 enzymes, 1 substrates, -1 the rarer one)."""
-        if tile_rows is not None:
-            check(lib().edm_set_tuning(self._h, _hip.TUNE_TILE_ROWS, int(tile_rows)),
-                  "edm_set_tuning")
-        if cat_side is not None:
-            check(lib().edm_set_tuning(self._h, _hip.TUNE_CAT_SIDE, int(cat_side)),
-                  "edm_set_tuning")
+        rows per strip of the tiled kernels (1..64), the side catalysis walks
+        its pairs from (0 enzymes, 1 substrates, -1 the rarer one), how
+        (1 work lists where possible, 0 LDS strips) and the lanes per entity
+        of the strip walk (a power of two in 1..64)."""
+        vals = dict(tile_rows=tile_rows, cat_side=cat_side, cat_walk=cat_walk,
+                    cat_lanes=cat_lanes)
+        for name, key in self._TUNE:
+            if vals[name] is not None:
+                check(lib().edm_set_tuning(self._h, key, int(vals[name])), "edm_set_tuning")
 
     def tuning(self) -> Dict[str, int]:
         out = {}
-        for name, key in (("tile_rows", _hip.TUNE_TILE_ROWS), ("cat_side", _hip.TUNE_CAT_SIDE)):
+        for name, key in self._TUNE:
             v = ctypes.c_int64(0)
             check(lib().edm_get_tuning(self._h, key, ctypes.byref(v)), "edm_get_tuning")
             out[name] = int(v.value)

@@ -1,8 +1,10 @@
 """Autotuning of the kernels' work decomposition.
 
-The knobs -- cell rows per strip of the tiled force and catalysis kernels,
-and the side (enzymes, substrates, or per run the side with fewer alive
-particles) catalysis walks its pairs from -- leave
+The knobs -- how catalysis walks its pairs (work lists of its enzymes or
+substrates, or LDS strips of cells), from which side (enzymes, substrates,
+or per run the side with fewer alive particles), with how many lanes per
+entity, and the cell rows per strip of the tiled force and catalysis
+kernels -- leave
 every result bit-identical, so the fastest setting can be found by timing.
 `autotune` builds a scratch engine from the configuration (its own initial
 state, Philox draws), times each candidate over a few steps and returns the
@@ -19,13 +21,19 @@ from .config import SimulationConfig
 
 TILE_ROWS = (8, 16, 32, 64)
 CAT_SIDES = (-1, 0, 1)
+CAT_WALKS = (1, 0)
+CAT_LANES = (64, 16, 4, 1)
 
 
 def autotune(config: SimulationConfig, precision: str = "fp32", steps: int = 20,
              warmup: int = 3, tile_rows: Iterable[int] = TILE_ROWS,
-             cat_sides: Iterable[int] = CAT_SIDES) -> Dict:
-    """Fastest tile_rows and cat_side for this configuration; `times_ms`
-    holds the measured milliseconds per step of every candidate."""
+             cat_sides: Iterable[int] = CAT_SIDES,
+             cat_walks: Iterable[int] = CAT_WALKS,
+             cat_lanes: Iterable[int] = CAT_LANES) -> Dict:
+    """Fastest cat_walk, cat_side, cat_lanes and tile_rows for this
+    configuration (tuned one after another in that order, each with the
+    best of the earlier ones); `times_ms` holds the measured milliseconds
+    per step of every candidate."""
     from .dynamics import DynamicsEngine
     from .particles import ParticleState
 
@@ -43,12 +51,24 @@ def autotune(config: SimulationConfig, precision: str = "fp32", steps: int = 20,
 
     times: Dict[str, float] = {}
     try:
+        walks = list(cat_walks)
+        for w in walks:
+            eng.set_tuning(cat_walk=w)
+            times[f"cat_walk={w}"] = per_step()
+        best_walk = min(walks, key=lambda w: times[f"cat_walk={w}"])
+        eng.set_tuning(cat_walk=best_walk)
         sides = list(cat_sides)
         for s in sides:
             eng.set_tuning(cat_side=s)
             times[f"cat_side={s}"] = per_step()
         best_side = min(sides, key=lambda s: times[f"cat_side={s}"])
         eng.set_tuning(cat_side=best_side)
+        lanes = list(cat_lanes)
+        for g in lanes:
+            eng.set_tuning(cat_lanes=g)
+            times[f"cat_lanes={g}"] = per_step()
+        best_lanes = min(lanes, key=lambda g: times[f"cat_lanes={g}"])
+        eng.set_tuning(cat_lanes=best_lanes)
         rows = list(tile_rows)
         for r in rows:
             eng.set_tuning(tile_rows=r)
@@ -57,4 +77,5 @@ def autotune(config: SimulationConfig, precision: str = "fp32", steps: int = 20,
     finally:
         st.mark_stale()
         eng.close()
-    return dict(tile_rows=best_rows, cat_side=best_side, times_ms=times)
+    return dict(tile_rows=best_rows, cat_side=best_side, cat_walk=best_walk,
+                cat_lanes=best_lanes, times_ms=times)

@@ -181,9 +181,16 @@ int edm_photon_counters(edm_membrane *m, int64_t out[2]);
  *                       kernels, 1..64 (default 64)
  *   EDM_TUNE_CAT_SIDE   catalysis walks its pairs from the enzymes (0) or
  *                       the substrates (1); -1 (default): the side with fewer
- *                       alive particles at upload                        */
+ *                       alive particles at upload
+ *   EDM_TUNE_CAT_WALK   0 (default): catalysis walks strips of cells staged
+ *                       in LDS; 1: runs walk the step's list of their
+ *                       enzymes or substrates where they can
+ *   EDM_TUNE_CAT_LANES  lanes that walk one entity's candidates in the strip
+ *                       walk: a power of two in 1..64 (default 16)        */
 #define EDM_TUNE_TILE_ROWS 1
 #define EDM_TUNE_CAT_SIDE 2
+#define EDM_TUNE_CAT_WALK 3
+#define EDM_TUNE_CAT_LANES 4
 int edm_set_tuning(edm_membrane *m, int32_t key, int64_t value);
 int edm_get_tuning(edm_membrane *m, int32_t key, int64_t *value);
 /* Reference pair list in reference order (CellList.get_neighbor_pairs,

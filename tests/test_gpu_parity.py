@@ -85,14 +85,18 @@ def test_plain_membrane_dynamics_fp64(oracle_lib, n_steps, exact):
     print(f"plain membrane {n_steps} steps: worst relative position error {worst:.3e}")
 
 
-@pytest.mark.parametrize("cat_side,tile_rows", [(0, 64), (1, 8)])
-def test_tuned_membrane_matches_oracle(oracle_lib, cat_side, tile_rows):
-    """Catalysis walked from either side, tiled kernels with short strips:
-    still the oracle's events and positions (FP64, injected draws)."""
+@pytest.mark.parametrize("cat_side,tile_rows,cat_walk,cat_lanes",
+                         [(0, 64, 0, 64), (1, 8, 0, 4), (0, 64, 1, 64), (1, 64, 1, 64),
+                          (0, 16, 0, 1)])
+def test_tuned_membrane_matches_oracle(oracle_lib, cat_side, tile_rows, cat_walk, cat_lanes):
+    """Catalysis walked from either side, over LDS strips (short ones too)
+    or work lists: still the oracle's events and positions (FP64, injected
+    draws)."""
     cfg = workloads.plain_membrane()
     st, eng, ost, odyn = _pair(cfg, exact=False)
-    eng.set_tuning(tile_rows=tile_rows, cat_side=cat_side)
-    assert eng.tuning() == dict(tile_rows=tile_rows, cat_side=cat_side)
+    tune = dict(tile_rows=tile_rows, cat_side=cat_side, cat_walk=cat_walk, cat_lanes=cat_lanes)
+    eng.set_tuning(**tune)
+    assert eng.tuning() == tune
     for k in range(100):
         eng.step()
         odyn.step()
