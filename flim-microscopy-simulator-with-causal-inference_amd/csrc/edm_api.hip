@@ -218,7 +218,7 @@ int edm_set_tuning(edm_membrane* m, int32_t key, int64_t value) {
     switch (key) {
         case EDM_TUNE_TILE_ROWS:
             if (value < 1 || value > kTileRows)
-                return invalid("edm_set_tuning: tile rows must be in 1..64");
+                return invalid("edm_set_tuning: tile rows must be in 1..256");
             m->tile_rows = (int)value;
             return 0;
         case EDM_TUNE_CAT_SIDE:

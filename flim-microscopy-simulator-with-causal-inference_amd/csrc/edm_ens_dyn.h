@@ -29,7 +29,7 @@ __device__ __forceinline__ Cells<Real> ens_cells(const EnsSeg& g, const Real* x,
     C.npairs = nullptr;
     C.cap_active = false;
     C.kbase = g.kbase;
-    C.tile_rows = kTileRows;
+    C.tile_rows = kTileRows;  // unused: the strip table gives each strip's rows
     return C;
 }
 
@@ -74,7 +74,7 @@ __device__ __forceinline__ StripGeo ens_strip(const EnsSeg& g, const EnsTile& T)
     S.tile = 0;
     S.cx = T.cx;
     S.cy0 = T.cy0;
-    S.rows = min(kTileRows, g.ncy - T.cy0);
+    S.rows = min(T.rows, g.ncy - T.cy0);
     S.nrow = S.rows + 2;
     S.ncells = 3 * S.nrow;
     return S;
@@ -345,7 +345,7 @@ static CatArgs<Real> ens_cat_host(const edm_ensemble* e, int start, int nf, cons
     A.step = 0;
     A.deferred = r.fb1.sensor >= 0 && (r.fb1.sensor == r.b || r.fb1.sensor == r.c);
     A.by_substrate = cat_by_substrate(e->seg_alive[&g - e->seg.data()], A.enz, nf, r.b);
-    A.group_shift = 6;
+    A.group_shift = 4;  // 16 lanes per entity
     return A;
 }
 
@@ -476,6 +476,20 @@ int ens_create(edm_ensemble* e) { return e->fp64 ? ens_create_t<double>(e) : ens
 // the strip table, and buffer growth.  Runs between steps only.
 int ens_layout(edm_ensemble* e) {
     const int ns = e->p.n_species;
+    // strip height: about 192 own particles per 256-thread workgroup at the
+    // live replicas' mean density, with the staged 3 x (rows + 2) cells well
+    // inside the LDS capacity (1024 particles); results do not depend on it
+    {
+        int64_t np = 0, nc = 0;
+        for (int r = 0; r < e->rmax; ++r)
+            if (e->seg[r].active) {
+                np += e->seg[r].n;
+                nc += (int64_t)e->seg[r].ncx * e->seg[r].ncy;
+            }
+        const double dens = std::max(nc > 0 ? (double)np / (double)nc : 1.0, 1e-3);
+        const int cap_rows = (int)(1024.0 / (4.5 * dens)) - 2;
+        e->tile_rows = std::max(16, std::min({(int)(192.0 / dens), cap_rows, kTileRows}));
+    }
     int64_t coff = 0, cnt = 0;
     e->tiles.clear();
     std::vector<int32_t> wrep[2];
@@ -495,9 +509,10 @@ int ens_layout(edm_ensemble* e) {
             g.goff[kp.first * ns + kp.second] = (int32_t)cnt;
             cnt += (int64_t)g.gdims[2 * kp.first] * g.gdims[2 * kp.first + 1];
         }
-        const int ty = (g.ncy + kTileRows - 1) / kTileRows;
+        const int tr = e->tile_rows;
+        const int ty = (g.ncy + tr - 1) / tr;
         for (int cx = 0; cx < g.ncx; ++cx)
-            for (int t = 0; t < ty; ++t) e->tiles.push_back(EnsTile{r, cx, t * kTileRows, 0});
+            for (int t = 0; t < ty; ++t) e->tiles.push_back(EnsTile{r, cx, t * tr, tr});
     }
     if (coff >= (int64_t)INT32_MAX - 2 || cnt >= (int64_t)INT32_MAX) {
         set_error("edm_ensemble: too many cells for 32-bit keys");

@@ -73,7 +73,7 @@ struct EnsChunk {
 
 // Strip tile of one replica.
 struct EnsTile {
-    int32_t r, cx, cy0, pad;
+    int32_t r, cx, cy0, rows;
 };
 
 struct edm_ensemble {
@@ -115,6 +115,7 @@ struct edm_ensemble {
     std::vector<EnsSeg> seg;  // host mirror, [rmax]
     EnsSeg* d_seg = nullptr;
     std::vector<EnsTile> tiles;
+    int tile_rows = 64;  // cell rows per strip, from the model's particle density
     EnsTile* d_tiles = nullptr;
     int64_t tile_cap = 0;
     int64_t n_total = 0;  // particles in the pool (live replicas first after a sort)

@@ -16,7 +16,7 @@
 #define EDM_FLAG_BLEACHED 4u
 
 // Cell rows per strip tile of the tiled dynamics kernels.
-static constexpr int kTileRows = 64;
+static constexpr int kTileRows = 256;  // most cell rows per strip of the tiled kernels
 // work lists: kWlPer slots per thread of a 256-thread build block; lists 0
 // (pass species), 1 (dye carriers), 2.. (catalysis walk entities)
 static constexpr int kWlPer = 8;
@@ -142,7 +142,7 @@ struct edm_membrane {
     int n_pass = 0;               // particles of those species (fixed at upload)
     std::vector<int64_t> alive_at_upload;  // alive particles per species at upload
     // tuning (every value gives the same results)
-    int tile_rows = kTileRows;  // cell rows per strip of the tiled force / catalysis kernels
+    int tile_rows = 64;         // cell rows per strip of the tiled force / catalysis kernels
     int cat_side = -1;          // catalysis walk: -1 the rarer side, 0 enzymes, 1 substrates
     int cat_walk = 0;           // 1: catalysis runs walk work lists where they can; 0: strips
     int cat_lanes = 16;         // lanes per entity of the strip walk (1, 2, 4, .., 64)

@@ -2,7 +2,8 @@
 // edm_dynamics.hip; relies on its Cells / ForceParams / Grids helpers).
 //
 // A workgroup of 256 threads owns a strip of C.tile_rows (at most
-// kTileRows) cells of one cell column.  The strip's 3 x (rows + 2) neighbourhood cells (columns cx-1..cx+1,
+// kTileRows) cells of one cell column (sparser membranes take taller strips,
+// so that a workgroup owns about as many particles as it has threads).  The strip's 3 x (rows + 2) neighbourhood cells (columns cx-1..cx+1,
 // rows cy0-1..cy0+rows, wrapped or clipped at the box edges) are staged in
 // LDS, one particle per thread, so that consecutive threads read consecutive
 // slots (strip_src).  Inside the strip every particle's 3x3
@@ -42,37 +43,55 @@ __device__ __forceinline__ StripGeo strip_geo(const Cells<Real>& C, int tile, in
 template <typename Real>
 __device__ __forceinline__ int strip_cells(const Cells<Real>& C, const StripGeo& S, int* s_gst,
                                            int* s_off, int* s_wsum) {
-    const int k = threadIdx.x;
-    int cnt = 0;
-    if (k < S.ncells) {
-        const int col = k / S.nrow, r = k % S.nrow;
-        int gx = S.cx + col - 1, gy = S.cy0 + r - 1;
-        bool ok = true;
-        if (C.px) gx = (gx + C.ncx) % C.ncx;
-        else if (gx < 0 || gx >= C.ncx) ok = false;
-        if (C.py) gy = (gy + C.ncy) % C.ncy;
-        else if (gy < 0 || gy >= C.ncy) ok = false;
-        int gs = 0;
-        if (ok) {
-            const int c = gx * C.ncy + gy;
-            gs = (int)C.start[c];
-            cnt = (int)C.start[c + 1] - gs;
+    // thread i takes the consecutive cells i*per .. i*per + per - 1 (one
+    // cell per thread for strips of up to 256 cells)
+    constexpr int kPer = (kStripCells + 255) / 256;
+    const int per = S.ncells <= 256 ? 1 : kPer;
+    int cnt[kPer];
+    int sum = 0;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        cnt[j] = 0;
+        if (j >= per) break;
+        const int k = (int)threadIdx.x * per + j;
+        if (k < S.ncells) {
+            const int col = (k >= S.nrow) + (k >= 2 * S.nrow), r = k - col * S.nrow;
+            int gx = S.cx + col - 1, gy = S.cy0 + r - 1;
+            bool ok = true;
+            if (C.px) gx = gx < 0 ? gx + C.ncx : (gx >= C.ncx ? gx - C.ncx : gx);
+            else if (gx < 0 || gx >= C.ncx) ok = false;
+            if (C.py) gy = gy < 0 ? gy + C.ncy : (gy >= C.ncy ? gy - C.ncy : gy);
+            else if (gy < 0 || gy >= C.ncy) ok = false;
+            int gs = 0;
+            if (ok) {
+                const int c = gx * C.ncy + gy;
+                gs = (int)C.start[c];
+                cnt[j] = (int)C.start[c + 1] - gs;
+            }
+            s_gst[k] = gs;
         }
-        s_gst[k] = gs;
+        sum += cnt[j];
     }
-    // block-wide exclusive scan: wavefront scans, then wavefront totals
-    const int lane = lane_id(), wave = k >> 6;
-    int v = cnt;
+    // block-wide exclusive scan of the threads' sums: wavefront scans, then
+    // wavefront totals
+    const int lane = lane_id(), wave = (int)(threadIdx.x >> 6);
+    int v = sum;
     for (int o = 1; o < 64; o <<= 1) {
         const int t = __shfl_up(v, o);
         if (lane >= o) v += t;
     }
     if (lane == 63) s_wsum[wave] = v;
     __syncthreads();
-    int base = 0;
-    for (int w = 0; w < wave; ++w) base += s_wsum[w];
-    if (k < S.ncells) s_off[k] = base + v - cnt;
-    if (k == S.ncells - 1) s_off[S.ncells] = base + v;
+    int run = v - sum;
+    for (int w = 0; w < wave; ++w) run += s_wsum[w];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        if (j >= per) break;
+        const int k = (int)threadIdx.x * per + j;
+        if (k < S.ncells) s_off[k] = run;
+        if (k == S.ncells - 1) s_off[S.ncells] = run + cnt[j];
+        run += cnt[j];
+    }
     __syncthreads();
     return s_off[S.ncells];
 }
@@ -107,7 +126,8 @@ __device__ __forceinline__ void force_strip(const Cells<Real>& C, const ForcePar
                                             uint32_t step, const Grids<Real>& G, int n_grids,
                                             uint32_t keep_mask, const StripGeo& S) {
     constexpr int kCap = 1024;
-    constexpr int kTab = 32 * 32;  // species-pair tables (at most 32 species)
+    constexpr int kTabSpecies = 16;  // species-pair tables in LDS up to this many species
+    constexpr int kTab = kTabSpecies * kTabSpecies;
     __shared__ Real s_x[kCap], s_y[kCap];
     __shared__ int8_t s_sp[kCap];
     __shared__ uint8_t s_fl[kCap];
@@ -116,13 +136,16 @@ __device__ __forceinline__ void force_strip(const Cells<Real>& C, const ForcePar
     __shared__ int s_wsum[4];
     __shared__ Real s_sig[kTab], s_eps[kTab], s_kh[kTab];
 
-    // the species-pair tables in LDS (read per candidate pair); the
-    // barriers of strip_cells and of the staging below order them
-    for (int k = (int)threadIdx.x; k < F.ns * F.ns; k += (int)blockDim.x) {
-        s_sig[k] = F.sig[k];
-        s_eps[k] = F.eps[k];
-        s_kh[k] = F.kh[k];
-    }
+    // the species-pair tables in LDS (read per candidate pair; beyond
+    // kTabSpecies species they stay in global memory); the barriers of
+    // strip_cells and of the staging below order them
+    const bool lds_tab = F.ns <= kTabSpecies;
+    if (lds_tab)
+        for (int k = (int)threadIdx.x; k < F.ns * F.ns; k += (int)blockDim.x) {
+            s_sig[k] = F.sig[k];
+            s_eps[k] = F.eps[k];
+            s_kh[k] = F.kh[k];
+        }
     ForceParams<Real> FL = F;
     FL.sig = s_sig;
     FL.eps = s_eps;
@@ -193,28 +216,34 @@ __device__ __forceinline__ void force_strip(const Cells<Real>& C, const ForcePar
                 Real Fx = 0, Fy = 0;
                 if (staged) {
                     const int r = strip_row(C, S, yt);
-                    for (int col = 0; col < 3; ++col) {
-                        const int k0 = col * S.nrow + r - 1;
-                        const int ub = s_off[k0 + 3];
-                        // (the particle itself is among the candidates: at d^2 = 0 no
-                        // potential acts)
-                        for (int u = s_off[k0]; u < ub; ++u) {
-                            Real dx, dy;
-                            const Real dsq = min_image(xt, yt, s_x[u], s_y[u], C.Lx, C.Ly, C.px,
-                                                       C.py, dx, dy);
-                            if (dsq >= rl2 || !(s_fl[u] & EDM_FLAG_ALIVE)) continue;
-                            const int su = s_sp[u];
-                            Real f;
-                            if (F.wca && pair_force(FL, 0, si, su, dsq, f)) {
-                                Fx += f * dx;
-                                Fy += f * dy;
-                            }
-                            if (F.harm && pair_force(FL, 1, si, su, dsq, f)) {
-                                Fx += f * dx;
-                                Fy += f * dy;
+                    // (the particle itself is among the candidates: at d^2 = 0 no
+                    // potential acts)
+                    auto sweep = [&](const ForceParams<Real>& T) {
+                        for (int col = 0; col < 3; ++col) {
+                            const int k0 = col * S.nrow + r - 1;
+                            const int ub = s_off[k0 + 3];
+                            for (int u = s_off[k0]; u < ub; ++u) {
+                                Real dx, dy;
+                                const Real dsq = min_image(xt, yt, s_x[u], s_y[u], C.Lx, C.Ly,
+                                                           C.px, C.py, dx, dy);
+                                if (dsq >= rl2 || !(s_fl[u] & EDM_FLAG_ALIVE)) continue;
+                                const int su = s_sp[u];
+                                Real f;
+                                if (F.wca && pair_force(T, 0, si, su, dsq, f)) {
+                                    Fx += f * dx;
+                                    Fy += f * dy;
+                                }
+                                if (F.harm && pair_force(T, 1, si, su, dsq, f)) {
+                                    Fx += f * dx;
+                                    Fy += f * dy;
+                                }
                             }
                         }
-                    }
+                    };
+                    if (lds_tab)
+                        sweep(FL);
+                    else
+                        sweep(F);
                 } else {
                     int cxx, cyy;
                     cell_of(C, xt, yt, cxx, cyy);

@@ -444,7 +444,7 @@ class DynamicsEngine:
     def set_tuning(self, tile_rows: Optional[int] = None, cat_side: Optional[int] = None,
                    cat_walk: Optional[int] = None, cat_lanes: Optional[int] = None) -> None:
         """Work-decomposition knobs that leave every result unchanged: cell
-        rows per strip of the tiled kernels (1..64), the side catalysis walks
+        rows per strip of the tiled kernels (1..256), the side catalysis walks
         its pairs from (0 enzymes, 1 substrates, -1 the rarer one), how
         (1 work lists where possible, 0 LDS strips) and the lanes per entity
         of the strip walk (a power of two in 1..64)."""

@@ -19,7 +19,7 @@ from typing import Dict, Iterable
 from . import _hip
 from .config import SimulationConfig
 
-TILE_ROWS = (8, 16, 32, 64)
+TILE_ROWS = (16, 32, 64, 128, 256)
 CAT_SIDES = (-1, 0, 1)
 CAT_WALKS = (1, 0)
 CAT_LANES = (64, 16, 4, 1)

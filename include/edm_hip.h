@@ -178,7 +178,7 @@ int edm_photon_counters(edm_membrane *m, int64_t out[2]);
  * Knobs of the kernels' work decomposition; every value gives bit-identical
  * results, so an autotuner may time them on the live membrane.
  *   EDM_TUNE_TILE_ROWS  cell rows per strip of the tiled force and catalysis
- *                       kernels, 1..64 (default 64)
+ *                       kernels, 1..256 (default 64)
  *   EDM_TUNE_CAT_SIDE   catalysis walks its pairs from the enzymes (0) or
  *                       the substrates (1); -1 (default): the side with fewer
  *                       alive particles at upload
