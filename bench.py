@@ -169,12 +169,17 @@ def ensemble_bench(n_rep: int, steps: int, plain_every: int):
     t_b = time.perf_counter() - t0
     eng.close()
     _progress(f"ensemble: batched {steps} steps in {t_b:.2f} s")
+    b_jobs = n_rep / t_b * (steps - 1) / steps  # the timed region ran steps - 1 steps
+    if plain_every <= 0:  # no plain sample (profiling the batched path)
+        return {"replicas": n_rep, "steps": steps, "particles": cap - 1024, "alive": alive,
+                "batched_s": t_b, "batched_jobs_per_s": b_jobs,
+                "batched_particle_steps_per_s": alive * (steps - 1) / t_b,
+                "final_frame_counts": int(counts)}
     sample = [Job(i, c, steps) for i, c in enumerate(cfgs[::plain_every])]
     t1 = time.perf_counter()
     serve_plain(sample, realtime=False, progress=_progress)
     t_p = time.perf_counter() - t1
     _progress(f"ensemble: plain {len(sample)} replicas in {t_p:.2f} s")
-    b_jobs = n_rep / t_b * (steps - 1) / steps  # the timed region ran steps - 1 steps
     p_jobs = len(sample) / t_p
     return {"replicas": n_rep, "steps": steps, "particles": cap - 1024, "alive": alive,
             "batched_s": t_b, "batched_jobs_per_s": b_jobs,
@@ -259,6 +264,9 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=5000)
     ap.add_argument("--plain-every", type=int, default=32)
     ap.add_argument("--served-jobs", type=int, default=48)
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="only the ensemble (and served) legs, for profiling them")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="copy this many MiB device-to-device first (counter calibration)")
     a = ap.parse_args()
@@ -268,6 +276,13 @@ def main():
         sys.exit(2)
     if a.copy_probe_mib:
         copy_probe(a.copy_probe_mib)
+    if a.ensemble_only:
+        out = {"metric": METRIC, "ensemble": ensemble_bench(a.ensemble_replicas, a.ensemble_steps,
+                                                            a.plain_every)}
+        if not a.no_served:
+            out["served"] = served_bench(a.served_jobs, 50.0, 96.0)
+        print(json.dumps(out))
+        return
     r = gpu_run(a.steps, a.warmup, a.n)
     _progress(f"1M membrane: {r['steps']} steps in {r['wall']:.3f} s")
     traffic = traffic_from_profile()
@@ -301,7 +316,8 @@ def main():
     }
     if not a.no_ensemble:
         out["ensemble"] = ensemble_bench(a.ensemble_replicas, a.ensemble_steps, a.plain_every)
-        out["served"] = served_bench(a.served_jobs, 50.0, 96.0)
+        if not a.no_served:
+            out["served"] = served_bench(a.served_jobs, 50.0, 96.0)
     if not a.no_cpu_baseline:
         c = cpu_baseline(a.n, a.cpu_budget)
         out["cpu_baseline"] = {"value": c["value"], "unit": "particle-steps/s", "cores": c["threads"],

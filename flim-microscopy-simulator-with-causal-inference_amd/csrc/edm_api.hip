@@ -188,16 +188,15 @@ int edm_upload_state(edm_membrane* m, const edm_host_state* s) {
     if (!dyes.empty())
         EDM_HIP(hipMemcpy(m->dye_ids, dyes.data(), dyes.size() * 4, hipMemcpyHostToDevice));
     m->n_dye = (int)dyes.size();
-    // particles of the species the per-particle passes can touch; the
-    // gather of every step lists their slots (and the dye carriers' slots)
+    // bonds can exist: given at upload, or made by a binding reaction
+    bool bonds = false;
+    for (auto& r : m->rx) bonds = bonds || r.kind == EDM_RX_BIND;
+    for (int i = 0; i < n && !bonds; ++i) bonds = bond[i] >= 0;
+    m->keep_slot_of = bonds;
+    m->slot_of_valid = true;  // the identity, copied above
     m->alive_at_upload.assign(m->p.n_species, 0);
     for (int i = 0; i < n; ++i)
         if (fl[i] & EDM_FLAG_ALIVE) ++m->alive_at_upload[sp[i]];
-    int n_pass = 0;
-    if (m->pass_species)
-        for (int i = 0; i < n; ++i)
-            if ((m->pass_species >> sp[i]) & 1u) ++n_pass;
-    m->n_pass = n_pass;
     const int nb = std::max(1, (n + kWlSlots - 1) / kWlSlots);
     if (nb != m->wl_nblk) {
         if (m->wl) EDM_HIP(hipFree(m->wl));

@@ -314,7 +314,8 @@ __global__ void k_sort_cells(int ncell, const uint32_t* __restrict__ start, uint
 // One thread per new slot: gather every field from its old slot (the
 // permutation is close to the identity, so reads stay nearly coalesced).
 template <typename Real>
-__global__ void k_gather(int n, const uint32_t* __restrict__ perm, DevArrays a) {
+__global__ void k_gather(int n, const uint32_t* __restrict__ perm, DevArrays a,
+                         bool write_slot_of) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const uint32_t s = perm[t];
@@ -327,7 +328,21 @@ __global__ void k_gather(int n, const uint32_t* __restrict__ perm, DevArrays a) 
     a.bond_t[t] = a.bond[s];
     a.ctmc_t[t] = a.ctmc[s];
     a.dye_t[t] = a.dye[s];
-    a.slot_of[pid] = t;
+    if (write_slot_of) a.slot_of[pid] = t;
+}
+
+__global__ void k_slot_of(int n, const int32_t* __restrict__ id, int32_t* __restrict__ slot_of) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) slot_of[id[t]] = t;
+}
+
+int dyn_ensure_slot_of(edm_membrane* m) {
+    if (m->slot_of_valid || m->n == 0) return 0;
+    hipLaunchKernelGGL(k_slot_of, dim3(nblk(m->n)), dim3(kBlock), 0, m->stream, m->n,
+                       (const int32_t*)m->a.id, m->a.slot_of);
+    EDM_HIP(hipGetLastError());
+    m->slot_of_valid = true;
+    return 0;
 }
 
 // Work lists, rebuilt after every sort: the slots of the particles the
@@ -345,6 +360,10 @@ struct WlBuild {
     int n_lists;
 };
 
+__device__ __forceinline__ bool wl_member(const WlSpec& S, int8_t sp, uint8_t fl) {
+    return ((S.species >> sp) & 1u) && (fl & S.flags) == S.flags;
+}
+
 __global__ __launch_bounds__(256) void k_wl_build(int n, const int8_t* __restrict__ sp,
                                                   const uint8_t* __restrict__ fl, WlBuild W) {
     constexpr int kW = 4;  // wavefronts per block
@@ -360,17 +379,13 @@ __global__ __launch_bounds__(256) void k_wl_build(int n, const int8_t* __restric
         vs[k] = t < end ? sp[t] : (int8_t)0;
         vf[k] = t < end ? fl[t] : (uint8_t)0;
     }
-    uint64_t bal[kWlMaxLists][kWlPer];
-#pragma unroll
-    for (int L = 0; L < kWlMaxLists; ++L) {
-        if (L >= W.n_lists) break;
+    for (int L = 0; L < W.n_lists; ++L) {
         const WlSpec S = W.spec[L];
 #pragma unroll
         for (int k = 0; k < kWlPer; ++k) {
             const bool live = beg + k * 256 + (int)threadIdx.x < end;
-            bal[L][k] = __ballot(live && ((S.species >> vs[k]) & 1u) &&
-                                 (vf[k] & S.flags) == S.flags);
-            if (lane == 0) s_off[L][k * kW + w] = __popcll(bal[L][k]);
+            const uint64_t b = __ballot(live && wl_member(S, vs[k], vf[k]));
+            if (lane == 0) s_off[L][k * kW + w] = __popcll(b);
         }
     }
     __syncthreads();
@@ -386,15 +401,16 @@ __global__ __launch_bounds__(256) void k_wl_build(int n, const int8_t* __restric
     }
     __syncthreads();
     const uint64_t below = (1ull << lane) - 1ull;
-#pragma unroll
-    for (int L = 0; L < kWlMaxLists; ++L) {
-        if (L >= W.n_lists) break;
+    for (int L = 0; L < W.n_lists; ++L) {
+        const WlSpec S = W.spec[L];
         int32_t* out = W.list[L] + (size_t)blockIdx.x * kWlSlots;
 #pragma unroll
-        for (int k = 0; k < kWlPer; ++k)
-            if ((bal[L][k] >> lane) & 1ull)
-                out[s_off[L][k * kW + w] + __popcll(bal[L][k] & below)] =
-                    beg + k * 256 + (int)threadIdx.x;
+        for (int k = 0; k < kWlPer; ++k) {
+            const int t = beg + k * 256 + (int)threadIdx.x;
+            const bool mem = t < end && wl_member(S, vs[k], vf[k]);
+            const uint64_t b = __ballot(mem);
+            if (mem) out[s_off[L][k * kW + w] + __popcll(b & below)] = t;
+        }
     }
 }
 
@@ -1202,7 +1218,7 @@ void plan_passes(const std::vector<edm_reaction>& rx, uint32_t sensor_mask,
 // Species the passes act on, closed under every reaction that turns one
 // species into another: a particle outside the set is never touched by a
 // pass and never enters the set, so the number of particles in the set is
-// fixed and the passes run over a list of them.
+// fixed (it sizes the batched ensemble's pass lists).
 uint32_t pass_species_of(const std::vector<edm_reaction>& rx,
                          const std::vector<std::vector<int>>& groups) {
     uint32_t S = 0;
@@ -1316,7 +1332,6 @@ static int create_t(edm_membrane* m) {
         EDM_HIP(hipMemset(m->w_adapt, 0, (size_t)g.nx * g.ny * sizeof(double)));
     }
     plan_passes(m->rx, m->sensor_mask_all, m->pass_groups, m->group_start, m->pass_deferred);
-    m->pass_species = pass_species_of(m->rx, m->pass_groups);
     // largest squared range of any pair potential (the tiled force kernel
     // skips candidates beyond it before looking up the species tables)
     {
@@ -1378,6 +1393,53 @@ static int cat_run_len(const std::vector<edm_reaction>& rx, size_t ri) {
 
 static uint32_t sbit(int s) { return s >= 0 ? 1u << s : 0u; }
 
+// Species whose particles reaction q can move into or out of a work list:
+// species changes, and with `alive` also changes of the alive flag.
+static uint32_t touched_species(const edm_reaction& q, bool alive) {
+    switch (q.kind) {
+        case EDM_RX_BIND: return sbit(q.a) | sbit(q.b) | sbit(q.c);
+        case EDM_RX_CAT: return sbit(q.b) | sbit(q.c);
+        case EDM_RX_CONVERT: return sbit(q.a) | sbit(q.b);
+        case EDM_RX_RECRUIT:
+        case EDM_RX_DETACH: return alive ? sbit(q.a) : 0u;
+        default: return 0u;
+    }
+}
+
+// The species a run of passes can touch: every pass's species a, and the
+// species a particle may hold before the run and still reach a pass as its
+// species a through the run's earlier conversions (reach).
+static uint32_t pass_reach(const std::vector<edm_reaction>& rx, const std::vector<int>& g, int k) {
+    uint32_t reach = sbit(rx[g[k]].a);
+    for (int j = k - 1; j >= 0; --j) {
+        const edm_reaction& qj = rx[g[j]];
+        if (qj.kind == EDM_RX_CONVERT && ((reach >> qj.b) & 1u)) reach |= sbit(qj.a);
+    }
+    return reach;
+}
+
+// This step's runs of passes, each with a work list of its species (alive or
+// not) -- built with the others after the sort, or just before the run when
+// an earlier reaction of the step can change the species of its particles.
+static void plan_pass_lists(edm_membrane* m, int& next) {
+    m->pass_plan.clear();
+    for (size_t gi = 0; gi < m->pass_groups.size(); ++gi) {
+        const std::vector<int>& g = m->pass_groups[gi];
+        PassPlan pp{};
+        pp.ri = g.front();
+        pp.list = -1;
+        uint32_t S = 0;
+        for (size_t k = 0; k < g.size(); ++k) S |= pass_reach(m->rx, g, (int)k);
+        if (next < kWlMaxLists) {
+            pp.list = next++;
+            pp.spec = WlSpec{S, 0u};
+            for (int k = 0; k < pp.ri; ++k)
+                if (touched_species(m->rx[k], false) & S) pp.late = true;
+        }
+        m->pass_plan.push_back(pp);
+    }
+}
+
 // This step's catalysis walks.  A run walks a work list of its entities
 // (alive enzymes, or alive substrates) unless an enzyme species is also its
 // substrate or product (an entity could then change identity during the
@@ -1385,9 +1447,8 @@ static uint32_t sbit(int s) { return s >= 0 ? 1u << s : 0u; }
 // list is built with the others after the sort, or -- when an earlier
 // reaction of the step can change the species or alive flag of its
 // entities -- just before the run.
-static void plan_cat_walks(edm_membrane* m) {
+static void plan_cat_walks(edm_membrane* m, int& next) {
     m->cat_plan.clear();
-    int next = kWlCat0;
     for (size_t ri = 0; ri < m->rx.size();) {
         const edm_reaction& r = m->rx[ri];
         if (r.kind != EDM_RX_CAT) {
@@ -1409,19 +1470,8 @@ static void plan_cat_walks(edm_membrane* m) {
         if (m->cat_walk == 1 && !(E & (sbit(r.b) | sbit(r.c))) && next < kWlMaxLists) {
             c.list = next++;
             c.spec = WlSpec{c.by_sub ? sbit(r.b) : E, (uint32_t)EDM_FLAG_ALIVE};
-            for (size_t k = 0; k < ri; ++k) {
-                const edm_reaction& q = m->rx[k];
-                uint32_t touched = 0;
-                switch (q.kind) {
-                    case EDM_RX_BIND: touched = sbit(q.a) | sbit(q.b) | sbit(q.c); break;
-                    case EDM_RX_CAT: touched = sbit(q.b) | sbit(q.c); break;
-                    case EDM_RX_RECRUIT:
-                    case EDM_RX_DETACH: touched = sbit(q.a); break;
-                    case EDM_RX_CONVERT: touched = sbit(q.a) | sbit(q.b); break;
-                    default: break;
-                }
-                if (touched & c.spec.species) c.late = true;
-            }
+            for (size_t k = 0; k < ri; ++k)
+                if (touched_species(m->rx[k], true) & c.spec.species) c.late = true;
         }
         m->cat_plan.push_back(c);
         ri += c.nf;
@@ -1461,23 +1511,27 @@ static int begin_t(edm_membrane* m) {
     hipLaunchKernelGGL(k_sort_cells, dim3(nblk(m->ncell)), dim3(kBlock), 0, st, m->ncell,
                        m->cell_start, m->perm, m->cell_cnt, (const int32_t*)m->a.id);
     hipLaunchKernelGGL(k_gather<Real>, dim3(nblk(n)), dim3(kBlock), 0, st, n,
-                       (const uint32_t*)m->perm, m->a);
+                       (const uint32_t*)m->perm, m->a, m->keep_slot_of);
+    m->slot_of_valid = m->keep_slot_of;
     if (m->wl) {
         // work lists of this step's slot order (the gather wrote it into the
-        // _t arrays, swapped below): pass species, dye carriers and the
+        // _t arrays, swapped below): dye carriers, and the runs of passes and
         // catalysis walks that no earlier reaction of the step can change
-        plan_cat_walks(m);
+        int next = kWlDye + 1;
+        plan_pass_lists(m, next);
+        plan_cat_walks(m, next);
         int lists[kWlMaxLists];
         WlSpec specs[kWlMaxLists];
         int nl = 0;
-        if (m->pass_species) {
-            lists[nl] = kWlPass;
-            specs[nl++] = WlSpec{m->pass_species, 0u};
-        }
         if (m->n_dye > 0) {
             lists[nl] = kWlDye;
             specs[nl++] = WlSpec{0xffffffffu, (uint32_t)EDM_FLAG_DYE};
         }
+        for (const PassPlan& pp : m->pass_plan)
+            if (pp.list >= 0 && !pp.late) {
+                lists[nl] = pp.list;
+                specs[nl++] = pp.spec;
+            }
         for (const CatPlan& c : m->cat_plan)
             if (c.list >= 0 && !c.late) {
                 lists[nl] = c.list;
@@ -1709,16 +1763,17 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
                         d.draw_off = draw_off[g[k]];
                         d.deferred = m->pass_deferred[g[k]];
                         defer_any = defer_any || d.deferred;
-                        uint32_t reach = 1u << q.a;
-                        for (int j = k - 1; j >= 0; --j) {
-                            const edm_reaction& qj = m->rx[g[j]];
-                            if (qj.kind == EDM_RX_CONVERT && ((reach >> qj.b) & 1u))
-                                reach |= 1u << qj.a;
-                        }
-                        d.reach = reach;
-                        grp.species |= reach;
+                        d.reach = pass_reach(m->rx, g, k);
+                        grp.species |= d.reach;
                     }
-                    const WlView V = wl_view(m, kWlPass);
+                    WlView V{nullptr, nullptr};
+                    for (const PassPlan& pp : m->pass_plan)
+                        if (pp.ri == (int)ri && pp.list >= 0 && m->wl_valid) {
+                            if (pp.late)  // the run's particles as they are now
+                                wl_build(m, (const int8_t*)m->a.sp, (const uint8_t*)m->a.fl,
+                                         &pp.list, &pp.spec, 1);
+                            V = wl_view(m, pp.list);
+                        }
                     hipLaunchKernelGGL(k_passes<Real>,
                                        V.list ? dim3(m->wl_nblk, kWlPer) : dim3(nblk(n)),
                                        dim3(kBlock), 0, st, n, V,

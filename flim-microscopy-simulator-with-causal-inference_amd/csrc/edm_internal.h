@@ -17,12 +17,13 @@
 
 // Cell rows per strip tile of the tiled dynamics kernels.
 static constexpr int kTileRows = 256;  // most cell rows per strip of the tiled kernels
-// work lists: kWlPer slots per thread of a 256-thread build block; lists 0
-// (pass species), 1 (dye carriers), 2.. (catalysis walk entities)
+// work lists: kWlPer slots per thread of a 256-thread build block; list 0
+// holds the dye carriers, the others are assigned each step to the reaction
+// passes' runs and the catalysis walks
 static constexpr int kWlPer = 8;
 static constexpr int kWlSlots = kWlPer * 256;
-static constexpr int kWlMaxLists = 6;
-static constexpr int kWlPass = 0, kWlDye = 1, kWlCat0 = 2;
+static constexpr int kWlMaxLists = 8;
+static constexpr int kWlDye = 0;
 
 // Work-list membership: species bit set and every flag of `flags` set.
 struct WlSpec {
@@ -46,6 +47,15 @@ __device__ __forceinline__ int wl_slice(const WlView& V) {
 __device__ __forceinline__ int wl_slot(const WlView& V) {
     return V.list[(size_t)blockIdx.x * kWlSlots + blockIdx.y * blockDim.x + threadIdx.x];
 }
+
+// The work list of one run of per-particle passes (starting at reaction ri)
+// this step: the particles of the species the run can touch.
+struct PassPlan {
+    int ri;
+    int list;  // or -1: the run sweeps every slot
+    bool late;
+    WlSpec spec;
+};
 
 // How one catalysis run (reactions ri .. ri + nf - 1, one launch) walks its
 // pairs this step.
@@ -137,10 +147,12 @@ struct edm_membrane {
 
     // fused per-particle passes: runs of detach / convert / recruit-flag
     std::vector<std::vector<int>> pass_groups;
-    uint32_t pass_species = 0;    // species closed under every transformation that
-                                  // touches a pass species (bit per species)
-    int n_pass = 0;               // particles of those species (fixed at upload)
     std::vector<int64_t> alive_at_upload;  // alive particles per species at upload
+    // slot_of (particle index -> slot) is kept by every gather only when bonds
+    // can exist (bond partners are looked up through it); otherwise it is
+    // rebuilt on demand before a kernel that reads it (dyn_ensure_slot_of)
+    bool keep_slot_of = true;
+    bool slot_of_valid = true;
     // tuning (every value gives the same results)
     int tile_rows = 64;         // cell rows per strip of the tiled force / catalysis kernels
     int cat_side = -1;          // catalysis walk: -1 the rarer side, 0 enzymes, 1 substrates
@@ -153,7 +165,8 @@ struct edm_membrane {
     int wl_nblk = 0;
     bool wl_valid = false;
     uint32_t wl_built = 0;  // lists built since the last sort (bit per list)
-    std::vector<CatPlan> cat_plan;  // this step's catalysis runs
+    std::vector<CatPlan> cat_plan;    // this step's catalysis runs
+    std::vector<PassPlan> pass_plan;  // this step's runs of per-particle passes
     std::vector<int> group_start;    // per reaction: group starting here, else -1
     std::vector<int> pass_deferred;  // per reaction: reads a count it writes
 
@@ -226,6 +239,7 @@ int dyn_step_begin(edm_membrane* m);
 int dyn_step_finish(edm_membrane* m, const double* d_draws_dev, bool hook);
 int dyn_pairs(edm_membrane* m, int32_t* pairs, int64_t cap, int64_t* n_out);
 int dyn_density(edm_membrane* m, int32_t species, int32_t grid, double* out);
+int dyn_ensure_slot_of(edm_membrane* m);
 int64_t hook_draws_per_step(const edm_membrane* m, int64_t n_pairs);
 // photon translation unit
 int ph_alloc(edm_membrane* m);

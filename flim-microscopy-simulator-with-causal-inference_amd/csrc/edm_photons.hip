@@ -745,6 +745,7 @@ static int scan_total(edm_membrane* m, const int64_t* in, const int64_t* out, in
 }
 
 int ph_fret(edm_membrane* m) {
+    if (int rc = dyn_ensure_slot_of(m)) return rc;
     const int n = m->n, nd = m->n_dye;
     hipStream_t st = m->stream;
     if (nd == 0) return 0;
@@ -777,6 +778,7 @@ int ph_fret(edm_membrane* m) {
 
 int ph_hook(edm_membrane* m, const double* u_pois, const double* u_trans, const double* u_arr,
             int64_t n_arr, int64_t* n_photons) {
+    if (int rc = dyn_ensure_slot_of(m)) return rc;
     const int n = m->n;
     hipStream_t st = m->stream;
     if (int rc = ph_fret(m)) return rc;
@@ -846,6 +848,8 @@ int ph_fused_image(edm_membrane* m, int64_t* n_emitted, int64_t* n_detected) {
     const edm_image_params* d_ip = dev_ip(m);
     // after a step the dye carriers' slots are listed in cell order
     const WlView V = prox ? WlView{nullptr, nullptr} : wl_view(m, kWlDye);
+    if (!V.list)  // the dye-index path looks carriers up through slot_of
+        if (int rc = dyn_ensure_slot_of(m)) return rc;
     const dim3 grid = V.list ? dim3(m->wl_nblk, kWlPer) : dim3(nb(m->n_dye));
     if (m->fp64)
         hipLaunchKernelGGL(k_pss_image<double>, grid, dim3(kB), 0, st, m->n_dye,
@@ -939,6 +943,7 @@ int img_detect(edm_membrane* m, const double* u_qe, int64_t n, int64_t* n_surv) 
 }
 
 int img_assign(edm_membrane* m, const double* u_pix, int64_t n, int64_t* n_asg) {
+    if (int rc = dyn_ensure_slot_of(m)) return rc;
     hipStream_t st = m->stream;
     if (n != m->n_surv) return EDM_EINVAL;
     if (n == 0) {
