@@ -98,7 +98,29 @@ __device__ __forceinline__ int cat_try(const Cells<Real>& C, const Post<Real>& P
     return 0;
 }
 
+// Appends v to the block's LDS queue for the lanes with `want` (one atomic
+// per wavefront); returns false for a lane whose entry did not fit.
+__device__ __forceinline__ bool queue_push(int* q, int* nq, int cap, bool want, int v) {
+    const uint64_t b = __ballot(want);
+    if (!b) return true;
+    const int leader = __ffsll((unsigned long long)b) - 1;
+    int base = 0;
+    if (lane_id() == leader) base = atomicAdd(nq, __popcll(b));
+    base = __shfl(base, leader);
+    if (!want) return true;
+    const int k = base + __popcll(b & ((1ull << lane_id()) - 1ull));
+    if (k >= cap) return false;
+    q[k] = v;
+    return true;
+}
+
 // Catalysis over one strip; every thread of the 256-wide block must call it.
+// Staged strips run in two phases: the entity groups scan their candidates
+// and queue the pairs within the cutoff in LDS, then every thread of the
+// block evaluates queued pairs (contact, feedback, draw, CAS), so that the
+// per-pair work runs on full wavefronts.  A pair's outcome does not depend
+// on when it is evaluated (its substrate converts once, on the first
+// accepted pair).
 template <typename Real>
 __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>& P,
                                           const CatArgs<Real>& A, uint32_t step,
@@ -113,8 +135,14 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
     __shared__ int s_gst[kStripCells];
     __shared__ int s_wsum[4];
     __shared__ int s_nlist;
+    constexpr int kQCap = 512;
+    __shared__ int s_q[kQCap];  // queued pairs: own index | candidate << 10 | member << 20
+    __shared__ int s_nq;
 
-    if (threadIdx.x == 0) s_nlist = 0;
+    if (threadIdx.x == 0) {
+        s_nlist = 0;
+        s_nq = 0;
+    }
     const int total = strip_cells(C, S, s_gst, s_off, s_wsum);
     const bool staged = total <= kCap;
     if (staged) {
@@ -219,12 +247,15 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
                     if (min_image(s_x[u], s_y[u], xt, yt, C.Lx, C.Ly, C.px, C.py, dx, dy) >=
                         C.cut2)
                         continue;
-                    const int tu = strip_src(S, s_off, s_gst, u);  // the enzyme
-                    const int iu = C.id[tu];
-                    for (int j = 0; j < A.n_enz; ++j)
-                        if (su == cat_pick(A.enz, j))
-                            hits += cat_try(C, P, A, step, j, G, n_grids, tu, iu, t, s_xn[u],
-                                            s_yn[u], xnt, ynt);
+                    for (int j = 0; j < A.n_enz; ++j) {
+                        const bool want = su == cat_pick(A.enz, j);
+                        if (!queue_push(s_q, &s_nq, kQCap, want,
+                                        (i - own_begin) | (u << 10) | (j << 20))) {
+                            const int tu = strip_src(S, s_off, s_gst, u);  // the enzyme
+                            hits += cat_try(C, P, A, step, j, G, n_grids, tu, C.id[tu], t,
+                                            s_xn[u], s_yn[u], xnt, ynt);
+                        }
+                    }
                 }
             } else {
                 const Real xt = C.x[t], yt = C.y[t], xnt = P.xn[t], ynt = P.yn[t];
@@ -277,8 +308,10 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
                 Real dx, dy;
                 if (min_image(xt, yt, s_x[u], s_y[u], C.Lx, C.Ly, C.px, C.py, dx, dy) >= C.cut2)
                     continue;
-                hits += cat_try(C, P, A, step, j, G, n_grids, t, it,
-                                strip_src(S, s_off, s_gst, u), xnt, ynt, s_xn[u], s_yn[u]);
+                if (!queue_push(s_q, &s_nq, kQCap, true,
+                                (i - own_begin) | (u << 10) | (j << 20)))
+                    hits += cat_try(C, P, A, step, j, G, n_grids, t, it,
+                                    strip_src(S, s_off, s_gst, u), xnt, ynt, s_xn[u], s_yn[u]);
             }
         } else {
             const Real xt = C.x[t], yt = C.y[t], xnt = P.xn[t], ynt = P.yn[t];
@@ -298,6 +331,21 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
                 }
             }
         }
+    }
+    // phase 2: the queued pairs, one per thread (own entity i, candidate u)
+    __syncthreads();
+    const int nq = min(s_nq, kQCap);
+    for (int k = (int)threadIdx.x; k < nq; k += (int)blockDim.x) {
+        const int v = s_q[k];
+        const int i = own_begin + (v & 1023), u = (v >> 10) & 1023, j = v >> 20;
+        const int to = gbase + (i - own_begin);       // the own entity's slot
+        const int tc = strip_src(S, s_off, s_gst, u);  // the candidate's slot
+        if (by_sub)  // enzyme u, substrate i
+            hits += cat_try(C, P, A, step, j, G, n_grids, tc, C.id[tc], to, s_xn[u], s_yn[u],
+                            s_xn[i], s_yn[i]);
+        else  // enzyme i, substrate u
+            hits += cat_try(C, P, A, step, j, G, n_grids, to, C.id[to], tc, s_xn[i], s_yn[i],
+                            s_xn[u], s_yn[u]);
     }
     unsigned long long v = hits;
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
