@@ -115,6 +115,7 @@ int edm_create(const edm_params* p, edm_membrane** out) {
 int edm_destroy(edm_membrane* m) {
     if (!m) return 0;
     if (m->stream) hipStreamSynchronize(m->stream);
+    rec_free(m);
     void* bufs[] = {m->a.x, m->a.y, m->a.xn, m->a.yn, m->a.xt, m->a.yt, m->a.sp, m->a.sp_t, m->a.fl,
                     m->a.fl_t, m->a.id, m->a.id_t, m->a.bond, m->a.bond_t, m->a.ctmc, m->a.ctmc_t,
                     m->a.dye, m->a.dye_t, m->a.slot_of, m->key, m->rank, m->cell_cnt,
@@ -349,6 +350,8 @@ int edm_run(edm_membrane* m, int32_t n_steps) {
         m->need_pair_offsets = m->p.pair_cap > 0;
         if (int rc = dyn_step_begin(m)) return rc;
         if (int rc = dyn_step_finish(m, nullptr, false)) return rc;
+        if (m->rec_flags & EDM_REC_AUTO)
+            if (int rc = rec_step(m)) return rc;
     }
     return 0;
 }
@@ -532,6 +535,8 @@ int edm_time_steps(edm_membrane* m, int32_t n_steps, int32_t image, double* tota
         if (image) {
             if (int rc2 = ph_fused_image(m, nullptr, nullptr)) return rc2;
         }
+        if (m->rec_flags & EDM_REC_AUTO)
+            if (int rc2 = rec_step(m)) return rc2;
     }
     EDM_HIP(hipEventRecord(m->ev1, m->stream));
     EDM_HIP(hipEventSynchronize(m->ev1));

@@ -219,6 +219,16 @@ struct edm_membrane {
     uint32_t* pix_tot = nullptr;
     int64_t* d_scal = nullptr;  // device scalars for reductions
 
+    // census: recording ring, block partials of the reduction, one-row
+    // result for ground truth, per-particle FRET ground truth
+    edm_census* rec = nullptr;
+    int rec_cap = 0, rec_n = 0, rec_flags = 0;
+    unsigned long long* cen_part = nullptr;
+    double* cen_part_e = nullptr;
+    unsigned int* cen_ticket = nullptr;
+    edm_census* cen_row = nullptr;
+    double *gt_e = nullptr, *gt_tau = nullptr;
+
     // timing
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> kev;
@@ -256,4 +266,7 @@ int frame_add(edm_membrane* m, const int64_t* flat, int64_t n);
 int frame_noise(edm_membrane* m, int64_t* n_dark, int64_t* n_ap);
 int frame_reset(edm_membrane* m);
 int stage_draws(edm_membrane* m, const double* host, int64_t n);
+// census translation unit
+int rec_step(edm_membrane* m);  // one census row into the ring (no sync)
+void rec_free(edm_membrane* m);
 }  // namespace edm

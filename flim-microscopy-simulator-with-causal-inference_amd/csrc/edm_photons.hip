@@ -656,6 +656,11 @@ __global__ void k_frame_noise(int64_t npix, int nb, uint32_t* flim, double lam, 
     noise_count(nd, na, out);
 }
 
+// cumulative photon counter of the hook path (the fused kernel counts its own)
+__global__ void k_ctr_add(unsigned long long* c, unsigned long long v) {
+    if (threadIdx.x == 0) *c += v;
+}
+
 __global__ void k_fill_u32(int64_t n, uint32_t* p, uint32_t v) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
@@ -809,6 +814,9 @@ int ph_hook(edm_membrane* m, const double* u_pois, const double* u_trans, const 
                        (const int64_t*)m->ph_off, (const double*)(m->d_draws + 2 * (int64_t)n),
                        n_arr, (int64_t)m->pp.max_photons, m->ph_pid, m->ph_t, m->ph_ch,
                        m->ph_fret);
+    if (np > 0)
+        hipLaunchKernelGGL(k_ctr_add, dim3(1), dim3(64), 0, st, m->d_ctr + 6,
+                           (unsigned long long)np);
     EDM_HIP(hipGetLastError());
     EDM_HIP(hipStreamSynchronize(st));
     m->n_ph = np;

@@ -100,6 +100,24 @@ class Replica(ctypes.Structure):
                 ("img_offx", c_double), ("img_offy", c_double)]
 
 
+class Census(ctypes.Structure):
+    """edm_census: one reduction over a membrane (pipeline.py:567-642, 357-455)."""
+    _fields_ = [("step", c_int64), ("n_bond_alive", c_int64), ("n_bond", c_int64),
+                ("n_bleached", c_int64), ("n_donor0", c_int64), ("n_free_dye", c_int64 * 8),
+                ("species_alive", c_int64 * 32), ("events", c_int64 * 6), ("photons", c_int64),
+                ("e_sum", c_double)]
+
+
+# numpy view of an array of edm_census rows
+CENSUS_DTYPE = np.dtype([("step", "<i8"), ("n_bond_alive", "<i8"), ("n_bond", "<i8"),
+                         ("n_bleached", "<i8"), ("n_donor0", "<i8"), ("n_free_dye", "<i8", (8,)),
+                         ("species_alive", "<i8", (32,)), ("events", "<i8", (6,)),
+                         ("photons", "<i8"), ("e_sum", "<f8")])
+
+# recording flags (edm_record_open)
+REC_FRET_SUM = 1
+REC_AUTO = 2
+
 # tuning keys (edm_set_tuning)
 TUNE_TILE_ROWS = 1
 TUNE_CAT_SIDE = 2
@@ -139,6 +157,14 @@ _SIGS = {
     "edm_synchronize": [c_void_p],
     "edm_time_steps": [c_void_p, c_int32, c_int32, ctypes.POINTER(c_double),
                        ctypes.POINTER(c_double), c_void_p],
+    "edm_run_frame": [c_void_p, c_int32],
+    "edm_record_open": [c_void_p, c_int32, c_int32],
+    "edm_record_step": [c_void_p],
+    "edm_record_rows": [c_void_p, ctypes.POINTER(c_int32)],
+    "edm_record_read": [c_void_p, c_void_p, c_int32, ctypes.POINTER(c_int32)],
+    "edm_record_close": [c_void_p],
+    "edm_ground_truth": [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
+                         ctypes.POINTER(Census)],
     "edm_ens_create": [ctypes.POINTER(Params), ctypes.POINTER(PhotoParams),
                        ctypes.POINTER(ImageParams), c_int64, c_int32, ctypes.POINTER(c_void_p)],
     "edm_ens_destroy": [c_void_p],

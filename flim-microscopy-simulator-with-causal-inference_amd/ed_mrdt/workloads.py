@@ -7,6 +7,7 @@
 * ``served_job()``       one job of the served stream (box side logU[5,50] um)
 * ``binding_fret()``     small test system with PHAkt + PIP3 <-> PHAkt_PIP3 binding,
                          a circular slow-diffusion region and bond-mode FRET
+                         (or proximity-mode FRET against PIP2)
 
 Everything is derived from ``examples/pip3_pten_plain.yaml`` by rescaling the
 box, the particle counts (species proportions kept), the cytosolic pools and
@@ -149,9 +150,11 @@ def served_job(j: int, rng_seed: int = 777, max_density: float = 100.0):
 
 def binding_fret(seed: int = 3, intramolecular: bool = True, side: float = 4.0,
                  reflective: bool = False, regions: bool = True,
-                 harmonic: bool = False) -> SimulationConfig:
+                 harmonic: bool = False, proximity: bool = False) -> SimulationConfig:
     """Small test system exercising binding/dissociation, bond FRET, the
-    smooth circular D(x) regions and (optionally) reflective walls."""
+    smooth circular D(x) regions and (optionally) reflective walls.  With
+    ``proximity`` the FRET pair is a proximity biosensor: a PHAkt donor is
+    quenched (E = 0.5) while a PIP2 lies within 50 nm, else E = 0.02."""
     raw = rescale(plain_raw(), side, int(round(35 * side * side)))
     raw["species"] += [
         {"name": "PHAkt", "diffusion_coefficient": 0.08, "radius": 3.0},
@@ -173,6 +176,10 @@ def binding_fret(seed: int = 3, intramolecular: bool = True, side: float = 4.0,
         raw["dye_attachments"] = [{"species": "PHAkt", "fluorophore": "mTFP1"},
                                   {"species": "PIP3", "fluorophore": "Venus",
                                    "labeling_efficiency": 0.7}]
+    if proximity:
+        raw["fret_pairs"][0].update(fret_mode="proximity", proximity_target_species="PIP2",
+                                    proximity_radius=50.0, proximity_E_high=0.5,
+                                    proximity_E_low=0.02)
     if regions:
         raw["diffusion_field"] = {"type": "regions", "regions": [
             {"shape": "circle", "center": [side / 2, side / 2], "radius": side / 5,

@@ -171,7 +171,8 @@ int edm_run(edm_membrane *m, int32_t n_steps);
 /* Event counters: bindings, dissociations, catalytic, recruited, detached,
  * converted (dynamics.py:2068-2071, 2147-2148), cumulative. */
 int edm_counters(edm_membrane *m, int64_t out[6]);
-/* Cumulative photons emitted and detected by edm_photons_image. */
+/* Cumulative photons emitted (edm_photons batch lengths and
+ * edm_photons_image) and detected (edm_photons_image). */
 int edm_photon_counters(edm_membrane *m, int64_t out[2]);
 
 /* ---- Tuning (no reference counterpart) ----------------------------------
@@ -238,11 +239,68 @@ int edm_frame_reset(edm_membrane *m);
 
 /* ---- timing (HIP events on the handle's stream) ------------------------ */
 int edm_synchronize(edm_membrane *m);
-/* Runs n_steps plain steps (+ fused imaging every step when image != 0),
- * returning total elapsed ms and the summed duration of the force/BD
- * kernel launches measured with events around each launch. */
+/* Runs n_steps plain steps (+ fused imaging every step when image != 0,
+ * + a census row when recording with EDM_REC_AUTO), returning total
+ * elapsed ms and the summed duration of the force/BD kernel launches
+ * measured with events around each launch. */
 int edm_time_steps(edm_membrane *m, int32_t n_steps, int32_t image, double *total_ms,
                    double *force_ms, double *breakdown_ms /* [8] or NULL */);
+/* Plain run of a stretch of a frame: n_steps x (dynamics step, then the
+ * fused photon + imaging step when photophysics and imaging are set, then
+ * a census row when recording with EDM_REC_AUTO), no host synchronisation
+ * (the BD step loop of SimulationPipeline.run, pipeline.py:556-564). */
+int edm_run_frame(edm_membrane *m, int32_t n_steps);
+
+/* ---- Caller-side recording (SimulationPipeline, pipeline.py) -----------
+ * A census is one device reduction over the membrane giving every count
+ * the reference's pipeline takes with numpy after a step
+ * (pipeline.py:567-642) or at a frame end (_extract_ground_truth,
+ * pipeline.py:357-455).  The counted fields are consecutive int64s from
+ * n_bond_alive to species_alive[31].                                     */
+typedef struct edm_census {
+    int64_t step;              /* dynamics steps completed                    */
+    int64_t n_bond_alive;      /* alive particles with a bond partner (:570)  */
+    int64_t n_bond;            /* particles with a bond partner (:418)        */
+    int64_t n_bleached;        /* particles with a bleached dye (:582, :419)  */
+    int64_t n_donor0;          /* dye of type 0, alive, not bleached (:594)   */
+    int64_t n_free_dye[8];     /* by dye type: not bleached, no bond partner
+                                  (:424-433)                                  */
+    int64_t species_alive[32]; /* alive particles by species (:622-625, :439) */
+    int64_t events[6];         /* cumulative, as edm_counters                 */
+    int64_t photons;           /* cumulative photons emitted (PhotonBatch
+                                  lengths of edm_photons / edm_photons_image) */
+    double e_sum;              /* EDM_REC_FRET_SUM: sum over the n_donor0
+                                  donors of k tau0 / (1 + k tau0), k = the
+                                  step's FRET rate, tau0 = tau[0] of the
+                                  photophysics tables (:600-605)             */
+} edm_census;
+
+#define EDM_REC_FRET_SUM 1 /* rows carry e_sum (proximity-mode FRET)        */
+#define EDM_REC_AUTO 2     /* edm_run, edm_run_frame and edm_time_steps
+                              append a row after every step                 */
+
+/* Recording: a device ring of `capacity` census rows.  edm_record_step
+ * appends one row (no host synchronisation; EDM_ECAPACITY when full);
+ * edm_record_read copies the rows out (cap >= edm_record_rows) and empties
+ * the ring.  Opening again resizes and empties it. */
+int edm_record_open(edm_membrane *m, int32_t capacity, int32_t flags);
+int edm_record_step(edm_membrane *m);
+int edm_record_rows(edm_membrane *m, int32_t *n_rows);
+int edm_record_read(edm_membrane *m, edm_census *rows, int32_t cap, int32_t *n_rows);
+int edm_record_close(edm_membrane *m);
+
+/* Frame-end ground truth (_extract_ground_truth, pipeline.py:357-455):
+ * FRET rates updated (update_fret_rates, as pipeline.py:649 does first),
+ * then for every dye carrier that is not bleached and whose dye type is
+ * donor[j] of a map j (later maps win): E = k tau_D[j] / (1 + k tau_D[j])
+ * and tau_DA = tau_D[j] / (1 + k tau_D[j]) for k > 0, else E = 0 and
+ * tau_DA = tau_D[j]; 0 for every other particle.  fret_E / tau_DA are [n]
+ * in particle-index order; any output may be NULL.  The caller passes the
+ * maps the reference does not skip (donor found, finite tau_D > 0), at
+ * most 8. */
+int edm_ground_truth(edm_membrane *m, int32_t n_maps, const int32_t *donor,
+                     const double *tau_D, double *fret_E, double *tau_DA,
+                     edm_census *census);
 
 /* ---- Batched ensemble ---------------------------------------------------
  * Many independent membranes of one model advanced by the same launches

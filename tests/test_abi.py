@@ -63,6 +63,8 @@ _STRUCTS = {
     "edm_host_state": ("HostState", []),
     "edm_photo_params": ("PhotoParams", []),
     "edm_image_params": ("ImageParams", []),
+    "edm_census": ("Census", ["n_bond_alive", "n_free_dye", "species_alive", "events",
+                              "photons", "e_sum"]),
 }
 
 
@@ -86,6 +88,15 @@ def test_ctypes_layout_matches_c(hip, tmp_path):
         assert ctypes.sizeof(S) == int(c[cname]), cname
         for f in fields:
             assert getattr(S, f).offset == int(c[f"{cname}.{f}"]), f"{cname}.{f}"
+
+
+def test_census_numpy_view_matches_the_struct(hip):
+    """The recorder reads rows through a numpy dtype: same size and offsets
+    as the ctypes (and so the C) layout."""
+    dt = hip.CENSUS_DTYPE
+    assert dt.itemsize == ctypes.sizeof(hip.Census)
+    for name, _ in hip.Census._fields_:
+        assert dt.fields[name][1] == getattr(hip.Census, name).offset, name
 
 
 def test_no_cpu_fallback_without_a_gpu(hip):
