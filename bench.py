@@ -16,6 +16,10 @@ particles per step (particle-steps/s).
 Prints one JSON line.  The CPU baseline is the oracle (a C restatement of
 the reference hot path, bd_step threaded like the reference's prange) timed
 on a bounded sample of the same workload on this machine's host cores.
+
+``recording``: the same steps with the per-step census of the pipeline's
+sub-frame recording appended on the device, against the reference's numpy
+recording of the 1M state on the host.
 """
 
 from __future__ import annotations
@@ -70,7 +74,7 @@ def traffic_from_profile():
     return c["fetch_kib"] * 1024.0 * c["fetch_scale"] + c["write_kib"] * 1024.0 * c["write_scale"]
 
 
-def gpu_run(steps: int, warmup: int, n_alive: int):
+def gpu_run(steps: int, warmup: int, n_alive: int, rec_steps: int = 0):
     import numpy as np
     from ed_mrdt import _hip, workloads
     from ed_mrdt.dynamics import DynamicsEngine
@@ -124,8 +128,32 @@ def gpu_run(steps: int, warmup: int, n_alive: int):
     ph1 = np.zeros(2, np.int64)
     _hip.check(L.edm_photon_counters(h, _hip.ptr(ph1)))
     alive = 0.5 * (alive0 + alive1)
-    return dict(cfg=cfg, N=st.N, alive=alive, wall=wall, event_ms=ms, force_ms=force_ms,
-                steps=steps, emitted=int(ph1[0] - ph0[0]), detected=int(ph1[1] - ph0[1]))
+    out = dict(cfg=cfg, N=st.N, alive=alive, wall=wall, event_ms=ms, force_ms=force_ms,
+               steps=steps, emitted=int(ph1[0] - ph0[0]), detected=int(ph1[1] - ph0[1]))
+    if rec_steps:
+        out["recording"] = recording_leg(L, h, run, rec_steps)
+    return out
+
+
+def recording_leg(L, h, run, k: int):
+    """The same steps with the per-step census appended on the device
+    (SimulationPipeline's sub-frame recording, EDM_REC_AUTO), against the
+    same number of steps without it, both timed with HIP events."""
+    from ed_mrdt import _hip
+    import numpy as np
+
+    ms_plain, _ = run(k)
+    _hip.check(L.edm_record_open(h, k, _hip.REC_AUTO))
+    ms_rec, _ = run(k)
+    rows = np.zeros(k, dtype=_hip.CENSUS_DTYPE)
+    n = ctypes.c_int32(0)
+    _hip.check(L.edm_record_read(h, _hip.ptr(rows), k, ctypes.byref(n)))
+    _hip.check(L.edm_record_close(h))
+    if n.value != k:
+        raise RuntimeError(f"recording leg: {n.value} census rows for {k} steps")
+    return {"steps": k, "ms_per_step_plain": ms_plain / k, "ms_per_step_recording": ms_rec / k,
+            "overhead_frac": ms_rec / ms_plain - 1.0,
+            "last_row_alive": int(rows["species_alive"][-1].sum())}
 
 
 def _progress(msg: str) -> None:
@@ -248,7 +276,18 @@ def cpu_baseline(n_alive: int, budget_s: float = 20.0):
         el = time.perf_counter() - t0
         if el >= budget_s or steps >= 200:
             break
-    return dict(value=ps / el, steps=steps, seconds=el, threads=threads)
+    # the reference's per-step recording (numpy over the 1M arrays,
+    # pipeline.py:567-642) on the same state
+    import recording as orec
+    sub = orec.OracleSubframe(cfg, dyn, ph)
+    t1 = time.perf_counter()
+    n_rec = 0
+    while n_rec < 20 and time.perf_counter() - t1 < 5.0:
+        sub.record(st, 0)
+        n_rec += 1
+    rec_ms = (time.perf_counter() - t1) * 1e3 / n_rec
+    return dict(value=ps / el, steps=steps, seconds=el, threads=threads, record_ms=rec_ms,
+                record_n=n_rec)
 
 
 def main():
@@ -267,6 +306,8 @@ def main():
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--ensemble-only", action="store_true",
                     help="only the ensemble (and served) legs, for profiling them")
+    ap.add_argument("--record-steps", type=int, default=200,
+                    help="steps of the recording leg (census every step); 0: none")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="copy this many MiB device-to-device first (counter calibration)")
     a = ap.parse_args()
@@ -283,7 +324,7 @@ def main():
             out["served"] = served_bench(a.served_jobs, 50.0, 96.0)
         print(json.dumps(out))
         return
-    r = gpu_run(a.steps, a.warmup, a.n)
+    r = gpu_run(a.steps, a.warmup, a.n, a.record_steps)
     _progress(f"1M membrane: {r['steps']} steps in {r['wall']:.3f} s")
     traffic = traffic_from_profile()
     value = r["alive"] * r["steps"] / r["wall"]
@@ -314,6 +355,8 @@ def main():
         "photons_per_s": r["emitted"] / max(r["wall"], 1e-9),
         "detected_per_s": r["detected"] / max(r["wall"], 1e-9),
     }
+    if "recording" in r:
+        out["recording"] = r["recording"]
     if not a.no_ensemble:
         out["ensemble"] = ensemble_bench(a.ensemble_replicas, a.ensemble_steps, a.plain_every)
         if not a.no_served:
@@ -325,6 +368,11 @@ def main():
                                "sample": f"{c['steps']} oracle steps (dynamics + photophysics, "
                                          f"no pair cap) of the same 1M workload in {c['seconds']:.1f} s"}
         out["speedup_vs_cpu"] = value / c["value"]
+        if "recording" in out:
+            out["recording"]["cpu_numpy_ms_per_step"] = c["record_ms"]
+            out["recording"]["cpu_numpy_sample"] = (
+                f"{c['record_n']} per-step recordings (numpy reductions of the reference's "
+                f"pipeline) of the 1M oracle state")
     print(json.dumps(out))
 
 

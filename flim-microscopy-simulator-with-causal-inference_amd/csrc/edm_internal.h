@@ -225,7 +225,6 @@ struct edm_membrane {
     int rec_cap = 0, rec_n = 0, rec_flags = 0;
     unsigned long long* cen_part = nullptr;
     double* cen_part_e = nullptr;
-    unsigned int* cen_ticket = nullptr;
     edm_census* cen_row = nullptr;
     double *gt_e = nullptr, *gt_tau = nullptr;
 

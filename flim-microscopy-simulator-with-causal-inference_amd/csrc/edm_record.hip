@@ -7,13 +7,14 @@
 // that a stretch of steps records with no host synchronisation.  The host
 // turns rows into the reference's SubFrameData / GroundTruth.
 //
-// Layout of the reduction (k_census): 1024-thread blocks, each lane reads
+// Layout of the reduction: k_census runs 1024-thread blocks; each lane reads
 // 4 consecutive slots with one 32-bit load of each byte array (flags,
 // species, dye type) and one 128-bit load of the bond partners.  Fixed
 // counts are per-lane registers reduced by wave shuffles; species counts are
-// wave ballots.  Each block writes its partial row; the last block to finish
-// (device-scope ticket) sums the partials in block order and writes the
-// census row, so the double FRET sum is deterministic run to run.
+// wave ballots.  Each block writes its partial row; k_census_final (one
+// block, next in stream order, so no device-scope fences are needed) sums
+// the partials in block order and writes the census row, so the double FRET
+// sum is deterministic run to run.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -49,10 +50,16 @@ struct CensusArgs {
     double tau0;
     unsigned long long* part;  // [gridDim.x][kCenCols]
     double* part_e;            // [gridDim.x]
-    unsigned int* ticket;
+};
+
+struct CensusFinal {
+    int nblk;  // blocks of the k_census launch
+    const unsigned long long* part;
+    const double* part_e;
     const unsigned long long* ctr;
     edm_census* row;
     int64_t step;
+    bool fret_sum;
 };
 
 __device__ __forceinline__ unsigned wave_sum_u32(unsigned v) {
@@ -67,70 +74,89 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
     return v;
 }
 
+// Four consecutive slots from quad g (slots past n read as flags 0 and no
+// partner: counted nowhere).
+struct Quad {
+    uint32_t f4, s4, d4;
+    int b[4];
+};
+
+__device__ __forceinline__ Quad load_quad(const CensusArgs& A, int64_t g) {
+    Quad Q{0u, 0u, 0u, {-1, -1, -1, -1}};
+    const int64_t i0 = g * 4;
+    if (i0 + 3 < A.n) {
+        Q.f4 = *(const uint32_t*)(A.fl + i0);
+        Q.s4 = *(const uint32_t*)(A.sp + i0);
+        Q.d4 = *(const uint32_t*)(A.dye + i0);
+        const int4 bb = *(const int4*)(A.bond + i0);
+        Q.b[0] = bb.x;
+        Q.b[1] = bb.y;
+        Q.b[2] = bb.z;
+        Q.b[3] = bb.w;
+    } else if (i0 < A.n) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (i0 + k < A.n) {
+                Q.f4 |= (uint32_t)A.fl[i0 + k] << (8 * k);
+                Q.s4 |= (uint32_t)(uint8_t)A.sp[i0 + k] << (8 * k);
+                Q.d4 |= (uint32_t)(uint8_t)A.dye[i0 + k] << (8 * k);
+                Q.b[k] = A.bond[i0 + k];
+            }
+        }
+    }
+    return Q;
+}
+
+// Counts of one quad: fixed columns into the lane's registers, alive per
+// species by wave ballot into lane q's register (every lane must call it).
+__device__ __forceinline__ void count_quad(const Quad& Q, int n_species, int lane,
+                                           unsigned (&c)[kCenFixed], unsigned& my_sc) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const unsigned f = (Q.f4 >> (8 * k)) & 0xffu;
+        const int s = (int)(int8_t)((Q.s4 >> (8 * k)) & 0xffu);
+        const int d = (int)(int8_t)((Q.d4 >> (8 * k)) & 0xffu);
+        const bool alive = f & EDM_FLAG_ALIVE, dyed = f & EDM_FLAG_DYE, bl = f & EDM_FLAG_BLEACHED;
+        const bool bound = Q.b[k] >= 0;
+        c[0] += alive && bound;
+        c[1] += bound;
+        c[2] += bl;
+        c[3] += dyed && alive && !bl && d == 0;
+        const bool free_dye = dyed && !bl && !bound;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) c[4 + t] += free_dye && d == t;
+        for (int q = 0; q < n_species; ++q) {
+            const unsigned cnt = (unsigned)__popcll(__ballot(alive && s == q));
+            my_sc += lane == q ? cnt : 0u;
+        }
+    }
+}
+
 __global__ __launch_bounds__(kCenThreads) void k_census(CensusArgs A) {
     __shared__ unsigned s_w[kCenWaves][kCenCols];
     __shared__ double s_e[kCenWaves];
-    __shared__ unsigned long long s_fin[kCenRows][kCenCols];
-    __shared__ int s_last;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int k = threadIdx.x; k < kCenWaves * kCenCols; k += blockDim.x) (&s_w[0][0])[k] = 0u;
-    __syncthreads();
 
     unsigned c[kCenFixed];
 #pragma unroll
     for (int k = 0; k < kCenFixed; ++k) c[k] = 0u;
+    unsigned my_sc = 0u;  // lane q < n_species: alive particles of species q
     const int64_t quads = ((int64_t)A.n + 3) / 4;
     const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+    // two quads per lane per pass, both loaded before either is counted;
     // block-uniform loop bound: every lane reaches the ballots
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < quads; base += gstride) {
-        const int64_t g = base + threadIdx.x;
-        const int64_t i0 = g * 4;
-        uint32_t f4 = 0u, s4 = 0u, d4 = 0u;
-        int b[4] = {-1, -1, -1, -1};
-        if (i0 + 3 < A.n) {
-            f4 = *(const uint32_t*)(A.fl + i0);
-            s4 = *(const uint32_t*)(A.sp + i0);
-            d4 = *(const uint32_t*)(A.dye + i0);
-            const int4 bb = *(const int4*)(A.bond + i0);
-            b[0] = bb.x;
-            b[1] = bb.y;
-            b[2] = bb.z;
-            b[3] = bb.w;
-        } else if (i0 < A.n) {
-            for (int k = 0; k < 4 && i0 + k < A.n; ++k) {
-                f4 |= (uint32_t)A.fl[i0 + k] << (8 * k);
-                s4 |= (uint32_t)(uint8_t)A.sp[i0 + k] << (8 * k);
-                d4 |= (uint32_t)(uint8_t)A.dye[i0 + k] << (8 * k);
-                b[k] = A.bond[i0 + k];
-            }
-        }
-        // slots past n read as flags 0 and no partner: counted nowhere
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const unsigned f = (f4 >> (8 * k)) & 0xffu;
-            const int s = (int)(int8_t)((s4 >> (8 * k)) & 0xffu);
-            const int d = (int)(int8_t)((d4 >> (8 * k)) & 0xffu);
-            const bool alive = f & EDM_FLAG_ALIVE, dyed = f & EDM_FLAG_DYE,
-                       bl = f & EDM_FLAG_BLEACHED;
-            const bool bound = b[k] >= 0;
-            c[0] += alive && bound;
-            c[1] += bound;
-            c[2] += bl;
-            c[3] += dyed && alive && !bl && d == 0;
-            const bool free_dye = dyed && !bl && !bound;
-#pragma unroll
-            for (int t = 0; t < 8; ++t) c[4 + t] += free_dye && d == t;
-            for (int q = 0; q < A.n_species; ++q) {
-                const unsigned cnt = (unsigned)__popcll(__ballot(alive && s == q));
-                if (lane == 0) s_w[w][kCenFixed + q] += cnt;
-            }
-        }
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < quads; base += 2 * gstride) {
+        const Quad Q0 = load_quad(A, base + threadIdx.x);
+        const Quad Q1 = load_quad(A, base + gstride + threadIdx.x);
+        count_quad(Q0, A.n_species, lane, c, my_sc);
+        count_quad(Q1, A.n_species, lane, c, my_sc);
     }
 #pragma unroll
     for (int k = 0; k < kCenFixed; ++k) {
         const unsigned v = wave_sum_u32(c[k]);
         if (lane == 0) s_w[w][k] = v;
     }
+    if (lane < 32) s_w[w][kCenFixed + lane] = my_sc;
 
     // proximity-mode FRET efficiency of the type-0 donors (pipeline.py:594-605)
     double e = 0.0;
@@ -159,42 +185,38 @@ __global__ __launch_bounds__(kCenThreads) void k_census(CensusArgs A) {
         for (int ww = 0; ww < kCenWaves; ++ww) v += s_e[ww];
         A.part_e[blockIdx.x] = v;
     }
-    // publish the partials; the last block sums them
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) s_last = atomicAdd(A.ticket, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
+}
 
-    const int nblk = gridDim.x;
+// Sum of the block partials in block order, the event and photon counters,
+// into one census row.
+__global__ __launch_bounds__(kCenThreads) void k_census_final(CensusFinal F) {
+    __shared__ unsigned long long s_fin[kCenRows][kCenCols];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (threadIdx.x < kCenRows * kCenCols) {
         const int col = threadIdx.x % kCenCols, r = threadIdx.x / kCenCols;
         unsigned long long v = 0ull;
-        for (int bk = r; bk < nblk; bk += kCenRows) v += A.part[(size_t)bk * kCenCols + col];
+#pragma unroll 8
+        for (int bk = r; bk < F.nblk; bk += kCenRows) v += F.part[(size_t)bk * kCenCols + col];
         s_fin[r][col] = v;
     }
     double ev = 0.0;
     if (w == 0) {
-        for (int bk = lane; bk < nblk; bk += 64) ev += A.part_e[bk];
+        for (int bk = lane; bk < F.nblk; bk += 64) ev += F.part_e[bk];
         ev = wave_sum_f64(ev);
     }
     __syncthreads();
-    int64_t* counts = &A.row->n_bond_alive;
+    int64_t* counts = &F.row->n_bond_alive;
     if (threadIdx.x < kCenCols) {
         unsigned long long v = 0ull;
         for (int r = 0; r < kCenRows; ++r) v += s_fin[r][threadIdx.x];
         counts[threadIdx.x] = (int64_t)v;
     }
     if (threadIdx.x == 64) {
-        A.row->step = A.step;
-        for (int k = 0; k < 6; ++k) A.row->events[k] = (int64_t)A.ctr[k];
-        A.row->photons = (int64_t)A.ctr[6];
+        F.row->step = F.step;
+        for (int k = 0; k < 6; ++k) F.row->events[k] = (int64_t)F.ctr[k];
+        F.row->photons = (int64_t)F.ctr[6];
     }
-    if (threadIdx.x == 0) {
-        A.row->e_sum = A.k_fret ? ev : 0.0;
-        *A.ticket = 0u;  // ready for the next census (stream order)
-    }
+    if (threadIdx.x == 0) F.row->e_sum = F.fret_sum ? ev : 0.0;
 }
 
 // Frame-end FRET ground truth of the dye carriers (pipeline.py:384-411).
@@ -239,9 +261,7 @@ static int cen_scratch(edm_membrane* m) {
     if (m->cen_part) return 0;
     EDM_HIP(hipMalloc((void**)&m->cen_part, sizeof(unsigned long long) * kCenMaxBlocks * kCenCols));
     EDM_HIP(hipMalloc((void**)&m->cen_part_e, sizeof(double) * kCenMaxBlocks));
-    EDM_HIP(hipMalloc((void**)&m->cen_ticket, 256));
     EDM_HIP(hipMalloc((void**)&m->cen_row, sizeof(edm_census)));
-    EDM_HIP(hipMemsetAsync(m->cen_ticket, 0, 256, m->stream));
     return 0;
 }
 
@@ -270,15 +290,14 @@ static int census(edm_membrane* m, edm_census* row, bool fret_sum) {
     }
     A.part = m->cen_part;
     A.part_e = m->cen_part_e;
-    A.ticket = m->cen_ticket;
-    A.ctr = m->d_ctr;
-    A.row = row;
-    A.step = (int64_t)m->step;
     const int64_t quads = ((int64_t)m->n + 3) / 4;
-    const int64_t want = std::max<int64_t>((quads + kCenThreads - 1) / kCenThreads,
+    const int64_t want = std::max<int64_t>((quads + 2 * kCenThreads - 1) / (2 * kCenThreads),
                                            ((int64_t)m->n_dye + kCenThreads - 1) / kCenThreads);
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(kCenMaxBlocks, want));
     hipLaunchKernelGGL(k_census, dim3(grid), dim3(kCenThreads), 0, m->stream, A);
+    CensusFinal F{grid, m->cen_part, m->cen_part_e, m->d_ctr, row, (int64_t)m->step,
+                  A.k_fret != nullptr};
+    hipLaunchKernelGGL(k_census_final, dim3(1), dim3(kCenThreads), 0, m->stream, F);
     EDM_HIP(hipGetLastError());
     return 0;
 }
@@ -298,14 +317,12 @@ int rec_step(edm_membrane* m) {
 }
 
 void rec_free(edm_membrane* m) {
-    void* bufs[] = {m->rec, m->cen_part, m->cen_part_e, m->cen_ticket, m->cen_row, m->gt_e,
-                    m->gt_tau};
+    void* bufs[] = {m->rec, m->cen_part, m->cen_part_e, m->cen_row, m->gt_e, m->gt_tau};
     for (void* b : bufs)
         if (b) hipFree(b);
     m->rec = nullptr;
     m->cen_part = nullptr;
     m->cen_part_e = nullptr;
-    m->cen_ticket = nullptr;
     m->cen_row = nullptr;
     m->gt_e = m->gt_tau = nullptr;
     m->rec_cap = m->rec_n = m->rec_flags = 0;
