@@ -81,21 +81,32 @@ int ens_frame_reset(edm_ensemble* e, int slot) {
 
 // Frame end of several replicas at once (blockIdx.y = position in the slot
 // list): detector noise into each listed stack, counted per replica.
-__global__ void k_ens_frame_noise(const EnsSeg* __restrict__ segs, const int32_t* __restrict__ slots,
-                                  unsigned long long* noise_ctr, uint32_t gstep) {
+__global__ __launch_bounds__(256) void k_ens_frame_noise(const EnsSeg* __restrict__ segs,
+                                                         const int32_t* __restrict__ slots,
+                                                         unsigned long long* noise_ctr,
+                                                         uint32_t gstep) {
     const int r = slots[blockIdx.y];
     const EnsSeg& g = segs[r];
+    // the few replica fields the frame end needs, once
+    uint32_t* const flim = g.flim;
+    const int nb = g.ip.n_bins;
     const int64_t npix = (int64_t)g.ip.n_ch * g.ip.nx * g.ip.ny;
+    const double lam = g.ip.dark_lambda, p_ap = g.ip.p_ap;
     const Rng rng{g.seed_lo ^ 0x2545F491u, g.seed_hi};
     const uint32_t frame = gstep - g.step0;
+    if (!flim) return;  // block-uniform
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t end = (npix + 63) / 64 * 64;  // whole wavefronts take part in the counts
+#pragma unroll 1
     for (int64_t q0 = (int64_t)blockIdx.x * blockDim.x; q0 < end; q0 += stride) {
+        // this wavefront's 64 pixels: totals read coalesced, then one pixel per lane
+        const int64_t qw = q0 + (threadIdx.x & ~63u);
         const int64_t q = q0 + threadIdx.x;
         unsigned long long nd = 0, na = 0;
-        if (q < npix && g.flim)
-            frame_noise_pixel(q, g.ip.n_bins, g.flim, g.ip.dark_lambda, g.ip.p_ap, rng, frame, nd,
-                              na);
+        if (qw < npix) {
+            const uint32_t tot0 = p_ap > 0.0 ? wave_pixel_totals(flim, qw, npix, nb) : 0u;
+            if (q < npix) frame_noise_pixel(q, nb, flim, tot0, lam, p_ap, rng, frame, nd, na);
+        }
         noise_count(nd, na, noise_ctr + 2 * r);
     }
 }

@@ -550,13 +550,41 @@ __global__ void k_bin_tcspc(int64_t n, const edm_image_params* ip, uint32_t* fli
 // ----------------------------------------------------------------------
 // Frame end (microscope.py:859-920)
 // ----------------------------------------------------------------------
+// Totals of the pixel histograms q0 .. q0 + 63 (nb bins each), every row
+// read coalesced by the whole wavefront (16 B per lane when nb % 4 == 0);
+// lane j returns the total of pixel q0 + j (0 past npix).  Wave-uniform
+// arguments; every lane must call it.
+__device__ __forceinline__ uint32_t wave_pixel_totals(const uint32_t* flim, int64_t q0,
+                                                      int64_t npix, int nb) {
+    const int lane = lane_id();
+    uint32_t mine = 0u;
+    const int jn = (int)min<int64_t>(64, npix - q0);
+#pragma unroll 4
+    for (int j = 0; j < jn; ++j) {
+        const uint32_t* row = flim + (q0 + j) * nb;
+        uint32_t s = 0u;
+        if ((nb & 3) == 0) {
+            const uint4* r4 = (const uint4*)row;
+            for (int b = lane; b < (nb >> 2); b += 64) {
+                const uint4 v = r4[b];
+                s += v.x + v.y + v.z + v.w;
+            }
+        } else {
+            for (int b = lane; b < nb; b += 64) s += row[b];
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        mine = lane == j ? s : mine;
+    }
+    return mine;
+}
+
 __global__ void k_pixel_totals(int64_t npix, int nb, const uint32_t* flim, uint32_t* tot) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= npix) return;
-    uint32_t s = 0;
-    const uint32_t* row = flim + q * nb;
-    for (int b = 0; b < nb; ++b) s += row[b];
-    tot[q] = s;
+    const int64_t q0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) & ~(int64_t)63;
+    if (q0 >= npix) return;  // whole wavefronts
+    const uint32_t s = wave_pixel_totals(flim, q0, npix, nb);
+    const int64_t q = q0 + lane_id();
+    if (q < npix) tot[q] = s;
 }
 
 __global__ void k_add_counts(int64_t n, const int64_t* idx, uint32_t* flim, int64_t nelem) {
@@ -602,13 +630,15 @@ __device__ __forceinline__ int binomial_draw(int nn, double p, double u, double 
 }
 
 // Dark counts (Poisson, uniform bins) and afterpulses (binomial of the
-// pixel's counts, uniform bins) of one pixel histogram.
-__device__ __forceinline__ void frame_noise_pixel(int64_t q, int nb, uint32_t* flim, double lam,
-                                                  double p_ap, Rng rng, uint32_t frame,
-                                                  unsigned long long& nd,
+// pixel's counts after the dark counts, uniform bins) of one pixel
+// histogram whose counts before the dark counts total `tot0`.
+__device__ __forceinline__ void frame_noise_pixel(int64_t q, int nb, uint32_t* flim, uint32_t tot0,
+                                                  double lam, double p_ap, Rng rng,
+                                                  uint32_t frame, unsigned long long& nd,
                                                   unsigned long long& na) {
     uint32_t* row = flim + q * nb;
     const uint32_t lo = (uint32_t)q, hi = (uint32_t)(q >> 32);
+    uint32_t tot = tot0;
     if (lam >= 1e-10) {
         const U4 r = rng.draw(frame, lo, hi, kTagDark);
         const int k = poisson_inv(lam, u01d(r.x, r.y));
@@ -617,10 +647,9 @@ __device__ __forceinline__ void frame_noise_pixel(int64_t q, int nb, uint32_t* f
             row[(int)(u01d(rb.x, rb.y) * nb) % nb] += 1u;
         }
         nd += (unsigned long long)k;
+        tot += (uint32_t)k;
     }
     if (p_ap > 0.0) {
-        uint32_t tot = 0;
-        for (int b = 0; b < nb; ++b) tot += row[b];
         if (tot) {
             const U4 r = rng.draw(frame, lo, hi, kTagAfterpulse);
             double z0, z1;
@@ -648,11 +677,16 @@ __device__ __forceinline__ void noise_count(unsigned long long nd, unsigned long
     }
 }
 
+// One wavefront per 64 pixels: their totals read coalesced, then each
+// lane draws its own pixel's noise.
 __global__ void k_frame_noise(int64_t npix, int nb, uint32_t* flim, double lam, double p_ap,
                               Rng rng, uint32_t frame, unsigned long long* out) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t q0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) & ~(int64_t)63;
+    if (q0 >= npix) return;  // whole wavefronts
+    const uint32_t tot0 = p_ap > 0.0 ? wave_pixel_totals(flim, q0, npix, nb) : 0u;
+    const int64_t q = q0 + lane_id();
     unsigned long long nd = 0, na = 0;
-    if (q < npix) frame_noise_pixel(q, nb, flim, lam, p_ap, rng, frame, nd, na);
+    if (q < npix) frame_noise_pixel(q, nb, flim, tot0, lam, p_ap, rng, frame, nd, na);
     noise_count(nd, na, out);
 }
 

@@ -268,3 +268,33 @@ def test_plain_config_validates(c0):
     c0.validate()
     assert len(c0.species) == 5
     assert len(c0.catalytic_reactions) == 3 and len(c0.compartment_exchanges) == 3
+
+
+# ---- Philox4x32-10 restatement used by the frame-noise GPU test -----------
+
+def test_philox_known_answers():
+    """Known-answer vectors of Philox4x32-10 published with Random123
+    (kat_vectors: counter, key -> output)."""
+    from philox_ref import philox4x32
+    assert philox4x32((0, 0, 0, 0), 0, 0) == (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)
+    assert philox4x32((0xffffffff,) * 4, 0xffffffff, 0xffffffff) == (
+        0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)
+    assert philox4x32((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), 0xa4093822,
+                      0x299f31d0) == (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)
+
+
+def test_frame_noise_restatement_limits():
+    """Poisson and binomial inversion against scipy's distributions (CDF
+    at the drawn value brackets the uniform)."""
+    from scipy import stats
+    from philox_ref import binomial_inv, poisson_inv
+    for lam in (0.02, 0.7, 3.0):
+        for u in (0.0, 0.3, 0.9, 0.999):
+            k = poisson_inv(lam, u)
+            assert stats.poisson.cdf(k, lam) >= u - 1e-12
+            assert k == 0 or stats.poisson.cdf(k - 1, lam) < u + 1e-12
+    for n, p in ((10, 0.005), (500, 0.005), (3000, 0.005)):
+        for u in (0.0, 0.2, 0.95, 0.9999):
+            k = binomial_inv(n, p, u)
+            assert stats.binom.cdf(k, n, p) >= u - 1e-9
+            assert k == 0 or stats.binom.cdf(k - 1, n, p) < u + 1e-9
