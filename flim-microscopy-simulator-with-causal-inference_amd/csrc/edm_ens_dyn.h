@@ -237,19 +237,22 @@ __global__ __launch_bounds__(256) void k_ens_passes(int n_waves, const int32_t* 
     const PassGroupDev& grp = pg[(size_t)W.r * n_groups + gi];
     const int j = W.j + lane_id();
     int det = 0, conv = 0;
+    Cand<Real> cand{false, 0, (Real)0, (Real)0};
+    int t = -1;
     if (j < min(g.wl_n[0], (int)rcnt[2 * W.r])) {
-        const int t = list[g.wl_off[0] + j];
+        t = list[g.wl_off[0] + j];
         const int s0 = Q.sp[t];
         if ((grp.species >> s0) & 1u) {
             const Grids<Real> G = ens_grids<Real>(g, Q.counts, Q.delta, Q.ns);
-            // this replica's candidate region is its own slot range
-            const int pbeg = (int)Q.start[g.coff];
             passes_particle(t, s0, g.n, Q.id, g.kbase, ens_post(Q), grp, (const double*)nullptr,
                             ens_rng(g), gstep - g.step0, (Real)g.Lx, (Real)g.Ly, G, Q.n_grids,
-                            cid + pbeg, cslot + pbeg, cxo + pbeg, cyo + pbeg, nc + W.r, det,
-                            conv);
+                            cand, det, conv);
         }
     }
+    // the wave is one replica's: its candidate region is that replica's own
+    // slot range, its counter the replica's
+    const int pbeg = (int)Q.start[g.coff];
+    append_candidates(cand, t, cid + pbeg, cslot + pbeg, cxo + pbeg, cyo + pbeg, nc + W.r);
     passes_count<Real>(det, conv, ctr + 8 * W.r);
 }
 

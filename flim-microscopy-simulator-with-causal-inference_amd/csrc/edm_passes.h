@@ -29,6 +29,39 @@ struct PassGroupDev {
     PassDev p[kMaxPasses];
 };
 
+// A recruitment candidate of this thread's particle (at most one per run:
+// a recruitment pass ends its run).
+template <typename Real>
+struct Cand {
+    bool on;
+    int it;
+    Real x, y;
+};
+
+// Appends the wavefront's candidates with one atomic on the counter (the
+// order of candidates in the array does not matter: the cap is applied by
+// particle index).  Every lane of the wavefront must call it; `nc` must be
+// the same for the whole wavefront.
+template <typename Real>
+__device__ __forceinline__ void append_candidates(const Cand<Real>& c, int t, int32_t* cid,
+                                                  int32_t* cslot, double* cxo, double* cyo,
+                                                  unsigned int* nc) {
+    const unsigned long long b = __ballot(c.on);
+    if (!b) return;  // uniform
+    const int lane = lane_id();
+    const int leader = __ffsll((long long)b) - 1;
+    unsigned int base = 0u;
+    if (lane == leader) base = atomicAdd(nc, (unsigned int)__popcll(b));
+    base = (unsigned int)__builtin_amdgcn_readlane((int)base, leader);
+    if (c.on) {
+        const unsigned int k = base + (unsigned int)__popcll(b & ((1ull << lane) - 1ull));
+        cid[k] = c.it;
+        cslot[k] = t;
+        cxo[k] = (double)c.x;
+        cyo[k] = (double)c.y;
+    }
+}
+
 // Applies a run of passes to the particle in slot t (species s0, known to
 // be in the run's species set).  Draw keys use the particle index minus
 // kbase; hook draws are indexed by the particle index.
@@ -38,9 +71,8 @@ __device__ __forceinline__ void passes_particle(int t, int s0, int n, const int3
                                                 const PassGroupDev& grp,
                                                 const double* __restrict__ draws, Rng rng,
                                                 uint32_t step, Real Lx, Real Ly,
-                                                const Grids<Real>& G, int n_grids, int32_t* cid,
-                                                int32_t* cslot, double* cxo, double* cyo,
-                                                unsigned int* nc, int& det_hits, int& conv_hits) {
+                                                const Grids<Real>& G, int n_grids,
+                                                Cand<Real>& cand, int& det_hits, int& conv_hits) {
     {
         uint8_t f = P.fl[t];
         const uint8_t f0 = f;
@@ -125,11 +157,10 @@ __device__ __forceinline__ void passes_particle(int t, int s0, int n, const int3
                     pe = (Real)d.prob * fb_hill<Real>(d.f1, k1[k]);
                 }
                 if (u[k] < (double)pe) {
-                    const unsigned int c = atomicAdd(nc, 1u);
-                    cid[c] = it;
-                    cslot[c] = t;
-                    cxo[c] = (double)px[k];
-                    cyo[c] = (double)py[k];
+                    cand.on = true;
+                    cand.it = it;
+                    cand.x = px[k];
+                    cand.y = py[k];
                 }
             }
         }
@@ -171,11 +202,13 @@ __global__ __launch_bounds__(256) void k_passes(int n, WlView V, const int32_t* 
         if (q < n) t = q;
     }
     int det_hits = 0, conv_hits = 0;
+    Cand<Real> cand{false, 0, (Real)0, (Real)0};
     if (t >= 0) {
         const int s0 = P.sp[t];
         if ((grp.species >> s0) & 1u)
-            passes_particle(t, s0, n, id, 0, P, grp, draws, rng, step, Lx, Ly, G, n_grids, cid,
-                            cslot, cxo, cyo, nc, det_hits, conv_hits);
+            passes_particle(t, s0, n, id, 0, P, grp, draws, rng, step, Lx, Ly, G, n_grids, cand,
+                            det_hits, conv_hits);
     }
+    append_candidates(cand, t, cid, cslot, cxo, cyo, nc);
     passes_count<Real>(det_hits, conv_hits, ctr);
 }
