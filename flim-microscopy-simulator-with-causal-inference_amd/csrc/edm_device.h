@@ -113,6 +113,11 @@ __device__ __forceinline__ void normal2(const U4& r, Real& z0, Real& z1) {
 // --------------------------------------------------------- reference math
 template <typename Real>
 __device__ __forceinline__ Real py_mod(Real x, Real L) {
+    // a moved coordinate is within one box length of the box: there the
+    // result is x, or x - L which is exact for L <= x < 2L (Sterbenz), as
+    // fmod's; everything else (and +-0) takes the general path
+    if (x > (Real)0 && x < L) return x;
+    if (x >= L && x < (Real)2 * L) return x - L;
     Real r = fmod(x, L);
     if (r != (Real)0) {
         if ((L < (Real)0) != (r < (Real)0)) r += L;

@@ -1616,7 +1616,8 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
     else {
         const int ntile_y = (p.ncy + m->tile_rows - 1) / m->tile_rows;
         const int ntiles = p.ncx * ntile_y;
-        hipLaunchKernelGGL((k_force_tile<Real>), dim3(8 * ((ntiles + 7) / 8)), dim3(256), 0, st, C,
+        hipLaunchKernelGGL((k_force_tile<Real>), dim3(8 * ((ntiles + 7) / 8)), dim3(256),
+                           force_sub_lds(m->tile_rows), st, C,
                            F, (Real*)m->a.xn, (Real*)m->a.yn, noise, rng, step, G, ng,
                            m->sensor_mask_all, ntile_y, ntiles);
     }

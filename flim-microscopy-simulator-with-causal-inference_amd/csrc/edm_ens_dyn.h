@@ -684,7 +684,7 @@ static int ens_step_t(edm_ensemble* e) {
     F.rint2 = (Real)e->rint2;
     const dim3 tgrid(8 * ((ntiles + 7) / 8));
     const uint32_t gstep = e->gstep;
-    hipLaunchKernelGGL((k_ens_force<Real>), tgrid, dim3(256), 0, st,
+    hipLaunchKernelGGL((k_ens_force<Real>), tgrid, dim3(256), force_sub_lds(e->tile_rows), st,
                        (const EnsTile*)e->d_tiles, ntiles, (const EnsSeg*)e->d_seg, Q, F,
                        e->sensor_mask_all, gstep);
 

@@ -117,8 +117,39 @@ __device__ __forceinline__ int strip_row(const Cells<Real>& C, const StripGeo& S
     return trunc_cell(y, C.csy, C.ncy) - S.cy0 + 1;
 }
 
+// Force sub-rows.  The potentials reach far less than a cell (about 11 nm
+// against the 165 nm pair-list cells of the reaction contacts), so each
+// cell row is cut into nsub sub-rows at least 1.01 x the longest potential
+// range tall (at most kMaxSub): a pair within range is then at most one
+// sub-row apart, and a particle's force candidates in a column are the
+// particles of its own sub-row and the two next to it.
+static constexpr int kMaxSub = 8;
+
+// Dynamic LDS bytes of a force strip launch whose strips have at most
+// `rows` cell rows.
+static inline size_t force_sub_lds(int rows) {
+    return (size_t)(3 * (rows + 2) * kMaxSub + 1) * sizeof(int16_t);
+}
+
+template <typename Real>
+__device__ __forceinline__ int force_nsub(const Cells<Real>& C, Real rl2) {
+    if (!(rl2 > (Real)0)) return kMaxSub;
+    const Real q = C.csy / ((Real)1.01 * sqrt(rl2));
+    return q >= (Real)kMaxSub ? kMaxSub : max(1, (int)q);
+}
+
+// Sub-row (0..nsub-1) of a particle at y in global cell row gy.
+template <typename Real>
+__device__ __forceinline__ int sub_row(const Cells<Real>& C, Real y, int gy, int nsub,
+                                       Real inv_h) {
+    const int b = (int)((y - (Real)gy * C.csy) * inv_h);
+    return min(max(b, 0), nsub - 1);
+}
+
 // Forces + Ermak-McCammon move over one strip (any pair summation order).
-// Every thread of the 256-wide block must call it.
+// Every thread of the 256-wide block must call it.  A particle's candidates
+// are enumerated by column, then sub-row, then slot order, in both the LDS
+// and the global-memory walk, so the sums do not depend on the strip shape.
 template <typename Real>
 __device__ __forceinline__ void force_strip(const Cells<Real>& C, const ForceParams<Real>& F,
                                             Real* __restrict__ xn, Real* __restrict__ yn,
@@ -135,6 +166,12 @@ __device__ __forceinline__ void force_strip(const Cells<Real>& C, const ForcePar
     __shared__ int s_gst[kStripCells];
     __shared__ int s_wsum[4];
     __shared__ Real s_sig[kTab], s_eps[kTab], s_kh[kTab];
+    // staged positions in (cell, sub-row, slot) order, each staged
+    // particle's sub-row, and (dynamic LDS of force_sub_lds(rows) bytes) the
+    // sub-row starts in s_ord: index cell * nsub + sub-row, then the total
+    __shared__ int16_t s_ord[kCap];
+    __shared__ uint8_t s_b[kCap];
+    extern __shared__ int16_t s_sub[];
 
     // the species-pair tables in LDS (read per candidate pair; beyond
     // kTabSpecies species they stay in global memory); the barriers of
@@ -190,6 +227,45 @@ __device__ __forceinline__ void force_strip(const Cells<Real>& C, const ForcePar
     // pairs beyond the pair-list cutoff or the longest potential range
     // contribute nothing
     const Real rl2 = min(C.cut2, F.rint2);
+    const int nsub = force_nsub(C, rl2);
+    const Real inv_h = (Real)nsub / C.csy;
+    if (staged) {
+        // counting sort of every staged cell by sub-row (stable: slot order
+        // inside a sub-row), one thread per cell
+        for (int k = (int)threadIdx.x; k < S.ncells; k += (int)blockDim.x) {
+            const int col = (k >= S.nrow) + (k >= 2 * S.nrow), r = k - col * S.nrow;
+            int gy = S.cy0 + r - 1;
+            gy = gy < 0 ? gy + C.ncy : (gy >= C.ncy ? gy - C.ncy : gy);
+            const int o0 = s_off[k], o1 = s_off[k + 1];
+            int cnt[kMaxSub];
+#pragma unroll
+            for (int q = 0; q < kMaxSub; ++q) cnt[q] = 0;
+            for (int o = o0; o < o1; ++o) {
+                const int bq = sub_row(C, s_y[o], gy, nsub, inv_h);
+                s_b[o] = (uint8_t)bq;
+#pragma unroll
+                for (int q = 0; q < kMaxSub; ++q) cnt[q] += bq == q;
+            }
+            int run = o0;
+#pragma unroll
+            for (int q = 0; q < kMaxSub; ++q) {
+                if (q < nsub) s_sub[k * nsub + q] = (int16_t)run;
+                const int c = cnt[q];
+                cnt[q] = run;  // next position of sub-row q
+                run += c;
+            }
+            for (int o = o0; o < o1; ++o) {
+                const int bq = s_b[o];
+                int pos = 0;
+#pragma unroll
+                for (int q = 0; q < kMaxSub; ++q)
+                    if (bq == q) pos = cnt[q]++;
+                s_ord[pos] = (int16_t)o;
+            }
+        }
+        if (threadIdx.x == 0) s_sub[S.ncells * nsub] = (int16_t)total;
+        __syncthreads();
+    }
     for (int i0 = own_begin; i0 < own_end; i0 += blockDim.x) {
         const int i = i0 + (int)threadIdx.x;
         bool take_any = false;
@@ -215,14 +291,17 @@ __device__ __forceinline__ void force_strip(const Cells<Real>& C, const ForcePar
             if (ft & EDM_FLAG_ALIVE) {
                 Real Fx = 0, Fy = 0;
                 if (staged) {
-                    const int r = strip_row(C, S, yt);
-                    // (the particle itself is among the candidates: at d^2 = 0 no
-                    // potential acts)
+                    // own sub-row g of the strip's column (rows 1..rows): the
+                    // candidates of a column are sub-rows g-1 .. g+1, one
+                    // contiguous range of s_ord (the particle itself is among
+                    // them: at d^2 = 0 no potential acts)
+                    const int g = strip_row(C, S, yt) * nsub + s_b[i];
                     auto sweep = [&](const ForceParams<Real>& T) {
                         for (int col = 0; col < 3; ++col) {
-                            const int k0 = col * S.nrow + r - 1;
-                            const int ub = s_off[k0 + 3];
-                            for (int u = s_off[k0]; u < ub; ++u) {
+                            const int base = col * S.nrow * nsub;
+                            const int v1 = s_sub[base + g + 2];
+                            for (int v = s_sub[base + g - 1]; v < v1; ++v) {
+                                const int u = s_ord[v];
                                 Real dx, dy;
                                 const Real dsq = min_image(xt, yt, s_x[u], s_y[u], C.Lx, C.Ly,
                                                            C.px, C.py, dx, dy);
@@ -247,11 +326,19 @@ __device__ __forceinline__ void force_strip(const Cells<Real>& C, const ForcePar
                 } else {
                     int cxx, cyy;
                     cell_of(C, xt, yt, cxx, cyy);
+                    const int bo = sub_row(C, yt, cyy, nsub, inv_h);
+                    // columns, then sub-rows bo-1 .. bo+1 (possibly in the
+                    // rows above / below), then slot order: the LDS order
                     for (int k = 0; k < 9; ++k) {
-                        const int c = nb_cell(C, cxx, cyy, k);
+                        const int q = bo + (k % 3) - 1;  // sub-row, own-row relative
+                        const int dr = q < 0 ? -1 : (q >= nsub ? 1 : 0);
+                        const int qs = q - dr * nsub;
+                        const int c = nb_cell(C, cxx, cyy, 3 * (k / 3) + dr + 1);
                         if (c < 0) continue;
+                        const int gyc = c % C.ncy;
                         for (int u = (int)C.start[c]; u < (int)C.start[c + 1]; ++u) {
                             if (u == t) continue;
+                            if (sub_row(C, C.y[u], gyc, nsub, inv_h) != qs) continue;
                             Real dx, dy;
                             const Real dsq = min_image(xt, yt, C.x[u], C.y[u], C.Lx, C.Ly, C.px,
                                                        C.py, dx, dy);
