@@ -196,8 +196,13 @@ int edm_upload_state(edm_membrane* m, const edm_host_state* s) {
     m->keep_slot_of = bonds;
     m->slot_of_valid = true;  // the identity, copied above
     m->alive_at_upload.assign(m->p.n_species, 0);
+    int64_t n_alive = 0;
     for (int i = 0; i < n; ++i)
-        if (fl[i] & EDM_FLAG_ALIVE) ++m->alive_at_upload[sp[i]];
+        if (fl[i] & EDM_FLAG_ALIVE) {
+            ++m->alive_at_upload[sp[i]];
+            ++n_alive;
+        }
+    m->sub_rows = force_sub_rows(n_alive, (int64_t)m->p.ncx * m->p.ncy);
     const int nb = std::max(1, (n + kWlSlots - 1) / kWlSlots);
     if (nb != m->wl_nblk) {
         if (m->wl) EDM_HIP(hipFree(m->wl));

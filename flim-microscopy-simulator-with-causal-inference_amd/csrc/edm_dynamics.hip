@@ -43,6 +43,7 @@ struct Cells {
     int kbase;  // first particle index of this membrane (0 unless batched); draw keys
                 // use the index relative to it
     int tile_rows;  // cell rows per strip of the tiled kernels (1..kTileRows)
+    bool sub_rows;  // force strips enumerate candidates by force sub-row
 };
 
 template <typename Real>
@@ -1121,6 +1122,7 @@ static Cells<Real> make_cells(edm_membrane* m) {
     C.cap_active = m->need_pair_offsets && m->n_pairs < m->n_pairs_found;
     C.kbase = 0;
     C.tile_rows = m->tile_rows;
+    C.sub_rows = m->sub_rows;
     return C;
 }
 
@@ -1616,10 +1618,16 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
     else {
         const int ntile_y = (p.ncy + m->tile_rows - 1) / m->tile_rows;
         const int ntiles = p.ncx * ntile_y;
-        hipLaunchKernelGGL((k_force_tile<Real>), dim3(8 * ((ntiles + 7) / 8)), dim3(256),
-                           force_sub_lds(m->tile_rows), st, C,
-                           F, (Real*)m->a.xn, (Real*)m->a.yn, noise, rng, step, G, ng,
-                           m->sensor_mask_all, ntile_y, ntiles);
+        const dim3 grid(8 * ((ntiles + 7) / 8));
+        if (m->sub_rows)
+            hipLaunchKernelGGL((k_force_tile<Real, true>), grid, dim3(256),
+                               force_sub_lds(m->tile_rows), st, C, F, (Real*)m->a.xn,
+                               (Real*)m->a.yn, noise, rng, step, G, ng, m->sensor_mask_all,
+                               ntile_y, ntiles);
+        else
+            hipLaunchKernelGGL((k_force_tile<Real, false>), grid, dim3(256), 0, st, C, F,
+                               (Real*)m->a.xn, (Real*)m->a.yn, noise, rng, step, G, ng,
+                               m->sensor_mask_all, ntile_y, ntiles);
     }
     if (m->time_force) EDM_HIP(hipEventRecord(m->kev[1], st));
 

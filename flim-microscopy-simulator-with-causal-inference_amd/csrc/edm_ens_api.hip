@@ -236,6 +236,9 @@ int edm_ens_add(edm_ensemble* e, const edm_replica* r, const edm_host_state* s, 
     EDM_HIP(hipMemcpy(e->rep + beg, rep.data(), n * 2, hipMemcpyHostToDevice));
     EDM_HIP(hipMemset(e->ctr + 8 * (size_t)k, 0, 8 * sizeof(unsigned long long)));
     EDM_HIP(hipMemset(e->noise_ctr + 2 * (size_t)k, 0, 2 * sizeof(unsigned long long)));
+    int64_t n_alive = 0;
+    for (int i = 0; i < n; ++i) n_alive += (fl[i] & EDM_FLAG_ALIVE) ? 1 : 0;
+    g.sub_rows = force_sub_rows(n_alive, (int64_t)g.ncx * g.ncy) ? 1 : 0;
     e->seg[k] = g;
     e->fresh_beg[k] = beg;
     int64_t npass = 0, ndye = 0;

@@ -30,6 +30,7 @@ __device__ __forceinline__ Cells<Real> ens_cells(const EnsSeg& g, const Real* x,
     C.cap_active = false;
     C.kbase = g.kbase;
     C.tile_rows = kTileRows;  // unused: the strip table gives each strip's rows
+    C.sub_rows = g.sub_rows != 0;
     return C;
 }
 
@@ -179,7 +180,7 @@ __device__ __forceinline__ Post<Real> ens_post(const EnsPool<Real>& Q) {
     return P;
 }
 
-template <typename Real>
+template <typename Real, bool kSub>
 __global__ __launch_bounds__(256) void k_ens_force(const EnsTile* __restrict__ tiles, int ntiles,
                                                    const EnsSeg* __restrict__ segs,
                                                    EnsPool<Real> Q, ForceParams<Real> F,
@@ -193,8 +194,8 @@ __global__ __launch_bounds__(256) void k_ens_force(const EnsTile* __restrict__ t
     Fr.sin_kx = (Real)g.sin_kx;
     Fr.sin_ky = (Real)g.sin_ky;
     const Grids<Real> G = ens_grids<Real>(g, Q.counts, Q.delta, Q.ns);
-    force_strip(C, Fr, Q.xn, Q.yn, (const double*)nullptr, ens_rng(g), gstep - g.step0, G,
-                Q.n_grids, keep_mask, ens_strip<Real>(g, T));
+    force_strip<Real, kSub>(C, Fr, Q.xn, Q.yn, (const double*)nullptr, ens_rng(g),
+                            gstep - g.step0, G, Q.n_grids, keep_mask, ens_strip<Real>(g, T));
 }
 
 // Catalysis for one run of catalytic reactions; the run's parameter block
@@ -684,9 +685,16 @@ static int ens_step_t(edm_ensemble* e) {
     F.rint2 = (Real)e->rint2;
     const dim3 tgrid(8 * ((ntiles + 7) / 8));
     const uint32_t gstep = e->gstep;
-    hipLaunchKernelGGL((k_ens_force<Real>), tgrid, dim3(256), force_sub_lds(e->tile_rows), st,
-                       (const EnsTile*)e->d_tiles, ntiles, (const EnsSeg*)e->d_seg, Q, F,
-                       e->sensor_mask_all, gstep);
+    bool any_sub = false;
+    for (int r = 0; r < e->rmax; ++r) any_sub |= e->seg[r].active && e->seg[r].sub_rows;
+    if (any_sub)
+        hipLaunchKernelGGL((k_ens_force<Real, true>), tgrid, dim3(256),
+                           force_sub_lds(e->tile_rows), st, (const EnsTile*)e->d_tiles, ntiles,
+                           (const EnsSeg*)e->d_seg, Q, F, e->sensor_mask_all, gstep);
+    else
+        hipLaunchKernelGGL((k_ens_force<Real, false>), tgrid, dim3(256), 0, st,
+                           (const EnsTile*)e->d_tiles, ntiles, (const EnsSeg*)e->d_seg, Q, F,
+                           e->sensor_mask_all, gstep);
 
     for (size_t ri = 0; ri < e->rx.size(); ++ri) {
         const edm_reaction& r = e->rx[ri];

@@ -50,6 +50,7 @@ struct EnsSeg {
     // wl_n entries
     int32_t wl_off[2];
     int32_t wl_n[2];
+    int32_t sub_rows;  // force candidates by sub-row (dense replicas)
 };
 
 // The wave's replica and its first entry of list L: every wave of a list

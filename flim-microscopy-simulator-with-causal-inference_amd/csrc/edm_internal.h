@@ -70,6 +70,15 @@ struct CatPlan {
 
 namespace edm {
 
+// Force strips enumerate candidates by sub-row only in membranes with at
+// least two alive particles per cell at upload: below that the per-cell
+// sort costs more than the candidates it removes.  A property of the
+// membrane (its own state and cell grid), so a replica of an ensemble makes
+// the same choice as the same membrane alone.
+inline bool force_sub_rows(int64_t n_alive, int64_t n_cells) {
+    return n_cells > 0 && n_alive >= 2 * n_cells;
+}
+
 void set_error(const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
 
@@ -158,6 +167,7 @@ struct edm_membrane {
     int cat_side = -1;          // catalysis walk: -1 the rarer side, 0 enzymes, 1 substrates
     int cat_walk = 0;           // 1: catalysis runs walk work lists where they can; 0: strips
     int cat_lanes = 16;         // lanes per entity of the strip walk (1, 2, 4, .., 64)
+    bool sub_rows = false;      // force strips by sub-row (dense membranes; set at upload)
     // Work lists rebuilt after every sort (valid once a step has run since
     // the last upload): kWlMaxLists lists of wl_nblk regions of kWlSlots
     int32_t* wl = nullptr;       // [kWlMaxLists][wl_nblk * kWlSlots]

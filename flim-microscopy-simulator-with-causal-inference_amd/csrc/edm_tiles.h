@@ -150,7 +150,10 @@ __device__ __forceinline__ int sub_row(const Cells<Real>& C, Real y, int gy, int
 // Every thread of the 256-wide block must call it.  A particle's candidates
 // are enumerated by column, then sub-row, then slot order, in both the LDS
 // and the global-memory walk, so the sums do not depend on the strip shape.
-template <typename Real>
+// kSub = false: no membrane of the launch uses sub-rows (one per cell; no
+// sort, no dynamic LDS); kSub = true: a membrane without them sorts with
+// one sub-row per cell, which keeps its order.
+template <typename Real, bool kSub>
 __device__ __forceinline__ void force_strip(const Cells<Real>& C, const ForceParams<Real>& F,
                                             Real* __restrict__ xn, Real* __restrict__ yn,
                                             const double* __restrict__ noise_hook, Rng rng,
@@ -227,9 +230,9 @@ __device__ __forceinline__ void force_strip(const Cells<Real>& C, const ForcePar
     // pairs beyond the pair-list cutoff or the longest potential range
     // contribute nothing
     const Real rl2 = min(C.cut2, F.rint2);
-    const int nsub = force_nsub(C, rl2);
+    const int nsub = kSub && C.sub_rows ? force_nsub(C, rl2) : 1;
     const Real inv_h = (Real)nsub / C.csy;
-    if (staged) {
+    if (kSub && staged) {
         // counting sort of every staged cell by sub-row (stable: slot order
         // inside a sub-row), one thread per cell
         for (int k = (int)threadIdx.x; k < S.ncells; k += (int)blockDim.x) {
@@ -295,13 +298,23 @@ __device__ __forceinline__ void force_strip(const Cells<Real>& C, const ForcePar
                     // candidates of a column are sub-rows g-1 .. g+1, one
                     // contiguous range of s_ord (the particle itself is among
                     // them: at d^2 = 0 no potential acts)
-                    const int g = strip_row(C, S, yt) * nsub + s_b[i];
+                    // (one sub-row per cell: the column's three cells)
+                    const int r = strip_row(C, S, yt);
+                    const int g = r * nsub + (kSub ? (int)s_b[i] : 0);
                     auto sweep = [&](const ForceParams<Real>& T) {
                         for (int col = 0; col < 3; ++col) {
-                            const int base = col * S.nrow * nsub;
-                            const int v1 = s_sub[base + g + 2];
-                            for (int v = s_sub[base + g - 1]; v < v1; ++v) {
-                                const int u = s_ord[v];
+                            int v0, v1;
+                            if constexpr (kSub) {
+                                const int base = col * S.nrow * nsub;
+                                v0 = s_sub[base + g - 1];
+                                v1 = s_sub[base + g + 2];
+                            } else {
+                                const int k0 = col * S.nrow + r - 1;
+                                v0 = s_off[k0];
+                                v1 = s_off[k0 + 3];
+                            }
+                            for (int v = v0; v < v1; ++v) {
+                                const int u = kSub ? (int)s_ord[v] : v;
                                 Real dx, dy;
                                 const Real dsq = min_image(xt, yt, s_x[u], s_y[u], C.Lx, C.Ly,
                                                            C.px, C.py, dx, dy);
@@ -367,7 +380,7 @@ __device__ __forceinline__ void force_strip(const Cells<Real>& C, const ForcePar
     }
 }
 
-template <typename Real>
+template <typename Real, bool kSub>
 __global__ __launch_bounds__(256) void k_force_tile(Cells<Real> C, ForceParams<Real> F,
                                                     Real* __restrict__ xn, Real* __restrict__ yn,
                                                     const double* __restrict__ noise_hook,
@@ -376,6 +389,6 @@ __global__ __launch_bounds__(256) void k_force_tile(Cells<Real> C, ForceParams<R
                                                     int ntiles) {
     const int tile = xcd_tile(ntiles);
     if (tile >= ntiles) return;  // uniform across the block
-    force_strip(C, F, xn, yn, noise_hook, rng, step, G, n_grids, keep_mask,
-                strip_geo(C, tile, ntile_y));
+    force_strip<Real, kSub>(C, F, xn, yn, noise_hook, rng, step, G, n_grids, keep_mask,
+                            strip_geo(C, tile, ntile_y));
 }
