@@ -1,6 +1,8 @@
 // Tiled catalysis pass (dynamics.py:826-1090), included by edm_dynamics.hip
 // after the reaction helpers.  Each workgroup stages its strip's
-// neighbourhood (pre-move and post-move positions, species, flags), lists
+// neighbourhood (pre-move positions, species, flags; the post-move
+// positions of the few pairs within the cutoff are read when those pairs
+// are evaluated), lists
 // the strip's own alive enzymes in LDS and tests every enzyme against the
 // substrates of its 3x3 cells, one wavefront per enzyme.  When substrates
 // are the rarer side the walk starts from them instead (own alive
@@ -127,7 +129,7 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
                                           unsigned long long* ctr, const Grids<Real>& G,
                                           int n_grids, const StripGeo& S) {
     constexpr int kCap = 1024;
-    __shared__ Real s_x[kCap], s_y[kCap], s_xn[kCap], s_yn[kCap];
+    __shared__ Real s_x[kCap], s_y[kCap];
     __shared__ int8_t s_sp[kCap];
     __shared__ uint8_t s_fl[kCap];
     __shared__ int s_list[kCap];
@@ -148,21 +150,19 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
     if (staged) {
         // all loads first, then the LDS stores: one memory round trip
         constexpr int kPer = kCap / 256;
-        Real vx[kPer], vy[kPer], vxn[kPer], vyn[kPer];
+        Real vx[kPer], vy[kPer];
         int8_t vs[kPer];
         uint8_t vf[kPer];
 #pragma unroll
         for (int j = 0; j < kPer; ++j) {
             const int o = (int)threadIdx.x + 256 * j;
-            vx[j] = vy[j] = vxn[j] = vyn[j] = (Real)0;
+            vx[j] = vy[j] = (Real)0;
             vs[j] = 0;
             vf[j] = 0;
             if (o < total) {
                 const int g = strip_src(S, s_off, s_gst, o);
                 vx[j] = C.x[g];
                 vy[j] = C.y[g];
-                vxn[j] = P.xn[g];
-                vyn[j] = P.yn[g];
                 vs[j] = P.sp[g];
                 vf[j] = C.fl[g];
             }
@@ -173,8 +173,6 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
             if (o < total) {
                 s_x[o] = vx[j];
                 s_y[o] = vy[j];
-                s_xn[o] = vxn[j];
-                s_yn[o] = vyn[j];
                 s_sp[o] = vs[j];
                 s_fl[o] = vf[j];
             }
@@ -229,7 +227,7 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
                 if (s != A.sub || !(f & EDM_FLAG_ALIVE)) continue;  // group-uniform
             }
             if (staged) {
-                const Real xt = s_x[i], yt = s_y[i], xnt = s_xn[i], ynt = s_yn[i];
+                const Real xt = s_x[i], yt = s_y[i];
                 const int r = strip_row(C, S, yt);
                 const int b0 = s_off[r - 1], l0 = s_off[r + 2] - b0;
                 const int b1 = s_off[S.nrow + r - 1], l1 = s_off[S.nrow + r + 2] - b1;
@@ -253,7 +251,7 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
                                         (i - own_begin) | (u << 10) | (j << 20))) {
                             const int tu = strip_src(S, s_off, s_gst, u);  // the enzyme
                             hits += cat_try(C, P, A, step, j, G, n_grids, tu, C.id[tu], t,
-                                            s_xn[u], s_yn[u], xnt, ynt);
+                                            P.xn[tu], P.yn[tu], P.xn[t], P.yn[t]);
                         }
                     }
                 }
@@ -297,7 +295,7 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
         }
         const int it = C.id[t];
         if (staged) {
-            const Real xt = s_x[i], yt = s_y[i], xnt = s_xn[i], ynt = s_yn[i];
+            const Real xt = s_x[i], yt = s_y[i];
             const int r = strip_row(C, S, yt);
             const int b0 = s_off[r - 1], l0 = s_off[r + 2] - b0;
             const int b1 = s_off[S.nrow + r - 1], l1 = s_off[S.nrow + r + 2] - b1;
@@ -309,9 +307,11 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
                 if (min_image(xt, yt, s_x[u], s_y[u], C.Lx, C.Ly, C.px, C.py, dx, dy) >= C.cut2)
                     continue;
                 if (!queue_push(s_q, &s_nq, kQCap, true,
-                                (i - own_begin) | (u << 10) | (j << 20)))
-                    hits += cat_try(C, P, A, step, j, G, n_grids, t, it,
-                                    strip_src(S, s_off, s_gst, u), xnt, ynt, s_xn[u], s_yn[u]);
+                                (i - own_begin) | (u << 10) | (j << 20))) {
+                    const int tu = strip_src(S, s_off, s_gst, u);  // the substrate
+                    hits += cat_try(C, P, A, step, j, G, n_grids, t, it, tu, P.xn[t], P.yn[t],
+                                    P.xn[tu], P.yn[tu]);
+                }
             }
         } else {
             const Real xt = C.x[t], yt = C.y[t], xnt = P.xn[t], ynt = P.yn[t];
@@ -340,12 +340,11 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
         const int i = own_begin + (v & 1023), u = (v >> 10) & 1023, j = v >> 20;
         const int to = gbase + (i - own_begin);       // the own entity's slot
         const int tc = strip_src(S, s_off, s_gst, u);  // the candidate's slot
+        const Real xo = P.xn[to], yo = P.yn[to], xc = P.xn[tc], yc = P.yn[tc];
         if (by_sub)  // enzyme u, substrate i
-            hits += cat_try(C, P, A, step, j, G, n_grids, tc, C.id[tc], to, s_xn[u], s_yn[u],
-                            s_xn[i], s_yn[i]);
+            hits += cat_try(C, P, A, step, j, G, n_grids, tc, C.id[tc], to, xc, yc, xo, yo);
         else  // enzyme i, substrate u
-            hits += cat_try(C, P, A, step, j, G, n_grids, to, C.id[to], tc, s_xn[i], s_yn[i],
-                            s_xn[u], s_yn[u]);
+            hits += cat_try(C, P, A, step, j, G, n_grids, to, C.id[to], tc, xo, yo, xc, yc);
     }
     unsigned long long v = hits;
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
