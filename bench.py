@@ -347,7 +347,7 @@ def main():
                                "200x200 density grid, 1024x1024x256 TCSPC frame every 100 steps",
                    "particles": r["N"], "alive_particles": int(r["alive"]),
                    "step": "bin+force+BD+reactions+PSS photons+PSF/TCSPC imaging"},
-        "roofline": {"bound": "hbm", "kernel": "k_force_tile<float>",
+        "roofline": {"bound": "hbm", "kernel": "k_force_tile<float, true>",
                      "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
                      "avg_launch_us": force_avg_s * 1e6,
