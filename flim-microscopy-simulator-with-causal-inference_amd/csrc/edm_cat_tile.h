@@ -123,7 +123,7 @@ __device__ __forceinline__ bool queue_push(int* q, int* nq, int cap, bool want, 
 // per-pair work runs on full wavefronts.  A pair's outcome does not depend
 // on when it is evaluated (its substrate converts once, on the first
 // accepted pair).
-template <typename Real>
+template <typename Real, int kCols>
 __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>& P,
                                           const CatArgs<Real>& A, uint32_t step,
                                           unsigned long long* ctr, const Grids<Real>& G,
@@ -133,8 +133,8 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
     __shared__ int8_t s_sp[kCap];
     __shared__ uint8_t s_fl[kCap];
     __shared__ int s_list[kCap];
-    __shared__ int s_off[kStripCells + 1];
-    __shared__ int s_gst[kStripCells];
+    __shared__ int s_off[strip_cells_max(kCols) + 1];
+    __shared__ int s_gst[kStripGst];
     __shared__ int s_wsum[4];
     __shared__ int s_nlist;
     constexpr int kQCap = 512;
@@ -145,7 +145,7 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
         s_nlist = 0;
         s_nq = 0;
     }
-    const int total = strip_cells(C, S, s_gst, s_off, s_wsum);
+    const int total = strip_cells<kCols>(C, S, s_gst, s_off, s_wsum);
     const bool staged = total <= kCap;
     if (staged) {
         // all loads first, then the LDS stores: one memory round trip
@@ -160,7 +160,7 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
             vs[j] = 0;
             vf[j] = 0;
             if (o < total) {
-                const int g = strip_src(S, s_off, s_gst, o);
+                const int g = strip_src<kCols>(S, s_off, s_gst, o);
                 vx[j] = C.x[g];
                 vy[j] = C.y[g];
                 vs[j] = P.sp[g];
@@ -179,20 +179,30 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
         }
     }
     __syncthreads();
-    const int own_begin = s_off[S.nrow + 1], own_end = s_off[S.nrow + S.rows + 1];
-    const int gbase = s_gst[S.nrow + 1];
+    // own particles: rows 1..rows of the own columns; an own entity is named
+    // by its staged position (a position of the own ranges also when the
+    // strip is not staged)
+    const OwnCol O1 = own_col(S, s_off, s_gst, 1);
+    const OwnCol O2 = kCols > 1 && S.ncol > 1 ? own_col(S, s_off, s_gst, 2) : OwnCol{0, 0, 0};
+    const int n1 = O1.end - O1.begin, nown = n1 + (O2.end - O2.begin);
+    auto own_at = [&](int q) { return q < n1 ? O1.begin + q : O2.begin + (q - n1); };
+    auto own_c2 = [&](int i) { return kCols > 1 && S.ncol > 1 && i >= O2.begin; };
+    auto own_slot = [&](int i) {
+        return own_c2(i) ? O2.gbase + (i - O2.begin) : O1.gbase + (i - O1.begin);
+    };
     // the strip's own alive enzymes as (member << 28 | own index) entries, or
     // its own alive substrates as (own index) entries
     const bool by_sub = A.by_substrate;
-    for (int i = own_begin + (int)threadIdx.x; i < own_end; i += blockDim.x) {
-        const int t = gbase + (i - own_begin);
+    for (int q = (int)threadIdx.x; q < nown; q += blockDim.x) {
+        const int i = own_at(q);
+        const int t = own_slot(i);
         const int s = staged ? s_sp[i] : P.sp[t];
         const uint8_t f = staged ? s_fl[i] : C.fl[t];
         if (!(f & EDM_FLAG_ALIVE)) continue;
         if (by_sub) {
             if (s == A.sub) {
                 const int e = atomicAdd(&s_nlist, 1);
-                if (e < kCap) s_list[e] = i - own_begin;
+                if (e < kCap) s_list[e] = i;
             }
             continue;
         }
@@ -200,14 +210,13 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
         for (int j = 0; j < kCatFuse; ++j)
             if (j < A.n_enz && s == A.enz[j]) {
                 const int e = atomicAdd(&s_nlist, 1);
-                if (e < kCap) s_list[e] = (j << 28) | (i - own_begin);
+                if (e < kCap) s_list[e] = (j << 28) | i;
             }
     }
     __syncthreads();
     // more entries than list slots (only in pathologically dense strips):
     // walk the own range directly, once per member
     const bool use_list = s_nlist <= kCap;
-    const int nown = own_end - own_begin;
     const int nl = use_list ? s_nlist : (by_sub ? nown : nown * A.n_enz);
     // a group of 2^group_shift lanes per entity (a wavefront at most), its
     // lanes over the entity's candidates: wide groups run the per-candidate
@@ -219,8 +228,9 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
     int hits = 0;
     if (by_sub) {
         for (int e = wave; e < nl; e += nwave) {
-            const int i = own_begin + (use_list ? s_list[e] : e);
-            const int t = gbase + (i - own_begin);  // the substrate
+            const int i = use_list ? s_list[e] : own_at(e);
+            const int t = own_slot(i);  // the substrate
+            const int c0 = own_c2(i) ? 1 : 0;  // staged column left of the own one
             if (!use_list) {
                 const int s = staged ? s_sp[i] : P.sp[t];
                 const uint8_t f = staged ? s_fl[i] : C.fl[t];
@@ -229,9 +239,10 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
             if (staged) {
                 const Real xt = s_x[i], yt = s_y[i];
                 const int r = strip_row(C, S, yt);
-                const int b0 = s_off[r - 1], l0 = s_off[r + 2] - b0;
-                const int b1 = s_off[S.nrow + r - 1], l1 = s_off[S.nrow + r + 2] - b1;
-                const int b2 = s_off[2 * S.nrow + r - 1], l2 = s_off[2 * S.nrow + r + 2] - b2;
+                const int k0 = c0 * S.nrow + r - 1;
+                const int b0 = s_off[k0], l0 = s_off[k0 + 3] - b0;
+                const int b1 = s_off[k0 + S.nrow], l1 = s_off[k0 + S.nrow + 3] - b1;
+                const int b2 = s_off[k0 + 2 * S.nrow], l2 = s_off[k0 + 2 * S.nrow + 3] - b2;
                 for (int c = lane; c < l0 + l1 + l2; c += gsize) {
                     const int u =
                         c < l0 ? b0 + c : (c < l0 + l1 ? b1 + (c - l0) : b2 + (c - l0 - l1));
@@ -247,9 +258,8 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
                         continue;
                     for (int j = 0; j < A.n_enz; ++j) {
                         const bool want = su == cat_pick(A.enz, j);
-                        if (!queue_push(s_q, &s_nq, kQCap, want,
-                                        (i - own_begin) | (u << 10) | (j << 20))) {
-                            const int tu = strip_src(S, s_off, s_gst, u);  // the enzyme
+                        if (!queue_push(s_q, &s_nq, kQCap, want, i | (u << 10) | (j << 20))) {
+                            const int tu = strip_src<kCols>(S, s_off, s_gst, u);  // the enzyme
                             hits += cat_try(C, P, A, step, j, G, n_grids, tu, C.id[tu], t,
                                             P.xn[tu], P.yn[tu], P.xn[t], P.yn[t]);
                         }
@@ -284,10 +294,11 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
         }
     }
     for (int e = wave; e < (by_sub ? 0 : nl); e += nwave) {
-        const int ent = use_list ? s_list[e] : ((e / nown) << 28) | (e % nown);
+        const int ent = use_list ? s_list[e] : ((e / nown) << 28) | own_at(e % nown);
         const int j = ent >> 28;
-        const int i = own_begin + (ent & 0x0fffffff);
-        const int t = gbase + (i - own_begin);
+        const int i = ent & 0x0fffffff;
+        const int t = own_slot(i);
+        const int c0 = own_c2(i) ? 1 : 0;  // staged column left of the own one
         if (!use_list) {
             const int s = staged ? s_sp[i] : P.sp[t];
             const uint8_t f = staged ? s_fl[i] : C.fl[t];
@@ -297,18 +308,18 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
         if (staged) {
             const Real xt = s_x[i], yt = s_y[i];
             const int r = strip_row(C, S, yt);
-            const int b0 = s_off[r - 1], l0 = s_off[r + 2] - b0;
-            const int b1 = s_off[S.nrow + r - 1], l1 = s_off[S.nrow + r + 2] - b1;
-            const int b2 = s_off[2 * S.nrow + r - 1], l2 = s_off[2 * S.nrow + r + 2] - b2;
+            const int k0 = c0 * S.nrow + r - 1;
+            const int b0 = s_off[k0], l0 = s_off[k0 + 3] - b0;
+            const int b1 = s_off[k0 + S.nrow], l1 = s_off[k0 + S.nrow + 3] - b1;
+            const int b2 = s_off[k0 + 2 * S.nrow], l2 = s_off[k0 + 2 * S.nrow + 3] - b2;
             for (int c = lane; c < l0 + l1 + l2; c += gsize) {
                 const int u = c < l0 ? b0 + c : (c < l0 + l1 ? b1 + (c - l0) : b2 + (c - l0 - l1));
                 if (u == i || s_sp[u] != A.sub || !(s_fl[u] & EDM_FLAG_ALIVE)) continue;
                 Real dx, dy;
                 if (min_image(xt, yt, s_x[u], s_y[u], C.Lx, C.Ly, C.px, C.py, dx, dy) >= C.cut2)
                     continue;
-                if (!queue_push(s_q, &s_nq, kQCap, true,
-                                (i - own_begin) | (u << 10) | (j << 20))) {
-                    const int tu = strip_src(S, s_off, s_gst, u);  // the substrate
+                if (!queue_push(s_q, &s_nq, kQCap, true, i | (u << 10) | (j << 20))) {
+                    const int tu = strip_src<kCols>(S, s_off, s_gst, u);  // the substrate
                     hits += cat_try(C, P, A, step, j, G, n_grids, t, it, tu, P.xn[t], P.yn[t],
                                     P.xn[tu], P.yn[tu]);
                 }
@@ -337,9 +348,9 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
     const int nq = min(s_nq, kQCap);
     for (int k = (int)threadIdx.x; k < nq; k += (int)blockDim.x) {
         const int v = s_q[k];
-        const int i = own_begin + (v & 1023), u = (v >> 10) & 1023, j = v >> 20;
-        const int to = gbase + (i - own_begin);       // the own entity's slot
-        const int tc = strip_src(S, s_off, s_gst, u);  // the candidate's slot
+        const int i = v & 1023, u = (v >> 10) & 1023, j = v >> 20;
+        const int to = own_slot(i);                   // the own entity's slot
+        const int tc = strip_src<kCols>(S, s_off, s_gst, u);  // the candidate's slot
         const Real xo = P.xn[to], yo = P.yn[to], xc = P.xn[tc], yc = P.yn[tc];
         if (by_sub)  // enzyme u, substrate i
             hits += cat_try(C, P, A, step, j, G, n_grids, tc, C.id[tc], to, xc, yc, xo, yo);
@@ -362,7 +373,7 @@ __global__ __launch_bounds__(256) void k_cat_tile(Cells<Real> C, Post<Real> P, C
     if (tile >= ntiles) return;  // uniform across the block
     if (threadIdx.x == 0) sA = A;
     __syncthreads();
-    cat_strip(C, P, sA, A.step, ctr, G, n_grids, strip_geo(C, tile, ntile_y));
+    cat_strip<Real, 1>(C, P, sA, A.step, ctr, G, n_grids, strip_geo(C, tile, ntile_y));
 }
 
 // Catalysis over a work list of its walk entities, one listed slot per

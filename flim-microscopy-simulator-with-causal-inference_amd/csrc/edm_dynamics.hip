@@ -1621,7 +1621,7 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
         const dim3 grid(8 * ((ntiles + 7) / 8));
         if (m->sub_rows)
             hipLaunchKernelGGL((k_force_tile<Real, true>), grid, dim3(256),
-                               force_sub_lds(m->tile_rows), st, C, F, (Real*)m->a.xn,
+                               force_sub_lds(m->tile_rows, 1), st, C, F, (Real*)m->a.xn,
                                (Real*)m->a.yn, noise, rng, step, G, ng, m->sensor_mask_all,
                                ntile_y, ntiles);
         else

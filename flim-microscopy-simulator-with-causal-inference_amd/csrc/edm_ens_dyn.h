@@ -77,7 +77,9 @@ __device__ __forceinline__ StripGeo ens_strip(const EnsSeg& g, const EnsTile& T)
     S.cy0 = T.cy0;
     S.rows = min(T.rows, g.ncy - T.cy0);
     S.nrow = S.rows + 2;
-    S.ncells = 3 * S.nrow;
+    S.ncol = T.ncol;
+    S.nscol = T.ncol + 2;
+    S.ncells = S.nscol * S.nrow;
     return S;
 }
 
@@ -194,8 +196,9 @@ __global__ __launch_bounds__(256) void k_ens_force(const EnsTile* __restrict__ t
     Fr.sin_kx = (Real)g.sin_kx;
     Fr.sin_ky = (Real)g.sin_ky;
     const Grids<Real> G = ens_grids<Real>(g, Q.counts, Q.delta, Q.ns);
-    force_strip<Real, kSub>(C, Fr, Q.xn, Q.yn, (const double*)nullptr, ens_rng(g),
-                            gstep - g.step0, G, Q.n_grids, keep_mask, ens_strip<Real>(g, T));
+    force_strip<Real, kSub, kMaxOwnCols>(C, Fr, Q.xn, Q.yn, (const double*)nullptr, ens_rng(g),
+                                         gstep - g.step0, G, Q.n_grids, keep_mask,
+                                         ens_strip<Real>(g, T));
 }
 
 // Catalysis for one run of catalytic reactions; the run's parameter block
@@ -213,7 +216,7 @@ __global__ __launch_bounds__(256) void k_ens_cat(const EnsTile* __restrict__ til
     const Cells<Real> C = ens_cells<Real>(g, Q.x, Q.y, Q.sp, Q.fl, Q.id, Q.start);
     const CatArgs<Real>& A = cats[(size_t)T.r * n_runs + ci];
     const Grids<Real> G = ens_grids<Real>(g, Q.counts, Q.delta, Q.ns);
-    cat_strip(C, ens_post(Q), A, gstep - g.step0, ctr + 8 * T.r, G, Q.n_grids,
+    cat_strip<Real, kMaxOwnCols>(C, ens_post(Q), A, gstep - g.step0, ctr + 8 * T.r, G, Q.n_grids,
               ens_strip<Real>(g, T));
 }
 
@@ -483,17 +486,8 @@ int ens_layout(edm_ensemble* e) {
     // strip height: about 192 own particles per 256-thread workgroup at the
     // live replicas' mean density, with the staged 3 x (rows + 2) cells well
     // inside the LDS capacity (1024 particles); results do not depend on it
-    {
-        int64_t np = 0, nc = 0;
-        for (int r = 0; r < e->rmax; ++r)
-            if (e->seg[r].active) {
-                np += e->seg[r].n;
-                nc += (int64_t)e->seg[r].ncx * e->seg[r].ncy;
-            }
-        const double dens = std::max(nc > 0 ? (double)np / (double)nc : 1.0, 1e-3);
-        const int cap_rows = (int)(1024.0 / (4.5 * dens)) - 2;
-        e->tile_rows = std::max(16, std::min({(int)(192.0 / dens), cap_rows, kTileRows}));
-    }
+    e->tile_rows = 1;
+    e->tile_cols = 1;
     int64_t coff = 0, cnt = 0;
     e->tiles.clear();
     std::vector<int32_t> wrep[2];
@@ -513,10 +507,25 @@ int ens_layout(edm_ensemble* e) {
             g.goff[kp.first * ns + kp.second] = (int32_t)cnt;
             cnt += (int64_t)g.gdims[2 * kp.first] * g.gdims[2 * kp.first + 1];
         }
-        const int tr = e->tile_rows;
-        const int ty = (g.ncy + tr - 1) / tr;
-        for (int cx = 0; cx < g.ncx; ++cx)
-            for (int t = 0; t < ty; ++t) e->tiles.push_back(EnsTile{r, cx, t * tr, tr});
+        // strips of about 192 own particles: a column that holds fewer than
+        // 128 is taken together with its neighbour (boxes of 4 columns or
+        // more), a long one is cut into equal runs of rows; at most ~700
+        // particles (1024 with room for fluctuations) staged per strip
+        {
+            const double dens = std::max((double)g.n / ((double)g.ncx * g.ncy), 1e-3);
+            const double col_parts = dens * g.ncy;
+            const int ncol = (col_parts < 128.0 && g.ncx >= 4) ? 2 : 1;
+            int ty = std::max(1, (int)std::lround(col_parts * ncol / 192.0));
+            ty = std::max(ty, (int)std::ceil((ncol + 2) * (g.ncy + 2) * dens / 700.0));
+            ty = std::max(ty, (g.ncy + kTileRows - 1) / kTileRows);
+            ty = std::min(ty, g.ncy);
+            const int rows = (g.ncy + ty - 1) / ty;
+            e->tile_rows = std::max(e->tile_rows, rows);
+            e->tile_cols = std::max(e->tile_cols, ncol);
+            for (int cx = 0; cx < g.ncx; cx += ncol)
+                for (int cy0 = 0; cy0 < g.ncy; cy0 += rows)
+                    e->tiles.push_back(EnsTile{r, cx, cy0, rows, std::min(ncol, g.ncx - cx)});
+        }
     }
     if (coff >= (int64_t)INT32_MAX - 2 || cnt >= (int64_t)INT32_MAX) {
         set_error("edm_ensemble: too many cells for 32-bit keys");
@@ -689,7 +698,8 @@ static int ens_step_t(edm_ensemble* e) {
     for (int r = 0; r < e->rmax; ++r) any_sub |= e->seg[r].active && e->seg[r].sub_rows;
     if (any_sub)
         hipLaunchKernelGGL((k_ens_force<Real, true>), tgrid, dim3(256),
-                           force_sub_lds(e->tile_rows), st, (const EnsTile*)e->d_tiles, ntiles,
+                           force_sub_lds(e->tile_rows, e->tile_cols), st,
+                           (const EnsTile*)e->d_tiles, ntiles,
                            (const EnsSeg*)e->d_seg, Q, F, e->sensor_mask_all, gstep);
     else
         hipLaunchKernelGGL((k_ens_force<Real, false>), tgrid, dim3(256), 0, st,

@@ -74,7 +74,7 @@ struct EnsChunk {
 
 // Strip tile of one replica.
 struct EnsTile {
-    int32_t r, cx, cy0, rows;
+    int32_t r, cx, cy0, rows, ncol;  // ncol own columns cx .. cx + ncol - 1
 };
 
 struct edm_ensemble {
@@ -116,7 +116,8 @@ struct edm_ensemble {
     std::vector<EnsSeg> seg;  // host mirror, [rmax]
     EnsSeg* d_seg = nullptr;
     std::vector<EnsTile> tiles;
-    int tile_rows = 64;  // cell rows per strip, from the model's particle density
+    int tile_rows = 64;  // most cell rows of any strip (sizes the strips' dynamic LDS)
+    int tile_cols = 1;   // most own columns of any strip
     EnsTile* d_tiles = nullptr;
     int64_t tile_cap = 0;
     int64_t n_total = 0;  // particles in the pool (live replicas first after a sort)

@@ -2,23 +2,51 @@
 // edm_dynamics.hip; relies on its Cells / ForceParams / Grids helpers).
 //
 // A workgroup of 256 threads owns a strip of C.tile_rows (at most
-// kTileRows) cells of one cell column (sparser membranes take taller strips,
-// so that a workgroup owns about as many particles as it has threads).  The strip's 3 x (rows + 2) neighbourhood cells (columns cx-1..cx+1,
-// rows cy0-1..cy0+rows, wrapped or clipped at the box edges) are staged in
-// LDS, one particle per thread, so that consecutive threads read consecutive
-// slots (strip_src).  Inside the strip every particle's 3x3
-// cell neighbourhood is then three contiguous LDS ranges, one per column.
+// kTileRows) cells of one cell column, or of two adjacent columns (sparse
+// membranes whose columns are short: a workgroup then owns about as many
+// particles as it has threads).  The strip's (ncol + 2) x (rows + 2)
+// neighbourhood cells (columns cx-1..cx+ncol, rows cy0-1..cy0+rows, wrapped
+// or clipped at the box edges) are staged in LDS, one particle per thread,
+// so that consecutive threads read consecutive slots (strip_src).  Inside
+// the strip every particle's 3x3 cell neighbourhood is then three
+// contiguous LDS ranges, one per column.
 // Blocks that the dispatcher sends to the same XCD (blockIdx % 8) take
 // adjacent strips, so a strip's neighbour columns are usually in that XCD's
 // L2.  A strip whose neighbourhood exceeds the LDS capacity is handled with
 // global-memory neighbour walks instead.
 #pragma once
 
-static constexpr int kStripCells = 3 * (kTileRows + 2);
+static constexpr int kMaxOwnCols = 2;
+// staged cells of a strip kernel built for at most kCols own columns
+__host__ __device__ constexpr int strip_cells_max(int kCols) { return (kCols + 2) * (kTileRows + 2); }
+static constexpr int kStripCells = strip_cells_max(kMaxOwnCols);
+// global start of a staged column's first row, middle rows and last row:
+// s_gst[3 * column + 0 / 1 / 2] (the middle rows are one slot range)
+static constexpr int kStripGst = 3 * (kMaxOwnCols + 2);
 
+// ncol own columns (1 or 2) = staged columns 1..ncol; nscol = ncol + 2
+// staged columns of nrow = rows + 2 cells each, column-major.
 struct StripGeo {
-    int tile, cx, cy0, rows, nrow, ncells;
+    int tile, cx, cy0, rows, nrow, ncells, ncol, nscol;
 };
+
+// Staged column of staged cell k or of a staged position o.
+__device__ __forceinline__ int strip_col_of_cell(const StripGeo& S, int k) {
+    return (k >= S.nrow) + (k >= 2 * S.nrow) + (k >= 3 * S.nrow);
+}
+
+// The own particles of own column c (1..ncol): staged positions [begin,
+// end) and the global slot of begin (a column's middle rows are one
+// contiguous slot range).
+struct OwnCol {
+    int begin, end, gbase;
+};
+
+__device__ __forceinline__ OwnCol own_col(const StripGeo& S, const int* s_off, const int* s_gst,
+                                          int c) {
+    const int k = c * S.nrow + 1;
+    return OwnCol{s_off[k], s_off[k + S.rows], s_gst[3 * c + 1]};
+}
 
 __device__ __forceinline__ int xcd_tile(int ntiles) {
     const int per = (ntiles + 7) / 8;
@@ -33,19 +61,22 @@ __device__ __forceinline__ StripGeo strip_geo(const Cells<Real>& C, int tile, in
     S.cy0 = (tile % ntile_y) * C.tile_rows;
     S.rows = min(C.tile_rows, C.ncy - S.cy0);
     S.nrow = S.rows + 2;
+    S.ncol = 1;
+    S.nscol = 3;
     S.ncells = 3 * S.nrow;
     return S;
 }
 
-// Global start and LDS offset of every staged cell: s_gst[k], s_off[k]
+// LDS offset of every staged cell, s_off[k], and the global starts
+// s_gst[kStripGst]
 // (exclusive scan of the cell counts, s_off[ncells] = total).  Every
 // thread of the 256-wide block must call it.  Returns the total.
-template <typename Real>
+template <int kCols, typename Real>
 __device__ __forceinline__ int strip_cells(const Cells<Real>& C, const StripGeo& S, int* s_gst,
                                            int* s_off, int* s_wsum) {
     // thread i takes the consecutive cells i*per .. i*per + per - 1 (one
     // cell per thread for strips of up to 256 cells)
-    constexpr int kPer = (kStripCells + 255) / 256;
+    constexpr int kPer = (strip_cells_max(kCols) + 255) / 256;
     const int per = S.ncells <= 256 ? 1 : kPer;
     int cnt[kPer];
     int sum = 0;
@@ -55,7 +86,7 @@ __device__ __forceinline__ int strip_cells(const Cells<Real>& C, const StripGeo&
         if (j >= per) break;
         const int k = (int)threadIdx.x * per + j;
         if (k < S.ncells) {
-            const int col = (k >= S.nrow) + (k >= 2 * S.nrow), r = k - col * S.nrow;
+            const int col = strip_col_of_cell(S, k), r = k - col * S.nrow;
             int gx = S.cx + col - 1, gy = S.cy0 + r - 1;
             bool ok = true;
             if (C.px) gx = gx < 0 ? gx + C.ncx : (gx >= C.ncx ? gx - C.ncx : gx);
@@ -68,7 +99,8 @@ __device__ __forceinline__ int strip_cells(const Cells<Real>& C, const StripGeo&
                 gs = (int)C.start[c];
                 cnt[j] = (int)C.start[c + 1] - gs;
             }
-            s_gst[k] = gs;
+            if (r <= 1) s_gst[3 * col + r] = gs;
+            if (r == S.nrow - 1) s_gst[3 * col + 2] = gs;
         }
         sum += cnt[j];
     }
@@ -101,14 +133,22 @@ __device__ __forceinline__ int strip_cells(const Cells<Real>& C, const StripGeo&
 // form one contiguous slot range; only its first and last staged rows can
 // be wrapped or clipped cells, and those are mapped on their own.  Staging
 // therefore reads consecutive addresses with consecutive threads.
+template <int kCols>
 __device__ __forceinline__ int strip_src(const StripGeo& S, const int* s_off, const int* s_gst,
                                          int o) {
-    const int col = (o >= s_off[S.nrow]) + (o >= s_off[2 * S.nrow]);
+    // s_off[3 * nrow] is the total when there are three staged columns
+    int col = (o >= s_off[S.nrow]) + (o >= s_off[2 * S.nrow]);
+    if constexpr (kCols > 1) col += o >= s_off[3 * S.nrow];
     const int k0 = col * S.nrow;
-    int k = k0 + 1;
-    if (o < s_off[k0 + 1]) k = k0;
-    else if (o >= s_off[k0 + S.nrow - 1]) k = k0 + S.nrow - 1;
-    return s_gst[k] + (o - s_off[k]);
+    int k = k0 + 1, w = 1;
+    if (o < s_off[k0 + 1]) {
+        k = k0;
+        w = 0;
+    } else if (o >= s_off[k0 + S.nrow - 1]) {
+        k = k0 + S.nrow - 1;
+        w = 2;
+    }
+    return s_gst[3 * col + w] + (o - s_off[k]);
 }
 
 // Staged row (1..rows) of an own particle at pre-move position y.
@@ -126,9 +166,9 @@ __device__ __forceinline__ int strip_row(const Cells<Real>& C, const StripGeo& S
 static constexpr int kMaxSub = 8;
 
 // Dynamic LDS bytes of a force strip launch whose strips have at most
-// `rows` cell rows.
-static inline size_t force_sub_lds(int rows) {
-    return (size_t)(3 * (rows + 2) * kMaxSub + 1) * sizeof(int16_t);
+// `rows` cell rows and `ncol` own columns.
+static inline size_t force_sub_lds(int rows, int ncol) {
+    return (size_t)((ncol + 2) * (rows + 2) * kMaxSub + 1) * sizeof(int16_t);
 }
 
 template <typename Real>
@@ -153,7 +193,7 @@ __device__ __forceinline__ int sub_row(const Cells<Real>& C, Real y, int gy, int
 // kSub = false: no membrane of the launch uses sub-rows (one per cell; no
 // sort, no dynamic LDS); kSub = true: a membrane without them sorts with
 // one sub-row per cell, which keeps its order.
-template <typename Real, bool kSub>
+template <typename Real, bool kSub, int kCols>
 __device__ __forceinline__ void force_strip(const Cells<Real>& C, const ForceParams<Real>& F,
                                             Real* __restrict__ xn, Real* __restrict__ yn,
                                             const double* __restrict__ noise_hook, Rng rng,
@@ -165,8 +205,8 @@ __device__ __forceinline__ void force_strip(const Cells<Real>& C, const ForcePar
     __shared__ Real s_x[kCap], s_y[kCap];
     __shared__ int8_t s_sp[kCap];
     __shared__ uint8_t s_fl[kCap];
-    __shared__ int s_off[kStripCells + 1];
-    __shared__ int s_gst[kStripCells];
+    __shared__ int s_off[strip_cells_max(kCols) + 1];
+    __shared__ int s_gst[kStripGst];
     __shared__ int s_wsum[4];
     __shared__ Real s_sig[kTab], s_eps[kTab], s_kh[kTab];
     // staged positions in (cell, sub-row, slot) order, each staged
@@ -190,7 +230,7 @@ __device__ __forceinline__ void force_strip(const Cells<Real>& C, const ForcePar
     FL.sig = s_sig;
     FL.eps = s_eps;
     FL.kh = s_kh;
-    const int total = strip_cells(C, S, s_gst, s_off, s_wsum);
+    const int total = strip_cells<kCols>(C, S, s_gst, s_off, s_wsum);
     const bool staged = total <= kCap;
     if (staged) {
         // all loads first, then the LDS stores: one memory round trip
@@ -205,7 +245,7 @@ __device__ __forceinline__ void force_strip(const Cells<Real>& C, const ForcePar
             vs[j] = 0;
             vf[j] = 0;
             if (o < total) {
-                const int g = strip_src(S, s_off, s_gst, o);
+                const int g = strip_src<kCols>(S, s_off, s_gst, o);
                 vx[j] = C.x[g];
                 vy[j] = C.y[g];
                 vs[j] = C.sp[g];
@@ -224,9 +264,11 @@ __device__ __forceinline__ void force_strip(const Cells<Real>& C, const ForcePar
         }
     }
     __syncthreads();
-    // own particles: the middle column's rows 1..rows, contiguous in memory
-    const int own_begin = s_off[S.nrow + 1], own_end = s_off[S.nrow + S.rows + 1];
-    const int gbase = s_gst[S.nrow + 1];
+    // own particles: rows 1..rows of the own columns, each column's
+    // contiguous in memory
+    const OwnCol O1 = own_col(S, s_off, s_gst, 1);
+    const OwnCol O2 = kCols > 1 && S.ncol > 1 ? own_col(S, s_off, s_gst, 2) : OwnCol{0, 0, 0};
+    const int n1 = O1.end - O1.begin, n_own = n1 + (O2.end - O2.begin);
     // pairs beyond the pair-list cutoff or the longest potential range
     // contribute nothing
     const Real rl2 = min(C.cut2, F.rint2);
@@ -236,7 +278,7 @@ __device__ __forceinline__ void force_strip(const Cells<Real>& C, const ForcePar
         // counting sort of every staged cell by sub-row (stable: slot order
         // inside a sub-row), one thread per cell
         for (int k = (int)threadIdx.x; k < S.ncells; k += (int)blockDim.x) {
-            const int col = (k >= S.nrow) + (k >= 2 * S.nrow), r = k - col * S.nrow;
+            const int col = strip_col_of_cell(S, k), r = k - col * S.nrow;
             int gy = S.cy0 + r - 1;
             gy = gy < 0 ? gy + C.ncy : (gy >= C.ncy ? gy - C.ncy : gy);
             const int o0 = s_off[k], o1 = s_off[k + 1];
@@ -269,13 +311,16 @@ __device__ __forceinline__ void force_strip(const Cells<Real>& C, const ForcePar
         if (threadIdx.x == 0) s_sub[S.ncells * nsub] = (int16_t)total;
         __syncthreads();
     }
-    for (int i0 = own_begin; i0 < own_end; i0 += blockDim.x) {
-        const int i = i0 + (int)threadIdx.x;
+    for (int q0 = 0; q0 < n_own; q0 += blockDim.x) {
+        const int q = q0 + (int)threadIdx.x;
         bool take_any = false;
         Real nxp = 0, nyp = 0;
         int si = 0;
-        if (i < own_end) {
-            const int t = gbase + (i - own_begin);
+        if (q < n_own) {
+            const bool c1 = kCols == 1 || q < n1;
+            const int oc = c1 ? 1 : 2;  // own column
+            const int i = c1 ? O1.begin + q : O2.begin + (q - n1);
+            const int t = c1 ? O1.gbase + q : O2.gbase + (q - n1);
             Real xt, yt;
             uint8_t ft;
             if (staged) {
@@ -302,7 +347,7 @@ __device__ __forceinline__ void force_strip(const Cells<Real>& C, const ForcePar
                     const int r = strip_row(C, S, yt);
                     const int g = r * nsub + (kSub ? (int)s_b[i] : 0);
                     auto sweep = [&](const ForceParams<Real>& T) {
-                        for (int col = 0; col < 3; ++col) {
+                        for (int col = oc - 1; col <= oc + 1; ++col) {
                             int v0, v1;
                             if constexpr (kSub) {
                                 const int base = col * S.nrow * nsub;
@@ -389,6 +434,6 @@ __global__ __launch_bounds__(256) void k_force_tile(Cells<Real> C, ForceParams<R
                                                     int ntiles) {
     const int tile = xcd_tile(ntiles);
     if (tile >= ntiles) return;  // uniform across the block
-    force_strip<Real, kSub>(C, F, xn, yn, noise_hook, rng, step, G, n_grids, keep_mask,
-                            strip_geo(C, tile, ntile_y));
+    force_strip<Real, kSub, 1>(C, F, xn, yn, noise_hook, rng, step, G, n_grids, keep_mask,
+                               strip_geo(C, tile, ntile_y));
 }
