@@ -3,7 +3,7 @@
 // the step's work list emit with their replica's draws, image geometry and
 // TCSPC stack.
 
-template <typename Real>
+template <typename Real, int kWin>
 __global__ __launch_bounds__(256) void k_ens_photons(int n_waves, const int32_t* __restrict__ list,
                                                      const int32_t* __restrict__ wrep,
                                                      const uint32_t* __restrict__ rcnt,
@@ -19,33 +19,42 @@ __global__ __launch_bounds__(256) void k_ens_photons(int n_waves, const int32_t*
     const EnsSeg& g = segs[W.r];
     const int j = W.j + lane_id();
     int emitted = 0, detected = 0;
+    const Rng rng{g.seed_lo ^ 0x5bd1e995u, g.seed_hi};
+    Emit<Real> E{0, 0, 0u, 0, 0, 0, 0, (Real)0, (Real)0};
     if (j < min(g.wl_n[1], (int)rcnt[2 * W.r + 1])) {
         const int t = list[g.wl_off[1] + j];
-        const Rng rng{g.seed_lo ^ 0x5bd1e995u, g.seed_hi};
-        emit_image<Real>(pp, &g.ip, t, (uint32_t)(id[t] - g.kbase), x, y, fl, dye, ctmc, nullptr,
-                         nullptr, true, 0.0, -1, g.flim, rng, gstep - g.step0, emitted,
-                         detected);
+        E = emit_pss<Real>(pp, t, (uint32_t)(id[t] - g.kbase), fl, dye, ctmc, nullptr, nullptr,
+                           true, 0.0, -1, rng, gstep - g.step0, emitted);
     }
+    // the wave's emitters are all of replica W.r: its geometry, stack and draws
+    wave_photons<Real, kWin>(E, &g.ip, x, y, g.flim, rng, gstep - g.step0, detected);
     photon_count(emitted, detected, ctr + 8 * W.r);
+}
+
+template <typename Real, int kWin>
+static void launch_ens_photons(edm_ensemble* e, int nt) {
+    hipLaunchKernelGGL((k_ens_photons<Real, kWin>), dim3(nb(nt)), dim3(256), 0, e->stream, nt / 64,
+                       (const int32_t*)e->wl_dye, (const int32_t*)e->wl_wrep_dye,
+                       (const uint32_t*)e->wl_rcnt, (const EnsSeg*)e->d_seg,
+                       (const edm_photo_params*)e->d_pp, (const Real*)e->a.x,
+                       (const Real*)e->a.y, e->a.fl, (const int8_t*)e->a.dye, e->a.ctmc,
+                       (const int32_t*)e->a.id, e->ctr, e->gstep);
 }
 
 int ens_photons(edm_ensemble* e) {
     const int nt = (int)e->wl_ndye;
     if (nt == 0 || !e->has_pp) return 0;
-    if (e->fp64)
-        hipLaunchKernelGGL(k_ens_photons<double>, dim3(nb(nt)), dim3(256), 0, e->stream, nt / 64,
-                           (const int32_t*)e->wl_dye, (const int32_t*)e->wl_wrep_dye,
-                           (const uint32_t*)e->wl_rcnt, (const EnsSeg*)e->d_seg,
-                           (const edm_photo_params*)e->d_pp, (const double*)e->a.x,
-                           (const double*)e->a.y, e->a.fl, (const int8_t*)e->a.dye, e->a.ctmc,
-                           (const int32_t*)e->a.id, e->ctr, e->gstep);
-    else
-        hipLaunchKernelGGL(k_ens_photons<float>, dim3(nb(nt)), dim3(256), 0, e->stream, nt / 64,
-                           (const int32_t*)e->wl_dye, (const int32_t*)e->wl_wrep_dye,
-                           (const uint32_t*)e->wl_rcnt, (const EnsSeg*)e->d_seg,
-                           (const edm_photo_params*)e->d_pp, (const float*)e->a.x,
-                           (const float*)e->a.y, e->a.fl, (const int8_t*)e->a.dye, e->a.ctmc,
-                           (const int32_t*)e->a.id, e->ctr, e->gstep);
+    // the PSF window capacity of the launch covers every replica's window
+    bool small = true;
+    for (int r = 0; r < e->rmax; ++r)
+        if (e->seg[r].active) small = small && psf_small(e->seg[r].ip.R);
+    if (e->fp64) {
+        if (small) launch_ens_photons<double, 8>(e, nt);
+        else launch_ens_photons<double, 31>(e, nt);
+    } else {
+        if (small) launch_ens_photons<float, 8>(e, nt);
+        else launch_ens_photons<float, 31>(e, nt);
+    }
     EDM_HIP(hipGetLastError());
     return 0;
 }

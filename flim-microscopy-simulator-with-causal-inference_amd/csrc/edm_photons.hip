@@ -334,11 +334,11 @@ __device__ __forceinline__ void fret_inline(const edm_photo_params* pp, int t, c
 // exp(-dx^2 k) exp(-dy^2 k): the same discrete distribution as the
 // reference's joint CDF walk (di-major), evaluated with 2(2R+1) exps in
 // the working precision.  Windows wider than 31 pixels are rejected when
-// the imaging parameters are set.
-template <typename Real>
+// the imaging parameters are set; kMax (8 or 31) is the window the launch was
+// built for (the weights live in registers).
+template <typename Real, int kMax>
 __device__ __forceinline__ bool psf_pick_sep(const edm_image_params* ip, Real x, Real y, int ch,
                                              Real u, int& opx, int& opy) {
-    constexpr int kMax = 31;
     const int R = ip->R, nx = ip->nx, ny = ip->ny;
     if (ch < 0) ch = 0;
     if (ch >= ip->n_sigma) ch = 0;
@@ -394,19 +394,28 @@ __device__ __forceinline__ bool psf_pick_sep(const edm_image_params* ip, Real x,
     return true;
 }
 
-// One emitter of the fused photon path with Philox draws: FRET rate, PSS
-// counts and CTMC transition, then for every photon its arrival time,
-// detection, PSF pixel, IRF jitter and TCSPC bin.  `key` is the particle's
-// draw key; FRET comes from (kf_in, acc_in) when have_kf, else from the
-// bond maps (acceptor lookups need slot_of).
+// The photons of one emitter after its PSS step: n = nd donor photons then
+// n - nd FRET-sensitised acceptor photons, drawn with the emitter's key and
+// the photon's index.
 template <typename Real>
-__device__ __forceinline__ void emit_image(const edm_photo_params* pp, const edm_image_params* ip,
-                                           int t, uint32_t key, const Real* x, const Real* y,
-                                           uint8_t* fl, const int8_t* dye, int8_t* ctmc,
-                                           const int32_t* bond, const int32_t* slot_of,
-                                           bool have_kf, double kf_in, int acc_in,
-                                           uint32_t* flim, Rng rng, uint32_t step, int& emitted,
-                                           int& detected) {
+struct Emit {
+    int n, nd;
+    uint32_t key;
+    int es_d, ch_d, es_a, ch_a;  // emitting slot and channel of donor / acceptor photons
+    Real tau_d, tau_a;
+};
+
+// One emitter of the fused photon path with Philox draws: FRET rate, PSS
+// counts and CTMC transition.  `key` is the particle's draw key; FRET comes
+// from (kf_in, acc_in) when have_kf, else from the bond maps (acceptor
+// lookups need slot_of).
+template <typename Real>
+__device__ __forceinline__ Emit<Real> emit_pss(const edm_photo_params* pp, int t, uint32_t key,
+                                               uint8_t* fl, const int8_t* dye, int8_t* ctmc,
+                                               const int32_t* bond, const int32_t* slot_of,
+                                               bool have_kf, double kf_in, int acc_in, Rng rng,
+                                               uint32_t step, int& emitted) {
+    Emit<Real> E{0, 0, key, t, 0, t, 0, (Real)0, (Real)0};
     uint8_t f = fl[t];
     int8_t c = ctmc[t];
     double kf;
@@ -422,35 +431,82 @@ __device__ __forceinline__ void emit_image(const edm_photo_params* pp, const edm
     PssOut o{0, 0, 0.0, 0};
     if (pss_eval(pp, f, c, dye[t], kf, aid, up, ut, o)) {
         emitted += o.nd + o.nf;
-        const Real qe = (Real)ip->qe;
-        const Real sirf = (Real)ip->sigma_irf;
-        for (int k = 0; k < o.nd + o.nf; ++k) {
-            const bool donor = k < o.nd;
-            int es = t, ch = o.ty;
-            Real tau = (Real)o.tau;
-            if (!donor) {
-                es = slot_of[aid];
-                ch = dye[es];
-                tau = (ch >= 0 && ch < pp->n_fluoro) ? (Real)pp->tau[ch] : (Real)o.tau;
-            }
-            const U4 ra = rng.draw(step, key, (uint32_t)k, kTagArrival);
-            if (Uni<Real>::second(ra) >= qe) continue;  // detection efficiency
-            Real ua = Uni<Real>::first(ra);
-            if (ua < (Real)1e-30) ua = (Real)1e-30;
-            const Real ta = -tau * log(ua);
-            const U4 rp = rng.draw(step, key, (uint32_t)k, kTagPixel);
-            int px_, py_;
-            if (!psf_pick_sep<Real>(ip, x[es], y[es], ch, Uni<Real>::first(rp), px_, py_))
-                continue;
-            Real z0, z1;
-            normal2<Real>(rng.draw(step, key, (uint32_t)k, kTagJitter), z0, z1);
-            tcspc_add(ip, flim, ch, px_, py_, (double)(ta + sirf * z0));
-            ++detected;
+        E.n = o.nd + o.nf;
+        E.nd = o.nd;
+        E.ch_d = o.ty;
+        E.tau_d = (Real)o.tau;
+        if (o.nf > 0) {
+            E.es_a = slot_of[aid];
+            E.ch_a = dye[E.es_a];
+            E.tau_a = (E.ch_a >= 0 && E.ch_a < pp->n_fluoro) ? (Real)pp->tau[E.ch_a] : (Real)o.tau;
         }
     }
     fl[t] = f;
     ctmc[t] = c;
+    return E;
 }
+
+// Arrival time, detection, PSF pixel, IRF jitter and TCSPC bin of every
+// photon of the wavefront's emitters, spread over its lanes (few emitters
+// emit in a step, so walking each emitter's photons in its own lane leaves
+// most lanes idle).  Photon k of an emitter uses the draws keyed by
+// (step, emitter key, k) whichever lane takes it.  Every lane must call it;
+// ip, flim, rng and step are the same for the whole wavefront.
+template <typename Real, int kWin>
+__device__ __forceinline__ void wave_photons(const Emit<Real>& E, const edm_image_params* ip,
+                                             const Real* x, const Real* y, uint32_t* flim,
+                                             Rng rng, uint32_t step, int& detected) {
+    const int lane = lane_id();
+    int incl = E.n;  // photons of lanes 0 .. lane
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o);
+        if (lane >= o) incl += v;
+    }
+    const int total = __shfl(incl, 63);
+    const Real qe = (Real)ip->qe;
+    const Real sirf = (Real)ip->sigma_irf;
+    for (int p0 = 0; p0 < total; p0 += 64) {
+        const int p = p0 + lane;
+        // the emitter lane of photon p: lanes whose photons all come before p
+        int lo = 0;
+#pragma unroll
+        for (int s = 32; s > 0; s >>= 1) {
+            const int v = __shfl(incl, lo + s - 1);
+            if (v <= p) lo += s;
+        }
+        const int ow = min(lo, 63);
+        const int k = p - (__shfl(incl, ow) - __shfl(E.n, ow));
+        const uint32_t key = (uint32_t)__shfl((int)E.key, ow);
+        const int nd = __shfl(E.nd, ow);
+        const int es_d = __shfl(E.es_d, ow), ch_d = __shfl(E.ch_d, ow);
+        const int es_a = __shfl(E.es_a, ow), ch_a = __shfl(E.ch_a, ow);
+        const Real tau_d = __shfl(E.tau_d, ow), tau_a = __shfl(E.tau_a, ow);
+        if (p < total) {
+            const bool donor = k < nd;
+            const int es = donor ? es_d : es_a, ch = donor ? ch_d : ch_a;
+            const Real tau = donor ? tau_d : tau_a;
+            const U4 ra = rng.draw(step, key, (uint32_t)k, kTagArrival);
+            if (Uni<Real>::second(ra) < qe) {  // detection efficiency
+                Real ua = Uni<Real>::first(ra);
+                if (ua < (Real)1e-30) ua = (Real)1e-30;
+                const Real ta = -tau * log(ua);
+                const U4 rp = rng.draw(step, key, (uint32_t)k, kTagPixel);
+                int px_, py_;
+                if (psf_pick_sep<Real, kWin>(ip, x[es], y[es], ch, Uni<Real>::first(rp), px_,
+                                             py_)) {
+                    Real z0, z1;
+                    normal2<Real>(rng.draw(step, key, (uint32_t)k, kTagJitter), z0, z1);
+                    tcspc_add(ip, flim, ch, px_, py_, (double)(ta + sirf * z0));
+                    ++detected;
+                }
+            }
+        }
+    }
+}
+
+// PSF window capacity of a launch: 8 pixels when 2R + 1 <= 8, else 31.
+static inline bool psf_small(int R) { return 2 * R + 1 <= 8; }
 
 __device__ __forceinline__ void photon_count(int emitted, int detected, unsigned long long* ctr) {
     unsigned long long e = emitted, d = detected;
@@ -465,8 +521,8 @@ __device__ __forceinline__ void photon_count(int emitted, int detected, unsigned
 }
 
 // Single membrane: one thread per dye carrier.
-template <typename Real>
-__global__ void k_pss_image(int n_dye, const int32_t* dye_ids, WlView V, const int32_t* id,
+template <typename Real, int kWin>
+__global__ __launch_bounds__(256) void k_pss_image(int n_dye, const int32_t* dye_ids, WlView V, const int32_t* id,
                             const edm_photo_params* pp, const edm_image_params* ip, const Real* x,
                             const Real* y, uint8_t* fl, const int8_t* dye, int8_t* ctmc,
                             const int32_t* bond, const double* k_fret, const int32_t* acc,
@@ -474,21 +530,23 @@ __global__ void k_pss_image(int n_dye, const int32_t* dye_ids, WlView V, const i
                             unsigned long long* ctr) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     int emitted = 0, detected = 0;
+    Emit<Real> E{0, 0, 0u, 0, 0, 0, 0, (Real)0, (Real)0};
     if (V.list) {
         // this step's dye work list (slots); FRET from the bond maps
         const int m = wl_slice(V);
         if (m == 0) return;  // uniform: an empty slice
         if ((int)threadIdx.x < m) {
             const int t = wl_slot(V);
-            emit_image<Real>(pp, ip, t, (uint32_t)id[t], x, y, fl, dye, ctmc, bond, slot_of,
-                             false, 0.0, -1, flim, rng, step, emitted, detected);
+            E = emit_pss<Real>(pp, t, (uint32_t)id[t], fl, dye, ctmc, bond, slot_of, false, 0.0,
+                               -1, rng, step, emitted);
         }
     } else if (q < n_dye) {
         const int it = dye_ids[q];
-        emit_image<Real>(pp, ip, slot_of[it], (uint32_t)it, x, y, fl, dye, ctmc, bond, slot_of,
-                         k_fret != nullptr, k_fret ? k_fret[q] : 0.0, acc ? acc[q] : -1, flim,
-                         rng, step, emitted, detected);
+        E = emit_pss<Real>(pp, slot_of[it], (uint32_t)it, fl, dye, ctmc, bond, slot_of,
+                           k_fret != nullptr, k_fret ? k_fret[q] : 0.0, acc ? acc[q] : -1, rng,
+                           step, emitted);
     }
+    wave_photons<Real, kWin>(E, ip, x, y, flim, rng, step, detected);
     photon_count(emitted, detected, ctr);
 }
 
@@ -862,6 +920,17 @@ static const edm_image_params* dev_ip(edm_membrane* m) {
     return (const edm_image_params*)(m->pcx + m->ip.nx + m->ip.ny);
 }
 
+template <typename Real, int kWin>
+static void launch_pss_image(edm_membrane* m, dim3 grid, WlView V, const edm_photo_params* dpp,
+                             const edm_image_params* d_ip, const double* kf, const int32_t* ac,
+                             Rng rng, uint32_t step) {
+    hipLaunchKernelGGL((k_pss_image<Real, kWin>), grid, dim3(kB), 0, m->stream, m->n_dye,
+                       (const int32_t*)m->dye_ids, V, (const int32_t*)m->a.id, dpp, d_ip,
+                       (const Real*)m->a.x, (const Real*)m->a.y, m->a.fl, (const int8_t*)m->a.dye,
+                       m->a.ctmc, (const int32_t*)m->a.bond, kf, ac, (const int32_t*)m->a.slot_of,
+                       m->flim, rng, step, m->d_ctr);
+}
+
 int ph_fused_image(edm_membrane* m, int64_t* n_emitted, int64_t* n_detected) {
     if (!m->has_ip) return EDM_ESTATE;
     if (2 * m->ip.R + 1 > 31) {
@@ -893,22 +962,14 @@ int ph_fused_image(edm_membrane* m, int64_t* n_emitted, int64_t* n_detected) {
     if (!V.list)  // the dye-index path looks carriers up through slot_of
         if (int rc = dyn_ensure_slot_of(m)) return rc;
     const dim3 grid = V.list ? dim3(m->wl_nblk, kWlPer) : dim3(nb(m->n_dye));
-    if (m->fp64)
-        hipLaunchKernelGGL(k_pss_image<double>, grid, dim3(kB), 0, st, m->n_dye,
-                           (const int32_t*)m->dye_ids, V, (const int32_t*)m->a.id, dpp, d_ip,
-                           (const double*)m->a.x,
-                           (const double*)m->a.y,
-                           m->a.fl, (const int8_t*)m->a.dye, m->a.ctmc,
-                           (const int32_t*)m->a.bond, kf, ac, (const int32_t*)m->a.slot_of,
-                           m->flim, rng, step, m->d_ctr);
-    else
-        hipLaunchKernelGGL(k_pss_image<float>, grid, dim3(kB), 0, st, m->n_dye,
-                           (const int32_t*)m->dye_ids, V, (const int32_t*)m->a.id, dpp, d_ip,
-                           (const float*)m->a.x,
-                           (const float*)m->a.y,
-                           m->a.fl, (const int8_t*)m->a.dye, m->a.ctmc,
-                           (const int32_t*)m->a.bond, kf, ac, (const int32_t*)m->a.slot_of,
-                           m->flim, rng, step, m->d_ctr);
+    const bool small = psf_small(m->ip.R);
+    if (m->fp64) {
+        if (small) launch_pss_image<double, 8>(m, grid, V, dpp, d_ip, kf, ac, rng, step);
+        else launch_pss_image<double, 31>(m, grid, V, dpp, d_ip, kf, ac, rng, step);
+    } else {
+        if (small) launch_pss_image<float, 8>(m, grid, V, dpp, d_ip, kf, ac, rng, step);
+        else launch_pss_image<float, 31>(m, grid, V, dpp, d_ip, kf, ac, rng, step);
+    }
     EDM_HIP(hipGetLastError());
     if (n_emitted || n_detected) {
         unsigned long long c[2];
