@@ -76,6 +76,9 @@ int edm_create(const edm_params* p, edm_membrane** out) {
             return invalid("edm_create: catalysis needs enzyme distinct from substrate and product");
         if (r.kind < EDM_RX_BIND || r.kind > EDM_RX_ADAPT)
             return invalid("edm_create: unknown reaction kind");
+        if (r.kind == EDM_RX_RECRUIT && r.use_adapt &&
+            (r.fb1.sensor < 0 || r.fb2.sensor >= 0 || !(r.contact >= 0.0)))
+            return invalid("edm_create: local recruitment needs one feedback rule and sigma >= 0");
     }
     if (p->adapt_grid >= p->n_grids) return invalid("edm_create: bad adaptation grid");
     edm_membrane* m = new edm_membrane();
@@ -122,7 +125,7 @@ int edm_destroy(edm_membrane* m) {
                     m->cell_start, m->perm, m->cub_tmp, m->sig, m->eps, m->kh, m->Dbase, m->rcx,
                     m->rcy, m->rr, m->rf, m->d_count_off, m->d_grid_dims, m->d_grid_s, m->counts,
                     m->delta, m->pair_cnt, m->pair_off, m->d_ctr, m->d_draws, m->edge_key,
-                    m->edge_a, m->edge_b, m->n_edges, m->best, m->cand_id, m->cand_slot,
+                    m->edge_a, m->edge_b, m->n_edges, m->best, m->cand_id, m->cand_slot, m->rcdf,
                     m->cand_x, m->cand_y, m->n_cand, m->w_adapt, m->d_pp, m->k_fret, m->acc,
                     m->ph_nd, m->ph_nf, m->ph_cnt, m->ph_off, m->ph_pid, m->ph_t, m->ph_ch,
                     m->ph_fret, m->flim, m->pcx, m->det_flag, m->det_pos, m->sv_key, m->sv_key2,

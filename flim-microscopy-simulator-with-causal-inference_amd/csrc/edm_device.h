@@ -56,6 +56,8 @@ enum Tag : uint32_t {
     kTagJitter = 14,
     kTagDark = 15,
     kTagAfterpulse = 16,
+    kTagRecruitLoc = 17,
+    kTagRecruitOff = 18,
 };
 
 struct Rng {

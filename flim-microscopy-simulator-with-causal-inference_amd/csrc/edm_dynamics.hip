@@ -1021,6 +1021,45 @@ __global__ void k_zero_u32(int n, unsigned int* p) {
 
 #include "edm_cat_tile.h"
 
+// CDF of density-biased recruitment (dynamics.py:2355-2370): over the grid
+// cells in index order, the running sum of max(density, 0.01) divided by the
+// total.  The sum is sequential, in the reference's order, so the CDF is the
+// reference's to the last bit: the floored densities are written in
+// parallel, one wavefront accumulates them (the running total is uniform
+// across its lanes; lane j keeps the value after cell j of each chunk of
+// 64), then every thread divides.  (The reference's uniform fallback for a
+// zero total cannot happen: every cell adds at least 0.01.)  One
+// workgroup of 1024 threads.
+template <typename Real>
+__global__ __launch_bounds__(1024) void k_recruit_cdf(const int32_t* __restrict__ counts, int nc,
+                                                      Real inv, double* cdf) {
+    __shared__ double s_total;
+    for (int k = threadIdx.x; k < nc; k += blockDim.x) {
+        const double d = (double)density_of<Real>(counts[k], inv);
+        cdf[k] = d < 0.01 ? 0.01 : d;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        double total = 0.0;
+        for (int base = 0; base < nc; base += 64) {
+            const int k = base + lane;
+            const double v = k < nc ? cdf[k] : 0.0;
+            const int m = min(64, nc - base);
+            double mine = 0.0;
+            for (int j = 0; j < m; ++j) {
+                total += __shfl(v, j);
+                if (lane == j) mine = total;
+            }
+            if (k < nc) cdf[k] = mine;
+        }
+        if (lane == 0) s_total = total;
+    }
+    __syncthreads();
+    const double total = s_total;
+    for (int k = threadIdx.x; k < nc; k += blockDim.x) cdf[k] = cdf[k] / total;
+}
+
 // Accept the max_recruits candidates with the lowest particle index (the
 // reference stops its index-order scan after max_recruits), one workgroup.
 // Index-order recruitment cap over one membrane's candidates (particle
@@ -1353,6 +1392,11 @@ static int create_t(edm_membrane* m) {
         dalloc(&m->cand_id, n) || dalloc(&m->cand_slot, n) || dalloc(&m->cand_x, n) ||
         dalloc(&m->cand_y, n) || dalloc(&m->n_cand, 4) || dalloc(&m->d_ctr, 8))
         return EDM_EHIP;
+    size_t ncdf = 0;
+    for (const auto& r : m->rx)
+        if (r.kind == EDM_RX_RECRUIT && r.use_adapt)
+            ncdf = std::max(ncdf, (size_t)m->grids[r.fb1.grid].nx * m->grids[r.fb1.grid].ny);
+    if (ncdf && dalloc(&m->rcdf, ncdf)) return EDM_EHIP;
     EDM_HIP(hipMemset(m->d_ctr, 0, 8 * sizeof(unsigned long long)));
     EDM_HIP(hipMemset(m->n_edges, 0, 16));
     EDM_HIP(hipMemset(m->n_cand, 0, 16));
@@ -1361,13 +1405,20 @@ static int create_t(edm_membrane* m) {
 
 int dyn_create(edm_membrane* m) { return m->fp64 ? create_t<double>(m) : create_t<float>(m); }
 
+// Hook draws of one recruitment: u_acc[n] and u_pos[n,2]; the
+// density-biased placement adds u_loc[n] and xi_off[n,2]
+// (dynamics.py:2301-2302, 2366-2367).
+static int64_t recruit_draws(const edm_reaction& r, int64_t n) {
+    return (r.use_adapt ? 6 : 3) * n;
+}
+
 int64_t hook_draws_per_step(const edm_membrane* m, int64_t n_pairs) {
     int64_t total = 2 * (int64_t)m->n;
     for (const auto& r : m->rx) {
         switch (r.kind) {
             case EDM_RX_BIND: total += n_pairs + m->n; break;
             case EDM_RX_CAT: total += n_pairs; break;
-            case EDM_RX_RECRUIT: total += 3 * (int64_t)m->n; break;
+            case EDM_RX_RECRUIT: total += recruit_draws(r, m->n); break;
             case EDM_RX_DETACH:
             case EDM_RX_CONVERT: total += m->n; break;
             default: break;
@@ -1642,7 +1693,7 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
             switch (m->rx[ri].kind) {
                 case EDM_RX_BIND: o += npairs + n; break;
                 case EDM_RX_CAT: o += npairs; break;
-                case EDM_RX_RECRUIT: o += 3 * (int64_t)n; break;
+                case EDM_RX_RECRUIT: o += recruit_draws(m->rx[ri], n); break;
                 case EDM_RX_DETACH:
                 case EDM_RX_CONVERT: o += n; break;
                 default: break;
@@ -1774,6 +1825,17 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
                         defer_any = defer_any || d.deferred;
                         d.reach = pass_reach(m->rx, g, k);
                         grp.species |= d.reach;
+                        d.cdf = nullptr;
+                        d.sigma = 0.0;
+                        if (q.kind == EDM_RX_RECRUIT && q.use_adapt) {
+                            // the CDF of this step's counts on the rule's grid;
+                            // no earlier pass of the run changes them
+                            d.cdf = m->rcdf;
+                            d.sigma = q.contact;
+                            hipLaunchKernelGGL(k_recruit_cdf<Real>, dim3(1), dim3(1024), 0, st,
+                                               (const int32_t*)m->counts + d.f1.cbase,
+                                               d.f1.gnx * d.f1.gny, (Real)d.f1.ginv, m->rcdf);
+                        }
                     }
                     WlView V{nullptr, nullptr};
                     for (const PassPlan& pp : m->pass_plan)
@@ -1783,7 +1845,8 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
                                          &pp.list, &pp.spec, 1);
                             V = wl_view(m, pp.list);
                         }
-                    hipLaunchKernelGGL(k_passes<Real>,
+                    const bool local = grp.p[grp.n - 1].cdf != nullptr;
+                    hipLaunchKernelGGL((local ? k_passes<Real, true> : k_passes<Real, false>),
                                        V.list ? dim3(m->wl_nblk, kWlPer) : dim3(nblk(n)),
                                        dim3(kBlock), 0, st, n, V,
                                        (const int32_t*)m->a.id, P, grp, hook ? draws : nullptr,
@@ -1801,7 +1864,7 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
                         hipLaunchKernelGGL(k_fold, dim3(nblk(m->n_count)), dim3(kBlock), 0, st,
                                            m->n_count, m->counts, m->delta);
                 }
-                off += (r.kind == EDM_RX_RECRUIT) ? 3 * (int64_t)n : (int64_t)n;
+                off += (r.kind == EDM_RX_RECRUIT) ? recruit_draws(r, n) : (int64_t)n;
                 break;
             }
             default: break;

@@ -58,6 +58,9 @@ int edm_ens_create(const edm_params* p, const edm_photo_params* pp, const edm_im
         const edm_reaction& r = p->reactions[i];
         if (r.kind == EDM_RX_BIND || r.kind == EDM_RX_ADAPT)
             return ens_invalid("edm_ens_create: binding reactions are not batched");
+        if (r.kind == EDM_RX_RECRUIT && r.use_adapt)
+            return ens_invalid("edm_ens_create: density-biased (local) recruitment is "
+                               "per-membrane; not batched");
         if (r.kind == EDM_RX_CAT && (r.a == r.b || r.a == r.c))
             return ens_invalid("edm_ens_create: catalysis needs enzyme distinct from substrate "
                                "and product");

@@ -140,6 +140,7 @@ struct edm_membrane {
     uint32_t sensor_mask_all = 0;  // species kept on any grid
 
     // reference pair bookkeeping (parity runs and the faithful pair cap)
+    double* rcdf = nullptr;  // CDF of the density-biased recruitment (largest grid)
     uint64_t* pair_cnt = nullptr;  // by particle index
     uint64_t* pair_off = nullptr;  // exclusive scan of pair_cnt
     int64_t n_pairs_found = 0, n_pairs = 0;

@@ -19,6 +19,9 @@ struct PassDev {
     int deferred;
     uint32_t reach;  // species a particle may hold before the run and still
                      // reach this pass as species a (through earlier conversions)
+    const double* cdf;  // RECRUIT, density-biased placement: the CDF over f1's
+                        // grid cells (nullptr: uniform placement)
+    double sigma;       // its Gaussian offset around the chosen cell centre
 };
 
 static constexpr int kMaxPasses = 8;
@@ -62,10 +65,37 @@ __device__ __forceinline__ void append_candidates(const Cand<Real>& c, int t, in
     }
 }
 
+// Cell chosen by a uniform r on a CDF of nc cells: the first cell whose
+// CDF value is not below r, found by bisection (dynamics.py:1163-1172).
+__device__ __forceinline__ int cdf_cell(const double* __restrict__ cdf, int nc, double r) {
+    int lo = 0, hi = nc - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cdf[mid] < r) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Density-biased recruitment site (dynamics.py:1162-1184): the centre of the
+// cell the CDF draw picks, plus sigma times two standard normals, wrapped
+// into the box.
+template <typename Real>
+__device__ __forceinline__ void local_site(const PassDev& d, double r, Real o0, Real o1, Real Lx,
+                                           Real Ly, Real& px, Real& py) {
+    const int c = cdf_cell(d.cdf, d.f1.gnx * d.f1.gny, r);
+    const int gx = c / d.f1.gny, gy = c - gx * d.f1.gny;
+    const Real cx = ((Real)gx + (Real)0.5) * (Real)d.f1.gsx + o0 * (Real)d.sigma;
+    const Real cy = ((Real)gy + (Real)0.5) * (Real)d.f1.gsy + o1 * (Real)d.sigma;
+    px = py_mod(cx, Lx);
+    py = py_mod(cy, Ly);
+}
+
 // Applies a run of passes to the particle in slot t (species s0, known to
 // be in the run's species set).  Draw keys use the particle index minus
-// kbase; hook draws are indexed by the particle index.
-template <typename Real>
+// kbase; hook draws are indexed by the particle index.  kLocal: the run may
+// end with a density-biased recruitment (its code is compiled only then).
+template <typename Real, bool kLocal = false>
 __device__ __forceinline__ void passes_particle(int t, int s0, int n, const int32_t* __restrict__ id,
                                                 int kbase, const Post<Real>& P,
                                                 const PassGroupDev& grp,
@@ -111,8 +141,26 @@ __device__ __forceinline__ void passes_particle(int t, int s0, int n, const int3
                     ux = Uni<Real>::first(w);
                     uy = Uni<Real>::second(w);
                 }
-                px[k] = ux * Lx;
-                py[k] = uy * Ly;
+                if (kLocal && d.cdf) {
+                    // the uniform positions are drawn but unused here, as
+                    // the reference draws them in this mode too
+                    double rl;
+                    Real o0, o1;
+                    if (draws) {
+                        rl = draws[d.draw_off + 3 * (int64_t)n + it];
+                        o0 = (Real)draws[d.draw_off + 4 * (int64_t)n + 2 * (int64_t)it];
+                        o1 = (Real)draws[d.draw_off + 4 * (int64_t)n + 2 * (int64_t)it + 1];
+                    } else {
+                        const U4 w = rng.draw(step, key, 0u, (kTagRecruitLoc << 16) | d.rxi);
+                        rl = u01d(w.x, w.y);
+                        normal2<Real>(rng.draw(step, key, 0u, (kTagRecruitOff << 16) | d.rxi),
+                                      o0, o1);
+                    }
+                    local_site<Real>(d, rl, o0, o1, Lx, Ly, px[k], py[k]);
+                } else {
+                    px[k] = ux * Lx;
+                    py[k] = uy * Ly;
+                }
                 if (d.f2.sensor >= 0) k2[k] = G.counts[d.f2.cbase + fb_cell(d.f2, px[k], py[k])];
             } else {
                 const uint32_t tag = d.kind == EDM_RX_DETACH ? kTagDetach : kTagConvert;
@@ -184,7 +232,7 @@ __device__ __forceinline__ void passes_count(int det_hits, int conv_hits, unsign
 
 // One listed slot per thread (the step's pass-species work list), or one
 // slot per thread without a list.
-template <typename Real>
+template <typename Real, bool kLocal>
 __global__ __launch_bounds__(256) void k_passes(int n, WlView V, const int32_t* __restrict__ id,
                                                 Post<Real> P, PassGroupDev grp,
                                                 const double* __restrict__ draws, Rng rng,
@@ -206,8 +254,8 @@ __global__ __launch_bounds__(256) void k_passes(int n, WlView V, const int32_t* 
     if (t >= 0) {
         const int s0 = P.sp[t];
         if ((grp.species >> s0) & 1u)
-            passes_particle(t, s0, n, id, 0, P, grp, draws, rng, step, Lx, Ly, G, n_grids, cand,
-                            det_hits, conv_hits);
+            passes_particle<Real, kLocal>(t, s0, n, id, 0, P, grp, draws, rng, step, Lx, Ly, G,
+                                          n_grids, cand, det_hits, conv_hits);
     }
     append_candidates(cand, t, cid, cslot, cxo, cyo, nc);
     passes_count<Real>(det_hits, conv_hits, ctr);

@@ -196,16 +196,20 @@ class _Plan:
             fb = feedback(c.feedback_rule)
             adapt = 1 if (self.adapt_grid >= 0 and "PI3K" in c.enzyme and fb.sensor >= 0) else 0
             rx(_hip.EDM_RX_CAT, e, s, p, contact=s_eff, prob=pc, adapt=adapt, fb1=fb, label="cat")
+        # density-biased recruitment (dynamics.py:2034-2041, 2116-2119, 2354):
+        # an exchange whose rule name holds 'pip3_activation' and that has
+        # exactly one feedback rule, when the ad-hoc config flag is set
         use_local = bool(getattr(config, "use_local_pi3k_recruitment", False))
+        sigma = float(getattr(config, "recruit_sigma", 0.5))
         for ex in config.compartment_exchanges:
             sid = name_to_id[ex.species]
-            if use_local and ex.feedback_rule and "pip3_activation" in ex.feedback_rule:
-                raise NotImplementedError(
-                    "density-biased (local) recruitment is not implemented on the GPU path")
             fb1 = feedback(ex.feedback_rule)
             fb2 = feedback(ex.secondary_feedback_rule) if fb1.sensor >= 0 else _hip.no_feedback()
+            local = (use_local and bool(ex.feedback_rule) and "pip3_activation" in ex.feedback_rule
+                     and fb1.sensor >= 0 and fb2.sensor < 0)
             rx(_hip.EDM_RX_RECRUIT, sid, prob=min(1.0, ex.k_on * dt), maxr=ex.max_recruits_per_step,
-               fb1=fb1, fb2=fb2, label="recruit")
+               fb1=fb1, fb2=fb2, adapt=1 if local else 0, contact=sigma if local else 0.0,
+               label="recruit_local" if local else "recruit")
             rx(_hip.EDM_RX_DETACH, sid, prob=min(1.0, ex.k_off * dt),
                fb1=feedback(ex.detachment_feedback_rule), label="detach")
         for cv in config.unimolecular_conversions:
@@ -367,9 +371,12 @@ class DynamicsEngine:
                 parts.append(g.random(N))
             elif kind == "cat":
                 parts.append(g.random(n_pairs))
-            elif kind == "recruit":
+            elif kind in ("recruit", "recruit_local"):
                 parts.append(g.random(N))
                 parts.append(g.random((N, 2)).ravel())
+                if kind == "recruit_local":
+                    parts.append(g.random(N))
+                    parts.append(g.standard_normal((N, 2)).ravel())
             elif kind in ("detach", "convert"):
                 parts.append(g.random(N))
         return np.ascontiguousarray(np.concatenate(parts))

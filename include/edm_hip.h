@@ -64,12 +64,20 @@ typedef struct edm_reaction {
     int32_t a, b, c;       /* BIND: ra, rb, product; CAT: enzyme, substrate,
                               product; RECRUIT/DETACH: species; CONVERT:
                               from, to                                      */
-    double contact;        /* BIND/CAT: effective contact radius (um)       */
+    double contact;        /* BIND/CAT: effective contact radius (um);
+                              RECRUIT with use_adapt = 1: recruit_sigma (um),
+                              the Gaussian offset around the chosen cell     */
     double prob;           /* per-step probability: P_bind, P_cat, P_on,
                               P_off, P_convert                              */
     double prob2;          /* BIND: P_dissoc                                */
     int32_t max_recruits;  /* RECRUIT cap per step (index order)            */
-    int32_t use_adapt;     /* CAT: multiply the adaptation factor            */
+    int32_t use_adapt;     /* CAT: multiply the adaptation factor; RECRUIT:
+                              1 = density-biased (local) placement, the
+                              cell drawn from the CDF of max(rho, 0.01) on
+                              fb1's grid (dynamics.py:1116-1217, 2355-2370;
+                              needs fb1 and no fb2). Hook draws of such a
+                              reaction: u_acc[n], u_pos[n,2], u_loc[n],
+                              xi_off[n,2]                                   */
     edm_feedback fb1, fb2; /* RECRUIT may use both (dual feedback)          */
 } edm_reaction;
 

@@ -18,7 +18,7 @@ from typing import Optional, Tuple
 
 import numpy as np
 
-from reference_model import D, I, LL, P, lib
+from reference_model import D, I, LL, P, build_recruit_cdf, lib
 
 __all__ = [
     "_min_image_dist_sq", "py_mod", "CellList", "bd_step",
@@ -366,6 +366,28 @@ def apply_membrane_recruitment_with_dual_feedback(species_id, is_alive, position
     nx, ny, sx, sy, K, n, inhibition, basal)."""
     return _recruit(species_id, is_alive, positions, N, target_species_id, prob_recruit_base,
                     Lx, Ly, rng_accept, rng_positions, max_recruits, fb1, fb2)
+
+
+def apply_membrane_recruitment_local(species_id, is_alive, positions, N, target_species_id,
+                                     prob_recruit_base, Lx, Ly, rng_accept, rng_offsets,
+                                     max_recruits, density_field, nx_grid, ny_grid, cell_sx,
+                                     cell_sy, K_half, n_hill, is_inhibition, basal_fraction,
+                                     cdf, rng_location, recruit_sigma) -> int:
+    """dynamics.py:1116-1217: the candidate site is the centre of the cell
+    the CDF bisection picks plus sigma * offsets, wrapped; Hill feedback is
+    read at that site; index-order cap."""
+    al = _Bytes(is_alive)
+    pos = _InPlace(positions, np.float64)
+    f = lib().orc_recruit_local
+    f.restype = I
+    res = f(P(_i32(species_id)), P(al.b), P(pos.b), I(N), I(target_species_id),
+            D(prob_recruit_base), D(Lx), D(Ly), P(_f64(rng_accept)), P(_f64(rng_offsets)),
+            I(max_recruits), P(_f64(density_field)), I(nx_grid), I(ny_grid), D(cell_sx),
+            D(cell_sy), D(K_half), D(n_hill), I(int(bool(is_inhibition))), D(basal_fraction),
+            P(_f64(cdf)), P(_f64(rng_location)), D(recruit_sigma))
+    al.back()
+    pos.back()
+    return res
 
 
 def _detach(species_id, is_alive, positions, N, target, prob, rng_uniform, fb=_NOFB) -> int:

@@ -46,6 +46,27 @@ _i8p = ctypes.POINTER(ctypes.c_int8)
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 
 
+def build_recruit_cdf(density_field, nx_grid, ny_grid) -> np.ndarray:
+    """dynamics.py:2355-2370: running sum of max(density, 0.01) over the
+    cells in index order (gx major), divided by the total; the uniform CDF
+    when the total is zero."""
+    dens = np.asarray(density_field, dtype=np.float64).reshape(-1)
+    nc = nx_grid * ny_grid
+    cdf = np.empty(nc)
+    total = 0.0
+    for k in range(nc):
+        total += max(float(dens[k]), 0.01)
+        cdf[k] = total
+    if total > 0:
+        for k in range(nc):
+            cdf[k] /= total
+    else:
+        for k in range(nc):
+            cdf[k] = (k + 1.0) / nc
+    return cdf
+
+
+
 def lib():
     """Load (building first if needed) the oracle shared library."""
     global _lib
@@ -448,15 +469,7 @@ class OracleDynamics:
             elif fb is not None:
                 self._density(fb)
                 if self.use_local and ex['local']:
-                    dens = fb['density']
-                    cdf = np.empty(fb['nxg'] * fb['nyg'])
-                    total = 0.0
-                    for k in range(fb['nxg'] * fb['nyg']):
-                        total += max(dens[k], 0.01)
-                        cdf[k] = total
-                    if total > 0:
-                        for k in range(cdf.size):
-                            cdf[k] /= total
+                    cdf = build_recruit_cdf(fb['density'], fb['nxg'], fb['nyg'])
                     u_loc = self.rng.random(N)
                     off = self.rng.standard_normal((N, 2))
                     nr = L.orc_recruit_local(P(sp), P(alive), P(pos), I(N), I(ex['sid']),

@@ -149,6 +149,78 @@ def test_binding_fret_full_path_fp64(oracle_lib, variant):
           f"bindings {eng.n_bindings}, counts {fr.total_counts}")
 
 
+def _local_config(sigma, k_on=150.0):
+    """The plain membrane with density-biased PI3K recruitment
+    (use_local_pi3k_recruitment; the PI3K exchange has the single rule
+    pip3_activation) and a recruitment rate high enough that every step
+    recruits."""
+    cfg = workloads.plain_membrane()
+    cfg.use_local_pi3k_recruitment = True
+    cfg.recruit_sigma = sigma
+    for ex in cfg.compartment_exchanges:
+        if ex.species == "PI3K":
+            ex.k_on = k_on
+            ex.k_off = 20.0
+    return cfg
+
+
+@pytest.mark.parametrize("sigma", [0.5, 0.02])
+def test_local_recruitment_fp64(oracle_lib, sigma):
+    """Density-biased recruitment (dynamics.py:1116-1217, CDF :2355-2370)
+    against the oracle with injected draws: the same particles come alive
+    at the same sites (positions within 1e-9 relative), the same counts."""
+    cfg = _local_config(sigma)
+    st, eng, ost, odyn = _pair(cfg, exact=False)
+    assert "recruit_local" in eng.plan.kinds
+    for k in range(40):
+        eng.step()
+        odyn.step()
+        if (k + 1) % 10 == 0:
+            compare_states(host_arrays(st), oracle_arrays(ost), label=f"step {k + 1}: ")
+            oc = odyn.counters()
+            assert (eng.n_recruited, eng.n_detached, eng.n_catalytic, eng.n_converted) == (
+                oc["n_recruited"], oc["n_detached"], oc["n_catalytic"], oc["n_converted"])
+    assert eng.n_recruited >= 40
+    print(f"local recruitment sigma={sigma}: recruited {eng.n_recruited}")
+
+
+def _recruit_sites(local, steps=30):
+    """Philox run with every PIP3 moved into x < L/4 first: the x of every
+    PI3K that comes alive, at the step it does (recruitment is the step's
+    last move of a particle)."""
+    cfg = _local_config(0.1)
+    cfg.use_local_pi3k_recruitment = local
+    st = ParticleState(cfg)
+    names = [s.name for s in cfg.species]
+    pip3 = st.species_id == names.index("PIP3")
+    Lx = cfg.membrane.size[0]
+    st.positions[pip3, 0] *= 0.25
+    eng = DynamicsEngine(cfg, st, precision="fp64", rng="philox")
+    pi3k = st.species_id == names.index("PI3K")
+    alive = st.is_alive.copy()
+    xs = []
+    for _ in range(steps):
+        eng.step()
+        now = st.is_alive.copy()
+        new = now & ~alive & (st.species_id == names.index("PI3K"))
+        xs.extend(st.positions[new, 0].tolist())
+        alive = now
+    assert pi3k.sum() > 0
+    return np.array(xs), Lx
+
+
+def test_local_recruitment_philox_follows_the_sensor():
+    """With the sensor (PIP3) confined to x < L/4, density-biased recruits
+    land there (within a few sigma); uniform recruits spread over the box."""
+    xs, L = _recruit_sites(True)
+    band = (xs < 0.25 * L + 0.5) | (xs > L - 0.5)
+    assert xs.size >= 30 and band.mean() > 0.9, (xs.size, band.mean())
+    xu, _ = _recruit_sites(False)
+    bu = (xu < 0.25 * L + 0.5) | (xu > L - 0.5)
+    assert xu.size >= 30 and bu.mean() < 0.6, (xu.size, bu.mean())
+    print(f"philox local: {band.mean():.3f} of {xs.size} in the band; uniform {bu.mean():.3f}")
+
+
 def test_fp32_philox_large_runs():
     cfg = workloads.large_membrane(side=20.0, n_alive=40_000, pixels=128)
     st = ParticleState(cfg)

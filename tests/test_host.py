@@ -19,7 +19,8 @@ import pytest
 import reference_model as rm
 from ed_mrdt import workloads
 from ed_mrdt.config import ConfigError, config_from_dict
-from ed_mrdt.dynamics import binding_probability, derive_constants
+from ed_mrdt import _hip
+from ed_mrdt.dynamics import _Plan, binding_probability, derive_constants
 from ed_mrdt.microscope import PSF, ConfocalScanner, TCSPCAccumulator
 from ed_mrdt.particles import ParticleState
 from ed_mrdt.photophysics import RateMatrix, build_pss_matrices, fret_efficiency, fret_rate
@@ -298,3 +299,26 @@ def test_frame_noise_restatement_limits():
             k = binomial_inv(n, p, u)
             assert stats.binom.cdf(k, n, p) >= u - 1e-9
             assert k == 0 or stats.binom.cdf(k - 1, n, p) < u + 1e-9
+
+
+def test_local_recruitment_plan():
+    """dynamics.py:2034-2041, 2116-2119, 2354: with use_local_pi3k_recruitment
+    only the exchange whose single rule is pip3_activation (PI3K) is planned
+    as density-biased, carrying recruit_sigma; PTEN_T1 (two rules) and
+    PTEN_T2 (none) keep their placement; without the flag nothing changes."""
+    cfg = workloads.plain_membrane()
+    base = _Plan(cfg, derive_constants(cfg))
+    cfg.use_local_pi3k_recruitment = True
+    cfg.recruit_sigma = 0.25
+    plan = _Plan(cfg, derive_constants(cfg))
+    rec = [(r, k) for r, k in zip(plan.reactions, plan.kinds) if k.startswith("recruit")]
+    names = [s.name for s in cfg.species]
+    assert [(names[r.a], k) for r, k in rec] == [
+        ("PI3K", "recruit_local"), ("PTEN_T1", "recruit"), ("PTEN_T2", "recruit")]
+    r = rec[0][0]
+    assert r.kind == _hip.EDM_RX_RECRUIT and r.use_adapt == 1 and r.contact == 0.25
+    assert r.fb1.sensor == names.index("PIP3") and r.fb2.sensor < 0
+    assert all(r2.use_adapt == 0 for r2, _ in rec[1:])
+    assert base.kinds.count("recruit_local") == 0
+    assert [k for k in base.kinds] == [k.replace("recruit_local", "recruit") for k in plan.kinds]
+

@@ -31,10 +31,12 @@ from kernels import (CellList, _compute_fret_rates_numba, _compute_fret_rates_pr
                      apply_diffusion_field_regions_smooth, apply_diffusion_field_sinusoidal,
                      apply_dissociation, apply_membrane_detachment,
                      apply_membrane_detachment_with_feedback, apply_membrane_recruitment,
+                     apply_membrane_recruitment_local,
                      apply_membrane_recruitment_with_dual_feedback,
                      apply_membrane_recruitment_with_feedback, apply_species_conversion,
                      apply_species_conversion_with_feedback, assign_emitter_centric, bd_step,
-                     compute_density_field, compute_forces_harmonic, compute_forces_wca,
+                     build_recruit_cdf, compute_density_field, compute_forces_harmonic,
+                     compute_forces_wca,
                      hill_factor, py_mod, update_adaptation_field)
 
 
@@ -200,6 +202,67 @@ def test_recruitment_feedback_at_candidate_site():
                                                       1.0, 1.0, np.full(N, 0.5), rpos, 10, fb1,
                                                       fb2)
     assert n == 1 and alive.tolist() == [False, False, False, True]
+
+
+def test_recruit_cdf_floor_and_cell_order():
+    """dynamics.py:2355-2370 on a 3 x 5 grid (gx major): a zero field gives
+    the uniform CDF; one hot cell puts the whole jump at its flat index
+    gx * 5 + gy; densities below 0.01 count as 0.01 and only those."""
+    cdf = build_recruit_cdf(np.zeros((3, 5)), 3, 5)
+    np.testing.assert_allclose(cdf, np.arange(1, 16) / 15.0, rtol=1e-14)
+    assert cdf[-1] == 1.0
+    d = np.zeros((3, 5))
+    d[2, 1] = 100.0
+    cdf = build_recruit_cdf(d, 3, 5)
+    total = 100.0 + 14 * 0.01
+    np.testing.assert_allclose(cdf[10], 11 * 0.01 / total, rtol=1e-12)
+    np.testing.assert_allclose(cdf[11], (11 * 0.01 + 100.0) / total, rtol=1e-12)
+    assert np.searchsorted(cdf, 0.5) == 11
+    d = np.array([[0.005, 0.02, 0.0], [3.0, 0.0099, 0.5]])
+    cdf = build_recruit_cdf(d, 2, 3)
+    w = np.array([0.01, 0.02, 0.01, 3.0, 0.01, 0.5])
+    np.testing.assert_allclose(cdf, np.cumsum(w) / w.sum(), rtol=1e-13)
+
+
+def test_local_recruitment_site_offsets_wrap_and_feedback():
+    """dynamics.py:1116-1217 on a 3 x 5 grid of 1 um cells over a 3 x 5 um
+    box, one hot cell (gx=2, gy=1), activation Hill with no basal rate, so a
+    candidate is accepted (u = 0.5) only on the hot cell.  The chosen cell is
+    the first with CDF >= u_loc (numpy's searchsorted); the site is its
+    centre + sigma * (xi_x, xi_y), wrapped; Hill is read at the site."""
+    d = np.zeros((3, 5))
+    d[2, 1] = 100.0
+    cdf = build_recruit_cdf(d, 3, 5)
+    hot = 0.5 * (cdf[10] + cdf[11])
+    # slot 0 alive, slot 1 another species, slots 2.. dormant targets
+    sp = np.array([0, 1, 0, 0, 0, 0, 0], dtype=np.int32)
+    alive = np.array([1, 0, 0, 0, 0, 0, 0], dtype=bool)
+    u_loc = np.array([hot, hot, hot, hot, 0.0, 1e-9, cdf[12] + 1e-9])
+    # sigma = 0.4: xi = 2.5 moves the site 1 um
+    off = np.array([[0, 0], [0, 0], [0.0, 0.0], [2.5, 0.0], [-2.5, 2.5], [2.5, -2.5],
+                    [-2.5, 0.0]])
+    pos = np.full((7, 2), -1.0)
+    n = apply_membrane_recruitment_local(sp, alive, pos, 7, 0, 1.0, 3.0, 5.0, np.full(7, 0.5),
+                                         off, 10, d, 3, 5, 1.0, 1.0, 10.0, 2.0, False, 0.0, cdf,
+                                         u_loc, 0.4)
+    # slot 2: hot cell, no offset -> (2.5, 1.5)
+    # slot 3: hot cell, +1 um in x -> 3.5 wraps to 0.5: cold, refused
+    # slot 4: cell 0 (0.5, 0.5) moved (-1, +1) -> (2.5, 1.5) after the wrap: hot
+    # slot 5: cell 0 moved (+1, -1) -> (1.5, 4.5): cold (the transposed offsets
+    #         would have landed on the hot cell)
+    # slot 6: cell 13 = (2, 3) (2.5, 3.5) moved -1 um in x -> (1.5, 3.5): cold
+    assert n == 2
+    assert alive.tolist() == [True, False, True, False, True, False, False]
+    np.testing.assert_allclose(pos[2], [2.5, 1.5], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(pos[4], [2.5, 1.5], rtol=0, atol=1e-12)
+    assert (pos[[1, 3, 5, 6]] == -1.0).all()
+    assert np.searchsorted(cdf, u_loc[6]) == 13
+    # the index-order cap: with room for one, only slot 2 comes alive
+    alive[:] = [True, False, False, False, False, False, False]
+    n = apply_membrane_recruitment_local(sp, alive, pos, 7, 0, 1.0, 3.0, 5.0, np.full(7, 0.5),
+                                         off, 1, d, 3, 5, 1.0, 1.0, 10.0, 2.0, False, 0.0, cdf,
+                                         u_loc, 0.4)
+    assert n == 1 and alive.tolist() == [True, False, True, False, False, False, False]
 
 
 def test_membrane_detachment_counts():
