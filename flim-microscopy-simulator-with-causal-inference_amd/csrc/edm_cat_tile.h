@@ -363,7 +363,7 @@ __device__ __forceinline__ void cat_strip(const Cells<Real>& C, const Post<Real>
 }
 
 template <typename Real>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStripWaves<Real, 6>))) void k_cat_tile(Cells<Real> C, Post<Real> P, CatArgs<Real> A,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFp32Waves<Real, 6>))) void k_cat_tile(Cells<Real> C, Post<Real> P, CatArgs<Real> A,
                                                   unsigned long long* ctr, Grids<Real> G,
                                                   int n_grids, int ntile_y, int ntiles) {
     // the parameter block is indexed with per-lane values: keep it in LDS

@@ -112,6 +112,15 @@ __device__ __forceinline__ void normal2(const U4& r, Real& z0, Real& z1) {
     }
 }
 
+// Waves per SIMD some FP32 kernels are compiled for (a register cap, the
+// minimum of amdgpu_waves_per_eu): they are bound by memory latency, and the
+// extra waves hide more of it than the few spills the cap causes cost
+// (measured per kernel: the single-membrane force kernel at 7, catalysis and
+// the fused passes at 6, the ensemble's force and catalysis at 7).  FP64
+// instantiations are left uncapped.
+template <typename Real, int kWaves>
+inline constexpr int kFp32Waves = sizeof(Real) == 4 ? kWaves : 1;
+
 // --------------------------------------------------------- reference math
 template <typename Real>
 __device__ __forceinline__ Real py_mod(Real x, Real L) {

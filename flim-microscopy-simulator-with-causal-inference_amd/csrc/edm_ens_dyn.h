@@ -183,7 +183,7 @@ __device__ __forceinline__ Post<Real> ens_post(const EnsPool<Real>& Q) {
 }
 
 template <typename Real, bool kSub>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStripWaves<Real, 7>))) void k_ens_force(const EnsTile* __restrict__ tiles, int ntiles,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFp32Waves<Real, 7>))) void k_ens_force(const EnsTile* __restrict__ tiles, int ntiles,
                                                    const EnsSeg* __restrict__ segs,
                                                    EnsPool<Real> Q, ForceParams<Real> F,
                                                    uint32_t keep_mask, uint32_t gstep) {
@@ -204,7 +204,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStripWaves
 // Catalysis for one run of catalytic reactions; the run's parameter block
 // for each replica (probabilities, feedback grid, seed) is precomputed.
 template <typename Real>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStripWaves<Real, 7>))) void k_ens_cat(const EnsTile* __restrict__ tiles, int ntiles,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFp32Waves<Real, 7>))) void k_ens_cat(const EnsTile* __restrict__ tiles, int ntiles,
                                                  const EnsSeg* __restrict__ segs, EnsPool<Real> Q,
                                                  const CatArgs<Real>* __restrict__ cats,
                                                  int n_runs, int ci, unsigned long long* ctr,

@@ -233,7 +233,7 @@ __device__ __forceinline__ void passes_count(int det_hits, int conv_hits, unsign
 // One listed slot per thread (the step's pass-species work list), or one
 // slot per thread without a list.
 template <typename Real, bool kLocal>
-__global__ __launch_bounds__(256) void k_passes(int n, WlView V, const int32_t* __restrict__ id,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFp32Waves<Real, 6>))) void k_passes(int n, WlView V, const int32_t* __restrict__ id,
                                                 Post<Real> P, PassGroupDev grp,
                                                 const double* __restrict__ draws, Rng rng,
                                                 uint32_t step, Real Lx, Real Ly,

@@ -20,13 +20,6 @@ static constexpr int kMaxOwnCols = 2;
 // staged cells of a strip kernel built for at most kCols own columns
 __host__ __device__ constexpr int strip_cells_max(int kCols) { return (kCols + 2) * (kTileRows + 2); }
 
-// Waves per SIMD the FP32 strip kernels are compiled for (a register cap):
-// they are bound by the latency of their phases, and the extra waves hide
-// more of it than the few spills the cap causes cost (measured per kernel:
-// the single-membrane force kernel at 7 and catalysis at 6, the ensemble's
-// force and catalysis at 7).  FP64 instantiations are left uncapped.
-template <typename Real, int kWaves>
-inline constexpr int kStripWaves = sizeof(Real) == 4 ? kWaves : 1;
 static constexpr int kStripCells = strip_cells_max(kMaxOwnCols);
 // global start of a staged column's first row, middle rows and last row:
 // s_gst[3 * column + 0 / 1 / 2] (the middle rows are one slot range)
@@ -434,7 +427,7 @@ __device__ __forceinline__ void force_strip(const Cells<Real>& C, const ForcePar
 }
 
 template <typename Real, bool kSub>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStripWaves<Real, 7>))) void k_force_tile(Cells<Real> C, ForceParams<Real> F,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFp32Waves<Real, 7>))) void k_force_tile(Cells<Real> C, ForceParams<Real> F,
                                                     Real* __restrict__ xn, Real* __restrict__ yn,
                                                     const double* __restrict__ noise_hook,
                                                     Rng rng, uint32_t step, Grids<Real> G,
