@@ -118,3 +118,15 @@ def test_ensemble_conservation_64_replicas():
         total_cat += ctr["n_catalytic"]
     assert total_cat > 0
     e.close()
+
+
+def test_local_recruitment_is_not_batched():
+    """Density-biased recruitment keeps a per-membrane CDF; the ensemble
+    refuses such a model with an error instead of placing recruits
+    uniformly."""
+    from ed_mrdt._hip import HipError
+    cfg = workloads.ensemble_replica(0)
+    cfg.use_local_pi3k_recruitment = True
+    with pytest.raises(HipError, match="local"):
+        EnsembleEngine(cfg, 100_000, max_replicas=4)
+

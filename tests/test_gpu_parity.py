@@ -184,6 +184,30 @@ def test_local_recruitment_fp64(oracle_lib, sigma):
     print(f"local recruitment sigma={sigma}: recruited {eng.n_recruited}")
 
 
+def test_local_recruitment_rejects_a_bad_reaction():
+    """edm_create refuses a density-biased recruitment without its feedback
+    rule or with a negative sigma."""
+    import ctypes
+    from ed_mrdt import _hip
+    from ed_mrdt.dynamics import _Plan, build_params, derive_constants
+    cfg = _local_config(0.3)
+    c = derive_constants(cfg)
+    plan = _Plan(cfg, c)
+    k = plan.kinds.index("recruit_local")
+    for bad in ("no_feedback", "negative_sigma"):
+        p2 = _Plan(cfg, c)
+        r = p2.reactions[k]
+        if bad == "no_feedback":
+            r.fb1 = _hip.no_feedback()
+        else:
+            r.contact = -1.0
+        built = build_params(cfg, c, p2, ParticleState(cfg).N)
+        h = ctypes.c_void_p()
+        rc = _hip.lib().edm_create(ctypes.byref(built.params), ctypes.byref(h))
+        assert rc == -1 and not h.value, bad
+        assert b"local recruitment" in _hip.lib().edm_last_error()
+
+
 def _recruit_sites(local, steps=30):
     """Philox run with every PIP3 moved into x < L/4 first: the x of every
     PI3K that comes alive, at the step it does (recruitment is the step's
