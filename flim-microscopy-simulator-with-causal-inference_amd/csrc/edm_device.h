@@ -116,8 +116,8 @@ __device__ __forceinline__ void normal2(const U4& r, Real& z0, Real& z1) {
 // minimum of amdgpu_waves_per_eu): they are bound by memory latency, and the
 // extra waves hide more of it than the few spills the cap causes cost
 // (measured per kernel: the single-membrane force kernel at 7, catalysis and
-// the fused passes at 6, the ensemble's force and catalysis at 7).  FP64
-// instantiations are left uncapped.
+// the fused passes at 6, the ensemble's force, catalysis and passes at 7).
+// FP64 instantiations are left uncapped.
 template <typename Real, int kWaves>
 inline constexpr int kFp32Waves = sizeof(Real) == 4 ? kWaves : 1;
 

@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFp32Waves<
 // A run of per-particle passes over the strip's own particles, with the
 // run's parameter block of the strip's replica.
 template <typename Real>
-__global__ __launch_bounds__(256) void k_ens_passes(int n_waves, const int32_t* __restrict__ list,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFp32Waves<Real, 7>))) void k_ens_passes(int n_waves, const int32_t* __restrict__ list,
                                                     const int32_t* __restrict__ wrep,
                                                     const uint32_t* __restrict__ rcnt,
                                                     const EnsSeg* __restrict__ segs,
