@@ -149,6 +149,54 @@ def test_binding_fret_full_path_fp64(oracle_lib, variant):
           f"bindings {eng.n_bindings}, counts {fr.total_counts}")
 
 
+# um after 100 reaction-free steps: (median, 99th percentile, worst) of the
+# distance between the FP32 and FP64 positions of a particle
+# (measured: plain 1.6e-6 / 1.1e-4 / 8.2e-3, binding_regions 5.5e-7 / 1.1e-5 / 6.0e-5)
+FP32_POS_BOUND = {"plain": (1e-5, 1e-3, 5e-2), "binding_regions": (1e-5, 1e-4, 1e-3)}
+
+
+@pytest.mark.parametrize("workload", list(FP32_POS_BOUND))
+def test_fp32_positions_within_bound(oracle_lib, workload):
+    """The FP32 dynamics (forces, Ermak-McCammon step, D(x), boundaries)
+    with the reference's numpy noise injected, against the FP64 oracle, over
+    100 steps with the reactions removed: the median, 99th percentile and
+    worst distance between a particle's FP32 and FP64 positions within
+    FP32_POS_BOUND (um).  With reactions, an event whose probability or
+    contact test sits within FP32 rounding of its threshold can come out the
+    other way and the recruitment cap carries that on; and the dense
+    harmonic membrane (k = 1e7 kT/um^2, displacement-capped steps) is
+    chaotic: its FP32 and FP64 trajectories part by a median 84 nm within
+    100 steps.  Those are compared with the oracle through distributions
+    (test_gpu_statistics.py)."""
+    if workload == "plain":
+        cfg = workloads.plain_membrane()
+    else:
+        cfg = workloads.binding_fret(seed=4, reflective=True)
+    cfg.reactions = []
+    cfg.catalytic_reactions = []
+    cfg.compartment_exchanges = []
+    cfg.unimolecular_conversions = []
+    st = ParticleState(cfg)
+    eng = DynamicsEngine(cfg, st, precision="fp32", rng="numpy")
+    ost = rm.OracleState(cfg)
+    odyn = rm.OracleDynamics(cfg, ost)
+    for _ in range(100):
+        eng.step()
+        odyn.step()
+    g, o = host_arrays(st), oracle_arrays(ost)
+    for k in ("species_id", "is_alive"):
+        np.testing.assert_array_equal(np.asarray(g[k]), np.asarray(o[k]), err_msg=k)
+    L = np.asarray(cfg.membrane.size, dtype=np.float64)
+    d = np.asarray(g["positions"], dtype=np.float64) - o["positions"]
+    d -= L * np.round(d / L)
+    e = np.sqrt(np.sum(d * d, axis=1))
+    p50, p99, worst = (float(v) for v in np.percentile(e, [50, 99, 100]))
+    print(f"fp32 {workload}: position error after 100 steps: median {p50:.3e}, 99th percentile "
+          f"{p99:.3e}, worst {worst:.3e} um")
+    b50, b99, bmax = FP32_POS_BOUND[workload]
+    assert p50 <= b50 and p99 <= b99 and worst <= bmax
+
+
 def _local_config(sigma, k_on=150.0):
     """The plain membrane with density-biased PI3K recruitment
     (use_local_pi3k_recruitment; the PI3K exchange has the single rule
