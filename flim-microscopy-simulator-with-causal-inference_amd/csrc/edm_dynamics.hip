@@ -1029,10 +1029,10 @@ __global__ void k_zero_u32(int n, unsigned int* p) {
 // across its lanes; lane j keeps the value after cell j of each chunk of
 // 64), then every thread divides.  (The reference's uniform fallback for a
 // zero total cannot happen: every cell adds at least 0.01.)  One
-// workgroup of 1024 threads.
+// workgroup of 1024 threads per CDF.
 template <typename Real>
-__global__ __launch_bounds__(1024) void k_recruit_cdf(const int32_t* __restrict__ counts, int nc,
-                                                      Real inv, double* cdf) {
+__device__ __forceinline__ void recruit_cdf_block(const int32_t* __restrict__ counts, int nc,
+                                                  Real inv, double* cdf) {
     __shared__ double s_total;
     for (int k = threadIdx.x; k < nc; k += blockDim.x) {
         const double d = (double)density_of<Real>(counts[k], inv);
@@ -1058,6 +1058,12 @@ __global__ __launch_bounds__(1024) void k_recruit_cdf(const int32_t* __restrict_
     __syncthreads();
     const double total = s_total;
     for (int k = threadIdx.x; k < nc; k += blockDim.x) cdf[k] = cdf[k] / total;
+}
+
+template <typename Real>
+__global__ __launch_bounds__(1024) void k_recruit_cdf(const int32_t* __restrict__ counts, int nc,
+                                                      Real inv, double* cdf) {
+    recruit_cdf_block<Real>(counts, nc, inv, cdf);
 }
 
 // Accept the max_recruits candidates with the lowest particle index (the

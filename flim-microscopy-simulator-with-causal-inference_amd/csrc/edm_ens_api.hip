@@ -26,8 +26,10 @@ static void ens_free(edm_ensemble* e) {
                     e->wl_wrep_pass, e->wl_wrep_dye, e->d_wl_chunks};
     for (void* b : bufs)
         if (b) hipFree(b);
-    for (auto& g : e->seg)
+    for (auto& g : e->seg) {
         if (g.flim) hipFree(g.flim);
+        if (g.rcdf) hipFree(g.rcdf);
+    }
     if (e->stream) hipStreamDestroy(e->stream);
     delete e;
 }
@@ -58,9 +60,10 @@ int edm_ens_create(const edm_params* p, const edm_photo_params* pp, const edm_im
         const edm_reaction& r = p->reactions[i];
         if (r.kind == EDM_RX_BIND || r.kind == EDM_RX_ADAPT)
             return ens_invalid("edm_ens_create: binding reactions are not batched");
-        if (r.kind == EDM_RX_RECRUIT && r.use_adapt)
-            return ens_invalid("edm_ens_create: density-biased (local) recruitment is "
-                               "per-membrane; not batched");
+        if (r.kind == EDM_RX_RECRUIT && r.use_adapt &&
+            (r.fb1.sensor < 0 || r.fb2.sensor >= 0 || !(r.contact >= 0.0)))
+            return ens_invalid("edm_ens_create: local recruitment needs one feedback rule and "
+                               "sigma >= 0");
         if (r.kind == EDM_RX_CAT && (r.a == r.b || r.a == r.c))
             return ens_invalid("edm_ens_create: catalysis needs enzyme distinct from substrate "
                                "and product");
@@ -190,6 +193,15 @@ int edm_ens_add(edm_ensemble* e, const edm_replica* r, const edm_host_state* s, 
         g.prob[q] = r->prob[q];
         g.maxr[q] = r->max_recruits[q];
     }
+    // the CDF of density-biased recruitment, sized by this replica's grid
+    g.rcdf = nullptr;
+    {
+        size_t ncdf = 0;
+        for (const auto& rx : e->rx)
+            if (rx.kind == EDM_RX_RECRUIT && rx.use_adapt)
+                ncdf = std::max(ncdf, (size_t)g.gdims[2 * rx.fb1.grid] * g.gdims[2 * rx.fb1.grid + 1]);
+        if (ncdf) EDM_HIP(hipMalloc(&g.rcdf, ncdf * sizeof(double)));
+    }
     if (imaging) {
         g.ip = e->ip;
         g.ip.nx = r->img_nx;
@@ -268,6 +280,9 @@ int edm_ens_remove(edm_ensemble* e, int32_t slot) {
     EnsSeg& g = e->seg[slot];
     if (g.flim) EDM_HIP(hipFree(g.flim));
     g.flim = nullptr;
+    // the next step's layout uploads this slot's tables without the pointer
+    if (g.rcdf) EDM_HIP(hipFree(g.rcdf));
+    g.rcdf = nullptr;
     g.active = 0;
     e->n_retired += g.n;
     e->draining.push_back(slot);

@@ -222,7 +222,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFp32Waves<
 
 // A run of per-particle passes over the strip's own particles, with the
 // run's parameter block of the strip's replica.
-template <typename Real>
+template <typename Real, bool kLocal>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFp32Waves<Real, 7>))) void k_ens_passes(int n_waves, const int32_t* __restrict__ list,
                                                     const int32_t* __restrict__ wrep,
                                                     const uint32_t* __restrict__ rcnt,
@@ -248,9 +248,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFp32Waves<
         const int s0 = Q.sp[t];
         if ((grp.species >> s0) & 1u) {
             const Grids<Real> G = ens_grids<Real>(g, Q.counts, Q.delta, Q.ns);
-            passes_particle(t, s0, g.n, Q.id, g.kbase, ens_post(Q), grp, (const double*)nullptr,
-                            ens_rng(g), gstep - g.step0, (Real)g.Lx, (Real)g.Ly, G, Q.n_grids,
-                            cand, det, conv);
+            passes_particle<Real, kLocal>(t, s0, g.n, Q.id, g.kbase, ens_post(Q), grp,
+                                          (const double*)nullptr, ens_rng(g), gstep - g.step0,
+                                          (Real)g.Lx, (Real)g.Ly, G, Q.n_grids, cand, det,
+                                          conv);
         }
     }
     // the wave is one replica's: its candidate region is that replica's own
@@ -297,6 +298,21 @@ static FbDev ens_fb_host(const edm_feedback& f, const EnsSeg& g, int ns) {
 }
 
 // Pass group gi of the model with replica g's probabilities and grids.
+// CDFs of a run's density-biased recruitment, one workgroup per replica slot
+// (each from its own counts on its own grid, as k_recruit_cdf does for one
+// membrane); slots without one (free, or no such pass) do nothing.
+template <typename Real>
+__global__ __launch_bounds__(1024) void k_ens_recruit_cdf(const PassGroupDev* __restrict__ pg,
+                                                          int n_groups, int gi,
+                                                          const int32_t* __restrict__ counts) {
+    const PassGroupDev& grp = pg[(size_t)blockIdx.x * n_groups + gi];
+    if (grp.n <= 0) return;  // uniform: a free slot's table is empty
+    const PassDev& d = grp.p[grp.n - 1];
+    if (!d.cdf) return;
+    recruit_cdf_block<Real>(counts + d.f1.cbase, d.f1.gnx * d.f1.gny, (Real)d.f1.ginv,
+                            const_cast<double*>(d.cdf));
+}
+
 static PassGroupDev ens_group_host(const edm_ensemble* e, int gi, const EnsSeg& g) {
     const std::vector<int>& ix = e->pass_groups[gi];
     const int ns = e->p.n_species;
@@ -321,6 +337,10 @@ static PassGroupDev ens_group_host(const edm_ensemble* e, int gi, const EnsSeg& 
         }
         d.reach = reach;
         grp.species |= reach;
+        if (q.kind == EDM_RX_RECRUIT && q.use_adapt && g.rcdf) {
+            d.cdf = g.rcdf;
+            d.sigma = q.contact;
+        }
     }
     return grp;
 }
@@ -729,8 +749,15 @@ static int ens_step_t(edm_ensemble* e) {
             const std::vector<int>& grp_ix = e->pass_groups[gi];
             bool defer_any = false;
             for (int k : grp_ix) defer_any = defer_any || e->pass_deferred[k];
+            const edm_reaction& last_rx = e->rx[grp_ix.back()];
+            const bool local = last_rx.kind == EDM_RX_RECRUIT && last_rx.use_adapt;
+            if (local)
+                hipLaunchKernelGGL((k_ens_recruit_cdf<Real>), dim3(e->rmax), dim3(1024), 0, st,
+                                   (const PassGroupDev*)e->d_pgrp, (int)e->pass_groups.size(), gi,
+                                   (const int32_t*)e->counts);
             if (e->wl_npass > 0)
-                hipLaunchKernelGGL((k_ens_passes<Real>), dim3(nblk(e->wl_npass)), dim3(256), 0,
+                hipLaunchKernelGGL((local ? k_ens_passes<Real, true> : k_ens_passes<Real, false>),
+                                   dim3(nblk(e->wl_npass)), dim3(256), 0,
                                    st, (int)(e->wl_npass / 64), (const int32_t*)e->wl_pass,
                                    (const int32_t*)e->wl_wrep_pass, (const uint32_t*)e->wl_rcnt,
                                    (const EnsSeg*)e->d_seg, Q, (const PassGroupDev*)e->d_pgrp,

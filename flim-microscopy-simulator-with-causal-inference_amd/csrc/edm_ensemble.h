@@ -44,6 +44,7 @@ struct EnsSeg {
     edm_image_params ip;  // the model's imaging constants with this replica's geometry
     uint32_t* flim;       // this replica's TCSPC stack
     int64_t flim_elems;
+    double* rcdf;         // CDF of density-biased recruitment on this replica's grid, or nullptr
     // work lists: this replica's region of the pass list [0] and of the dye
     // list [1] starts at entry wl_off (a multiple of the wavefront size, so
     // that every wave of a list launch belongs to one replica) and holds

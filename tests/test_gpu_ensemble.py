@@ -120,13 +120,37 @@ def test_ensemble_conservation_64_replicas():
     e.close()
 
 
-def test_local_recruitment_is_not_batched():
-    """Density-biased recruitment keeps a per-membrane CDF; the ensemble
-    refuses such a model with an error instead of placing recruits
-    uniformly."""
-    from ed_mrdt._hip import HipError
-    cfg = workloads.ensemble_replica(0)
+def _local_replica(r):
+    cfg = workloads.ensemble_replica(r)
     cfg.use_local_pi3k_recruitment = True
-    with pytest.raises(HipError, match="local"):
-        EnsembleEngine(cfg, 100_000, max_replicas=4)
+    cfg.recruit_sigma = 0.3
+    for ex in cfg.compartment_exchanges:
+        if ex.species == "PI3K":
+            ex.k_on = 150.0
+    return cfg
 
+
+def test_local_recruitment_replicas_match_single_membrane_path():
+    """Density-biased recruitment in the ensemble (a CDF per replica on its
+    own grid): every replica of a two-replica ensemble evolves bit for bit as
+    the same configuration does on the single-membrane path."""
+    steps = 30
+    singles = {}
+    for r in (3, 7):
+        cfg = _local_replica(r)
+        st = ParticleState(cfg)
+        eng = DynamicsEngine(cfg, st, precision="fp32", rng="philox", pair_cap=False)
+        assert "recruit_local" in eng.plan.kinds
+        eng.run(steps)
+        singles[r] = (dict(positions=st.positions.copy(), species_id=st.species_id.copy(),
+                           is_alive=st.is_alive.copy(), is_bleached=st.is_bleached.copy(),
+                           ctmc_state=st.ctmc_state.copy()), eng.n_recruited)
+        assert eng.n_recruited > 0
+        eng.close()
+    ens = EnsembleEngine(_local_replica(3), capacity=200_000, max_replicas=4, image=False)
+    slots = {r: ens.add(_local_replica(r)) for r in (3, 7)}
+    ens.run(steps)
+    for r, k in slots.items():
+        _same(ens.state(k), singles[r][0], f"replica {r}")
+        assert ens.counters(k)["n_recruited"] == singles[r][1]
+    ens.close()
