@@ -310,7 +310,7 @@ __global__ __launch_bounds__(1024) void k_ens_recruit_cdf(const PassGroupDev* __
     const PassDev& d = grp.p[grp.n - 1];
     if (!d.cdf) return;
     recruit_cdf_block<Real>(counts + d.f1.cbase, d.f1.gnx * d.f1.gny, (Real)d.f1.ginv,
-                            const_cast<double*>(d.cdf));
+                            d.cdf);
 }
 
 static PassGroupDev ens_group_host(const edm_ensemble* e, int gi, const EnsSeg& g) {

@@ -19,8 +19,9 @@ struct PassDev {
     int deferred;
     uint32_t reach;  // species a particle may hold before the run and still
                      // reach this pass as species a (through earlier conversions)
-    const double* cdf;  // RECRUIT, density-biased placement: the CDF over f1's
-                        // grid cells (nullptr: uniform placement)
+    double* cdf;  // RECRUIT, density-biased placement: the CDF over f1's grid
+                  // cells, written by the CDF kernel before the run (nullptr:
+                  // uniform placement)
     double sigma;       // its Gaussian offset around the chosen cell centre
 };
 
