@@ -133,7 +133,8 @@ def _local_replica(r):
 def test_local_recruitment_replicas_match_single_membrane_path():
     """Density-biased recruitment in the ensemble (a CDF per replica on its
     own grid): every replica of a two-replica ensemble evolves bit for bit as
-    the same configuration does on the single-membrane path."""
+    the same configuration does on the single-membrane path, also after the
+    other replica (and its CDF) is removed mid-run."""
     steps = 30
     singles = {}
     for r in (3, 7):
@@ -141,7 +142,7 @@ def test_local_recruitment_replicas_match_single_membrane_path():
         st = ParticleState(cfg)
         eng = DynamicsEngine(cfg, st, precision="fp32", rng="philox", pair_cap=False)
         assert "recruit_local" in eng.plan.kinds
-        eng.run(steps)
+        eng.run(steps if r == 7 else 2 * steps)
         singles[r] = (dict(positions=st.positions.copy(), species_id=st.species_id.copy(),
                            is_alive=st.is_alive.copy(), is_bleached=st.is_bleached.copy(),
                            ctmc_state=st.ctmc_state.copy()), eng.n_recruited)
@@ -150,7 +151,10 @@ def test_local_recruitment_replicas_match_single_membrane_path():
     ens = EnsembleEngine(_local_replica(3), capacity=200_000, max_replicas=4, image=False)
     slots = {r: ens.add(_local_replica(r)) for r in (3, 7)}
     ens.run(steps)
-    for r, k in slots.items():
-        _same(ens.state(k), singles[r][0], f"replica {r}")
-        assert ens.counters(k)["n_recruited"] == singles[r][1]
+    _same(ens.state(slots[7]), singles[7][0], "replica 7")
+    assert ens.counters(slots[7])["n_recruited"] == singles[7][1]
+    ens.remove(slots[7])
+    ens.run(steps)
+    _same(ens.state(slots[3]), singles[3][0], "replica 3 after the removal")
+    assert ens.counters(slots[3])["n_recruited"] == singles[3][1]
     ens.close()
