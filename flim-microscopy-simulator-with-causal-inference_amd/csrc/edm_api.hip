@@ -262,7 +262,7 @@ int edm_get_tuning(edm_membrane* m, int32_t key, int64_t* value) {
 int edm_download_state(edm_membrane* m, edm_host_state* s) {
     if (!m || !s) return invalid("edm_download_state: null argument");
     const int n = m->n;
-    EDM_HIP(hipStreamSynchronize(m->stream));
+    if (int rc = dyn_check_overflow(m)) return rc;
     std::vector<double> x(n), y(n);
     if (m->fp64) {
         EDM_HIP(hipMemcpy(x.data(), m->a.x, n * 8, hipMemcpyDeviceToHost));
@@ -325,6 +325,7 @@ int edm_set_imaging(edm_membrane* m, const edm_image_params* ip) {
 int edm_step_begin(edm_membrane* m, int64_t* n_pairs) {
     if (!m) return invalid("edm_step_begin: null handle");
     m->need_pair_offsets = true;
+    if (int rc = dyn_check_overflow(m)) return rc;
     if (int rc = dyn_step_begin(m)) return rc;
     m->begun = true;
     if (n_pairs) *n_pairs = m->n_pairs;
@@ -368,7 +369,7 @@ int edm_counters(edm_membrane* m, int64_t out[6]) {
     if (!m || !out) return invalid("edm_counters: null argument");
     unsigned long long c[8];
     EDM_HIP(hipMemcpyAsync(c, m->d_ctr, sizeof(c), hipMemcpyDeviceToHost, m->stream));
-    EDM_HIP(hipStreamSynchronize(m->stream));
+    if (int rc = dyn_check_overflow(m)) return rc;
     for (int i = 0; i < 6; ++i) out[i] = (int64_t)c[i];
     return 0;
 }
@@ -509,8 +510,7 @@ int edm_frame_reset(edm_membrane* m) {
 
 int edm_synchronize(edm_membrane* m) {
     if (!m) return invalid("edm_synchronize: null handle");
-    EDM_HIP(hipStreamSynchronize(m->stream));
-    return 0;
+    return dyn_check_overflow(m);
 }
 
 int edm_time_steps(edm_membrane* m, int32_t n_steps, int32_t image, double* total_ms,

@@ -69,13 +69,23 @@ __device__ __forceinline__ int cat_try(const Cells<Real>& C, const Post<Real>& P
     if (A.fb.sensor >= 0) {
         const Real hf = fb_factor(G, A.fb, xnu, ynu);
         if (A.w) {
-            int gx = trunc_cell(xnu, (Real)A.fb.gsx, A.fb.gnx);
-            int gy = trunc_cell(ynu, (Real)A.fb.gsy, A.fb.gny);
-            gx = clamp_cell(gx, A.wnx);
-            gy = clamp_cell(gy, A.wny);
             Real af = (Real)1;
             if (A.Kw > (Real)0) {
-                const Real wr = pow((Real)A.w[gx * A.wny + gy] / A.Kw, A.nw);
+                // dynamics.py:1078 reads adaptation_field[gx, gy] with the
+                // feedback grid's indices on the adaptation grid's array
+                // (no bounds check): element gx * wny + gy where that lies in
+                // the array; past it the reference has no defined value and
+                // the field is read at the substrate's own adaptation cell
+                const int gx = trunc_cell(xnu, (Real)A.fb.gsx, A.fb.gnx);
+                const int gy = trunc_cell(ynu, (Real)A.fb.gsy, A.fb.gny);
+                int wc = gx * A.wny + gy;
+                if (wc >= A.wnx * A.wny) {
+                    const double asx = ((double)A.fb.gnx * A.fb.gsx) / A.wnx;
+                    const double asy = ((double)A.fb.gny * A.fb.gsy) / A.wny;
+                    wc = trunc_cell(xnu, (Real)asx, A.wnx) * A.wny +
+                         trunc_cell(ynu, (Real)asy, A.wny);
+                }
+                const Real wr = pow((Real)A.w[wc] / A.Kw, A.nw);
                 af = (Real)1 / ((Real)1 + wr);
             }
             pe = prob * hf * af;

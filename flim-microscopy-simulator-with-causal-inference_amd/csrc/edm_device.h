@@ -138,6 +138,45 @@ __device__ __forceinline__ Real py_mod(Real x, Real L) {
     return r;
 }
 
+// sin and cos of x in FP64 by Cody-Waite reduction modulo pi/2 (pi/2 split
+// into two 33-bit parts and a 53-bit tail, so k * part is exact for
+// |k| < 2^20) and Taylor polynomials on [-pi/4, pi/4] (through r^17 and
+// r^18; the first omitted terms are below 5e-17 relative) in Horner form.
+// Only IEEE add, multiply and rint, so a host build at -ffp-contract=off
+// computes the same bits (the sinusoidal D(x) field of the FP64 parity
+// path).
+__device__ __forceinline__ void det_sincos(double x, double& s, double& c) {
+    const double k = rint(x * 0x1.45f306dc9c883p-1);  // x * 2/pi
+    const double r = ((x - k * 0x1.921fb54400000p+0) - k * 0x1.0b4611a600000p-34) -
+                     k * 0x1.3198a2e037073p-69;
+    const double r2 = r * r;
+    double ps = 0x1.952c77030ad4ap-49;  // 1/17!
+    ps = -0x1.ae7f3e733b81fp-41 + r2 * ps;
+    ps = 0x1.6124613a86d09p-33 + r2 * ps;
+    ps = -0x1.ae64567f544e4p-26 + r2 * ps;
+    ps = 0x1.71de3a556c734p-19 + r2 * ps;
+    ps = -0x1.a01a01a01a01ap-13 + r2 * ps;
+    ps = 0x1.1111111111111p-7 + r2 * ps;
+    ps = -0x1.5555555555555p-3 + r2 * ps;
+    const double sr = r + r * (r2 * ps);
+    double pc = -0x1.6827863b97d97p-53;  // -1/18!
+    pc = 0x1.ae7f3e733b81fp-45 + r2 * pc;
+    pc = -0x1.93974a8c07c9dp-37 + r2 * pc;
+    pc = 0x1.1eed8eff8d898p-29 + r2 * pc;
+    pc = -0x1.27e4fb7789f5cp-22 + r2 * pc;
+    pc = 0x1.a01a01a01a01ap-16 + r2 * pc;
+    pc = -0x1.6c16c16c16c17p-10 + r2 * pc;
+    pc = 0x1.5555555555555p-5 + r2 * pc;
+    pc = -0x1.0000000000000p-1 + r2 * pc;
+    const double cr = 1.0 + r2 * pc;
+    switch (((long long)k) & 3) {
+        case 0: s = sr; c = cr; break;
+        case 1: s = cr; c = -sr; break;
+        case 2: s = -sr; c = -cr; break;
+        default: s = -cr; c = sr; break;
+    }
+}
+
 __device__ __forceinline__ int clamp_cell(int v, int n) {
     if (v >= n) v = n - 1;
     if (v < 0) v = 0;

@@ -605,8 +605,16 @@ __device__ __forceinline__ void dfield_eval(const ForceParams<Real>& F, int s, R
                                             Real& D, Real& gx, Real& gy) {
     const Real base = F.Dbase[s];
     if (F.dfield == 1) {
-        const Real sx = sin(F.sin_kx * x), cx = cos(F.sin_kx * x);
-        const Real sy = sin(F.sin_ky * y), cy = cos(F.sin_ky * y);
+        Real sx, cx, sy, cy;
+        if constexpr (sizeof(Real) == 8) {
+            det_sincos(F.sin_kx * x, sx, cx);
+            det_sincos(F.sin_ky * y, sy, cy);
+        } else {
+            sx = sin(F.sin_kx * x);
+            cx = cos(F.sin_kx * x);
+            sy = sin(F.sin_ky * y);
+            cy = cos(F.sin_ky * y);
+        }
         D = base * ((Real)1 + F.sin_amp * sx * sy);
         gx = base * F.sin_amp * F.sin_kx * cx * sy;
         gy = base * F.sin_amp * F.sin_ky * sx * cy;
@@ -916,11 +924,15 @@ __global__ __launch_bounds__(1024) void k_bind_match(Post<Real> P, const int32_t
                                                      int ra, int rb, int prod,
                                                      const unsigned long long* ekey,
                                                      const int32_t* ea, const int32_t* eb,
-                                                     const unsigned int* ne, unsigned int cap,
+                                                     unsigned int* ne, unsigned int cap,
                                                      unsigned long long* best,
                                                      unsigned long long* ctr, Grids<Real> G,
                                                      int n_grids) {
     const unsigned int m = min(*ne, cap);
+    // more accepted edges than the buffer holds: the matching below would
+    // differ from the reference's sequential pass, so keep the largest such
+    // count in ne[1] for the host to report (EDM_ECAPACITY)
+    if (threadIdx.x == 0 && *ne > cap) atomicMax(&ne[1], *ne);
     volatile int8_t* vsp = P.sp;
     volatile uint8_t* vfl = P.fl;
     volatile int32_t* vbd = P.bond;
@@ -971,7 +983,7 @@ __global__ __launch_bounds__(1024) void k_bind_match(Post<Real> P, const int32_t
         if (!__syncthreads_or(live)) break;
     }
     __syncthreads();
-    if (threadIdx.x == 0) *const_cast<unsigned int*>(ne) = 0u;  // ready for the next pass
+    if (threadIdx.x == 0) *ne = 0u;  // ready for the next pass
 }
 
 // dynamics.py:610-654 apply_dissociation
@@ -1202,6 +1214,23 @@ static int dalloc(T** p, size_t count) {
     return 0;
 }
 
+// Binding-edge buffers of at least `need` edges (hook draws: one per listed
+// pair at most).  Called between steps; the stream is drained first.
+static int ensure_edge_cap(edm_membrane* m, size_t need) {
+    if (need <= m->edge_cap) return 0;
+    EDM_HIP(hipStreamSynchronize(m->stream));
+    EDM_HIP(hipFree(m->edge_key));
+    EDM_HIP(hipFree(m->edge_a));
+    EDM_HIP(hipFree(m->edge_b));
+    m->edge_key = nullptr;
+    m->edge_a = m->edge_b = nullptr;
+    m->edge_cap = need + need / 4;
+    if (dalloc(&m->edge_key, m->edge_cap) || dalloc(&m->edge_a, m->edge_cap) ||
+        dalloc(&m->edge_b, m->edge_cap))
+        return EDM_EHIP;
+    return 0;
+}
+
 template <typename Real>
 static int upload_real(void** dst, const double* src, size_t n) {
     std::vector<Real> tmp(std::max<size_t>(n, 1));
@@ -1392,7 +1421,14 @@ static int create_t(edm_membrane* m) {
         m->rint2 = r2 * (1.0 + 1e-6);
     }
     // reaction scratch
-    m->edge_cap = (size_t)std::max(n, 1024);
+    // accepted binding edges: with hook draws at most one per listed pair
+    // (grown to the pair count before the step); with Philox draws at most
+    // one per A-B pair within the cutoff, so 4 per particle covers a
+    // membrane up to ~8x the 1M workload's density, and a step that accepts
+    // more is reported (ne[1], EDM_ECAPACITY) rather than truncated silently
+    bool has_bind = false;
+    for (const auto& r : m->rx) has_bind = has_bind || r.kind == EDM_RX_BIND;
+    m->edge_cap = has_bind ? (size_t)std::max<int64_t>(4 * (int64_t)n, 4096) : 1024;
     if (dalloc(&m->edge_key, m->edge_cap) || dalloc(&m->edge_a, m->edge_cap) ||
         dalloc(&m->edge_b, m->edge_cap) || dalloc(&m->n_edges, 4) || dalloc(&m->best, n) ||
         dalloc(&m->cand_id, n) || dalloc(&m->cand_slot, n) || dalloc(&m->cand_x, n) ||
@@ -1629,6 +1665,20 @@ static int begin_t(edm_membrane* m) {
 
 int dyn_step_begin(edm_membrane* m) { return m->fp64 ? begin_t<double>(m) : begin_t<float>(m); }
 
+int dyn_check_overflow(edm_membrane* m) {
+    if (!m->n_edges) return 0;
+    unsigned int ne[2];
+    EDM_HIP(hipMemcpyAsync(ne, m->n_edges, sizeof(ne), hipMemcpyDeviceToHost, m->stream));
+    EDM_HIP(hipStreamSynchronize(m->stream));
+    if (ne[1] > 0) {
+        set_error("binding: a step accepted " + std::to_string(ne[1]) +
+                  " candidate edges, more than the edge buffer holds (" +
+                  std::to_string(m->edge_cap) + "); the matching of that step is incomplete");
+        return EDM_ECAPACITY;
+    }
+    return 0;
+}
+
 template <typename Real>
 static int finish_t(edm_membrane* m, const double* draws, bool hook) {
     const int n = m->n;
@@ -1662,6 +1712,12 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
     F.dt = (Real)p.dt;
     F.max_disp = (Real)p.max_disp;
     F.rint2 = (Real)m->rint2;
+    if (hook)
+        for (const auto& r : m->rx)
+            if (r.kind == EDM_RX_BIND) {
+                if (int rc = ensure_edge_cap(m, (size_t)m->n_pairs)) return rc;
+                break;
+            }
     const double* noise = hook ? draws : nullptr;
     if (m->time_force) EDM_HIP(hipEventRecord(m->kev[0], st));
     if (hook && m->fp64 && p.exact_force_order)
@@ -1722,7 +1778,7 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
                                    (const int32_t*)m->a.id, r.a, r.b, r.c,
                                    (const unsigned long long*)m->edge_key,
                                    (const int32_t*)m->edge_a, (const int32_t*)m->edge_b,
-                                   (const unsigned int*)m->n_edges, (unsigned int)m->edge_cap,
+                                   m->n_edges, (unsigned int)m->edge_cap,
                                    m->best, m->d_ctr, G, ng);
                 hipLaunchKernelGGL(k_dissoc<Real>, dim3(nblk(n)), dim3(kBlock), 0, st, n,
                                    (const int32_t*)m->a.id, P, r.c, r.a, r.b, (Real)r.prob2, dr,

@@ -160,6 +160,17 @@ int edm_ens_add(edm_ensemble* e, const edm_replica* r, const edm_host_state* s, 
         if (s->species_id[i] < 0 || s->species_id[i] >= ns)
             return ens_invalid("edm_ens_add: species id out of range");
     EnsSeg g{};
+    // g owns its CDF and TCSPC stack until it is stored in e->seg[k]: any
+    // error return before that frees them
+    struct SegGuard {
+        EnsSeg* g;
+        bool armed;
+        ~SegGuard() {
+            if (!armed) return;
+            if (g->rcdf) hipFree(g->rcdf);
+            if (g->flim) hipFree(g->flim);
+        }
+    } guard{&g, true};
     g.active = 1;
     g.kbase = (int32_t)e->next_id;
     g.n = n;
@@ -255,6 +266,7 @@ int edm_ens_add(edm_ensemble* e, const edm_replica* r, const edm_host_state* s, 
     for (int i = 0; i < n; ++i) n_alive += (fl[i] & EDM_FLAG_ALIVE) ? 1 : 0;
     g.sub_rows = force_sub_rows(n_alive, (int64_t)g.ncx * g.ncy) ? 1 : 0;
     e->seg[k] = g;
+    guard.armed = false;
     e->fresh_beg[k] = beg;
     int64_t npass = 0, ndye = 0;
     for (int i = 0; i < n; ++i) {

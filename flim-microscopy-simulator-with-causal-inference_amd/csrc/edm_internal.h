@@ -257,6 +257,8 @@ namespace edm {
 int dyn_create(edm_membrane* m);
 int dyn_step_begin(edm_membrane* m);
 int dyn_step_finish(edm_membrane* m, const double* d_draws_dev, bool hook);
+// EDM_ECAPACITY once a binding step overflowed its edge buffer (synchronises)
+int dyn_check_overflow(edm_membrane* m);
 int dyn_pairs(edm_membrane* m, int32_t* pairs, int64_t cap, int64_t* n_out);
 int dyn_density(edm_membrane* m, int32_t species, int32_t grid, double* out);
 int dyn_ensure_slot_of(edm_membrane* m);

@@ -407,6 +407,11 @@ class DynamicsEngine:
                 print(f"  t={(k + 1) * self.dt:.4f}s | sum bind={self.n_bindings} "
                       f"sum dissoc={self.n_dissociations}")
 
+    def synchronize(self) -> None:
+        """Wait for the queued steps; raises HipError if one of them
+        overflowed a device buffer (EDM_ECAPACITY)."""
+        check(lib().edm_synchronize(self._h), "edm_synchronize")
+
     def step_begin(self) -> int:
         """First half of a parity step: cell list on pre-move positions;
         returns the reference pair-list length."""

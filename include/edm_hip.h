@@ -172,6 +172,11 @@ int edm_set_imaging(edm_membrane *m, const edm_image_params *ip);
  *   edm_step_finish  forces, Ermak-McCammon step and every reaction, taking
  *                    `draws` = the concatenation of the step's numpy arrays
  *                    in the reference's draw order (NULL = Philox)          */
+/* Binding keeps its accepted candidate edges in a device buffer: with hook
+ * draws it holds one edge per listed pair (grown before the step); with
+ * Philox draws 4 per particle.  A step that accepts more edges than that is
+ * not truncated silently: the next edm_step_begin, edm_counters,
+ * edm_synchronize or edm_download_state returns EDM_ECAPACITY.            */
 int edm_step_begin(edm_membrane *m, int64_t *n_pairs);
 int edm_step_finish(edm_membrane *m, const double *draws, int64_t n_draws);
 int edm_run(edm_membrane *m, int32_t n_steps);

@@ -201,6 +201,40 @@ void orc_diffusion_field_smooth(double *D, double *gradD, const double *D_base,
 
 /* Sinusoidal D(x) field of the wave-system workload (not in the reference):
  * D = D_s (1 + a sin(kx x) sin(ky y)), grad D analytic. */
+/* sin and cos by Cody-Waite reduction modulo pi/2 (two 33-bit parts of
+ * pi/2 and a 53-bit tail) and Taylor polynomials through r^17 / r^18 on
+ * [-pi/4, pi/4], Horner form; IEEE add, multiply and rint only.  The
+ * sinusoidal field is not in the reference, so the field's definition is
+ * this build's: this method, checked against libm sin/cos to 2e-16 and by
+ * the closed-form gradient (tests/test_oracle_kernels.py). */
+void orc_sincos(double x, double *s, double *c) {
+    const double k = rint(x * 0x1.45f306dc9c883p-1);
+    const double r = ((x - k * 0x1.921fb54400000p+0) - k * 0x1.0b4611a600000p-34) -
+                     k * 0x1.3198a2e037073p-69;
+    const double r2 = r * r;
+    static const double S[8] = {0x1.952c77030ad4ap-49, -0x1.ae7f3e733b81fp-41,
+                                0x1.6124613a86d09p-33, -0x1.ae64567f544e4p-26,
+                                0x1.71de3a556c734p-19, -0x1.a01a01a01a01ap-13,
+                                0x1.1111111111111p-7, -0x1.5555555555555p-3};
+    static const double C[9] = {-0x1.6827863b97d97p-53, 0x1.ae7f3e733b81fp-45,
+                                -0x1.93974a8c07c9dp-37, 0x1.1eed8eff8d898p-29,
+                                -0x1.27e4fb7789f5cp-22, 0x1.a01a01a01a01ap-16,
+                                -0x1.6c16c16c16c17p-10, 0x1.5555555555555p-5,
+                                -0x1.0000000000000p-1};
+    double ps = S[0];
+    for (int q = 1; q < 8; ++q) ps = S[q] + r2 * ps;
+    double pc = C[0];
+    for (int q = 1; q < 9; ++q) pc = C[q] + r2 * pc;
+    const double sr = r + r * (r2 * ps);
+    const double cr = 1.0 + r2 * pc;
+    switch (((long long)k) & 3) {
+        case 0: *s = sr; *c = cr; break;
+        case 1: *s = cr; *c = -sr; break;
+        case 2: *s = -sr; *c = -cr; break;
+        default: *s = -cr; *c = sr; break;
+    }
+}
+
 void orc_diffusion_field_sinusoidal(double *D, double *gradD,
                                     const double *D_base, const int *species,
                                     const double *pos, int N, double amp,
@@ -208,8 +242,9 @@ void orc_diffusion_field_sinusoidal(double *D, double *gradD,
     for (int i = 0; i < N; ++i) {
         double x = pos[2 * i], y = pos[2 * i + 1];
         double base = D_base[species[i]];
-        double sx = sin(kx * x), cx = cos(kx * x);
-        double sy = sin(ky * y), cy = cos(ky * y);
+        double sx, cx, sy, cy;
+        orc_sincos(kx * x, &sx, &cx);
+        orc_sincos(ky * y, &sy, &cy);
         D[i] = base * (1.0 + amp * sx * sy);
         gradD[2 * i] = base * amp * kx * cx * sy;
         gradD[2 * i + 1] = base * amp * ky * sx * cy;
@@ -421,17 +456,42 @@ int orc_dissociation(int *species, int *bond, uint8_t *alive, double *pos,
     return nd;
 }
 
+/* Cell of the adaptation field read for a substrate in feedback-grid cell
+ * (gx, gy).  dynamics.py:1078 reads adaptation_field[gx, gy] with the
+ * FEEDBACK grid's indices on the adaptation grid's [anx, any] array, and
+ * numba does not bounds-check: the element read is the flat one,
+ * gx * any + gy, which is another cell when the two grids differ and lies
+ * past the array when gx * any + gy >= anx * any.  Where it lies inside the
+ * array this follows the reference exactly; past it the reference reads
+ * memory it does not own (no defined value), and this restatement reads
+ * the adaptation field at the substrate's own cell instead (the adaptation
+ * grid spans the feedback grid's extent nxg * sx by nyg * sy). */
+static inline int adapt_cell(double x, double y, int gx, int gy, int nxg, int nyg,
+                             double sx, double sy, int anx, int any) {
+    long flat = (long)gx * any + gy;
+    if (flat < (long)anx * any) return (int)flat;
+    double asx = (nxg * sx) / anx, asy = (nyg * sy) / any;
+    int ax = clampi((int)(x / asx), 0, anx);
+    int ay = clampi((int)(y / asy), 0, any);
+    return ax * any + ay;
+}
+
+int orc_adapt_cell(double x, double y, int gx, int gy, int nxg, int nyg, double sx,
+                   double sy, int anx, int any) {
+    return adapt_cell(x, y, gx, gy, nxg, nyg, sx, sy, anx, any);
+}
+
 /* dynamics.py:826-1090: apply_catalytic_reaction, _with_feedback and
  * _with_feedback_and_adaptation in one body.  fb->density == NULL selects
- * the plain kernel; adapt != NULL multiplies the adaptation factor read at
- * the feedback grid cell (the reference reads w at the same gx, gy). */
+ * the plain kernel; adapt != NULL ([anx, any]) multiplies the adaptation
+ * factor read as adapt_cell gives it (dynamics.py:1078-1084). */
 int orc_catalytic(const double *pos, int *species, const uint8_t *alive,
                   int enz, int sub, int prod, double contact, double prob,
                   const int *pairs, int n_pairs, const double *u, double Lx,
                   double Ly, int px, int py, const double *density, int nxg,
                   int nyg, double gsx, double gsy, double K, double n,
                   int inhibition, double basal, const double *adapt,
-                  double K_adapt, double n_adapt) {
+                  int anx, int any, double K_adapt, double n_adapt) {
     int nc = 0;
     double c2 = contact * contact;
     for (int p = 0; p < n_pairs; ++p) {
@@ -454,7 +514,9 @@ int orc_catalytic(const double *pos, int *species, const uint8_t *alive,
             if (adapt) {
                 double af = 1.0;
                 if (K_adapt > 0) {
-                    double wr = pow(adapt[c] / K_adapt, n_adapt);
+                    int wc = adapt_cell(pos[2 * s], pos[2 * s + 1], gx, gy, nxg, nyg, gsx,
+                                        gsy, anx, any);
+                    double wr = pow(adapt[wc] / K_adapt, n_adapt);
                     af = 1.0 / (1.0 + wr);
                 }
                 pe = prob * hf * af;

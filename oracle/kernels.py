@@ -259,11 +259,13 @@ def _catalytic(positions, species_id, is_alive, enzyme_id, substrate_id, product
     f.restype = I
     dens = None if density is None else _f64(density)
     ad = None if adapt is None else _f64(adapt)
+    anx, any_ = (1, 1) if adapt is None else np.shape(adapt)
     res = f(P(_f64(positions)), P(sp.b), P(np.ascontiguousarray(is_alive, dtype=np.uint8)),
             I(enzyme_id), I(substrate_id), I(product_id), D(contact_radius_um), D(prob),
             P(_i32(pairs)), I(n_pairs), P(_f64(rng_uniform)), D(Lx), D(Ly),
             I(int(bool(px))), I(int(bool(py))), P(dens), I(nxg), I(nyg), D(sx), D(sy), D(K),
-            D(n), I(int(bool(inh))), D(basal), P(ad), D(K_adapt), D(n_adapt))
+            D(n), I(int(bool(inh))), D(basal), P(ad), I(anx), I(any_), D(K_adapt),
+            D(n_adapt))
     sp.back()
     return res
 

@@ -444,13 +444,13 @@ class OracleDynamics:
                     D(c['contact']), D(c['prob']), P(pairs), I(npairs), P(u), Lx, Ly, px, py,
                     P(fb['density']), I(fb['nxg']), I(fb['nyg']), D(fb['sx']), D(fb['sy']),
                     D(fb['K']), D(fb['n']), I(fb['inh']), D(fb['basal']), P(adapt),
-                    D(self.K_w), D(self.n_w))
+                    I(self.anx), I(self.any), D(self.K_w), D(self.n_w))
             else:
                 self.n_catalytic += L.orc_catalytic(
                     P(pos), P(sp), P(alive), I(c['e']), I(c['s']), I(c['p']),
                     D(c['contact']), D(c['prob']), P(pairs), I(npairs), P(u), Lx, Ly, px, py,
                     None, I(0), I(0), D(1.0), D(1.0), D(0.0), D(0.0), I(0), D(0.0), None,
-                    D(0.0), D(0.0))
+                    I(1), I(1), D(0.0), D(0.0))
         for ex in self.exch:
             u_acc = self.rng.random(N)
             u_pos = self.rng.random((N, 2))

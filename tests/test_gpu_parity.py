@@ -151,8 +151,11 @@ def test_binding_fret_full_path_fp64(oracle_lib, variant):
 
 # um after 100 reaction-free steps: (median, 99th percentile, worst) of the
 # distance between the FP32 and FP64 positions of a particle
-# (measured: plain 1.6e-6 / 1.1e-4 / 8.2e-3, binding_regions 5.5e-7 / 1.1e-5 / 6.0e-5)
-FP32_POS_BOUND = {"plain": (1e-5, 1e-3, 5e-2), "binding_regions": (1e-5, 1e-4, 1e-3)}
+# (measured: plain 1.6e-6 / 1.1e-4 / 8.2e-3, binding_regions 5.5e-7 / 1.1e-5 / 6.0e-5,
+# wave 2.0e-6 / 5.9e-3 / 4.5e-2: the 13,000-particle wave system at 130 / um^2
+# has more WCA contacts, whose displacement-capped kicks amplify rounding)
+FP32_POS_BOUND = {"plain": (1e-5, 1e-3, 5e-2), "binding_regions": (1e-5, 1e-4, 1e-3),
+                  "wave": (1e-5, 2e-2, 1e-1)}
 
 
 @pytest.mark.parametrize("workload", list(FP32_POS_BOUND))
@@ -170,6 +173,8 @@ def test_fp32_positions_within_bound(oracle_lib, workload):
     (test_gpu_statistics.py)."""
     if workload == "plain":
         cfg = workloads.plain_membrane()
+    elif workload == "wave":
+        cfg = workloads.wave_system(n_steps=100)
     else:
         cfg = workloads.binding_fret(seed=4, reflective=True)
     cfg.reactions = []
