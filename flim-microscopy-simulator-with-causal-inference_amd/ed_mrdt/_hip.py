@@ -180,6 +180,51 @@ _SIGS = {
     "edm_ens_counters": [c_void_p, c_int32, c_void_p],
     "edm_ens_info": [c_void_p, c_void_p],
     "edm_ens_synchronize": [c_void_p],
+    # device memory for the free kernels
+    "edm_device_alloc": [c_int64, ctypes.POINTER(c_void_p)],
+    "edm_device_free": [c_void_p],
+    "edm_copy_to_device": [c_void_p, c_void_p, c_int64],
+    "edm_copy_to_host": [c_void_p, c_void_p, c_int64],
+    # the reference's free kernels on device arrays (dynamics.py:38-60)
+    "edm_cell_pairs": [c_void_p, c_int32, c_int32, c_int32, c_double, c_double, c_double,
+                       c_double, c_int32, c_int32, c_double, c_int64, c_void_p,
+                       ctypes.POINTER(c_int64), ctypes.POINTER(c_int64), c_void_p],
+    "edm_bd_step": [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double, c_double,
+                    c_int32, c_int32, c_int32, c_void_p, c_void_p, c_double, c_void_p],
+    "edm_diffusion_field_regions": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32,
+                                    c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_double,
+                                    c_void_p],
+    "edm_pair_forces": [c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p,
+                        c_int64, c_void_p, c_void_p, c_int32, c_double, c_double, c_int32,
+                        c_int32, c_void_p],
+    "edm_binding": [c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int32, c_int32,
+                    c_double, c_double, c_void_p, c_int64, c_void_p, c_double, c_double, c_int32,
+                    c_int32, ctypes.POINTER(c_int64), c_void_p],
+    "edm_dissociation": [c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int32,
+                         c_int32, c_double, c_void_p, ctypes.POINTER(c_int64), c_void_p],
+    "edm_catalysis": [c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int32, c_int32, c_double,
+                      c_double, c_void_p, c_int64, c_void_p, c_double, c_double, c_int32,
+                      c_int32, c_void_p, c_int32, c_int32, c_double, c_double, c_double,
+                      c_double, c_int32, c_double, c_void_p, c_int32, c_int32, c_double,
+                      c_double, ctypes.POINTER(c_int64), c_void_p],
+    "edm_recruitment": [c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_double, c_double,
+                        c_double, c_void_p, c_void_p, c_int32,
+                        c_void_p, c_int32, c_int32, c_double, c_double, c_double, c_double,
+                        c_int32, c_double,
+                        c_void_p, c_int32, c_int32, c_double, c_double, c_double, c_double,
+                        c_int32, c_double,
+                        c_void_p, c_void_p, c_void_p, c_double, ctypes.POINTER(c_int64),
+                        c_void_p],
+    "edm_detachment": [c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_double, c_void_p,
+                       c_void_p, c_int32, c_int32, c_double, c_double, c_double, c_double,
+                       c_int32, c_double, ctypes.POINTER(c_int64), c_void_p],
+    "edm_conversion": [c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int32, c_double,
+                       c_void_p, c_void_p, c_int32, c_int32, c_double, c_double, c_double,
+                       c_double, c_int32, c_double, ctypes.POINTER(c_int64), c_void_p],
+    "edm_density_field": [c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int32, c_int32,
+                          c_double, c_double, c_void_p, c_void_p],
+    "edm_adaptation_update": [c_void_p, c_void_p, c_int32, c_int32, c_double, c_double,
+                              c_void_p],
 }
 
 EXPORTS = tuple(_SIGS) + ("edm_last_error",)

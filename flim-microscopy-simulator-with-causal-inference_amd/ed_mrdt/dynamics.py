@@ -31,7 +31,43 @@ from ._hip import check, lib, ptr
 from .config import ConfigError, SimulationConfig
 from .particles import ParticleState
 
-__all__ = ["DynamicsEngine", "derive_constants"]
+from .free_kernels import (CellList, DeviceArray, apply_binding_reactions,  # noqa: F401
+                           apply_catalytic_reaction, apply_catalytic_with_feedback,
+                           apply_catalytic_with_feedback_and_adaptation,
+                           apply_diffusion_field_regions_smooth, apply_dissociation,
+                           apply_membrane_detachment, apply_membrane_detachment_with_feedback,
+                           apply_membrane_recruitment, apply_membrane_recruitment_local,
+                           apply_membrane_recruitment_with_dual_feedback,
+                           apply_membrane_recruitment_with_feedback, apply_species_conversion,
+                           apply_species_conversion_with_feedback, bd_step,
+                           compute_density_field, compute_forces_harmonic, compute_forces_wca,
+                           update_adaptation_field)
+
+# the reference's export list (dynamics.py:38-60) plus this module's helper
+__all__ = [
+    "CellList",
+    "bd_step",
+    "apply_binding_reactions",
+    "apply_dissociation",
+    "compute_forces_wca",
+    "DynamicsEngine",
+    "apply_catalytic_reaction",
+    "apply_catalytic_with_feedback",
+    "apply_membrane_recruitment",
+    "apply_membrane_recruitment_with_feedback",
+    "apply_membrane_recruitment_with_dual_feedback",
+    "apply_membrane_detachment",
+    "apply_membrane_detachment_with_feedback",
+    "apply_species_conversion",
+    "apply_species_conversion_with_feedback",
+    "compute_density_field",
+    "compute_forces_harmonic",
+    "apply_diffusion_field_regions_smooth",
+    "apply_catalytic_with_feedback_and_adaptation",
+    "update_adaptation_field",
+    "apply_membrane_recruitment_local",
+    "derive_constants",
+]
 
 
 def _fb_key(fb, Lx, Ly):

@@ -315,6 +315,119 @@ int edm_ground_truth(edm_membrane *m, int32_t n_maps, const int32_t *donor,
                      const double *tau_D, double *fret_E, double *tau_DA,
                      edm_census *census);
 
+/* ---- The reference's free kernels on device arrays ----------------------
+ * dynamics.py exports its numba kernels (dynamics.py:38-60, __all__) and
+ * callers (the reference's own tests among them) run them on numpy arrays
+ * in place.  These entry points are the same kernels on DEVICE arrays in
+ * the reference's layouts: positions / forces / grad_D / noise / offsets
+ * f64[n, 2] row-major, species and bond partners int32[n], is_alive
+ * uint8[n] (0/1), pair lists int32[m, 2], density and adaptation fields
+ * f64[nx, ny] row-major, uniforms f64.  They keep the reference kernels'
+ * sequential results (pair order, index-order caps, sequential FP64 sums)
+ * and mutate in place as the reference does.  `stream` is a hipStream_t
+ * (NULL: the null stream).  An entry point with a count output waits for
+ * its stream and writes the count to host memory; the others only enqueue.
+ * A feedback density pointer of NULL drops that Hill factor (the plain
+ * form of the kernel). */
+
+/* Device memory for callers of these kernels that have no allocator of
+ * their own (no reference counterpart: plumbing).  The copies are
+ * synchronous. */
+int edm_device_alloc(int64_t bytes, void **out);
+int edm_device_free(void *p);
+int edm_copy_to_device(void *dst, const void *src, int64_t bytes);
+int edm_copy_to_host(void *dst, const void *src, int64_t bytes);
+
+/* CellList.build + get_neighbor_pairs (dynamics.py:120-175, 177-278): the
+ * pairs (i < j, d^2 < cutoff^2) in the reference's order (i ascending,
+ * neighbour cells (dcx, dcy) lexicographic, each cell's members in
+ * descending index), at most max_pairs stored (the reference's cap is
+ * min(30 n, 500000)); *n_pairs = stored, *n_found = all (may be NULL). */
+int edm_cell_pairs(const double *positions, int32_t n, int32_t nx, int32_t ny, double cell_sx,
+                   double cell_sy, double Lx, double Ly, int32_t periodic_x, int32_t periodic_y,
+                   double cutoff, int64_t max_pairs, int32_t *pairs, int64_t *n_pairs,
+                   int64_t *n_found, void *stream);
+/* bd_step (dynamics.py:323-406) */
+int edm_bd_step(double *positions, const double *D, const double *forces, const double *grad_D,
+                double dt, double Lx, double Ly, int32_t periodic_x, int32_t periodic_y, int32_t n,
+                const double *noise, const uint8_t *is_alive, double max_displacement,
+                void *stream);
+/* apply_diffusion_field_regions_smooth (dynamics.py:441-519) */
+int edm_diffusion_field_regions(double *D, double *grad_D, const double *D_base,
+                                const int32_t *species_id, const double *positions, int32_t n,
+                                const double *region_cx, const double *region_cy,
+                                const double *region_r, const double *region_Dfactor,
+                                int32_t n_regions, double smoothing_width, void *stream);
+/* compute_forces_wca (kind 0, param = epsilon; dynamics.py:666-739) and
+ * compute_forces_harmonic (kind 1, param = spring constant;
+ * dynamics.py:753-816): forces += pair forces, each particle's additions in
+ * pair order; matrices [n_species, n_species] row-major */
+int edm_pair_forces(int32_t kind, const double *positions, double *forces,
+                    const int32_t *species_id, const uint8_t *is_alive, int32_t n,
+                    const int32_t *pairs, int64_t n_pairs, const double *sigma_matrix,
+                    const double *param_matrix, int32_t n_species, double Lx, double Ly,
+                    int32_t periodic_x, int32_t periodic_y, void *stream);
+/* apply_binding_reactions (dynamics.py:529-606) */
+int edm_binding(const double *positions, int32_t *species_id, int32_t *bond_partner,
+                uint8_t *is_alive, int32_t n, int32_t reactant_a, int32_t reactant_b,
+                int32_t product_id, double contact_radius_um, double prob_bind,
+                const int32_t *pairs, int64_t n_pairs, const double *rng_uniform, double Lx,
+                double Ly, int32_t periodic_x, int32_t periodic_y, int64_t *n_bound,
+                void *stream);
+/* apply_dissociation (dynamics.py:610-654); the product must differ from
+ * both reactants */
+int edm_dissociation(int32_t *species_id, int32_t *bond_partner, uint8_t *is_alive,
+                     double *positions, int32_t n, int32_t product_id, int32_t reactant_a,
+                     int32_t reactant_b, double prob_dissoc, const double *rng_uniform,
+                     int64_t *n_dissociated, void *stream);
+/* apply_catalytic_reaction, _with_feedback, _with_feedback_and_adaptation
+ * (dynamics.py:826-1090); the enzyme must differ from the substrate and the
+ * product.  adaptation_field [nx_adapt, ny_adapt] (NULL: none) is read at
+ * the feedback grid's indices, as dynamics.py:1078 reads it */
+int edm_catalysis(const double *positions, int32_t *species_id, const uint8_t *is_alive,
+                  int32_t n, int32_t enzyme_id, int32_t substrate_id, int32_t product_id,
+                  double contact_radius_um, double prob, const int32_t *pairs, int64_t n_pairs,
+                  const double *rng_uniform, double Lx, double Ly, int32_t periodic_x,
+                  int32_t periodic_y, const double *density_field, int32_t nx_grid,
+                  int32_t ny_grid, double cell_sx, double cell_sy, double K_half, double n_hill,
+                  int32_t is_inhibition, double basal_fraction, const double *adaptation_field,
+                  int32_t nx_adapt, int32_t ny_adapt, double K_adapt, double n_adapt,
+                  int64_t *n_converted, void *stream);
+/* apply_membrane_recruitment, _with_feedback, _with_dual_feedback
+ * (dynamics.py:1230-1458; density_1 / density_2 NULL as the form needs) and
+ * apply_membrane_recruitment_local (dynamics.py:1116-1217; cdf != NULL,
+ * with rng_location, rng_offsets [n, 2] and density_1 its grid) */
+int edm_recruitment(const int32_t *species_id, uint8_t *is_alive, double *positions, int32_t n,
+                    int32_t target_species_id, double prob, double Lx, double Ly,
+                    const double *rng_accept, const double *rng_positions, int32_t max_recruits,
+                    const double *density_1, int32_t nx_1, int32_t ny_1, double sx_1, double sy_1,
+                    double K_1, double n_1, int32_t inh_1, double basal_1, const double *density_2,
+                    int32_t nx_2, int32_t ny_2, double sx_2, double sy_2, double K_2, double n_2,
+                    int32_t inh_2, double basal_2, const double *cdf, const double *rng_location,
+                    const double *rng_offsets, double recruit_sigma, int64_t *n_recruited,
+                    void *stream);
+/* apply_membrane_detachment(_with_feedback) (dynamics.py:1462-1555) */
+int edm_detachment(const int32_t *species_id, uint8_t *is_alive, const double *positions,
+                   int32_t n, int32_t target_species_id, double prob, const double *rng_uniform,
+                   const double *density_field, int32_t nx_grid, int32_t ny_grid, double cell_sx,
+                   double cell_sy, double K_half, double n_hill, int32_t is_inhibition,
+                   double basal_fraction, int64_t *n_detached, void *stream);
+/* apply_species_conversion(_with_feedback) (dynamics.py:1566-1657) */
+int edm_conversion(int32_t *species_id, const uint8_t *is_alive, const double *positions,
+                   int32_t n, int32_t from_id, int32_t to_id, double prob,
+                   const double *rng_uniform, const double *density_field, int32_t nx_grid,
+                   int32_t ny_grid, double cell_sx, double cell_sy, double K_half, double n_hill,
+                   int32_t is_inhibition, double basal_fraction, int64_t *n_converted,
+                   void *stream);
+/* compute_density_field (dynamics.py:1668-1703): density_out += 1/area per
+ * alive particle of the species, each cell the reference's sequential sum */
+int edm_density_field(const double *positions, const int32_t *species_id, const uint8_t *is_alive,
+                      int32_t n, int32_t target_species_id, int32_t nx_grid, int32_t ny_grid,
+                      double cell_sx, double cell_sy, double *density_out, void *stream);
+/* update_adaptation_field (dynamics.py:1094-1112) */
+int edm_adaptation_update(double *adaptation_field, const double *density_field, int32_t nx,
+                          int32_t ny, double dt, double tau_w, void *stream);
+
 /* ---- Batched ensemble ---------------------------------------------------
  * Many independent membranes of one model advanced by the same launches
  * (the reference runs replicas one after another: one SimulationPipeline

@@ -360,12 +360,17 @@ def apply_membrane_recruitment_with_feedback(species_id, is_alive, positions, N,
                     Lx, Ly, rng_accept, rng_positions, max_recruits, fb)
 
 
-def apply_membrane_recruitment_with_dual_feedback(species_id, is_alive, positions, N,
-                                                  target_species_id, prob_recruit_base, Lx,
-                                                  Ly, rng_accept, rng_positions,
-                                                  max_recruits, fb1, fb2) -> int:
-    """dynamics.py:1347-1458: recruitment x Hill_1 x Hill_2; fb = (density,
-    nx, ny, sx, sy, K, n, inhibition, basal)."""
+def apply_membrane_recruitment_with_dual_feedback(
+        species_id, is_alive, positions, N, target_species_id, prob_recruit_base, Lx, Ly,
+        rng_accept, rng_positions, max_recruits, density_field_1, nx_grid_1, ny_grid_1,
+        cell_sx_1, cell_sy_1, K_half_1, n_hill_1, is_inhibition_1, basal_fraction_1,
+        density_field_2, nx_grid_2, ny_grid_2, cell_sx_2, cell_sy_2, K_half_2, n_hill_2,
+        is_inhibition_2, basal_fraction_2) -> int:
+    """dynamics.py:1347-1458: recruitment x Hill_1 x Hill_2 at the site."""
+    fb1 = (density_field_1, nx_grid_1, ny_grid_1, cell_sx_1, cell_sy_1, K_half_1, n_hill_1,
+           is_inhibition_1, basal_fraction_1)
+    fb2 = (density_field_2, nx_grid_2, ny_grid_2, cell_sx_2, cell_sy_2, K_half_2, n_hill_2,
+           is_inhibition_2, basal_fraction_2)
     return _recruit(species_id, is_alive, positions, N, target_species_id, prob_recruit_base,
                     Lx, Ly, rng_accept, rng_positions, max_recruits, fb1, fb2)
 

@@ -53,6 +53,25 @@ def test_library_exports_every_declared_symbol(hip):
     assert not unbound, f"declared in edm_hip.h but not bound in _hip.py: {unbound}"
 
 
+def _prototypes():
+    """name -> parameter count of every function the header declares."""
+    text = open(HEADER, encoding="utf-8").read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    out = {}
+    for m in re.finditer(r"\b(edm_[a-z0-9_]+)\s*\(([^()]*)\)\s*;", text):
+        args = m.group(2).strip()
+        out[m.group(1)] = 0 if args in ("", "void") else args.count(",") + 1
+    return out
+
+
+def test_ctypes_argument_counts_match_the_header(hip):
+    protos = _prototypes()
+    assert len(protos) >= 60
+    bad = {n: (k, len(hip._SIGS[n])) for n, k in protos.items()
+           if n in hip._SIGS and len(hip._SIGS[n]) != k}
+    assert not bad, f"header vs ctypes parameter counts: {bad}"
+
+
 _STRUCTS = {
     "edm_feedback": ("Feedback", ["sensor", "grid", "K_half", "basal"]),
     "edm_grid": ("Grid", ["nx", "sx", "sy"]),

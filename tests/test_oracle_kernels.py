@@ -234,8 +234,8 @@ def test_recruitment_feedback_at_candidate_site():
     assert n == 2 and alive.tolist() == [False, True, False, True]
     alive[:] = False
     n = apply_membrane_recruitment_with_dual_feedback(sp, alive, np.zeros((N, 2)), N, 0, 1.0,
-                                                      1.0, 1.0, np.full(N, 0.5), rpos, 10, fb1,
-                                                      fb2)
+                                                      1.0, 1.0, np.full(N, 0.5), rpos, 10, *fb1,
+                                                      *fb2)
     assert n == 1 and alive.tolist() == [False, False, False, True]
 
 
