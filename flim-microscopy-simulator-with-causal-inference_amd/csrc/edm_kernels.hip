@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "edm_device.h"
@@ -330,6 +331,41 @@ int run_pairs(const double* pos, int32_t n, int32_t nx, int32_t ny, double csx, 
     EDM_HIP(hipGetLastError());
     *n_pairs = std::min<int64_t>((int64_t)total, max_pairs);
     if (n_found) *n_found = (int64_t)total;
+    return 0;
+}
+
+// ------------------------------------------- allocation-free variant's scratch
+// edm_cell_pairs_noalloc builds its pair list in this array instead of a
+// per-call allocation: cell starts, arrival ranks, cell members, pair
+// offsets and scan partials for at most EDM_NOALLOC_MAX_PARTICLES particles
+// and EDM_NOALLOC_MAX_CELLS cells (20 MiB; every GPU the library runs on
+// holds its own copy until the process ends).  Calls share it: overlapping
+// calls on one GPU race.
+constexpr int64_t kNoAllocMaxN = EDM_NOALLOC_MAX_PARTICLES;
+constexpr int64_t kNoAllocMaxCells = EDM_NOALLOC_MAX_CELLS;
+constexpr size_t kNoAllocBytes = pair_scratch_bytes(kNoAllocMaxN, kNoAllocMaxCells);
+
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// The device address of `scratch` on each device (each GPU holds its own
+// copy at its own address).  The array's name in host code is a host-side
+// stand-in, so host code only ever passes these addresses.
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+int noalloc_scratch(void** out) {
+    int dev = 0;
+    EDM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return kinvalid("edm_cell_pairs_noalloc: device id out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        EDM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
     return 0;
 }
 
@@ -841,6 +877,24 @@ int edm_cell_pairs(const double* positions, int32_t n, int32_t nx, int32_t ny, d
     if (int rc = buf.alloc(pair_scratch_bytes(n, (int64_t)nx * ny), st)) return rc;
     return run_pairs(positions, n, nx, ny, cell_sx, cell_sy, Lx, Ly, periodic_x, periodic_y,
                      cutoff, max_pairs, pairs, n_pairs, n_found, buf.p, st);
+}
+
+int edm_cell_pairs_noalloc(const double* positions, int32_t n, int32_t nx, int32_t ny,
+                           double cell_sx, double cell_sy, double Lx, double Ly,
+                           int32_t periodic_x, int32_t periodic_y, double cutoff, int64_t max_pairs,
+                           int32_t* pairs, int64_t* n_pairs, int64_t* n_found, void* stream) {
+    if (int rc = check_pair_args(positions, n, nx, ny, cell_sx, cell_sy, cutoff, max_pairs, pairs,
+                                 n_pairs))
+        return rc;
+    if ((int64_t)n > kNoAllocMaxN || (int64_t)nx * ny > kNoAllocMaxCells)
+        return kinvalid("edm_cell_pairs_noalloc: at most " + std::to_string(kNoAllocMaxN) +
+                        " particles and " + std::to_string(kNoAllocMaxCells) +
+                        " cells fit its static scratch; got " + std::to_string(n) +
+                        " particles and " + std::to_string((int64_t)nx * ny) + " cells");
+    void* buf = nullptr;
+    if (int rc = noalloc_scratch(&buf)) return rc;
+    return run_pairs(positions, n, nx, ny, cell_sx, cell_sy, Lx, Ly, periodic_x, periodic_y,
+                     cutoff, max_pairs, pairs, n_pairs, n_found, buf, (hipStream_t)stream);
 }
 
 int edm_bd_step(double* positions, const double* D, const double* forces, const double* grad_D,

@@ -189,6 +189,10 @@ _SIGS = {
     "edm_cell_pairs": [c_void_p, c_int32, c_int32, c_int32, c_double, c_double, c_double,
                        c_double, c_int32, c_int32, c_double, c_int64, c_void_p,
                        ctypes.POINTER(c_int64), ctypes.POINTER(c_int64), c_void_p],
+    "edm_cell_pairs_noalloc": [c_void_p, c_int32, c_int32, c_int32, c_double, c_double,
+                               c_double, c_double, c_int32, c_int32, c_double, c_int64,
+                               c_void_p, ctypes.POINTER(c_int64), ctypes.POINTER(c_int64),
+                               c_void_p],
     "edm_bd_step": [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double, c_double,
                     c_int32, c_int32, c_int32, c_void_p, c_void_p, c_double, c_void_p],
     "edm_diffusion_field_regions": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32,

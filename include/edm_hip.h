@@ -347,6 +347,20 @@ int edm_cell_pairs(const double *positions, int32_t n, int32_t nx, int32_t ny, d
                    double cell_sy, double Lx, double Ly, int32_t periodic_x, int32_t periodic_y,
                    double cutoff, int64_t max_pairs, int32_t *pairs, int64_t *n_pairs,
                    int64_t *n_found, void *stream);
+/* No counterpart in the reference: edm_cell_pairs without any device
+ * allocation, for callers that cannot allocate (code that owns its memory
+ * budget, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as edm_cell_pairs; the build's scratch is one static
+ * device array per GPU (20 MiB), so n is at most EDM_NOALLOC_MAX_PARTICLES
+ * and nx * ny at most EDM_NOALLOC_MAX_CELLS (larger calls return
+ * EDM_EINVAL), and calls that overlap on one GPU race. */
+#define EDM_NOALLOC_MAX_PARTICLES (1 << 20)
+#define EDM_NOALLOC_MAX_CELLS (1 << 20)
+int edm_cell_pairs_noalloc(const double *positions, int32_t n, int32_t nx, int32_t ny,
+                           double cell_sx, double cell_sy, double Lx, double Ly,
+                           int32_t periodic_x, int32_t periodic_y, double cutoff,
+                           int64_t max_pairs, int32_t *pairs, int64_t *n_pairs, int64_t *n_found,
+                           void *stream);
 /* bd_step (dynamics.py:323-406) */
 int edm_bd_step(double *positions, const double *D, const double *forces, const double *grad_D,
                 double dt, double Lx, double Ly, int32_t periodic_x, int32_t periodic_y, int32_t n,
