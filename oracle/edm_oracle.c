@@ -12,11 +12,10 @@
  * this library, and only as the checker or the timed CPU baseline.  The
  * product path (the HIP library) never calls it.
  *
- * Pinning (which check fixes each function) is listed in oracle/README.md.
- * parity unpinned: orc_pss_kernel's photon order and arrival-array
- * consumption (no reference test fixes them; checked only against this
- * restatement's reading of photophysics.py:339-483 and frozen by the golden
- * vectors of tests/golden/).
+ * Pinning (which check fixes each function) is listed in oracle/README.md;
+ * orc_pss_kernel's photon order, arrival-array consumption, max_photons cut,
+ * exhaustion and transitions are fixed by a hand-built known answer and by
+ * the CTMC statistics in tests/test_oracle_pins.py.
  *
  * Build: oracle/build_oracle.py (gcc -O2 -ffp-contract=off -fopenmp).
  */
