@@ -30,6 +30,7 @@ static void ens_free(edm_ensemble* e) {
         if (g.flim) hipFree(g.flim);
         if (g.rcdf) hipFree(g.rcdf);
     }
+    ens_frames_free(e);
     if (e->stream) hipStreamDestroy(e->stream);
     delete e;
 }
@@ -323,6 +324,25 @@ int edm_ens_frame(edm_ensemble* e, int32_t slot, uint32_t* flim, int64_t n_elems
     if (int rc = ens_frame_reset(e, slot)) return rc;
     EDM_HIP(hipStreamSynchronize(e->stream));
     return 0;
+}
+
+int edm_ens_frames_collect(edm_ensemble* e, const int32_t* slots, int32_t n_slots,
+                           int32_t* ticket) {
+    if (!e || !ticket || n_slots < 0 || (n_slots > 0 && !slots))
+        return ens_invalid("edm_ens_frames_collect: null argument");
+    for (int i = 0; i < n_slots; ++i) {
+        if (slots[i] < 0 || slots[i] >= e->rmax || !e->seg[slots[i]].active)
+            return ens_invalid("edm_ens_frames_collect: no such replica");
+        for (int j = 0; j < i; ++j)
+            if (slots[j] == slots[i]) return ens_invalid("edm_ens_frames_collect: slot listed twice");
+    }
+    return ens_frames_collect(e, slots, n_slots, ticket);
+}
+
+int edm_ens_frames_fetch(edm_ensemble* e, int32_t ticket, int64_t* counts, int64_t* meta,
+                         uint32_t* flat_index, uint32_t* count, int64_t cap, int64_t* total) {
+    if (!e) return ens_invalid("edm_ens_frames_fetch: null handle");
+    return ens_frames_fetch(e, ticket, counts, meta, flat_index, count, cap, total);
 }
 
 int edm_ens_frame_end(edm_ensemble* e, const int32_t* slots, int32_t n_slots) {

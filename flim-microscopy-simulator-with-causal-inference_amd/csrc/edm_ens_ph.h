@@ -153,3 +153,251 @@ int ens_frame_end(edm_ensemble* e, const int32_t* slots, int32_t n) {
     EDM_HIP(hipStreamSynchronize(e->stream));
     return 0;
 }
+
+// ---------------------------------------------------------------- streamed frames
+// A collect ends the frames of the listed replicas and returns every frame,
+// sparse: after the detector noise, each stack's nonzero bins are written as
+// (flat index, count) pairs, in index order, replica after replica in list
+// order, then the stacks are reset.  Each stack is cut into kSpBlocks equal
+// chunks; a first pass counts the nonzero bins per chunk, an exclusive scan
+// turns the counts into offsets, a second pass writes each chunk's pairs
+// from its offset (ordered within the chunk by a block-wide prefix of wave
+// ballots).
+static constexpr int kSpBlocks = 64;
+
+__device__ __forceinline__ void sp_chunk(const EnsSeg& g, int b, int64_t& c0, int64_t& c1) {
+    const int64_t n = g.flim ? g.flim_elems : 0;
+    const int64_t per = (n + kSpBlocks - 1) / kSpBlocks;
+    c0 = min(n, (int64_t)b * per);
+    c1 = min(n, c0 + per);
+}
+
+__global__ __launch_bounds__(256) void k_sp_count(const EnsSeg* __restrict__ segs,
+                                                  const int32_t* __restrict__ slots,
+                                                  uint32_t* blk) {
+    const EnsSeg& g = segs[slots[blockIdx.y]];
+    int64_t c0, c1;
+    sp_chunk(g, blockIdx.x, c0, c1);
+    uint32_t c = 0;
+    for (int64_t i = c0 + threadIdx.x; i < c1; i += blockDim.x) c += g.flim[i] != 0u;
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o);
+    __shared__ uint32_t ws[4];
+    if (lane_id() == 0) ws[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        blk[(size_t)blockIdx.y * kSpBlocks + blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+// exclusive scan of blk[0 .. n) in place, blk[n] = total (one block)
+__global__ __launch_bounds__(1024) void k_sp_scan(uint32_t* blk, int n) {
+    __shared__ uint32_t s[1024];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += 1024) {
+        const int i = base + (int)threadIdx.x;
+        const uint32_t v = i < n ? blk[i] : 0u;
+        s[threadIdx.x] = v;
+        __syncthreads();
+        for (int w = 1; w < 1024; w <<= 1) {
+            const uint32_t t = (int)threadIdx.x >= w ? s[threadIdx.x - w] : 0u;
+            __syncthreads();
+            s[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < n) blk[i] = carry + s[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += s[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) blk[n] = carry;
+}
+
+__global__ __launch_bounds__(256) void k_sp_write(const EnsSeg* __restrict__ segs,
+                                                  const int32_t* __restrict__ slots,
+                                                  const uint32_t* __restrict__ off,
+                                                  uint32_t* oidx, uint32_t* ocnt) {
+    const EnsSeg& g = segs[slots[blockIdx.y]];
+    int64_t c0, c1;
+    sp_chunk(g, blockIdx.x, c0, c1);
+    uint32_t base = off[(size_t)blockIdx.y * kSpBlocks + blockIdx.x];
+    __shared__ uint32_t wtot[4];
+    const int w = threadIdx.x >> 6;
+    for (int64_t t0 = c0; t0 < c1; t0 += blockDim.x) {
+        const int64_t i = t0 + threadIdx.x;
+        const uint32_t v = i < c1 ? g.flim[i] : 0u;
+        const unsigned long long m = __ballot(v != 0u);
+        const uint32_t below = (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull));
+        if (lane_id() == 0) wtot[w] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t before = 0;
+        for (int q = 0; q < w; ++q) before += wtot[q];
+        if (v != 0u) {
+            oidx[base + before + below] = (uint32_t)i;
+            ocnt[base + before + below] = v;
+        }
+        const uint32_t tile = wtot[0] + wtot[1] + wtot[2] + wtot[3];
+        __syncthreads();
+        base += tile;
+    }
+}
+
+// per listed replica: cumulative dark counts, afterpulses and detected photons
+__global__ void k_sp_meta(const int32_t* __restrict__ slots, int n,
+                          const unsigned long long* __restrict__ noise_ctr,
+                          const unsigned long long* __restrict__ ctr, unsigned long long* meta) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int r = slots[q];
+    meta[3 * q] = noise_ctr[2 * r];
+    meta[3 * q + 1] = noise_ctr[2 * r + 1];
+    meta[3 * q + 2] = ctr[8 * r + 7];
+}
+
+void ens_frames_free(edm_ensemble* e) {
+    for (auto& b : e->fbuf) {
+        void* dev[] = {b.d_slots, b.d_blk, b.d_idx, b.d_cnt, b.d_meta};
+        for (void* p : dev)
+            if (p) (void)hipFree(p);
+        void* host[] = {b.h_blk, b.h_idx, b.h_cnt, b.h_meta};
+        for (void* p : host)
+            if (p) (void)hipHostFree(p);
+        if (b.copied) (void)hipEventDestroy(b.copied);
+        if (b.written) (void)hipEventDestroy(b.written);
+        b = edm_ensemble::FrameBuf{};
+    }
+    if (e->copy_stream) (void)hipStreamDestroy(e->copy_stream);
+    e->copy_stream = nullptr;
+}
+
+static int sp_grow_slots(edm_ensemble::FrameBuf& b, int64_t n) {
+    if (n <= b.slot_cap) return 0;
+    const int64_t c = std::max<int64_t>(n, 64);
+    if (b.d_slots) EDM_HIP(hipFree(b.d_slots));
+    if (b.d_blk) EDM_HIP(hipFree(b.d_blk));
+    if (b.d_meta) EDM_HIP(hipFree(b.d_meta));
+    if (b.h_blk) EDM_HIP(hipHostFree(b.h_blk));
+    if (b.h_meta) EDM_HIP(hipHostFree(b.h_meta));
+    b.d_slots = nullptr;
+    b.d_blk = nullptr;
+    b.d_meta = nullptr;
+    b.h_blk = nullptr;
+    b.h_meta = nullptr;
+    EDM_HIP(hipMalloc(&b.d_slots, c * sizeof(int32_t)));
+    EDM_HIP(hipMalloc(&b.d_blk, (c * kSpBlocks + 1) * sizeof(uint32_t)));
+    EDM_HIP(hipMalloc(&b.d_meta, c * 3 * sizeof(unsigned long long)));
+    EDM_HIP(hipHostMalloc((void**)&b.h_blk, (c * kSpBlocks + 1) * sizeof(uint32_t)));
+    EDM_HIP(hipHostMalloc((void**)&b.h_meta, c * 3 * sizeof(unsigned long long)));
+    b.slot_cap = c;
+    return 0;
+}
+
+static int sp_grow_entries(edm_ensemble::FrameBuf& b, int64_t n) {
+    if (n <= b.cap) return 0;
+    const int64_t c = std::max<int64_t>(n + n / 2, 1 << 16);
+    if (b.d_idx) EDM_HIP(hipFree(b.d_idx));
+    if (b.d_cnt) EDM_HIP(hipFree(b.d_cnt));
+    if (b.h_idx) EDM_HIP(hipHostFree(b.h_idx));
+    if (b.h_cnt) EDM_HIP(hipHostFree(b.h_cnt));
+    b.d_idx = b.d_cnt = b.h_idx = b.h_cnt = nullptr;
+    EDM_HIP(hipMalloc(&b.d_idx, c * sizeof(uint32_t)));
+    EDM_HIP(hipMalloc(&b.d_cnt, c * sizeof(uint32_t)));
+    EDM_HIP(hipHostMalloc((void**)&b.h_idx, c * sizeof(uint32_t)));
+    EDM_HIP(hipHostMalloc((void**)&b.h_cnt, c * sizeof(uint32_t)));
+    b.cap = c;
+    return 0;
+}
+
+int ens_frames_collect(edm_ensemble* e, const int32_t* slots, int32_t n, int32_t* ticket) {
+    edm_ensemble::FrameBuf& b = e->fbuf[e->next_ticket & 1];
+    if (b.ticket >= 0) {
+        set_error("edm_ens_frames_collect: two collects are pending; fetch the older one first");
+        return EDM_ESTATE;
+    }
+    if (e->layout_dirty)
+        if (int rc = ens_layout(e)) return rc;
+    if (!e->copy_stream) EDM_HIP(hipStreamCreate(&e->copy_stream));
+    if (!b.copied) EDM_HIP(hipEventCreateWithFlags(&b.copied, hipEventDisableTiming));
+    if (!b.written) EDM_HIP(hipEventCreateWithFlags(&b.written, hipEventDisableTiming));
+    if (int rc = sp_grow_slots(b, n)) return rc;
+    hipStream_t st = e->stream;
+    EDM_HIP(hipMemcpyAsync(b.d_slots, slots, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice,
+                           st));
+    const int nblk = n * kSpBlocks;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_ens_frame_noise, dim3(32, n), dim3(kB), 0, st,
+                           (const EnsSeg*)e->d_seg, (const int32_t*)b.d_slots, e->noise_ctr,
+                           e->gstep);
+        hipLaunchKernelGGL(k_sp_count, dim3(kSpBlocks, n), dim3(256), 0, st,
+                           (const EnsSeg*)e->d_seg, (const int32_t*)b.d_slots, b.d_blk);
+    }
+    hipLaunchKernelGGL(k_sp_scan, dim3(1), dim3(1024), 0, st, b.d_blk, nblk);
+    EDM_HIP(hipMemcpyAsync(b.h_blk, b.d_blk, ((size_t)nblk + 1) * sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, st));
+    // the offsets size the entry buffers: wait for the noise and the counts
+    EDM_HIP(hipStreamSynchronize(st));
+    const int64_t total = b.h_blk[nblk];
+    if (int rc = sp_grow_entries(b, total)) return rc;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_sp_write, dim3(kSpBlocks, n), dim3(256), 0, st,
+                           (const EnsSeg*)e->d_seg, (const int32_t*)b.d_slots,
+                           (const uint32_t*)b.d_blk, b.d_idx, b.d_cnt);
+        hipLaunchKernelGGL(k_ens_frame_reset, dim3(64, n), dim3(kB), 0, st,
+                           (const EnsSeg*)e->d_seg, (const int32_t*)b.d_slots);
+        hipLaunchKernelGGL(k_sp_meta, dim3(nb(n)), dim3(kB), 0, st, (const int32_t*)b.d_slots, n,
+                           (const unsigned long long*)e->noise_ctr,
+                           (const unsigned long long*)e->ctr, b.d_meta);
+    }
+    EDM_HIP(hipGetLastError());
+    // the copy stream moves the entries once the compute stream has written
+    // them; the compute stream goes on with the next steps meanwhile
+    EDM_HIP(hipEventRecord(b.written, st));
+    EDM_HIP(hipStreamWaitEvent(e->copy_stream, b.written, 0));
+    if (total > 0) {
+        EDM_HIP(hipMemcpyAsync(b.h_idx, b.d_idx, (size_t)total * 4, hipMemcpyDeviceToHost,
+                               e->copy_stream));
+        EDM_HIP(hipMemcpyAsync(b.h_cnt, b.d_cnt, (size_t)total * 4, hipMemcpyDeviceToHost,
+                               e->copy_stream));
+    }
+    if (n > 0)
+        EDM_HIP(hipMemcpyAsync(b.h_meta, b.d_meta, (size_t)n * 3 * sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, e->copy_stream));
+    EDM_HIP(hipEventRecord(b.copied, e->copy_stream));
+    b.slots.assign(slots, slots + n);
+    b.ticket = e->next_ticket++;
+    *ticket = b.ticket;
+    return 0;
+}
+
+int ens_frames_fetch(edm_ensemble* e, int32_t ticket, int64_t* counts, int64_t* meta,
+                     uint32_t* idx, uint32_t* cnt, int64_t cap, int64_t* total) {
+    edm_ensemble::FrameBuf& b = e->fbuf[ticket & 1];
+    if (ticket < 0 || b.ticket != ticket) {
+        set_error("edm_ens_frames_fetch: no pending collect with that ticket");
+        return EDM_ESTATE;
+    }
+    EDM_HIP(hipEventSynchronize(b.copied));
+    const int n = (int)b.slots.size();
+    const int64_t tot = b.h_blk[(size_t)n * kSpBlocks];
+    if (total) *total = tot;
+    if (idx || cnt) {
+        if (cap < tot) {
+            set_error("edm_ens_frames_fetch: output holds " + std::to_string(cap) +
+                      " entries, the frames have " + std::to_string(tot));
+            return EDM_EINVAL;
+        }
+        if (idx) std::memcpy(idx, b.h_idx, (size_t)tot * 4);
+        if (cnt) std::memcpy(cnt, b.h_cnt, (size_t)tot * 4);
+    }
+    for (int q = 0; q < n; ++q) {
+        if (counts)
+            counts[q] = (int64_t)b.h_blk[(size_t)(q + 1) * kSpBlocks] -
+                        (int64_t)b.h_blk[(size_t)q * kSpBlocks];
+        if (meta)
+            for (int k = 0; k < 3; ++k) meta[3 * q + k] = (int64_t)b.h_meta[3 * q + k];
+    }
+    // a call without output arrays only reports the sizes; one with them
+    // releases the buffer for a later collect
+    if (idx || cnt) b.ticket = -1;
+    return 0;
+}

@@ -158,6 +158,26 @@ struct edm_ensemble {
     std::vector<EnsChunk> wl_chunks;           // the list build's blocks
     EnsChunk* d_wl_chunks = nullptr;
     int64_t wl_chunk_cap = 0;
+
+    // Streamed frames (edm_ens_frames_collect / edm_ens_frames_fetch): two
+    // buffers used in turn.  A collect compacts the due stacks into the
+    // device buffer and a copy stream moves the entries to the pinned host
+    // buffer while the compute stream runs on; the fetch waits for that copy.
+    struct FrameBuf {
+        int32_t ticket = -1;   // the collect it holds, -1: free
+        int32_t* d_slots = nullptr;
+        uint32_t* d_blk = nullptr;  // per (slot, block): nonzero bins, then offsets
+        uint32_t *d_idx = nullptr, *d_cnt = nullptr;
+        unsigned long long* d_meta = nullptr;  // per slot: dark, afterpulses, detected
+        int64_t slot_cap = 0, cap = 0;
+        uint32_t* h_blk = nullptr;  // pinned
+        uint32_t *h_idx = nullptr, *h_cnt = nullptr;
+        unsigned long long* h_meta = nullptr;
+        std::vector<int32_t> slots;
+        hipEvent_t written = nullptr, copied = nullptr;
+    } fbuf[2];
+    hipStream_t copy_stream = nullptr;
+    int32_t next_ticket = 0;
 };
 
 namespace edm {
@@ -168,6 +188,10 @@ int ens_photons(edm_ensemble* e);
 int ens_frame_noise(edm_ensemble* e, int slot, int64_t* n_dark, int64_t* n_ap);
 int ens_frame_reset(edm_ensemble* e, int slot);
 int ens_frame_end(edm_ensemble* e, const int32_t* slots, int32_t n);
+int ens_frames_collect(edm_ensemble* e, const int32_t* slots, int32_t n, int32_t* ticket);
+int ens_frames_fetch(edm_ensemble* e, int32_t ticket, int64_t* counts, int64_t* meta,
+                     uint32_t* idx, uint32_t* cnt, int64_t cap, int64_t* total);
+void ens_frames_free(edm_ensemble* e);
 void plan_passes(const std::vector<edm_reaction>& rx, uint32_t sensor_mask,
                  std::vector<std::vector<int>>& groups, std::vector<int>& group_start,
                  std::vector<int>& deferred);

@@ -176,6 +176,9 @@ _SIGS = {
                       ctypes.POINTER(c_int64)],
     "edm_ens_frame_end": [c_void_p, c_void_p, c_int32],
     "edm_ens_noise_counters": [c_void_p, c_int32, c_void_p],
+    "edm_ens_frames_collect": [c_void_p, c_void_p, c_int32, ctypes.POINTER(c_int32)],
+    "edm_ens_frames_fetch": [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_int64,
+                             ctypes.POINTER(c_int64)],
     "edm_ens_download": [c_void_p, c_int32, ctypes.POINTER(HostState)],
     "edm_ens_counters": [c_void_p, c_int32, c_void_p],
     "edm_ens_info": [c_void_p, c_void_p],

@@ -24,7 +24,7 @@ from . import _hip
 from ._hip import check, lib, ptr
 from .config import ConfigError, SimulationConfig
 from .dynamics import _Plan, build_params, derive_constants
-from .microscope import FLIMFrame, image_params
+from .microscope import FLIMFrame, SparseFrame, image_params
 from .particles import ParticleState
 from .photophysics import photo_params
 
@@ -81,6 +81,9 @@ class EnsembleEngine:
         self._n_ch = ip.n_ch if ip is not None else 0
         self._n_bins = ip.n_bins if ip is not None else 0
         self.replicas: Dict[int, Dict] = {}
+        self._tickets: Dict[int, List] = {}   # pending collects: (slot, uid, shape, frame)
+        self._frame_meta: Dict[int, tuple] = {}  # per replica uid: cumulative dark, AP, detected
+        self._uid = 0
 
     # ---- membership --------------------------------------------------------
     def add(self, config: SimulationConfig, state: Optional[ParticleState] = None) -> int:
@@ -129,7 +132,8 @@ class EnsembleEngine:
               "edm_ens_add")
         k = int(slot.value)
         self._pooled += st.N
-        self.replicas[k] = dict(config=config, N=st.N, shape=shape, frame_index=0,
+        self._uid += 1
+        self.replicas[k] = dict(config=config, N=st.N, shape=shape, frame_index=0, uid=self._uid,
                                 frame_every=max(1, int(round(config.scan.frame_interval /
                                                              config.bd_timestep))))
         return k
@@ -190,6 +194,51 @@ class EnsembleEngine:
         check(lib().edm_ens_frame_end(self._h, ptr(sl), int(sl.size)), "edm_ens_frame_end")
         for k in sl.tolist():
             self.replicas[k]["frame_index"] += 1
+
+    def collect_frames(self, slots) -> int:
+        """End the current frame of several replicas and stream every one
+        back (edm_ens_frames_collect): detector noise, compaction of each
+        stack's nonzero bins on the device, reset; the copy to host memory
+        runs while later steps do.  Returns a ticket for fetch_frames; at
+        most two collects may be pending."""
+        sl = np.ascontiguousarray(np.asarray(list(slots), dtype=np.int32))
+        t = ctypes.c_int32(-1)
+        check(lib().edm_ens_frames_collect(self._h, ptr(sl), int(sl.size), ctypes.byref(t)),
+              "edm_ens_frames_collect")
+        rec = []
+        for k in sl.tolist():
+            info = self.replicas[k]
+            rec.append((k, info["uid"], info["shape"], info["frame_index"]))
+            info["frame_index"] += 1
+        self._tickets[int(t.value)] = rec
+        return int(t.value)
+
+    def fetch_frames(self, ticket: int) -> Dict[int, SparseFrame]:
+        """The frames of a collect, by slot (waits for their copy)."""
+        rec = self._tickets.pop(ticket)
+        n = len(rec)
+        counts = np.zeros(max(n, 1), dtype=np.int64)
+        meta = np.zeros((max(n, 1), 3), dtype=np.int64)
+        total = ctypes.c_int64(0)
+        check(lib().edm_ens_frames_fetch(self._h, int(ticket), ptr(counts), ptr(meta), None,
+                                         None, 0, ctypes.byref(total)), "edm_ens_frames_fetch")
+        tot = int(total.value)
+        idx = np.empty(max(tot, 1), dtype=np.uint32)
+        cnt = np.empty(max(tot, 1), dtype=np.uint32)
+        check(lib().edm_ens_frames_fetch(self._h, int(ticket), None, None, ptr(idx), ptr(cnt),
+                                         idx.size, ctypes.byref(total)), "edm_ens_frames_fetch")
+        out: Dict[int, SparseFrame] = {}
+        off = 0
+        for q, (k, uid, shape, fi) in enumerate(rec):
+            c = int(counts[q])
+            last = self._frame_meta.get(uid, (0, 0, 0))
+            m = tuple(int(v) for v in meta[q])
+            self._frame_meta[uid] = m
+            out[k] = SparseFrame(shape, idx[off:off + c].copy(), cnt[off:off + c].copy(), fi,
+                                 n_signal_photons=m[2] - last[2], n_dark_counts=m[0] - last[0],
+                                 n_afterpulses=m[1] - last[1])
+            off += c
+        return out
 
     def noise_counters(self, slot: int) -> Dict[str, int]:
         """Dark counts and afterpulses added by end_frames so far."""

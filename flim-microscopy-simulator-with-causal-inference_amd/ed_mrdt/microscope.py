@@ -136,6 +136,43 @@ class FLIMFrame:
         return self.flim_stack.shape
 
 
+@dataclass
+class SparseFrame:
+    """One FLIM frame as its nonzero TCSPC bins: ``flat_index`` into the
+    stack ``[n_ch, nx, ny, n_bins]`` (C order, ascending) and ``count``.  A
+    served or batched job streams its frames back in this form;
+    ``to_frame()`` gives the reference's dense ``FLIMFrame``."""
+    shape: Tuple[int, int, int, int]
+    flat_index: np.ndarray
+    count: np.ndarray
+    frame_index: int = 0
+    n_signal_photons: int = 0
+    n_dark_counts: int = 0
+    n_afterpulses: int = 0
+
+    @staticmethod
+    def from_stack(flim_stack, frame_index=0, n_signal=0, n_dark=0, n_ap=0) -> "SparseFrame":
+        flat = np.ascontiguousarray(flim_stack).reshape(-1)
+        nz = np.flatnonzero(flat)
+        return SparseFrame(tuple(int(v) for v in flim_stack.shape), nz.astype(np.uint32),
+                           flat[nz].astype(np.uint32), frame_index, n_signal, n_dark, n_ap)
+
+    def to_frame(self) -> FLIMFrame:
+        stack = np.zeros(self.shape, dtype=np.uint32)
+        stack.reshape(-1)[self.flat_index.astype(np.int64)] = self.count
+        return FLIMFrame.from_stack(stack, frame_index=self.frame_index,
+                                    n_signal=self.n_signal_photons, n_dark=self.n_dark_counts,
+                                    n_ap=self.n_afterpulses)
+
+    @property
+    def total_counts(self) -> int:
+        return int(self.count.sum(dtype=np.int64))
+
+    def same_counts(self, other: "SparseFrame") -> bool:
+        return (self.shape == other.shape and np.array_equal(self.flat_index, other.flat_index)
+                and np.array_equal(self.count, other.count))
+
+
 def image_params(config: SimulationConfig, psf: "PSF" = None, scanner: "ConfocalScanner" = None,
                  tcspc: "TCSPCAccumulator" = None,
                  detector: "DetectorModel" = None) -> _hip.ImageParams:

@@ -6,15 +6,18 @@ as soon as its device memory fits the HBM budget, its particles fit the
 pool and a replica slot is free; admitted jobs become replicas of one
 resident ``EnsembleEngine`` and advance together.  The loop runs the
 resident replicas up to the next event (a job reaching its step count or a
-frame boundary), ends the frames that are due on the device in one batch,
-streams the finished jobs' final FLIM stacks back, and frees their memory,
-which lets waiting jobs in.  A job's result depends only on its own
-configuration (counter-based draws keyed per job), so it is the same alone
-and in any mix.
+frame boundary) and streams back every frame that is due: the frames are
+ended and compacted to their nonzero TCSPC bins on the device in one batch,
+and their copy to host memory runs while the next batch of steps does (the
+reference keeps every frame of a run, pipeline.py:645).  A job is complete
+once its last frame has arrived; finished jobs free their memory, which
+lets waiting jobs in.  A job's result depends only on its own configuration
+(counter-based draws keyed per job), so it is the same alone and in any
+mix.
 
 ``serve_plain`` runs the same jobs one after another on the single-
-membrane path (one job per launch sequence), the baseline of the batched
-figure.
+membrane path (one job per launch sequence), copying each frame out whole,
+the baseline of the batched figure.
 """
 
 from __future__ import annotations
@@ -29,7 +32,7 @@ import numpy as np
 
 from .config import SimulationConfig
 from .ensemble import EnsembleEngine, replica_bytes
-from .microscope import FLIMFrame
+from .microscope import FLIMFrame, SparseFrame
 
 
 @dataclass
@@ -42,6 +45,9 @@ class Job:
 
 @dataclass
 class JobResult:
+    """A served job: every frame in ``frame_list`` (``SparseFrame`` from the
+    server, ``FLIMFrame`` from the plain path), ``frame`` the last one as a
+    dense ``FLIMFrame``."""
     job_id: int
     arrival: float
     admitted: float
@@ -49,13 +55,25 @@ class JobResult:
     steps: int
     particles: int
     frames: int
-    frame: Optional[FLIMFrame] = None
+    frame_list: List = field(default_factory=list)
     counters: Dict[str, int] = field(default_factory=dict)
     state: Optional[Dict[str, np.ndarray]] = None
 
     @property
     def latency(self) -> float:
         return self.finished - self.arrival
+
+    @property
+    def frame(self) -> Optional[FLIMFrame]:
+        if not self.frame_list:
+            return None
+        f = self.frame_list[-1]
+        return f if isinstance(f, FLIMFrame) else f.to_frame()
+
+    def sparse_frames(self) -> List[SparseFrame]:
+        return [f if isinstance(f, SparseFrame) else SparseFrame.from_stack(
+            f.flim_stack, f.frame_index, f.n_signal_photons, f.n_dark_counts, f.n_afterpulses)
+            for f in self.frame_list]
 
 
 def n_particles(config: SimulationConfig) -> int:
@@ -109,13 +127,30 @@ class Server:
               progress: Optional[Callable[[str], None]] = None) -> List[JobResult]:
         """Serve the jobs; with realtime, a job is not admitted before its
         arrival time (measured from the call).  Returns results in
-        completion order."""
+        completion order; a job completes when its last frame is on the
+        host."""
         eng = self.engine
         pending = deque(sorted(jobs, key=lambda j: (j.arrival, j.job_id)))
         resident: Dict[int, Dict] = {}
         results: List[JobResult] = []
+        frames_of: Dict[int, List] = {}   # job id -> frames received so far
+        collect = None  # in flight: (ticket, {slot: job id}, finished jobs' records)
         t0 = clock()
-        while pending or resident:
+
+        def fetch(c):
+            ticket, job_of, done_jobs = c
+            got = eng.fetch_frames(ticket)
+            for slot, fr in got.items():
+                frames_of[job_of[slot]].append(fr)
+            for slot, r in done_jobs:
+                job = r["job"]
+                fl = frames_of.pop(job.job_id)
+                results.append(JobResult(job.job_id, job.arrival, r["admitted"], clock() - t0,
+                                         job.steps, r["n"], len(fl), fl, r["ctr"], r["state"]))
+                if progress:
+                    progress(f"finished job {job.job_id} at {clock() - t0:.2f} s")
+
+        while pending or resident or collect is not None:
             now = clock() - t0
             while pending and (not realtime or pending[0].arrival <= now):
                 job = pending[0]
@@ -128,12 +163,16 @@ class Server:
                 n = n_particles(job.config)
                 resident[slot] = dict(job=job, done=0, bytes=b, n=n,
                                       fe=frame_steps(job.config), admitted=clock() - t0)
+                frames_of[job.job_id] = []
                 if progress:
                     progress(f"admitted job {job.job_id} ({n} particles, {job.steps} steps) "
                              f"at {clock() - t0:.2f} s, {len(resident)} resident")
             self.peak_resident = max(self.peak_resident, len(resident))
             if not resident:
-                if pending and (not realtime or pending[0].arrival <= clock() - t0):
+                if collect is not None:
+                    fetch(collect)
+                    collect = None
+                elif pending and (not realtime or pending[0].arrival <= clock() - t0):
                     # the head job has arrived but waits for the space of
                     # removed jobs: one empty step returns it
                     eng.run(1)
@@ -147,28 +186,30 @@ class Server:
                 k = min(k, r["job"].steps - r["done"], r["fe"] - r["done"] % r["fe"])
             eng.run(k)
             self.launch_batches += 1
-            finished, ends = [], []
+            # the previous batch's frames: their copy ran beside these steps
+            if collect is not None:
+                fetch(collect)
+                collect = None
+            finished, due = [], []
             for slot, r in resident.items():
                 r["done"] += k
                 if r["done"] >= r["job"].steps:
                     finished.append(slot)
+                    due.append(slot)
                 elif r["done"] % r["fe"] == 0:
-                    ends.append(slot)
-            if ends:
-                eng.end_frames(ends)
-            for slot in finished:
-                r = resident.pop(slot)
-                fr = eng.frame(slot, download=True)
-                ctr = eng.counters(slot)
-                st = eng.state(slot) if self.keep_state else None
-                frames = eng.replicas[slot]["frame_index"]
-                eng.remove(slot)
-                self.used_bytes -= r["bytes"]
-                job = r["job"]
-                results.append(JobResult(job.job_id, job.arrival, r["admitted"], clock() - t0,
-                                         job.steps, r["n"], frames, fr, ctr, st))
-                if progress:
-                    progress(f"finished job {job.job_id} at {clock() - t0:.2f} s")
+                    due.append(slot)
+            if due:
+                ticket = eng.collect_frames(due)
+                job_of = {slot: resident[slot]["job"].job_id for slot in due}
+                done_jobs = []
+                for slot in finished:
+                    r = resident.pop(slot)
+                    r["ctr"] = eng.counters(slot)
+                    r["state"] = eng.state(slot) if self.keep_state else None
+                    eng.remove(slot)
+                    self.used_bytes -= r["bytes"]
+                    done_jobs.append((slot, r))
+                collect = (ticket, job_of, done_jobs)
         return results
 
 
@@ -177,8 +218,8 @@ def serve_plain(jobs: List[Job], realtime: bool = True,
                 keep_state: bool = False,
                 progress: Optional[Callable[[str], None]] = None) -> List[JobResult]:
     """The same jobs one at a time, first come first served, each on the
-    single-membrane path (FP32, Philox, fused imaging), with its frames ended
-    on schedule and its final frame copied out."""
+    single-membrane path (FP32, Philox, fused imaging), every frame ended on
+    schedule and copied out (VirtualMicroscope.finalize_frame)."""
     from . import _hip
     from .dynamics import DynamicsEngine
     from .microscope import VirtualMicroscope
@@ -202,21 +243,17 @@ def serve_plain(jobs: List[Job], realtime: bool = True,
         mic = VirtualMicroscope(cfg, st)
         fe = frame_steps(cfg)
         done = 0
-        frames = 0
+        frame_list = []
         while done < job.steps:
             k = min(fe - done % fe, job.steps - done)
             _hip.check(L.edm_time_steps(eng.handle, k, 1, ctypes.byref(tot), ctypes.byref(fms),
                                         None))
             done += k
-            if done < job.steps and done % fe == 0:
-                _hip.check(L.edm_frame_noise(eng.handle, None, None))
-                _hip.check(L.edm_frame_reset(eng.handle))
-                frames += 1
+            if done % fe == 0 or done == job.steps:
+                frame_list.append(mic.finalize_frame())
         # the steps ran through the C ABI: the host mirrors are stale
         st.mark_stale()
         eng._ctr_cache = None
-        fr = mic.finalize_frame()
-        frames += 1
         ctr = dict(n_catalytic=eng.n_catalytic, n_recruited=eng.n_recruited,
                    n_detached=eng.n_detached, n_converted=eng.n_converted)
         state = None
@@ -225,7 +262,7 @@ def serve_plain(jobs: List[Job], realtime: bool = True,
                          is_alive=st.is_alive.copy())
         eng.close()
         results.append(JobResult(job.job_id, job.arrival, admitted, clock() - t0, job.steps,
-                                 st.N, frames, fr, ctr, state))
+                                 st.N, len(frame_list), frame_list, ctr, state))
         if progress:
             progress(f"plain job {job.job_id}: {st.N} particles, {job.steps} steps, "
                      f"done at {clock() - t0:.2f} s")

@@ -488,6 +488,22 @@ int edm_ens_frame(edm_ensemble *e, int32_t slot, uint32_t *flim, int64_t n_elems
  * the noise counts accumulate per replica (edm_ens_noise_counters) */
 int edm_ens_frame_end(edm_ensemble *e, const int32_t *slots, int32_t n_slots);
 int edm_ens_noise_counters(edm_ensemble *e, int32_t slot, int64_t out[2]);
+/* Frame end of several replicas with every frame returned (the reference
+ * keeps each frame's FLIMFrame, pipeline.py:645): detector noise, then each
+ * stack's nonzero bins as (flat index into [n_ch, nx, ny, n_bins], count)
+ * pairs in index order, replica after replica in list order, then the
+ * reset.  The pairs travel to pinned host memory on a copy stream while the
+ * compute stream runs the next steps; edm_ens_frames_fetch waits for that
+ * copy and returns counts[n_slots] (pairs per replica), meta[n_slots][3]
+ * (cumulative dark counts, afterpulses and detected photons after the frame
+ * end) and the pairs (cap >= *total).  A fetch with flat_index and count
+ * both NULL only waits and reports the sizes; one with them copies the pairs
+ * out and ends the collect.  At most two collects may wait for their
+ * fetch. */
+int edm_ens_frames_collect(edm_ensemble *e, const int32_t *slots, int32_t n_slots,
+                           int32_t *ticket);
+int edm_ens_frames_fetch(edm_ensemble *e, int32_t ticket, int64_t *counts, int64_t *meta,
+                         uint32_t *flat_index, uint32_t *count, int64_t cap, int64_t *total);
 int edm_ens_download(edm_ensemble *e, int32_t slot, edm_host_state *s);
 /* the replica's counters: bindings, dissociations, catalytic, recruited,
  * detached, converted, photons emitted, photons detected */

@@ -37,6 +37,18 @@ class _Engine:
             self.replicas[s]["frame_index"] += 1
         self.log.append(("end", tuple(sorted(slots)), self.step))
 
+    def collect_frames(self, slots):
+        self.end_frames(slots)
+        self.tickets = getattr(self, "tickets", {})
+        t = len(self.tickets)
+        self.tickets[t] = {s: ("frame", s, self.replicas[s]["frame_index"] - 1) for s in slots}
+        assert sum(1 for v in self.tickets.values() if v is not None) <= 2
+        return t
+
+    def fetch_frames(self, ticket):
+        got, self.tickets[ticket] = self.tickets[ticket], None
+        return got
+
     def frame(self, slot, download=True):
         self.replicas[slot]["frame_index"] += 1
         return None
@@ -84,9 +96,13 @@ def test_schedule_respects_budget_fifo_and_frame_boundaries(monkeypatch):
     for r in res:
         assert r.counters["steps"] == r.steps
     assert all(e[1] <= 64 for e in eng.log if e[0] == "run")
-    # frames: one per 100 steps (frame_interval 0.1 s at dt = 1 ms) plus the final one
+    # frames: one per 100 steps (frame_interval 0.1 s at dt = 1 ms) plus the final
+    # one, every one of them returned, in order
     for r in res:
-        assert r.frames == (r.steps - 1) // 100 + 1
+        assert r.frames == (r.steps - 1) // 100 + 1 == len(r.frame_list)
+        assert [f[2] for f in r.frame_list] == list(range(r.frames))
+    # a job completes only once its last frame was fetched, after the next launch
+    assert all(r.finished >= r.admitted for r in res)
 
 
 def test_oversized_job_is_rejected():
