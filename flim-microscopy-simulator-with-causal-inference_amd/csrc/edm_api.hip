@@ -119,6 +119,7 @@ int edm_destroy(edm_membrane* m) {
     if (!m) return 0;
     if (m->stream) hipStreamSynchronize(m->stream);
     rec_free(m);
+    stats_free(m);
     void* bufs[] = {m->a.x, m->a.y, m->a.xn, m->a.yn, m->a.xt, m->a.yt, m->a.sp, m->a.sp_t, m->a.fl,
                     m->a.fl_t, m->a.id, m->a.id_t, m->a.bond, m->a.bond_t, m->a.ctmc, m->a.ctmc_t,
                     m->a.dye, m->a.dye_t, m->a.slot_of, m->key, m->rank, m->cell_cnt,

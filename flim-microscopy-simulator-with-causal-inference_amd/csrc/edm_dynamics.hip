@@ -1941,7 +1941,10 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
 }
 
 int dyn_step_finish(edm_membrane* m, const double* d_draws_dev, bool hook) {
-    return m->fp64 ? finish_t<double>(m, d_draws_dev, hook) : finish_t<float>(m, d_draws_dev, hook);
+    const int rc = m->fp64 ? finish_t<double>(m, d_draws_dev, hook)
+                           : finish_t<float>(m, d_draws_dev, hook);
+    if (rc || !m->st_acc) return rc;
+    return stats_accumulate(m);
 }
 
 template <typename Real>

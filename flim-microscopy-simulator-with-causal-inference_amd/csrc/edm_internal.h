@@ -239,6 +239,16 @@ struct edm_membrane {
     edm_census* cen_row = nullptr;
     double *gt_e = nullptr, *gt_tau = nullptr;
 
+    // observables (edm_stats_*): per particle index, the unwrapped
+    // displacement since edm_stats_open and the position after the last step
+    double* st_acc = nullptr;  // [n][2]
+    void* st_prev = nullptr;   // Real [n][2]
+    double* st_part = nullptr;  // block partial sums
+    unsigned long long* st_hist = nullptr;
+    int st_hist_cap = 0;
+    double* st_pix = nullptr;  // per-pixel sums of the FLIM reduction
+    size_t st_pix_cap = 0;
+
     // timing
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> kev;
@@ -281,4 +291,7 @@ int stage_draws(edm_membrane* m, const double* host, int64_t n);
 // census translation unit
 int rec_step(edm_membrane* m);  // one census row into the ring (no sync)
 void rec_free(edm_membrane* m);
+// observables translation unit
+int stats_accumulate(edm_membrane* m);  // after a step (no sync)
+void stats_free(edm_membrane* m);
 }  // namespace edm

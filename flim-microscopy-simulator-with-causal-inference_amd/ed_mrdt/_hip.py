@@ -183,6 +183,13 @@ _SIGS = {
     "edm_ens_counters": [c_void_p, c_int32, c_void_p],
     "edm_ens_info": [c_void_p, c_void_p],
     "edm_ens_synchronize": [c_void_p],
+    # observables
+    "edm_stats_open": [c_void_p],
+    "edm_stats_close": [c_void_p],
+    "edm_stats_msd": [c_void_p, ctypes.c_uint32, ctypes.POINTER(c_double),
+                      ctypes.POINTER(c_int64)],
+    "edm_stats_pair_hist": [c_void_p, ctypes.c_uint32, c_int32, c_double, c_void_p],
+    "edm_stats_flim": [c_void_p, c_int32, c_int32, c_void_p],
     # device memory for the free kernels
     "edm_device_alloc": [c_int64, ctypes.POINTER(c_void_p)],
     "edm_device_free": [c_void_p],

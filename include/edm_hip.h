@@ -442,6 +442,34 @@ int edm_density_field(const double *positions, const int32_t *species_id, const 
 int edm_adaptation_update(double *adaptation_field, const double *density_field, int32_t nx,
                           int32_t ny, double dt, double tau_w, void *stream);
 
+/* ---- Observables on the device ------------------------------------------
+ * The statistics the reference computes on the host from copied state.
+ *   edm_stats_open       starts (or restarts) the mean-squared-displacement
+ *                        accumulators: from here on every step adds each
+ *                        particle's minimum-image displacement (the unwrapping
+ *                        of validation.py:276-290, B1)
+ *   edm_stats_msd        sum over the alive particles of the species in
+ *                        species_mask (bit s = species s) of |r(t) - r(0)|^2,
+ *                        and their number (MSD = sum / count)
+ *   edm_stats_pair_hist  unordered pairs of alive particles of the species in
+ *                        species_mask by distance, bins of r_max / n_bins
+ *                        (minimum image): the counts behind g(r) of
+ *                        spatial_correlation_ground_truth (analysis.py:
+ *                        733-790, which counts each pair twice)
+ *   edm_stats_flim       for each pixel of the current frame's channel,
+ *                        out[pixel][4] = sum I, sum I cos(phi_k), sum I
+ *                        sin(phi_k), sum I t_k (ns) over the TCSPC bins k,
+ *                        phi_k = 2 pi harmonic (k + 0.5) / n_bins, t_k the bin
+ *                        centre: mean_arrival_time (analysis.py:87-127) and
+ *                        phasor_transform (analysis.py:497-566) are ratios
+ *                        of these */
+int edm_stats_open(edm_membrane *m);
+int edm_stats_close(edm_membrane *m);
+int edm_stats_msd(edm_membrane *m, uint32_t species_mask, double *sum_r2, int64_t *count);
+int edm_stats_pair_hist(edm_membrane *m, uint32_t species_mask, int32_t n_bins, double r_max,
+                        uint64_t *counts);
+int edm_stats_flim(edm_membrane *m, int32_t channel, int32_t harmonic, double *out);
+
 /* ---- Batched ensemble ---------------------------------------------------
  * Many independent membranes of one model advanced by the same launches
  * (the reference runs replicas one after another: one SimulationPipeline

@@ -1,127 +1,116 @@
 """Long-run statistics of the fast path (FP32, Philox draws) against the
-oracle (FP64, the reference's numpy draws): independent replicas compared
-through ensemble means of MSD, pair-distance counts (the RDF's histogram),
-species-count trajectories and, with imaging, the phasor of the TCSPC
-decay.  Two systems: the binding / FRET system with imaging (24 replicas x
-100 steps) and the dense harmonic membrane of the 1M workload at 14 x 14 um
-(12 replicas x 60 steps), whose stiff, displacement-capped dynamics make
-FP32 and FP64 trajectories decorrelate within a few tens of steps, so that
-only distributions can agree.
+oracle's ensembles (FP64, the reference's numpy draws), with the
+statistics computed on the device (ed_mrdt.observables: MSD with
+periodic unwrapping, pair-distance histogram, TCSPC moments) and the
+species counts read from the state.
 
-Bar: every observable's two ensemble means agree within 4.5 combined
-standard errors (Welch), i.e. within the ensembles' confidence intervals.
-The initial states are the same per seed on both sides (bit-exact
-initialisation).
+Systems (tests/ensemble_stats.py): the binding / FRET system with imaging,
+the 13,000-particle wave system with its sinusoidal D(x) and 128 x 128 px
+imaging, and the dense harmonic membrane of the 1M workload at 14 x 14 um,
+whose displacement-capped dynamics decorrelate FP32 from FP64 within tens
+of steps.  32 replicas per side, 1000 steps each; MSD at 100, 250, 500 and
+1000 steps, species counts at 250, 500 and 1000, pair distances at the end,
+phasor and mean arrival time over every frame.
+
+Bar: for every observable o of a case, the two ensemble means agree within
+the 95 % confidence interval of their difference, Bonferroni-corrected for
+the K observables of the case (the family-wise error stays 5 %):
+|mean_gpu - mean_oracle| <= z(1 - 0.025 / K) * sqrt(se_gpu^2 + se_oracle^2).
+The oracle's ensembles are fixtures written by
+tests/golden/make_ensemble_stats.py.
+
+Power: with 32 replicas a side the bar's half-width on the MSD at 1000
+steps is about 1.4 % of the MSD per unit z (z = 3.2 for the 36 observables
+of the binding case), so a diffusion coefficient off by 5 % (a dropped
+sqrt(2) is 41 %) or a binding probability off by a quarter fails it; the
+pair bins catch a contact radius or a WCA sigma off by a few per cent.
 """
 
 from __future__ import annotations
 
+import ctypes
+import os
+
 import numpy as np
 import pytest
-from scipy.spatial import cKDTree
+from scipy.stats import norm
 
-import reference_model as rm
-from ed_mrdt import workloads
+import ensemble_stats as es
+from ed_mrdt import _hip
 from ed_mrdt.dynamics import DynamicsEngine
 from ed_mrdt.microscope import VirtualMicroscope
+from ed_mrdt.observables import MSDTracker, flim_moments, pair_distance_histogram
 from ed_mrdt.particles import ParticleState
 from ed_mrdt.photophysics import PhotonSimulator
 
 pytestmark = pytest.mark.gpu
 
-CASES = {
-    # name: (config of a seed, seeds, steps, imaging, pair-distance bin edges in um)
-    "binding": (lambda s: workloads.binding_fret(seed=s), 24, 100, True,
-                np.array([0.0, 0.02, 0.05, 0.1, 0.2, 0.4])),
-    "dense_harmonic": (lambda s: workloads.large_membrane(seed=s, side=14.0, n_alive=19_600,
-                                                          pixels=64), 12, 60, False,
-                       np.array([0.0, 0.005, 0.01, 0.02, 0.05, 0.1])),
-}
+FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                       "oracle_ensemble_stats.npz")
 
 
-def _observables(cfg, edges, pos0, alive0, pos, alive, species, flim, counts_mid):
-    L = np.asarray(cfg.membrane.size, dtype=np.float64)
-    keep = alive0 & alive
-    d = pos[keep] - pos0[keep]
-    d -= L * np.round(d / L)  # periodic box, displacements well below L / 2
-    msd = float(np.mean(np.sum(d * d, axis=1)))
-    tree = cKDTree(np.mod(pos[alive], L), boxsize=L)
-    cum = tree.count_neighbors(tree, edges).astype(np.float64)
-    rdf = np.diff(cum) / 2.0  # unordered pairs per distance bin
-    n_sp = len(cfg.species)
-    counts_end = np.bincount(species[alive], minlength=n_sp).astype(np.float64)
-    out = [[msd], rdf, counts_mid, counts_end]
-    if flim is not None:
-        h = flim.reshape(-1, flim.shape[-1]).sum(axis=0).astype(np.float64)
-        ph = 2.0 * np.pi * (np.arange(h.size) + 0.5) / h.size
-        out.append([float(np.sum(h * np.cos(ph)) / h.sum()),
-                    float(np.sum(h * np.sin(ph)) / h.sum())])
-    return np.concatenate(out)
+def _run_steps(eng, k, imaging):
+    tot, fms = ctypes.c_double(0), ctypes.c_double(0)
+    _hip.check(_hip.lib().edm_time_steps(eng.handle, int(k), 1 if imaging else 0,
+                                         ctypes.byref(tot), ctypes.byref(fms), None))
 
 
-def _gpu_replica(case, seed):
-    make, _, steps, imaging, edges = CASES[case]
+def gpu_replica(case, seed):
+    make, imaging, nb, r_max = es.CASES[case]
     cfg = make(seed)
     st = ParticleState(cfg)
-    pos0, alive0 = st.positions.copy(), st.is_alive.astype(bool).copy()
     eng = DynamicsEngine(cfg, st, precision="fp32", rng="philox")
     if imaging:
-        ph = PhotonSimulator(cfg, st, min(1.0, cfg.scan.dwell_time / cfg.scan.frame_interval))
+        PhotonSimulator(cfg, st, min(1.0, cfg.scan.dwell_time / cfg.scan.frame_interval))
         mic = VirtualMicroscope(cfg, st)
-    mid = None
-    for k in range(steps):
-        eng.step()
-        if imaging:
-            mic.process_photons(ph.evolve_one_bd_step())
-        if k + 1 == steps // 2:
-            a = st.is_alive.astype(bool)
-            mid = np.bincount(st.species_id[a], minlength=len(cfg.species)).astype(np.float64)
-    flim = mic.finalize_frame().flim_stack if imaging else None
-    return _observables(cfg, edges, pos0, alive0, st.positions.astype(np.float64),
-                        st.is_alive.astype(bool), st.species_id, flim, mid)
-
-
-def _oracle_replica(case, seed):
-    make, _, steps, imaging, edges = CASES[case]
-    cfg = make(seed)
-    ost = rm.OracleState(cfg)
-    pos0, alive0 = ost.positions.copy(), ost.is_alive.astype(bool).copy()
-    odyn = rm.OracleDynamics(cfg, ost)
+        fe = es.frame_every(cfg)
+        mom = np.zeros(4)
+    tracker = MSDTracker(eng)
+    marks = sorted(set(es.MSD_AT) | set(es.COUNT_AT) |
+                   (set(range(fe, es.STEPS + 1, fe)) if imaging else set()) | {es.STEPS})
+    msd, counts = [], []
+    done = 0
+    for t in marks:
+        _run_steps(eng, t - done, imaging)
+        done = t
+        if imaging and (t % fe == 0 or t == es.STEPS):
+            mom += flim_moments(mic, 0).reshape(-1, 4).sum(axis=0)
+            mic.finalize_frame()
+        if t in es.MSD_AT:
+            msd.append(tracker.msd())
+        if t in es.COUNT_AT:
+            st.mark_stale()
+            alive = st.is_alive.astype(bool)
+            counts += np.bincount(st.species_id[alive], minlength=len(cfg.species)).tolist()
+    pairs = pair_distance_histogram(eng, None, nb, r_max).astype(np.float64)
+    out = msd + counts + pairs.tolist()
     if imaging:
-        oph = rm.OraclePhotons(cfg, ost, min(1.0, cfg.scan.dwell_time / cfg.scan.frame_interval))
-        omic = rm.OracleMicroscope(cfg, ost)
-    mid = None
-    for k in range(steps):
-        odyn.step()
-        if imaging:
-            omic.process_photons(oph.evolve_one_bd_step())
-        if k + 1 == steps // 2:
-            a = ost.is_alive.astype(bool)
-            mid = np.bincount(ost.species_id[a], minlength=len(cfg.species)).astype(np.float64)
-    flim = omic.finalize_frame()["flim_stack"] if imaging else None
-    return _observables(cfg, edges, pos0, alive0, ost.positions, ost.is_alive.astype(bool),
-                        ost.species_id, flim, mid)
+        out += es.phasor_from(mom)
+    eng.close()
+    return np.array(out, dtype=np.float64)
 
 
-@pytest.mark.parametrize("case", list(CASES))
-def test_fp32_philox_statistics_match_oracle(oracle_lib, case):
-    make, n, _, imaging, edges = CASES[case]
-    a = np.array([_gpu_replica(case, s) for s in range(n)])
-    b = np.array([_oracle_replica(case, s) for s in range(n)])
+@pytest.mark.parametrize("case", list(es.CASES))
+def test_fp32_philox_statistics_match_oracle_ensembles(case):
+    b = np.load(FIXTURE)[case]
+    n = es.REPLICAS
+    assert b.shape[0] == n
+    a = np.array([gpu_replica(case, s) for s in range(n)])
+    names = es.observable_names(case)
+    assert a.shape == b.shape == (n, len(names))
+    K = len(names)
+    z = norm.ppf(1.0 - 0.025 / K)
     ma, mb = a.mean(axis=0), b.mean(axis=0)
     se = np.sqrt(a.var(axis=0, ddof=1) / n + b.var(axis=0, ddof=1) / n)
-    cfg = make(0)
-    n_sp = len(cfg.species)
-    nb = edges.size - 1
-    names = (["msd"] + [f"pairs[{edges[i]:.3f},{edges[i + 1]:.3f})" for i in range(nb)]
-             + [f"mid:{s.name}" for s in cfg.species] + [f"end:{s.name}" for s in cfg.species]
-             + (["phasor g", "phasor s"] if imaging else []))
-    assert len(names) == ma.size
+    bad = []
     for i, nm in enumerate(names):
-        print(f"{case} {nm:26s} fp32 {ma[i]:12.6g}  oracle {mb[i]:12.6g}  se {se[i]:10.3g}")
-    bad = [nm for i, nm in enumerate(names)
-           if abs(ma[i] - mb[i]) > 4.5 * se[i] + 1e-12 * abs(mb[i])]
-    assert not bad, f"outside 4.5 standard errors: {bad}"
-    # the observables move: diffusion and reactions happened
-    assert ma[0] > 1e-4
-    assert np.any(a[:, 1 + nb + n_sp:1 + nb + 2 * n_sp] != a[:, 1 + nb:1 + nb + n_sp])
+        tol = z * se[i] if se[i] > 0 else 0.0
+        ok = abs(ma[i] - mb[i]) <= tol + 1e-12 * abs(mb[i])
+        print(f"{case} {nm:26s} fp32 {ma[i]:12.6g}  oracle {mb[i]:12.6g}  "
+              f"|d|/se {abs(ma[i] - mb[i]) / se[i] if se[i] > 0 else 0.0:5.2f}"
+              f"{'' if ok else '  <-- outside'}")
+        if not ok:
+            bad.append(nm)
+    assert not bad, f"outside the Bonferroni-corrected 95 % interval (z = {z:.2f}): {bad}"
+    # the observables move: diffusion happened and the MSD grows
+    assert np.all(np.diff(ma[:len(es.MSD_AT)]) > 0) and ma[0] > 1e-4
