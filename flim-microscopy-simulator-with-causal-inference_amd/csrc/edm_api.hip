@@ -132,7 +132,8 @@ int edm_destroy(edm_membrane* m) {
                     m->ph_fret, m->flim, m->pcx, m->det_flag, m->det_pos, m->sv_key, m->sv_key2,
                     m->sv_val, m->sv_val2, m->asg_px, m->asg_py, m->asg_flag, m->asg_pos,
                     m->asg_t, m->asg_ch, m->sv_px, m->sv_py, m->pix_tot, m->d_scal,
-                    m->dye_ids, m->wl, m->wl_cnt};
+                    m->dye_ids, m->wl, m->wl_cnt, m->tg_cnt, m->tg_fill, m->tg_start,
+                    m->tg_x, m->tg_y, m->tg_u};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (auto e : m->kev) hipEventDestroy(e);
@@ -244,6 +245,11 @@ int edm_set_tuning(edm_membrane* m, int32_t key, int64_t value) {
                 return invalid("edm_set_tuning: catalysis lanes must be a power of two in 1..64");
             m->cat_lanes = (int)value;
             return 0;
+        case EDM_TUNE_FRET_SEARCH:
+            if (value < 0 || value > 1)
+                return invalid("edm_set_tuning: FRET search must be 0 (grid) or 1 (scan)");
+            m->fret_search = (int)value;
+            return 0;
         default:
             return invalid("edm_set_tuning: unknown key");
     }
@@ -256,6 +262,7 @@ int edm_get_tuning(edm_membrane* m, int32_t key, int64_t* value) {
         case EDM_TUNE_CAT_SIDE: *value = m->cat_side; return 0;
         case EDM_TUNE_CAT_WALK: *value = m->cat_walk; return 0;
         case EDM_TUNE_CAT_LANES: *value = m->cat_lanes; return 0;
+        case EDM_TUNE_FRET_SEARCH: *value = m->fret_search; return 0;
         default: return invalid("edm_get_tuning: unknown key");
     }
 }

@@ -200,6 +200,15 @@ struct edm_membrane {
     int n_dye = 0;
     double* k_fret = nullptr;    // by position in dye_ids
     int32_t* acc = nullptr;      // by position in dye_ids: acceptor particle index or -1
+    // proximity FRET: the alive targets of a map binned on a grid of cells
+    // no smaller than the proximity radius (counts, fill cursors, starts,
+    // then the binned targets' positions and slots)
+    int fret_search = 0;         // 0: target grid, 1: scan every particle
+    int32_t *tg_cnt = nullptr, *tg_fill = nullptr, *tg_start = nullptr;
+    int tg_cell_cap = 0;
+    void *tg_x = nullptr, *tg_y = nullptr;
+    int32_t* tg_u = nullptr;
+    int tg_cap = 0;
     int32_t* ph_nd = nullptr;    // by particle index
     int32_t* ph_nf = nullptr;
     int64_t* ph_cnt = nullptr;   // by particle index, then scanned in place

@@ -96,6 +96,116 @@ __global__ void k_fret_prox(int n_dye, const int32_t* dye_ids, const int32_t* sl
     acc[q] = -1;
 }
 
+// The same search through a grid of the map's alive targets.  Cells are no
+// narrower than the proximity radius (with a margin), so every target within
+// the radius of a donor, minimum image included, lies in the donor's cell or
+// one of its 8 neighbours; the cell index is taken in FP64 so rounding cannot
+// move a particle by a whole cell.  "Some target within the radius" does not
+// depend on the order in which targets are met, and the distance test is the
+// brute-force kernel's expression on the same values, so both kernels give
+// the same answer.
+struct TgGrid {
+    int ncx, ncy;
+    double ix, iy;  // cells per um
+    __device__ __forceinline__ int cell(double x, double y) const {
+        int cx = (int)floor(x * ix), cy = (int)floor(y * iy);
+        cx = cx < 0 ? 0 : (cx >= ncx ? ncx - 1 : cx);
+        cy = cy < 0 ? 0 : (cy >= ncy ? ncy - 1 : cy);
+        return cx * ncy + cy;
+    }
+};
+
+__global__ void k_tg_zero(int ncell, int32_t* cnt, int32_t* fill) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > ncell) return;
+    cnt[c] = 0;
+    if (c < ncell) fill[c] = 0;
+}
+
+template <typename Real>
+__global__ void k_tg_count(int n, const Real* x, const Real* y, const int8_t* sp,
+                           const uint8_t* fl, int target, TgGrid g, int32_t* cnt) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    if (!(fl[u] & EDM_FLAG_ALIVE) || sp[u] != target) return;
+    atomicAdd(&cnt[g.cell((double)x[u], (double)y[u])], 1);
+}
+
+template <typename Real>
+__global__ void k_tg_scatter(int n, const Real* x, const Real* y, const int8_t* sp,
+                             const uint8_t* fl, int target, TgGrid g, const int32_t* start,
+                             int32_t* fill, Real* tx, Real* ty, int32_t* tu) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    if (!(fl[u] & EDM_FLAG_ALIVE) || sp[u] != target) return;
+    const Real xu = x[u], yu = y[u];
+    const int c = g.cell((double)xu, (double)yu);
+    const int k = start[c] + atomicAdd(&fill[c], 1);
+    tx[k] = xu;
+    ty[k] = yu;
+    tu[k] = u;
+}
+
+// neighbour cells of c along one axis: distinct indices, wrapped on a
+// periodic axis, clipped on a walled one
+__device__ __forceinline__ int tg_axis(int c, int nc, bool periodic, int* out) {
+    if (periodic && nc < 3) {
+        for (int k = 0; k < nc; ++k) out[k] = k;
+        return nc;
+    }
+    int m = 0;
+    for (int o = -1; o <= 1; ++o) {
+        int v = c + o;
+        if (periodic) v = (v + nc) % nc;
+        else if (v < 0 || v >= nc) continue;
+        out[m++] = v;
+    }
+    return m;
+}
+
+template <typename Real>
+__global__ void k_fret_prox_grid(int n_dye, const int32_t* dye_ids, const int32_t* slot_of,
+                                 const edm_photo_params* pp, int j, const Real* x, const Real* y,
+                                 const uint8_t* fl, const int8_t* dye, Real Lx, Real Ly, bool px,
+                                 bool py, TgGrid g, const int32_t* start, const Real* tx,
+                                 const Real* ty, const int32_t* tu, double* k_fret, int32_t* acc) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n_dye) return;
+    const int t = slot_of[dye_ids[q]];
+    const uint8_t f = fl[t];
+    if (!(f & EDM_FLAG_DYE) || !(f & EDM_FLAG_ALIVE) || (f & EDM_FLAG_BLEACHED)) return;
+    if (dye[t] != pp->fm_donor[j]) return;
+    const Real r2 = (Real)pp->fm_radius[j] * (Real)pp->fm_radius[j];
+    const Real xi = x[t], yi = y[t];
+    const int c = g.cell((double)xi, (double)yi);
+    int cxs[3], cys[3];
+    const int nx = tg_axis(c / g.ncy, g.ncx, px, cxs);
+    const int ny = tg_axis(c % g.ncy, g.ncy, py, cys);
+    bool found = false;
+    for (int a = 0; a < nx && !found; ++a)
+        for (int b = 0; b < ny && !found; ++b) {
+            const int cc = cxs[a] * g.ncy + cys[b];
+            for (int k = start[cc], e = start[cc + 1]; k < e; ++k) {
+                if (tu[k] == t) continue;
+                Real dx = tx[k] - xi, dy = ty[k] - yi;
+                if (px) {
+                    if (dx > Lx / (Real)2) dx -= Lx;
+                    else if (dx < -Lx / (Real)2) dx += Lx;
+                }
+                if (py) {
+                    if (dy > Ly / (Real)2) dy -= Ly;
+                    else if (dy < -Ly / (Real)2) dy += Ly;
+                }
+                if (dx * dx + dy * dy < r2) {
+                    found = true;
+                    break;
+                }
+            }
+        }
+    k_fret[q] = found ? pp->fm_k_hi[j] : pp->fm_k_lo[j];
+    acc[q] = -1;
+}
+
 // ----------------------------------------------------------------------
 // PSS kernel (photophysics.py:339-483)
 // ----------------------------------------------------------------------
@@ -843,6 +953,82 @@ static int scan_total(edm_membrane* m, const int64_t* in, const int64_t* out, in
     return 0;
 }
 
+// Grid of map j's alive targets and the donors' search through it.  Cell
+// sides are at least the radius plus a margin that covers the rounding of
+// the distance test (a few ulps of the box side in the position type), and
+// the grid holds at most max(n, 64) cells.
+template <typename Real>
+static int fret_prox_grid(edm_membrane* m, int j) {
+    const int n = m->n, nd = m->n_dye;
+    hipStream_t st = m->stream;
+    const double Lx = m->p.Lx, Ly = m->p.Ly, r = m->pp.fm_radius[j];
+    const double ulp = sizeof(Real) == 4 ? 0x1p-23 : 0x1p-52;
+    const double cmin = r * 1.001 + 64.0 * ulp * std::max(Lx, Ly);
+    long ncx = std::max(1L, std::min(4096L, (long)std::floor(Lx / cmin)));
+    long ncy = std::max(1L, std::min(4096L, (long)std::floor(Ly / cmin)));
+    const long maxc = std::max<long>(n, 64);
+    while (ncx * ncy > maxc) {
+        if (ncx >= ncy) ncx = (ncx + 1) / 2;
+        else ncy = (ncy + 1) / 2;
+    }
+    TgGrid g{(int)ncx, (int)ncy, (double)ncx / Lx, (double)ncy / Ly};
+    const int ncell = (int)(ncx * ncy);
+    if (ncell + 1 > m->tg_cell_cap) {
+        EDM_HIP(hipStreamSynchronize(st));
+        hipFree(m->tg_cnt);
+        hipFree(m->tg_fill);
+        hipFree(m->tg_start);
+        m->tg_cnt = m->tg_fill = m->tg_start = nullptr;
+        m->tg_cell_cap = 0;
+        if (dalloc(&m->tg_cnt, ncell + 1) || dalloc(&m->tg_fill, ncell + 1) ||
+            dalloc(&m->tg_start, ncell + 1))
+            return EDM_EHIP;
+        m->tg_cell_cap = ncell + 1;
+    }
+    if (n > m->tg_cap) {
+        EDM_HIP(hipStreamSynchronize(st));
+        hipFree(m->tg_x);
+        hipFree(m->tg_y);
+        hipFree(m->tg_u);
+        m->tg_x = m->tg_y = nullptr;
+        m->tg_u = nullptr;
+        m->tg_cap = 0;
+        if (dalloc((double**)&m->tg_x, n) || dalloc((double**)&m->tg_y, n) ||
+            dalloc(&m->tg_u, n))
+            return EDM_EHIP;
+        m->tg_cap = n;
+    }
+    const int target = m->pp.fm_target[j];
+    const Real* x = (const Real*)m->a.x;
+    const Real* y = (const Real*)m->a.y;
+    hipLaunchKernelGGL(k_tg_zero, dim3(nb(ncell + 1)), dim3(kB), 0, st, ncell, m->tg_cnt,
+                       m->tg_fill);
+    hipLaunchKernelGGL(k_tg_count<Real>, dim3(nb(n)), dim3(kB), 0, st, n, x, y, m->a.sp, m->a.fl,
+                       target, g, m->tg_cnt);
+    size_t need = 0;
+    EDM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, m->tg_cnt, m->tg_start, ncell + 1,
+                                             st));
+    if (need > m->cub_tmp_bytes) {
+        EDM_HIP(hipStreamSynchronize(st));
+        EDM_HIP(hipFree(m->cub_tmp));
+        m->cub_tmp_bytes = need + 256;
+        EDM_HIP(hipMalloc(&m->cub_tmp, m->cub_tmp_bytes));
+    }
+    size_t tb = m->cub_tmp_bytes;
+    EDM_HIP(hipcub::DeviceScan::ExclusiveSum(m->cub_tmp, tb, m->tg_cnt, m->tg_start, ncell + 1,
+                                             st));
+    hipLaunchKernelGGL(k_tg_scatter<Real>, dim3(nb(n)), dim3(kB), 0, st, n, x, y, m->a.sp,
+                       m->a.fl, target, g, (const int32_t*)m->tg_start, m->tg_fill,
+                       (Real*)m->tg_x, (Real*)m->tg_y, m->tg_u);
+    hipLaunchKernelGGL(k_fret_prox_grid<Real>, dim3(nb(nd)), dim3(kB), 0, st, nd,
+                       (const int32_t*)m->dye_ids, (const int32_t*)m->a.slot_of,
+                       (const edm_photo_params*)m->d_pp, j, x, y, m->a.fl, m->a.dye, (Real)Lx,
+                       (Real)Ly, m->p.periodic_x != 0, m->p.periodic_y != 0, g,
+                       (const int32_t*)m->tg_start, (const Real*)m->tg_x, (const Real*)m->tg_y,
+                       (const int32_t*)m->tg_u, m->k_fret, m->acc);
+    return 0;
+}
+
 int ph_fret(edm_membrane* m) {
     if (int rc = dyn_ensure_slot_of(m)) return rc;
     const int n = m->n, nd = m->n_dye;
@@ -858,6 +1044,11 @@ int ph_fret(edm_membrane* m) {
     }
     for (int j = 0; j < m->pp.n_fret_maps; ++j) {
         if (!m->pp.fm_proximity[j] || m->pp.fm_skip[j]) continue;
+        if (m->fret_search == 0) {
+            if (int rc = m->fp64 ? fret_prox_grid<double>(m, j) : fret_prox_grid<float>(m, j))
+                return rc;
+            continue;
+        }
         if (m->fp64)
             hipLaunchKernelGGL(k_fret_prox<double>, dim3(nb(nd)), dim3(kB), 0, st, nd,
                                (const int32_t*)m->dye_ids, (const int32_t*)m->a.slot_of, n, dpp, j,

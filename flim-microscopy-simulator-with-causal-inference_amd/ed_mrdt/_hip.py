@@ -123,6 +123,7 @@ TUNE_TILE_ROWS = 1
 TUNE_CAT_SIDE = 2
 TUNE_CAT_WALK = 3
 TUNE_CAT_LANES = 4
+TUNE_FRET_SEARCH = 5
 
 # function name -> (argtypes) ; every function returns int32 status
 _SIGS = {

@@ -487,17 +487,21 @@ class DynamicsEngine:
 
     # ---- tuning (no reference counterpart) ----------------------------------
     _TUNE = (("tile_rows", _hip.TUNE_TILE_ROWS), ("cat_side", _hip.TUNE_CAT_SIDE),
-             ("cat_walk", _hip.TUNE_CAT_WALK), ("cat_lanes", _hip.TUNE_CAT_LANES))
+             ("cat_walk", _hip.TUNE_CAT_WALK), ("cat_lanes", _hip.TUNE_CAT_LANES),
+             ("fret_search", _hip.TUNE_FRET_SEARCH))
 
     def set_tuning(self, tile_rows: Optional[int] = None, cat_side: Optional[int] = None,
-                   cat_walk: Optional[int] = None, cat_lanes: Optional[int] = None) -> None:
+                   cat_walk: Optional[int] = None, cat_lanes: Optional[int] = None,
+                   fret_search: Optional[int] = None) -> None:
         """Work-decomposition knobs that leave every result unchanged: cell
         rows per strip of the tiled kernels (1..256), the side catalysis walks
         its pairs from (0 enzymes, 1 substrates, -1 the rarer one), how
-        (1 work lists where possible, 0 LDS strips) and the lanes per entity
-        of the strip walk (a power of two in 1..64)."""
+        (1 work lists where possible, 0 LDS strips), the lanes per entity
+        of the strip walk (a power of two in 1..64) and how proximity FRET
+        finds its targets (0 a grid of the targets, 1 a scan of every
+        particle per donor, the reference's search)."""
         vals = dict(tile_rows=tile_rows, cat_side=cat_side, cat_walk=cat_walk,
-                    cat_lanes=cat_lanes)
+                    cat_lanes=cat_lanes, fret_search=fret_search)
         for name, key in self._TUNE:
             if vals[name] is not None:
                 check(lib().edm_set_tuning(self._h, key, int(vals[name])), "edm_set_tuning")

@@ -96,10 +96,18 @@ def _harmonic_potentials(k_kT_per_um2: float, sigma_nm: float):
 
 
 def large_membrane(seed: int = 0, side: float = 100.0, n_alive: int = 1_000_000,
-                   pixels: int = 1024) -> SimulationConfig:
+                   pixels: int = 1024, proximity_target: Optional[str] = None,
+                   proximity_radius_nm: float = 50.0) -> SimulationConfig:
     """1M particles on 100 x 100 um, harmonic k = 10 kT/nm^2 = 1e7 kT/um^2,
-    r_c = sigma = 10 nm, density grid 200 x 200 (0.5 um cells)."""
+    r_c = sigma = 10 nm, density grid 200 x 200 (0.5 um cells).  With
+    ``proximity_target`` the PIP3-borne mTFP1 is a proximity biosensor of
+    that species (E = 0.5 within the radius, else 0.02)."""
     raw = rescale(plain_raw(), side, n_alive)
+    if proximity_target:
+        raw["fret_pairs"][0].update(fret_mode="proximity",
+                                    proximity_target_species=proximity_target,
+                                    proximity_radius=float(proximity_radius_nm),
+                                    proximity_E_high=0.5, proximity_E_low=0.02)
     raw["name"] = f"PIP3/PTEN large membrane ({n_alive})"
     raw["potentials"] = _harmonic_potentials(1.0e7, 10.0)
     raw["scan"].update(pixels_x=pixels, pixels_y=pixels, pixel_size=side * 1000.0 / pixels,

@@ -200,11 +200,15 @@ int edm_photon_counters(edm_membrane *m, int64_t out[2]);
  *                       in LDS; 1: runs walk the step's list of their
  *                       enzymes or substrates where they can
  *   EDM_TUNE_CAT_LANES  lanes that walk one entity's candidates in the strip
- *                       walk: a power of two in 1..64 (default 16)        */
+ *                       walk: a power of two in 1..64 (default 16)
+ *   EDM_TUNE_FRET_SEARCH  proximity FRET finds targets through a grid of the
+ *                       alive targets (0, default) or by scanning every
+ *                       particle for every donor (1), as the reference does */
 #define EDM_TUNE_TILE_ROWS 1
 #define EDM_TUNE_CAT_SIDE 2
 #define EDM_TUNE_CAT_WALK 3
 #define EDM_TUNE_CAT_LANES 4
+#define EDM_TUNE_FRET_SEARCH 5
 int edm_set_tuning(edm_membrane *m, int32_t key, int64_t value);
 int edm_get_tuning(edm_membrane *m, int32_t key, int64_t *value);
 /* Reference pair list in reference order (CellList.get_neighbor_pairs,
