@@ -26,7 +26,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.normpath(os.path.join(HERE, "..", "include"))
 OUT = os.path.join(HERE, "ed_mrdt", "libedm_hip.so")
 SOURCES = ["edm_api.hip", "edm_dynamics.hip", "edm_photons.hip", "edm_ens_api.hip",
-           "edm_record.hip", "edm_kernels.hip", "edm_stats.hip"]
+           "edm_record.hip", "edm_kernels.hip", "edm_stats.hip", "edm_graph.hip"]
 BUILD_DIR = os.path.join(HERE, "_build")
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-ffp-contract=off",

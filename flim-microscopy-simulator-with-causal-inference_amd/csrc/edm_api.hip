@@ -118,6 +118,8 @@ int edm_create(const edm_params* p, edm_membrane** out) {
 int edm_destroy(edm_membrane* m) {
     if (!m) return 0;
     if (m->stream) hipStreamSynchronize(m->stream);
+    graphs_free(m);
+    if (m->d_step) hipFree(m->d_step);
     rec_free(m);
     stats_free(m);
     void* bufs[] = {m->a.x, m->a.y, m->a.xn, m->a.yn, m->a.xt, m->a.yt, m->a.sp, m->a.sp_t, m->a.fl,
@@ -147,6 +149,7 @@ int edm_destroy(edm_membrane* m) {
 int edm_upload_state(edm_membrane* m, const edm_host_state* s) {
     if (!m || !s || !s->positions || !s->species_id || !s->is_alive)
         return invalid("edm_upload_state: null argument");
+    graph_touch(m);
     const int n = m->n;
     std::vector<double> x(n), y(n);
     std::vector<int8_t> sp(n), ctmc(n, 0), dye(n, -1);
@@ -225,6 +228,7 @@ int edm_upload_state(edm_membrane* m, const edm_host_state* s) {
 
 int edm_set_tuning(edm_membrane* m, int32_t key, int64_t value) {
     if (!m) return invalid("edm_set_tuning: null membrane");
+    graph_touch(m);
     switch (key) {
         case EDM_TUNE_TILE_ROWS:
             if (value < 1 || value > kTileRows)
@@ -250,6 +254,13 @@ int edm_set_tuning(edm_membrane* m, int32_t key, int64_t value) {
                 return invalid("edm_set_tuning: FRET search must be 0 (grid) or 1 (scan)");
             m->fret_search = (int)value;
             return 0;
+        case EDM_TUNE_GRAPHS:
+            if (value < 0 || value > 1) return invalid("edm_set_tuning: graphs must be 0 or 1");
+            m->use_graphs = (int)value;
+            return 0;
+        case EDM_TUNE_GRAPH_REPLAYS:
+        case EDM_TUNE_GRAPH_STATUS:
+            return invalid("edm_set_tuning: read-only key");
         default:
             return invalid("edm_set_tuning: unknown key");
     }
@@ -263,6 +274,9 @@ int edm_get_tuning(edm_membrane* m, int32_t key, int64_t* value) {
         case EDM_TUNE_CAT_WALK: *value = m->cat_walk; return 0;
         case EDM_TUNE_CAT_LANES: *value = m->cat_lanes; return 0;
         case EDM_TUNE_FRET_SEARCH: *value = m->fret_search; return 0;
+        case EDM_TUNE_GRAPHS: *value = m->use_graphs; return 0;
+        case EDM_TUNE_GRAPH_REPLAYS: *value = (int64_t)m->graph_replays; return 0;
+        case EDM_TUNE_GRAPH_STATUS: *value = m->graph_status; return 0;
         default: return invalid("edm_get_tuning: unknown key");
     }
 }
@@ -313,6 +327,7 @@ int edm_download_state(edm_membrane* m, edm_host_state* s) {
 
 int edm_set_photophysics(edm_membrane* m, const edm_photo_params* pp) {
     if (!m || !pp) return invalid("edm_set_photophysics: null argument");
+    graph_touch(m);
     if (pp->n_fluoro < 0 || pp->n_fluoro > 8 || pp->n_fret_maps < 0 || pp->n_fret_maps > 8)
         return invalid("edm_set_photophysics: at most 8 fluorophores and 8 FRET pairs");
     m->pp = *pp;
@@ -322,6 +337,7 @@ int edm_set_photophysics(edm_membrane* m, const edm_photo_params* pp) {
 
 int edm_set_imaging(edm_membrane* m, const edm_image_params* ip) {
     if (!m || !ip) return invalid("edm_set_imaging: null argument");
+    graph_touch(m);
     if (ip->nx <= 0 || ip->ny <= 0 || ip->n_bins <= 0 || ip->n_ch <= 0 || ip->n_sigma <= 0 ||
         ip->n_sigma > 8 || ip->R < 0)
         return invalid("edm_set_imaging: bad image geometry");
@@ -332,7 +348,9 @@ int edm_set_imaging(edm_membrane* m, const edm_image_params* ip) {
 
 int edm_step_begin(edm_membrane* m, int64_t* n_pairs) {
     if (!m) return invalid("edm_step_begin: null handle");
+    graph_touch(m);
     m->need_pair_offsets = true;
+    m->pair_sync = true;
     if (int rc = dyn_check_overflow(m)) return rc;
     if (int rc = dyn_step_begin(m)) return rc;
     m->begun = true;
@@ -342,6 +360,7 @@ int edm_step_begin(edm_membrane* m, int64_t* n_pairs) {
 
 int edm_step_finish(edm_membrane* m, const double* draws, int64_t n_draws) {
     if (!m) return invalid("edm_step_finish: null handle");
+    graph_touch(m);
     if (!m->begun) {
         set_error("edm_step_finish: no edm_step_begin for this step");
         return EDM_ESTATE;
@@ -359,18 +378,12 @@ int edm_step_finish(edm_membrane* m, const double* draws, int64_t n_draws) {
 
 int edm_run(edm_membrane* m, int32_t n_steps) {
     if (!m) return invalid("edm_run: null handle");
+    graph_touch(m);
     if (m->begun) {
         set_error("edm_run: a parity step is open");
         return EDM_ESTATE;
     }
-    for (int k = 0; k < n_steps; ++k) {
-        m->need_pair_offsets = m->p.pair_cap > 0;
-        if (int rc = dyn_step_begin(m)) return rc;
-        if (int rc = dyn_step_finish(m, nullptr, false)) return rc;
-        if (m->rec_flags & EDM_REC_AUTO)
-            if (int rc = rec_step(m)) return rc;
-    }
-    return 0;
+    return steps_run(m, n_steps, false);
 }
 
 int edm_counters(edm_membrane* m, int64_t out[6]) {
@@ -534,9 +547,24 @@ int edm_time_steps(edm_membrane* m, int32_t n_steps, int32_t image, double* tota
     }
     std::vector<hipEvent_t> saved = m->kev;
     EDM_HIP(hipStreamSynchronize(m->stream));
+    if (graphs_eligible(m)) {
+        // replayed graphs: the force launches are not timed one by one
+        EDM_HIP(hipEventRecord(m->ev0, m->stream));
+        if (int rc = steps_run(m, n_steps, image != 0)) return rc;
+        EDM_HIP(hipEventRecord(m->ev1, m->stream));
+        EDM_HIP(hipEventSynchronize(m->ev1));
+        float ms = 0.f;
+        EDM_HIP(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+        if (total_ms) *total_ms = ms;
+        if (force_ms) *force_ms = -1.0;
+        if (breakdown_ms)
+            for (int i = 0; i < 8; ++i) breakdown_ms[i] = 0.0;
+        return 0;
+    }
     EDM_HIP(hipEventRecord(m->ev0, m->stream));
     for (int k = 0; k < n_steps; ++k) {
         m->need_pair_offsets = m->p.pair_cap > 0;
+        m->pair_sync = false;
         if (int rc = dyn_step_begin(m)) return rc;
         m->time_force = true;
         // finish records kev[0] / kev[1] around the force launch: point them

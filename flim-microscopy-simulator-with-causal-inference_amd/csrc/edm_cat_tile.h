@@ -38,6 +38,7 @@ struct CatArgs {
     Draws dr;
     Rng rng;
     uint32_t step;
+    const uint32_t* step_base;  // when set, the launch's steps count from step_base[0]
     bool deferred;
     bool by_substrate;  // walk the pairs from the substrates' side (the rarer one)
     int group_shift;    // log2 of the lanes that walk one entity's candidates (0..6)
@@ -381,9 +382,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFp32Waves<
     __shared__ CatArgs<Real> sA;
     const int tile = xcd_tile(ntiles);
     if (tile >= ntiles) return;  // uniform across the block
-    if (threadIdx.x == 0) sA = A;
+    if (threadIdx.x == 0) {
+        sA = A;
+        if (A.step_base) sA.step += *A.step_base;
+    }
     __syncthreads();
-    cat_strip<Real, 1>(C, P, sA, A.step, ctr, G, n_grids, strip_geo(C, tile, ntile_y));
+    cat_strip<Real, 1>(C, P, sA, sA.step, ctr, G, n_grids, strip_geo(C, tile, ntile_y));
 }
 
 // Catalysis over a work list of its walk entities, one listed slot per
@@ -455,7 +459,10 @@ __global__ __launch_bounds__(256) void k_cat_list(Cells<Real> C, Post<Real> P, C
     // the parameter block is indexed with per-lane values: keep it in LDS
     // rather than in per-lane scratch
     __shared__ CatArgs<Real> sA;
-    if (threadIdx.x == 0) sA = A;
+    if (threadIdx.x == 0) {
+        sA = A;
+        if (A.step_base) sA.step += *A.step_base;
+    }
     __syncthreads();
     const int hits = (int)threadIdx.x < m ? cat_walk(C, P, sA, G, n_grids, wl_slot(V)) : 0;
     unsigned long long v = hits;

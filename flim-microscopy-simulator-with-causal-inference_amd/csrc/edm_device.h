@@ -60,6 +60,16 @@ enum Tag : uint32_t {
     kTagRecruitOff = 18,
 };
 
+// Step number of a launch's draw keys.  Launches recorded into a graph that
+// is replayed read the replay's first step from base[0] (written by the host
+// before every replay) and add their offset in the recording; other launches
+// carry the step itself (base == nullptr).
+struct StepRef {
+    const uint32_t* base;
+    uint32_t off;
+    __device__ __forceinline__ uint32_t get() const { return base ? *base + off : off; }
+};
+
 struct Rng {
     uint32_t k0, k1;
     __device__ __forceinline__ U4 draw(uint32_t step, uint32_t a, uint32_t b,

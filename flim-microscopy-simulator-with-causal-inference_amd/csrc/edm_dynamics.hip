@@ -38,8 +38,10 @@ struct Cells {
     // reference pair bookkeeping (by particle index)
     const uint64_t* pcnt;
     const uint64_t* poff;
-    const int64_t* npairs;  // device scalar: length of the reference pair list
-    bool cap_active;
+    const int64_t* npairs;  // device scalars: length of the reference pair list, pairs found
+    bool cap_active;        // the host knows the cap truncates this step's list
+    bool cap_dev;           // the device decides: the cap truncates when npairs[0] < npairs[1]
+    int8_t cap_gate;        // +1: the launch runs only when the cap truncates, -1: only when not
     int kbase;  // first particle index of this membrane (0 unless batched); draw keys
                 // use the index relative to it
     int tile_rows;  // cell rows per strip of the tiled kernels (1..kTileRows)
@@ -91,11 +93,23 @@ __device__ int64_t pair_rank(const Cells<Real>& C, int sa, int sb) {
     return -1;
 }
 
+// Does the faithful pair cap truncate this step's pair list?
+template <typename Real>
+__device__ __forceinline__ bool cap_on(const Cells<Real>& C) {
+    return C.cap_active || (C.cap_dev && C.npairs[0] < C.npairs[1]);
+}
+
+// A launch gated on the cap (cap_gate) that has nothing to do this step.
+template <typename Real>
+__device__ __forceinline__ bool cap_gated_off(const Cells<Real>& C) {
+    return C.cap_gate != 0 && (C.cap_gate > 0) != cap_on(C);
+}
+
 // Does the pair (sa, sb) exist in the reference's capped pair list?
 // sa must hold the lower particle index.
 template <typename Real>
 __device__ __forceinline__ bool pair_in_list(const Cells<Real>& C, int sa, int sb) {
-    if (!C.cap_active) return true;
+    if (!cap_on(C)) return true;
     const int ia = C.id[sa];
     const uint64_t off = C.poff[ia], cnt = C.pcnt[ia];
     const uint64_t np = (uint64_t)*C.npairs;
@@ -721,8 +735,10 @@ template <typename Real, bool kOrdered>
 __global__ __launch_bounds__(256) void k_force_bd(int n, Cells<Real> C, ForceParams<Real> F,
                                                   Real* __restrict__ xn, Real* __restrict__ yn,
                                                   const double* __restrict__ noise_hook, Rng rng,
-                                                  uint32_t step, Grids<Real> G, int n_grids,
+                                                  StepRef sref, Grids<Real> G, int n_grids,
                                                   uint32_t keep_mask) {
+    if (cap_gated_off(C)) return;  // uniform across the launch
+    const uint32_t step = sref.get();
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     bool take_any = false;
     Real nxp = 0, nyp = 0;
@@ -803,7 +819,7 @@ __global__ __launch_bounds__(256) void k_force_bd(int n, Cells<Real> C, ForcePar
                                                        C.py, dx, dy);
                             if (dsq >= C.cut2) continue;
                             if (!(C.fl[u] & EDM_FLAG_ALIVE)) continue;
-                            if (C.cap_active) {
+                            if (cap_on(C)) {
                                 const bool lower = C.id[u] < it;
                                 if (!(lower ? pair_in_list(C, u, t) : pair_in_list(C, t, u)))
                                     continue;
@@ -866,9 +882,10 @@ __device__ __forceinline__ bool cas_byte(int8_t* base, int idx, int8_t expect, i
 // per-pair uniform (dynamics.py:529-606).
 template <typename Real>
 __global__ void k_bind_edges(int n, Cells<Real> C, Post<Real> P, int ra, int rb, Real contact2,
-                             Real prob, Draws dr, Rng rng, uint32_t step, uint32_t rxi,
+                             Real prob, Draws dr, Rng rng, StepRef sref, uint32_t rxi,
                              unsigned long long* ekey, int32_t* ea, int32_t* eb, unsigned int* ne,
                              unsigned int cap) {
+    const uint32_t step = sref.get();
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     if (C.sp[t] != ra || !(C.fl[t] & EDM_FLAG_ALIVE) || P.bond[t] >= 0) return;
@@ -989,8 +1006,9 @@ __global__ __launch_bounds__(1024) void k_bind_match(Post<Real> P, const int32_t
 // dynamics.py:610-654 apply_dissociation
 template <typename Real>
 __global__ void k_dissoc(int n, const int32_t* __restrict__ id, Post<Real> P, int prod, int ra,
-                         int rb, Real prob, Draws dr, Rng rng, uint32_t step, uint32_t rxi,
+                         int rb, Real prob, Draws dr, Rng rng, StepRef sref, uint32_t rxi,
                          unsigned long long* ctr, Grids<Real> G, int n_grids) {
+    const uint32_t step = sref.get();
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     bool hit = false;
     if (t < n && (P.fl[t] & EDM_FLAG_ALIVE) && P.sp[t] == prod && P.bond[t] >= 0) {
@@ -1176,7 +1194,9 @@ static Cells<Real> make_cells(edm_membrane* m) {
     C.pcnt = m->pair_cnt;
     C.poff = m->pair_off;
     C.npairs = m->d_scal;
-    C.cap_active = m->need_pair_offsets && m->n_pairs < m->n_pairs_found;
+    C.cap_active = m->need_pair_offsets && m->pair_sync && m->n_pairs < m->n_pairs_found;
+    C.cap_dev = m->need_pair_offsets && !m->pair_sync;
+    C.cap_gate = 0;
     C.kbase = 0;
     C.tile_rows = m->tile_rows;
     C.sub_rows = m->sub_rows;
@@ -1218,6 +1238,7 @@ static int dalloc(T** p, size_t count) {
 // pair at most).  Called between steps; the stream is drained first.
 static int ensure_edge_cap(edm_membrane* m, size_t need) {
     if (need <= m->edge_cap) return 0;
+    graph_touch(m);
     EDM_HIP(hipStreamSynchronize(m->stream));
     EDM_HIP(hipFree(m->edge_key));
     EDM_HIP(hipFree(m->edge_a));
@@ -1653,11 +1674,18 @@ static int begin_t(edm_membrane* m) {
                                                  m->pair_off, n, st));
         hipLaunchKernelGGL(k_pair_total, dim3(1), dim3(1), 0, st, n, m->pair_cnt,
                            (const uint64_t*)m->pair_off, (int64_t)m->p.pair_cap, m->d_scal);
-        int64_t sc[2];
-        EDM_HIP(hipMemcpyAsync(sc, m->d_scal, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
-        EDM_HIP(hipStreamSynchronize(st));
-        m->n_pairs = sc[0];
-        m->n_pairs_found = sc[1];
+        if (m->pair_sync) {
+            // parity steps hand the pair count to the caller (hook draws)
+            int64_t sc[2];
+            EDM_HIP(hipMemcpyAsync(sc, m->d_scal, 2 * sizeof(int64_t), hipMemcpyDeviceToHost,
+                                   st));
+            EDM_HIP(hipStreamSynchronize(st));
+            m->n_pairs = sc[0];
+            m->n_pairs_found = sc[1];
+        } else {
+            // the kernels read the counts from d_scal (Cells::cap_dev)
+            m->n_pairs = m->n_pairs_found = -1;
+        }
     }
     EDM_HIP(hipGetLastError());
     return 0;
@@ -1689,7 +1717,7 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
     Post<Real> P = make_post<Real>(m);
     const int ng = (int)m->grids.size();
     Rng rng{(uint32_t)(p.seed & 0xffffffffu), (uint32_t)(p.seed >> 32)};
-    const uint32_t step = m->step;
+    const StepRef step = step_ref(m);
 
     ForceParams<Real> F;
     F.sig = (const Real*)m->sig;
@@ -1729,16 +1757,27 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
                            (Real*)m->a.xn, (Real*)m->a.yn, noise, rng, step, G, ng,
                            m->sensor_mask_all);
     else {
+        // with the cap decided on the device, both paths are launched and
+        // exactly one of them runs (the other returns at once)
+        Cells<Real> Ct = C;
+        if (C.cap_dev) {
+            Cells<Real> Cb = C;
+            Cb.cap_gate = 1;
+            Ct.cap_gate = -1;
+            hipLaunchKernelGGL((k_force_bd<Real, false>), dim3(nblk(n)), dim3(kBlock), 0, st, n,
+                               Cb, F, (Real*)m->a.xn, (Real*)m->a.yn, noise, rng, step, G, ng,
+                               m->sensor_mask_all);
+        }
         const int ntile_y = (p.ncy + m->tile_rows - 1) / m->tile_rows;
         const int ntiles = p.ncx * ntile_y;
         const dim3 grid(8 * ((ntiles + 7) / 8));
         if (m->sub_rows)
             hipLaunchKernelGGL((k_force_tile<Real, true>), grid, dim3(256),
-                               force_sub_lds(m->tile_rows, 1), st, C, F, (Real*)m->a.xn,
+                               force_sub_lds(m->tile_rows, 1), st, Ct, F, (Real*)m->a.xn,
                                (Real*)m->a.yn, noise, rng, step, G, ng, m->sensor_mask_all,
                                ntile_y, ntiles);
         else
-            hipLaunchKernelGGL((k_force_tile<Real, false>), grid, dim3(256), 0, st, C, F,
+            hipLaunchKernelGGL((k_force_tile<Real, false>), grid, dim3(256), 0, st, Ct, F,
                                (Real*)m->a.xn, (Real*)m->a.yn, noise, rng, step, G, ng,
                                m->sensor_mask_all, ntile_y, ntiles);
     }
@@ -1831,7 +1870,8 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
                 A.nw = (Real)p.adapt_n;
                 A.dr = dr;
                 A.rng = rng;
-                A.step = step;
+                A.step = step.off;
+                A.step_base = step.base;
                 A.deferred = defer;
                 A.group_shift = __builtin_ctz((unsigned)m->cat_lanes);
                 A.by_substrate = cp ? cp->by_sub

@@ -28,6 +28,8 @@ __device__ __forceinline__ Cells<Real> ens_cells(const EnsSeg& g, const Real* x,
     C.poff = nullptr;
     C.npairs = nullptr;
     C.cap_active = false;
+    C.cap_dev = false;
+    C.cap_gate = 0;
     C.kbase = g.kbase;
     C.tile_rows = kTileRows;  // unused: the strip table gives each strip's rows
     C.sub_rows = g.sub_rows != 0;

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "edm_device.h"
 #include "edm_hip.h"
 
 #define EDM_FLAG_ALIVE 1u
@@ -104,6 +105,23 @@ struct DevArrays {
     int32_t* slot_of = nullptr;
 };
 
+// Host-side state that a step changes besides device memory: the buffer
+// rotation of the particle arrays and the validity of slot_of and the work
+// lists.  A recorded stretch of steps starts from one such state and leaves
+// another.
+struct StepHost {
+    DevArrays a;
+    bool slot_of_valid, wl_valid;
+    uint32_t wl_built;
+};
+
+struct StepGraph {
+    StepHost h0, h1;
+    int n;
+    bool image, stats;
+    hipGraphExec_t exec;
+};
+
 }  // namespace edm
 
 struct edm_membrane {
@@ -145,6 +163,9 @@ struct edm_membrane {
     uint64_t* pair_off = nullptr;  // exclusive scan of pair_cnt
     int64_t n_pairs_found = 0, n_pairs = 0;
     bool need_pair_offsets = false;
+    // parity steps read the pair count back every step; step loops leave it
+    // on the device (the kernels then decide the cap themselves)
+    bool pair_sync = true;
 
     // step state
     uint32_t step = 0;
@@ -258,12 +279,37 @@ struct edm_membrane {
     double* st_pix = nullptr;  // per-pixel sums of the FLIM reduction
     size_t st_pix_cap = 0;
 
+    // step-loop graphs (edm_run_frame, edm_time_steps): stretches of steps
+    // recorded once and replayed.  d_step holds the first step of a replay;
+    // while recording, launches take their step as an offset from it
+    // (step_ref).  graph_gen counts the changes a recording depends on
+    // (allocations, parameters, tuning); recordings of an older generation
+    // are dropped.
+    uint32_t* d_step = nullptr;
+    bool graph_rec = false;
+    uint32_t graph_step0 = 0;
+    int use_graphs = 0;
+    uint64_t graph_gen = 1;
+    uint64_t warm_gen[2] = {0, 0};  // generation after a clean eager step (no image, image)
+    bool graph_failed = false;      // a recording failed: run eagerly from then on
+    uint64_t graph_replays = 0;
+    int graph_status = 0;           // 0, or why the last recording failed
+    std::vector<edm::StepGraph> graphs;
+
     // timing
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> kev;
     bool time_force = false;
     double force_ms_acc = 0.0;
 };
+
+// Step of the launches being issued: while a graph is being recorded, the
+// offset from the replay's first step (read from d_step at run time);
+// otherwise the step itself.
+static inline edm::StepRef step_ref(const edm_membrane* m) {
+    return m->graph_rec ? edm::StepRef{m->d_step, m->step - m->graph_step0}
+                         : edm::StepRef{nullptr, m->step};
+}
 
 static inline WlView wl_view(const edm_membrane* m, int L) {
     if (!m->wl_valid || !((m->wl_built >> L) & 1u)) return WlView{nullptr, nullptr};
@@ -299,6 +345,14 @@ int frame_reset(edm_membrane* m);
 int stage_draws(edm_membrane* m, const double* host, int64_t n);
 // census translation unit
 int rec_step(edm_membrane* m);  // one census row into the ring (no sync)
+// n steps (dynamics, then the fused photon / imaging pass when image),
+// through recorded graphs where the membrane allows it, else launch by
+// launch
+int steps_run(edm_membrane* m, int n, bool image);
+bool graphs_eligible(const edm_membrane* m);
+// a change that recorded step graphs depend on: drop them
+void graph_touch(edm_membrane* m);
+void graphs_free(edm_membrane* m);
 void rec_free(edm_membrane* m);
 // observables translation unit
 int stats_accumulate(edm_membrane* m);  // after a step (no sync)

@@ -237,10 +237,11 @@ template <typename Real, bool kLocal>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFp32Waves<Real, 6>))) void k_passes(int n, WlView V, const int32_t* __restrict__ id,
                                                 Post<Real> P, PassGroupDev grp,
                                                 const double* __restrict__ draws, Rng rng,
-                                                uint32_t step, Real Lx, Real Ly,
+                                                StepRef sref, Real Lx, Real Ly,
                                                 unsigned long long* ctr, Grids<Real> G,
                                                 int n_grids, int32_t* cid, int32_t* cslot,
                                                 double* cxo, double* cyo, unsigned int* nc) {
+    const uint32_t step = sref.get();
     int t = -1;
     if (V.list) {
         const int m = wl_slice(V);

@@ -638,8 +638,9 @@ __global__ __launch_bounds__(256) void k_pss_image(int n_dye, const int32_t* dye
                             const edm_photo_params* pp, const edm_image_params* ip, const Real* x,
                             const Real* y, uint8_t* fl, const int8_t* dye, int8_t* ctmc,
                             const int32_t* bond, const double* k_fret, const int32_t* acc,
-                            const int32_t* slot_of, uint32_t* flim, Rng rng, uint32_t step,
+                            const int32_t* slot_of, uint32_t* flim, Rng rng, StepRef sref,
                             unsigned long long* ctr) {
+    const uint32_t step = sref.get();
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     int emitted = 0, detected = 0;
     Emit<Real> E{0, 0, 0u, 0, 0, 0, 0, (Real)0, (Real)0};
@@ -876,6 +877,7 @@ __global__ void k_fill_u32(int64_t n, uint32_t* p, uint32_t v) {
 // ----------------------------------------------------------------------
 int ph_alloc(edm_membrane* m) {
     const int n = m->n;
+    graph_touch(m);
     if (!m->d_pp) EDM_HIP(hipMalloc((void**)&m->d_pp, sizeof(edm_photo_params)));
     EDM_HIP(hipMemcpy(m->d_pp, &m->pp, sizeof(edm_photo_params), hipMemcpyHostToDevice));
     if (!m->k_fret) {
@@ -893,6 +895,7 @@ int ph_alloc(edm_membrane* m) {
 
 static int ensure_photon_cap(edm_membrane* m, int64_t need) {
     if (need <= m->ph_cap) return 0;
+    graph_touch(m);
     int64_t cap = std::max<int64_t>(need, 1024);
     cap = cap + cap / 2;
     hipFree(m->ph_pid);
@@ -928,6 +931,7 @@ static int scan_i64(edm_membrane* m, const int64_t* in, int64_t* out, int64_t n)
     size_t need = 0;
     EDM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, in, out, (int)n, m->stream));
     if (need > m->cub_tmp_bytes) {
+        graph_touch(m);
         EDM_HIP(hipStreamSynchronize(m->stream));
         EDM_HIP(hipFree(m->cub_tmp));
         m->cub_tmp_bytes = need + 256;
@@ -974,6 +978,7 @@ static int fret_prox_grid(edm_membrane* m, int j) {
     TgGrid g{(int)ncx, (int)ncy, (double)ncx / Lx, (double)ncy / Ly};
     const int ncell = (int)(ncx * ncy);
     if (ncell + 1 > m->tg_cell_cap) {
+        graph_touch(m);
         EDM_HIP(hipStreamSynchronize(st));
         hipFree(m->tg_cnt);
         hipFree(m->tg_fill);
@@ -986,6 +991,7 @@ static int fret_prox_grid(edm_membrane* m, int j) {
         m->tg_cell_cap = ncell + 1;
     }
     if (n > m->tg_cap) {
+        graph_touch(m);
         EDM_HIP(hipStreamSynchronize(st));
         hipFree(m->tg_x);
         hipFree(m->tg_y);
@@ -1009,6 +1015,7 @@ static int fret_prox_grid(edm_membrane* m, int j) {
     EDM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, m->tg_cnt, m->tg_start, ncell + 1,
                                              st));
     if (need > m->cub_tmp_bytes) {
+        graph_touch(m);
         EDM_HIP(hipStreamSynchronize(st));
         EDM_HIP(hipFree(m->cub_tmp));
         m->cub_tmp_bytes = need + 256;
@@ -1116,7 +1123,7 @@ static const edm_image_params* dev_ip(edm_membrane* m) {
 template <typename Real, int kWin>
 static void launch_pss_image(edm_membrane* m, dim3 grid, WlView V, const edm_photo_params* dpp,
                              const edm_image_params* d_ip, const double* kf, const int32_t* ac,
-                             Rng rng, uint32_t step) {
+                             Rng rng, StepRef step) {
     hipLaunchKernelGGL((k_pss_image<Real, kWin>), grid, dim3(kB), 0, m->stream, m->n_dye,
                        (const int32_t*)m->dye_ids, V, (const int32_t*)m->a.id, dpp, d_ip,
                        (const Real*)m->a.x, (const Real*)m->a.y, m->a.fl, (const int8_t*)m->a.dye,
@@ -1148,7 +1155,7 @@ int ph_fused_image(edm_membrane* m, int64_t* n_emitted, int64_t* n_detected) {
     const edm_photo_params* dpp = (const edm_photo_params*)m->d_pp;
     Rng rng{(uint32_t)(m->p.seed & 0xffffffffu) ^ 0x5bd1e995u, (uint32_t)(m->p.seed >> 32)};
     // the photon step uses the count of completed dynamics steps as its counter
-    const uint32_t step = m->step;
+    const StepRef step = step_ref(m);
     const edm_image_params* d_ip = dev_ip(m);
     // after a step the dye carriers' slots are listed in cell order
     const WlView V = prox ? WlView{nullptr, nullptr} : wl_view(m, kWlDye);
@@ -1177,6 +1184,7 @@ int ph_fused_image(edm_membrane* m, int64_t* n_emitted, int64_t* n_detected) {
 // Imaging buffers: flim stack, pixel centres and the parameter block live
 // in one allocation after the pixel-centre arrays.
 int img_alloc(edm_membrane* m) {
+    graph_touch(m);
     const edm_image_params& ip = m->ip;
     const size_t elems = (size_t)ip.n_ch * ip.nx * ip.ny * ip.n_bins;
     if (elems != m->flim_elems) {
@@ -1222,6 +1230,7 @@ int img_detect(edm_membrane* m, const double* u_qe, int64_t n, int64_t* n_surv) 
         EDM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, need, m->sv_key, m->sv_key2, m->sv_val,
                                                    m->sv_val2, (int)ns, 0, 32, st));
         if (need > m->cub_tmp_bytes) {
+            graph_touch(m);
             EDM_HIP(hipStreamSynchronize(st));
             EDM_HIP(hipFree(m->cub_tmp));
             m->cub_tmp_bytes = need + 256;

@@ -345,23 +345,14 @@ int edm_run_frame(edm_membrane* m, int32_t n_steps) {
         set_error("edm_run_frame: a parity step is open");
         return EDM_ESTATE;
     }
-    const bool image = m->has_pp && m->has_ip;
-    for (int k = 0; k < n_steps; ++k) {
-        m->need_pair_offsets = m->p.pair_cap > 0;
-        if (int rc = dyn_step_begin(m)) return rc;
-        if (int rc = dyn_step_finish(m, nullptr, false)) return rc;
-        if (image)
-            if (int rc = ph_fused_image(m, nullptr, nullptr)) return rc;
-        if (m->rec_flags & EDM_REC_AUTO)
-            if (int rc = rec_step(m)) return rc;
-    }
-    return 0;
+    return steps_run(m, n_steps, m->has_pp && m->has_ip);
 }
 
 int edm_record_open(edm_membrane* m, int32_t capacity, int32_t flags) {
     if (!m || capacity <= 0 || capacity > (1 << 24))
         return invalid("edm_record_open: capacity must be 1 .. 2^24 rows");
     if (flags & ~(EDM_REC_FRET_SUM | EDM_REC_AUTO)) return invalid("edm_record_open: unknown flag");
+    graph_touch(m);
     if ((flags & EDM_REC_FRET_SUM) && !m->has_pp) {
         set_error("edm_record_open: the FRET sum needs the photophysics tables");
         return EDM_ESTATE;
@@ -412,6 +403,7 @@ int edm_record_read(edm_membrane* m, edm_census* rows, int32_t cap, int32_t* n_r
 
 int edm_record_close(edm_membrane* m) {
     if (!m) return invalid("edm_record_close: null handle");
+    graph_touch(m);
     if (m->rec) {
         EDM_HIP(hipStreamSynchronize(m->stream));
         EDM_HIP(hipFree(m->rec));

@@ -272,6 +272,7 @@ extern "C" {
 
 int edm_stats_open(edm_membrane* m) {
     if (!m) return sinvalid("edm_stats_open: null handle");
+    graph_touch(m);
     if (m->begun) {
         set_error("edm_stats_open: a parity step is open");
         return EDM_ESTATE;
@@ -281,6 +282,7 @@ int edm_stats_open(edm_membrane* m) {
 
 int edm_stats_close(edm_membrane* m) {
     if (!m) return sinvalid("edm_stats_close: null handle");
+    graph_touch(m);
     EDM_HIP(hipStreamSynchronize(m->stream));
     stats_free(m);
     return 0;

@@ -430,11 +430,12 @@ template <typename Real, bool kSub>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFp32Waves<Real, 7>))) void k_force_tile(Cells<Real> C, ForceParams<Real> F,
                                                     Real* __restrict__ xn, Real* __restrict__ yn,
                                                     const double* __restrict__ noise_hook,
-                                                    Rng rng, uint32_t step, Grids<Real> G,
+                                                    Rng rng, StepRef sref, Grids<Real> G,
                                                     int n_grids, uint32_t keep_mask, int ntile_y,
                                                     int ntiles) {
     const int tile = xcd_tile(ntiles);
     if (tile >= ntiles) return;  // uniform across the block
-    force_strip<Real, kSub, 1>(C, F, xn, yn, noise_hook, rng, step, G, n_grids, keep_mask,
+    if (cap_gated_off(C)) return;  // uniform across the launch
+    force_strip<Real, kSub, 1>(C, F, xn, yn, noise_hook, rng, sref.get(), G, n_grids, keep_mask,
                                strip_geo(C, tile, ntile_y));
 }

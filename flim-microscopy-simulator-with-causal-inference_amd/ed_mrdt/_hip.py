@@ -124,6 +124,9 @@ TUNE_CAT_SIDE = 2
 TUNE_CAT_WALK = 3
 TUNE_CAT_LANES = 4
 TUNE_FRET_SEARCH = 5
+TUNE_GRAPHS = 6
+TUNE_GRAPH_REPLAYS = 7
+TUNE_GRAPH_STATUS = 8
 
 # function name -> (argtypes) ; every function returns int32 status
 _SIGS = {

@@ -488,20 +488,22 @@ class DynamicsEngine:
     # ---- tuning (no reference counterpart) ----------------------------------
     _TUNE = (("tile_rows", _hip.TUNE_TILE_ROWS), ("cat_side", _hip.TUNE_CAT_SIDE),
              ("cat_walk", _hip.TUNE_CAT_WALK), ("cat_lanes", _hip.TUNE_CAT_LANES),
-             ("fret_search", _hip.TUNE_FRET_SEARCH))
+             ("fret_search", _hip.TUNE_FRET_SEARCH), ("graphs", _hip.TUNE_GRAPHS))
 
     def set_tuning(self, tile_rows: Optional[int] = None, cat_side: Optional[int] = None,
                    cat_walk: Optional[int] = None, cat_lanes: Optional[int] = None,
-                   fret_search: Optional[int] = None) -> None:
+                   fret_search: Optional[int] = None, graphs: Optional[int] = None) -> None:
         """Work-decomposition knobs that leave every result unchanged: cell
         rows per strip of the tiled kernels (1..256), the side catalysis walks
         its pairs from (0 enzymes, 1 substrates, -1 the rarer one), how
         (1 work lists where possible, 0 LDS strips), the lanes per entity
         of the strip walk (a power of two in 1..64) and how proximity FRET
         finds its targets (0 a grid of the targets, 1 a scan of every
-        particle per donor, the reference's search)."""
+        particle per donor, the reference's search), and whether runs of
+        steps are recorded into HIP graphs and replayed (1) or launched one
+        by one (0)."""
         vals = dict(tile_rows=tile_rows, cat_side=cat_side, cat_walk=cat_walk,
-                    cat_lanes=cat_lanes, fret_search=fret_search)
+                    cat_lanes=cat_lanes, fret_search=fret_search, graphs=graphs)
         for name, key in self._TUNE:
             if vals[name] is not None:
                 check(lib().edm_set_tuning(self._h, key, int(vals[name])), "edm_set_tuning")
@@ -513,6 +515,23 @@ class DynamicsEngine:
             check(lib().edm_get_tuning(self._h, key, ctypes.byref(v)), "edm_get_tuning")
             out[name] = int(v.value)
         return out
+
+    @property
+    def graph_replays(self) -> int:
+        """Step graphs replayed so far (see set_tuning(graphs=...))."""
+        v = ctypes.c_int64(0)
+        check(lib().edm_get_tuning(self._h, _hip.TUNE_GRAPH_REPLAYS, ctypes.byref(v)),
+              "edm_get_tuning")
+        return int(v.value)
+
+    @property
+    def graph_status(self) -> int:
+        """0, or the code of the last failed graph recording (the library's
+        header lists them); the membrane then steps launch by launch."""
+        v = ctypes.c_int64(0)
+        check(lib().edm_get_tuning(self._h, _hip.TUNE_GRAPH_STATUS, ctypes.byref(v)),
+              "edm_get_tuning")
+        return int(v.value)
 
     # ---- counters (dynamics.py:2068-2071, 2147-2148) ----------------------
     def _counters(self):

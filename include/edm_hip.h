@@ -203,12 +203,31 @@ int edm_photon_counters(edm_membrane *m, int64_t out[2]);
  *                       walk: a power of two in 1..64 (default 16)
  *   EDM_TUNE_FRET_SEARCH  proximity FRET finds targets through a grid of the
  *                       alive targets (0, default) or by scanning every
- *                       particle for every donor (1), as the reference does */
+ *                       particle for every donor (1), as the reference does
+ *   EDM_TUNE_GRAPHS     1: edm_run, edm_run_frame and edm_time_steps
+ *                       record stretches of up to 256 steps into HIP graphs
+ *                       and replay them (Philox draws, no per-step census;
+ *                       other steps, and runs shorter than 4 steps, go
+ *                       launch by launch); 0 (default): every step launch
+ *                       by launch.  On ROCm 7.2 a replayed step takes
+ *                       20-35 us longer than the same launches issued one
+ *                       by one, for 4k to 1M particles
+ *   EDM_TUNE_GRAPH_REPLAYS  (read only) graph replays so far
+ *   EDM_TUNE_GRAPH_STATUS   (read only) 0, or why a recording failed (the
+ *                       membrane then steps launch by launch): 1000 + the
+ *                       HIP error of hipStreamBeginCapture, 2000 + the EDM
+ *                       error of a recorded step, 3000 + the HIP error of
+ *                       hipStreamEndCapture, 4000 + that of
+ *                       hipGraphInstantiate, 5000 an allocation while
+ *                       recording                                          */
 #define EDM_TUNE_TILE_ROWS 1
 #define EDM_TUNE_CAT_SIDE 2
 #define EDM_TUNE_CAT_WALK 3
 #define EDM_TUNE_CAT_LANES 4
 #define EDM_TUNE_FRET_SEARCH 5
+#define EDM_TUNE_GRAPHS 6
+#define EDM_TUNE_GRAPH_REPLAYS 7
+#define EDM_TUNE_GRAPH_STATUS 8
 int edm_set_tuning(edm_membrane *m, int32_t key, int64_t value);
 int edm_get_tuning(edm_membrane *m, int32_t key, int64_t *value);
 /* Reference pair list in reference order (CellList.get_neighbor_pairs,
@@ -259,7 +278,9 @@ int edm_synchronize(edm_membrane *m);
 /* Runs n_steps plain steps (+ fused imaging every step when image != 0,
  * + a census row when recording with EDM_REC_AUTO), returning total
  * elapsed ms and the summed duration of the force/BD kernel launches
- * measured with events around each launch. */
+ * measured with events around each launch.  When the steps run as replayed
+ * graphs (EDM_TUNE_GRAPHS) the launches are not timed one by one and
+ * *force_ms is -1. */
 int edm_time_steps(edm_membrane *m, int32_t n_steps, int32_t image, double *total_ms,
                    double *force_ms, double *breakdown_ms /* [8] or NULL */);
 /* Plain run of a stretch of a frame: n_steps x (dynamics step, then the
