@@ -390,16 +390,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFp32Waves<
     cat_strip<Real, 1>(C, P, sA, sA.step, ctr, G, n_grids, strip_geo(C, tile, ntile_y));
 }
 
-// Catalysis over a work list of its walk entities, one listed slot per
-// thread: an alive enzyme against the substrates of its 3x3 cells, or (by
-// substrate) an alive substrate against the enzymes of its 3x3 cells, read
-// from global memory.  Used when no enzyme species is the run's substrate
-// or product, so that no entity changes identity during the run: species
-// read as the run goes then agree with the run's start.
+// Catalysis over a work list of its walk entities: an alive enzyme against
+// the substrates of its 3x3 cells, or (by substrate) an alive substrate
+// against the enzymes of its 3x3 cells, read from global memory.  A group
+// of 2^group_shift lanes takes one listed entity and splits the candidates
+// of its nine cells (their slot ranges concatenated) between its lanes, so
+// a walk costs a couple of dependent loads per lane instead of one per
+// candidate.  Used when no enzyme species is the run's substrate or
+// product, so that no entity changes identity during the run: species read
+// as the run goes then agree with the run's start.
 template <typename Real>
-__device__ __forceinline__ int cat_walk(const Cells<Real>& C, const Post<Real>& P,
-                                        const CatArgs<Real>& A, const Grids<Real>& G,
-                                        int n_grids, int t) {
+__device__ __forceinline__ int cat_walk_group(const Cells<Real>& C, const Post<Real>& P,
+                                              const CatArgs<Real>& A, const Grids<Real>& G,
+                                              int n_grids, int t, int lane, int gsize) {
     if (!(C.fl[t] & EDM_FLAG_ALIVE)) return 0;
     const int st = P.sp[t];
     const bool by_sub = A.by_substrate;
@@ -409,53 +412,71 @@ __device__ __forceinline__ int cat_walk(const Cells<Real>& C, const Post<Real>& 
         for (int j = 0; j < kCatFuse; ++j) enz |= j < A.n_enz && st == A.enz[j];
         if (by_sub || !enz) return 0;
     }
-    const Real xt = C.x[t], yt = C.y[t], xnt = P.xn[t], ynt = P.yn[t];
+    const Real xt = C.x[t], yt = C.y[t];
     const int it = C.id[t];
     int cx, cy;
     cell_of(C, xt, yt, cx, cy);
-    int hits = 0;
+    int beg[9], len[9];
+    int total = 0;
+#pragma unroll
     for (int k = 0; k < 9; ++k) {
         const int c = nb_cell(C, cx, cy, k);
-        if (c < 0) continue;
-        const int ue = (int)C.start[c + 1];
-        for (int u = (int)C.start[c]; u < ue; ++u) {
-            if (u == t || !(C.fl[u] & EDM_FLAG_ALIVE)) continue;
-            const int su = P.sp[u];
-            Real dx, dy;
-            if (by_sub) {  // u a candidate enzyme of the substrate t
-                bool enz = false;
+        beg[k] = c < 0 ? 0 : (int)C.start[c];
+        len[k] = c < 0 ? 0 : (int)C.start[c + 1] - beg[k];
+        total += len[k];
+    }
+    int hits = 0;
+    for (int q = lane; q < total; q += gsize) {
+        // candidate q of the concatenated ranges (unrolled: no indexed
+        // private arrays)
+        int u = -1, r = q;
 #pragma unroll
-                for (int j = 0; j < kCatFuse; ++j) enz |= j < A.n_enz && su == A.enz[j];
-                if (!enz) continue;
-                if (min_image(C.x[u], C.y[u], xt, yt, C.Lx, C.Ly, C.px, C.py, dx, dy) >= C.cut2)
-                    continue;
-                const int iu = C.id[u];
-                const Real xnu = P.xn[u], ynu = P.yn[u];
-                for (int j = 0; j < A.n_enz; ++j)
-                    if (su == cat_pick(A.enz, j))
-                        hits += cat_try(C, P, A, A.step, j, G, n_grids, u, iu, t, xnu, ynu, xnt,
-                                        ynt);
-            } else {  // u a candidate substrate of the enzyme t
-                if (su != A.sub) continue;
-                if (min_image(xt, yt, C.x[u], C.y[u], C.Lx, C.Ly, C.px, C.py, dx, dy) >= C.cut2)
-                    continue;
-                const Real xnu = P.xn[u], ynu = P.yn[u];
-                for (int j = 0; j < A.n_enz; ++j)
-                    if (st == cat_pick(A.enz, j))
-                        hits += cat_try(C, P, A, A.step, j, G, n_grids, t, it, u, xnt, ynt, xnu,
-                                        ynu);
+        for (int k = 0; k < 9; ++k) {
+            if (u < 0) {
+                if (r < len[k]) u = beg[k] + r;
+                else r -= len[k];
             }
+        }
+        if (u == t || !(C.fl[u] & EDM_FLAG_ALIVE)) continue;
+        const int su = P.sp[u];
+        Real dx, dy;
+        if (by_sub) {  // u a candidate enzyme of the substrate t
+            bool enz = false;
+#pragma unroll
+            for (int j = 0; j < kCatFuse; ++j) enz |= j < A.n_enz && su == A.enz[j];
+            if (!enz) continue;
+            if (min_image(C.x[u], C.y[u], xt, yt, C.Lx, C.Ly, C.px, C.py, dx, dy) >= C.cut2)
+                continue;
+            const int iu = C.id[u];
+            const Real xnu = P.xn[u], ynu = P.yn[u], xnt = P.xn[t], ynt = P.yn[t];
+            for (int j = 0; j < A.n_enz; ++j)
+                if (su == cat_pick(A.enz, j))
+                    hits += cat_try(C, P, A, A.step, j, G, n_grids, u, iu, t, xnu, ynu, xnt, ynt);
+        } else {  // u a candidate substrate of the enzyme t
+            if (su != A.sub) continue;
+            if (min_image(xt, yt, C.x[u], C.y[u], C.Lx, C.Ly, C.px, C.py, dx, dy) >= C.cut2)
+                continue;
+            const Real xnu = P.xn[u], ynu = P.yn[u], xnt = P.xn[t], ynt = P.yn[t];
+            for (int j = 0; j < A.n_enz; ++j)
+                if (st == cat_pick(A.enz, j))
+                    hits += cat_try(C, P, A, A.step, j, G, n_grids, t, it, u, xnt, ynt, xnu, ynu);
         }
     }
     return hits;
 }
 
+// Grid (list regions, kWlPer): block (b, y) takes the entities y * per,
+// y * per + kWlPer * per, ... of region b, per = 256 >> group_shift of them
+// at a time.
 template <typename Real>
 __global__ __launch_bounds__(256) void k_cat_list(Cells<Real> C, Post<Real> P, CatArgs<Real> A,
                                                   WlView V, unsigned long long* ctr,
                                                   Grids<Real> G, int n_grids) {
-    const int m = wl_slice(V);
-    if (m == 0) return;  // uniform: an empty slice
+    const int cnt = (int)V.cnt[blockIdx.x];
+    const int gs = A.group_shift;
+    const int per = (int)blockDim.x >> gs;
+    const int first = (int)blockIdx.y * per;
+    if (first >= cnt) return;  // uniform: nothing listed for this block
     // the parameter block is indexed with per-lane values: keep it in LDS
     // rather than in per-lane scratch
     __shared__ CatArgs<Real> sA;
@@ -464,7 +485,11 @@ __global__ __launch_bounds__(256) void k_cat_list(Cells<Real> C, Post<Real> P, C
         if (A.step_base) sA.step += *A.step_base;
     }
     __syncthreads();
-    const int hits = (int)threadIdx.x < m ? cat_walk(C, P, sA, G, n_grids, wl_slot(V)) : 0;
+    const int grp = (int)threadIdx.x >> gs, lane = (int)threadIdx.x & ((1 << gs) - 1);
+    int hits = 0;
+    for (int e = first + grp; e < cnt; e += (int)gridDim.y * per)
+        hits += cat_walk_group(C, P, sA, G, n_grids, V.list[(size_t)blockIdx.x * kWlSlots + e],
+                               lane, 1 << gs);
     unsigned long long v = hits;
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
     if (lane_id() == 0 && v) atomicAdd(&ctr[2], v);

@@ -56,13 +56,16 @@ __device__ __forceinline__ void cell_of(const Cells<Real>& C, Real x, Real y, in
 
 // Neighbour cell k (0..8) of (cx, cy) in the reference's (dcx, dcy)
 // lexicographic order (dynamics.py:244-258); -1 when outside a wall.
+// (cx, cy) is a cell of the grid and the offsets are -1..1, so a periodic
+// wrap is one conditional add or subtract: the same cell as the modulo.
 template <typename Real>
 __device__ __forceinline__ int nb_cell(const Cells<Real>& C, int cx, int cy, int k) {
-    int nx = cx + (k / 3) - 1;
-    int ny = cy + (k % 3) - 1;
-    if (C.px) nx = ((nx % C.ncx) + C.ncx) % C.ncx;
+    const int q = k >= 6 ? 2 : (k >= 3 ? 1 : 0);
+    int nx = cx + q - 1;
+    int ny = cy + (k - 3 * q) - 1;
+    if (C.px) nx = nx < 0 ? nx + C.ncx : (nx >= C.ncx ? nx - C.ncx : nx);
     else if (nx < 0 || nx >= C.ncx) return -1;
-    if (C.py) ny = ((ny % C.ncy) + C.ncy) % C.ncy;
+    if (C.py) ny = ny < 0 ? ny + C.ncy : (ny >= C.ncy ? ny - C.ncy : ny);
     else if (ny < 0 || ny >= C.ncy) return -1;
     return nx * C.ncy + ny;
 }

@@ -435,6 +435,43 @@ int edm_ens_counters(edm_ensemble* e, int32_t slot, int64_t out[8]) {
     return 0;
 }
 
+int edm_ens_set_tuning(edm_ensemble* e, int32_t key, int64_t value) {
+    if (!e) return ens_invalid("edm_ens_set_tuning: null handle");
+    switch (key) {
+        case EDM_ENS_TUNE_STRIP_PARTICLES:
+            if (value < 32 || value > 512)
+                return ens_invalid("edm_ens_set_tuning: strip particles must be in 32..512");
+            e->strip_parts = (int)value;
+            break;
+        case EDM_ENS_TUNE_PAIR_BELOW:
+            if (value < 0 || value > 1024)
+                return ens_invalid("edm_ens_set_tuning: pair-below must be in 0..1024");
+            e->pair_below = (int)value;
+            break;
+        case EDM_ENS_TUNE_CAT_LANES:
+            if (value < 1 || value > 64 || (value & (value - 1)))
+                return ens_invalid("edm_ens_set_tuning: catalysis lanes must be a power of two "
+                                   "in 1..64");
+            e->cat_lanes = (int)value;
+            break;
+        default:
+            return ens_invalid("edm_ens_set_tuning: unknown key");
+    }
+    // strips and catalysis tables are rebuilt before the next step
+    e->layout_dirty = true;
+    return 0;
+}
+
+int edm_ens_get_tuning(edm_ensemble* e, int32_t key, int64_t* value) {
+    if (!e || !value) return ens_invalid("edm_ens_get_tuning: null argument");
+    switch (key) {
+        case EDM_ENS_TUNE_STRIP_PARTICLES: *value = e->strip_parts; return 0;
+        case EDM_ENS_TUNE_PAIR_BELOW: *value = e->pair_below; return 0;
+        case EDM_ENS_TUNE_CAT_LANES: *value = e->cat_lanes; return 0;
+        default: return ens_invalid("edm_ens_get_tuning: unknown key");
+    }
+}
+
 int edm_ens_info(edm_ensemble* e, int64_t out[4]) {
     if (!e || !out) return ens_invalid("edm_ens_info: null argument");
     int64_t live = 0;

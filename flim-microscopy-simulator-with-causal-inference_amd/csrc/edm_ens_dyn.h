@@ -374,7 +374,7 @@ static CatArgs<Real> ens_cat_host(const edm_ensemble* e, int start, int nf, cons
     A.step = 0;
     A.deferred = r.fb1.sensor >= 0 && (r.fb1.sensor == r.b || r.fb1.sensor == r.c);
     A.by_substrate = cat_by_substrate(e->seg_alive[&g - e->seg.data()], A.enz, nf, r.b);
-    A.group_shift = 4;  // 16 lanes per entity
+    A.group_shift = __builtin_ctz((unsigned)e->cat_lanes);
     return A;
 }
 
@@ -505,9 +505,10 @@ int ens_create(edm_ensemble* e) { return e->fp64 ? ens_create_t<double>(e) : ens
 // the strip table, and buffer growth.  Runs between steps only.
 int ens_layout(edm_ensemble* e) {
     const int ns = e->p.n_species;
-    // strip height: about 192 own particles per 256-thread workgroup at the
-    // live replicas' mean density, with the staged 3 x (rows + 2) cells well
-    // inside the LDS capacity (1024 particles); results do not depend on it
+    // strip height: about strip_parts (192 by default) own particles per
+    // 256-thread workgroup at the live replicas' mean density, with the
+    // staged 3 x (rows + 2) cells well inside the LDS capacity (1024
+    // particles); results do not depend on it
     e->tile_rows = 1;
     e->tile_cols = 1;
     int64_t coff = 0, cnt = 0;
@@ -529,15 +530,16 @@ int ens_layout(edm_ensemble* e) {
             g.goff[kp.first * ns + kp.second] = (int32_t)cnt;
             cnt += (int64_t)g.gdims[2 * kp.first] * g.gdims[2 * kp.first + 1];
         }
-        // strips of about 192 own particles: a column that holds fewer than
-        // 128 is taken together with its neighbour (boxes of 4 columns or
-        // more), a long one is cut into equal runs of rows; at most ~700
-        // particles (1024 with room for fluctuations) staged per strip
+        // strips of about strip_parts own particles: a column that holds
+        // fewer than pair_below is taken together with its neighbour (boxes
+        // of 4 columns or more), a long one is cut into equal runs of rows;
+        // at most ~700 particles (1024 with room for fluctuations) staged
+        // per strip
         {
             const double dens = std::max((double)g.n / ((double)g.ncx * g.ncy), 1e-3);
             const double col_parts = dens * g.ncy;
-            const int ncol = (col_parts < 128.0 && g.ncx >= 4) ? 2 : 1;
-            int ty = std::max(1, (int)std::lround(col_parts * ncol / 192.0));
+            const int ncol = (col_parts < (double)e->pair_below && g.ncx >= 4) ? 2 : 1;
+            int ty = std::max(1, (int)std::lround(col_parts * ncol / (double)e->strip_parts));
             ty = std::max(ty, (int)std::ceil((ncol + 2) * (g.ncy + 2) * dens / 700.0));
             ty = std::max(ty, (g.ncy + kTileRows - 1) / kTileRows);
             ty = std::min(ty, g.ncy);

@@ -119,6 +119,13 @@ struct edm_ensemble {
     std::vector<EnsTile> tiles;
     int tile_rows = 64;  // most cell rows of any strip (sizes the strips' dynamic LDS)
     int tile_cols = 1;   // most own columns of any strip
+    // tuning (every value gives the same results): own particles a strip
+    // aims at, columns holding fewer particles than pair_below are taken
+    // two at a time (boxes of 4 columns or more; 0: never), lanes per
+    // entity of the catalysis walk
+    int strip_parts = 192;
+    int pair_below = 128;
+    int cat_lanes = 16;
     EnsTile* d_tiles = nullptr;
     int64_t tile_cap = 0;
     int64_t n_total = 0;  // particles in the pool (live replicas first after a sort)

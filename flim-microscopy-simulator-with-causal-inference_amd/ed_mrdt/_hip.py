@@ -127,6 +127,9 @@ TUNE_FRET_SEARCH = 5
 TUNE_GRAPHS = 6
 TUNE_GRAPH_REPLAYS = 7
 TUNE_GRAPH_STATUS = 8
+ENS_TUNE_STRIP_PARTICLES = 1
+ENS_TUNE_PAIR_BELOW = 2
+ENS_TUNE_CAT_LANES = 3
 
 # function name -> (argtypes) ; every function returns int32 status
 _SIGS = {
@@ -186,6 +189,8 @@ _SIGS = {
     "edm_ens_download": [c_void_p, c_int32, ctypes.POINTER(HostState)],
     "edm_ens_counters": [c_void_p, c_int32, c_void_p],
     "edm_ens_info": [c_void_p, c_void_p],
+    "edm_ens_set_tuning": [c_void_p, c_int32, c_int64],
+    "edm_ens_get_tuning": [c_void_p, c_int32, c_void_p],
     "edm_ens_synchronize": [c_void_p],
     # observables
     "edm_stats_open": [c_void_p],

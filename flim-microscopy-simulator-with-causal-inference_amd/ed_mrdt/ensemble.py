@@ -271,6 +271,31 @@ class EnsembleEngine:
         check(lib().edm_ens_counters(self._h, int(slot), ptr(out)), "edm_ens_counters")
         return dict(zip(self._CTR, (int(v) for v in out)))
 
+    # ---- tuning (no reference counterpart) ------------------------------------
+    _TUNE = (("strip_particles", _hip.ENS_TUNE_STRIP_PARTICLES),
+             ("pair_below", _hip.ENS_TUNE_PAIR_BELOW), ("cat_lanes", _hip.ENS_TUNE_CAT_LANES))
+
+    def set_tuning(self, strip_particles: Optional[int] = None, pair_below: Optional[int] = None,
+                   cat_lanes: Optional[int] = None) -> None:
+        """Work decomposition of the batched step, leaving every result
+        unchanged: own particles a strip aims at (32..512), the column
+        population below which two columns share a strip (0..1024, 0 never)
+        and the lanes per catalysis entity (a power of two in 1..64).
+        Applies from the next step."""
+        vals = dict(strip_particles=strip_particles, pair_below=pair_below, cat_lanes=cat_lanes)
+        for name, key in self._TUNE:
+            if vals[name] is not None:
+                check(lib().edm_ens_set_tuning(self._h, key, int(vals[name])),
+                      "edm_ens_set_tuning")
+
+    def tuning(self) -> Dict[str, int]:
+        out = {}
+        for name, key in self._TUNE:
+            v = ctypes.c_int64(0)
+            check(lib().edm_ens_get_tuning(self._h, key, ctypes.byref(v)), "edm_ens_get_tuning")
+            out[name] = int(v.value)
+        return out
+
     def info(self) -> Dict[str, int]:
         out = np.zeros(4, dtype=np.int64)
         check(lib().edm_ens_info(self._h, ptr(out)), "edm_ens_info")

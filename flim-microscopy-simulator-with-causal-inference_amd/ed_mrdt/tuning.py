@@ -9,11 +9,20 @@ every result bit-identical, so the fastest setting can be found by timing.
 `autotune` builds a scratch engine from the configuration (its own initial
 state, Philox draws), times each candidate over a few steps and returns the
 fastest settings; `DynamicsEngine.set_tuning(**settings)` applies them.
+
+`autotune_ensemble` does the same for the batched step of a set of
+replicas: the own particles per strip workgroup, the column population
+below which two columns share a strip, and the lanes per catalysis entity
+(`EnsembleEngine.set_tuning(**settings)`).  It times the candidates on the
+live engine it is given (the settings leave results unchanged, so the
+timed steps are ordinary steps of the run), or on a scratch engine built
+from the replicas' configurations.
 """
 
 from __future__ import annotations
 
 import ctypes
+import time
 from typing import Dict, Iterable
 
 from . import _hip
@@ -79,3 +88,56 @@ def autotune(config: SimulationConfig, precision: str = "fp32", steps: int = 20,
         eng.close()
     return dict(tile_rows=best_rows, cat_side=best_side, cat_walk=best_walk,
                 cat_lanes=best_lanes, times_ms=times)
+
+
+ENS_STRIP_PARTICLES = (96, 128, 192, 256, 384)
+ENS_PAIR_BELOW = (0, 64, 128, 256)
+ENS_CAT_LANES = (64, 16, 4)
+
+
+def autotune_ensemble(engine=None, configs=None, steps: int = 10, warmup: int = 2,
+                      strip_particles: Iterable[int] = ENS_STRIP_PARTICLES,
+                      pair_below: Iterable[int] = ENS_PAIR_BELOW,
+                      cat_lanes: Iterable[int] = ENS_CAT_LANES) -> Dict:
+    """Fastest cat_lanes, pair_below and strip_particles of the batched step
+    (tuned one after another in that order, each with the best of the earlier
+    ones).  With `engine` the candidates are timed on it (its replicas
+    advance by the timed steps, with results independent of the settings)
+    and the best settings are left applied; with `configs` a scratch
+    ensemble of those replicas is built and closed.  `times_ms` holds the
+    milliseconds per step of every candidate."""
+    from .ensemble import EnsembleEngine
+    from .particles import ParticleState
+
+    own = engine is None
+    if own:
+        if not configs:
+            raise ValueError("autotune_ensemble needs an engine or replica configurations")
+        states = [ParticleState(c) for c in configs]
+        cap = sum(s.N for s in states) + 1024
+        engine = EnsembleEngine(configs[0], cap, max_replicas=len(configs))
+        for c, s in zip(configs, states):
+            engine.add(c, s)
+
+    def per_step() -> float:
+        engine.run(int(warmup))
+        engine.synchronize()
+        t0 = time.perf_counter()
+        engine.run(int(steps))
+        engine.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps
+
+    times: Dict[str, float] = {}
+    try:
+        best = {}
+        for name, cands in (("cat_lanes", list(cat_lanes)), ("pair_below", list(pair_below)),
+                            ("strip_particles", list(strip_particles))):
+            for v in cands:
+                engine.set_tuning(**{name: v})
+                times[f"{name}={v}"] = per_step()
+            best[name] = min(cands, key=lambda v: times[f"{name}={v}"])
+            engine.set_tuning(**{name: best[name]})
+    finally:
+        if own:
+            engine.close()
+    return dict(best, times_ms=times)

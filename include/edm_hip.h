@@ -563,6 +563,21 @@ int edm_ens_download(edm_ensemble *e, int32_t slot, edm_host_state *s);
 int edm_ens_counters(edm_ensemble *e, int32_t slot, int64_t out[8]);
 /* pooled particles, live cells, resident replicas, particle capacity */
 int edm_ens_info(edm_ensemble *e, int64_t out[4]);
+/* Tuning of the batched step's work decomposition (no reference
+ * counterpart); every value gives bit-identical results, so an autotuner
+ * may time them on the live ensemble.  Takes effect at the next step.
+ *   EDM_ENS_TUNE_STRIP_PARTICLES  own particles a strip workgroup aims at,
+ *                                 32..512 (default 192)
+ *   EDM_ENS_TUNE_PAIR_BELOW       columns holding fewer particles are
+ *                                 walked two at a time, 0..1024 (default
+ *                                 128; 0: never)
+ *   EDM_ENS_TUNE_CAT_LANES        lanes per catalysis entity, a power of
+ *                                 two in 1..64 (default 16)                */
+#define EDM_ENS_TUNE_STRIP_PARTICLES 1
+#define EDM_ENS_TUNE_PAIR_BELOW 2
+#define EDM_ENS_TUNE_CAT_LANES 3
+int edm_ens_set_tuning(edm_ensemble *e, int32_t key, int64_t value);
+int edm_ens_get_tuning(edm_ensemble *e, int32_t key, int64_t *value);
 int edm_ens_synchronize(edm_ensemble *e);
 
 #ifdef __cplusplus

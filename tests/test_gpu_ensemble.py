@@ -158,3 +158,59 @@ def test_local_recruitment_replicas_match_single_membrane_path():
     _same(ens.state(slots[3]), singles[3][0], "replica 3 after the removal")
     assert ens.counters(slots[3])["n_recruited"] == singles[3][1]
     ens.close()
+
+
+def _tuned_run(tune_schedule, n_rep=12, steps=(60, 40, 50)):
+    cfgs = [workloads.ensemble_replica(r) for r in range(n_rep)]
+    states = [ParticleState(c) for c in cfgs]
+    ens = EnsembleEngine(cfgs[0], sum(s.N for s in states) + 1024, max_replicas=n_rep)
+    slots = [ens.add(c, s) for c, s in zip(cfgs, states)]
+    frames = []
+    for k, n in enumerate(steps):
+        if tune_schedule[k]:
+            ens.set_tuning(**tune_schedule[k])
+        ens.run(n)
+        frames.append([ens.frame(sl).flim_stack.copy() for sl in slots])
+    out = [ens.state(sl) for sl in slots], frames, [ens.counters(sl) for sl in slots]
+    tuned = ens.tuning()
+    ens.close()
+    return out, tuned
+
+
+def test_ensemble_tuning_leaves_results_unchanged():
+    """Strip sizes, column pairing and catalysis lanes (set before the run
+    and changed between frames) give the batched step's results bit for
+    bit: states, counters and every frame of 12 replicas of 0.9k-31k
+    particles."""
+    (ref, _) = _tuned_run([None, None, None])
+    sched = [dict(strip_particles=96, pair_below=0, cat_lanes=4),
+             dict(strip_particles=384, pair_below=256, cat_lanes=64),
+             dict(strip_particles=128, pair_below=1024, cat_lanes=1)]
+    (got, tuned) = _tuned_run(sched)
+    assert tuned == sched[-1]
+    for i, (a, b) in enumerate(zip(ref[0], got[0])):
+        _same(a, b, f"replica {i}")
+    for fa, fb in zip(ref[1], got[1]):
+        for x, y in zip(fa, fb):
+            np.testing.assert_array_equal(x, y)
+    assert ref[2] == got[2]
+    assert sum(int(x.sum()) for x in ref[1][-1]) > 0
+
+
+def test_ensemble_autotuner_picks_from_timed_candidates():
+    from ed_mrdt import tuning
+    cfgs = [workloads.ensemble_replica(r) for r in range(24)]
+    states = [ParticleState(c) for c in cfgs]
+    ens = EnsembleEngine(cfgs[0], sum(s.N for s in states) + 1024, max_replicas=24)
+    for c, s in zip(cfgs, states):
+        ens.add(c, s)
+    res = tuning.autotune_ensemble(ens, steps=5, warmup=1)
+    t = res["times_ms"]
+    for name, cands in (("cat_lanes", tuning.ENS_CAT_LANES),
+                        ("pair_below", tuning.ENS_PAIR_BELOW),
+                        ("strip_particles", tuning.ENS_STRIP_PARTICLES)):
+        assert res[name] == min(cands, key=lambda v: t[f"{name}={v}"])
+    assert ens.tuning() == {k: res[k] for k in ("strip_particles", "pair_below", "cat_lanes")}
+    print("[ens-tune]", {k: res[k] for k in ("strip_particles", "pair_below", "cat_lanes")},
+          {k: round(v, 3) for k, v in t.items()}, flush=True)
+    ens.close()
