@@ -135,6 +135,73 @@ int orc_cell_list_pairs(const double *pos, int N, const int *head,
     return (int)n;
 }
 
+/* orc_cell_list_pairs on n_threads threads (the timing variant with every
+ * loop threaded; the serial routine above is the restatement): the pairs of
+ * each i are counted in parallel, an exclusive scan over i gives every i its
+ * offset in the list, and each i's pairs are written there, so the list, its
+ * order and the cap are the serial routine's. */
+static long long pairs_of(const double *pos, const int *head, const int *next,
+                          int nx, int ny, double csx, double csy, double Lx,
+                          double Ly, int px, int py, double cut2, int i,
+                          int *out, long long room) {
+    double xi = pos[2 * i], yi = pos[2 * i + 1];
+    int cxi = clampi((int)(xi / csx), 0, nx);
+    int cyi = clampi((int)(yi / csy), 0, ny);
+    long long k = 0;
+    for (int dcx = -1; dcx <= 1; ++dcx) {
+        for (int dcy = -1; dcy <= 1; ++dcy) {
+            int ncx = cxi + dcx, ncy = cyi + dcy;
+            if (px) ncx = ((ncx % nx) + nx) % nx;
+            else if (ncx < 0 || ncx >= nx) continue;
+            if (py) ncy = ((ncy % ny) + ny) % ny;
+            else if (ncy < 0 || ncy >= ny) continue;
+            for (int j = head[ncx * ny + ncy]; j >= 0; j = next[j]) {
+                if (j <= i) continue;
+                double dx, dy;
+                if (min_image(xi, yi, pos[2 * j], pos[2 * j + 1], Lx, Ly, px, py,
+                              &dx, &dy) < cut2) {
+                    if (out && k < room) {
+                        out[2 * k] = i;
+                        out[2 * k + 1] = j;
+                    }
+                    ++k;
+                }
+            }
+        }
+    }
+    return k;
+}
+
+int orc_cell_list_pairs_par(const double *pos, int N, const int *head,
+                            const int *next, int nx, int ny, double csx,
+                            double csy, double Lx, double Ly, int px, int py,
+                            double cutoff, long long max_pairs, int *pairs,
+                            long long *n_found, int n_threads) {
+    const double cut2 = cutoff * cutoff;
+    const int nt = n_threads > 0 ? n_threads : 1;
+    long long *off = (long long *)malloc(((size_t)N + 1) * sizeof(long long));
+    if (!off) return -1;
+#pragma omp parallel for schedule(static) num_threads(nt)
+    for (int i = 0; i < N; ++i)
+        off[i] = pairs_of(pos, head, next, nx, ny, csx, csy, Lx, Ly, px, py, cut2, i, NULL, 0);
+    long long run = 0;
+    for (int i = 0; i < N; ++i) {
+        const long long c = off[i];
+        off[i] = run;
+        run += c;
+    }
+    off[N] = run;
+#pragma omp parallel for schedule(static) num_threads(nt)
+    for (int i = 0; i < N; ++i) {
+        if (off[i] >= max_pairs || off[i + 1] == off[i]) continue;
+        pairs_of(pos, head, next, nx, ny, csx, csy, Lx, Ly, px, py, cut2, i,
+                 pairs + 2 * off[i], max_pairs - off[i]);
+    }
+    if (n_found) *n_found = run;
+    free(off);
+    return (int)(run < max_pairs ? run : max_pairs);
+}
+
 /* dynamics.py:291-307 _brute_force_pairs_numba (validation helper). */
 int orc_brute_force_pairs(const double *pos, int N, double cutoff, double Lx,
                           double Ly, int px, int py, long long max_pairs,
@@ -303,6 +370,52 @@ void orc_forces_harmonic(const double *pos, double *forces, const int *species,
         forces[2 * i] += fx;
         forces[2 * i + 1] += fy;
         forces[2 * j] -= fx;
+        forces[2 * j + 1] -= fy;
+    }
+}
+
+/* orc_forces_wca / orc_forces_harmonic over the pair list on n_threads
+ * threads (the timing variant with every loop threaded): the same pair
+ * terms, added with atomic updates, so the sums may differ from the serial
+ * routines' in the last bits. */
+void orc_forces_par(int harmonic, const double *pos, double *forces,
+                    const int *species, const uint8_t *alive, const int *pairs,
+                    int n_pairs, const double *sig_m, const double *p_m, int n_sp,
+                    double Lx, double Ly, int px, int py, int n_threads) {
+    const int nt = n_threads > 0 ? n_threads : 1;
+#pragma omp parallel for schedule(static) num_threads(nt)
+    for (int p = 0; p < n_pairs; ++p) {
+        int i = pairs[2 * p], j = pairs[2 * p + 1];
+        if (!alive[i] || !alive[j]) continue;
+        double sigma = sig_m[species[i] * n_sp + species[j]];
+        double a = p_m[species[i] * n_sp + species[j]];
+        if (sigma <= 0.0 || a <= 0.0) continue;
+        double dx, dy;
+        double dsq = min_image(pos[2 * i], pos[2 * i + 1], pos[2 * j],
+                               pos[2 * j + 1], Lx, Ly, px, py, &dx, &dy);
+        double f;
+        if (harmonic) {
+            if (dsq >= sigma * sigma || dsq < 1e-30) continue;
+            double r = sqrt(dsq);
+            f = a * (sigma - r) / r;
+        } else {
+            double rcut2 = sigma * sigma * 1.2599210498948732;
+            if (dsq >= rcut2 || dsq < 1e-30) continue;
+            double inv_r2 = 1.0 / dsq;
+            double s2 = sigma * sigma * inv_r2;
+            double s6 = s2 * s2 * s2;
+            f = 24.0 * a * (2.0 * s6 * s6 - s6) * inv_r2;
+            double fmax = 2400.0 * a / (sigma * sigma);
+            if (f > fmax) f = fmax;
+        }
+        double fx = f * dx, fy = f * dy;
+#pragma omp atomic
+        forces[2 * i] += fx;
+#pragma omp atomic
+        forces[2 * i + 1] += fy;
+#pragma omp atomic
+        forces[2 * j] -= fx;
+#pragma omp atomic
         forces[2 * j + 1] -= fy;
     }
 }

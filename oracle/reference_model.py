@@ -76,6 +76,7 @@ def lib():
             build()
         _lib = ctypes.CDLL(LIB_PATH)
         _lib.orc_cell_list_pairs.restype = ctypes.c_int
+        _lib.orc_cell_list_pairs_par.restype = ctypes.c_int
         _lib.orc_poisson_draw.restype = ctypes.c_int
         _lib.orc_poisson_draw.argtypes = [ctypes.c_double, ctypes.c_double]
         _lib.orc_hill.restype = ctypes.c_double
@@ -178,11 +179,16 @@ class OracleDynamics:
     """Faithful FP64 restatement of DynamicsEngine (host constants + step)."""
 
     def __init__(self, config, state: OracleState, pair_cap: bool = True,
-                 n_threads: int = 1):
+                 n_threads: int = 1, all_loops: bool = False):
+        """n_threads: threads of bd_step (the reference's prange); with
+        all_loops the pair list and the pair forces run on them too (a
+        timing variant: the same pairs in the same order, forces summed
+        with atomic adds)."""
         self.config = config
         self.state = state
         self.pair_cap = pair_cap
         self.n_threads = n_threads
+        self.all_loops = all_loops
         Lx, Ly = config.membrane.size
         self.Lx, self.Ly = float(Lx), float(Ly)
         self.px, self.py = (1 if p else 0 for p in config.membrane.periodic)
@@ -382,10 +388,15 @@ class OracleDynamics:
         mp = self.max_pairs()
         pairs = np.empty((mp, 2), dtype=np.int32)
         found = ctypes.c_longlong(0)
-        n = L.orc_cell_list_pairs(P(st.positions), I(st.N), P(self.head), P(self.next),
-                                  I(self.nx), I(self.ny), D(self.csx), D(self.csy),
-                                  D(self.Lx), D(self.Ly), I(self.px), I(self.py),
-                                  D(self.cutoff), LL(mp), P(pairs), ctypes.byref(found))
+        args = (P(st.positions), I(st.N), P(self.head), P(self.next), I(self.nx), I(self.ny),
+                D(self.csx), D(self.csy), D(self.Lx), D(self.Ly), I(self.px), I(self.py),
+                D(self.cutoff), LL(mp), P(pairs), ctypes.byref(found))
+        if self.all_loops:
+            n = L.orc_cell_list_pairs_par(*args, I(self.n_threads))
+            if n < 0:
+                raise MemoryError("orc_cell_list_pairs_par: no memory for the pair offsets")
+        else:
+            n = L.orc_cell_list_pairs(*args)
         self.last_n_found = found.value
         return np.ascontiguousarray(pairs[:n])
 
@@ -408,12 +419,23 @@ class OracleDynamics:
         npairs = pairs.shape[0]
         self.last_pairs = pairs
         self.forces[:] = 0.0
-        if self.use_wca:
-            L.orc_forces_wca(P(pos), P(self.forces), P(sp), P(alive), P(pairs), I(npairs),
-                             P(self.sig), P(self.eps), I(self.n_sp), Lx, Ly, px, py)
-        if self.use_harm:
-            L.orc_forces_harmonic(P(pos), P(self.forces), P(sp), P(alive), P(pairs), I(npairs),
-                                  P(self.sig), P(self.kh), I(self.n_sp), Lx, Ly, px, py)
+        if self.all_loops:
+            if self.use_wca:
+                L.orc_forces_par(I(0), P(pos), P(self.forces), P(sp), P(alive), P(pairs),
+                                 I(npairs), P(self.sig), P(self.eps), I(self.n_sp), Lx, Ly, px,
+                                 py, I(self.n_threads))
+            if self.use_harm:
+                L.orc_forces_par(I(1), P(pos), P(self.forces), P(sp), P(alive), P(pairs),
+                                 I(npairs), P(self.sig), P(self.kh), I(self.n_sp), Lx, Ly, px,
+                                 py, I(self.n_threads))
+        else:
+            if self.use_wca:
+                L.orc_forces_wca(P(pos), P(self.forces), P(sp), P(alive), P(pairs), I(npairs),
+                                 P(self.sig), P(self.eps), I(self.n_sp), Lx, Ly, px, py)
+            if self.use_harm:
+                L.orc_forces_harmonic(P(pos), P(self.forces), P(sp), P(alive), P(pairs),
+                                      I(npairs), P(self.sig), P(self.kh), I(self.n_sp), Lx, Ly,
+                                      px, py)
         noise = self.rng.standard_normal((N, 2))
         L.orc_bd_step(P(pos), P(self.Dp), P(self.forces), P(self.gradD), D(self.dt), Lx, Ly,
                       px, py, I(N), P(noise), P(alive), D(self.max_disp), I(self.n_threads))
