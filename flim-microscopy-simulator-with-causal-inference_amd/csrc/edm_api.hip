@@ -134,7 +134,7 @@ int edm_destroy(edm_membrane* m) {
                     m->ph_fret, m->flim, m->pcx, m->det_flag, m->det_pos, m->sv_key, m->sv_key2,
                     m->sv_val, m->sv_val2, m->asg_px, m->asg_py, m->asg_flag, m->asg_pos,
                     m->asg_t, m->asg_ch, m->sv_px, m->sv_py, m->pix_tot, m->d_scal,
-                    m->dye_ids, m->wl, m->wl_cnt, m->tg_cnt, m->tg_fill, m->tg_start,
+                    m->dye_ids, m->wl, m->wl_cnt, m->fstart, m->fkey, m->tg_cnt, m->tg_fill, m->tg_start,
                     m->tg_x, m->tg_y, m->tg_u};
     for (void* b : bufs)
         if (b) hipFree(b);
@@ -258,6 +258,10 @@ int edm_set_tuning(edm_membrane* m, int32_t key, int64_t value) {
             if (value < 0 || value > 1) return invalid("edm_set_tuning: graphs must be 0 or 1");
             m->use_graphs = (int)value;
             return 0;
+        case EDM_TUNE_FORCE_LISTS:
+            if (value < 0 || value > 1) return invalid("edm_set_tuning: force lists must be 0 or 1");
+            m->fine_force = (int)value;
+            return 0;
         case EDM_TUNE_GRAPH_REPLAYS:
         case EDM_TUNE_GRAPH_STATUS:
             return invalid("edm_set_tuning: read-only key");
@@ -277,6 +281,7 @@ int edm_get_tuning(edm_membrane* m, int32_t key, int64_t* value) {
         case EDM_TUNE_GRAPHS: *value = m->use_graphs; return 0;
         case EDM_TUNE_GRAPH_REPLAYS: *value = (int64_t)m->graph_replays; return 0;
         case EDM_TUNE_GRAPH_STATUS: *value = m->graph_status; return 0;
+        case EDM_TUNE_FORCE_LISTS: *value = m->fine_force; return 0;
         default: return invalid("edm_get_tuning: unknown key");
     }
 }

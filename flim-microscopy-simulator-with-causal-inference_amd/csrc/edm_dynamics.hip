@@ -240,16 +240,25 @@ template <typename Real>
 __global__ void k_bin(int n, const Real* __restrict__ x, const Real* __restrict__ y, Real csx,
                       Real csy, int ncx, int ncy, uint32_t* __restrict__ key,
                       uint32_t* __restrict__ rank, uint32_t* __restrict__ cnt, int32_t* counts,
-                      int n_counts) {
+                      int n_counts, const int32_t* __restrict__ id, uint32_t* __restrict__ kid,
+                      int nsub, Real inv_h) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < n_counts) counts[s] = 0;  // density counts are rebuilt each step
     const bool live = s < n;          // live lanes are a prefix of the wavefront
     uint32_t c = 0xffffffffu;
     if (live) {
+        const Real ys = y[s];
         const int cx = trunc_cell(x[s], csx, ncx);
-        const int cy = trunc_cell(y[s], csy, ncy);
+        const int cy = trunc_cell(ys, csy, ncy);
         c = (uint32_t)(cx * ncy + cy);
         key[s] = c;
+        if (kid) {
+            // the force sub-row lists' sort key: sub-row (as sub_row computes
+            // it) above the particle index
+            int b = (int)((ys - (Real)cy * csy) * inv_h);
+            b = min(max(b, 0), nsub - 1);
+            kid[s] = ((uint32_t)b << 28) | (uint32_t)id[s];
+        }
     }
     // Slots are in the previous step's cell order, so equal keys mostly sit
     // in adjacent lanes: one atomic per run of equal keys, ranks inside the
@@ -846,6 +855,188 @@ __global__ __launch_bounds__(256) void k_force_bd(int n, Cells<Real> C, ForcePar
 }
 
 #include "edm_tiles.h"
+
+// ----------------------------------------------------------------------
+// Force sub-row lists (dense membranes, steps without the reference pair
+// order): the cell sort orders every cell by (force sub-row, particle
+// index) and writes where each sub-row starts (fstart: kMaxSub entries per
+// cell, those past the cell's nsub sub-rows holding the cell's end), so
+// that a particle's force candidates in a column are one contiguous slot
+// range — its own sub-row and the two next to it, which run on into the
+// cells above and below — read straight from global memory with no strip
+// staging.  Sub-rows are at least 1.01 x the longest potential range tall,
+// so every pair within range is at most one sub-row apart (the 1 % margin
+// is far above the rounding of the sub-row index, as for the strips).
+// ----------------------------------------------------------------------
+__global__ void k_sort_cells_fine(int ncell, const uint32_t* __restrict__ start, uint32_t* perm,
+                                  uint32_t* cnt, const uint32_t* __restrict__ kid, int nsub,
+                                  uint32_t* __restrict__ fstart) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    cnt[c] = 0;
+    const int lo = (int)start[c], hi = (int)start[c + 1];
+    const int m = hi - lo;
+    // (sub-row << 28 | particle index), written by k_bin
+    auto key = [&](uint32_t v) { return (int32_t)kid[v]; };
+    constexpr int K = 8;
+    int nb[kMaxSub + 1];
+#pragma unroll
+    for (int q = 0; q <= kMaxSub; ++q) nb[q] = 0;  // members below sub-row q
+    if (m <= K) {
+        uint32_t v[K];
+        int32_t k[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) v[j] = j < m ? perm[lo + j] : 0u;
+#pragma unroll
+        for (int j = 0; j < K; ++j) k[j] = j < m ? key(v[j]) : 0x7fffffff;
+        if (m >= 2) {
+#pragma unroll
+            for (int r = 0; r < K; ++r) {
+#pragma unroll
+                for (int j = r & 1; j + 1 < K; j += 2) {
+                    if (k[j] > k[j + 1]) {
+                        const int32_t tk = k[j];
+                        k[j] = k[j + 1];
+                        k[j + 1] = tk;
+                        const uint32_t tv = v[j];
+                        v[j] = v[j + 1];
+                        v[j + 1] = tv;
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < K; ++j)
+                if (j < m) perm[lo + j] = v[j];
+        }
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            if (j >= m) break;
+            const int b = (int)((uint32_t)k[j] >> 28);
+#pragma unroll
+            for (int q = 1; q <= kMaxSub; ++q) nb[q] += b < q;
+        }
+    } else {
+        for (int i = lo + 1; i < hi; ++i) {
+            const uint32_t v = perm[i];
+            const int32_t vk = key(v);
+            int j = i - 1;
+            while (j >= lo && key(perm[j]) > vk) {
+                perm[j + 1] = perm[j];
+                --j;
+            }
+            perm[j + 1] = v;
+        }
+        for (int i = lo; i < hi; ++i) {
+            const int b = (int)(kid[perm[i]] >> 28);
+#pragma unroll
+            for (int q = 1; q <= kMaxSub; ++q) nb[q] += b < q;
+        }
+    }
+    // the cell's 8 entries as two 16-byte stores (rows of 32 bytes, so a
+    // wavefront writes one contiguous 2 KiB run)
+    static_assert(kMaxSub == 8, "fstart rows are two uint4");
+    uint4* row = reinterpret_cast<uint4*>(fstart + (size_t)c * kMaxSub);
+    row[0] = make_uint4((uint32_t)(lo + nb[0]), (uint32_t)(lo + nb[1]), (uint32_t)(lo + nb[2]),
+                        (uint32_t)(lo + nb[3]));
+    row[1] = make_uint4((uint32_t)(lo + nb[4]), (uint32_t)(lo + nb[5]), (uint32_t)(lo + nb[6]),
+                        (uint32_t)(lo + nb[7]));
+    if (c == ncell - 1) fstart[(size_t)ncell * kMaxSub] = (uint32_t)hi;
+}
+
+// Forces + Ermak-McCammon move over the sub-row lists, one particle per
+// thread.  Candidates by column (left to right), then by (row, sub-row,
+// particle index) with a periodic wrap's far row where the strip kernel
+// stages it (below row 0, above the last row): the strip kernel's order, so
+// both kernels sum every particle's forces in the same order and give the
+// same bits.
+template <typename Real>
+__global__ __launch_bounds__(256) void k_force_fine(int n, Cells<Real> C, ForceParams<Real> F,
+                                                    Real* __restrict__ xn, Real* __restrict__ yn,
+                                                    Rng rng, StepRef sref, Grids<Real> G,
+                                                    int n_grids, uint32_t keep_mask, int nsub,
+                                                    Real inv_h,
+                                                    const uint32_t* __restrict__ fstart) {
+    if (cap_gated_off(C)) return;  // uniform across the launch
+    const uint32_t step = sref.get();
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool take_any = false;
+    Real nxp = 0, nyp = 0;
+    int si = 0;
+    if (t < n) {
+        const Real xt = C.x[t], yt = C.y[t];
+        si = C.sp[t];
+        nxp = xt;
+        nyp = yt;
+        if (C.fl[t] & EDM_FLAG_ALIVE) {
+            const Real rl2 = min(C.cut2, F.rint2);
+            int cx, cy;
+            cell_of(C, xt, yt, cx, cy);
+            const int b = sub_row(C, yt, cy, nsub, inv_h);
+            Real Fx = 0, Fy = 0;
+            for (int dc = -1; dc <= 1; ++dc) {
+                int gx = cx + dc;
+                if (C.px) gx = gx < 0 ? gx + C.ncx : (gx >= C.ncx ? gx - C.ncx : gx);
+                else if (gx < 0 || gx >= C.ncx) continue;
+                const int col = gx * C.ncy;
+                // sub-rows b-1 .. b+1 of row cy: one run of slots, from the
+                // start of the first to the end of the last (they run on
+                // into the rows above and below), plus the far row's end
+                // sub-row when the run wraps round a periodic y edge.  Keys
+                // are cell * S + sub-row
+                constexpr int S = kMaxSub;
+                const int kc = (col + cy) * S;
+                int k0 = b > 0 ? kc + b - 1 : kc - S + nsub - 1;  // first sub-row's key
+                int k1 = b < nsub - 1 ? kc + b + 1 : kc + S;      // last sub-row's key
+                int below = -1, above = -1;  // wrapped far-row sub-row keys
+                if (b == 0 && cy == 0) {
+                    k0 = kc;
+                    if (C.py) below = (col + C.ncy - 1) * S + nsub - 1;
+                } else if (b == nsub - 1 && cy == C.ncy - 1) {
+                    k1 = kc + nsub - 1;
+                    if (C.py) above = col * S;
+                }
+                for (int r = 0; r < 3; ++r) {
+                    int u0, u1;
+                    const uint32_t* fs = fstart;
+                    // the slot after a sub-row: the next key's start (past
+                    // the cell's nsub sub-rows, entries hold the cell's end)
+                    if (r == 1) {
+                        u0 = (int)fs[k0];
+                        u1 = (int)fs[k1 + 1];
+                    } else {
+                        const int e = r == 0 ? below : above;
+                        if (e < 0) continue;
+                        u0 = (int)fs[e];
+                        u1 = (int)fs[e + 1];
+                    }
+                    for (int u = u0; u < u1; ++u) {
+                        if (u == t) continue;
+                        Real dx, dy;
+                        const Real dsq = min_image(xt, yt, C.x[u], C.y[u], C.Lx, C.Ly, C.px,
+                                                   C.py, dx, dy);
+                        if (dsq >= rl2 || !(C.fl[u] & EDM_FLAG_ALIVE)) continue;
+                        const int su = C.sp[u];
+                        Real f;
+                        if (F.wca && pair_force(F, 0, si, su, dsq, f)) {
+                            Fx += f * dx;
+                            Fy += f * dy;
+                        }
+                        if (F.harm && pair_force(F, 1, si, su, dsq, f)) {
+                            Fx += f * dx;
+                            Fy += f * dy;
+                        }
+                    }
+                }
+            }
+            move_particle(C, F, si, C.id[t] - C.kbase, xt, yt, Fx, Fy,
+                          (const double*)nullptr, rng, step, nxp, nyp);
+            take_any = ((keep_mask >> si) & 1u) != 0;
+        }
+        xn[t] = nxp;
+        yn[t] = nyp;
+    }
+    add_density(G, n_grids, take_any, si, nxp, nyp);
+}
 
 // ----------------------------------------------------------------------
 // Reaction kernels
@@ -1619,16 +1810,47 @@ static int begin_t(edm_membrane* m) {
     const int n = m->n;
     hipStream_t st = m->stream;
     // bin on the current positions, counting sort, order cells by index
+    // force sub-row lists: dense membranes, steps that do not need the
+    // reference's in-cell order (parity draws, the pair cap)
+    m->fine_active = m->fine_force == 1 && m->sub_rows && !m->need_pair_offsets &&
+                     m->p.ncx >= 3 && m->p.ncy >= 3;
+    if (m->fine_active) {
+        const size_t need = (size_t)m->ncell * kMaxSub + 1;
+        if (need > m->fstart_cap || !m->fkey) {
+            graph_touch(m);
+            EDM_HIP(hipStreamSynchronize(st));
+            if (m->fstart) EDM_HIP(hipFree(m->fstart));
+            if (m->fkey) EDM_HIP(hipFree(m->fkey));
+            m->fstart = m->fkey = nullptr;
+            m->fstart_cap = 0;
+            EDM_HIP(hipMalloc((void**)&m->fstart, need * sizeof(uint32_t)));
+            EDM_HIP(hipMalloc((void**)&m->fkey, (size_t)std::max(n, 1) * sizeof(uint32_t)));
+            m->fstart_cap = need;
+        }
+        // sub-rows at least 1.01 x the longest potential range tall
+        const double rl = std::sqrt(std::min(m->p.cutoff * m->p.cutoff, m->rint2));
+        const double q = rl > 0.0 ? m->p.csy / (1.01 * rl) : (double)kMaxSub;
+        m->fine_nsub = q >= kMaxSub ? kMaxSub : std::max(1, (int)q);
+        m->fine_inv_h = (double)m->fine_nsub / m->p.csy;
+    }
     hipLaunchKernelGGL(k_bin<Real>, dim3(nblk(std::max(n, m->n_count))), dim3(kBlock), 0, st, n,
                        (const Real*)m->a.x, (const Real*)m->a.y, (Real)m->p.csx, (Real)m->p.csy,
-                       m->p.ncx, m->p.ncy, m->key, m->rank, m->cell_cnt, m->counts, m->n_count);
+                       m->p.ncx, m->p.ncy, m->key, m->rank, m->cell_cnt, m->counts, m->n_count,
+                       (const int32_t*)m->a.id, m->fine_active ? m->fkey : nullptr,
+                       m->fine_nsub, (Real)m->fine_inv_h);
     size_t tb = m->cub_tmp_bytes;
     EDM_HIP(hipcub::DeviceScan::ExclusiveSum(m->cub_tmp, tb, m->cell_cnt, m->cell_start,
                                              m->ncell + 1, st));
     hipLaunchKernelGGL(k_scatter, dim3(nblk(n)), dim3(kBlock), 0, st, n, m->key, m->rank,
                        m->cell_start, m->perm);
-    hipLaunchKernelGGL(k_sort_cells, dim3(nblk(m->ncell)), dim3(kBlock), 0, st, m->ncell,
-                       m->cell_start, m->perm, m->cell_cnt, (const int32_t*)m->a.id);
+    if (m->fine_active) {
+        hipLaunchKernelGGL(k_sort_cells_fine, dim3(nblk(m->ncell)), dim3(kBlock), 0, st,
+                           m->ncell, m->cell_start, m->perm, m->cell_cnt,
+                           (const uint32_t*)m->fkey, m->fine_nsub, m->fstart);
+    } else {
+        hipLaunchKernelGGL(k_sort_cells, dim3(nblk(m->ncell)), dim3(kBlock), 0, st, m->ncell,
+                           m->cell_start, m->perm, m->cell_cnt, (const int32_t*)m->a.id);
+    }
     hipLaunchKernelGGL(k_gather<Real>, dim3(nblk(n)), dim3(kBlock), 0, st, n,
                        (const uint32_t*)m->perm, m->a, m->keep_slot_of);
     m->slot_of_valid = m->keep_slot_of;
@@ -1774,7 +1996,12 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
         const int ntile_y = (p.ncy + m->tile_rows - 1) / m->tile_rows;
         const int ntiles = p.ncx * ntile_y;
         const dim3 grid(8 * ((ntiles + 7) / 8));
-        if (m->sub_rows)
+        if (m->fine_active && !hook)
+            hipLaunchKernelGGL((k_force_fine<Real>), dim3(nblk(n)), dim3(256), 0, st, n, Ct, F,
+                               (Real*)m->a.xn, (Real*)m->a.yn, rng, step, G, ng,
+                               m->sensor_mask_all, m->fine_nsub, (Real)m->fine_inv_h,
+                               (const uint32_t*)m->fstart);
+        else if (m->sub_rows)
             hipLaunchKernelGGL((k_force_tile<Real, true>), grid, dim3(256),
                                force_sub_lds(m->tile_rows, 1), st, Ct, F, (Real*)m->a.xn,
                                (Real*)m->a.yn, noise, rng, step, G, ng, m->sensor_mask_all,

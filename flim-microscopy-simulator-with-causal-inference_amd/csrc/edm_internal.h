@@ -190,6 +190,16 @@ struct edm_membrane {
     int cat_walk = 0;           // 1: catalysis runs walk work lists where they can; 0: strips
     int cat_lanes = 16;         // lanes per entity of the strip walk (1, 2, 4, .., 64)
     bool sub_rows = false;      // force strips by sub-row (dense membranes; set at upload)
+    // force sub-row lists: cells ordered by (sub-row, index) with sub-row
+    // starts, forces read from global memory (dense membranes, steps that do
+    // not need the reference's in-cell order)
+    int fine_force = 1;
+    bool fine_active = false;   // this step's sort wrote fstart
+    int fine_nsub = 1;
+    double fine_inv_h = 0.0;
+    uint32_t* fstart = nullptr;  // [ncell * nsub + 1] sub-row starts
+    uint32_t* fkey = nullptr;    // [n] sort keys from k_bin (sub-row << 28 | index)
+    size_t fstart_cap = 0;
     // Work lists rebuilt after every sort (valid once a step has run since
     // the last upload): kWlMaxLists lists of wl_nblk regions of kWlSlots
     int32_t* wl = nullptr;       // [kWlMaxLists][wl_nblk * kWlSlots]

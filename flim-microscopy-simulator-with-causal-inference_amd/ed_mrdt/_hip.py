@@ -127,6 +127,7 @@ TUNE_FRET_SEARCH = 5
 TUNE_GRAPHS = 6
 TUNE_GRAPH_REPLAYS = 7
 TUNE_GRAPH_STATUS = 8
+TUNE_FORCE_LISTS = 9
 ENS_TUNE_STRIP_PARTICLES = 1
 ENS_TUNE_PAIR_BELOW = 2
 ENS_TUNE_CAT_LANES = 3

@@ -488,11 +488,13 @@ class DynamicsEngine:
     # ---- tuning (no reference counterpart) ----------------------------------
     _TUNE = (("tile_rows", _hip.TUNE_TILE_ROWS), ("cat_side", _hip.TUNE_CAT_SIDE),
              ("cat_walk", _hip.TUNE_CAT_WALK), ("cat_lanes", _hip.TUNE_CAT_LANES),
-             ("fret_search", _hip.TUNE_FRET_SEARCH), ("graphs", _hip.TUNE_GRAPHS))
+             ("fret_search", _hip.TUNE_FRET_SEARCH), ("graphs", _hip.TUNE_GRAPHS),
+             ("force_lists", _hip.TUNE_FORCE_LISTS))
 
     def set_tuning(self, tile_rows: Optional[int] = None, cat_side: Optional[int] = None,
                    cat_walk: Optional[int] = None, cat_lanes: Optional[int] = None,
-                   fret_search: Optional[int] = None, graphs: Optional[int] = None) -> None:
+                   fret_search: Optional[int] = None, graphs: Optional[int] = None,
+                   force_lists: Optional[int] = None) -> None:
         """Work-decomposition knobs that leave every result unchanged: cell
         rows per strip of the tiled kernels (1..256), the side catalysis walks
         its pairs from (0 enzymes, 1 substrates, -1 the rarer one), how
@@ -503,7 +505,8 @@ class DynamicsEngine:
         steps are recorded into HIP graphs and replayed (1) or launched one
         by one (0)."""
         vals = dict(tile_rows=tile_rows, cat_side=cat_side, cat_walk=cat_walk,
-                    cat_lanes=cat_lanes, fret_search=fret_search, graphs=graphs)
+                    cat_lanes=cat_lanes, fret_search=fret_search, graphs=graphs,
+                    force_lists=force_lists)
         for name, key in self._TUNE:
             if vals[name] is not None:
                 check(lib().edm_set_tuning(self._h, key, int(vals[name])), "edm_set_tuning")

@@ -1,10 +1,11 @@
 """Autotuning of the kernels' work decomposition.
 
-The knobs -- how catalysis walks its pairs (work lists of its enzymes or
-substrates, or LDS strips of cells), from which side (enzymes, substrates,
-or per run the side with fewer alive particles), with how many lanes per
-entity, and the cell rows per strip of the tiled force and catalysis
-kernels -- leave
+The knobs -- how the forces of dense membranes find their candidates
+(sub-row lists or LDS strips), how catalysis walks its pairs (work lists of
+its enzymes or substrates, or LDS strips of cells), from which side
+(enzymes, substrates, or per run the side with fewer alive particles), with
+how many lanes per entity, and the cell rows per strip of the tiled force
+and catalysis kernels -- leave
 every result bit-identical, so the fastest setting can be found by timing.
 `autotune` builds a scratch engine from the configuration (its own initial
 state, Philox draws), times each candidate over a few steps and returns the
@@ -32,17 +33,19 @@ TILE_ROWS = (16, 32, 64, 128, 256)
 CAT_SIDES = (-1, 0, 1)
 CAT_WALKS = (1, 0)
 CAT_LANES = (64, 16, 4, 1)
+FORCE_LISTS = (1, 0)
 
 
 def autotune(config: SimulationConfig, precision: str = "fp32", steps: int = 20,
              warmup: int = 3, tile_rows: Iterable[int] = TILE_ROWS,
              cat_sides: Iterable[int] = CAT_SIDES,
              cat_walks: Iterable[int] = CAT_WALKS,
-             cat_lanes: Iterable[int] = CAT_LANES) -> Dict:
-    """Fastest cat_walk, cat_side, cat_lanes and tile_rows for this
-    configuration (tuned one after another in that order, each with the
-    best of the earlier ones); `times_ms` holds the measured milliseconds
-    per step of every candidate."""
+             cat_lanes: Iterable[int] = CAT_LANES,
+             force_lists: Iterable[int] = FORCE_LISTS) -> Dict:
+    """Fastest force_lists, cat_walk, cat_side, cat_lanes and tile_rows for
+    this configuration (tuned one after another in that order, each with
+    the best of the earlier ones); `times_ms` holds the measured
+    milliseconds per step of every candidate."""
     from .dynamics import DynamicsEngine
     from .particles import ParticleState
 
@@ -60,6 +63,12 @@ def autotune(config: SimulationConfig, precision: str = "fp32", steps: int = 20,
 
     times: Dict[str, float] = {}
     try:
+        lists = list(force_lists)
+        for v in lists:
+            eng.set_tuning(force_lists=v)
+            times[f"force_lists={v}"] = per_step()
+        best_lists = min(lists, key=lambda v: times[f"force_lists={v}"])
+        eng.set_tuning(force_lists=best_lists)
         walks = list(cat_walks)
         for w in walks:
             eng.set_tuning(cat_walk=w)
@@ -86,8 +95,8 @@ def autotune(config: SimulationConfig, precision: str = "fp32", steps: int = 20,
     finally:
         st.mark_stale()
         eng.close()
-    return dict(tile_rows=best_rows, cat_side=best_side, cat_walk=best_walk,
-                cat_lanes=best_lanes, times_ms=times)
+    return dict(force_lists=best_lists, tile_rows=best_rows, cat_side=best_side,
+                cat_walk=best_walk, cat_lanes=best_lanes, times_ms=times)
 
 
 ENS_STRIP_PARTICLES = (96, 128, 192, 256, 384)

@@ -212,6 +212,13 @@ int edm_photon_counters(edm_membrane *m, int64_t out[2]);
  *                       by launch.  On ROCm 7.2 a replayed step takes
  *                       20-35 us longer than the same launches issued one
  *                       by one, for 4k to 1M particles
+ *   EDM_TUNE_FORCE_LISTS  dense membranes (>= 2 alive particles per cell), in
+ *                       steps that do not need the reference's in-cell order
+ *                       (no hook draws, no pair cap): 1 (default) the cell
+ *                       sort orders cells by force sub-row and the forces
+ *                       read their candidates through the sub-row starts;
+ *                       0 the forces stage strips of cells in LDS.  Both
+ *                       visit the same candidates in the same order
  *   EDM_TUNE_GRAPH_REPLAYS  (read only) graph replays so far
  *   EDM_TUNE_GRAPH_STATUS   (read only) 0, or why a recording failed (the
  *                       membrane then steps launch by launch): 1000 + the
@@ -228,6 +235,7 @@ int edm_photon_counters(edm_membrane *m, int64_t out[2]);
 #define EDM_TUNE_GRAPHS 6
 #define EDM_TUNE_GRAPH_REPLAYS 7
 #define EDM_TUNE_GRAPH_STATUS 8
+#define EDM_TUNE_FORCE_LISTS 9
 int edm_set_tuning(edm_membrane *m, int32_t key, int64_t value);
 int edm_get_tuning(edm_membrane *m, int32_t key, int64_t *value);
 /* Reference pair list in reference order (CellList.get_neighbor_pairs,

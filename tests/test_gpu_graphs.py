@@ -116,6 +116,7 @@ def test_device_decided_pair_cap_equals_host_decided():
                     f"the cap must truncate ({npairs.value} pairs)"
                 _hip.check(L.edm_step_finish(eng.handle, None, 0), "edm_step_finish")
         else:
+            eng.set_tuning(graphs=1)
             _hip.check(L.edm_run(eng.handle, 40), "edm_run")
             assert eng.graph_replays > 0 and eng.graph_status == 0
         st.mark_stale()

@@ -9,7 +9,7 @@ import pytest
 from ed_mrdt import workloads
 from ed_mrdt.dynamics import DynamicsEngine
 from ed_mrdt.particles import ParticleState
-from ed_mrdt.tuning import CAT_LANES, CAT_SIDES, CAT_WALKS, TILE_ROWS, autotune
+from ed_mrdt.tuning import CAT_LANES, CAT_SIDES, CAT_WALKS, FORCE_LISTS, TILE_ROWS, autotune
 
 pytestmark = pytest.mark.gpu
 
@@ -33,7 +33,8 @@ def test_tuning_leaves_results_unchanged():
     for tune in (dict(cat_side=1, tile_rows=64, cat_walk=0), dict(cat_side=1, tile_rows=16),
                  dict(cat_side=0, tile_rows=5, cat_walk=0), dict(cat_side=-1, tile_rows=32),
                  dict(cat_side=0, cat_walk=1), dict(cat_side=1, cat_walk=1),
-                 dict(cat_side=0, cat_lanes=1), dict(cat_side=1, cat_lanes=8, tile_rows=16)):
+                 dict(cat_side=0, cat_lanes=1), dict(cat_side=1, cat_lanes=8, tile_rows=16),
+                 dict(cat_side=0, force_lists=0), dict(cat_side=1, force_lists=0, tile_rows=16)):
         got = _run(cfg, 60, **tune)
         for k in ("positions", "species_id", "is_alive"):
             np.testing.assert_array_equal(got[k], ref[k], err_msg=f"{tune}: {k}")
@@ -45,6 +46,7 @@ def test_autotune_picks_a_candidate():
     res = autotune(cfg, steps=5, warmup=1)
     assert res["tile_rows"] in TILE_ROWS and res["cat_side"] in CAT_SIDES
     assert res["cat_walk"] in CAT_WALKS and res["cat_lanes"] in CAT_LANES
+    assert res["force_lists"] in FORCE_LISTS
     assert len(res["times_ms"]) == (len(TILE_ROWS) + len(CAT_SIDES) + len(CAT_WALKS) +
-                                    len(CAT_LANES))
+                                    len(CAT_LANES) + len(FORCE_LISTS))
     assert all(t > 0 for t in res["times_ms"].values())
