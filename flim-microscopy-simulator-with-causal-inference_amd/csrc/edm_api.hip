@@ -119,6 +119,10 @@ int edm_destroy(edm_membrane* m) {
     if (!m) return 0;
     if (m->stream) hipStreamSynchronize(m->stream);
     graphs_free(m);
+    frames_free(m);
+    if (m->fr_stream) hipStreamDestroy(m->fr_stream);
+    if (m->fr_photons) hipEventDestroy(m->fr_photons);
+    if (m->fr_done) hipEventDestroy(m->fr_done);
     if (m->d_step) hipFree(m->d_step);
     rec_free(m);
     stats_free(m);
@@ -527,6 +531,17 @@ int edm_frame_download(edm_membrane* m, uint32_t* flim, int64_t n_elems) {
     EDM_HIP(hipStreamSynchronize(m->stream));
     EDM_HIP(hipMemcpy(flim, m->flim, m->flim_elems * 4, hipMemcpyDeviceToHost));
     return 0;
+}
+
+int edm_frame_collect(edm_membrane* m, int32_t* ticket) {
+    if (!m || !ticket || !m->has_ip) return invalid("edm_frame_collect: imaging not set");
+    return frame_collect(m, ticket);
+}
+
+int edm_frame_fetch(edm_membrane* m, int32_t ticket, int64_t* meta, uint32_t* flat_index,
+                    uint32_t* count, int64_t cap, int64_t* total) {
+    if (!m) return invalid("edm_frame_fetch: null handle");
+    return frame_fetch(m, ticket, meta, flat_index, count, cap, total);
 }
 
 int edm_frame_reset(edm_membrane* m) {

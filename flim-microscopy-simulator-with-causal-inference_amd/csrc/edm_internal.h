@@ -255,6 +255,17 @@ struct edm_membrane {
     edm_image_params ip{};
     uint32_t* flim = nullptr;
     size_t flim_elems = 0;
+    // streamed frames (edm_frame_collect / edm_frame_fetch): a second stack
+    // the steps write into while the ended one gets its detector noise, is
+    // compacted to its nonzero bins and reset on fr_stream; the entries and
+    // the dark / afterpulse / detected counts of the one pending frame
+    uint32_t* flim_alt = nullptr;
+    hipStream_t fr_stream = nullptr;
+    hipEvent_t fr_photons = nullptr, fr_done = nullptr;
+    uint32_t *fr_blk = nullptr, *fr_idx = nullptr, *fr_cnt = nullptr;
+    unsigned long long* fr_meta = nullptr;
+    int fr_nchunk = 0;
+    int32_t fr_ticket = -1, fr_next = 0;
     double *pcx = nullptr, *pcy = nullptr;
     // hook-run photon staging
     int64_t* det_flag = nullptr;
@@ -355,6 +366,10 @@ int frame_reset(edm_membrane* m);
 int stage_draws(edm_membrane* m, const double* host, int64_t n);
 // census translation unit
 int rec_step(edm_membrane* m);  // one census row into the ring (no sync)
+int frame_collect(edm_membrane* m, int32_t* ticket);
+int frame_fetch(edm_membrane* m, int32_t ticket, int64_t* meta, uint32_t* idx, uint32_t* cnt,
+                int64_t cap, int64_t* total);
+void frames_free(edm_membrane* m);
 // n steps (dynamics, then the fused photon / imaging pass when image),
 // through recorded graphs where the membrane allows it, else launch by
 // launch

@@ -1188,6 +1188,7 @@ int img_alloc(edm_membrane* m) {
     const edm_image_params& ip = m->ip;
     const size_t elems = (size_t)ip.n_ch * ip.nx * ip.ny * ip.n_bins;
     if (elems != m->flim_elems) {
+        frames_free(m);  // a pending streamed frame of the old geometry is dropped
         hipFree(m->flim);
         hipFree(m->pix_tot);
         m->flim = nullptr;
@@ -1349,5 +1350,164 @@ int frame_reset(edm_membrane* m) {
 }
 
 #include "edm_ens_ph.h"
+
+// ----------------------------------------------------------------------
+// Streamed frames of one membrane (microscope.py:1159-1203 for every frame,
+// the reference keeping them all, pipeline.py:645).  edm_frame_collect ends
+// the current frame: the steps that follow write their photons into a
+// second stack, while a second stream gives the ended stack its detector
+// noise (the same draws as edm_frame_noise at this step), compacts it to
+// (flat index, count) pairs in index order, and resets it.  edm_frame_fetch
+// waits for that work and copies the pairs out.  One frame may be pending.
+// ----------------------------------------------------------------------
+static constexpr int64_t kFrChunk = 1 << 16;  // bins per compaction block
+
+__global__ __launch_bounds__(256) void k_fr_count(const uint32_t* __restrict__ flim, int64_t n,
+                                                  uint32_t* blk) {
+    const int64_t c0 = (int64_t)blockIdx.x * kFrChunk, c1 = min(n, c0 + kFrChunk);
+    uint32_t c = 0;
+    for (int64_t i = c0 + threadIdx.x; i < c1; i += blockDim.x) c += flim[i] != 0u;
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o);
+    __shared__ uint32_t ws[4];
+    if (lane_id() == 0) ws[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) blk[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+__global__ __launch_bounds__(256) void k_fr_write(const uint32_t* __restrict__ flim, int64_t n,
+                                                  const uint32_t* __restrict__ off, uint32_t* oidx,
+                                                  uint32_t* ocnt) {
+    const int64_t c0 = (int64_t)blockIdx.x * kFrChunk, c1 = min(n, c0 + kFrChunk);
+    uint32_t base = off[blockIdx.x];
+    __shared__ uint32_t wtot[4];
+    const int w = threadIdx.x >> 6;
+    for (int64_t t0 = c0; t0 < c1; t0 += blockDim.x) {
+        const int64_t i = t0 + threadIdx.x;
+        const uint32_t v = i < c1 ? flim[i] : 0u;
+        const unsigned long long mk = __ballot(v != 0u);
+        const uint32_t below = (uint32_t)__popcll(mk & ((1ull << lane_id()) - 1ull));
+        if (lane_id() == 0) wtot[w] = (uint32_t)__popcll(mk);
+        __syncthreads();
+        uint32_t before = 0;
+        for (int q = 0; q < w; ++q) before += wtot[q];
+        if (v != 0u) {
+            oidx[base + before + below] = (uint32_t)i;
+            ocnt[base + before + below] = v;
+        }
+        const uint32_t tile = wtot[0] + wtot[1] + wtot[2] + wtot[3];
+        __syncthreads();
+        base += tile;
+    }
+}
+
+// the detected-photon counter at the frame's end, beside its noise counts
+__global__ void k_fr_meta(const unsigned long long* __restrict__ ctr, unsigned long long* meta) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) meta[2] = ctr[7];
+}
+
+void frames_free(edm_membrane* m) {
+    if (m->fr_stream) (void)hipStreamSynchronize(m->fr_stream);
+    void* dev[] = {m->flim_alt, m->fr_blk, m->fr_idx, m->fr_cnt, m->fr_meta};
+    for (void* p : dev)
+        if (p) (void)hipFree(p);
+    m->flim_alt = nullptr;
+    m->fr_blk = m->fr_idx = m->fr_cnt = nullptr;
+    m->fr_meta = nullptr;
+    m->fr_ticket = -1;
+    m->fr_nchunk = 0;
+}
+
+int frame_collect(edm_membrane* m, int32_t* ticket) {
+    if (m->fr_ticket >= 0) {
+        set_error("edm_frame_collect: a collected frame is pending; fetch it first");
+        return EDM_ESTATE;
+    }
+    const int64_t n = (int64_t)m->flim_elems;
+    if (n >= (int64_t)UINT32_MAX) {
+        set_error("edm_frame_collect: the stack has more than 2^32 bins");
+        return EDM_EINVAL;
+    }
+    hipStream_t st = m->stream;
+    if (!m->fr_stream) EDM_HIP(hipStreamCreate(&m->fr_stream));
+    if (!m->fr_photons) EDM_HIP(hipEventCreateWithFlags(&m->fr_photons, hipEventDisableTiming));
+    if (!m->fr_done) EDM_HIP(hipEventCreateWithFlags(&m->fr_done, hipEventDisableTiming));
+    if (!m->flim_alt) {
+        graph_touch(m);
+        m->fr_nchunk = (int)((n + kFrChunk - 1) / kFrChunk);
+        if (dalloc(&m->flim_alt, (size_t)n) || dalloc(&m->fr_blk, (size_t)m->fr_nchunk + 1) ||
+            dalloc(&m->fr_idx, (size_t)n) || dalloc(&m->fr_cnt, (size_t)n) ||
+            dalloc(&m->fr_meta, 4))
+            return EDM_EHIP;
+        hipLaunchKernelGGL(k_fill_u32, dim3(std::min(nb(n), 8192)), dim3(kB), 0, st, n,
+                           m->flim_alt, 0u);
+    }
+    // kernel arguments of recorded step graphs name the stack being swapped
+    graph_touch(m);
+    uint32_t* ended = m->flim;
+    EDM_HIP(hipEventRecord(m->fr_photons, st));
+    EDM_HIP(hipStreamWaitEvent(m->fr_stream, m->fr_photons, 0));
+    std::swap(m->flim, m->flim_alt);
+    hipStream_t fs = m->fr_stream;
+    const int64_t npix = (int64_t)m->ip.n_ch * m->ip.nx * m->ip.ny;
+    Rng rng{(uint32_t)(m->p.seed & 0xffffffffu) ^ 0x2545F491u, (uint32_t)(m->p.seed >> 32)};
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, fs, (int64_t)8, (uint32_t*)m->fr_meta,
+                       0u);
+    hipLaunchKernelGGL(k_fr_meta, dim3(1), dim3(1), 0, fs, (const unsigned long long*)m->d_ctr,
+                       m->fr_meta);
+    hipLaunchKernelGGL(k_frame_noise, dim3(nb(npix)), dim3(kB), 0, fs, npix, m->ip.n_bins, ended,
+                       m->ip.dark_lambda, m->ip.p_ap, rng, m->step, m->fr_meta);
+    hipLaunchKernelGGL(k_fr_count, dim3(m->fr_nchunk), dim3(256), 0, fs,
+                       (const uint32_t*)ended, n, m->fr_blk);
+    hipLaunchKernelGGL(k_sp_scan, dim3(1), dim3(1024), 0, fs, m->fr_blk, m->fr_nchunk);
+    hipLaunchKernelGGL(k_fr_write, dim3(m->fr_nchunk), dim3(256), 0, fs, (const uint32_t*)ended,
+                       n, (const uint32_t*)m->fr_blk, m->fr_idx, m->fr_cnt);
+    hipLaunchKernelGGL(k_fill_u32, dim3(std::min(nb(n), 8192)), dim3(kB), 0, fs, n, ended, 0u);
+    EDM_HIP(hipGetLastError());
+    EDM_HIP(hipEventRecord(m->fr_done, fs));
+    // the steps after this one write into the other stack: zeroed when it was
+    // allocated, or reset by the previous collect, whose fetch waited for it
+    m->fr_ticket = m->fr_next++;
+    *ticket = m->fr_ticket;
+    return 0;
+}
+
+int frame_fetch(edm_membrane* m, int32_t ticket, int64_t* meta, uint32_t* idx, uint32_t* cnt,
+                int64_t cap, int64_t* total) {
+    if (ticket < 0 || ticket != m->fr_ticket) {
+        set_error("edm_frame_fetch: no pending frame with that ticket");
+        return EDM_ESTATE;
+    }
+    EDM_HIP(hipEventSynchronize(m->fr_done));
+    uint32_t tot32 = 0;
+    unsigned long long mt[3] = {0, 0, 0};
+    EDM_HIP(hipMemcpyAsync(&tot32, m->fr_blk + m->fr_nchunk, 4, hipMemcpyDeviceToHost,
+                           m->fr_stream));
+    EDM_HIP(hipMemcpyAsync(mt, m->fr_meta, sizeof(mt), hipMemcpyDeviceToHost, m->fr_stream));
+    EDM_HIP(hipStreamSynchronize(m->fr_stream));
+    const int64_t tot = tot32;
+    if (total) *total = tot;
+    if (meta)
+        for (int k = 0; k < 3; ++k) meta[k] = (int64_t)mt[k];
+    if (idx || cnt) {
+        if (cap < tot) {
+            set_error("edm_frame_fetch: output holds " + std::to_string(cap) +
+                      " entries, the frame has " + std::to_string(tot));
+            return EDM_EINVAL;
+        }
+        if (tot > 0) {
+            if (idx)
+                EDM_HIP(hipMemcpyAsync(idx, m->fr_idx, (size_t)tot * 4, hipMemcpyDeviceToHost,
+                                       m->fr_stream));
+            if (cnt)
+                EDM_HIP(hipMemcpyAsync(cnt, m->fr_cnt, (size_t)tot * 4, hipMemcpyDeviceToHost,
+                                       m->fr_stream));
+            EDM_HIP(hipStreamSynchronize(m->fr_stream));
+        }
+        // a call without output arrays only reports the sizes; one with them
+        // releases the frame for the next collect
+        m->fr_ticket = -1;
+    }
+    return 0;
+}
 
 }  // namespace edm

@@ -166,6 +166,8 @@ _SIGS = {
     "edm_time_steps": [c_void_p, c_int32, c_int32, ctypes.POINTER(c_double),
                        ctypes.POINTER(c_double), c_void_p],
     "edm_run_frame": [c_void_p, c_int32],
+    "edm_frame_collect": [c_void_p, c_void_p],
+    "edm_frame_fetch": [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_int64, c_void_p],
     "edm_record_open": [c_void_p, c_int32, c_int32],
     "edm_record_step": [c_void_p],
     "edm_record_rows": [c_void_p, ctypes.POINTER(c_int32)],

@@ -318,4 +318,52 @@ class VirtualMicroscope:
         self._frame_index += 1
         self._n_signal = 0
         self._n_bd_steps_in_frame = 0
+        if self._dev.rng_mode == "philox":
+            self._last_detected = self._detected_counter()
         return frame
+
+    # ---- streamed frames (no reference counterpart: the reference keeps every
+    # frame of a run, pipeline.py:645, by copying each dense stack) ------------
+    def _detected_counter(self) -> int:
+        c = np.zeros(2, dtype=np.int64)
+        check(lib().edm_photon_counters(self._dev.handle, ptr(c)), "edm_photon_counters")
+        return int(c[1])
+
+    def collect_frame(self) -> int:
+        """End the current frame as finalize_frame does (Philox draws), but
+        leave its detector noise, compaction to nonzero bins and reset to a
+        second stream while the next steps run into a second stack; returns
+        a ticket for fetch_frame.  One frame may be pending."""
+        if self._dev.rng_mode != "philox":
+            raise ValueError("collect_frame needs the Philox draws; use finalize_frame")
+        t = ctypes.c_int32(-1)
+        check(lib().edm_frame_collect(self._dev.handle, ctypes.byref(t)), "edm_frame_collect")
+        self._pending = (int(t.value), self._frame_index)
+        self._frame_index += 1
+        self._n_signal = 0
+        self._n_bd_steps_in_frame = 0
+        return int(t.value)
+
+    def fetch_frame(self, ticket: int) -> SparseFrame:
+        """The collected frame (waits for its noise and compaction)."""
+        pend = getattr(self, "_pending", None)
+        if pend is None or pend[0] != ticket:
+            raise ValueError("no pending frame with that ticket")
+        L, h = lib(), self._dev.handle
+        meta = np.zeros(3, dtype=np.int64)
+        total = ctypes.c_int64(0)
+        check(L.edm_frame_fetch(h, int(ticket), ptr(meta), None, None, 0, ctypes.byref(total)),
+              "edm_frame_fetch")
+        n = int(total.value)
+        idx = np.empty(max(n, 1), dtype=np.uint32)
+        cnt = np.empty(max(n, 1), dtype=np.uint32)
+        check(L.edm_frame_fetch(h, int(ticket), None, ptr(idx), ptr(cnt), idx.size,
+                                ctypes.byref(total)), "edm_frame_fetch")
+        self._pending = None
+        n_dark, n_ap = int(meta[0]), int(meta[1])
+        n_signal = int(meta[2]) - getattr(self, "_last_detected", 0)
+        self._last_detected = int(meta[2])
+        self.total_dark_counts += n_dark
+        self.total_afterpulses += n_ap
+        return SparseFrame(self.frame_shape, idx[:n].copy(), cnt[:n].copy(), pend[1],
+                           n_signal_photons=n_signal, n_dark_counts=n_dark, n_afterpulses=n_ap)

@@ -280,6 +280,20 @@ int edm_frame_add_counts(edm_membrane *m, const int64_t *flat_idx, int64_t n);
 int edm_frame_noise(edm_membrane *m, int64_t *n_dark, int64_t *n_afterpulse);
 int edm_frame_download(edm_membrane *m, uint32_t *flim, int64_t n_elems);
 int edm_frame_reset(edm_membrane *m);
+/* Streamed frames (finalize_frame, microscope.py:1159-1203, for every frame
+ * of a run, as pipeline.py:645 keeps them): edm_frame_collect ends the
+ * current frame and returns a ticket; the steps that follow write into a
+ * second stack while a second stream adds the detector noise (the draws
+ * edm_frame_noise would make at this step), compacts the ended stack to its
+ * nonzero bins and resets it.  edm_frame_fetch waits for that and returns
+ * meta = {dark counts, afterpulses, cumulative detected photons at the
+ * frame's end}, *total entries and, when the arrays are given (cap entries
+ * each), the (flat index into [n_ch, nx, ny, n_bins], count) pairs in index
+ * order; a call with the arrays releases the frame.  One frame may be
+ * pending; the photon counts need Philox imaging (edm_photons_image). */
+int edm_frame_collect(edm_membrane *m, int32_t *ticket);
+int edm_frame_fetch(edm_membrane *m, int32_t ticket, int64_t *meta, uint32_t *flat_index,
+                    uint32_t *count, int64_t cap, int64_t *total);
 
 /* ---- timing (HIP events on the handle's stream) ------------------------ */
 int edm_synchronize(edm_membrane *m);
