@@ -145,6 +145,8 @@ int edm_destroy(edm_membrane* m) {
     for (auto e : m->kev) hipEventDestroy(e);
     if (m->ev0) hipEventDestroy(m->ev0);
     if (m->ev1) hipEventDestroy(m->ev1);
+    if (m->mark0) hipEventDestroy(m->mark0);
+    if (m->mark1) hipEventDestroy(m->mark1);
     if (m->stream) hipStreamDestroy(m->stream);
     delete m;
     return 0;
@@ -616,6 +618,28 @@ int edm_time_steps(edm_membrane* m, int32_t n_steps, int32_t image, double* tota
     if (force_ms) *force_ms = fsum;
     if (breakdown_ms)
         for (int i = 0; i < 8; ++i) breakdown_ms[i] = 0.0;
+    return 0;
+}
+
+int edm_time_mark(edm_membrane* m, int32_t end, double* elapsed_ms) {
+    if (!m || (end != 0 && end != 1)) return invalid("edm_time_mark: bad argument");
+    if (!m->mark0) EDM_HIP(hipEventCreate(&m->mark0));
+    if (!m->mark1) EDM_HIP(hipEventCreate(&m->mark1));
+    if (!end) {
+        EDM_HIP(hipEventRecord(m->mark0, m->stream));
+        m->mark_open = true;
+        return 0;
+    }
+    if (!m->mark_open) {
+        set_error("edm_time_mark: no window was opened");
+        return EDM_ESTATE;
+    }
+    EDM_HIP(hipEventRecord(m->mark1, m->stream));
+    EDM_HIP(hipEventSynchronize(m->mark1));
+    m->mark_open = false;
+    float ms = 0.f;
+    EDM_HIP(hipEventElapsedTime(&ms, m->mark0, m->mark1));
+    if (elapsed_ms) *elapsed_ms = ms;
     return 0;
 }
 

@@ -27,7 +27,7 @@ __global__ __launch_bounds__(256) void k_ens_photons(int n_waves, const int32_t*
                            true, 0.0, -1, rng, gstep - g.step0, emitted);
     }
     // the wave's emitters are all of replica W.r: its geometry, stack and draws
-    wave_photons<Real, kWin>(E, &g.ip, x, y, g.flim, rng, gstep - g.step0, detected);
+    wave_photons<Real, kWin>(E, &g.ip, x, y, g.flim, nullptr, rng, gstep - g.step0, detected);
     photon_count(emitted, detected, ctr + 8 * W.r);
 }
 
@@ -67,7 +67,8 @@ int ens_frame_noise(edm_ensemble* e, int slot, int64_t* n_dark, int64_t* n_ap) {
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, e->stream, (int64_t)4,
                        (uint32_t*)e->frame_out, 0u);
     hipLaunchKernelGGL(k_frame_noise, dim3(nb(npix)), dim3(kB), 0, e->stream, npix, ip.n_bins,
-                       g.flim, ip.dark_lambda, ip.p_ap, rng, e->gstep - g.step0, e->frame_out);
+                       g.flim, (uint8_t*)nullptr, ip.dark_lambda, ip.p_ap, rng,
+                       e->gstep - g.step0, e->frame_out);
     EDM_HIP(hipGetLastError());
     if (n_dark || n_ap) {
         unsigned long long c[2];
@@ -117,7 +118,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
         unsigned long long nd = 0, na = 0;
         if (qw < npix) {
             const uint32_t tot0 = p_ap > 0.0 ? wave_pixel_totals(flim, qw, npix, nb) : 0u;
-            if (q < npix) frame_noise_pixel(q, nb, flim, tot0, lam, p_ap, rng, frame, nd, na);
+            if (q < npix)
+                (void)frame_noise_pixel(q, nb, flim, nullptr, tot0, lam, p_ap, rng, frame, nd,
+                                        na);
         }
         noise_count(nd, na, noise_ctr + 2 * r);
     }

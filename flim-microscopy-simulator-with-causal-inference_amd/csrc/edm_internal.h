@@ -255,6 +255,7 @@ struct edm_membrane {
     edm_image_params ip{};
     uint32_t* flim = nullptr;
     size_t flim_elems = 0;
+    size_t flim_words = 0;  // the counts and one flag byte per pixel row
     // streamed frames (edm_frame_collect / edm_frame_fetch): a second stack
     // the steps write into while the ended one gets its detector noise, is
     // compacted to its nonzero bins and reset on fr_stream; the entries and
@@ -264,7 +265,7 @@ struct edm_membrane {
     hipEvent_t fr_photons = nullptr, fr_done = nullptr;
     uint32_t *fr_blk = nullptr, *fr_idx = nullptr, *fr_cnt = nullptr;
     unsigned long long* fr_meta = nullptr;
-    int fr_nchunk = 0;
+    int fr_nwave = 0;
     int32_t fr_ticket = -1, fr_next = 0;
     double *pcx = nullptr, *pcy = nullptr;
     // hook-run photon staging
@@ -319,6 +320,8 @@ struct edm_membrane {
 
     // timing
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t mark0 = nullptr, mark1 = nullptr;  // edm_time_mark's window
+    bool mark_open = false;
     std::vector<hipEvent_t> kev;
     bool time_force = false;
     double force_ms_acc = 0.0;

@@ -404,14 +404,25 @@ __device__ __forceinline__ bool psf_pick(const edm_image_params* ip, const doubl
     return true;
 }
 
-__device__ __forceinline__ void tcspc_add(const edm_image_params* ip, uint32_t* flim, int ch,
-                                          int px, int py, double t) {
+// Row flags of a single membrane's stacks: one byte per pixel histogram,
+// stored right after the stack's n_pix x n_bins counts and set by every
+// kernel that adds counts to a row, so that a clear flag means an empty row
+// and a frame end reads only the flagged rows.  Concurrent writers all
+// store 1.  The ensemble's per-replica stacks keep no flags (nullptr).
+__device__ __forceinline__ uint8_t* row_flags(uint32_t* flim, int64_t npix, int nb) {
+    return (uint8_t*)(flim + npix * (int64_t)nb);
+}
+
+__device__ __forceinline__ void tcspc_add(const edm_image_params* ip, uint32_t* flim,
+                                          uint8_t* flags, int ch, int px, int py, double t) {
     if (ch < 0 || ch >= ip->n_ch) return;
     if (px < 0 || px >= ip->nx || py < 0 || py >= ip->ny) return;
     if (t < 0.0) t = 0.0;
     const long long b = (long long)(t / ip->bin_w);
     if (b >= ip->n_bins) return;
-    atomicAdd(&flim[(((size_t)ch * ip->nx + px) * ip->ny + py) * ip->n_bins + b], 1u);
+    const int64_t q = ((int64_t)ch * ip->nx + px) * ip->ny + py;
+    atomicAdd(&flim[q * ip->n_bins + b], 1u);
+    if (flags) flags[q] = 1u;
 }
 
 // ----------------------------------------------------------------------
@@ -563,11 +574,13 @@ __device__ __forceinline__ Emit<Real> emit_pss(const edm_photo_params* pp, int t
 // emit in a step, so walking each emitter's photons in its own lane leaves
 // most lanes idle).  Photon k of an emitter uses the draws keyed by
 // (step, emitter key, k) whichever lane takes it.  Every lane must call it;
-// ip, flim, rng and step are the same for the whole wavefront.
+// ip, flim, flags (row flags or nullptr), rng and step are the same for
+// the whole wavefront.
 template <typename Real, int kWin>
 __device__ __forceinline__ void wave_photons(const Emit<Real>& E, const edm_image_params* ip,
                                              const Real* x, const Real* y, uint32_t* flim,
-                                             Rng rng, uint32_t step, int& detected) {
+                                             uint8_t* flags, Rng rng, uint32_t step,
+                                             int& detected) {
     const int lane = lane_id();
     int incl = E.n;  // photons of lanes 0 .. lane
 #pragma unroll
@@ -609,7 +622,7 @@ __device__ __forceinline__ void wave_photons(const Emit<Real>& E, const edm_imag
                                              py_)) {
                     Real z0, z1;
                     normal2<Real>(rng.draw(step, key, (uint32_t)k, kTagJitter), z0, z1);
-                    tcspc_add(ip, flim, ch, px_, py_, (double)(ta + sirf * z0));
+                    tcspc_add(ip, flim, flags, ch, px_, py_, (double)(ta + sirf * z0));
                     ++detected;
                 }
             }
@@ -659,7 +672,9 @@ __global__ __launch_bounds__(256) void k_pss_image(int n_dye, const int32_t* dye
                            k_fret != nullptr, k_fret ? k_fret[q] : 0.0, acc ? acc[q] : -1, rng,
                            step, emitted);
     }
-    wave_photons<Real, kWin>(E, ip, x, y, flim, rng, step, detected);
+    wave_photons<Real, kWin>(E, ip, x, y, flim,
+                             row_flags(flim, (int64_t)ip->n_ch * ip->nx * ip->ny, ip->n_bins),
+                             rng, step, detected);
     photon_count(emitted, detected, ctr);
 }
 
@@ -715,7 +730,8 @@ __global__ void k_bin_tcspc(int64_t n, const edm_image_params* ip, uint32_t* fli
                             const int8_t* ach, const double* jitter) {
     const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n) return;
-    tcspc_add(ip, flim, ach[a], apx[a], apy[a], at[a] + jitter[a]);
+    tcspc_add(ip, flim, row_flags(flim, (int64_t)ip->n_ch * ip->nx * ip->ny, ip->n_bins), ach[a],
+              apx[a], apy[a], at[a] + jitter[a]);
 }
 
 // ----------------------------------------------------------------------
@@ -758,11 +774,15 @@ __global__ void k_pixel_totals(int64_t npix, int nb, const uint32_t* flim, uint3
     if (q < npix) tot[q] = s;
 }
 
-__global__ void k_add_counts(int64_t n, const int64_t* idx, uint32_t* flim, int64_t nelem) {
+__global__ void k_add_counts(int64_t n, const int64_t* idx, uint32_t* flim, int64_t nelem,
+                             int nb) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int64_t k = idx[i];
-    if (k >= 0 && k < nelem) atomicAdd(&flim[k], 1u);
+    if (k >= 0 && k < nelem) {
+        atomicAdd(&flim[k], 1u);
+        ((uint8_t*)(flim + nelem))[k / nb] = 1u;
+    }
 }
 
 // Poisson by sequential inversion (small means: dark counts per pixel).
@@ -800,23 +820,35 @@ __device__ __forceinline__ int binomial_draw(int nn, double p, double u, double 
     return k;
 }
 
+// One count added to a bin by the thread that owns its row; returns 1 if
+// the bin was empty.
+__device__ __forceinline__ uint32_t bump(uint32_t* row, int b) {
+    const uint32_t o = row[b];
+    row[b] = o + 1u;
+    return o == 0u;
+}
+
 // Dark counts (Poisson, uniform bins) and afterpulses (binomial of the
 // pixel's counts after the dark counts, uniform bins) of one pixel
-// histogram whose counts before the dark counts total `tot0`.
-__device__ __forceinline__ void frame_noise_pixel(int64_t q, int nb, uint32_t* flim, uint32_t tot0,
-                                                  double lam, double p_ap, Rng rng,
-                                                  uint32_t frame, unsigned long long& nd,
-                                                  unsigned long long& na) {
+// histogram whose counts before the dark counts total `tot0`, by the one
+// thread that owns the row.  Sets the row's flag (if the stack keeps flags)
+// when dark counts land and returns how many counts landed on empty bins.
+__device__ __forceinline__ uint32_t frame_noise_pixel(int64_t q, int nb, uint32_t* flim,
+                                                      uint8_t* flags, uint32_t tot0, double lam,
+                                                      double p_ap, Rng rng, uint32_t frame,
+                                                      unsigned long long& nd,
+                                                      unsigned long long& na) {
     uint32_t* row = flim + q * nb;
     const uint32_t lo = (uint32_t)q, hi = (uint32_t)(q >> 32);
-    uint32_t tot = tot0;
+    uint32_t tot = tot0, fresh = 0u;
     if (lam >= 1e-10) {
         const U4 r = rng.draw(frame, lo, hi, kTagDark);
         const int k = poisson_inv(lam, u01d(r.x, r.y));
         for (int i = 0; i < k; ++i) {
             const U4 rb = rng.draw(frame, lo, hi + 0x10000u * (uint32_t)(i + 1), kTagDark);
-            row[(int)(u01d(rb.x, rb.y) * nb) % nb] += 1u;
+            fresh += bump(row, (int)(u01d(rb.x, rb.y) * nb) % nb);
         }
+        if (k > 0 && flags) flags[q] = 1u;
         nd += (unsigned long long)k;
         tot += (uint32_t)k;
     }
@@ -829,11 +861,12 @@ __device__ __forceinline__ void frame_noise_pixel(int64_t q, int nb, uint32_t* f
             for (int i = 0; i < k; ++i) {
                 const U4 rb =
                     rng.draw(frame, lo, hi + 0x10000u * (uint32_t)(i + 1), kTagAfterpulse);
-                row[(int)(u01d(rb.x, rb.y) * nb) % nb] += 1u;
+                fresh += bump(row, (int)(u01d(rb.x, rb.y) * nb) % nb);
             }
             na += (unsigned long long)k;
         }
     }
+    return fresh;
 }
 
 __device__ __forceinline__ void noise_count(unsigned long long nd, unsigned long long na,
@@ -849,15 +882,17 @@ __device__ __forceinline__ void noise_count(unsigned long long nd, unsigned long
 }
 
 // One wavefront per 64 pixels: their totals read coalesced, then each
-// lane draws its own pixel's noise.
-__global__ void k_frame_noise(int64_t npix, int nb, uint32_t* flim, double lam, double p_ap,
-                              Rng rng, uint32_t frame, unsigned long long* out) {
+// lane draws its own pixel's noise.  flags: the stack's row flags, or
+// nullptr for a stack that keeps none.
+__global__ void k_frame_noise(int64_t npix, int nb, uint32_t* flim, uint8_t* flags, double lam,
+                              double p_ap, Rng rng, uint32_t frame, unsigned long long* out) {
     const int64_t q0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) & ~(int64_t)63;
     if (q0 >= npix) return;  // whole wavefronts
     const uint32_t tot0 = p_ap > 0.0 ? wave_pixel_totals(flim, q0, npix, nb) : 0u;
     const int64_t q = q0 + lane_id();
     unsigned long long nd = 0, na = 0;
-    if (q < npix) frame_noise_pixel(q, nb, flim, tot0, lam, p_ap, rng, frame, nd, na);
+    if (q < npix)
+        (void)frame_noise_pixel(q, nb, flim, flags, tot0, lam, p_ap, rng, frame, nd, na);
     noise_count(nd, na, out);
 }
 
@@ -1183,18 +1218,27 @@ int ph_fused_image(edm_membrane* m, int64_t* n_emitted, int64_t* n_detected) {
 
 // Imaging buffers: flim stack, pixel centres and the parameter block live
 // in one allocation after the pixel-centre arrays.
+// 32-bit words of a stack allocation: the n_pix x n_bins counts, then one
+// flag byte per pixel row (row_flags), padded to 256 bytes
+static size_t stack_words(const edm_image_params& ip) {
+    const size_t npix = (size_t)ip.n_ch * ip.nx * ip.ny;
+    return npix * ip.n_bins + (npix + 255) / 256 * 64;
+}
+
 int img_alloc(edm_membrane* m) {
     graph_touch(m);
     const edm_image_params& ip = m->ip;
     const size_t elems = (size_t)ip.n_ch * ip.nx * ip.ny * ip.n_bins;
-    if (elems != m->flim_elems) {
+    if (elems != m->flim_elems || stack_words(ip) != m->flim_words) {
         frames_free(m);  // a pending streamed frame of the old geometry is dropped
         hipFree(m->flim);
         hipFree(m->pix_tot);
         m->flim = nullptr;
-        if (dalloc(&m->flim, elems) || dalloc(&m->pix_tot, (size_t)ip.n_ch * ip.nx * ip.ny))
+        if (dalloc(&m->flim, stack_words(ip)) ||
+            dalloc(&m->pix_tot, (size_t)ip.n_ch * ip.nx * ip.ny))
             return EDM_EHIP;
         m->flim_elems = elems;
+        m->flim_words = stack_words(ip);
         if (int rc = frame_reset(m)) return rc;
     }
     hipFree(m->pcx);
@@ -1316,7 +1360,7 @@ int frame_add(edm_membrane* m, const int64_t* flat, int64_t n) {
     EDM_HIP(hipMalloc(&d, (size_t)n * 8));
     EDM_HIP(hipMemcpyAsync(d, flat, (size_t)n * 8, hipMemcpyHostToDevice, m->stream));
     hipLaunchKernelGGL(k_add_counts, dim3(nb(n)), dim3(kB), 0, m->stream, n, (const int64_t*)d,
-                       m->flim, (int64_t)m->flim_elems);
+                       m->flim, (int64_t)m->flim_elems, m->ip.n_bins);
     EDM_HIP(hipStreamSynchronize(m->stream));
     EDM_HIP(hipFree(d));
     return 0;
@@ -1328,7 +1372,8 @@ int frame_noise(edm_membrane* m, int64_t* n_dark, int64_t* n_ap) {
     unsigned long long* out = (unsigned long long*)(m->d_scal + 8);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, m->stream, (int64_t)4, (uint32_t*)out, 0u);
     hipLaunchKernelGGL(k_frame_noise, dim3(nb(npix)), dim3(kB), 0, m->stream, npix, m->ip.n_bins,
-                       m->flim, m->ip.dark_lambda, m->ip.p_ap, rng, m->step, out);
+                       m->flim, (uint8_t*)(m->flim + m->flim_elems), m->ip.dark_lambda,
+                       m->ip.p_ap, rng, m->step, out);
     EDM_HIP(hipGetLastError());
     if (n_dark || n_ap) {
         unsigned long long c[2];
@@ -1342,7 +1387,7 @@ int frame_noise(edm_membrane* m, int64_t* n_dark, int64_t* n_ap) {
 
 int frame_reset(edm_membrane* m) {
     if (!m->flim) return 0;
-    const int64_t n = (int64_t)m->flim_elems;
+    const int64_t n = (int64_t)m->flim_words;  // counts and row flags
     hipLaunchKernelGGL(k_fill_u32, dim3(std::min(nb(n), 8192)), dim3(kB), 0, m->stream, n, m->flim,
                        0u);
     EDM_HIP(hipGetLastError());
@@ -1360,44 +1405,131 @@ int frame_reset(edm_membrane* m) {
 // (flat index, count) pairs in index order, and resets it.  edm_frame_fetch
 // waits for that work and copies the pairs out.  One frame may be pending.
 // ----------------------------------------------------------------------
-static constexpr int64_t kFrChunk = 1 << 16;  // bins per compaction block
+// Both passes run one wavefront per 64 pixel rows and touch only the rows
+// whose flag is set (at 1024 x 1024 px and ~3.6e5 counts a frame, ~30 % of
+// them), so a frame end moves about a fifth of the stack instead of reading
+// it whole three times and writing it once.
 
-__global__ __launch_bounds__(256) void k_fr_count(const uint32_t* __restrict__ flim, int64_t n,
-                                                  uint32_t* blk) {
-    const int64_t c0 = (int64_t)blockIdx.x * kFrChunk, c1 = min(n, c0 + kFrChunk);
-    uint32_t c = 0;
-    for (int64_t i = c0 + threadIdx.x; i < c1; i += blockDim.x) c += flim[i] != 0u;
-    for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o);
-    __shared__ uint32_t ws[4];
-    if (lane_id() == 0) ws[threadIdx.x >> 6] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) blk[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+// Sum and nonzero-bin count of one row, read coalesced by the wavefront
+// (16 B per lane when nb % 4 == 0); wave-uniform arguments, every lane calls.
+__device__ __forceinline__ void wave_row_sum_nz(const uint32_t* row, int nb, uint32_t& sum,
+                                                uint32_t& nz) {
+    const int lane = lane_id();
+    uint32_t s = 0u, c = 0u;
+    if ((nb & 3) == 0) {
+        const uint4* r4 = (const uint4*)row;
+        for (int b = lane; b < (nb >> 2); b += 64) {
+            const uint4 v = r4[b];
+            s += v.x + v.y + v.z + v.w;
+            c += (v.x != 0u) + (v.y != 0u) + (v.z != 0u) + (v.w != 0u);
+        }
+    } else {
+        for (int b = lane; b < nb; b += 64) {
+            const uint32_t v = row[b];
+            s += v;
+            c += v != 0u;
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        s += __shfl_xor(s, o);
+        c += __shfl_xor(c, o);
+    }
+    sum = s;
+    nz = c;
 }
 
-__global__ __launch_bounds__(256) void k_fr_write(const uint32_t* __restrict__ flim, int64_t n,
-                                                  const uint32_t* __restrict__ off, uint32_t* oidx,
-                                                  uint32_t* ocnt) {
-    const int64_t c0 = (int64_t)blockIdx.x * kFrChunk, c1 = min(n, c0 + kFrChunk);
-    uint32_t base = off[blockIdx.x];
-    __shared__ uint32_t wtot[4];
-    const int w = threadIdx.x >> 6;
-    for (int64_t t0 = c0; t0 < c1; t0 += blockDim.x) {
-        const int64_t i = t0 + threadIdx.x;
-        const uint32_t v = i < c1 ? flim[i] : 0u;
-        const unsigned long long mk = __ballot(v != 0u);
-        const uint32_t below = (uint32_t)__popcll(mk & ((1ull << lane_id()) - 1ull));
-        if (lane_id() == 0) wtot[w] = (uint32_t)__popcll(mk);
-        __syncthreads();
-        uint32_t before = 0;
-        for (int q = 0; q < w; ++q) before += wtot[q];
-        if (v != 0u) {
-            oidx[base + before + below] = (uint32_t)i;
-            ocnt[base + before + below] = v;
+// First pass: the flagged rows' totals (for the afterpulses) and nonzero
+// bins, then every pixel's detector noise drawn as edm_frame_noise draws it
+// (frame_noise_pixel), and the wavefront's count of nonzero bins after it.
+__global__ __launch_bounds__(256) void k_fr_noise_count(int64_t npix, int nb, uint32_t* flim,
+                                                        double lam, double p_ap, Rng rng,
+                                                        uint32_t frame,
+                                                        unsigned long long* noise,
+                                                        uint32_t* wcnt) {
+    const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t q0 = w << 6;
+    if (q0 >= npix) return;  // whole wavefronts
+    uint8_t* flags = row_flags(flim, npix, nb);
+    const int lane = lane_id();
+    const int64_t q = q0 + lane;
+    const bool in = q < npix;
+    uint32_t tot0 = 0u, nz = 0u;
+    for (unsigned long long act = __ballot(in && flags[q] != 0u); act; act &= act - 1ull) {
+        const int j = __builtin_ctzll(act);
+        uint32_t s, c;
+        wave_row_sum_nz(flim + (q0 + j) * nb, nb, s, c);
+        if (lane == j) {
+            tot0 = s;
+            nz = c;
         }
-        const uint32_t tile = wtot[0] + wtot[1] + wtot[2] + wtot[3];
-        __syncthreads();
-        base += tile;
     }
+    unsigned long long nd = 0, na = 0;
+    if (in) nz += frame_noise_pixel(q, nb, flim, flags, tot0, lam, p_ap, rng, frame, nd, na);
+    noise_count(nd, na, noise);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) nz += __shfl_xor(nz, o);
+    if (lane == 0) wcnt[w] = nz;
+}
+
+// Second pass: the flagged rows' nonzero bins written as (flat index,
+// count) pairs from the wavefront's scanned offset, in index order; the
+// bins written are zeroed and the flags cleared, which resets the stack.
+__global__ __launch_bounds__(256) void k_fr_emit(int64_t npix, int nb, uint32_t* flim,
+                                                 const uint32_t* __restrict__ off, uint32_t* oidx,
+                                                 uint32_t* ocnt) {
+    const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t q0 = w << 6;
+    if (q0 >= npix) return;  // whole wavefronts
+    uint8_t* flags = row_flags(flim, npix, nb);
+    const int lane = lane_id();
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    const int64_t q = q0 + lane;
+    const bool in = q < npix;
+    const bool mine = in && flags[q] != 0u;
+    uint32_t base = off[w];
+    for (unsigned long long act = __ballot(mine); act; act &= act - 1ull) {
+        const int j = __builtin_ctzll(act);
+        uint32_t* row = flim + (q0 + j) * nb;
+        const uint32_t r0 = (uint32_t)((q0 + j) * nb);
+        if ((nb & 3) == 0) {
+            uint4* r4 = (uint4*)row;
+            const int n4 = nb >> 2;
+            for (int b0 = 0; b0 < n4; b0 += 64) {
+                const int b = b0 + lane;
+                const uint4 v = b < n4 ? r4[b] : make_uint4(0u, 0u, 0u, 0u);
+                const unsigned long long m0 = __ballot(v.x != 0u), m1 = __ballot(v.y != 0u),
+                                         m2 = __ballot(v.z != 0u), m3 = __ballot(v.w != 0u);
+                uint32_t pos = base + (uint32_t)(__popcll(m0 & lt) + __popcll(m1 & lt) +
+                                                 __popcll(m2 & lt) + __popcll(m3 & lt));
+                const uint32_t i0 = r0 + 4u * (uint32_t)b;
+                const uint32_t vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (vv[k]) {
+                        oidx[pos] = i0 + k;
+                        ocnt[pos] = vv[k];
+                        ++pos;
+                    }
+                if ((v.x | v.y | v.z | v.w) != 0u) r4[b] = make_uint4(0u, 0u, 0u, 0u);
+                base += (uint32_t)(__popcll(m0) + __popcll(m1) + __popcll(m2) + __popcll(m3));
+            }
+        } else {
+            for (int b0 = 0; b0 < nb; b0 += 64) {
+                const int b = b0 + lane;
+                const uint32_t v = b < nb ? row[b] : 0u;
+                const unsigned long long mk = __ballot(v != 0u);
+                if (v) {
+                    const uint32_t pos = base + (uint32_t)__popcll(mk & lt);
+                    oidx[pos] = r0 + (uint32_t)b;
+                    ocnt[pos] = v;
+                    row[b] = 0u;
+                }
+                base += (uint32_t)__popcll(mk);
+            }
+        }
+    }
+    if (mine) flags[q] = 0u;
 }
 
 // the detected-photon counter at the frame's end, beside its noise counts
@@ -1414,7 +1546,7 @@ void frames_free(edm_membrane* m) {
     m->fr_blk = m->fr_idx = m->fr_cnt = nullptr;
     m->fr_meta = nullptr;
     m->fr_ticket = -1;
-    m->fr_nchunk = 0;
+    m->fr_nwave = 0;
 }
 
 int frame_collect(edm_membrane* m, int32_t* ticket) {
@@ -1427,19 +1559,21 @@ int frame_collect(edm_membrane* m, int32_t* ticket) {
         set_error("edm_frame_collect: the stack has more than 2^32 bins");
         return EDM_EINVAL;
     }
+    const int64_t npix = (int64_t)m->ip.n_ch * m->ip.nx * m->ip.ny;
+    const int64_t nwave = (npix + 63) / 64;
     hipStream_t st = m->stream;
     if (!m->fr_stream) EDM_HIP(hipStreamCreate(&m->fr_stream));
     if (!m->fr_photons) EDM_HIP(hipEventCreateWithFlags(&m->fr_photons, hipEventDisableTiming));
     if (!m->fr_done) EDM_HIP(hipEventCreateWithFlags(&m->fr_done, hipEventDisableTiming));
     if (!m->flim_alt) {
         graph_touch(m);
-        m->fr_nchunk = (int)((n + kFrChunk - 1) / kFrChunk);
-        if (dalloc(&m->flim_alt, (size_t)n) || dalloc(&m->fr_blk, (size_t)m->fr_nchunk + 1) ||
+        m->fr_nwave = (int)nwave;
+        if (dalloc(&m->flim_alt, m->flim_words) || dalloc(&m->fr_blk, (size_t)nwave + 1) ||
             dalloc(&m->fr_idx, (size_t)n) || dalloc(&m->fr_cnt, (size_t)n) ||
             dalloc(&m->fr_meta, 4))
             return EDM_EHIP;
-        hipLaunchKernelGGL(k_fill_u32, dim3(std::min(nb(n), 8192)), dim3(kB), 0, st, n,
-                           m->flim_alt, 0u);
+        hipLaunchKernelGGL(k_fill_u32, dim3(std::min(nb((int64_t)m->flim_words), 8192)), dim3(kB),
+                           0, st, (int64_t)m->flim_words, m->flim_alt, 0u);
     }
     // kernel arguments of recorded step graphs name the stack being swapped
     graph_touch(m);
@@ -1448,20 +1582,17 @@ int frame_collect(edm_membrane* m, int32_t* ticket) {
     EDM_HIP(hipStreamWaitEvent(m->fr_stream, m->fr_photons, 0));
     std::swap(m->flim, m->flim_alt);
     hipStream_t fs = m->fr_stream;
-    const int64_t npix = (int64_t)m->ip.n_ch * m->ip.nx * m->ip.ny;
     Rng rng{(uint32_t)(m->p.seed & 0xffffffffu) ^ 0x2545F491u, (uint32_t)(m->p.seed >> 32)};
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, fs, (int64_t)8, (uint32_t*)m->fr_meta,
                        0u);
     hipLaunchKernelGGL(k_fr_meta, dim3(1), dim3(1), 0, fs, (const unsigned long long*)m->d_ctr,
                        m->fr_meta);
-    hipLaunchKernelGGL(k_frame_noise, dim3(nb(npix)), dim3(kB), 0, fs, npix, m->ip.n_bins, ended,
-                       m->ip.dark_lambda, m->ip.p_ap, rng, m->step, m->fr_meta);
-    hipLaunchKernelGGL(k_fr_count, dim3(m->fr_nchunk), dim3(256), 0, fs,
-                       (const uint32_t*)ended, n, m->fr_blk);
-    hipLaunchKernelGGL(k_sp_scan, dim3(1), dim3(1024), 0, fs, m->fr_blk, m->fr_nchunk);
-    hipLaunchKernelGGL(k_fr_write, dim3(m->fr_nchunk), dim3(256), 0, fs, (const uint32_t*)ended,
-                       n, (const uint32_t*)m->fr_blk, m->fr_idx, m->fr_cnt);
-    hipLaunchKernelGGL(k_fill_u32, dim3(std::min(nb(n), 8192)), dim3(kB), 0, fs, n, ended, 0u);
+    const int nblk = (int)((nwave + 3) / 4);
+    hipLaunchKernelGGL(k_fr_noise_count, dim3(nblk), dim3(256), 0, fs, npix, m->ip.n_bins, ended,
+                       m->ip.dark_lambda, m->ip.p_ap, rng, m->step, m->fr_meta, m->fr_blk);
+    hipLaunchKernelGGL(k_sp_scan, dim3(1), dim3(1024), 0, fs, m->fr_blk, (int)nwave);
+    hipLaunchKernelGGL(k_fr_emit, dim3(nblk), dim3(256), 0, fs, npix, m->ip.n_bins, ended,
+                       (const uint32_t*)m->fr_blk, m->fr_idx, m->fr_cnt);
     EDM_HIP(hipGetLastError());
     EDM_HIP(hipEventRecord(m->fr_done, fs));
     // the steps after this one write into the other stack: zeroed when it was
@@ -1480,7 +1611,7 @@ int frame_fetch(edm_membrane* m, int32_t ticket, int64_t* meta, uint32_t* idx, u
     EDM_HIP(hipEventSynchronize(m->fr_done));
     uint32_t tot32 = 0;
     unsigned long long mt[3] = {0, 0, 0};
-    EDM_HIP(hipMemcpyAsync(&tot32, m->fr_blk + m->fr_nchunk, 4, hipMemcpyDeviceToHost,
+    EDM_HIP(hipMemcpyAsync(&tot32, m->fr_blk + m->fr_nwave, 4, hipMemcpyDeviceToHost,
                            m->fr_stream));
     EDM_HIP(hipMemcpyAsync(mt, m->fr_meta, sizeof(mt), hipMemcpyDeviceToHost, m->fr_stream));
     EDM_HIP(hipStreamSynchronize(m->fr_stream));

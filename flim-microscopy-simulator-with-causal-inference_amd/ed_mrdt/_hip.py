@@ -165,6 +165,7 @@ _SIGS = {
     "edm_synchronize": [c_void_p],
     "edm_time_steps": [c_void_p, c_int32, c_int32, ctypes.POINTER(c_double),
                        ctypes.POINTER(c_double), c_void_p],
+    "edm_time_mark": [c_void_p, c_int32, ctypes.POINTER(c_double)],
     "edm_run_frame": [c_void_p, c_int32],
     "edm_frame_collect": [c_void_p, c_void_p],
     "edm_frame_fetch": [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_int64, c_void_p],

@@ -305,6 +305,12 @@ int edm_synchronize(edm_membrane *m);
  * *force_ms is -1. */
 int edm_time_steps(edm_membrane *m, int32_t n_steps, int32_t image, double *total_ms,
                    double *force_ms, double *breakdown_ms /* [8] or NULL */);
+/* A timed window over whatever is queued on the handle's stream between two
+ * calls, with no synchronisation in between: end = 0 records the opening
+ * event; end = 1 records the closing one, waits for it and returns the
+ * milliseconds between them.  No reference counterpart (the reference times
+ * its loop with time.perf_counter, pipeline.py:555). */
+int edm_time_mark(edm_membrane *m, int32_t end, double *elapsed_ms);
 /* Plain run of a stretch of a frame: n_steps x (dynamics step, then the
  * fused photon + imaging step when photophysics and imaging are set, then
  * a census row when recording with EDM_REC_AUTO), no host synchronisation

@@ -2,7 +2,10 @@
 against frames ended in place (finalize_frame: noise, dense copy, reset):
 the same stacks bin for bin, the same dark counts, afterpulses and signal
 photons per frame, with the next frame's steps queued before each fetch so
-that the compaction on the second stream overlaps them."""
+that the compaction on the second stream overlaps them.  The streamed frame
+end reads only the pixel rows whose flag a count has set; the odd geometry
+puts dark counts on rows no photon reached and rows that are not whole
+wavefronts or 16-byte vectors."""
 
 from __future__ import annotations
 
@@ -19,8 +22,20 @@ from ed_mrdt.photophysics import PhotonSimulator
 
 pytestmark = pytest.mark.gpu
 
+def _odd_geometry():
+    """37 x 29 px (rows not a multiple of 64), 250 bins (not a multiple of 4),
+    dark counts on a fifth of the pixels and 5 % afterpulses."""
+    cfg = workloads.binding_fret(seed=5)
+    cfg.scan.pixels_x, cfg.scan.pixels_y, cfg.scan.pixel_size = 37, 29, 108.0
+    cfg.tcspc.n_bins = 250
+    cfg.detector.dark_count_rate = 2e4
+    cfg.detector.afterpulsing_prob = 0.05
+    return cfg
+
+
 CASES = {
     "binding-fret": (lambda: workloads.binding_fret(seed=4), 40, 5),
+    "odd-geometry": (_odd_geometry, 30, 4),
     "wave-128px": (lambda: workloads.wave_system(seed=3, n_steps=1000), 100, 4),
     "membrane-512px": (lambda: workloads.large_membrane(side=31.6, n_alive=100_000,
                                                         pixels=512), 100, 3),
