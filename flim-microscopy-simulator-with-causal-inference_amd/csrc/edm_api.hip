@@ -72,8 +72,8 @@ int edm_create(const edm_params* p, edm_membrane** out) {
         for (auto f : fbs)
             if (f->sensor >= 0 && (f->grid < 0 || f->grid >= p->n_grids || f->sensor >= p->n_species))
                 return invalid("edm_create: feedback refers to a missing grid or species");
-        if (r.kind == EDM_RX_CAT && (r.a == r.b || r.a == r.c))
-            return invalid("edm_create: catalysis needs enzyme distinct from substrate and product");
+        if (r.kind == EDM_RX_CAT && r.a == r.b)
+            return invalid("edm_create: catalysis needs an enzyme distinct from its substrate");
         if (r.kind < EDM_RX_BIND || r.kind > EDM_RX_ADAPT)
             return invalid("edm_create: unknown reaction kind");
         if (r.kind == EDM_RX_RECRUIT && r.use_adapt &&
@@ -88,6 +88,8 @@ int edm_create(const edm_params* p, edm_membrane** out) {
     m->rs = m->fp64 ? 8 : 4;
     m->grids.assign(p->grids, p->grids + p->n_grids);
     m->rx.assign(p->reactions, p->reactions + p->n_reactions);
+    for (const auto& r : m->rx)
+        if (r.kind == EDM_RX_CAT && r.a == r.c) m->has_autocat = true;
     m->p.grids = nullptr;
     m->p.reactions = nullptr;
     m->p.sigma = m->p.epsilon = m->p.k_harm = m->p.D_base = nullptr;
@@ -132,7 +134,7 @@ int edm_destroy(edm_membrane* m) {
                     m->cell_start, m->perm, m->cub_tmp, m->sig, m->eps, m->kh, m->Dbase, m->rcx,
                     m->rcy, m->rr, m->rf, m->d_count_off, m->d_grid_dims, m->d_grid_s, m->counts,
                     m->delta, m->pair_cnt, m->pair_off, m->d_ctr, m->d_draws, m->edge_key,
-                    m->edge_a, m->edge_b, m->n_edges, m->best, m->cand_id, m->cand_slot, m->rcdf,
+                    m->edge_a, m->edge_b, m->n_edges, m->best, m->tconv, m->cand_id, m->cand_slot, m->rcdf,
                     m->cand_x, m->cand_y, m->n_cand, m->w_adapt, m->d_pp, m->k_fret, m->acc,
                     m->ph_nd, m->ph_nf, m->ph_cnt, m->ph_off, m->ph_pid, m->ph_t, m->ph_ch,
                     m->ph_fret, m->flim, m->pcx, m->det_flag, m->det_pos, m->sv_key, m->sv_key2,
@@ -585,7 +587,7 @@ int edm_time_steps(edm_membrane* m, int32_t n_steps, int32_t image, double* tota
     }
     EDM_HIP(hipEventRecord(m->ev0, m->stream));
     for (int k = 0; k < n_steps; ++k) {
-        m->need_pair_offsets = m->p.pair_cap > 0;
+        m->need_pair_offsets = loop_pair_offsets(m);
         m->pair_sync = false;
         if (int rc = dyn_step_begin(m)) return rc;
         m->time_force = true;

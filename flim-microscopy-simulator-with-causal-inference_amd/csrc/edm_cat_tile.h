@@ -53,6 +53,49 @@ __device__ __forceinline__ T cat_pick(const T (&a)[kCatFuse], int j) {
     return v;
 }
 
+// Probability with which member j's pair converts a substrate at (xs, ys)
+// (post-move): P_cat x Hill(rho at the substrate) [x adaptation].
+template <typename Real>
+__device__ __forceinline__ Real cat_prob(const CatArgs<Real>& A, const Grids<Real>& G, int j,
+                                         Real xs, Real ys) {
+    const Real prob = cat_pick(A.prob, j);
+    if (A.fb.sensor < 0) return prob;
+    const Real hf = fb_factor(G, A.fb, xs, ys);
+    if (!A.w) return prob * hf;
+    Real af = (Real)1;
+    if (A.Kw > (Real)0) {
+        // dynamics.py:1078 reads adaptation_field[gx, gy] with the
+        // feedback grid's indices on the adaptation grid's array
+        // (no bounds check): element gx * wny + gy where that lies in
+        // the array; past it the reference has no defined value and
+        // the field is read at the substrate's own adaptation cell
+        const int gx = trunc_cell(xs, (Real)A.fb.gsx, A.fb.gnx);
+        const int gy = trunc_cell(ys, (Real)A.fb.gsy, A.fb.gny);
+        int wc = gx * A.wny + gy;
+        if (wc >= A.wnx * A.wny) {
+            const double asx = ((double)A.fb.gnx * A.fb.gsx) / A.wnx;
+            const double asy = ((double)A.fb.gny * A.fb.gsy) / A.wny;
+            wc = trunc_cell(xs, (Real)asx, A.wnx) * A.wny + trunc_cell(ys, (Real)asy, A.wny);
+        }
+        const Real wr = pow((Real)A.w[wc] / A.Kw, A.nw);
+        af = (Real)1 / ((Real)1 + wr);
+    }
+    return prob * hf * af;
+}
+
+// Member j's uniform for the pair (sa, sb) (sa holds the lower particle
+// index): the hook draw at the pair's index in the reference's pair list,
+// or the Philox draw of the pair's two particle indices.
+template <typename Real>
+__device__ __forceinline__ double cat_uniform(const Cells<Real>& C, const CatArgs<Real>& A,
+                                              uint32_t step, int j, int sa, int sb, int it,
+                                              int iu) {
+    if (A.dr.base) return A.dr.base[cat_pick(A.doff, j) + pair_index(C, sa, sb)];
+    const uint32_t lo = (uint32_t)(min(it, iu) - C.kbase);
+    const uint32_t hi = (uint32_t)(max(it, iu) - C.kbase);
+    return (double)Uni<Real>::first(A.rng.draw(step, lo, hi, (kTagCat << 16) | cat_pick(A.rxi, j)));
+}
+
 template <typename Real>
 __device__ __forceinline__ int cat_try(const Cells<Real>& C, const Post<Real>& P,
                                        const CatArgs<Real>& A, uint32_t step, int j,
@@ -65,50 +108,149 @@ __device__ __forceinline__ int cat_try(const Cells<Real>& C, const Post<Real>& P
     const bool tlow = it < iu;
     const int sa = tlow ? t : u, sb = tlow ? u : t;
     if (!pair_in_list(C, sa, sb)) return 0;
-    const Real prob = cat_pick(A.prob, j);
-    Real pe = prob;
-    if (A.fb.sensor >= 0) {
-        const Real hf = fb_factor(G, A.fb, xnu, ynu);
-        if (A.w) {
-            Real af = (Real)1;
-            if (A.Kw > (Real)0) {
-                // dynamics.py:1078 reads adaptation_field[gx, gy] with the
-                // feedback grid's indices on the adaptation grid's array
-                // (no bounds check): element gx * wny + gy where that lies in
-                // the array; past it the reference has no defined value and
-                // the field is read at the substrate's own adaptation cell
-                const int gx = trunc_cell(xnu, (Real)A.fb.gsx, A.fb.gnx);
-                const int gy = trunc_cell(ynu, (Real)A.fb.gsy, A.fb.gny);
-                int wc = gx * A.wny + gy;
-                if (wc >= A.wnx * A.wny) {
-                    const double asx = ((double)A.fb.gnx * A.fb.gsx) / A.wnx;
-                    const double asy = ((double)A.fb.gny * A.fb.gsy) / A.wny;
-                    wc = trunc_cell(xnu, (Real)asx, A.wnx) * A.wny +
-                         trunc_cell(ynu, (Real)asy, A.wny);
-                }
-                const Real wr = pow((Real)A.w[wc] / A.Kw, A.nw);
-                af = (Real)1 / ((Real)1 + wr);
-            }
-            pe = prob * hf * af;
-        } else {
-            pe = prob * hf;
-        }
-    }
-    double uu;
-    if (A.dr.base) {
-        uu = A.dr.base[cat_pick(A.doff, j) + pair_index(C, sa, sb)];
-    } else {
-        const uint32_t lo = (uint32_t)(min(it, iu) - C.kbase);
-        const uint32_t hi = (uint32_t)(max(it, iu) - C.kbase);
-        uu = (double)Uni<Real>::first(
-            A.rng.draw(step, lo, hi, (kTagCat << 16) | cat_pick(A.rxi, j)));
-    }
+    const Real pe = cat_prob(A, G, j, xnu, ynu);
+    const double uu = cat_uniform(C, A, step, j, sa, sb, it, iu);
     if (uu < (double)pe && cas_byte(P.sp, u, (int8_t)A.sub, (int8_t)A.prod)) {
         count_update(G, n_grids, A.sub, xnu, ynu, -1, A.deferred);
         count_update(G, n_grids, A.prod, xnu, ynu, +1, A.deferred);
         return 1;
     }
     return 0;
+}
+
+// ----------------------------------------------------------------------
+// Autocatalysis: enzyme == product, E + S -> 2E (dynamics.py:826-1090 with
+// enzyme_id == product_id).  The reference's loop runs over the pair list
+// in order, so a substrate converted at pair p is an enzyme for every later
+// pair and conversions cascade along the pair order.  Substrate s converts
+// at
+//   t(s) = min { p : pair p holds s, is in the (capped) list and within
+//                contact after the move, its uniform accepts with the Hill
+//                factor read at s, and its partner is an enzyme at p: one
+//                from the start of the pass, or a substrate with
+//                t(partner) < p },
+// or never.  The condition at p looks only at conversions before p, so t
+// has exactly one solution, the loop's outcome.  k_autocat_edges lists
+// every enzyme-substrate and substrate-substrate pair of the list with its
+// index and the directions its uniform accepts; k_autocat_resolve lowers t
+// from "never" in rounds over those edges until a round changes nothing.
+// Each value it holds stays at or above the solution (a lowering needs a
+// partner already converted before p), so the round without a change ends
+// on the solution; then the substrates with a finite t convert.  Pair
+// indices come from the pair-list offsets in both draw modes, so Philox
+// runs cascade in the reference's pair order too.
+// ----------------------------------------------------------------------
+static constexpr unsigned long long kAcNever = ~0ull;
+static constexpr unsigned long long kAcIndex = (1ull << 61) - 1ull;  // edge key: pair index
+static constexpr int kAcFwd = 62, kAcBwd = 61;  // edge key bits: a converts b, b converts a
+
+// One thread per particle of the enzyme or substrate species; each pair is
+// listed once, from its lower particle index (ea = that slot).
+template <typename Real>
+__global__ void k_autocat_edges(int n, Cells<Real> C, Post<Real> P, CatArgs<Real> A,
+                                Grids<Real> G, unsigned long long* ekey, int32_t* ea,
+                                int32_t* eb, unsigned int* ne, unsigned int cap) {
+    const uint32_t step = A.step + (A.step_base ? *A.step_base : 0u);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int E = A.enz[0], S = A.sub;
+    const int st = P.sp[t];
+    if ((st != E && st != S) || !(C.fl[t] & EDM_FLAG_ALIVE)) return;
+    const Real xt = C.x[t], yt = C.y[t];
+    const Real xnt = P.xn[t], ynt = P.yn[t];
+    const int it = C.id[t];
+    int cx, cy;
+    cell_of(C, xt, yt, cx, cy);
+    for (int k = 0; k < 9; ++k) {
+        const int c = nb_cell(C, cx, cy, k);
+        if (c < 0) continue;
+        for (int u = (int)C.start[c]; u < (int)C.start[c + 1]; ++u) {
+            const int su = P.sp[u];
+            if (u == t || (su != E && su != S) || (st == E && su == E)) continue;
+            const int iu = C.id[u];
+            if (iu < it || !(C.fl[u] & EDM_FLAG_ALIVE)) continue;
+            Real dx, dy;
+            if (min_image(xt, yt, C.x[u], C.y[u], C.Lx, C.Ly, C.px, C.py, dx, dy) >= C.cut2)
+                continue;
+            if (!pair_in_list(C, t, u)) continue;
+            const Real xnu = P.xn[u], ynu = P.yn[u];
+            if (min_image(xnt, ynt, xnu, ynu, C.Lx, C.Ly, C.px, C.py, dx, dy) > A.contact2[0])
+                continue;
+            const double uu = cat_uniform(C, A, step, 0, t, u, it, iu);
+            const bool fwd = su == S && uu < (double)cat_prob(A, G, 0, xnu, ynu);
+            const bool bwd = st == S && uu < (double)cat_prob(A, G, 0, xnt, ynt);
+            if (!fwd && !bwd) continue;
+            const unsigned int e = atomicAdd(ne, 1u);
+            if (e < cap) {
+                ekey[e] = ((unsigned long long)pair_index(C, t, u) & kAcIndex) |
+                          ((unsigned long long)fwd << kAcFwd) |
+                          ((unsigned long long)bwd << kAcBwd);
+                ea[e] = t;
+                eb[e] = u;
+            }
+        }
+    }
+}
+
+// One workgroup: the rounds, the conversions, and t reset to "never" on
+// the edges' particles (tconv holds "never" for every slot between passes).
+template <typename Real>
+__global__ __launch_bounds__(1024) void k_autocat_resolve(Post<Real> P, CatArgs<Real> A,
+                                                          const unsigned long long* ekey,
+                                                          const int32_t* ea, const int32_t* eb,
+                                                          unsigned int* ne, unsigned int cap,
+                                                          unsigned long long* tconv,
+                                                          unsigned long long* ctr, Grids<Real> G,
+                                                          int n_grids) {
+    const unsigned int m = min(*ne, cap);
+    // more accepted pairs than the buffer holds: the outcome would differ
+    // from the reference's loop, so keep the count in ne[1] for the host to
+    // report (EDM_ECAPACITY)
+    if (threadIdx.x == 0 && *ne > cap) atomicMax(&ne[1], *ne);
+    const int E = A.enz[0], S = A.sub;
+    const int8_t* sp = P.sp;  // the pass's starting species until the conversions
+    for (;;) {
+        int changed = 0;
+        for (unsigned int e = threadIdx.x; e < m; e += blockDim.x) {
+            const unsigned long long k = ekey[e];
+            const unsigned long long p = k & kAcIndex;
+            const int a = ea[e], b = eb[e];
+            // t is read with atomics: lowered by other threads of the round
+            if ((k >> kAcFwd) & 1ull) {
+                const bool enz = sp[a] == E || atomicAdd(&tconv[a], 0ull) < p;
+                if (enz && p < atomicAdd(&tconv[b], 0ull) && atomicMin(&tconv[b], p) > p)
+                    changed = 1;
+            }
+            if ((k >> kAcBwd) & 1ull) {
+                const bool enz = sp[b] == E || atomicAdd(&tconv[b], 0ull) < p;
+                if (enz && p < atomicAdd(&tconv[a], 0ull) && atomicMin(&tconv[a], p) > p)
+                    changed = 1;
+            }
+        }
+        __threadfence();
+        if (!__syncthreads_or(changed)) break;
+    }
+    for (unsigned int e = threadIdx.x; e < m; e += blockDim.x) {
+        const int ends[2] = {ea[e], eb[e]};
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int x = ends[q];
+            if (atomicAdd(&tconv[x], 0ull) != kAcNever &&
+                cas_byte(P.sp, x, (int8_t)S, (int8_t)E)) {
+                atomicAdd(&ctr[2], 1ull);
+                count_update(G, n_grids, S, P.xn[x], P.yn[x], -1, A.deferred);
+                count_update(G, n_grids, E, P.xn[x], P.yn[x], +1, A.deferred);
+            }
+        }
+    }
+    __threadfence();
+    __syncthreads();
+    for (unsigned int e = threadIdx.x; e < m; e += blockDim.x) {
+        tconv[ea[e]] = kAcNever;
+        tconv[eb[e]] = kAcNever;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *ne = 0u;  // ready for the next pass
 }
 
 // Appends v to the block's LDS queue for the lanes with `want` (one atomic

@@ -1641,9 +1641,15 @@ static int create_t(edm_membrane* m) {
     // one per A-B pair within the cutoff, so 4 per particle covers a
     // membrane up to ~8x the 1M workload's density, and a step that accepts
     // more is reported (ne[1], EDM_ECAPACITY) rather than truncated silently
-    bool has_bind = false;
+    // (autocatalysis lists its accepted enzyme-substrate and substrate-
+    // substrate pairs in the same buffer)
+    bool has_bind = m->has_autocat;
     for (const auto& r : m->rx) has_bind = has_bind || r.kind == EDM_RX_BIND;
     m->edge_cap = has_bind ? (size_t)std::max<int64_t>(4 * (int64_t)n, 4096) : 1024;
+    if (m->has_autocat) {
+        if (dalloc(&m->tconv, n)) return EDM_EHIP;
+        EDM_HIP(hipMemset(m->tconv, 0xff, (size_t)n * sizeof(unsigned long long)));
+    }
     if (dalloc(&m->edge_key, m->edge_cap) || dalloc(&m->edge_a, m->edge_cap) ||
         dalloc(&m->edge_b, m->edge_cap) || dalloc(&m->n_edges, 4) || dalloc(&m->best, n) ||
         dalloc(&m->cand_id, n) || dalloc(&m->cand_slot, n) || dalloc(&m->cand_x, n) ||
@@ -1689,12 +1695,13 @@ int64_t hook_draws_per_step(const edm_membrane* m, int64_t n_pairs) {
 // without feedback or adaptation.
 static int cat_run_len(const std::vector<edm_reaction>& rx, size_t ri) {
     const edm_reaction& r = rx[ri];
-    const bool plain = r.fb1.sensor < 0 && !r.use_adapt;
+    // autocatalysis depends on the order of the reactions' pairs: never fused
+    const bool plain = r.fb1.sensor < 0 && !r.use_adapt && r.a != r.c;
     int nf = 1;
     while (plain && nf < kCatFuse && ri + nf < rx.size()) {
         const edm_reaction& r2 = rx[ri + nf];
         if (r2.kind != EDM_RX_CAT || r2.b != r.b || r2.c != r.c || r2.fb1.sensor >= 0 ||
-            r2.use_adapt)
+            r2.use_adapt || r2.a == r2.c)
             break;
         ++nf;
     }
@@ -1924,9 +1931,9 @@ int dyn_check_overflow(edm_membrane* m) {
     EDM_HIP(hipMemcpyAsync(ne, m->n_edges, sizeof(ne), hipMemcpyDeviceToHost, m->stream));
     EDM_HIP(hipStreamSynchronize(m->stream));
     if (ne[1] > 0) {
-        set_error("binding: a step accepted " + std::to_string(ne[1]) +
-                  " candidate edges, more than the edge buffer holds (" +
-                  std::to_string(m->edge_cap) + "); the matching of that step is incomplete");
+        set_error("binding / autocatalysis: a step accepted " + std::to_string(ne[1]) +
+                  " candidate pairs, more than the edge buffer holds (" +
+                  std::to_string(m->edge_cap) + "); the outcome of that step is incomplete");
         return EDM_ECAPACITY;
     }
     return 0;
@@ -1967,7 +1974,7 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
     F.rint2 = (Real)m->rint2;
     if (hook)
         for (const auto& r : m->rx)
-            if (r.kind == EDM_RX_BIND) {
+            if (r.kind == EDM_RX_BIND || (r.kind == EDM_RX_CAT && r.a == r.c)) {
                 if (int rc = ensure_edge_cap(m, (size_t)m->n_pairs)) return rc;
                 break;
             }
@@ -2109,7 +2116,17 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
                                           ? m->cat_side == 1
                                           : cat_by_substrate(m->alive_at_upload, A.enz, nf, r.b);
                 bool walked = false;
-                if (cp && cp->list >= 0 && m->wl_valid) {
+                if (r.a == r.c) {  // autocatalysis: conversions cascade in pair order
+                    hipLaunchKernelGGL((k_autocat_edges<Real>), dim3(nblk(n)), dim3(kBlock), 0,
+                                       st, n, C, P, A, G, m->edge_key, m->edge_a, m->edge_b,
+                                       m->n_edges, (unsigned int)m->edge_cap);
+                    hipLaunchKernelGGL((k_autocat_resolve<Real>), dim3(1), dim3(1024), 0, st, P,
+                                       A, (const unsigned long long*)m->edge_key,
+                                       (const int32_t*)m->edge_a, (const int32_t*)m->edge_b,
+                                       m->n_edges, (unsigned int)m->edge_cap, m->tconv,
+                                       m->d_ctr, G, ng);
+                    walked = true;
+                } else if (cp && cp->list >= 0 && m->wl_valid) {
                     if (cp->late)  // entities as they are now
                         wl_build(m, (const int8_t*)m->a.sp, (const uint8_t*)m->a.fl, &cp->list,
                                  &cp->spec, 1);

@@ -69,7 +69,7 @@ static bool same_host(const StepHost& u, const StepHost& v) {
 }
 
 static int eager_step(edm_membrane* m, bool image) {
-    m->need_pair_offsets = m->p.pair_cap > 0;
+    m->need_pair_offsets = loop_pair_offsets(m);
     m->pair_sync = false;
     if (int rc = dyn_step_begin(m)) return rc;
     if (int rc = dyn_step_finish(m, nullptr, false)) return rc;

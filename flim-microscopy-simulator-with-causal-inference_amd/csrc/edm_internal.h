@@ -163,6 +163,8 @@ struct edm_membrane {
     uint64_t* pair_off = nullptr;  // exclusive scan of pair_cnt
     int64_t n_pairs_found = 0, n_pairs = 0;
     bool need_pair_offsets = false;
+    bool has_autocat = false;             // a catalysis whose enzyme is its product
+    unsigned long long* tconv = nullptr;  // [n] autocatalysis: pair index of conversion
     // parity steps read the pair count back every step; step loops leave it
     // on the device (the kernels then decide the cap themselves)
     bool pair_sync = true;
@@ -333,6 +335,12 @@ struct edm_membrane {
 static inline edm::StepRef step_ref(const edm_membrane* m) {
     return m->graph_rec ? edm::StepRef{m->d_step, m->step - m->graph_step0}
                          : edm::StepRef{nullptr, m->step};
+}
+
+// Pair-list offsets of a step in a step loop: for the faithful pair cap,
+// and for autocatalysis, whose conversions follow the reference's pair order.
+static inline bool loop_pair_offsets(const edm_membrane* m) {
+    return m->p.pair_cap > 0 || m->has_autocat;
 }
 
 static inline WlView wl_view(const edm_membrane* m, int L) {

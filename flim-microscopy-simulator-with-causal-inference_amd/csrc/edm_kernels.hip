@@ -643,6 +643,29 @@ struct AdaptArgs {
     double K, n;
 };
 
+// Conversion probability of a pair whose substrate sits at (xs, ys):
+// prob x Hill(rho) [x adaptation] (dynamics.py:826-1090).
+__device__ __forceinline__ double cat_free_prob(double prob, const FbArgs& fb,
+                                                const AdaptArgs& ad, double xs, double ys) {
+    if (!fb.density) return prob;
+    int gx, gy;
+    const double hf = fb_at(fb, xs, ys, &gx, &gy);
+    if (!ad.w) return prob * hf;
+    double af = 1.0;
+    if (ad.K > 0.0) {
+        // adaptation_field[gx, gy] with the feedback grid's indices
+        // (dynamics.py:1078); past the array the reference has no defined
+        // value and the substrate's own adaptation cell is read instead
+        int wc = gx * ad.ny + gy;
+        if (wc >= ad.nx * ad.ny) {
+            const double asx = (fb.nx * fb.sx) / ad.nx, asy = (fb.ny * fb.sy) / ad.ny;
+            wc = ref_cell(xs, asx, ad.nx) * ad.ny + ref_cell(ys, asy, ad.ny);
+        }
+        af = 1.0 / (1.0 + pow(ad.w[wc] / ad.K, ad.n));
+    }
+    return prob * hf * af;
+}
+
 // dynamics.py:826-1090: the plain, feedback and feedback + adaptation forms
 __global__ void k_cat_free(const double* __restrict__ pos, int32_t* sp,
                            const uint8_t* __restrict__ alive, int enz, int sub, int prod, double c2,
@@ -662,31 +685,76 @@ __global__ void k_cat_free(const double* __restrict__ pos, int32_t* sp,
     if (min_image_d(pos[2 * i], pos[2 * i + 1], pos[2 * j], pos[2 * j + 1], Lx, Ly, px, py, dx,
                     dy) > c2)
         return;
-    double pe = prob;
-    if (fb.density) {
-        int gx, gy;
-        const double xs = pos[2 * s], ys = pos[2 * s + 1];
-        const double hf = fb_at(fb, xs, ys, &gx, &gy);
-        if (ad.w) {
-            double af = 1.0;
-            if (ad.K > 0.0) {
-                // adaptation_field[gx, gy] with the feedback grid's indices
-                // (dynamics.py:1078); past the array the reference has no
-                // defined value and the substrate's own adaptation cell is
-                // read instead
-                int wc = gx * ad.ny + gy;
-                if (wc >= ad.nx * ad.ny) {
-                    const double asx = (fb.nx * fb.sx) / ad.nx, asy = (fb.ny * fb.sy) / ad.ny;
-                    wc = ref_cell(xs, asx, ad.nx) * ad.ny + ref_cell(ys, asy, ad.ny);
-                }
-                af = 1.0 / (1.0 + pow(ad.w[wc] / ad.K, ad.n));
-            }
-            pe = prob * hf * af;
-        } else {
-            pe = prob * hf;
-        }
-    }
+    const double pe = cat_free_prob(prob, fb, ad, pos[2 * s], pos[2 * s + 1]);
     if (u[p] < pe && atomicCAS(&sp[s], sub, prod) == sub) atomicAdd(count, 1u);
+}
+
+// Autocatalysis (enzyme == product) over a given pair list: the pair's
+// index is its time in the reference's loop, so a substrate converted at
+// pair p acts as an enzyme from pair p + 1 on.  Same fixpoint as the
+// membrane's k_autocat_resolve: t(s) is the first pair that converts s,
+// lowered from "never" in rounds until none changes.  Candidates: pairs of
+// an enzyme or substrate with a substrate, within contact, with the
+// directions their uniform accepts (bit 62: i converts j, bit 61: j
+// converts i) above the pair index.
+__global__ void k_ac_free_cand(const double* __restrict__ pos, const int32_t* __restrict__ sp,
+                               const uint8_t* __restrict__ alive, int enz, int sub, double c2,
+                               double prob, const int32_t* __restrict__ pairs, int64_t m,
+                               const double* __restrict__ u, double Lx, double Ly, int px, int py,
+                               FbArgs fb, AdaptArgs ad, uint64_t* cand, uint32_t* n_cand,
+                               int32_t* ea, int32_t* eb) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= m) return;
+    const int i = pairs[2 * p], j = pairs[2 * p + 1];
+    if (!alive[i] || !alive[j]) return;
+    const int si = sp[i], sj = sp[j];
+    if ((si != enz && si != sub) || (sj != enz && sj != sub) || (si == enz && sj == enz)) return;
+    double dx, dy;
+    if (min_image_d(pos[2 * i], pos[2 * i + 1], pos[2 * j], pos[2 * j + 1], Lx, Ly, px, py, dx,
+                    dy) > c2)
+        return;
+    const bool fwd = sj == sub && u[p] < cat_free_prob(prob, fb, ad, pos[2 * j], pos[2 * j + 1]);
+    const bool bwd = si == sub && u[p] < cat_free_prob(prob, fb, ad, pos[2 * i], pos[2 * i + 1]);
+    if (!fwd && !bwd) return;
+    const uint32_t e = atomicAdd(n_cand, 1u);
+    cand[e] = (uint64_t)p | ((uint64_t)fwd << 62) | ((uint64_t)bwd << 61);
+    ea[e] = i;
+    eb[e] = j;
+}
+
+__global__ __launch_bounds__(1024) void k_ac_free_resolve(const uint32_t* __restrict__ n_cand,
+                                                          const uint64_t* __restrict__ cand,
+                                                          const int32_t* __restrict__ ea,
+                                                          const int32_t* __restrict__ eb,
+                                                          int32_t* sp, int enz, int sub,
+                                                          unsigned long long* t,
+                                                          uint32_t* count) {
+    constexpr unsigned long long kNever = ~0ull, kIdx = (1ull << 61) - 1ull;
+    const uint32_t m = *n_cand;
+    for (;;) {
+        int changed = 0;
+        for (uint32_t e = threadIdx.x; e < m; e += blockDim.x) {
+            const uint64_t k = cand[e];
+            const unsigned long long p = k & kIdx;
+            const int a = ea[e], b = eb[e];
+            if ((k >> 62) & 1ull) {
+                const bool enzyme = sp[a] == enz || atomicAdd(&t[a], 0ull) < p;
+                if (enzyme && p < atomicAdd(&t[b], 0ull) && atomicMin(&t[b], p) > p) changed = 1;
+            }
+            if ((k >> 61) & 1ull) {
+                const bool enzyme = sp[b] == enz || atomicAdd(&t[b], 0ull) < p;
+                if (enzyme && p < atomicAdd(&t[a], 0ull) && atomicMin(&t[a], p) > p) changed = 1;
+            }
+        }
+        __threadfence();
+        if (!__syncthreads_or(changed)) break;
+    }
+    for (uint32_t e = threadIdx.x; e < m; e += blockDim.x) {
+        const int ends[2] = {ea[e], eb[e]};
+        for (int q = 0; q < 2; ++q)
+            if (atomicAdd(&t[ends[q]], 0ull) != kNever && atomicCAS(&sp[ends[q]], sub, enz) == sub)
+                atomicAdd(count, 1u);
+    }
 }
 
 // ----------------------------------------------------------- recruitment
@@ -1036,8 +1104,8 @@ int edm_catalysis(const double* positions, int32_t* species_id, const uint8_t* i
     if (n < 0 || n_pairs < 0 || (n_pairs > 0 && (!positions || !species_id || !is_alive || !pairs ||
                                                  !rng_uniform)))
         return kinvalid("edm_catalysis: null argument");
-    if (enzyme_id == substrate_id || enzyme_id == product_id)
-        return kinvalid("edm_catalysis: the enzyme must differ from the substrate and the product");
+    if (enzyme_id == substrate_id)
+        return kinvalid("edm_catalysis: the enzyme must differ from the substrate");
     if (bad_grid(density_field, nx_grid, ny_grid, cell_sx, cell_sy) ||
         (adaptation_field && (!density_field || nx_adapt <= 0 || ny_adapt <= 0)))
         return kinvalid("edm_catalysis: bad density or adaptation grid");
@@ -1047,6 +1115,32 @@ int edm_catalysis(const double* positions, int32_t* species_id, const uint8_t* i
     FbArgs fb = make_fb(density_field, nx_grid, ny_grid, cell_sx, cell_sy, K_half, n_hill,
                         is_inhibition, basal_fraction);
     AdaptArgs ad{adaptation_field, nx_adapt, ny_adapt, K_adapt, n_adapt};
+    if (n_pairs > 0 && enzyme_id == product_id) {
+        if (n_pairs > 0xffffffffLL) return kinvalid("edm_catalysis: more than 2^32 pairs");
+        const size_t b_c = align256((size_t)n_pairs * 8), b_e = align256((size_t)n_pairs * 4),
+                     b_t = align256((size_t)std::max(n, 1) * 8);
+        AsyncBuf ab;
+        if (int rc = ab.alloc(b_c + 2 * b_e + b_t + 256, st)) return rc;
+        char* buf = ab.p;
+        uint64_t* cand = (uint64_t*)buf;
+        int32_t* ea = (int32_t*)(buf + b_c);
+        int32_t* eb = (int32_t*)(buf + b_c + b_e);
+        unsigned long long* t = (unsigned long long*)(buf + b_c + 2 * b_e);
+        uint32_t* nc = (uint32_t*)(buf + b_c + 2 * b_e + b_t);
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, nc, 1, 0u);
+        if (n > 0)
+            hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(2 * (int64_t)n)), dim3(kB), 0, st,
+                               (uint32_t*)t, 2 * (int64_t)n, 0xffffffffu);
+        hipLaunchKernelGGL(k_ac_free_cand, dim3(blocks_for(n_pairs)), dim3(kB), 0, st, positions,
+                           (const int32_t*)species_id, is_alive, enzyme_id, substrate_id,
+                           contact_radius_um * contact_radius_um, prob, pairs, n_pairs,
+                           rng_uniform, Lx, Ly, periodic_x ? 1 : 0, periodic_y ? 1 : 0, fb, ad,
+                           cand, nc, ea, eb);
+        hipLaunchKernelGGL(k_ac_free_resolve, dim3(1), dim3(1024), 0, st, (const uint32_t*)nc,
+                           (const uint64_t*)cand, (const int32_t*)ea, (const int32_t*)eb,
+                           species_id, enzyme_id, substrate_id, t, c.d);
+        return c.read(n_converted);
+    }
     if (n_pairs > 0)
         hipLaunchKernelGGL(k_cat_free, dim3(blocks_for(n_pairs)), dim3(kB), 0, st, positions,
                            species_id, is_alive, enzyme_id, substrate_id, product_id,

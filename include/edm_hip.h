@@ -448,9 +448,12 @@ int edm_dissociation(int32_t *species_id, int32_t *bond_partner, uint8_t *is_ali
                      int32_t reactant_b, double prob_dissoc, const double *rng_uniform,
                      int64_t *n_dissociated, void *stream);
 /* apply_catalytic_reaction, _with_feedback, _with_feedback_and_adaptation
- * (dynamics.py:826-1090); the enzyme must differ from the substrate and the
- * product.  adaptation_field [nx_adapt, ny_adapt] (NULL: none) is read at
- * the feedback grid's indices, as dynamics.py:1078 reads it */
+ * (dynamics.py:826-1090); the enzyme must differ from the substrate.  With
+ * enzyme == product (autocatalysis, E + S -> 2E) a substrate converted at
+ * pair p acts as an enzyme for the pairs after p, as the reference's
+ * sequential loop over the pair list leaves it.  adaptation_field
+ * [nx_adapt, ny_adapt] (NULL: none) is read at the feedback grid's indices,
+ * as dynamics.py:1078 reads it */
 int edm_catalysis(const double *positions, int32_t *species_id, const uint8_t *is_alive,
                   int32_t n, int32_t enzyme_id, int32_t substrate_id, int32_t product_id,
                   double contact_radius_um, double prob, const int32_t *pairs, int64_t n_pairs,

@@ -35,3 +35,20 @@ def compare_states(g, o, pos_rtol=1e-9, label=""):
     worst = float(np.max(d / scale)) if d.size else 0.0
     assert worst <= pos_rtol, f"{label}positions differ: max rel {worst:.3e} > {pos_rtol}"
     return worst
+
+
+def autocatalytic_config(feedback: bool, seed: int = 5, k_cat: float = 8.0):
+    """The plain PIP3/PTEN membrane with an autocatalytic reaction added
+    after PI3K's: PIP3 + PIP2 -> 2 PIP3 within 150 nm (per-pair probability
+    k_cat dt / (pi 0.15^2) = 0.11 at k_cat = 8), optionally under the
+    pip3_activation feedback (its sensor is the enzyme, so density updates
+    are deferred)."""
+    from ed_mrdt import workloads
+    raw = workloads.plain_raw()
+    rx = {"name": "PIP3_autocatalysis", "enzyme": "PIP3", "substrate": "PIP2",
+          "product": "PIP3", "k_cat": float(k_cat), "contact_radius": 150.0}
+    if feedback:
+        rx["feedback_rule"] = "pip3_activation"
+    raw["catalytic_reactions"].insert(1, rx)
+    raw["random_seed"] = seed
+    return workloads.config_from_dict(raw)

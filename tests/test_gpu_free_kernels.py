@@ -231,6 +231,37 @@ def test_catalysis_matches_oracle(oracle_lib, form):
     np.testing.assert_array_equal(sg, so)
 
 
+@pytest.mark.parametrize("form", ["plain", "feedback", "adaptation"])
+def test_autocatalysis_matches_oracle(oracle_lib, form):
+    """enzyme == product: conversions cascade in the order of the given pair
+    list (shuffled here), as the oracle's sequential loop leaves them."""
+    g, pos, sp, alive, pairs, dens = _reaction_inputs(11)
+    pairs = pairs[g.permutation(len(pairs))]
+    u = g.random(len(pairs))
+    fb = (dens, 8, 8, 0.5, 0.5, 20.0, 2.0, False, 0.1)
+    w = g.uniform(0.0, 30.0, (3, 5))
+    out = []
+    for prod in (1, 3):  # 1: autocatalytic; 3: a product that is no enzyme
+        so, sg = sp.copy(), sp.copy()
+        args = (alive, 5000, 1, 0, prod, 0.15, 0.6, pairs, len(pairs), u, 4.0, 4.0, True, True)
+        if form == "plain":
+            no = ok.apply_catalytic_reaction(pos, so, *args)
+            ng = gd.apply_catalytic_reaction(pos, sg, *args)
+        elif form == "feedback":
+            no = ok.apply_catalytic_with_feedback(pos, so, *args, *fb)
+            ng = gd.apply_catalytic_with_feedback(pos, sg, *args, *fb)
+        else:
+            no = ok.apply_catalytic_with_feedback_and_adaptation(pos, so, *args, *fb, w, 10.0,
+                                                                 2.0)
+            ng = gd.apply_catalytic_with_feedback_and_adaptation(pos, sg, *args, *fb, w, 10.0,
+                                                                 2.0)
+        assert ng == no
+        np.testing.assert_array_equal(sg, so)
+        out.append(no)
+    auto, direct = out
+    assert auto > direct > 5, "the autocatalytic run must convert through cascades"
+
+
 @pytest.mark.parametrize("form", ["plain", "feedback", "dual", "local"])
 def test_recruitment_matches_oracle(oracle_lib, form):
     g, pos, sp, alive, pairs, dens = _reaction_inputs(8)

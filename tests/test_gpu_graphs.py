@@ -7,7 +7,9 @@ states, event and photon counters, FLIM frames and the device observables.
 Runs are split so that stretches start from every phase of the buffer
 rotation, cross the longest recording (256 steps), include runs too short to
 record (< 4 steps), end frames between runs, and change a tuning knob and
-open the MSD tracker mid-run (both drop the recordings)."""
+open the MSD tracker mid-run (both drop the recordings).  The
+autocatalytic membrane also orders its conversions by the reference's pair
+index, which step loops then compute on the device every step."""
 
 from __future__ import annotations
 
@@ -22,6 +24,7 @@ from ed_mrdt.microscope import VirtualMicroscope
 from ed_mrdt.observables import MSDTracker
 from ed_mrdt.particles import ParticleState
 from ed_mrdt.photophysics import PhotonSimulator
+from parity_util import autocatalytic_config
 
 pytestmark = pytest.mark.gpu
 
@@ -31,6 +34,7 @@ CASES = {
     "proximity-walled-fp64": lambda: workloads.binding_fret(seed=8, proximity=True,
                                                             reflective=True),
     "wave-13k-fp32": lambda: workloads.wave_system(seed=2, n_steps=2000),
+    "autocatalysis-fp32": lambda: autocatalytic_config(feedback=True, k_cat=2.0),
 }
 PRECISION = {"proximity-walled-fp64": "fp64"}
 

@@ -15,7 +15,7 @@ from ed_mrdt.dynamics import DynamicsEngine
 from ed_mrdt.microscope import VirtualMicroscope
 from ed_mrdt.particles import ParticleState
 from ed_mrdt.photophysics import PhotonSimulator
-from parity_util import compare_states, host_arrays, oracle_arrays
+from parity_util import autocatalytic_config, compare_states, host_arrays, oracle_arrays
 
 pytestmark = pytest.mark.gpu
 
@@ -106,6 +106,31 @@ def test_tuned_membrane_matches_oracle(oracle_lib, cat_side, tile_rows, cat_walk
     assert eng.n_catalytic == oc["n_catalytic"] and oc["n_catalytic"] > 0
     assert (eng.n_recruited, eng.n_detached, eng.n_converted) == (
         oc["n_recruited"], oc["n_detached"], oc["n_converted"])
+
+
+@pytest.mark.parametrize("feedback", [False, True])
+def test_autocatalysis_membrane_fp64(oracle_lib, feedback):
+    """PIP3 + PIP2 -> 2 PIP3 (enzyme == product) in the plain membrane: a
+    PIP2 converted at pair p is an enzyme for the pairs after p, so
+    conversions cascade in the reference's pair order; the oracle runs the
+    reference's sequential loop (FP64, injected draws)."""
+    cfg = autocatalytic_config(feedback)
+    st, eng, ost, odyn = _pair(cfg, exact=False)
+    pip3 = [s.name for s in cfg.species].index("PIP3")
+    n0 = int(np.sum((st.species_id == pip3) & st.is_alive.astype(bool)))
+    for k in range(60):
+        eng.step()
+        odyn.step()
+        if (k + 1) % 20 == 0:
+            compare_states(host_arrays(st), oracle_arrays(ost), label=f"step {k + 1}: ")
+            oc = odyn.counters()
+            assert (eng.n_catalytic, eng.n_recruited, eng.n_detached, eng.n_converted) == (
+                oc["n_catalytic"], oc["n_recruited"], oc["n_detached"], oc["n_converted"])
+    st.mark_stale()
+    n1 = int(np.sum((st.species_id == pip3) & st.is_alive.astype(bool)))
+    print(f"autocatalysis feedback={feedback}: PIP3 {n0} -> {n1}, "
+          f"{eng.n_catalytic} catalytic events")
+    assert n1 > 2 * n0, "the autocatalytic reaction must grow PIP3"
 
 
 @pytest.mark.parametrize("variant", [dict(), dict(intramolecular=False),

@@ -86,6 +86,36 @@ def test_catalytic_enzyme_unchanged_and_either_pair_order():
     assert sp.tolist() == [2, 0, 0]
 
 
+def test_autocatalysis_cascades_in_pair_order():
+    """enzyme == product (E + S -> 2E): the reference's loop visits the pairs
+    in order, so a substrate converted at pair p acts as an enzyme for the
+    pairs after p only.  A row E, S1, S2, S3 at 10 nm spacing (contact 15 nm:
+    only neighbours touch), prob 0.5."""
+    pos = np.array([[0.50, 0.5], [0.51, 0.5], [0.52, 0.5], [0.53, 0.5]])
+
+    def run(pairs, u=None):
+        sp = np.array([1, 0, 0, 0], dtype=np.int32)  # 1: enzyme and product, 0: substrate
+        pr = np.array(pairs, dtype=np.int32)
+        uu = np.zeros(len(pr)) if u is None else np.array(u, dtype=np.float64)
+        n = apply_catalytic_reaction(pos, sp, np.ones(4, bool), 4, 1, 0, 1, 0.015, 0.5, pr,
+                                     len(pr), uu, 1.0, 1.0, True, True)
+        return n, sp.tolist()
+
+    # in chain order every link fires: E -> S1 -> S2 -> S3
+    assert run([[0, 1], [1, 2], [2, 3]]) == (3, [1, 1, 1, 1])
+    # reversed, the later links ran while S1 and S2 were still substrates
+    assert run([[2, 3], [1, 2], [0, 1]]) == (1, [1, 1, 0, 0])
+    # a pair's orientation does not matter
+    assert run([[1, 0], [2, 1], [3, 2]]) == (3, [1, 1, 1, 1])
+    # two substrates alone do nothing; S3's only pair came before S2 converted
+    assert run([[1, 2], [2, 3]]) == (0, [1, 0, 0, 0])
+    assert run([[0, 1], [2, 3], [1, 2]]) == (2, [1, 1, 1, 0])
+    # a rejected link (u >= prob) stops the cascade there
+    assert run([[0, 1], [1, 2], [2, 3]], u=[0.0, 0.5, 0.0]) == (1, [1, 1, 0, 0])
+    # the pair (E, S2) is beyond contact
+    assert run([[0, 2]]) == (0, [1, 0, 0, 0])
+
+
 def test_catalytic_strict_acceptance_and_contact_boundary():
     pos = np.array([[0.0, 0.0], [0.03, 0.04]])  # distance exactly 0.05
     pairs = np.array([[0, 1]], dtype=np.int32)
