@@ -96,15 +96,18 @@ __device__ __forceinline__ double cat_uniform(const Cells<Real>& C, const CatArg
     return (double)Uni<Real>::first(A.rng.draw(step, lo, hi, (kTagCat << 16) | cat_pick(A.rxi, j)));
 }
 
+// Member j's pair of enzyme t (particle index it) and substrate u (index
+// iu), post-move positions given: converts u if the pair is within contact
+// and its uniform accepts.
 template <typename Real>
-__device__ __forceinline__ int cat_try(const Cells<Real>& C, const Post<Real>& P,
-                                       const CatArgs<Real>& A, uint32_t step, int j,
-                                       const Grids<Real>& G, int n_grids, int t, int it, int u,
-                                       Real xnt, Real ynt, Real xnu, Real ynu) {
+__device__ __forceinline__ int cat_try_ids(const Cells<Real>& C, const Post<Real>& P,
+                                           const CatArgs<Real>& A, uint32_t step, int j,
+                                           const Grids<Real>& G, int n_grids, int t, int it,
+                                           int u, int iu, Real xnt, Real ynt, Real xnu,
+                                           Real ynu) {
     Real dx, dy;
     if (min_image(xnt, ynt, xnu, ynu, C.Lx, C.Ly, C.px, C.py, dx, dy) > cat_pick(A.contact2, j))
         return 0;
-    const int iu = C.id[u];
     const bool tlow = it < iu;
     const int sa = tlow ? t : u, sb = tlow ? u : t;
     if (!pair_in_list(C, sa, sb)) return 0;
@@ -116,6 +119,14 @@ __device__ __forceinline__ int cat_try(const Cells<Real>& C, const Post<Real>& P
         return 1;
     }
     return 0;
+}
+
+template <typename Real>
+__device__ __forceinline__ int cat_try(const Cells<Real>& C, const Post<Real>& P,
+                                       const CatArgs<Real>& A, uint32_t step, int j,
+                                       const Grids<Real>& G, int n_grids, int t, int it, int u,
+                                       Real xnt, Real ynt, Real xnu, Real ynu) {
+    return cat_try_ids(C, P, A, step, j, G, n_grids, t, it, u, (int)C.id[u], xnt, ynt, xnu, ynu);
 }
 
 // ----------------------------------------------------------------------
@@ -532,107 +543,215 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFp32Waves<
     cat_strip<Real, 1>(C, P, sA, sA.step, ctr, G, n_grids, strip_geo(C, tile, ntile_y));
 }
 
-// Catalysis over a work list of its walk entities: an alive enzyme against
-// the substrates of its 3x3 cells, or (by substrate) an alive substrate
-// against the enzymes of its 3x3 cells, read from global memory.  A group
-// of 2^group_shift lanes takes one listed entity and splits the candidates
-// of its nine cells (their slot ranges concatenated) between its lanes, so
-// a walk costs a couple of dependent loads per lane instead of one per
-// candidate.  Used when no enzyme species is the run's substrate or
-// product, so that no entity changes identity during the run: species read
-// as the run goes then agree with the run's start.
+// Position of the k-th set bit of m (k < popcount(m)).
+__device__ __forceinline__ int nth_set_bit(uint64_t m, int k) {
+    int base = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+        const uint64_t low = m & ((1ull << w) - 1ull);
+        const int c = __popcll(low);
+        if (k >= c) {
+            k -= c;
+            m >>= w;
+            base += w;
+        } else {
+            m = low;
+        }
+    }
+    return base;
+}
+
+// LDS of one wavefront of the flat catalysis walk: the batch's entities,
+// their runs of cells, and the queue of candidate pairs within the cutoff.
 template <typename Real>
-__device__ __forceinline__ int cat_walk_group(const Cells<Real>& C, const Post<Real>& P,
-                                              const CatArgs<Real>& A, const Grids<Real>& G,
-                                              int n_grids, int t, int lane, int gsize) {
-    if (!(C.fl[t] & EDM_FLAG_ALIVE)) return 0;
-    const int st = P.sp[t];
-    const bool by_sub = A.by_substrate;
-    if (by_sub ? st != A.sub : true) {
-        bool enz = false;
-#pragma unroll
-        for (int j = 0; j < kCatFuse; ++j) enz |= j < A.n_enz && st == A.enz[j];
-        if (by_sub || !enz) return 0;
-    }
-    const Real xt = C.x[t], yt = C.y[t];
-    const int it = C.id[t];
-    int cx, cy;
-    cell_of(C, xt, yt, cx, cy);
-    int beg[9], len[9];
-    int total = 0;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        const int c = nb_cell(C, cx, cy, k);
-        beg[k] = c < 0 ? 0 : (int)C.start[c];
-        len[k] = c < 0 ? 0 : (int)C.start[c + 1] - beg[k];
-        total += len[k];
-    }
+struct CatFlatLds {
+    int inc[64];            // inclusive candidate counts of the batch's entities
+    int slot[64];           // the entities' slots
+    Real x[64], y[64];      // their pre-move positions
+    int pb[64 * 9];         // run p of entity e: slots [pb, pb + pe - pe_prev)
+    int pe[64 * 9];         // cumulative run ends
+    int qa[64], qb[64];     // queued (walk entity, candidate) slots within the cutoff
+};
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Drains the queue: lane l < n evaluates queued pair l.
+template <typename Real>
+__device__ __forceinline__ int cat_flat_drain(const Cells<Real>& C, const Post<Real>& P,
+                                              const CatArgs<Real>& A, uint32_t step, bool by_sub,
+                                              const Grids<Real>& G, int n_grids,
+                                              const CatFlatLds<Real>& L, int n) {
+    const int lane = lane_id();
+    if (lane >= n) return 0;
+    // enzyme e, substrate s
+    const int e = by_sub ? L.qb[lane] : L.qa[lane];
+    const int s = by_sub ? L.qa[lane] : L.qb[lane];
+    const int se = P.sp[e], ie = C.id[e], is = C.id[s];
+    const Real xne = P.xn[e], yne = P.yn[e], xns = P.xn[s], yns = P.yn[s];
     int hits = 0;
-    for (int q = lane; q < total; q += gsize) {
-        // candidate q of the concatenated ranges (unrolled: no indexed
-        // private arrays)
-        int u = -1, r = q;
+    for (int j = 0; j < A.n_enz; ++j)
+        if (se == cat_pick(A.enz, j))
+            hits += cat_try_ids(C, P, A, step, j, G, n_grids, e, ie, s, is, xne, yne, xns, yns);
+    return hits;
+}
+
+// Flat walk of one wavefront.  Lane l holds walk entity t (an alive enzyme
+// of the run, or with by_sub an alive substrate) when bit l of `mask` is
+// set.  The entities are taken `per` at a time; each entity's nine cells
+// are merged into runs of consecutive cells (contiguous slot ranges), and
+// the batch's candidates -- every slot of every entity's runs -- are
+// spread over the wavefront's lanes (owner by binary search over the
+// inclusive counts), so lanes do no redundant work whatever the cell
+// populations.  Candidates of the right species, alive and within the
+// pair-list cutoff (pre-move) are queued in LDS and evaluated 64 at a time
+// by the full wavefront.  The candidates of an entity are the slots of its
+// nine cells, as in every other catalysis walk, so the outcome is the same.
+template <typename Real>
+__device__ __forceinline__ int cat_flat_wave(const Cells<Real>& C, const Post<Real>& P,
+                                             const CatArgs<Real>& A, uint32_t step, bool by_sub,
+                                             const Grids<Real>& G, int n_grids, int t_lane,
+                                             uint64_t mask, int per, CatFlatLds<Real>& L) {
+    const int lane = lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;
+    const int n_ent = __popcll(mask);
+    int hits = 0;
+    int qn = 0;  // queued pairs (uniform)
+    for (int b0 = 0; b0 < n_ent; b0 += per) {  // uniform
+        const int ne = min(per, n_ent - b0);
+        const bool has = lane < ne;
+        const int t = __shfl(t_lane, has ? nth_set_bit(mask, b0 + lane) : 0);
+        int tot = 0;
+        if (has) {
+            const Real xt = C.x[t], yt = C.y[t];
+            L.slot[lane] = t;
+            L.x[lane] = xt;
+            L.y[lane] = yt;
+            int cx, cy;
+            cell_of(C, xt, yt, cx, cy);
+            int np = 0, pbeg = 0, plen = 0, prevc = -2;
 #pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            if (u < 0) {
-                if (r < len[k]) u = beg[k] + r;
-                else r -= len[k];
+            for (int k = 0; k < 9; ++k) {
+                const int c = nb_cell(C, cx, cy, k);
+                if (c < 0) continue;
+                const int b = (int)C.start[c], l = (int)C.start[c + 1] - b;
+                if (c != prevc + 1 && plen > 0) {
+                    L.pb[lane * 9 + np] = pbeg;
+                    L.pe[lane * 9 + np] = tot;
+                    ++np;
+                    plen = 0;
+                }
+                if (plen == 0) pbeg = b;
+                plen += l;
+                tot += l;
+                prevc = c;
+            }
+            if (plen > 0) {
+                L.pb[lane * 9 + np] = pbeg;
+                L.pe[lane * 9 + np] = tot;
             }
         }
-        if (u == t || !(C.fl[u] & EDM_FLAG_ALIVE)) continue;
-        const int su = P.sp[u];
-        Real dx, dy;
-        if (by_sub) {  // u a candidate enzyme of the substrate t
-            bool enz = false;
+        // inclusive scan of the candidate counts
+        int inc = tot;
 #pragma unroll
-            for (int j = 0; j < kCatFuse; ++j) enz |= j < A.n_enz && su == A.enz[j];
-            if (!enz) continue;
-            if (min_image(C.x[u], C.y[u], xt, yt, C.Lx, C.Ly, C.px, C.py, dx, dy) >= C.cut2)
-                continue;
-            const int iu = C.id[u];
-            const Real xnu = P.xn[u], ynu = P.yn[u], xnt = P.xn[t], ynt = P.yn[t];
-            for (int j = 0; j < A.n_enz; ++j)
-                if (su == cat_pick(A.enz, j))
-                    hits += cat_try(C, P, A, A.step, j, G, n_grids, u, iu, t, xnu, ynu, xnt, ynt);
-        } else {  // u a candidate substrate of the enzyme t
-            if (su != A.sub) continue;
-            if (min_image(xt, yt, C.x[u], C.y[u], C.Lx, C.Ly, C.px, C.py, dx, dy) >= C.cut2)
-                continue;
-            const Real xnu = P.xn[u], ynu = P.yn[u], xnt = P.xn[t], ynt = P.yn[t];
-            for (int j = 0; j < A.n_enz; ++j)
-                if (st == cat_pick(A.enz, j))
-                    hits += cat_try(C, P, A, A.step, j, G, n_grids, t, it, u, xnt, ynt, xnu, ynu);
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(inc, o);
+            if (lane >= o) inc += v;
         }
+        L.inc[lane] = inc;
+        const int T = __shfl(inc, 63);
+        wave_lds_sync();
+        for (int c0 = 0; c0 < T; c0 += 64) {  // uniform
+            const int c = c0 + lane;
+            bool pass = false;
+            int te = 0, u = 0;
+            if (c < T) {
+                // owner: the first entity whose inclusive count exceeds c
+                int o = 0;
+#pragma unroll
+                for (int w = 32; w >= 1; w >>= 1)
+                    if (o + w - 1 < ne && L.inc[o + w - 1] <= c) o += w;
+                const int r = c - (o > 0 ? L.inc[o - 1] : 0);
+                int k = 0;
+                while (L.pe[o * 9 + k] <= r) ++k;
+                u = L.pb[o * 9 + k] + r - (k > 0 ? L.pe[o * 9 + k - 1] : 0);
+                te = L.slot[o];
+                const Real xt = L.x[o], yt = L.y[o];
+                const uint8_t fu = C.fl[u];
+                const int su = P.sp[u];
+                const Real xu = C.x[u], yu = C.y[u];
+                if (u != te && (fu & EDM_FLAG_ALIVE)) {
+                    bool sp_ok;
+                    if (by_sub) {
+                        sp_ok = false;
+#pragma unroll
+                        for (int j = 0; j < kCatFuse; ++j) sp_ok |= j < A.n_enz && su == A.enz[j];
+                    } else {
+                        sp_ok = su == A.sub;
+                    }
+                    Real dx, dy;
+                    pass = sp_ok && min_image(xt, yt, xu, yu, C.Lx, C.Ly, C.px, C.py, dx, dy) <
+                                        C.cut2;
+                }
+            }
+            const uint64_t m = __ballot(pass);
+            const int np = __popcll(m);
+            if (qn + np > 64) {  // uniform
+                wave_lds_sync();
+                hits += cat_flat_drain(C, P, A, step, by_sub, G, n_grids, L, qn);
+                qn = 0;
+                wave_lds_sync();
+            }
+            if (pass) {
+                const int pos = qn + __popcll(m & below);
+                L.qa[pos] = te;
+                L.qb[pos] = u;
+            }
+            qn += np;
+        }
+        // the next batch rewrites the entity tables
+        wave_lds_sync();
+    }
+    if (qn > 0) {
+        wave_lds_sync();
+        hits += cat_flat_drain(C, P, A, step, by_sub, G, n_grids, L, qn);
     }
     return hits;
 }
 
-// Grid (list regions, kWlPer): block (b, y) takes the entities y * per,
-// y * per + kWlPer * per, ... of region b, per = 256 >> group_shift of them
-// at a time.
+// Grid (list regions, kWlPer): each wavefront takes per = 64 >> group_shift
+// consecutive entries of its region at a time (chunks y * 4 + wave,
+// stepping by 4 * kWlPer) and walks them with cat_flat_wave.
 template <typename Real>
 __global__ __launch_bounds__(256) void k_cat_list(Cells<Real> C, Post<Real> P, CatArgs<Real> A,
                                                   WlView V, unsigned long long* ctr,
                                                   Grids<Real> G, int n_grids) {
     const int cnt = (int)V.cnt[blockIdx.x];
-    const int gs = A.group_shift;
-    const int per = (int)blockDim.x >> gs;
-    const int first = (int)blockIdx.y * per;
-    if (first >= cnt) return;  // uniform: nothing listed for this block
+    const int per = 64 >> A.group_shift;
+    if ((int)blockIdx.y * 4 * per >= cnt) return;  // uniform: nothing listed for this block
     // the parameter block is indexed with per-lane values: keep it in LDS
     // rather than in per-lane scratch
     __shared__ CatArgs<Real> sA;
+    __shared__ CatFlatLds<Real> sL[4];
     if (threadIdx.x == 0) {
         sA = A;
         if (A.step_base) sA.step += *A.step_base;
     }
     __syncthreads();
-    const int grp = (int)threadIdx.x >> gs, lane = (int)threadIdx.x & ((1 << gs) - 1);
+    const int lane = lane_id();
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     int hits = 0;
-    for (int e = first + grp; e < cnt; e += (int)gridDim.y * per)
-        hits += cat_walk_group(C, P, sA, G, n_grids, V.list[(size_t)blockIdx.x * kWlSlots + e],
-                               lane, 1 << gs);
+    for (int ch = (int)blockIdx.y * 4 + wave; ch * per < cnt; ch += 4 * (int)gridDim.y) {
+        const int e = ch * per + lane;
+        const bool has = lane < per && e < cnt;
+        const int t = has ? V.list[(size_t)blockIdx.x * kWlSlots + e] : 0;
+        hits += cat_flat_wave(C, P, sA, sA.step, sA.by_substrate, G, n_grids, t, __ballot(has),
+                              64, sL[wave]);
+    }
     unsigned long long v = hits;
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
-    if (lane_id() == 0 && v) atomicAdd(&ctr[2], v);
+    if (lane == 0 && v) atomicAdd(&ctr[2], v);
 }

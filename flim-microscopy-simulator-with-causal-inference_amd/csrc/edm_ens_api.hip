@@ -454,6 +454,11 @@ int edm_ens_set_tuning(edm_ensemble* e, int32_t key, int64_t value) {
                                    "in 1..64");
             e->cat_lanes = (int)value;
             break;
+        case EDM_ENS_TUNE_CAT_WALK:
+            if (value != 0 && value != 1)
+                return ens_invalid("edm_ens_set_tuning: catalysis walk must be 0 or 1");
+            e->cat_walk = (int)value;
+            break;
         default:
             return ens_invalid("edm_ens_set_tuning: unknown key");
     }
@@ -468,6 +473,7 @@ int edm_ens_get_tuning(edm_ensemble* e, int32_t key, int64_t* value) {
         case EDM_ENS_TUNE_STRIP_PARTICLES: *value = e->strip_parts; return 0;
         case EDM_ENS_TUNE_PAIR_BELOW: *value = e->pair_below; return 0;
         case EDM_ENS_TUNE_CAT_LANES: *value = e->cat_lanes; return 0;
+        case EDM_ENS_TUNE_CAT_WALK: *value = e->cat_walk; return 0;
         default: return ens_invalid("edm_ens_get_tuning: unknown key");
     }
 }

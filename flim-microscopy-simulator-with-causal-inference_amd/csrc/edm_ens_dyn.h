@@ -222,6 +222,54 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFp32Waves<
               ens_strip<Real>(g, T));
 }
 
+// Catalysis run walked from its enzymes over the pass work list (every
+// particle of the pass species, alive or not, in slot order; each wave is
+// one replica's).  A wave keeps its alive particles of the run's enzyme
+// species and walks them 64 >> group_shift at a time with cat_flat_wave,
+// as the single-membrane list walk does.
+// Every pair is decided by its own uniform and a substrate converts once
+// (compare-and-swap of its species byte), so the outcome is the strip
+// kernel's.  Used for runs whose enzyme species are all pass species and
+// none of them is the run's substrate or product.
+template <typename Real>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFp32Waves<Real, 7>))) void k_ens_cat_list(int n_waves, const int32_t* __restrict__ list,
+                                                      const int32_t* __restrict__ wrep,
+                                                      const uint32_t* __restrict__ rcnt,
+                                                      const EnsSeg* __restrict__ segs,
+                                                      EnsPool<Real> Q,
+                                                      const CatArgs<Real>* __restrict__ cats,
+                                                      int n_runs, int ci,
+                                                      unsigned long long* ctr, uint32_t gstep) {
+    if ((int)blockIdx.x * (int)(blockDim.x >> 6) +
+            __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) >= n_waves)
+        return;  // whole waves
+    const EnsWave W = ens_wave(wrep, segs, 0);
+    const EnsSeg& g = segs[W.r];
+    const CatArgs<Real>& A = cats[(size_t)W.r * n_runs + ci];
+    const int lane = lane_id();
+    const int j = W.j + lane;
+    int t = 0;
+    bool ent = false;
+    if (j < min(g.wl_n[0], (int)rcnt[2 * W.r])) {
+        t = list[g.wl_off[0] + j];
+        if (Q.fl[t] & EDM_FLAG_ALIVE) {
+            const int s = Q.sp[t];
+#pragma unroll
+            for (int k = 0; k < kCatFuse; ++k) ent |= k < A.n_enz && s == A.enz[k];
+        }
+    }
+    const uint64_t mask = __ballot(ent);
+    if (!mask) return;  // uniform: no enzyme among the wave's entries
+    __shared__ CatFlatLds<Real> sL[4];
+    const Cells<Real> C = ens_cells<Real>(g, Q.x, Q.y, Q.sp, Q.fl, Q.id, Q.start);
+    const Grids<Real> G = ens_grids<Real>(g, Q.counts, Q.delta, Q.ns);
+    const int hits = cat_flat_wave(C, ens_post(Q), A, gstep - g.step0, false, G, Q.n_grids, t,
+                                   mask, 64 >> A.group_shift, sL[threadIdx.x >> 6]);
+    unsigned long long v = hits;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+    if (lane == 0 && v) atomicAdd(&ctr[8 * W.r + 2], v);
+}
+
 // A run of per-particle passes over the strip's own particles, with the
 // run's parameter block of the strip's replica.
 template <typename Real, bool kLocal>
@@ -473,6 +521,7 @@ static int ens_create_t(edm_ensemble* e) {
     // catalysis runs: back-to-back reactions with one substrate and product and
     // no feedback share a launch (as for one membrane)
     e->cat_runs.clear();
+    e->cat_list_ok.clear();
     e->cat_run_of.assign(e->rx.size(), -1);
     for (size_t ri = 0; ri < e->rx.size();) {
         const edm_reaction& r = e->rx[ri];
@@ -489,8 +538,16 @@ static int ens_create_t(edm_ensemble* e) {
                 break;
             ++nf;
         }
+        // the list walk needs every enzyme in the pass work list and no
+        // enzyme that the run can turn into another species
+        bool listable = true;
+        for (int j = 0; j < nf; ++j) {
+            const int a = e->rx[ri + j].a;
+            listable = listable && ((e->pass_species >> a) & 1u) && a != r.b && a != r.c;
+        }
         e->cat_run_of[ri] = (int)e->cat_runs.size();
         e->cat_runs.emplace_back((int)ri, nf);
+        e->cat_list_ok.push_back(listable ? 1 : 0);
         ri += nf;
     }
     e->seg.assign(e->rmax, EnsSeg{});
@@ -736,10 +793,17 @@ static int ens_step_t(edm_ensemble* e) {
             const int ci = e->cat_run_of[ri];
             if (ci < 0) continue;  // member of a run launched before
             const int start = e->cat_runs[ci].first;
-            hipLaunchKernelGGL((k_ens_cat<Real>), tgrid, dim3(256), 0, st,
-                               (const EnsTile*)e->d_tiles, ntiles, (const EnsSeg*)e->d_seg, Q,
-                               (const CatArgs<Real>*)e->d_cats, (int)e->cat_runs.size(), ci,
-                               e->ctr, gstep);
+            if (e->cat_walk == 1 && e->cat_list_ok[ci] && e->wl_npass > 0)
+                hipLaunchKernelGGL((k_ens_cat_list<Real>), dim3(nblk(e->wl_npass)), dim3(256), 0,
+                                   st, (int)(e->wl_npass / 64), (const int32_t*)e->wl_pass,
+                                   (const int32_t*)e->wl_wrep_pass, (const uint32_t*)e->wl_rcnt,
+                                   (const EnsSeg*)e->d_seg, Q, (const CatArgs<Real>*)e->d_cats,
+                                   (int)e->cat_runs.size(), ci, e->ctr, gstep);
+            else
+                hipLaunchKernelGGL((k_ens_cat<Real>), tgrid, dim3(256), 0, st,
+                                   (const EnsTile*)e->d_tiles, ntiles, (const EnsSeg*)e->d_seg, Q,
+                                   (const CatArgs<Real>*)e->d_cats, (int)e->cat_runs.size(), ci,
+                                   e->ctr, gstep);
             const edm_reaction& r0 = e->rx[start];
             const bool defer =
                 r0.fb1.sensor >= 0 && (r0.fb1.sensor == r0.b || r0.fb1.sensor == r0.c);

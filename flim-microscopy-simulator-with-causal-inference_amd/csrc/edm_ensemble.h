@@ -126,6 +126,7 @@ struct edm_ensemble {
     int strip_parts = 192;
     int pair_below = 128;
     int cat_lanes = 16;
+    int cat_walk = 0;  // 1: catalysis runs walk their enzymes over the pass work list
     EnsTile* d_tiles = nullptr;
     int64_t tile_cap = 0;
     int64_t n_total = 0;  // particles in the pool (live replicas first after a sort)
@@ -144,6 +145,7 @@ struct edm_ensemble {
     std::vector<std::vector<int>> pass_groups;
     std::vector<int> group_start, pass_deferred;
     std::vector<std::pair<int, int>> cat_runs;  // (first reaction, reactions) per launch
+    std::vector<char> cat_list_ok;              // per run: the list walk may take it
     std::vector<int> cat_run_of;                // per reaction: run it starts, else -1
     // per replica slot and run / group: the kernels' parameter blocks with the
     // replica's probabilities, feedback grids and seed (rebuilt by ens_layout)

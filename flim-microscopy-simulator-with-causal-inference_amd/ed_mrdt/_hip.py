@@ -131,6 +131,7 @@ TUNE_FORCE_LISTS = 9
 ENS_TUNE_STRIP_PARTICLES = 1
 ENS_TUNE_PAIR_BELOW = 2
 ENS_TUNE_CAT_LANES = 3
+ENS_TUNE_CAT_WALK = 4
 
 # function name -> (argtypes) ; every function returns int32 status
 _SIGS = {

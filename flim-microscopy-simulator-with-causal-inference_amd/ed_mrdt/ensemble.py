@@ -273,16 +273,19 @@ class EnsembleEngine:
 
     # ---- tuning (no reference counterpart) ------------------------------------
     _TUNE = (("strip_particles", _hip.ENS_TUNE_STRIP_PARTICLES),
-             ("pair_below", _hip.ENS_TUNE_PAIR_BELOW), ("cat_lanes", _hip.ENS_TUNE_CAT_LANES))
+             ("pair_below", _hip.ENS_TUNE_PAIR_BELOW), ("cat_lanes", _hip.ENS_TUNE_CAT_LANES),
+             ("cat_walk", _hip.ENS_TUNE_CAT_WALK))
 
     def set_tuning(self, strip_particles: Optional[int] = None, pair_below: Optional[int] = None,
-                   cat_lanes: Optional[int] = None) -> None:
+                   cat_lanes: Optional[int] = None, cat_walk: Optional[int] = None) -> None:
         """Work decomposition of the batched step, leaving every result
         unchanged: own particles a strip aims at (32..512), the column
-        population below which two columns share a strip (0..1024, 0 never)
-        and the lanes per catalysis entity (a power of two in 1..64).
-        Applies from the next step."""
-        vals = dict(strip_particles=strip_particles, pair_below=pair_below, cat_lanes=cat_lanes)
+        population below which two columns share a strip (0..1024, 0 never),
+        the lanes per catalysis entity (a power of two in 1..64) and whether
+        catalysis walks strips of cells (0) or its enzymes from the pass
+        work list (1).  Applies from the next step."""
+        vals = dict(strip_particles=strip_particles, pair_below=pair_below, cat_lanes=cat_lanes,
+                    cat_walk=cat_walk)
         for name, key in self._TUNE:
             if vals[name] is not None:
                 check(lib().edm_ens_set_tuning(self._h, key, int(vals[name])),

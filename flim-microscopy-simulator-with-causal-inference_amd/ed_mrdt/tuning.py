@@ -42,8 +42,8 @@ def autotune(config: SimulationConfig, precision: str = "fp32", steps: int = 20,
              cat_walks: Iterable[int] = CAT_WALKS,
              cat_lanes: Iterable[int] = CAT_LANES,
              force_lists: Iterable[int] = FORCE_LISTS) -> Dict:
-    """Fastest force_lists, cat_walk, cat_side, cat_lanes and tile_rows for
-    this configuration (tuned one after another in that order, each with
+    """Fastest force_lists, (cat_walk, cat_lanes), cat_side and tile_rows
+    for this configuration (tuned one after another in that order, each with
     the best of the earlier ones); `times_ms` holds the measured
     milliseconds per step of every candidate."""
     from .dynamics import DynamicsEngine
@@ -69,24 +69,21 @@ def autotune(config: SimulationConfig, precision: str = "fp32", steps: int = 20,
             times[f"force_lists={v}"] = per_step()
         best_lists = min(lists, key=lambda v: times[f"force_lists={v}"])
         eng.set_tuning(force_lists=best_lists)
-        walks = list(cat_walks)
+        # the lanes mean different things to the two walks: tuned together
+        walks, lanes = list(cat_walks), list(cat_lanes)
         for w in walks:
-            eng.set_tuning(cat_walk=w)
-            times[f"cat_walk={w}"] = per_step()
-        best_walk = min(walks, key=lambda w: times[f"cat_walk={w}"])
-        eng.set_tuning(cat_walk=best_walk)
+            for g in lanes:
+                eng.set_tuning(cat_walk=w, cat_lanes=g)
+                times[f"cat_walk={w},cat_lanes={g}"] = per_step()
+        best_walk, best_lanes = min(((w, g) for w in walks for g in lanes),
+                                    key=lambda wg: times[f"cat_walk={wg[0]},cat_lanes={wg[1]}"])
+        eng.set_tuning(cat_walk=best_walk, cat_lanes=best_lanes)
         sides = list(cat_sides)
         for s in sides:
             eng.set_tuning(cat_side=s)
             times[f"cat_side={s}"] = per_step()
         best_side = min(sides, key=lambda s: times[f"cat_side={s}"])
         eng.set_tuning(cat_side=best_side)
-        lanes = list(cat_lanes)
-        for g in lanes:
-            eng.set_tuning(cat_lanes=g)
-            times[f"cat_lanes={g}"] = per_step()
-        best_lanes = min(lanes, key=lambda g: times[f"cat_lanes={g}"])
-        eng.set_tuning(cat_lanes=best_lanes)
         rows = list(tile_rows)
         for r in rows:
             eng.set_tuning(tile_rows=r)
@@ -101,16 +98,18 @@ def autotune(config: SimulationConfig, precision: str = "fp32", steps: int = 20,
 
 ENS_STRIP_PARTICLES = (96, 128, 192, 256, 384)
 ENS_PAIR_BELOW = (0, 64, 128, 256)
-ENS_CAT_LANES = (64, 16, 4)
+ENS_CAT_LANES = (64, 16, 4, 1)
+ENS_CAT_WALK = (0, 1)
 
 
 def autotune_ensemble(engine=None, configs=None, steps: int = 10, warmup: int = 2,
                       strip_particles: Iterable[int] = ENS_STRIP_PARTICLES,
                       pair_below: Iterable[int] = ENS_PAIR_BELOW,
-                      cat_lanes: Iterable[int] = ENS_CAT_LANES) -> Dict:
-    """Fastest cat_lanes, pair_below and strip_particles of the batched step
-    (tuned one after another in that order, each with the best of the earlier
-    ones).  With `engine` the candidates are timed on it (its replicas
+                      cat_lanes: Iterable[int] = ENS_CAT_LANES,
+                      cat_walk: Iterable[int] = ENS_CAT_WALK) -> Dict:
+    """Fastest (cat_walk, cat_lanes), pair_below and strip_particles of the
+    batched step (tuned one after another in that order, each with the best
+    of the earlier ones).  With `engine` the candidates are timed on it (its replicas
     advance by the timed steps, with results independent of the settings)
     and the best settings are left applied; with `configs` a scratch
     ensemble of those replicas is built and closed.  `times_ms` holds the
@@ -139,7 +138,17 @@ def autotune_ensemble(engine=None, configs=None, steps: int = 10, warmup: int = 
     times: Dict[str, float] = {}
     try:
         best = {}
-        for name, cands in (("cat_lanes", list(cat_lanes)), ("pair_below", list(pair_below)),
+        # the lanes mean different things to the two walks: tuned together
+        walks, lanes = list(cat_walk), list(cat_lanes)
+        for w in walks:
+            for g in lanes:
+                engine.set_tuning(cat_walk=w, cat_lanes=g)
+                times[f"cat_walk={w},cat_lanes={g}"] = per_step()
+        best["cat_walk"], best["cat_lanes"] = min(
+            ((w, g) for w in walks for g in lanes),
+            key=lambda wg: times[f"cat_walk={wg[0]},cat_lanes={wg[1]}"])
+        engine.set_tuning(cat_walk=best["cat_walk"], cat_lanes=best["cat_lanes"])
+        for name, cands in (("pair_below", list(pair_below)),
                             ("strip_particles", list(strip_particles))):
             for v in cands:
                 engine.set_tuning(**{name: v})

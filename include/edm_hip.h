@@ -199,8 +199,11 @@ int edm_photon_counters(edm_membrane *m, int64_t out[2]);
  *   EDM_TUNE_CAT_WALK   0 (default): catalysis walks strips of cells staged
  *                       in LDS; 1: runs walk the step's list of their
  *                       enzymes or substrates where they can
- *   EDM_TUNE_CAT_LANES  lanes that walk one entity's candidates in the strip
- *                       walk: a power of two in 1..64 (default 16)
+ *   EDM_TUNE_CAT_LANES  a power of two L in 1..64 (default 16): lanes that
+ *                       walk one entity's candidates in the strip walk; in
+ *                       the list walk each wavefront takes 64 / L entities
+ *                       at a time and spreads their candidates over its
+ *                       lanes
  *   EDM_TUNE_FRET_SEARCH  proximity FRET finds targets through a grid of the
  *                       alive targets (0, default) or by scanning every
  *                       particle for every donor (1), as the reference does
@@ -602,11 +605,18 @@ int edm_ens_info(edm_ensemble *e, int64_t out[4]);
  *   EDM_ENS_TUNE_PAIR_BELOW       columns holding fewer particles are
  *                                 walked two at a time, 0..1024 (default
  *                                 128; 0: never)
- *   EDM_ENS_TUNE_CAT_LANES        lanes per catalysis entity, a power of
- *                                 two in 1..64 (default 16)                */
+ *   EDM_ENS_TUNE_CAT_LANES        a power of two L in 1..64 (default 16):
+ *                                 lanes per catalysis entity of the strip
+ *                                 walk; the list walk takes 64 / L enzymes
+ *                                 of a wavefront at a time
+ *   EDM_ENS_TUNE_CAT_WALK         0 (default): catalysis walks strips of
+ *                                 cells; 1: runs whose enzymes are pass
+ *                                 species walk each alive enzyme's 3x3
+ *                                 cells from the pass work list            */
 #define EDM_ENS_TUNE_STRIP_PARTICLES 1
 #define EDM_ENS_TUNE_PAIR_BELOW 2
 #define EDM_ENS_TUNE_CAT_LANES 3
+#define EDM_ENS_TUNE_CAT_WALK 4
 int edm_ens_set_tuning(edm_ensemble *e, int32_t key, int64_t value);
 int edm_ens_get_tuning(edm_ensemble *e, int32_t key, int64_t *value);
 int edm_ens_synchronize(edm_ensemble *e);

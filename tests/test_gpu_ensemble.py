@@ -178,14 +178,14 @@ def _tuned_run(tune_schedule, n_rep=12, steps=(60, 40, 50)):
 
 
 def test_ensemble_tuning_leaves_results_unchanged():
-    """Strip sizes, column pairing and catalysis lanes (set before the run
-    and changed between frames) give the batched step's results bit for
-    bit: states, counters and every frame of 12 replicas of 0.9k-31k
-    particles."""
+    """Strip sizes, column pairing, catalysis lanes and the catalysis walk
+    (strips or enzymes from the pass work list; set before the run and
+    changed between frames) give the batched step's results bit for bit:
+    states, counters and every frame of 12 replicas of 0.9k-31k particles."""
     (ref, _) = _tuned_run([None, None, None])
-    sched = [dict(strip_particles=96, pair_below=0, cat_lanes=4),
-             dict(strip_particles=384, pair_below=256, cat_lanes=64),
-             dict(strip_particles=128, pair_below=1024, cat_lanes=1)]
+    sched = [dict(strip_particles=96, pair_below=0, cat_lanes=4, cat_walk=1),
+             dict(strip_particles=384, pair_below=256, cat_lanes=64, cat_walk=0),
+             dict(strip_particles=128, pair_below=1024, cat_lanes=1, cat_walk=1)]
     (got, tuned) = _tuned_run(sched)
     assert tuned == sched[-1]
     for i, (a, b) in enumerate(zip(ref[0], got[0])):
@@ -206,11 +206,14 @@ def test_ensemble_autotuner_picks_from_timed_candidates():
         ens.add(c, s)
     res = tuning.autotune_ensemble(ens, steps=5, warmup=1)
     t = res["times_ms"]
-    for name, cands in (("cat_lanes", tuning.ENS_CAT_LANES),
-                        ("pair_below", tuning.ENS_PAIR_BELOW),
+    walk = min(((w, g) for w in tuning.ENS_CAT_WALK for g in tuning.ENS_CAT_LANES),
+               key=lambda wg: t[f"cat_walk={wg[0]},cat_lanes={wg[1]}"])
+    assert (res["cat_walk"], res["cat_lanes"]) == walk
+    for name, cands in (("pair_below", tuning.ENS_PAIR_BELOW),
                         ("strip_particles", tuning.ENS_STRIP_PARTICLES)):
         assert res[name] == min(cands, key=lambda v: t[f"{name}={v}"])
-    assert ens.tuning() == {k: res[k] for k in ("strip_particles", "pair_below", "cat_lanes")}
-    print("[ens-tune]", {k: res[k] for k in ("strip_particles", "pair_below", "cat_lanes")},
+    keys = ("strip_particles", "pair_below", "cat_lanes", "cat_walk")
+    assert ens.tuning() == {k: res[k] for k in keys}
+    print("[ens-tune]", {k: res[k] for k in keys},
           {k: round(v, 3) for k, v in t.items()}, flush=True)
     ens.close()

@@ -87,7 +87,7 @@ def test_plain_membrane_dynamics_fp64(oracle_lib, n_steps, exact):
 
 @pytest.mark.parametrize("cat_side,tile_rows,cat_walk,cat_lanes",
                          [(0, 64, 0, 64), (1, 8, 0, 4), (0, 64, 1, 64), (1, 64, 1, 64),
-                          (0, 16, 0, 1)])
+                          (0, 64, 1, 4), (1, 64, 1, 1), (0, 16, 0, 1)])
 def test_tuned_membrane_matches_oracle(oracle_lib, cat_side, tile_rows, cat_walk, cat_lanes):
     """Catalysis walked from either side, over LDS strips (short ones too)
     or work lists: still the oracle's events and positions (FP64, injected

@@ -33,6 +33,9 @@ def test_tuning_leaves_results_unchanged():
     for tune in (dict(cat_side=1, tile_rows=64, cat_walk=0), dict(cat_side=1, tile_rows=16),
                  dict(cat_side=0, tile_rows=5, cat_walk=0), dict(cat_side=-1, tile_rows=32),
                  dict(cat_side=0, cat_walk=1), dict(cat_side=1, cat_walk=1),
+                 dict(cat_side=0, cat_walk=1, cat_lanes=1),
+                 dict(cat_side=1, cat_walk=1, cat_lanes=64),
+                 dict(cat_side=-1, cat_walk=1, cat_lanes=4),
                  dict(cat_side=0, cat_lanes=1), dict(cat_side=1, cat_lanes=8, tile_rows=16),
                  dict(cat_side=0, force_lists=0), dict(cat_side=1, force_lists=0, tile_rows=16)):
         got = _run(cfg, 60, **tune)
@@ -47,6 +50,6 @@ def test_autotune_picks_a_candidate():
     assert res["tile_rows"] in TILE_ROWS and res["cat_side"] in CAT_SIDES
     assert res["cat_walk"] in CAT_WALKS and res["cat_lanes"] in CAT_LANES
     assert res["force_lists"] in FORCE_LISTS
-    assert len(res["times_ms"]) == (len(TILE_ROWS) + len(CAT_SIDES) + len(CAT_WALKS) +
-                                    len(CAT_LANES) + len(FORCE_LISTS))
+    assert len(res["times_ms"]) == (len(TILE_ROWS) + len(CAT_SIDES) +
+                                    len(CAT_WALKS) * len(CAT_LANES) + len(FORCE_LISTS))
     assert all(t > 0 for t in res["times_ms"].values())
