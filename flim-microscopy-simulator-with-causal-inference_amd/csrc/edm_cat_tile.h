@@ -42,7 +42,34 @@ struct CatArgs {
     bool deferred;
     bool by_substrate;  // walk the pairs from the substrates' side (the rarer one)
     int group_shift;    // log2 of the lanes that walk one entity's candidates (0..6)
+    int32_t* app;       // a later run's work list: converted substrates are appended
+    uint32_t* app_cnt;  // (region u / kWlSlots of slot u; nullptr: none)
+    int32_t *fold_counts, *fold_delta;  // deferred count updates of the previous run,
+    int fold_n;                         // folded by this launch (fold_n > 0)
 };
+
+// Appends slot u to its region of a work list (one atomic; the order of a
+// region's entries does not matter to the list walks).
+__device__ __forceinline__ void wl_append(int32_t* list, uint32_t* cnt, int u) {
+    const int b = u / kWlSlots;
+    const uint32_t k = atomicAdd(&cnt[b], 1u);
+    list[(size_t)b * kWlSlots + k] = u;
+}
+
+// Folds the previous run's deferred count updates (counts += delta, delta
+// = 0) with the threads of this launch; the launch reads no counts and
+// adds its own updates atomically, so the order does not matter.
+__device__ __forceinline__ void fold_in_launch(int32_t* counts, int32_t* delta, int n) {
+    const int nb = (int)(gridDim.x * gridDim.y);
+    const int b = (int)(blockIdx.y * gridDim.x + blockIdx.x);
+    for (int i = b * (int)blockDim.x + (int)threadIdx.x; i < n; i += nb * (int)blockDim.x) {
+        const int d = delta[i];
+        if (d) {
+            atomicAdd(&counts[i], d);
+            delta[i] = 0;
+        }
+    }
+}
 
 template <typename T>
 __device__ __forceinline__ T cat_pick(const T (&a)[kCatFuse], int j) {
@@ -116,6 +143,7 @@ __device__ __forceinline__ int cat_try_ids(const Cells<Real>& C, const Post<Real
     if (uu < (double)pe && cas_byte(P.sp, u, (int8_t)A.sub, (int8_t)A.prod)) {
         count_update(G, n_grids, A.sub, xnu, ynu, -1, A.deferred);
         count_update(G, n_grids, A.prod, xnu, ynu, +1, A.deferred);
+        if (A.app) wl_append(A.app, A.app_cnt, u);
         return 1;
     }
     return 0;
@@ -533,6 +561,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFp32Waves<
     // the parameter block is indexed with per-lane values: keep it in LDS
     // rather than in per-lane scratch
     __shared__ CatArgs<Real> sA;
+    if (A.fold_n > 0) fold_in_launch(A.fold_counts, A.fold_delta, A.fold_n);
     const int tile = xcd_tile(ntiles);
     if (tile >= ntiles) return;  // uniform across the block
     if (threadIdx.x == 0) {
@@ -729,6 +758,7 @@ template <typename Real>
 __global__ __launch_bounds__(256) void k_cat_list(Cells<Real> C, Post<Real> P, CatArgs<Real> A,
                                                   WlView V, unsigned long long* ctr,
                                                   Grids<Real> G, int n_grids) {
+    if (A.fold_n > 0) fold_in_launch(A.fold_counts, A.fold_delta, A.fold_n);
     const int cnt = (int)V.cnt[blockIdx.x];
     const int per = 64 >> A.group_shift;
     if ((int)blockIdx.y * 4 * per >= cnt) return;  // uniform: nothing listed for this block

@@ -1338,14 +1338,23 @@ __device__ __forceinline__ void recruit_apply_block(const Post<Real>& P, int tar
     if (threadIdx.x == 0) *nc = 0u;  // ready for the next pass
 }
 
+// The recruitments of one run of passes: block r applies recruitment r
+// from its candidate buffer (entries r * stride .., counter nc[r]).
+struct RecSet {
+    int target[2], maxr[2];
+};
+
 template <typename Real>
-__global__ __launch_bounds__(1024) void k_recruit_apply(Post<Real> P, int target, int maxr, int n,
+__global__ __launch_bounds__(1024) void k_recruit_apply(Post<Real> P, RecSet R, int n,
                                                         const int32_t* cid, const int32_t* cslot,
                                                         const double* cxo, const double* cyo,
-                                                        unsigned int* nc,
+                                                        unsigned int* nc, int64_t stride,
                                                         unsigned long long* ctr, Grids<Real> G,
                                                         int n_grids) {
-    recruit_apply_block(P, target, maxr, 0, n - 1, cid, cslot, cxo, cyo, nc, ctr, G, n_grids);
+    const int r = (int)blockIdx.x;
+    const int64_t o = r * stride;
+    recruit_apply_block(P, R.target[r], R.maxr[r], 0, n - 1, cid + o, cslot + o, cxo + o,
+                        cyo + o, nc + r, ctr, G, n_grids);
 }
 
 #include "edm_passes.h"
@@ -1456,54 +1465,95 @@ static int upload_real(void** dst, const double* src, size_t n) {
 }
 
 // Fused per-particle passes.  Counts exist only for kept (sensor) species;
-// a pass "writes" the kept species whose counts it changes and "reads" its
-// feedback sensors.  A run is cut before a pass that reads what an earlier
-// pass of the run writes, after a pass that reads what it writes itself
-// (deferred, folded after the run) and after every recruitment pass (its
-// cap needs all candidates).  Passes with zero probability change nothing
-// and are skipped.
+// a pass "writes" the kept species whose counts it changes (a recruitment
+// when its cap is applied) and "reads" its feedback sensors, and it
+// "touches" the particles of its species (a conversion: of both species).
+// Two passes commute when neither reads what the other writes and they
+// touch no common species; a pass may then run before an earlier one.
+// Inside a run every thread applies the run's passes to its particle in
+// order, so a run is closed before a pass that reads what an earlier pass
+// of the run writes, after a pass that reads what it writes itself
+// (deferred, folded after the run), and before a pass that touches, or
+// reads the counts of, a species with a recruitment earlier in the run (a
+// recruitment's index-order cap needs all its candidates, so its particles
+// come alive only after the run).  Runs are formed greedily over each
+// stretch of consecutive per-particle passes: a pass joins the open run
+// unless one of those rules or a skipped earlier pass that it does not
+// commute with stops it.  A run holds at most max_rec recruitments, each
+// with its own candidate buffer, and a density-biased recruitment ends its
+// run (one CDF per run).  Passes with zero probability change nothing and
+// are skipped.  Every run starts at its lowest reaction index, and runs
+// are launched in that order.
 void plan_passes(const std::vector<edm_reaction>& rx, uint32_t sensor_mask,
                  std::vector<std::vector<int>>& groups, std::vector<int>& group_start,
-                 std::vector<int>& deferred) {
+                 std::vector<int>& deferred, int max_rec) {
     const size_t nr = rx.size();
     group_start.assign(nr, -1);
     deferred.assign(nr, 0);
     groups.clear();
     auto kept = [&](int s) { return s >= 0 && ((sensor_mask >> s) & 1u); };
-    std::vector<int> cur;
-    uint32_t wmask = 0;
-    auto close = [&]() {
-        if (!cur.empty()) {
-            group_start[cur.front()] = (int)groups.size();
-            groups.push_back(cur);
-        }
-        cur.clear();
-        wmask = 0;
+    auto sb = [](int s) { return s >= 0 ? 1u << s : 0u; };
+    auto reads = [&](const edm_reaction& r) { return sb(r.fb1.sensor) | sb(r.fb2.sensor); };
+    auto writes = [&](const edm_reaction& r) {
+        uint32_t w = kept(r.a) ? sb(r.a) : 0u;
+        if (r.kind == EDM_RX_CONVERT && kept(r.b)) w |= sb(r.b);
+        return w;
     };
-    for (size_t i = 0; i < nr; ++i) {
-        const edm_reaction& r = rx[i];
-        const bool per_particle =
-            r.kind == EDM_RX_DETACH || r.kind == EDM_RX_CONVERT || r.kind == EDM_RX_RECRUIT;
-        if (!per_particle) {
-            close();
+    auto touches = [&](const edm_reaction& r) {
+        return sb(r.a) | (r.kind == EDM_RX_CONVERT ? sb(r.b) : 0u);
+    };
+    auto commute = [&](const edm_reaction& p, const edm_reaction& q) {
+        return !(reads(p) & writes(q)) && !(reads(q) & writes(p)) && !(touches(p) & touches(q));
+    };
+    for (size_t i = 0; i < nr;) {
+        const bool pp = rx[i].kind == EDM_RX_DETACH || rx[i].kind == EDM_RX_CONVERT ||
+                        rx[i].kind == EDM_RX_RECRUIT;
+        if (!pp) {
+            ++i;
             continue;
         }
-        if (r.prob <= 0.0) continue;
-        uint32_t rmask = 0, own = 0;
-        if (r.fb1.sensor >= 0) rmask |= 1u << r.fb1.sensor;
-        if (r.fb2.sensor >= 0) rmask |= 1u << r.fb2.sensor;
-        if (r.kind == EDM_RX_DETACH && kept(r.a)) own |= 1u << r.a;
-        if (r.kind == EDM_RX_CONVERT) {
-            if (kept(r.a)) own |= 1u << r.a;
-            if (kept(r.b)) own |= 1u << r.b;
+        std::vector<int> rem;  // this stretch's passes, in order
+        for (; i < nr; ++i) {
+            const edm_reaction& r = rx[i];
+            if (r.kind != EDM_RX_DETACH && r.kind != EDM_RX_CONVERT && r.kind != EDM_RX_RECRUIT)
+                break;
+            if (r.prob > 0.0) rem.push_back((int)i);
         }
-        if (!cur.empty() && ((rmask & wmask) || cur.size() >= (size_t)kMaxPasses)) close();
-        deferred[i] = (rmask & own) ? 1 : 0;
-        cur.push_back((int)i);
-        wmask |= own;
-        if (deferred[i] || r.kind == EDM_RX_RECRUIT) close();
+        while (!rem.empty()) {
+            std::vector<int> cur, skipped;
+            uint32_t wmask = 0, pend = 0;
+            int nrec = 0;
+            bool closed = false;
+            for (int k : rem) {
+                const edm_reaction& r = rx[k];
+                bool wait = closed;
+                for (int q : skipped)
+                    if (!wait && !commute(rx[q], r)) wait = true;
+                const bool rec = r.kind == EDM_RX_RECRUIT;
+                const uint32_t rmask = reads(r);
+                const uint32_t own = r.kind == EDM_RX_RECRUIT ? 0u : writes(r);
+                if (!wait && !cur.empty() &&
+                    ((rmask & (wmask | pend)) || (touches(r) & pend) ||
+                     (rec && nrec >= max_rec) || cur.size() >= (size_t)kMaxPasses))
+                    wait = true;
+                if (wait) {
+                    skipped.push_back(k);
+                    continue;
+                }
+                deferred[k] = (rmask & own) ? 1 : 0;
+                cur.push_back(k);
+                wmask |= own;
+                if (rec) {
+                    pend |= sb(r.a);
+                    ++nrec;
+                }
+                if (deferred[k] || (rec && (r.use_adapt || nrec >= max_rec))) closed = true;
+            }
+            group_start[cur.front()] = (int)groups.size();
+            groups.push_back(cur);
+            rem.swap(skipped);
+        }
     }
-    close();
 }
 
 // Species the passes act on, closed under every reaction that turns one
@@ -1622,7 +1672,8 @@ static int create_t(edm_membrane* m) {
         if (dalloc(&m->w_adapt, (size_t)g.nx * g.ny)) return EDM_EHIP;
         EDM_HIP(hipMemset(m->w_adapt, 0, (size_t)g.nx * g.ny * sizeof(double)));
     }
-    plan_passes(m->rx, m->sensor_mask_all, m->pass_groups, m->group_start, m->pass_deferred);
+    plan_passes(m->rx, m->sensor_mask_all, m->pass_groups, m->group_start, m->pass_deferred,
+                kMaxRecruits);
     // largest squared range of any pair potential (the tiled force kernel
     // skips candidates beyond it before looking up the species tables)
     {
@@ -1652,8 +1703,9 @@ static int create_t(edm_membrane* m) {
     }
     if (dalloc(&m->edge_key, m->edge_cap) || dalloc(&m->edge_a, m->edge_cap) ||
         dalloc(&m->edge_b, m->edge_cap) || dalloc(&m->n_edges, 4) || dalloc(&m->best, n) ||
-        dalloc(&m->cand_id, n) || dalloc(&m->cand_slot, n) || dalloc(&m->cand_x, n) ||
-        dalloc(&m->cand_y, n) || dalloc(&m->n_cand, 4) || dalloc(&m->d_ctr, 8))
+        dalloc(&m->cand_id, kMaxRecruits * n) || dalloc(&m->cand_slot, kMaxRecruits * n) ||
+        dalloc(&m->cand_x, kMaxRecruits * n) || dalloc(&m->cand_y, kMaxRecruits * n) ||
+        dalloc(&m->n_cand, 4) || dalloc(&m->d_ctr, 8))
         return EDM_EHIP;
     size_t ncdf = 0;
     for (const auto& r : m->rx)
@@ -1750,8 +1802,14 @@ static void plan_pass_lists(edm_membrane* m, int& next) {
         if (next < kWlMaxLists) {
             pp.list = next++;
             pp.spec = WlSpec{S, 0u};
+            // reactions before the run: those before its first one, and
+            // the members of earlier runs (a run may hold later reactions
+            // that commute with the ones it skipped)
             for (int k = 0; k < pp.ri; ++k)
                 if (touched_species(m->rx[k], false) & S) pp.late = true;
+            for (size_t gj = 0; gj < gi; ++gj)
+                for (int k : m->pass_groups[gj])
+                    if (touched_species(m->rx[k], false) & S) pp.late = true;
         }
         m->pass_plan.push_back(pp);
     }
@@ -1787,8 +1845,26 @@ static void plan_cat_walks(edm_membrane* m, int& next) {
         if (m->cat_walk == 1 && !(E & (sbit(r.b) | sbit(r.c))) && next < kWlMaxLists) {
             c.list = next++;
             c.spec = WlSpec{c.by_sub ? sbit(r.b) : E, (uint32_t)EDM_FLAG_ALIVE};
-            for (size_t k = 0; k < ri; ++k)
-                if (touched_species(m->rx[k], true) & c.spec.species) c.late = true;
+            // An earlier catalysis run (not autocatalytic) that only converts
+            // substrates into the list's species can append them to the list
+            // built after the sort; any other earlier change to the entities
+            // means a build just before the run.
+            std::vector<size_t> app;
+            for (size_t k = 0; k < ri; ++k) {
+                const edm_reaction& q = m->rx[k];
+                if (!(touched_species(q, true) & c.spec.species)) continue;
+                CatPlan* src = nullptr;
+                for (CatPlan& e : m->cat_plan)
+                    if ((int)k >= e.ri && (int)k < e.ri + e.nf) src = &e;
+                const bool adds = q.kind == EDM_RX_CAT && q.a != q.c &&
+                                  (sbit(q.c) & c.spec.species) && !(sbit(q.b) & c.spec.species);
+                if (adds && src && (src->app_list < 0 || src->app_list == c.list))
+                    app.push_back((size_t)(src - m->cat_plan.data()));
+                else
+                    c.late = true;
+            }
+            if (!c.late)
+                for (size_t e : app) m->cat_plan[e].app_list = c.list;
         }
         m->cat_plan.push_back(c);
         ri += c.nf;
@@ -2039,6 +2115,14 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
         }
     }
     int cat_skip = 0;
+    bool fold_pending = false;  // the previous run's deferred updates, not yet folded
+    // a catalysis run that reads no counts and defers none can fold the
+    // previous run's deferred updates in its own launch
+    auto folds_in = [&](size_t j) {
+        if (j >= m->rx.size()) return false;
+        const edm_reaction& q = m->rx[j];
+        return q.kind == EDM_RX_CAT && q.a != q.c && q.fb1.sensor < 0;
+    };
     for (size_t ri = 0; ri < m->rx.size(); ++ri) {
         const edm_reaction& r = m->rx[ri];
         Draws dr{hook ? draws : nullptr, off, npairs, n};
@@ -2111,6 +2195,20 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
                 A.step_base = step.base;
                 A.deferred = defer;
                 A.group_shift = __builtin_ctz((unsigned)m->cat_lanes);
+                A.app = nullptr;
+                A.app_cnt = nullptr;
+                if (cp && cp->app_list >= 0 && m->wl_valid &&
+                    ((m->wl_built >> cp->app_list) & 1u)) {
+                    A.app = m->wl + (size_t)cp->app_list * m->wl_nblk * kWlSlots;
+                    A.app_cnt = m->wl_cnt + (size_t)cp->app_list * m->wl_nblk;
+                }
+                A.fold_counts = m->counts;
+                A.fold_delta = m->delta;
+                A.fold_n = 0;
+                if (fold_pending) {
+                    A.fold_n = m->n_count;
+                    fold_pending = false;
+                }
                 A.by_substrate = cp ? cp->by_sub
                                     : m->cat_side >= 0
                                           ? m->cat_side == 1
@@ -2144,9 +2242,13 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
                                        dim3(256), 0, st, C, P, A, m->d_ctr, G, ng, ntile_y,
                                        ntiles);
                 }
-                if (defer && m->n_count)
-                    hipLaunchKernelGGL(k_fold, dim3(nblk(m->n_count)), dim3(kBlock), 0, st,
-                                       m->n_count, m->counts, m->delta);
+                if (defer && m->n_count) {
+                    if (folds_in(ri + nf))
+                        fold_pending = true;
+                    else
+                        hipLaunchKernelGGL(k_fold, dim3(nblk(m->n_count)), dim3(kBlock), 0, st,
+                                           m->n_count, m->counts, m->delta);
+                }
                 cat_skip = nf - 1;
                 off += npairs;
                 break;
@@ -2158,10 +2260,19 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
                     const std::vector<int>& g = m->pass_groups[m->group_start[ri]];
                     PassGroupDev grp{};
                     grp.n = (int)g.size();
-                    bool defer_any = false;
+                    bool defer_any = false, local = false;
+                    RecSet R{};
+                    int nrec = 0;
                     for (int k = 0; k < grp.n; ++k) {
                         const edm_reaction& q = m->rx[g[k]];
                         PassDev& d = grp.p[k];
+                        d.rec = 0;
+                        if (q.kind == EDM_RX_RECRUIT) {
+                            d.rec = nrec;
+                            R.target[nrec] = q.a;
+                            R.maxr[nrec] = q.max_recruits;
+                            ++nrec;
+                        }
                         d.kind = q.kind;
                         d.a = q.a;
                         d.b = q.b;
@@ -2181,6 +2292,7 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
                             // no earlier pass of the run changes them
                             d.cdf = m->rcdf;
                             d.sigma = q.contact;
+                            local = true;
                             hipLaunchKernelGGL(k_recruit_cdf<Real>, dim3(1), dim3(1024), 0, st,
                                                (const int32_t*)m->counts + d.f1.cbase,
                                                d.f1.gnx * d.f1.gny, (Real)d.f1.ginv, m->rcdf);
@@ -2194,21 +2306,19 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
                                          &pp.list, &pp.spec, 1);
                             V = wl_view(m, pp.list);
                         }
-                    const bool local = grp.p[grp.n - 1].cdf != nullptr;
                     hipLaunchKernelGGL((local ? k_passes<Real, true> : k_passes<Real, false>),
                                        V.list ? dim3(m->wl_nblk, kWlPer) : dim3(nblk(n)),
                                        dim3(kBlock), 0, st, n, V,
                                        (const int32_t*)m->a.id, P, grp, hook ? draws : nullptr,
                                        rng, step, (Real)p.Lx, (Real)p.Ly, m->d_ctr, G, ng,
-                                       m->cand_id, m->cand_slot, m->cand_x, m->cand_y, m->n_cand);
-                    const edm_reaction& last = m->rx[g.back()];
-                    if (last.kind == EDM_RX_RECRUIT)
-                        hipLaunchKernelGGL(k_recruit_apply<Real>, dim3(1), dim3(1024), 0, st, P,
-                                           last.a, last.max_recruits, n,
-                                           (const int32_t*)m->cand_id,
+                                       m->cand_id, m->cand_slot, m->cand_x, m->cand_y, m->n_cand,
+                                       (int64_t)n);
+                    if (nrec > 0)
+                        hipLaunchKernelGGL(k_recruit_apply<Real>, dim3(nrec), dim3(1024), 0, st, P,
+                                           R, n, (const int32_t*)m->cand_id,
                                            (const int32_t*)m->cand_slot, (const double*)m->cand_x,
-                                           (const double*)m->cand_y, m->n_cand, m->d_ctr, G,
-                                           ng);
+                                           (const double*)m->cand_y, m->n_cand, (int64_t)n,
+                                           m->d_ctr, G, ng);
                     if (defer_any && m->n_count)
                         hipLaunchKernelGGL(k_fold, dim3(nblk(m->n_count)), dim3(kBlock), 0, st,
                                            m->n_count, m->counts, m->delta);

@@ -291,7 +291,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFp32Waves<
     const PassGroupDev& grp = pg[(size_t)W.r * n_groups + gi];
     const int j = W.j + lane_id();
     int det = 0, conv = 0;
-    Cand<Real> cand{false, 0, (Real)0, (Real)0};
+    Cand<Real> cand{false, 0, (Real)0, (Real)0, 0};
     int t = -1;
     if (j < min(g.wl_n[0], (int)rcnt[2 * W.r])) {
         t = list[g.wl_off[0] + j];
@@ -513,7 +513,7 @@ static int ens_create_t(edm_ensemble* e) {
                 e->kept.emplace_back(g, s);
                 e->sensor_mask_all |= 1u << s;
             }
-    plan_passes(e->rx, e->sensor_mask_all, e->pass_groups, e->group_start, e->pass_deferred);
+    plan_passes(e->rx, e->sensor_mask_all, e->pass_groups, e->group_start, e->pass_deferred, 1);
     e->pass_species = pass_species_of(e->rx, e->pass_groups);
     e->seg_npass.assign(e->rmax, 0);
     e->seg_ndye.assign(e->rmax, 0);

@@ -203,7 +203,7 @@ int ens_frames_fetch(edm_ensemble* e, int32_t ticket, int64_t* counts, int64_t* 
 void ens_frames_free(edm_ensemble* e);
 void plan_passes(const std::vector<edm_reaction>& rx, uint32_t sensor_mask,
                  std::vector<std::vector<int>>& groups, std::vector<int>& group_start,
-                 std::vector<int>& deferred);
+                 std::vector<int>& deferred, int max_rec);
 uint32_t pass_species_of(const std::vector<edm_reaction>& rx,
                          const std::vector<std::vector<int>>& groups);
 }  // namespace edm

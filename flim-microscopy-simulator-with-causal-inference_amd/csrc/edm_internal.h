@@ -67,6 +67,8 @@ struct CatPlan {
     bool late;    // an earlier reaction of the step may change its entities:
                   // the list is built just before the run
     WlSpec spec;
+    int app_list = -1;  // a later run's work list to which this run appends the
+                        // substrates it converts into that list's species
 };
 
 namespace edm {

@@ -6,8 +6,10 @@
 // which every thread applies the passes to its own particle in the
 // reference's order, provided no pass reads a density count that an
 // earlier pass of the run changes (the run is split there on the host).
-// A recruitment pass ends its run: the index-order cap needs all
-// candidates before any is accepted (k_recruit_apply).  With the step's
+// A recruitment's index-order cap needs all its candidates before any is
+// accepted (k_recruit_apply), so nothing after it in its run may depend on
+// it; a run may hold a few recruitments of different species, each with its
+// own candidate buffer.  With the step's
 // work list of the species the passes can touch a thread takes one listed
 // slot instead of one slot of the whole state.
 
@@ -23,6 +25,7 @@ struct PassDev {
                   // cells, written by the CDF kernel before the run (nullptr:
                   // uniform placement)
     double sigma;       // its Gaussian offset around the chosen cell centre
+    int rec;            // RECRUIT: its candidate buffer (0 .. kMaxRecruits - 1)
 };
 
 static constexpr int kMaxPasses = 8;
@@ -33,36 +36,49 @@ struct PassGroupDev {
     PassDev p[kMaxPasses];
 };
 
-// A recruitment candidate of this thread's particle (at most one per run:
-// a recruitment pass ends its run).
+// Recruitments of one run (each with its own candidate buffer).
+static constexpr int kMaxRecruits = 2;
+
+// A recruitment candidate of this thread's particle: at most one per run
+// (the run's recruitments are of different species, and a particle
+// recruited or not keeps its species through the run).
 template <typename Real>
 struct Cand {
     bool on;
     int it;
     Real x, y;
+    int rec;  // candidate buffer
 };
 
-// Appends the wavefront's candidates with one atomic on the counter (the
-// order of candidates in the array does not matter: the cap is applied by
-// particle index).  Every lane of the wavefront must call it; `nc` must be
-// the same for the whole wavefront.
+// Appends the wavefront's candidates with one atomic per buffer (the order
+// of candidates in a buffer does not matter: the cap is applied by particle
+// index).  Buffer r starts at r * stride entries and counts in nc[r *
+// nc_stride].  Every lane of the wavefront must call it; the buffers must
+// be the same for the whole wavefront.
 template <typename Real>
 __device__ __forceinline__ void append_candidates(const Cand<Real>& c, int t, int32_t* cid,
                                                   int32_t* cslot, double* cxo, double* cyo,
-                                                  unsigned int* nc) {
-    const unsigned long long b = __ballot(c.on);
-    if (!b) return;  // uniform
+                                                  unsigned int* nc, int64_t stride = 0,
+                                                  int nc_stride = 1) {
+    if (!__ballot(c.on)) return;  // uniform
     const int lane = lane_id();
-    const int leader = __ffsll((long long)b) - 1;
-    unsigned int base = 0u;
-    if (lane == leader) base = atomicAdd(nc, (unsigned int)__popcll(b));
-    base = (unsigned int)__builtin_amdgcn_readlane((int)base, leader);
-    if (c.on) {
-        const unsigned int k = base + (unsigned int)__popcll(b & ((1ull << lane) - 1ull));
-        cid[k] = c.it;
-        cslot[k] = t;
-        cxo[k] = (double)c.x;
-        cyo[k] = (double)c.y;
+#pragma unroll
+    for (int r = 0; r < kMaxRecruits; ++r) {
+        const bool mine = c.on && c.rec == r;
+        const unsigned long long b = __ballot(mine);
+        if (!b) continue;  // uniform
+        const int leader = __ffsll((long long)b) - 1;
+        unsigned int base = 0u;
+        if (lane == leader) base = atomicAdd(nc + r * nc_stride, (unsigned int)__popcll(b));
+        base = (unsigned int)__builtin_amdgcn_readlane((int)base, leader);
+        if (mine) {
+            const int64_t k =
+                r * stride + base + (unsigned int)__popcll(b & ((1ull << lane) - 1ull));
+            cid[k] = c.it;
+            cslot[k] = t;
+            cxo[k] = (double)c.x;
+            cyo[k] = (double)c.y;
+        }
     }
 }
 
@@ -210,6 +226,7 @@ __device__ __forceinline__ void passes_particle(int t, int s0, int n, const int3
                     cand.it = it;
                     cand.x = px[k];
                     cand.y = py[k];
+                    cand.rec = d.rec;
                 }
             }
         }
@@ -240,7 +257,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFp32Waves<
                                                 StepRef sref, Real Lx, Real Ly,
                                                 unsigned long long* ctr, Grids<Real> G,
                                                 int n_grids, int32_t* cid, int32_t* cslot,
-                                                double* cxo, double* cyo, unsigned int* nc) {
+                                                double* cxo, double* cyo, unsigned int* nc,
+                                                int64_t cand_stride) {
     const uint32_t step = sref.get();
     int t = -1;
     if (V.list) {
@@ -252,13 +270,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFp32Waves<
         if (q < n) t = q;
     }
     int det_hits = 0, conv_hits = 0;
-    Cand<Real> cand{false, 0, (Real)0, (Real)0};
+    Cand<Real> cand{false, 0, (Real)0, (Real)0, 0};
     if (t >= 0) {
         const int s0 = P.sp[t];
         if ((grp.species >> s0) & 1u)
             passes_particle<Real, kLocal>(t, s0, n, id, 0, P, grp, draws, rng, step, Lx, Ly, G,
                                           n_grids, cand, det_hits, conv_hits);
     }
-    append_candidates(cand, t, cid, cslot, cxo, cyo, nc);
+    append_candidates(cand, t, cid, cslot, cxo, cyo, nc, cand_stride);
     passes_count<Real>(det_hits, conv_hits, ctr);
 }
