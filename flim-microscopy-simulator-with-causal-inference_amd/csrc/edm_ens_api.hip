@@ -29,6 +29,8 @@ static void ens_free(edm_ensemble* e) {
     for (auto& g : e->seg) {
         if (g.flim) hipFree(g.flim);
         if (g.rcdf) hipFree(g.rcdf);
+        if (g.reg) hipFree(g.reg);
+        if (g.w_adapt) hipFree(g.w_adapt);
     }
     ens_frames_free(e);
     if (e->stream) hipStreamDestroy(e->stream);
@@ -53,14 +55,15 @@ int edm_ens_create(const edm_params* p, const edm_photo_params* pp, const edm_im
         return ens_invalid("edm_ens_create: at most 16 reactions");
     if (!p->sigma || !p->epsilon || !p->k_harm || !p->D_base)
         return ens_invalid("edm_ens_create: missing parameter arrays");
-    if (p->n_regions != 0)
-        return ens_invalid("edm_ens_create: circular D(x) regions are per-membrane; not batched");
-    if (p->adapt_grid >= 0)
-        return ens_invalid("edm_ens_create: the adaptation field is not batched");
+    if (p->adapt_grid >= p->n_grids ||
+        (p->adapt_grid >= 0 && (p->adapt_species < 0 || p->adapt_species >= p->n_species)))
+        return ens_invalid("edm_ens_create: bad adaptation grid or species");
     for (int i = 0; i < p->n_reactions; ++i) {
         const edm_reaction& r = p->reactions[i];
-        if (r.kind == EDM_RX_BIND || r.kind == EDM_RX_ADAPT)
+        if (r.kind == EDM_RX_BIND)
             return ens_invalid("edm_ens_create: binding reactions are not batched");
+        if (r.kind == EDM_RX_ADAPT && p->adapt_grid < 0)
+            return ens_invalid("edm_ens_create: adaptation step without an adaptation grid");
         if (r.kind == EDM_RX_RECRUIT && r.use_adapt &&
             (r.fb1.sensor < 0 || r.fb2.sensor >= 0 || !(r.contact >= 0.0)))
             return ens_invalid("edm_ens_create: local recruitment needs one feedback rule and "
@@ -135,6 +138,13 @@ int edm_ens_add(edm_ensemble* e, const edm_replica* r, const edm_host_state* s, 
         return ens_invalid("edm_ens_add: bad box or cell grid");
     if (r->n_grids != (int)e->grids.size() || r->n_reactions != (int)e->rx.size())
         return ens_invalid("edm_ens_add: the replica does not match the model's grids/reactions");
+    if (r->n_regions < 0 || r->n_regions > kEnsMaxRegions ||
+        (r->n_regions > 0 && e->p.dfield != 0))
+        return ens_invalid("edm_ens_add: 0..8 circular D(x) regions, and none with the "
+                           "sinusoidal field");
+    for (int q = 0; q < r->n_regions; ++q)
+        if (!(r->reg_r[q] >= 0.0) || !(r->reg_f[q] >= 0.0))
+            return ens_invalid("edm_ens_add: a D(x) region needs radius >= 0 and factor >= 0");
     if (e->n_total + r->n > e->cap) {
         set_error("edm_ens_add: particle pool full");
         return EDM_ECAPACITY;
@@ -170,6 +180,8 @@ int edm_ens_add(edm_ensemble* e, const edm_replica* r, const edm_host_state* s, 
             if (!armed) return;
             if (g->rcdf) hipFree(g->rcdf);
             if (g->flim) hipFree(g->flim);
+            if (g->reg) hipFree(g->reg);
+            if (g->w_adapt) hipFree(g->w_adapt);
         }
     } guard{&g, true};
     g.active = 1;
@@ -213,6 +225,31 @@ int edm_ens_add(edm_ensemble* e, const edm_replica* r, const edm_host_state* s, 
             if (rx.kind == EDM_RX_RECRUIT && rx.use_adapt)
                 ncdf = std::max(ncdf, (size_t)g.gdims[2 * rx.fb1.grid] * g.gdims[2 * rx.fb1.grid + 1]);
         if (ncdf) EDM_HIP(hipMalloc(&g.rcdf, ncdf * sizeof(double)));
+    }
+    // this replica's D(x) regions in the position type (as the single
+    // membrane's, upload_real) and its adaptation field, zero at admission
+    g.n_regions = r->n_regions;
+    if (r->n_regions > 0) {
+        const int R = kEnsMaxRegions;
+        std::vector<double> v(4 * R, 0.0);
+        for (int q = 0; q < r->n_regions; ++q) {
+            v[q] = r->reg_cx[q];
+            v[R + q] = r->reg_cy[q];
+            v[2 * R + q] = r->reg_r[q];
+            v[3 * R + q] = r->reg_f[q];
+        }
+        EDM_HIP(hipMalloc(&g.reg, v.size() * e->rs));
+        if (e->fp64) {
+            EDM_HIP(hipMemcpy(g.reg, v.data(), v.size() * 8, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> f(v.begin(), v.end());
+            EDM_HIP(hipMemcpy(g.reg, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+        }
+    }
+    if (e->p.adapt_grid >= 0) {
+        const size_t nw = (size_t)g.gdims[2 * e->p.adapt_grid] * g.gdims[2 * e->p.adapt_grid + 1];
+        EDM_HIP(hipMalloc(&g.w_adapt, std::max<size_t>(nw, 1) * sizeof(double)));
+        EDM_HIP(hipMemset(g.w_adapt, 0, std::max<size_t>(nw, 1) * sizeof(double)));
     }
     if (imaging) {
         g.ip = e->ip;
@@ -296,6 +333,10 @@ int edm_ens_remove(edm_ensemble* e, int32_t slot) {
     // the next step's layout uploads this slot's tables without the pointer
     if (g.rcdf) EDM_HIP(hipFree(g.rcdf));
     g.rcdf = nullptr;
+    if (g.reg) EDM_HIP(hipFree(g.reg));
+    g.reg = nullptr;
+    if (g.w_adapt) EDM_HIP(hipFree(g.w_adapt));
+    g.w_adapt = nullptr;
     g.active = 0;
     e->n_retired += g.n;
     e->draining.push_back(slot);

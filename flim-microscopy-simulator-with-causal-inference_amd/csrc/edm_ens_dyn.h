@@ -197,6 +197,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFp32Waves<
     ForceParams<Real> Fr = F;
     Fr.sin_kx = (Real)g.sin_kx;
     Fr.sin_ky = (Real)g.sin_ky;
+    Fr.n_regions = g.n_regions;
+    if (g.n_regions > 0) {
+        const Real* R = (const Real*)g.reg;
+        Fr.rcx = R;
+        Fr.rcy = R + kEnsMaxRegions;
+        Fr.rr = R + 2 * kEnsMaxRegions;
+        Fr.rf = R + 3 * kEnsMaxRegions;
+    }
     const Grids<Real> G = ens_grids<Real>(g, Q.counts, Q.delta, Q.ns);
     force_strip<Real, kSub, kMaxOwnCols>(C, Fr, Q.xn, Q.yn, (const double*)nullptr, ens_rng(g),
                                          gstep - g.step0, G, Q.n_grids, keep_mask,
@@ -329,6 +337,22 @@ __global__ __launch_bounds__(1024) void k_ens_recruit(const EnsSeg* __restrict__
                         Q.n_grids);
 }
 
+// Adaptation field of every replica (update_adaptation_field,
+// dynamics.py:1094-1112): block (x, r) updates cells x * 256 .. of replica
+// r's field from its sensor counts, as k_adapt does for one membrane.
+template <typename Real>
+__global__ void k_ens_adapt(const EnsSeg* __restrict__ segs, int32_t* counts, int grid,
+                            int species, double rate, int ns) {
+    const EnsSeg& g = segs[blockIdx.y];
+    if (!g.active || !g.w_adapt) return;
+    const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (c >= g.gdims[2 * grid] * g.gdims[2 * grid + 1]) return;
+    const Grids<Real> G = ens_grids<Real>(g, counts, nullptr, ns);
+    const double rho = (double)density_of<Real>(G.counts[G.off[grid * G.ns + species] + c],
+                                                G.inv[grid]);
+    g.w_adapt[c] += rate * (rho - g.w_adapt[c]);
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -415,6 +439,11 @@ static CatArgs<Real> ens_cat_host(const edm_ensemble* e, int start, int nf, cons
     A.fb = ens_fb_host(r.fb1, g, e->p.n_species);
     A.w = nullptr;
     A.wnx = A.wny = 1;
+    if (r.use_adapt && e->p.adapt_grid >= 0 && g.w_adapt) {
+        A.w = g.w_adapt;
+        A.wnx = g.gdims[2 * e->p.adapt_grid];
+        A.wny = g.gdims[2 * e->p.adapt_grid + 1];
+    }
     A.Kw = (Real)e->p.adapt_K;
     A.nw = (Real)e->p.adapt_n;
     A.dr = Draws{nullptr, 0, 0, 0};
@@ -506,6 +535,7 @@ static int ens_create_t(edm_ensemble* e) {
         if (r.fb1.sensor >= 0) keep[(size_t)r.fb1.grid * ns + r.fb1.sensor] = 1;
         if (r.fb2.sensor >= 0) keep[(size_t)r.fb2.grid * ns + r.fb2.sensor] = 1;
     }
+    if (p.adapt_grid >= 0) keep[(size_t)p.adapt_grid * ns + p.adapt_species] = 1;
     e->kept.clear();
     for (int g = 0; g < ng; ++g)
         for (int s = 0; s < ns; ++s)
@@ -789,7 +819,16 @@ static int ens_step_t(edm_ensemble* e) {
 
     for (size_t ri = 0; ri < e->rx.size(); ++ri) {
         const edm_reaction& r = e->rx[ri];
-        if (r.kind == EDM_RX_CAT) {
+        if (r.kind == EDM_RX_ADAPT) {
+            const int ag = p.adapt_grid;
+            int mc = 1;
+            for (int q = 0; q < e->rmax; ++q)
+                if (e->seg[q].active)
+                    mc = std::max(mc, e->seg[q].gdims[2 * ag] * e->seg[q].gdims[2 * ag + 1]);
+            hipLaunchKernelGGL(k_ens_adapt<Real>, dim3(nblk(mc), e->rmax), dim3(kBlock), 0, st,
+                               (const EnsSeg*)e->d_seg, e->counts, ag, p.adapt_species,
+                               p.dt / p.adapt_tau, p.n_species);
+        } else if (r.kind == EDM_RX_CAT) {
             const int ci = e->cat_run_of[ri];
             if (ci < 0) continue;  // member of a run launched before
             const int start = e->cat_runs[ci].first;

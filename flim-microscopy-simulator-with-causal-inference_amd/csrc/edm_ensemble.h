@@ -52,7 +52,12 @@ struct EnsSeg {
     int32_t wl_off[2];
     int32_t wl_n[2];
     int32_t sub_rows;  // force candidates by sub-row (dense replicas)
+    int32_t n_regions;  // circular D(x) regions of this replica
+    void* reg;          // their centres x, y, radii and D factors (Real[4][8]), or nullptr
+    double* w_adapt;    // this replica's adaptation field on its adaptation grid, or nullptr
 };
+
+static constexpr int kEnsMaxRegions = 8;
 
 // The wave's replica and its first entry of list L: every wave of a list
 // launch lies inside one replica's region.

@@ -97,7 +97,9 @@ class Replica(ctypes.Structure):
                 ("seed", ctypes.c_uint64), ("grids", Grid * 4), ("prob", c_double * 16),
                 ("max_recruits", c_int32 * 16), ("img_nx", c_int32), ("img_ny", c_int32),
                 ("img_R", c_int32), ("n_reactions", c_int32), ("img_pixel", c_double),
-                ("img_offx", c_double), ("img_offy", c_double)]
+                ("img_offx", c_double), ("img_offy", c_double), ("n_regions", c_int32),
+                ("pad_", c_int32), ("reg_cx", c_double * 8), ("reg_cy", c_double * 8),
+                ("reg_r", c_double * 8), ("reg_f", c_double * 8)]
 
 
 class Census(ctypes.Structure):

@@ -44,6 +44,8 @@ def _model_signature(config: SimulationConfig, c: Dict, plan: _Plan):
             tuple(rx), len(plan.grids), c["cutoff"], c["max_disp"], c["dt"],
             c["sigma"].tobytes(), c["epsilon"].tobytes(), c["k_harm"].tobytes(),
             tuple(config.membrane.periodic), config.diffusion_field.type,
+            float(config.region_smoothing_width),
+            (plan.adapt_grid >= 0, plan.adapt_species, tuple(plan.adapt)),
             tuple(f.name for f in config.fluorophores), _duty(config))
 
 
@@ -107,6 +109,16 @@ class EnsembleEngine:
             lx, ly = df.wavelength
             r.sin_kx = 2.0 * math.pi / (lx if lx > 0 else c["Lx"])
             r.sin_ky = 2.0 * math.pi / (ly if ly > 0 else c["Ly"])
+        else:
+            # the replica's own circular regions (dynamics.py:441-519)
+            circles = [g for g in df.regions if g.get("shape") == "circle"]
+            if len(circles) > 8:
+                raise ConfigError("a batched replica holds at most 8 circular D(x) regions")
+            r.n_regions = len(circles)
+            for i, g in enumerate(circles):
+                r.reg_cx[i], r.reg_cy[i] = float(g["center"][0]), float(g["center"][1])
+                r.reg_r[i] = float(g["radius"])
+                r.reg_f[i] = float(g.get("D_factor", 1.0))
         r.seed = int(config.random_seed) & 0xFFFFFFFFFFFFFFFF
         for i, (gx, gy, sx, sy) in enumerate(plan.grids):
             r.grids[i] = _hip.Grid(nx=gx, ny=gy, sx=sx, sy=sy)

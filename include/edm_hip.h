@@ -537,10 +537,11 @@ int edm_stats_flim(edm_membrane *m, int32_t channel, int32_t harmonic, double *o
  * displacement cap; n, box and cell fields ignored) plus the photophysics
  * tables and imaging constants; each replica brings its own box, particle
  * state, seed, reaction probabilities, recruitment caps, density-grid
- * shapes and image geometry.  Philox draws only (keys relative to each
- * replica's first particle and admission step), so a replica evolves
- * exactly as it does alone.  Not batched: binding, circular D(x) regions,
- * the adaptation field, proximity or intermolecular FRET.               */
+ * shapes, circular D(x) regions and image geometry; the adaptation field
+ * (model adapt_* fields) is kept per replica on its own grid.  Philox
+ * draws only (keys relative to each replica's first particle and admission
+ * step), so a replica evolves exactly as it does alone.  Not batched:
+ * binding, autocatalysis, proximity or intermolecular FRET.             */
 typedef struct edm_replica {
     int32_t n;                  /* particles (alive + dormant)              */
     int32_t ncx, ncy;           /* cell grid (CellList, dynamics.py:109-114) */
@@ -553,6 +554,10 @@ typedef struct edm_replica {
     int32_t max_recruits[16];
     int32_t img_nx, img_ny, img_R, n_reactions;
     double img_pixel, img_offx, img_offy;
+    /* circular D(x) regions of this replica (apply_diffusion_field_regions_smooth,
+     * dynamics.py:441-519), at most 8; the smoothing width is the model's */
+    int32_t n_regions, pad_;
+    double reg_cx[8], reg_cy[8], reg_r[8], reg_f[8];
 } edm_replica;
 
 typedef struct edm_ensemble edm_ensemble;

@@ -217,3 +217,79 @@ def test_ensemble_autotuner_picks_from_timed_candidates():
     print("[ens-tune]", {k: res[k] for k in keys},
           {k: round(v, 3) for k, v in t.items()}, flush=True)
     ens.close()
+
+
+def _regions_adapt_replica(r):
+    """An ensemble replica with two circular slow-diffusion regions of its
+    own and the adaptation field on, strong enough to change the PI3K
+    catalysis."""
+    cfg = workloads.ensemble_replica(r)
+    L = cfg.membrane.size[0]
+    cfg.diffusion_field.type = "regions"
+    cfg.diffusion_field.regions = [
+        {"shape": "circle", "center": [L / 3, L / 2], "radius": L / 6, "D_factor": 0.2},
+        {"shape": "circle", "center": [0.7 * L, 0.3 * L], "radius": L / 8, "D_factor": 2.0}]
+    cfg.use_adaptation = True
+    cfg.adaptation_tau_w = 0.02
+    cfg.adaptation_K_w = 1.0
+    cfg.adaptation_n_w = 2.0
+    for c in cfg.catalytic_reactions:
+        if "PI3K" in c.enzyme:
+            c.k_cat *= 200.0
+    return cfg
+
+
+def test_regions_and_adaptation_replicas_match_single_membrane_path():
+    """Circular D(x) regions (each replica its own) and the adaptation field
+    (each replica its own on its grid) in the batch: every replica of a
+    three-replica ensemble evolves bit for bit as on the single-membrane
+    path (FP32, Philox), through the steps of a frame and its end; the same
+    replicas without adaptation give other catalytic counts."""
+    steps = 50
+    rs = (2, 5, 9)
+    singles = {}
+    for r in rs:
+        cfg = _regions_adapt_replica(r)
+        st = ParticleState(cfg)
+        eng = DynamicsEngine(cfg, st, precision="fp32", rng="philox", pair_cap=False)
+        assert "adapt" in eng.plan.kinds
+        ph = PhotonSimulator(cfg, st, min(1.0, cfg.scan.dwell_time / cfg.scan.frame_interval))
+        mic = VirtualMicroscope(cfg, st)
+        for _ in range(steps):
+            eng.step()
+            mic.process_photons(ph.evolve_one_bd_step())
+        fr = mic.finalize_frame()
+        singles[r] = (dict(positions=st.positions.copy(), species_id=st.species_id.copy(),
+                           is_alive=st.is_alive.copy(), is_bleached=st.is_bleached.copy(),
+                           ctmc_state=st.ctmc_state.copy()),
+                      (eng.n_catalytic, eng.n_recruited, eng.n_detached, eng.n_converted),
+                      fr.flim_stack.copy())
+        eng.close()
+    ens = EnsembleEngine(_regions_adapt_replica(rs[0]), capacity=400_000, max_replicas=4)
+    slots = {r: ens.add(_regions_adapt_replica(r)) for r in rs}
+    ens.run(steps)
+    for r in rs:
+        _same(ens.state(slots[r]), singles[r][0], f"replica {r}")
+        c = ens.counters(slots[r])
+        assert (c["n_catalytic"], c["n_recruited"], c["n_detached"],
+                c["n_converted"]) == singles[r][1], r
+        np.testing.assert_array_equal(ens.frame(slots[r]).flim_stack, singles[r][2])
+    ens.close()
+    cfg0 = _regions_adapt_replica(rs[0])
+    cfg0.use_adaptation = False
+    plain = EnsembleEngine(cfg0, capacity=400_000, max_replicas=4)
+    k = plain.add(cfg0)
+    plain.run(steps)
+    assert plain.counters(k)["n_catalytic"] != singles[rs[0]][1][0]
+    plain.close()
+
+
+def test_ensemble_refuses_a_replica_with_more_than_8_regions():
+    from ed_mrdt.config import ConfigError
+    cfg = _regions_adapt_replica(1)
+    ens = EnsembleEngine(cfg, capacity=100_000, max_replicas=2)
+    bad = _regions_adapt_replica(1)
+    bad.diffusion_field.regions = bad.diffusion_field.regions * 5
+    with pytest.raises(ConfigError):
+        ens.add(bad)
+    ens.close()
