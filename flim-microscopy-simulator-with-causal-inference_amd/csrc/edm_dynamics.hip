@@ -1071,17 +1071,17 @@ __device__ __forceinline__ bool cas_byte(int8_t* base, int idx, int8_t expect, i
     }
 }
 
-// Candidate binding edges A(ra) - B(rb): both alive and free, in the
-// (capped) pair list, within contact after the move, and accepted by their
-// per-pair uniform (dynamics.py:529-606).
+// Candidate binding edges A(ra) - B(rb) of particle t (of species ra):
+// both alive and free, in the (capped) pair list, within contact after the
+// move, and accepted by their per-pair uniform (dynamics.py:529-606).
+// Philox keys and edge keys use particle indices relative to C.kbase.
 template <typename Real>
-__global__ void k_bind_edges(int n, Cells<Real> C, Post<Real> P, int ra, int rb, Real contact2,
-                             Real prob, Draws dr, Rng rng, StepRef sref, uint32_t rxi,
-                             unsigned long long* ekey, int32_t* ea, int32_t* eb, unsigned int* ne,
-                             unsigned int cap) {
-    const uint32_t step = sref.get();
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+__device__ __forceinline__ void bind_edges_one(const Cells<Real>& C, const Post<Real>& P, int t,
+                                               int ra, int rb, Real contact2, Real prob,
+                                               const Draws& dr, const Rng& rng, uint32_t step,
+                                               uint32_t rxi, unsigned long long* ekey,
+                                               int32_t* ea, int32_t* eb, unsigned int* ne,
+                                               unsigned int cap) {
     if (C.sp[t] != ra || !(C.fl[t] & EDM_FLAG_ALIVE) || P.bond[t] >= 0) return;
     const Real xt = C.x[t], yt = C.y[t];
     const int it = C.id[t];
@@ -1111,7 +1111,8 @@ __global__ void k_bind_edges(int n, Cells<Real> C, Post<Real> P, int ra, int rb,
                 key = (unsigned long long)p;
                 uu = dr.base[dr.off + p];
             } else {
-                const uint32_t lo = (uint32_t)min(it, iu), hi = (uint32_t)max(it, iu);
+                const uint32_t lo = (uint32_t)(min(it, iu) - C.kbase);
+                const uint32_t hi = (uint32_t)(max(it, iu) - C.kbase);
                 key = ((unsigned long long)lo << 32) | hi;
                 uu = (double)Uni<Real>::first(rng.draw(step, lo, hi, (kTagBind << 16) | rxi));
             }
@@ -1127,18 +1128,31 @@ __global__ void k_bind_edges(int n, Cells<Real> C, Post<Real> P, int ra, int rb,
     }
 }
 
+template <typename Real>
+__global__ void k_bind_edges(int n, Cells<Real> C, Post<Real> P, int ra, int rb, Real contact2,
+                             Real prob, Draws dr, Rng rng, StepRef sref, uint32_t rxi,
+                             unsigned long long* ekey, int32_t* ea, int32_t* eb, unsigned int* ne,
+                             unsigned int cap) {
+    const uint32_t step = sref.get();
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    bind_edges_one(C, P, t, ra, rb, contact2, prob, dr, rng, step, rxi, ekey, ea, eb, ne, cap);
+}
+
 // Greedy matching in key order = the reference's sequential pass: an edge
 // is accepted when it is the smallest live edge at both endpoints; repeat
-// until no live edge is left.  One workgroup.
+// until no live edge is left.  One workgroup over one membrane's edges
+// (best: per slot).  ne[1] keeps the largest count of accepted edges that
+// exceeded the buffer, for the host to report.
 template <typename Real>
-__global__ __launch_bounds__(1024) void k_bind_match(Post<Real> P, const int32_t* __restrict__ id,
-                                                     int ra, int rb, int prod,
-                                                     const unsigned long long* ekey,
-                                                     const int32_t* ea, const int32_t* eb,
-                                                     unsigned int* ne, unsigned int cap,
-                                                     unsigned long long* best,
-                                                     unsigned long long* ctr, Grids<Real> G,
-                                                     int n_grids) {
+__device__ __forceinline__ void bind_match_block(const Post<Real>& P,
+                                                 const int32_t* __restrict__ id, int ra, int rb,
+                                                 int prod, const unsigned long long* ekey,
+                                                 const int32_t* ea, const int32_t* eb,
+                                                 unsigned int* ne, unsigned int cap,
+                                                 unsigned long long* best,
+                                                 unsigned long long* ctr, const Grids<Real>& G,
+                                                 int n_grids) {
     const unsigned int m = min(*ne, cap);
     // more accepted edges than the buffer holds: the matching below would
     // differ from the reference's sequential pass, so keep the largest such
@@ -1197,34 +1211,56 @@ __global__ __launch_bounds__(1024) void k_bind_match(Post<Real> P, const int32_t
     if (threadIdx.x == 0) *ne = 0u;  // ready for the next pass
 }
 
-// dynamics.py:610-654 apply_dissociation
+template <typename Real>
+__global__ __launch_bounds__(1024) void k_bind_match(Post<Real> P, const int32_t* __restrict__ id,
+                                                     int ra, int rb, int prod,
+                                                     const unsigned long long* ekey,
+                                                     const int32_t* ea, const int32_t* eb,
+                                                     unsigned int* ne, unsigned int cap,
+                                                     unsigned long long* best,
+                                                     unsigned long long* ctr, Grids<Real> G,
+                                                     int n_grids) {
+    bind_match_block(P, id, ra, rb, prod, ekey, ea, eb, ne, cap, best, ctr, G, n_grids);
+}
+
+// dynamics.py:610-654 apply_dissociation of particle t: a complex splits
+// into its two species at the complex's position.  Bonds hold particle
+// indices; the partner's slot is P.slot_of[partner - kbase], and the draw
+// key is the particle index relative to kbase.
+template <typename Real>
+__device__ __forceinline__ bool dissoc_one(int t, const int32_t* __restrict__ id,
+                                           const Post<Real>& P, int prod, int ra, int rb,
+                                           Real prob, const Draws& dr, const Rng& rng,
+                                           uint32_t step, uint32_t rxi, int kbase,
+                                           const Grids<Real>& G, int n_grids) {
+    if (!(P.fl[t] & EDM_FLAG_ALIVE) || P.sp[t] != prod || P.bond[t] < 0) return false;
+    const int it = id[t];
+    const double u = dr.base ? dr.base[dr.off + dr.n_pairs + it]
+                             : (double)Uni<Real>::first(rng.draw(
+                                   step, (uint32_t)(it - kbase), 0u, (kTagDissoc << 16) | rxi));
+    if (!(u < (double)prob)) return false;
+    const int s = P.slot_of[P.bond[t] - kbase];
+    P.sp[t] = (int8_t)ra;
+    P.sp[s] = (int8_t)rb;
+    P.fl[s] = P.fl[s] | EDM_FLAG_ALIVE;
+    P.xn[s] = P.xn[t];
+    P.yn[s] = P.yn[t];
+    P.bond[t] = -1;
+    P.bond[s] = -1;
+    count_update(G, n_grids, prod, P.xn[t], P.yn[t], -1, false);
+    count_update(G, n_grids, ra, P.xn[t], P.yn[t], +1, false);
+    count_update(G, n_grids, rb, P.xn[t], P.yn[t], +1, false);
+    return true;
+}
+
 template <typename Real>
 __global__ void k_dissoc(int n, const int32_t* __restrict__ id, Post<Real> P, int prod, int ra,
                          int rb, Real prob, Draws dr, Rng rng, StepRef sref, uint32_t rxi,
                          unsigned long long* ctr, Grids<Real> G, int n_grids) {
     const uint32_t step = sref.get();
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    bool hit = false;
-    if (t < n && (P.fl[t] & EDM_FLAG_ALIVE) && P.sp[t] == prod && P.bond[t] >= 0) {
-        const int it = id[t];
-        const double u = dr.base ? dr.base[dr.off + dr.n_pairs + it]
-                                 : (double)Uni<Real>::first(
-                                       rng.draw(step, (uint32_t)it, 0u, (kTagDissoc << 16) | rxi));
-        if (u < (double)prob) {
-            const int s = P.slot_of[P.bond[t]];
-            P.sp[t] = (int8_t)ra;
-            P.sp[s] = (int8_t)rb;
-            P.fl[s] = P.fl[s] | EDM_FLAG_ALIVE;
-            P.xn[s] = P.xn[t];
-            P.yn[s] = P.yn[t];
-            P.bond[t] = -1;
-            P.bond[s] = -1;
-            count_update(G, n_grids, prod, P.xn[t], P.yn[t], -1, false);
-            count_update(G, n_grids, ra, P.xn[t], P.yn[t], +1, false);
-            count_update(G, n_grids, rb, P.xn[t], P.yn[t], +1, false);
-            hit = true;
-        }
-    }
+    const bool hit = t < n && dissoc_one(t, id, P, prod, ra, rb, prob, dr, rng, step, rxi, 0, G,
+                                         n_grids);
     wave_count(&ctr[1], hit);
 }
 

@@ -23,7 +23,8 @@ static void ens_free(edm_ensemble* e) {
                     e->cand_x, e->cand_y, e->n_cand, e->ctr, e->frame_out, e->sig, e->eps, e->kh,
                     e->Dbase, e->d_pp, e->d_seg, e->d_tiles, e->noise_ctr, e->d_slot_list,
                     e->d_cats, e->d_pgrp, e->wl_pass, e->wl_dye, e->wl_rcnt,
-                    e->wl_wrep_pass, e->wl_wrep_dye, e->d_wl_chunks};
+                    e->wl_wrep_pass, e->wl_wrep_dye, e->d_wl_chunks, e->a.bond, e->a.bond_t,
+                    e->ekey, e->ea, e->eb, e->ne, e->best};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (auto& g : e->seg) {
@@ -31,6 +32,7 @@ static void ens_free(edm_ensemble* e) {
         if (g.rcdf) hipFree(g.rcdf);
         if (g.reg) hipFree(g.reg);
         if (g.w_adapt) hipFree(g.w_adapt);
+        if (g.slot_of) hipFree(g.slot_of);
     }
     ens_frames_free(e);
     if (e->stream) hipStreamDestroy(e->stream);
@@ -60,8 +62,9 @@ int edm_ens_create(const edm_params* p, const edm_photo_params* pp, const edm_im
         return ens_invalid("edm_ens_create: bad adaptation grid or species");
     for (int i = 0; i < p->n_reactions; ++i) {
         const edm_reaction& r = p->reactions[i];
-        if (r.kind == EDM_RX_BIND)
-            return ens_invalid("edm_ens_create: binding reactions are not batched");
+        if (r.kind == EDM_RX_BIND && (r.a < 0 || r.b < 0 || r.c < 0 || r.a >= p->n_species ||
+                                      r.b >= p->n_species || r.c >= p->n_species))
+            return ens_invalid("edm_ens_create: binding refers to a missing species");
         if (r.kind == EDM_RX_ADAPT && p->adapt_grid < 0)
             return ens_invalid("edm_ens_create: adaptation step without an adaptation grid");
         if (r.kind == EDM_RX_RECRUIT && r.use_adapt &&
@@ -182,6 +185,7 @@ int edm_ens_add(edm_ensemble* e, const edm_replica* r, const edm_host_state* s, 
             if (g->flim) hipFree(g->flim);
             if (g->reg) hipFree(g->reg);
             if (g->w_adapt) hipFree(g->w_adapt);
+            if (g->slot_of) hipFree(g->slot_of);
         }
     } guard{&g, true};
     g.active = 1;
@@ -215,8 +219,11 @@ int edm_ens_add(edm_ensemble* e, const edm_replica* r, const edm_host_state* s, 
     }
     for (int q = 0; q < r->n_reactions; ++q) {
         g.prob[q] = r->prob[q];
+        g.prob2[q] = r->prob2[q];
         g.maxr[q] = r->max_recruits[q];
     }
+    // binding models: the slot of each particle, written by every sort
+    if (e->has_bind) EDM_HIP(hipMalloc(&g.slot_of, (size_t)std::max(n, 1) * sizeof(int32_t)));
     // the CDF of density-biased recruitment, sized by this replica's grid
     g.rcdf = nullptr;
     {
@@ -270,6 +277,7 @@ int edm_ens_add(edm_ensemble* e, const edm_replica* r, const edm_host_state* s, 
     std::vector<uint8_t> fl(n);
     std::vector<int32_t> id(n);
     std::vector<uint16_t> rep(n, (uint16_t)k);
+    std::vector<int32_t> bond(n, -1);
     for (int i = 0; i < n; ++i) {
         x[i] = s->positions[2 * i];
         y[i] = s->positions[2 * i + 1];
@@ -281,6 +289,11 @@ int edm_ens_add(edm_ensemble* e, const edm_replica* r, const edm_host_state* s, 
         id[i] = g.kbase + i;
         if (s->ctmc_state) ctmc[i] = s->ctmc_state[i];
         if (s->dye_type_id) dye[i] = (int8_t)s->dye_type_id[i];
+        if (s->bond_partner && s->bond_partner[i] >= 0) {
+            if (!e->has_bind || s->bond_partner[i] >= n)
+                return ens_invalid("edm_ens_add: bond partner without binding, or out of range");
+            bond[i] = g.kbase + s->bond_partner[i];
+        }
     }
     hipStream_t st = e->stream;
     EDM_HIP(hipStreamSynchronize(st));
@@ -298,6 +311,8 @@ int edm_ens_add(edm_ensemble* e, const edm_replica* r, const edm_host_state* s, 
     EDM_HIP(hipMemcpy(e->a.ctmc + beg, ctmc.data(), n, hipMemcpyHostToDevice));
     EDM_HIP(hipMemcpy(e->a.dye + beg, dye.data(), n, hipMemcpyHostToDevice));
     EDM_HIP(hipMemcpy(e->rep + beg, rep.data(), n * 2, hipMemcpyHostToDevice));
+    if (e->has_bind)
+        EDM_HIP(hipMemcpy(e->a.bond + beg, bond.data(), n * 4, hipMemcpyHostToDevice));
     EDM_HIP(hipMemset(e->ctr + 8 * (size_t)k, 0, 8 * sizeof(unsigned long long)));
     EDM_HIP(hipMemset(e->noise_ctr + 2 * (size_t)k, 0, 2 * sizeof(unsigned long long)));
     int64_t n_alive = 0;
@@ -337,6 +352,8 @@ int edm_ens_remove(edm_ensemble* e, int32_t slot) {
     g.reg = nullptr;
     if (g.w_adapt) EDM_HIP(hipFree(g.w_adapt));
     g.w_adapt = nullptr;
+    if (g.slot_of) EDM_HIP(hipFree(g.slot_of));
+    g.slot_of = nullptr;
     g.active = 0;
     e->n_retired += g.n;
     e->draining.push_back(slot);
@@ -348,7 +365,7 @@ int edm_ens_run(edm_ensemble* e, int32_t n_steps, int32_t image) {
     if (!e || n_steps < 0) return ens_invalid("edm_ens_run: bad argument");
     for (int k = 0; k < n_steps; ++k)
         if (int rc = ens_step(e, image != 0)) return rc;
-    return 0;
+    return e->has_bind && n_steps > 0 ? ens_check_edges(e) : 0;
 }
 
 int edm_ens_frame(edm_ensemble* e, int32_t slot, uint32_t* flim, int64_t n_elems, int64_t* n_dark,
@@ -446,6 +463,9 @@ int edm_ens_download(edm_ensemble* e, int32_t slot, edm_host_state* s) {
     EDM_HIP(hipMemcpy(id.data(), e->a.id + beg, n * 4, hipMemcpyDeviceToHost));
     EDM_HIP(hipMemcpy(ctmc.data(), e->a.ctmc + beg, n, hipMemcpyDeviceToHost));
     EDM_HIP(hipMemcpy(dye.data(), e->a.dye + beg, n, hipMemcpyDeviceToHost));
+    std::vector<int32_t> bond(n, -1);
+    if (e->has_bind)
+        EDM_HIP(hipMemcpy(bond.data(), e->a.bond + beg, n * 4, hipMemcpyDeviceToHost));
     for (int t = 0; t < n; ++t) {
         const int i = id[t] - g.kbase;
         if (i < 0 || i >= n) {
@@ -460,7 +480,7 @@ int edm_ens_download(edm_ensemble* e, int32_t slot, edm_host_state* s) {
         if (s->is_bleached) s->is_bleached[i] = (fl[t] & EDM_FLAG_BLEACHED) ? 1 : 0;
         if (s->ctmc_state) s->ctmc_state[i] = ctmc[t];
         if (s->dye_type_id) s->dye_type_id[i] = dye[t];
-        if (s->bond_partner) s->bond_partner[i] = -1;
+        if (s->bond_partner) s->bond_partner[i] = bond[t] >= 0 ? bond[t] - g.kbase : -1;
     }
     return 0;
 }

@@ -127,12 +127,20 @@ __global__ void k_ens_bin(int n, const Real* __restrict__ x, const Real* __restr
     if (live) rank[s] = base + (uint32_t)(lane - rs);
 }
 
-template <typename Real>
+// kBonds (binding models): the bonds move with their particles, and each
+// live replica's slot_of records where its particles went.
+template <typename Real, bool kBonds>
 __global__ void k_ens_gather(int n, const uint32_t* __restrict__ perm, DevArrays a,
-                             const uint16_t* __restrict__ rep, uint16_t* __restrict__ rep_t) {
+                             const uint16_t* __restrict__ rep, uint16_t* __restrict__ rep_t,
+                             const EnsSeg* __restrict__ segs) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const uint32_t s = perm[t];
+    if constexpr (kBonds) {
+        a.bond_t[t] = a.bond[s];
+        const EnsSeg& g = segs[rep[s]];
+        if (g.active && g.slot_of) g.slot_of[a.id[s] - g.kbase] = t;
+    }
     ((Real*)a.xt)[t] = ((const Real*)a.x)[s];
     ((Real*)a.yt)[t] = ((const Real*)a.y)[s];
     a.sp_t[t] = a.sp[s];
@@ -170,6 +178,7 @@ struct EnsPool {
     const uint32_t* start;
     int32_t *counts, *delta;
     int ns, n_grids;
+    int32_t* bond;  // binding models: partner particle index per slot, else nullptr
 };
 
 template <typename Real>
@@ -317,6 +326,104 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFp32Waves<
     const int pbeg = (int)Q.start[g.coff];
     append_candidates(cand, t, cid + pbeg, cslot + pbeg, cxo + pbeg, cyo + pbeg, nc + W.r);
     passes_count<Real>(det, conv, ctr + 8 * W.r);
+}
+
+// Binding (dynamics.py:529-654) in the batch.  The candidate edges of a
+// replica go to its own region of the edge buffers (entries 2 * first
+// slot .., at most 2 n; a larger count is recorded for the host to report),
+// one workgroup per replica then matches them greedily in key order, and
+// every complex may dissociate, its partner found through the replica's
+// slot_of -- the single-membrane bodies on the replica's views.
+template <typename Real>
+__device__ __forceinline__ Post<Real> ens_post_bonds(const EnsPool<Real>& Q, const EnsSeg& g) {
+    Post<Real> P = ens_post(Q);
+    P.bond = Q.bond;
+    P.slot_of = g.slot_of;
+    return P;
+}
+
+// Own slots of strip T: its columns' rows cy0 .. cy0 + rows - 1, one
+// contiguous range per column.
+template <typename Real>
+__device__ __forceinline__ void ens_own_range(const Cells<Real>& C, const EnsTile& T, int col,
+                                              int& beg, int& end) {
+    const int rows = min(T.rows, C.ncy - T.cy0);
+    const int c0 = (T.cx + col) * C.ncy + T.cy0;
+    beg = (int)C.start[c0];
+    end = (int)C.start[c0 + rows];
+}
+
+template <typename Real>
+__global__ __launch_bounds__(256) void k_ens_bind_edges(const EnsTile* __restrict__ tiles,
+                                                        int ntiles,
+                                                        const EnsSeg* __restrict__ segs,
+                                                        EnsPool<Real> Q, int ra, int rb,
+                                                        Real contact2, int rxi,
+                                                        unsigned long long* ekey, int32_t* ea,
+                                                        int32_t* eb, unsigned int* ne,
+                                                        uint32_t gstep) {
+    const int tile = xcd_tile(ntiles);
+    if (tile >= ntiles) return;
+    const EnsTile T = tiles[tile];
+    const EnsSeg& g = segs[T.r];
+    const Cells<Real> C = ens_cells<Real>(g, Q.x, Q.y, Q.sp, Q.fl, Q.id, Q.start);
+    const Post<Real> P = ens_post_bonds(Q, g);
+    const size_t base = 2 * (size_t)C.start[0];
+    const Draws dr{nullptr, 0, 0, 0};
+    for (int col = 0; col < T.ncol; ++col) {
+        int beg, end;
+        ens_own_range(C, T, col, beg, end);
+        for (int t = beg + (int)threadIdx.x; t < end; t += (int)blockDim.x)
+            bind_edges_one(C, P, t, ra, rb, contact2, (Real)g.prob[rxi], dr, ens_rng(g),
+                           gstep - g.step0, (uint32_t)rxi, ekey + base, ea + base, eb + base,
+                           ne + 2 * T.r, (unsigned int)(2 * g.n));
+    }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(1024) void k_ens_bind_match(const EnsSeg* __restrict__ segs,
+                                                         EnsPool<Real> Q, int ra, int rb,
+                                                         int prod,
+                                                         const unsigned long long* ekey,
+                                                         const int32_t* ea, const int32_t* eb,
+                                                         unsigned int* ne,
+                                                         unsigned long long* best,
+                                                         unsigned long long* ctr) {
+    const int r = (int)blockIdx.x;
+    const EnsSeg& g = segs[r];
+    if (!g.active) return;
+    const size_t base = 2 * (size_t)Q.start[g.coff];
+    const Grids<Real> G = ens_grids<Real>(g, Q.counts, Q.delta, Q.ns);
+    bind_match_block(ens_post_bonds(Q, g), Q.id, ra, rb, prod, ekey + base, ea + base,
+                     eb + base, ne + 2 * r, (unsigned int)(2 * g.n), best, ctr + 8 * r, G,
+                     Q.n_grids);
+}
+
+template <typename Real>
+__global__ __launch_bounds__(256) void k_ens_dissoc(const EnsTile* __restrict__ tiles, int ntiles,
+                                                    const EnsSeg* __restrict__ segs,
+                                                    EnsPool<Real> Q, int prod, int ra, int rb,
+                                                    int rxi, unsigned long long* ctr,
+                                                    uint32_t gstep) {
+    const int tile = xcd_tile(ntiles);
+    if (tile >= ntiles) return;
+    const EnsTile T = tiles[tile];
+    const EnsSeg& g = segs[T.r];
+    const Cells<Real> C = ens_cells<Real>(g, Q.x, Q.y, Q.sp, Q.fl, Q.id, Q.start);
+    const Post<Real> P = ens_post_bonds(Q, g);
+    const Grids<Real> G = ens_grids<Real>(g, Q.counts, Q.delta, Q.ns);
+    const Draws dr{nullptr, 0, 0, 0};
+    for (int col = 0; col < T.ncol; ++col) {
+        int beg, end;
+        ens_own_range(C, T, col, beg, end);
+        for (int t0 = beg; t0 < end; t0 += (int)blockDim.x) {  // uniform
+            const int t = t0 + (int)threadIdx.x;
+            const bool hit = t < end && dissoc_one(t, Q.id, P, prod, ra, rb, (Real)g.prob2[rxi],
+                                                   dr, ens_rng(g), gstep - g.step0,
+                                                   (uint32_t)rxi, g.kbase, G, Q.n_grids);
+            wave_count(&ctr[8 * T.r + 1], hit);
+        }
+    }
 }
 
 // Index-order recruitment cap of one recruitment reaction, one block per
@@ -504,6 +611,15 @@ static int ens_create_t(edm_ensemble* e) {
         dalloc(&e->noise_ctr, 2 * (size_t)e->rmax) ||
         dalloc(&e->d_seg, e->rmax))
         return EDM_EHIP;
+    for (const auto& r : e->rx) e->has_bind = e->has_bind || r.kind == EDM_RX_BIND;
+    if (e->has_bind) {
+        if (dalloc(&e->a.bond, cap) || dalloc(&e->a.bond_t, cap) || dalloc(&e->ekey, 2 * cap) ||
+            dalloc(&e->ea, 2 * cap) || dalloc(&e->eb, 2 * cap) ||
+            dalloc(&e->ne, 2 * (size_t)e->rmax) || dalloc(&e->best, cap))
+            return EDM_EHIP;
+        EDM_HIP(hipMemset(e->a.bond, 0xff, (size_t)cap * sizeof(int32_t)));
+        EDM_HIP(hipMemset(e->ne, 0, 2 * (size_t)e->rmax * sizeof(unsigned int)));
+    }
     // each replica's region of a list is padded to whole waves
     e->wl_cap = cap + 64 * (int64_t)e->rmax;
     if (dalloc(&e->wl_pass, e->wl_cap) || dalloc(&e->wl_dye, e->wl_cap) ||
@@ -745,8 +861,9 @@ static int ens_step_t(edm_ensemble* e) {
                                (const int32_t*)e->a.id);
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, 1,
                            (unsigned int*)(e->cell_cnt + e->n_cells));
-        hipLaunchKernelGGL(k_ens_gather<Real>, dim3(nblk(n)), dim3(kBlock), 0, st, n,
-                           (const uint32_t*)e->perm, e->a, (const uint16_t*)e->rep, e->rep_t);
+        hipLaunchKernelGGL((e->has_bind ? k_ens_gather<Real, true> : k_ens_gather<Real, false>),
+                           dim3(nblk(n)), dim3(kBlock), 0, st, n, (const uint32_t*)e->perm, e->a,
+                           (const uint16_t*)e->rep, e->rep_t, (const EnsSeg*)e->d_seg);
         // the gather wrote the new order into the _t arrays (swapped below)
         if (!e->wl_chunks.empty())
             hipLaunchKernelGGL(k_ens_wl_build, dim3((unsigned)e->wl_chunks.size()), dim3(256), 0,
@@ -760,6 +877,7 @@ static int ens_step_t(edm_ensemble* e) {
         std::swap(e->a.id, e->a.id_t);
         std::swap(e->a.ctmc, e->a.ctmc_t);
         std::swap(e->a.dye, e->a.dye_t);
+        std::swap(e->a.bond, e->a.bond_t);
         std::swap(e->rep, e->rep_t);
     }
     // particles of removed replicas now sit after every live one; their
@@ -785,6 +903,7 @@ static int ens_step_t(edm_ensemble* e) {
     Q.delta = e->delta;
     Q.ns = p.n_species;
     Q.n_grids = (int)e->grids.size();
+    Q.bond = e->a.bond;
 
     ForceParams<Real> F;
     F.sig = (const Real*)e->sig;
@@ -819,7 +938,19 @@ static int ens_step_t(edm_ensemble* e) {
 
     for (size_t ri = 0; ri < e->rx.size(); ++ri) {
         const edm_reaction& r = e->rx[ri];
-        if (r.kind == EDM_RX_ADAPT) {
+        if (r.kind == EDM_RX_BIND) {
+            const Real c2 = (Real)r.contact * (Real)r.contact;
+            hipLaunchKernelGGL(k_ens_bind_edges<Real>, tgrid, dim3(256), 0, st,
+                               (const EnsTile*)e->d_tiles, ntiles, (const EnsSeg*)e->d_seg, Q,
+                               r.a, r.b, c2, (int)ri, e->ekey, e->ea, e->eb, e->ne, gstep);
+            hipLaunchKernelGGL(k_ens_bind_match<Real>, dim3(e->rmax), dim3(1024), 0, st,
+                               (const EnsSeg*)e->d_seg, Q, r.a, r.b, r.c,
+                               (const unsigned long long*)e->ekey, (const int32_t*)e->ea,
+                               (const int32_t*)e->eb, e->ne, e->best, e->ctr);
+            hipLaunchKernelGGL(k_ens_dissoc<Real>, tgrid, dim3(256), 0, st,
+                               (const EnsTile*)e->d_tiles, ntiles, (const EnsSeg*)e->d_seg, Q,
+                               r.c, r.a, r.b, (int)ri, e->ctr, gstep);
+        } else if (r.kind == EDM_RX_ADAPT) {
             const int ag = p.adapt_grid;
             int mc = 1;
             for (int q = 0; q < e->rmax; ++q)
@@ -886,6 +1017,26 @@ static int ens_step_t(edm_ensemble* e) {
     std::swap(e->a.y, e->a.yn);
     e->gstep += 1;
     EDM_HIP(hipGetLastError());
+    return 0;
+}
+
+// Binding edges that did not fit a replica's region: the matching of that
+// step would differ from the reference's sequential pass, so the run is
+// reported as incomplete (EDM_ECAPACITY) and the record cleared.
+int ens_check_edges(edm_ensemble* e) {
+    std::vector<unsigned int> ne(2 * (size_t)e->rmax);
+    EDM_HIP(hipMemcpyAsync(ne.data(), e->ne, ne.size() * sizeof(unsigned int),
+                           hipMemcpyDeviceToHost, e->stream));
+    EDM_HIP(hipStreamSynchronize(e->stream));
+    for (int r = 0; r < e->rmax; ++r)
+        if (ne[2 * r + 1] > 0) {
+            EDM_HIP(hipMemset(e->ne, 0, ne.size() * sizeof(unsigned int)));
+            set_error("edm_ens_run: replica slot " + std::to_string(r) + " accepted " +
+                      std::to_string(ne[2 * r + 1]) +
+                      " candidate binding edges in a step, more than its edge region holds; "
+                      "the outcome of that step is incomplete");
+            return EDM_ECAPACITY;
+        }
     return 0;
 }
 

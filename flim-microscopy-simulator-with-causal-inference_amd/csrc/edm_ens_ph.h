@@ -11,6 +11,7 @@ __global__ __launch_bounds__(256) void k_ens_photons(int n_waves, const int32_t*
                                                      const edm_photo_params* pp, const Real* x,
                                                      const Real* y, uint8_t* fl, const int8_t* dye,
                                                      int8_t* ctmc, const int32_t* __restrict__ id,
+                                                     const int32_t* __restrict__ bond,
                                                      unsigned long long* ctr, uint32_t gstep) {
     if ((int)blockIdx.x * (int)(blockDim.x >> 6) +
             __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) >= n_waves)
@@ -23,8 +24,10 @@ __global__ __launch_bounds__(256) void k_ens_photons(int n_waves, const int32_t*
     Emit<Real> E{0, 0, 0u, 0, 0, 0, 0, (Real)0, (Real)0};
     if (j < min(g.wl_n[1], (int)rcnt[2 * W.r + 1])) {
         const int t = list[g.wl_off[1] + j];
-        E = emit_pss<Real>(pp, t, (uint32_t)(id[t] - g.kbase), fl, dye, ctmc, nullptr, nullptr,
-                           true, 0.0, -1, rng, gstep - g.step0, emitted);
+        // bond-mode FRET of intramolecular maps from the bonds (binding
+        // models); without bonds no donor is quenched
+        E = emit_pss<Real>(pp, t, (uint32_t)(id[t] - g.kbase), fl, dye, ctmc, bond, nullptr,
+                           bond == nullptr, 0.0, -1, rng, gstep - g.step0, emitted);
     }
     // the wave's emitters are all of replica W.r: its geometry, stack and draws
     wave_photons<Real, kWin>(E, &g.ip, x, y, g.flim, nullptr, rng, gstep - g.step0, detected);
@@ -38,7 +41,7 @@ static void launch_ens_photons(edm_ensemble* e, int nt) {
                        (const uint32_t*)e->wl_rcnt, (const EnsSeg*)e->d_seg,
                        (const edm_photo_params*)e->d_pp, (const Real*)e->a.x,
                        (const Real*)e->a.y, e->a.fl, (const int8_t*)e->a.dye, e->a.ctmc,
-                       (const int32_t*)e->a.id, e->ctr, e->gstep);
+                       (const int32_t*)e->a.id, (const int32_t*)e->a.bond, e->ctr, e->gstep);
 }
 
 int ens_photons(edm_ensemble* e) {

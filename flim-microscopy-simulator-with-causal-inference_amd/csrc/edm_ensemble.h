@@ -55,6 +55,8 @@ struct EnsSeg {
     int32_t n_regions;  // circular D(x) regions of this replica
     void* reg;          // their centres x, y, radii and D factors (Real[4][8]), or nullptr
     double* w_adapt;    // this replica's adaptation field on its adaptation grid, or nullptr
+    double prob2[kEnsMaxRx];  // BIND reactions: dissociation probability
+    int32_t* slot_of;   // binding models: slot of each particle (index - kbase)
 };
 
 static constexpr int kEnsMaxRegions = 8;
@@ -132,6 +134,15 @@ struct edm_ensemble {
     int pair_below = 128;
     int cat_lanes = 16;
     int cat_walk = 0;  // 1: catalysis runs walk their enzymes over the pass work list
+    // binding: bonds in the pool (partner particle indices), candidate edges
+    // in per-replica regions (entries 2 * first slot .., 2 * n of them),
+    // per replica the edge count and the largest overflow seen, the
+    // matching's per-slot minimum key
+    bool has_bind = false;
+    unsigned long long* ekey = nullptr;
+    int32_t *ea = nullptr, *eb = nullptr;
+    unsigned int* ne = nullptr;  // [rmax][2]
+    unsigned long long* best = nullptr;
     EnsTile* d_tiles = nullptr;
     int64_t tile_cap = 0;
     int64_t n_total = 0;  // particles in the pool (live replicas first after a sort)
@@ -198,6 +209,7 @@ namespace edm {
 int ens_create(edm_ensemble* e);
 int ens_layout(edm_ensemble* e);
 int ens_step(edm_ensemble* e, bool image);
+int ens_check_edges(edm_ensemble* e);
 int ens_photons(edm_ensemble* e);
 int ens_frame_noise(edm_ensemble* e, int slot, int64_t* n_dark, int64_t* n_ap);
 int ens_frame_reset(edm_ensemble* e, int slot);

@@ -99,7 +99,7 @@ class Replica(ctypes.Structure):
                 ("img_R", c_int32), ("n_reactions", c_int32), ("img_pixel", c_double),
                 ("img_offx", c_double), ("img_offy", c_double), ("n_regions", c_int32),
                 ("pad_", c_int32), ("reg_cx", c_double * 8), ("reg_cy", c_double * 8),
-                ("reg_r", c_double * 8), ("reg_f", c_double * 8)]
+                ("reg_r", c_double * 8), ("reg_f", c_double * 8), ("prob2", c_double * 16)]
 
 
 class Census(ctypes.Structure):

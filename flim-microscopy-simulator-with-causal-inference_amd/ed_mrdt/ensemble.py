@@ -124,6 +124,7 @@ class EnsembleEngine:
             r.grids[i] = _hip.Grid(nx=gx, ny=gy, sx=sx, sy=sy)
         for i, rx in enumerate(plan.reactions):
             r.prob[i] = rx.prob
+            r.prob2[i] = rx.prob2
             r.max_recruits[i] = rx.max_recruits
         r.n_reactions = len(plan.reactions)
         shape = None

@@ -541,7 +541,7 @@ int edm_stats_flim(edm_membrane *m, int32_t channel, int32_t harmonic, double *o
  * (model adapt_* fields) is kept per replica on its own grid.  Philox
  * draws only (keys relative to each replica's first particle and admission
  * step), so a replica evolves exactly as it does alone.  Not batched:
- * binding, autocatalysis, proximity or intermolecular FRET.             */
+ * autocatalysis, proximity or intermolecular FRET.                      */
 typedef struct edm_replica {
     int32_t n;                  /* particles (alive + dormant)              */
     int32_t ncx, ncy;           /* cell grid (CellList, dynamics.py:109-114) */
@@ -558,6 +558,8 @@ typedef struct edm_replica {
      * dynamics.py:441-519), at most 8; the smoothing width is the model's */
     int32_t n_regions, pad_;
     double reg_cx[8], reg_cy[8], reg_r[8], reg_f[8];
+    double prob2[16];           /* per model reaction: dissociation probability
+                                 * of a BIND reaction (P_off)               */
 } edm_replica;
 
 typedef struct edm_ensemble edm_ensemble;

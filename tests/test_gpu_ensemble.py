@@ -293,3 +293,69 @@ def test_ensemble_refuses_a_replica_with_more_than_8_regions():
     with pytest.raises(ConfigError):
         ens.add(bad)
     ens.close()
+
+
+def _single_run(cfg, steps):
+    st = ParticleState(cfg)
+    eng = DynamicsEngine(cfg, st, precision="fp32", rng="philox", pair_cap=False)
+    ph = PhotonSimulator(cfg, st, min(1.0, cfg.scan.dwell_time / cfg.scan.frame_interval))
+    mic = VirtualMicroscope(cfg, st)
+    for _ in range(steps):
+        eng.step()
+        mic.process_photons(ph.evolve_one_bd_step())
+    fr = mic.finalize_frame()
+    out = dict(positions=st.positions.copy(), species_id=st.species_id.copy(),
+               is_alive=st.is_alive.copy(), is_bleached=st.is_bleached.copy(),
+               ctmc_state=st.ctmc_state.copy(), bond_partner=st.bond_partner.copy())
+    ctr = (eng.n_bindings, eng.n_dissociations, eng.n_catalytic, eng.n_recruited,
+           eng.n_detached, eng.n_converted)
+    eng.close()
+    return out, ctr, fr.flim_stack.copy()
+
+
+def test_binding_replicas_match_single_membrane_path():
+    """Binding and dissociation (PHAkt + PIP3 <-> PHAkt_PIP3, dynamics.py:
+    529-654) with intramolecular bond FRET and circular D(x) regions in the
+    batch: each replica of a three-replica ensemble (different boxes and
+    seeds) evolves bit for bit as on the single-membrane path (FP32,
+    Philox): positions, species, flags, bonds, event counters and the frame."""
+    steps = 60
+    cfgs = {k: workloads.binding_fret(seed=s, side=side)
+            for k, (s, side) in enumerate(((3, 4.0), (4, 5.0), (7, 6.0)))}
+    for c in cfgs.values():
+        for r in c.reactions:
+            r.koff = 40.0  # complexes dissociate within the run too
+    singles = {k: _single_run(c, steps) for k, c in cfgs.items()}
+    ens = EnsembleEngine(cfgs[0], capacity=200_000, max_replicas=4)
+    slots = {k: ens.add(c) for k, c in cfgs.items()}
+    ens.run(steps)
+    for k in cfgs:
+        s = ens.state(slots[k])
+        _same(s, singles[k][0], f"replica {k}")
+        np.testing.assert_array_equal(s["bond_partner"], singles[k][0]["bond_partner"])
+        c = ens.counters(slots[k])
+        got = (c["n_bindings"], c["n_dissociations"], c["n_catalytic"], c["n_recruited"],
+               c["n_detached"], c["n_converted"])
+        assert got == singles[k][1], (k, got, singles[k][1])
+        np.testing.assert_array_equal(ens.frame(slots[k]).flim_stack, singles[k][2])
+    assert all(v[1][0] > 0 and v[1][1] > 0 for v in singles.values()), \
+        [v[1] for v in singles.values()]
+    ens.close()
+
+
+def test_ensemble_binding_edge_overflow_is_reported():
+    """A replica whose step accepts more candidate binding edges than its
+    edge region holds (2 n) makes the run fail with EDM_ECAPACITY instead of
+    matching an incomplete set."""
+    from ed_mrdt._hip import HipError
+    cfg = workloads.binding_fret(seed=5, side=2.0, regions=False)
+    pops = cfg.initial_populations
+    pops["PIP2"], pops["PIP3"], pops["PHAkt"] = 800, 1600, 1400
+    for r in cfg.reactions:
+        r.kon = 1.0e4
+        r.contact_radius = 150.0
+    ens = EnsembleEngine(cfg, capacity=50_000, max_replicas=2)
+    ens.add(cfg)
+    with pytest.raises(HipError, match="edge region"):
+        ens.run(3)
+    ens.close()
