@@ -83,8 +83,8 @@ int edm_ens_create(const edm_params* p, const edm_photo_params* pp, const edm_im
     }
     if (pp) {
         for (int j = 0; j < pp->n_fret_maps; ++j)
-            if (!pp->fm_skip[j] && (pp->fm_proximity[j] || !pp->fm_intra[j]))
-                return ens_invalid("edm_ens_create: only intramolecular bond FRET is batched");
+            if (!pp->fm_skip[j] && pp->fm_proximity[j])
+                return ens_invalid("edm_ens_create: proximity FRET is not batched");
     }
     edm_ensemble* e = new edm_ensemble();
     e->p = *p;

@@ -24,10 +24,11 @@ __global__ __launch_bounds__(256) void k_ens_photons(int n_waves, const int32_t*
     Emit<Real> E{0, 0, 0u, 0, 0, 0, 0, (Real)0, (Real)0};
     if (j < min(g.wl_n[1], (int)rcnt[2 * W.r + 1])) {
         const int t = list[g.wl_off[1] + j];
-        // bond-mode FRET of intramolecular maps from the bonds (binding
-        // models); without bonds no donor is quenched
-        E = emit_pss<Real>(pp, t, (uint32_t)(id[t] - g.kbase), fl, dye, ctmc, bond, nullptr,
-                           bond == nullptr, 0.0, -1, rng, gstep - g.step0, emitted);
+        // bond-mode FRET from the bonds (binding models; the acceptor of an
+        // intermolecular map through the replica's slot_of); without bonds
+        // no donor is quenched
+        E = emit_pss<Real>(pp, t, (uint32_t)(id[t] - g.kbase), fl, dye, ctmc, bond, g.slot_of,
+                           bond == nullptr, 0.0, -1, rng, gstep - g.step0, emitted, g.kbase);
     }
     // the wave's emitters are all of replica W.r: its geometry, stack and draws
     wave_photons<Real, kWin>(E, &g.ip, x, y, g.flim, nullptr, rng, gstep - g.step0, detected);

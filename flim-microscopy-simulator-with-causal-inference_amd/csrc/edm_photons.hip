@@ -430,9 +430,11 @@ __device__ __forceinline__ void tcspc_add(const edm_image_params* ip, uint32_t* 
 // ----------------------------------------------------------------------
 // Bond-mode FRET rate of one donor, all maps in config order (later maps
 // overwrite earlier ones, as update_fret_rates does; photophysics.py:529-588).
+// Bonds hold particle indices; the acceptor's slot is slot_of[index - kbase].
 __device__ __forceinline__ void fret_inline(const edm_photo_params* pp, int t, const uint8_t* fl,
                                             const int8_t* dye, const int32_t* bond,
-                                            const int32_t* slot_of, double& kf, int& acc) {
+                                            const int32_t* slot_of, double& kf, int& acc,
+                                            int kbase = 0) {
     kf = 0.0;
     acc = -1;
     if (!(fl[t] & EDM_FLAG_DYE) || (fl[t] & EDM_FLAG_BLEACHED)) return;
@@ -444,7 +446,7 @@ __device__ __forceinline__ void fret_inline(const edm_photo_params* pp, int t, c
             kf = pp->fm_kfret[j];
             acc = -1;
         } else if (slot_of) {
-            const int s = slot_of[partner];
+            const int s = slot_of[partner - kbase];
             if ((fl[s] & EDM_FLAG_DYE) && dye[s] == pp->fm_acceptor[j]) {
                 kf = pp->fm_kfret[j];
                 acc = partner;
@@ -537,7 +539,7 @@ __device__ __forceinline__ Emit<Real> emit_pss(const edm_photo_params* pp, int t
                                                uint8_t* fl, const int8_t* dye, int8_t* ctmc,
                                                const int32_t* bond, const int32_t* slot_of,
                                                bool have_kf, double kf_in, int acc_in, Rng rng,
-                                               uint32_t step, int& emitted) {
+                                               uint32_t step, int& emitted, int kbase = 0) {
     Emit<Real> E{0, 0, key, t, 0, t, 0, (Real)0, (Real)0};
     uint8_t f = fl[t];
     int8_t c = ctmc[t];
@@ -547,7 +549,7 @@ __device__ __forceinline__ Emit<Real> emit_pss(const edm_photo_params* pp, int t
         kf = kf_in;
         aid = acc_in;
     } else {
-        fret_inline(pp, t, fl, dye, bond, slot_of, kf, aid);
+        fret_inline(pp, t, fl, dye, bond, slot_of, kf, aid, kbase);
     }
     const U4 r0 = rng.draw(step, key, 0u, kTagPhotonPois);
     const double up = u01d(r0.x, r0.y), ut = u01d(r0.z, r0.w);
@@ -559,7 +561,7 @@ __device__ __forceinline__ Emit<Real> emit_pss(const edm_photo_params* pp, int t
         E.ch_d = o.ty;
         E.tau_d = (Real)o.tau;
         if (o.nf > 0) {
-            E.es_a = slot_of[aid];
+            E.es_a = slot_of[aid - kbase];
             E.ch_a = dye[E.es_a];
             E.tau_a = (E.ch_a >= 0 && E.ch_a < pp->n_fluoro) ? (Real)pp->tau[E.ch_a] : (Real)o.tau;
         }

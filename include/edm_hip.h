@@ -541,7 +541,7 @@ int edm_stats_flim(edm_membrane *m, int32_t channel, int32_t harmonic, double *o
  * (model adapt_* fields) is kept per replica on its own grid.  Philox
  * draws only (keys relative to each replica's first particle and admission
  * step), so a replica evolves exactly as it does alone.  Not batched:
- * autocatalysis, proximity or intermolecular FRET.                      */
+ * autocatalysis, proximity FRET.                                         */
 typedef struct edm_replica {
     int32_t n;                  /* particles (alive + dormant)              */
     int32_t ncx, ncy;           /* cell grid (CellList, dynamics.py:109-114) */
