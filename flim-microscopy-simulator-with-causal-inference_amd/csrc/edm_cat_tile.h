@@ -183,15 +183,15 @@ static constexpr unsigned long long kAcNever = ~0ull;
 static constexpr unsigned long long kAcIndex = (1ull << 61) - 1ull;  // edge key: pair index
 static constexpr int kAcFwd = 62, kAcBwd = 61;  // edge key bits: a converts b, b converts a
 
-// One thread per particle of the enzyme or substrate species; each pair is
-// listed once, from its lower particle index (ea = that slot).
+// The accepted pairs of particle t (of the enzyme or substrate species):
+// each pair is listed once, from its lower particle index (ea = that slot).
 template <typename Real>
-__global__ void k_autocat_edges(int n, Cells<Real> C, Post<Real> P, CatArgs<Real> A,
-                                Grids<Real> G, unsigned long long* ekey, int32_t* ea,
-                                int32_t* eb, unsigned int* ne, unsigned int cap) {
-    const uint32_t step = A.step + (A.step_base ? *A.step_base : 0u);
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+__device__ __forceinline__ void autocat_edges_one(const Cells<Real>& C, const Post<Real>& P,
+                                                  const CatArgs<Real>& A, uint32_t step,
+                                                  const Grids<Real>& G, int t,
+                                                  unsigned long long* ekey, int32_t* ea,
+                                                  int32_t* eb, unsigned int* ne,
+                                                  unsigned int cap) {
     const int E = A.enz[0], S = A.sub;
     const int st = P.sp[t];
     if ((st != E && st != S) || !(C.fl[t] & EDM_FLAG_ALIVE)) return;
@@ -231,16 +231,27 @@ __global__ void k_autocat_edges(int n, Cells<Real> C, Post<Real> P, CatArgs<Real
     }
 }
 
-// One workgroup: the rounds, the conversions, and t reset to "never" on
-// the edges' particles (tconv holds "never" for every slot between passes).
 template <typename Real>
-__global__ __launch_bounds__(1024) void k_autocat_resolve(Post<Real> P, CatArgs<Real> A,
-                                                          const unsigned long long* ekey,
-                                                          const int32_t* ea, const int32_t* eb,
-                                                          unsigned int* ne, unsigned int cap,
-                                                          unsigned long long* tconv,
-                                                          unsigned long long* ctr, Grids<Real> G,
-                                                          int n_grids) {
+__global__ void k_autocat_edges(int n, Cells<Real> C, Post<Real> P, CatArgs<Real> A,
+                                Grids<Real> G, unsigned long long* ekey, int32_t* ea,
+                                int32_t* eb, unsigned int* ne, unsigned int cap) {
+    const uint32_t step = A.step + (A.step_base ? *A.step_base : 0u);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    autocat_edges_one(C, P, A, step, G, t, ekey, ea, eb, ne, cap);
+}
+
+// One workgroup over one membrane's accepted pairs: the rounds, the
+// conversions, and t reset to "never" on the edges' particles (tconv holds
+// "never" for every slot between passes).
+template <typename Real>
+__device__ __forceinline__ void autocat_resolve_block(const Post<Real>& P, const CatArgs<Real>& A,
+                                                      const unsigned long long* ekey,
+                                                      const int32_t* ea, const int32_t* eb,
+                                                      unsigned int* ne, unsigned int cap,
+                                                      unsigned long long* tconv,
+                                                      unsigned long long* ctr,
+                                                      const Grids<Real>& G, int n_grids) {
     const unsigned int m = min(*ne, cap);
     // more accepted pairs than the buffer holds: the outcome would differ
     // from the reference's loop, so keep the count in ne[1] for the host to
@@ -290,6 +301,17 @@ __global__ __launch_bounds__(1024) void k_autocat_resolve(Post<Real> P, CatArgs<
     }
     __syncthreads();
     if (threadIdx.x == 0) *ne = 0u;  // ready for the next pass
+}
+
+template <typename Real>
+__global__ __launch_bounds__(1024) void k_autocat_resolve(Post<Real> P, CatArgs<Real> A,
+                                                          const unsigned long long* ekey,
+                                                          const int32_t* ea, const int32_t* eb,
+                                                          unsigned int* ne, unsigned int cap,
+                                                          unsigned long long* tconv,
+                                                          unsigned long long* ctr, Grids<Real> G,
+                                                          int n_grids) {
+    autocat_resolve_block(P, A, ekey, ea, eb, ne, cap, tconv, ctr, G, n_grids);
 }
 
 // Appends v to the block's LDS queue for the lanes with `want` (one atomic

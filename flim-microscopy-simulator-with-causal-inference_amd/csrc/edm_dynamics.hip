@@ -44,6 +44,7 @@ struct Cells {
     int8_t cap_gate;        // +1: the launch runs only when the cap truncates, -1: only when not
     int kbase;  // first particle index of this membrane (0 unless batched); draw keys
                 // use the index relative to it
+    int poff_base;  // pair offsets of particle index i at poff[i + poff_base]
     int tile_rows;  // cell rows per strip of the tiled kernels (1..kTileRows)
     bool sub_rows;  // force strips enumerate candidates by force sub-row
 };
@@ -123,7 +124,7 @@ __device__ __forceinline__ bool pair_in_list(const Cells<Real>& C, int sa, int s
 
 template <typename Real>
 __device__ __forceinline__ int64_t pair_index(const Cells<Real>& C, int sa, int sb) {
-    return (int64_t)C.poff[C.id[sa]] + pair_rank(C, sa, sb);
+    return (int64_t)C.poff[C.id[sa] + C.poff_base] + pair_rank(C, sa, sb);
 }
 
 // Density of a coarse-grid cell holding k sensor particles, as the
@@ -516,9 +517,7 @@ __device__ __forceinline__ void wl_build_block(int beg, int end, const PredP& in
 // partners j > i with d^2 < cutoff^2 (dynamics.py:237-276; dormant
 // particles included, as in the reference).
 template <typename Real>
-__global__ void k_pair_count(int n, Cells<Real> C, uint64_t* __restrict__ pcnt) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+__device__ __forceinline__ uint64_t pair_count_one(const Cells<Real>& C, int t) {
     const Real xt = C.x[t], yt = C.y[t];
     const int it = C.id[t];
     int cx, cy;
@@ -533,7 +532,16 @@ __global__ void k_pair_count(int n, Cells<Real> C, uint64_t* __restrict__ pcnt) 
             if (min_image(xt, yt, C.x[u], C.y[u], C.Lx, C.Ly, C.px, C.py, dx, dy) < C.cut2) ++cnt;
         }
     }
-    pcnt[it] = cnt;
+    return cnt;
+}
+
+// Pairs (i < j within the cutoff) listed from each particle, by particle
+// index: the reference's pair-list lengths per particle (dynamics.py:71-278).
+template <typename Real>
+__global__ void k_pair_count(int n, Cells<Real> C, uint64_t* __restrict__ pcnt) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    pcnt[C.id[t] + C.poff_base] = pair_count_one(C, t);
 }
 
 __global__ void k_pair_total(int n, const uint64_t* pcnt, const uint64_t* poff, int64_t cap,
@@ -1437,6 +1445,7 @@ static Cells<Real> make_cells(edm_membrane* m) {
     C.cap_dev = m->need_pair_offsets && !m->pair_sync;
     C.cap_gate = 0;
     C.kbase = 0;
+    C.poff_base = 0;
     C.tile_rows = m->tile_rows;
     C.sub_rows = m->sub_rows;
     return C;

@@ -24,7 +24,8 @@ static void ens_free(edm_ensemble* e) {
                     e->Dbase, e->d_pp, e->d_seg, e->d_tiles, e->noise_ctr, e->d_slot_list,
                     e->d_cats, e->d_pgrp, e->wl_pass, e->wl_dye, e->wl_rcnt,
                     e->wl_wrep_pass, e->wl_wrep_dye, e->d_wl_chunks, e->a.bond, e->a.bond_t,
-                    e->ekey, e->ea, e->eb, e->ne, e->best};
+                    e->ekey, e->ea, e->eb, e->ne, e->best, e->pcnt, e->poff, e->tconv,
+                    e->cub_tmp64};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (auto& g : e->seg) {
@@ -71,9 +72,9 @@ int edm_ens_create(const edm_params* p, const edm_photo_params* pp, const edm_im
             (r.fb1.sensor < 0 || r.fb2.sensor >= 0 || !(r.contact >= 0.0)))
             return ens_invalid("edm_ens_create: local recruitment needs one feedback rule and "
                                "sigma >= 0");
-        if (r.kind == EDM_RX_CAT && (r.a == r.b || r.a == r.c))
-            return ens_invalid("edm_ens_create: catalysis needs enzyme distinct from substrate "
-                               "and product");
+        if (r.kind == EDM_RX_CAT && (r.a == r.b || r.b == r.c))
+            return ens_invalid("edm_ens_create: catalysis needs enzyme and product distinct "
+                               "from the substrate");
         if (r.kind < EDM_RX_BIND || r.kind > EDM_RX_ADAPT)
             return ens_invalid("edm_ens_create: unknown reaction kind");
         const edm_feedback* fbs[2] = {&r.fb1, &r.fb2};
@@ -365,7 +366,7 @@ int edm_ens_run(edm_ensemble* e, int32_t n_steps, int32_t image) {
     if (!e || n_steps < 0) return ens_invalid("edm_ens_run: bad argument");
     for (int k = 0; k < n_steps; ++k)
         if (int rc = ens_step(e, image != 0)) return rc;
-    return e->has_bind && n_steps > 0 ? ens_check_edges(e) : 0;
+    return (e->has_bind || e->has_autocat) && n_steps > 0 ? ens_check_edges(e) : 0;
 }
 
 int edm_ens_frame(edm_ensemble* e, int32_t slot, uint32_t* flim, int64_t n_elems, int64_t* n_dark,

@@ -31,6 +31,9 @@ __device__ __forceinline__ Cells<Real> ens_cells(const EnsSeg& g, const Real* x,
     C.cap_dev = false;
     C.cap_gate = 0;
     C.kbase = g.kbase;
+    // pair offsets (autocatalysis) are kept per slot range of the replica:
+    // particle index i at the replica's first slot + (i - kbase)
+    C.poff_base = (int)start[g.coff] - g.kbase;
     C.tile_rows = kTileRows;  // unused: the strip table gives each strip's rows
     C.sub_rows = g.sub_rows != 0;
     return C;
@@ -426,6 +429,77 @@ __global__ __launch_bounds__(256) void k_ens_dissoc(const EnsTile* __restrict__ 
     }
 }
 
+// Autocatalysis (enzyme == product) in the batch: each replica's
+// pair-list offsets over its slot range, its accepted pairs in its own
+// region of the edge buffers, one workgroup per replica resolving the
+// conversion order -- the single-membrane bodies on the replica's views.
+template <typename Real>
+__global__ __launch_bounds__(256) void k_ens_pair_count(const EnsTile* __restrict__ tiles,
+                                                        int ntiles,
+                                                        const EnsSeg* __restrict__ segs,
+                                                        EnsPool<Real> Q, uint64_t* pcnt) {
+    const int tile = xcd_tile(ntiles);
+    if (tile >= ntiles) return;
+    const EnsTile T = tiles[tile];
+    const EnsSeg& g = segs[T.r];
+    const Cells<Real> C = ens_cells<Real>(g, Q.x, Q.y, Q.sp, Q.fl, Q.id, Q.start);
+    for (int col = 0; col < T.ncol; ++col) {
+        int beg, end;
+        ens_own_range(C, T, col, beg, end);
+        for (int t = beg + (int)threadIdx.x; t < end; t += (int)blockDim.x)
+            pcnt[C.id[t] + C.poff_base] = pair_count_one(C, t);
+    }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(256) void k_ens_autocat_edges(const EnsTile* __restrict__ tiles,
+                                                           int ntiles,
+                                                           const EnsSeg* __restrict__ segs,
+                                                           EnsPool<Real> Q,
+                                                           const CatArgs<Real>* __restrict__ cats,
+                                                           int n_runs, int ci,
+                                                           const uint64_t* poff,
+                                                           unsigned long long* ekey, int32_t* ea,
+                                                           int32_t* eb, unsigned int* ne,
+                                                           uint32_t gstep) {
+    const int tile = xcd_tile(ntiles);
+    if (tile >= ntiles) return;
+    const EnsTile T = tiles[tile];
+    const EnsSeg& g = segs[T.r];
+    Cells<Real> C = ens_cells<Real>(g, Q.x, Q.y, Q.sp, Q.fl, Q.id, Q.start);
+    C.poff = poff;
+    const CatArgs<Real>& A = cats[(size_t)T.r * n_runs + ci];
+    const Grids<Real> G = ens_grids<Real>(g, Q.counts, Q.delta, Q.ns);
+    const size_t base = 2 * (size_t)C.start[0];
+    for (int col = 0; col < T.ncol; ++col) {
+        int beg, end;
+        ens_own_range(C, T, col, beg, end);
+        for (int t = beg + (int)threadIdx.x; t < end; t += (int)blockDim.x)
+            autocat_edges_one(C, ens_post(Q), A, gstep - g.step0, G, t, ekey + base, ea + base,
+                              eb + base, ne + 2 * T.r, (unsigned int)(2 * g.n));
+    }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(1024) void k_ens_autocat_resolve(const EnsSeg* __restrict__ segs,
+                                                              EnsPool<Real> Q,
+                                                              const CatArgs<Real>* __restrict__ cats,
+                                                              int n_runs, int ci,
+                                                              const unsigned long long* ekey,
+                                                              const int32_t* ea, const int32_t* eb,
+                                                              unsigned int* ne,
+                                                              unsigned long long* tconv,
+                                                              unsigned long long* ctr) {
+    const int r = (int)blockIdx.x;
+    const EnsSeg& g = segs[r];
+    if (!g.active) return;
+    const size_t base = 2 * (size_t)Q.start[g.coff];
+    const Grids<Real> G = ens_grids<Real>(g, Q.counts, Q.delta, Q.ns);
+    autocat_resolve_block(ens_post(Q), cats[(size_t)r * n_runs + ci], ekey + base, ea + base,
+                          eb + base, ne + 2 * r, (unsigned int)(2 * g.n), tconv, ctr + 8 * r,
+                          G, Q.n_grids);
+}
+
 // Index-order recruitment cap of one recruitment reaction, one block per
 // replica slot.
 template <typename Real>
@@ -611,14 +685,28 @@ static int ens_create_t(edm_ensemble* e) {
         dalloc(&e->noise_ctr, 2 * (size_t)e->rmax) ||
         dalloc(&e->d_seg, e->rmax))
         return EDM_EHIP;
-    for (const auto& r : e->rx) e->has_bind = e->has_bind || r.kind == EDM_RX_BIND;
+    for (const auto& r : e->rx) {
+        e->has_bind = e->has_bind || r.kind == EDM_RX_BIND;
+        e->has_autocat = e->has_autocat || (r.kind == EDM_RX_CAT && r.a == r.c);
+    }
+    if (e->has_bind || e->has_autocat) {
+        if (dalloc(&e->ekey, 2 * cap) || dalloc(&e->ea, 2 * cap) || dalloc(&e->eb, 2 * cap) ||
+            dalloc(&e->ne, 2 * (size_t)e->rmax))
+            return EDM_EHIP;
+        EDM_HIP(hipMemset(e->ne, 0, 2 * (size_t)e->rmax * sizeof(unsigned int)));
+    }
     if (e->has_bind) {
-        if (dalloc(&e->a.bond, cap) || dalloc(&e->a.bond_t, cap) || dalloc(&e->ekey, 2 * cap) ||
-            dalloc(&e->ea, 2 * cap) || dalloc(&e->eb, 2 * cap) ||
-            dalloc(&e->ne, 2 * (size_t)e->rmax) || dalloc(&e->best, cap))
+        if (dalloc(&e->a.bond, cap) || dalloc(&e->a.bond_t, cap) || dalloc(&e->best, cap))
             return EDM_EHIP;
         EDM_HIP(hipMemset(e->a.bond, 0xff, (size_t)cap * sizeof(int32_t)));
-        EDM_HIP(hipMemset(e->ne, 0, 2 * (size_t)e->rmax * sizeof(unsigned int)));
+    }
+    if (e->has_autocat) {
+        if (dalloc(&e->pcnt, cap) || dalloc(&e->poff, cap) || dalloc(&e->tconv, cap))
+            return EDM_EHIP;
+        EDM_HIP(hipMemset(e->tconv, 0xff, (size_t)cap * sizeof(unsigned long long)));
+        EDM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, e->cub_bytes64, e->pcnt, e->poff,
+                                                 (int)cap));
+        EDM_HIP(hipMalloc(&e->cub_tmp64, std::max<size_t>(e->cub_bytes64, 16)));
     }
     // each replica's region of a list is padded to whole waves
     e->wl_cap = cap + 64 * (int64_t)e->rmax;
@@ -675,12 +763,13 @@ static int ens_create_t(edm_ensemble* e) {
             ++ri;
             continue;
         }
-        const bool plain = r.fb1.sensor < 0 && !r.use_adapt;
+        // autocatalysis depends on the order of the reactions' pairs: never fused
+        const bool plain = r.fb1.sensor < 0 && !r.use_adapt && r.a != r.c;
         int nf = 1;
         while (plain && nf < kCatFuse && ri + nf < e->rx.size()) {
             const edm_reaction& r2 = e->rx[ri + nf];
             if (r2.kind != EDM_RX_CAT || r2.b != r.b || r2.c != r.c || r2.fb1.sensor >= 0 ||
-                r2.use_adapt)
+                r2.use_adapt || r2.a == r2.c)
                 break;
             ++nf;
         }
@@ -926,6 +1015,16 @@ static int ens_step_t(edm_ensemble* e) {
     const uint32_t gstep = e->gstep;
     bool any_sub = false;
     for (int r = 0; r < e->rmax; ++r) any_sub |= e->seg[r].active && e->seg[r].sub_rows;
+    if (e->has_autocat) {
+        // the reference's pair-list offsets of every replica (its order of
+        // the pairs), over the pre-move positions
+        hipLaunchKernelGGL(k_ens_pair_count<Real>, tgrid, dim3(256), 0, st,
+                           (const EnsTile*)e->d_tiles, ntiles, (const EnsSeg*)e->d_seg, Q,
+                           e->pcnt);
+        size_t tb = e->cub_bytes64;
+        EDM_HIP(hipcub::DeviceScan::ExclusiveSum(e->cub_tmp64, tb, e->pcnt, e->poff,
+                                                 (int)e->n_total, st));
+    }
     if (any_sub)
         hipLaunchKernelGGL((k_ens_force<Real, true>), tgrid, dim3(256),
                            force_sub_lds(e->tile_rows, e->tile_cols), st,
@@ -963,7 +1062,17 @@ static int ens_step_t(edm_ensemble* e) {
             const int ci = e->cat_run_of[ri];
             if (ci < 0) continue;  // member of a run launched before
             const int start = e->cat_runs[ci].first;
-            if (e->cat_walk == 1 && e->cat_list_ok[ci] && e->wl_npass > 0)
+            if (e->rx[start].a == e->rx[start].c) {
+                hipLaunchKernelGGL(k_ens_autocat_edges<Real>, tgrid, dim3(256), 0, st,
+                                   (const EnsTile*)e->d_tiles, ntiles, (const EnsSeg*)e->d_seg, Q,
+                                   (const CatArgs<Real>*)e->d_cats, (int)e->cat_runs.size(), ci,
+                                   (const uint64_t*)e->poff, e->ekey, e->ea, e->eb, e->ne, gstep);
+                hipLaunchKernelGGL(k_ens_autocat_resolve<Real>, dim3(e->rmax), dim3(1024), 0, st,
+                                   (const EnsSeg*)e->d_seg, Q, (const CatArgs<Real>*)e->d_cats,
+                                   (int)e->cat_runs.size(), ci,
+                                   (const unsigned long long*)e->ekey, (const int32_t*)e->ea,
+                                   (const int32_t*)e->eb, e->ne, e->tconv, e->ctr);
+            } else if (e->cat_walk == 1 && e->cat_list_ok[ci] && e->wl_npass > 0)
                 hipLaunchKernelGGL((k_ens_cat_list<Real>), dim3(nblk(e->wl_npass)), dim3(256), 0,
                                    st, (int)(e->wl_npass / 64), (const int32_t*)e->wl_pass,
                                    (const int32_t*)e->wl_wrep_pass, (const uint32_t*)e->wl_rcnt,

@@ -139,6 +139,15 @@ struct edm_ensemble {
     // per replica the edge count and the largest overflow seen, the
     // matching's per-slot minimum key
     bool has_bind = false;
+    // autocatalysis: pair-list lengths and offsets per replica slot range
+    // (the replica's first slot + particle index - kbase), the pairs'
+    // conversion times per slot ("never" between passes); its accepted
+    // pairs share the binding edge buffers
+    bool has_autocat = false;
+    uint64_t *pcnt = nullptr, *poff = nullptr;
+    unsigned long long* tconv = nullptr;
+    void* cub_tmp64 = nullptr;
+    size_t cub_bytes64 = 0;
     unsigned long long* ekey = nullptr;
     int32_t *ea = nullptr, *eb = nullptr;
     unsigned int* ne = nullptr;  // [rmax][2]

@@ -361,3 +361,47 @@ def test_ensemble_binding_edge_overflow_is_reported():
     with pytest.raises(HipError, match="edge region"):
         ens.run(3)
     ens.close()
+
+
+def _autocat_replica(seed, side, feedback):
+    """The plain membrane on a side x side box at its density with the
+    autocatalytic PIP3 + PIP2 -> 2 PIP3 added after PI3K's reaction
+    (optionally under the pip3_activation feedback)."""
+    raw = workloads.rescale(workloads.plain_raw(), side, int(round(35.0 * side * side)))
+    rx = {"name": "PIP3_autocatalysis", "enzyme": "PIP3", "substrate": "PIP2",
+          "product": "PIP3", "k_cat": 8.0, "contact_radius": 150.0}
+    if feedback:
+        rx["feedback_rule"] = "pip3_activation"
+    raw["catalytic_reactions"].insert(1, rx)
+    raw["scan"].update(pixels_x=32, pixels_y=32, pixel_size=side * 1000.0 / 32,
+                       frame_interval=0.05)
+    raw["random_seed"] = seed
+    return workloads.config_from_dict(raw)
+
+
+@pytest.mark.parametrize("feedback", [False, True])
+def test_autocatalysis_replicas_match_single_membrane_path(feedback):
+    """Autocatalysis (enzyme == product) in the batch: each replica's
+    conversions cascade in its own reference pair order (pair-list offsets
+    over the replica's slot range), and every replica of a three-replica
+    ensemble evolves bit for bit as on the single-membrane path (FP32,
+    Philox)."""
+    steps = 40
+    cfgs = {k: _autocat_replica(s, side, feedback)
+            for k, (s, side) in enumerate(((5, 6.0), (8, 9.0), (11, 7.5)))}
+    singles = {k: _single_run(c, steps) for k, c in cfgs.items()}
+    ens = EnsembleEngine(cfgs[0], capacity=200_000, max_replicas=4)
+    slots = {k: ens.add(c) for k, c in cfgs.items()}
+    ens.run(steps)
+    for k, c in cfgs.items():
+        s = ens.state(slots[k])
+        _same(s, singles[k][0], f"replica {k}")
+        cc = ens.counters(slots[k])
+        got = (cc["n_bindings"], cc["n_dissociations"], cc["n_catalytic"], cc["n_recruited"],
+               cc["n_detached"], cc["n_converted"])
+        assert got == singles[k][1], (k, got, singles[k][1])
+        np.testing.assert_array_equal(ens.frame(slots[k]).flim_stack, singles[k][2])
+        pip3 = [sp.name for sp in c.species].index("PIP3")
+        n0 = int(np.sum((ParticleState(c).species_id == pip3)))
+        assert int(np.sum(s["species_id"] == pip3)) > 2 * n0, "PIP3 must grow"
+    ens.close()
