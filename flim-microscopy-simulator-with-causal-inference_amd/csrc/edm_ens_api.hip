@@ -25,7 +25,8 @@ static void ens_free(edm_ensemble* e) {
                     e->d_cats, e->d_pgrp, e->wl_pass, e->wl_dye, e->wl_rcnt,
                     e->wl_wrep_pass, e->wl_wrep_dye, e->d_wl_chunks, e->a.bond, e->a.bond_t,
                     e->ekey, e->ea, e->eb, e->ne, e->best, e->pcnt, e->poff, e->tconv,
-                    e->cub_tmp64};
+                    e->cub_tmp64, e->tg_cnt, e->tg_fill, e->tg_start, e->tg_u, e->tg_x,
+                    e->tg_y, e->cub_tmp_tg};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (auto& g : e->seg) {
@@ -82,10 +83,18 @@ int edm_ens_create(const edm_params* p, const edm_photo_params* pp, const edm_im
             if (f->sensor >= 0 && (f->grid < 0 || f->grid >= p->n_grids || f->sensor >= p->n_species))
                 return ens_invalid("edm_ens_create: feedback refers to a missing grid or species");
     }
+    int prox = -1;
     if (pp) {
         for (int j = 0; j < pp->n_fret_maps; ++j)
-            if (!pp->fm_skip[j] && pp->fm_proximity[j])
-                return ens_invalid("edm_ens_create: proximity FRET is not batched");
+            if (!pp->fm_skip[j] && pp->fm_proximity[j]) {
+                if (prox >= 0)
+                    return ens_invalid("edm_ens_create: at most one proximity FRET map is "
+                                       "batched");
+                if (!(pp->fm_radius[j] > 0.0) || pp->fm_target[j] < 0 ||
+                    pp->fm_target[j] >= p->n_species)
+                    return ens_invalid("edm_ens_create: bad proximity FRET map");
+                prox = j;
+            }
     }
     edm_ensemble* e = new edm_ensemble();
     e->p = *p;
@@ -106,6 +115,14 @@ int edm_ens_create(const edm_params* p, const edm_photo_params* pp, const edm_im
     e->p.reactions = nullptr;
     e->p.sigma = e->p.epsilon = e->p.k_harm = e->p.D_base = nullptr;
     e->p.rcx = e->p.rcy = e->p.rr = e->p.rf = nullptr;
+    if (!rc && pp && prox >= 0) {
+        e->prox_map = prox;
+        const size_t rb = e->rs;
+        if (hipMalloc(&e->tg_x, (size_t)capacity * rb + 16) != hipSuccess ||
+            hipMalloc(&e->tg_y, (size_t)capacity * rb + 16) != hipSuccess ||
+            hipMalloc(&e->tg_u, (size_t)capacity * sizeof(int32_t) + 16) != hipSuccess)
+            rc = hip_fail(hipErrorOutOfMemory, "edm_ens_create: proximity FRET buffers");
+    }
     if (!rc && pp) {
         e->pp = *pp;
         e->has_pp = true;
@@ -222,6 +239,25 @@ int edm_ens_add(edm_ensemble* e, const edm_replica* r, const edm_host_state* s, 
         g.prob[q] = r->prob[q];
         g.prob2[q] = r->prob2[q];
         g.maxr[q] = r->max_recruits[q];
+    }
+    // proximity FRET: this replica's target grid (as the single membrane's,
+    // fret_prox_grid: cells at least the radius plus a rounding margin, at
+    // most max(n, 64) of them)
+    if (e->prox_map >= 0) {
+        const double Lx = r->Lx, Ly = r->Ly, rad = e->pp.fm_radius[e->prox_map];
+        const double ulp = e->fp64 ? 0x1p-52 : 0x1p-23;
+        const double cmin = rad * 1.001 + 64.0 * ulp * std::max(Lx, Ly);
+        long ncx = std::max(1L, std::min(4096L, (long)std::floor(Lx / cmin)));
+        long ncy = std::max(1L, std::min(4096L, (long)std::floor(Ly / cmin)));
+        const long maxc = std::max<long>(n, 64);
+        while (ncx * ncy > maxc) {
+            if (ncx >= ncy) ncx = (ncx + 1) / 2;
+            else ncy = (ncy + 1) / 2;
+        }
+        g.tg_ncx = (int32_t)ncx;
+        g.tg_ncy = (int32_t)ncy;
+        g.tg_ix = (double)ncx / Lx;
+        g.tg_iy = (double)ncy / Ly;
     }
     // binding models: the slot of each particle, written by every sort
     if (e->has_bind) EDM_HIP(hipMalloc(&g.slot_of, (size_t)std::max(n, 1) * sizeof(int32_t)));

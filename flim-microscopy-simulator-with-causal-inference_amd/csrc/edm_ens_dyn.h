@@ -803,7 +803,7 @@ int ens_layout(edm_ensemble* e) {
     // particles); results do not depend on it
     e->tile_rows = 1;
     e->tile_cols = 1;
-    int64_t coff = 0, cnt = 0;
+    int64_t coff = 0, cnt = 0, tcoff = 0;
     e->tiles.clear();
     std::vector<int32_t> wrep[2];
     for (int r = 0; r < e->rmax; ++r) {
@@ -817,6 +817,10 @@ int ens_layout(edm_ensemble* e) {
         }
         g.coff = (int32_t)coff;
         coff += (int64_t)g.ncx * g.ncy;
+        if (e->prox_map >= 0) {
+            g.tg_off = (int32_t)tcoff;
+            tcoff += (int64_t)g.tg_ncx * g.tg_ncy;
+        }
         for (int k = 0; k < kEnsMaxGrids * kEnsMaxSpecies; ++k) g.goff[k] = -1;
         for (auto& kp : e->kept) {
             g.goff[kp.first * ns + kp.second] = (int32_t)cnt;
@@ -848,6 +852,29 @@ int ens_layout(edm_ensemble* e) {
         return EDM_ECAPACITY;
     }
     e->n_cells = coff;
+    if (e->prox_map >= 0) {
+        if (tcoff >= (int64_t)INT32_MAX - 2) {
+            set_error("edm_ensemble: too many proximity-FRET target cells");
+            return EDM_ECAPACITY;
+        }
+        e->n_tcells = tcoff;
+        if (tcoff + 1 > e->tcell_cap) {
+            hipFree(e->tg_cnt);
+            hipFree(e->tg_fill);
+            hipFree(e->tg_start);
+            hipFree(e->cub_tmp_tg);
+            e->tg_cnt = e->tg_fill = e->tg_start = nullptr;
+            e->cub_tmp_tg = nullptr;
+            e->tcell_cap = std::max<int64_t>(tcoff + 1, e->tcell_cap * 3 / 2);
+            if (dalloc(&e->tg_cnt, e->tcell_cap) || dalloc(&e->tg_fill, e->tcell_cap) ||
+                dalloc(&e->tg_start, e->tcell_cap))
+                return EDM_EHIP;
+            e->cub_bytes_tg = 0;
+            EDM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, e->cub_bytes_tg, e->tg_cnt,
+                                                     e->tg_start, (int)e->tcell_cap));
+            EDM_HIP(hipMalloc(&e->cub_tmp_tg, std::max<size_t>(e->cub_bytes_tg, 16)));
+        }
+    }
     e->n_counts = cnt;
     if (coff + 2 > e->cell_cap) {
         hipFree(e->cell_cnt);

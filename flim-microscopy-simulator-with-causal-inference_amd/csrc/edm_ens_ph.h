@@ -3,6 +3,51 @@
 // the step's work list emit with their replica's draws, image geometry and
 // TCSPC stack.
 
+// Proximity FRET in the batch: every replica's alive targets of the map
+// binned into its own grid (cells of the pooled target-cell space from its
+// tg_off), as the single membrane's fret_prox_grid does.
+template <typename Real>
+__global__ void k_ens_tg_count(int n, const Real* x, const Real* y, const int8_t* sp,
+                               const uint8_t* fl, const uint16_t* __restrict__ rep,
+                               const EnsSeg* __restrict__ segs, int target, int32_t* cnt) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    if (!(fl[u] & EDM_FLAG_ALIVE) || sp[u] != target) return;
+    const EnsSeg& g = segs[rep[u]];
+    if (!g.active) return;
+    const TgGrid tg{g.tg_ncx, g.tg_ncy, g.tg_ix, g.tg_iy};
+    atomicAdd(&cnt[g.tg_off + tg.cell((double)x[u], (double)y[u])], 1);
+}
+
+template <typename Real>
+__global__ void k_ens_tg_scatter(int n, const Real* x, const Real* y, const int8_t* sp,
+                                 const uint8_t* fl, const uint16_t* __restrict__ rep,
+                                 const EnsSeg* __restrict__ segs, int target,
+                                 const int32_t* start, int32_t* fill, Real* tx, Real* ty,
+                                 int32_t* tu) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    if (!(fl[u] & EDM_FLAG_ALIVE) || sp[u] != target) return;
+    const EnsSeg& g = segs[rep[u]];
+    if (!g.active) return;
+    const TgGrid tg{g.tg_ncx, g.tg_ncy, g.tg_ix, g.tg_iy};
+    const Real xu = x[u], yu = y[u];
+    const int c = g.tg_off + tg.cell((double)xu, (double)yu);
+    const int k = start[c] + atomicAdd(&fill[c], 1);
+    tx[k] = xu;
+    ty[k] = yu;
+    tu[k] = u;
+}
+
+// The proximity map's binned targets (nullptr: no proximity map).
+template <typename Real>
+struct EnsTargets {
+    int map;
+    const int32_t* start;
+    const Real *x, *y;
+    const int32_t* u;
+};
+
 template <typename Real, int kWin>
 __global__ __launch_bounds__(256) void k_ens_photons(int n_waves, const int32_t* __restrict__ list,
                                                      const int32_t* __restrict__ wrep,
@@ -12,6 +57,7 @@ __global__ __launch_bounds__(256) void k_ens_photons(int n_waves, const int32_t*
                                                      const Real* y, uint8_t* fl, const int8_t* dye,
                                                      int8_t* ctmc, const int32_t* __restrict__ id,
                                                      const int32_t* __restrict__ bond,
+                                                     EnsTargets<Real> TG,
                                                      unsigned long long* ctr, uint32_t gstep) {
     if ((int)blockIdx.x * (int)(blockDim.x >> 6) +
             __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) >= n_waves)
@@ -24,11 +70,29 @@ __global__ __launch_bounds__(256) void k_ens_photons(int n_waves, const int32_t*
     Emit<Real> E{0, 0, 0u, 0, 0, 0, 0, (Real)0, (Real)0};
     if (j < min(g.wl_n[1], (int)rcnt[2 * W.r + 1])) {
         const int t = list[g.wl_off[1] + j];
-        // bond-mode FRET from the bonds (binding models; the acceptor of an
-        // intermolecular map through the replica's slot_of); without bonds
-        // no donor is quenched
+        // FRET rates as the single membrane takes them (update_fret_rates,
+        // photophysics.py:1021-1114): bond maps from the bonds (binding
+        // models; an intermolecular acceptor through the replica's
+        // slot_of), then the proximity map from the replica's target grid
+        double kf = 0.0;
+        int aid = -1;
+        if (bond) fret_inline(pp, t, fl, dye, bond, g.slot_of, kf, aid, g.kbase);
+        if (TG.map >= 0) {
+            const int q = TG.map;
+            const uint8_t f = fl[t];
+            if ((f & EDM_FLAG_DYE) && (f & EDM_FLAG_ALIVE) && !(f & EDM_FLAG_BLEACHED) &&
+                dye[t] == pp->fm_donor[q]) {
+                const TgGrid tg{g.tg_ncx, g.tg_ncy, g.tg_ix, g.tg_iy};
+                const bool found = prox_search<Real>(tg, TG.start + g.tg_off, TG.x, TG.y, TG.u, t,
+                                                     x[t], y[t], (Real)pp->fm_radius[q],
+                                                     (Real)g.Lx, (Real)g.Ly, g.px != 0,
+                                                     g.py != 0);
+                kf = found ? pp->fm_k_hi[q] : pp->fm_k_lo[q];
+                aid = -1;
+            }
+        }
         E = emit_pss<Real>(pp, t, (uint32_t)(id[t] - g.kbase), fl, dye, ctmc, bond, g.slot_of,
-                           bond == nullptr, 0.0, -1, rng, gstep - g.step0, emitted, g.kbase);
+                           true, kf, aid, rng, gstep - g.step0, emitted, g.kbase);
     }
     // the wave's emitters are all of replica W.r: its geometry, stack and draws
     wave_photons<Real, kWin>(E, &g.ip, x, y, g.flim, nullptr, rng, gstep - g.step0, detected);
@@ -37,17 +101,43 @@ __global__ __launch_bounds__(256) void k_ens_photons(int n_waves, const int32_t*
 
 template <typename Real, int kWin>
 static void launch_ens_photons(edm_ensemble* e, int nt) {
+    EnsTargets<Real> TG{e->prox_map, (const int32_t*)e->tg_start, (const Real*)e->tg_x,
+                        (const Real*)e->tg_y, (const int32_t*)e->tg_u};
     hipLaunchKernelGGL((k_ens_photons<Real, kWin>), dim3(nb(nt)), dim3(256), 0, e->stream, nt / 64,
                        (const int32_t*)e->wl_dye, (const int32_t*)e->wl_wrep_dye,
                        (const uint32_t*)e->wl_rcnt, (const EnsSeg*)e->d_seg,
                        (const edm_photo_params*)e->d_pp, (const Real*)e->a.x,
                        (const Real*)e->a.y, e->a.fl, (const int8_t*)e->a.dye, e->a.ctmc,
-                       (const int32_t*)e->a.id, (const int32_t*)e->a.bond, e->ctr, e->gstep);
+                       (const int32_t*)e->a.id, (const int32_t*)e->a.bond, TG, e->ctr, e->gstep);
+}
+
+template <typename Real>
+static int ens_targets(edm_ensemble* e) {
+    const int n = (int)e->n_total;
+    const int nc = (int)e->n_tcells;
+    hipStream_t st = e->stream;
+    const int target = e->pp.fm_target[e->prox_map];
+    hipLaunchKernelGGL(k_tg_zero, dim3(nb(nc + 1)), dim3(kB), 0, st, nc, e->tg_cnt, e->tg_fill);
+    hipLaunchKernelGGL(k_ens_tg_count<Real>, dim3(nb(std::max(n, 1))), dim3(kB), 0, st, n,
+                       (const Real*)e->a.x, (const Real*)e->a.y, (const int8_t*)e->a.sp,
+                       (const uint8_t*)e->a.fl, (const uint16_t*)e->rep, (const EnsSeg*)e->d_seg,
+                       target, e->tg_cnt);
+    size_t tb = e->cub_bytes_tg;
+    EDM_HIP(hipcub::DeviceScan::ExclusiveSum(e->cub_tmp_tg, tb, e->tg_cnt, e->tg_start, nc + 1,
+                                             st));
+    hipLaunchKernelGGL(k_ens_tg_scatter<Real>, dim3(nb(std::max(n, 1))), dim3(kB), 0, st, n,
+                       (const Real*)e->a.x, (const Real*)e->a.y, (const int8_t*)e->a.sp,
+                       (const uint8_t*)e->a.fl, (const uint16_t*)e->rep, (const EnsSeg*)e->d_seg,
+                       target, (const int32_t*)e->tg_start, e->tg_fill, (Real*)e->tg_x,
+                       (Real*)e->tg_y, e->tg_u);
+    return 0;
 }
 
 int ens_photons(edm_ensemble* e) {
     const int nt = (int)e->wl_ndye;
     if (nt == 0 || !e->has_pp) return 0;
+    if (e->prox_map >= 0)
+        if (int rc = e->fp64 ? ens_targets<double>(e) : ens_targets<float>(e)) return rc;
     // the PSF window capacity of the launch covers every replica's window
     bool small = true;
     for (int r = 0; r < e->rmax; ++r)

@@ -57,6 +57,11 @@ struct EnsSeg {
     double* w_adapt;    // this replica's adaptation field on its adaptation grid, or nullptr
     double prob2[kEnsMaxRx];  // BIND reactions: dissociation probability
     int32_t* slot_of;   // binding models: slot of each particle (index - kbase)
+    // proximity FRET: this replica's grid of the map's targets (cells no
+    // narrower than the radius plus a rounding margin) and its first cell in
+    // the pooled target-cell space
+    int32_t tg_ncx, tg_ncy, tg_off;
+    double tg_ix, tg_iy;
 };
 
 static constexpr int kEnsMaxRegions = 8;
@@ -144,6 +149,14 @@ struct edm_ensemble {
     // conversion times per slot ("never" between passes); its accepted
     // pairs share the binding edge buffers
     bool has_autocat = false;
+    // proximity FRET (one map at most): the map, the pooled target cells
+    // (counts, fill cursors, starts) and the binned targets
+    int prox_map = -1;
+    int64_t n_tcells = 0, tcell_cap = 0;
+    int32_t *tg_cnt = nullptr, *tg_fill = nullptr, *tg_start = nullptr, *tg_u = nullptr;
+    void *tg_x = nullptr, *tg_y = nullptr;
+    void* cub_tmp_tg = nullptr;
+    size_t cub_bytes_tg = 0;
     uint64_t *pcnt = nullptr, *poff = nullptr;
     unsigned long long* tconv = nullptr;
     void* cub_tmp64 = nullptr;

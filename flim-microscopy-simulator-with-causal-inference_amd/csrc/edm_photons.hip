@@ -163,27 +163,20 @@ __device__ __forceinline__ int tg_axis(int c, int nc, bool periodic, int* out) {
     return m;
 }
 
+// Some alive target other than slot t within radius r of (xi, yi), through
+// the target grid: the donor's cell and its neighbours (start: the grid's
+// cell starts).
 template <typename Real>
-__global__ void k_fret_prox_grid(int n_dye, const int32_t* dye_ids, const int32_t* slot_of,
-                                 const edm_photo_params* pp, int j, const Real* x, const Real* y,
-                                 const uint8_t* fl, const int8_t* dye, Real Lx, Real Ly, bool px,
-                                 bool py, TgGrid g, const int32_t* start, const Real* tx,
-                                 const Real* ty, const int32_t* tu, double* k_fret, int32_t* acc) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n_dye) return;
-    const int t = slot_of[dye_ids[q]];
-    const uint8_t f = fl[t];
-    if (!(f & EDM_FLAG_DYE) || !(f & EDM_FLAG_ALIVE) || (f & EDM_FLAG_BLEACHED)) return;
-    if (dye[t] != pp->fm_donor[j]) return;
-    const Real r2 = (Real)pp->fm_radius[j] * (Real)pp->fm_radius[j];
-    const Real xi = x[t], yi = y[t];
+__device__ __forceinline__ bool prox_search(const TgGrid& g, const int32_t* start, const Real* tx,
+                                            const Real* ty, const int32_t* tu, int t, Real xi,
+                                            Real yi, Real r, Real Lx, Real Ly, bool px, bool py) {
+    const Real r2 = r * r;
     const int c = g.cell((double)xi, (double)yi);
     int cxs[3], cys[3];
     const int nx = tg_axis(c / g.ncy, g.ncx, px, cxs);
     const int ny = tg_axis(c % g.ncy, g.ncy, py, cys);
-    bool found = false;
-    for (int a = 0; a < nx && !found; ++a)
-        for (int b = 0; b < ny && !found; ++b) {
+    for (int a = 0; a < nx; ++a)
+        for (int b = 0; b < ny; ++b) {
             const int cc = cxs[a] * g.ncy + cys[b];
             for (int k = start[cc], e = start[cc + 1]; k < e; ++k) {
                 if (tu[k] == t) continue;
@@ -196,12 +189,26 @@ __global__ void k_fret_prox_grid(int n_dye, const int32_t* dye_ids, const int32_
                     if (dy > Ly / (Real)2) dy -= Ly;
                     else if (dy < -Ly / (Real)2) dy += Ly;
                 }
-                if (dx * dx + dy * dy < r2) {
-                    found = true;
-                    break;
-                }
+                if (dx * dx + dy * dy < r2) return true;
             }
         }
+    return false;
+}
+
+template <typename Real>
+__global__ void k_fret_prox_grid(int n_dye, const int32_t* dye_ids, const int32_t* slot_of,
+                                 const edm_photo_params* pp, int j, const Real* x, const Real* y,
+                                 const uint8_t* fl, const int8_t* dye, Real Lx, Real Ly, bool px,
+                                 bool py, TgGrid g, const int32_t* start, const Real* tx,
+                                 const Real* ty, const int32_t* tu, double* k_fret, int32_t* acc) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n_dye) return;
+    const int t = slot_of[dye_ids[q]];
+    const uint8_t f = fl[t];
+    if (!(f & EDM_FLAG_DYE) || !(f & EDM_FLAG_ALIVE) || (f & EDM_FLAG_BLEACHED)) return;
+    if (dye[t] != pp->fm_donor[j]) return;
+    const bool found = prox_search<Real>(g, start, tx, ty, tu, t, x[t], y[t],
+                                         (Real)pp->fm_radius[j], Lx, Ly, px, py);
     k_fret[q] = found ? pp->fm_k_hi[j] : pp->fm_k_lo[j];
     acc[q] = -1;
 }

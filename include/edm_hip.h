@@ -540,8 +540,9 @@ int edm_stats_flim(edm_membrane *m, int32_t channel, int32_t harmonic, double *o
  * shapes, circular D(x) regions and image geometry; the adaptation field
  * (model adapt_* fields) is kept per replica on its own grid.  Philox
  * draws only (keys relative to each replica's first particle and admission
- * step), so a replica evolves exactly as it does alone.  Not batched:
- * autocatalysis, proximity FRET.                                         */
+ * step), so a replica evolves exactly as it does alone.  Batched
+ * models carry at most one proximity FRET map (each replica bins its
+ * targets into its own grid).                                            */
 typedef struct edm_replica {
     int32_t n;                  /* particles (alive + dormant)              */
     int32_t ncx, ncy;           /* cell grid (CellList, dynamics.py:109-114) */

@@ -313,16 +313,19 @@ def _single_run(cfg, steps):
     return out, ctr, fr.flim_stack.copy()
 
 
-@pytest.mark.parametrize("intramolecular", [True, False])
-def test_binding_replicas_match_single_membrane_path(intramolecular):
+@pytest.mark.parametrize("variant", [dict(), dict(intramolecular=False),
+                                     dict(proximity=True)])
+def test_binding_replicas_match_single_membrane_path(variant):
     """Binding and dissociation (PHAkt + PIP3 <-> PHAkt_PIP3, dynamics.py:
-    529-654) with bond FRET (intramolecular, or a labelled PIP3 acceptor
-    found through the replica's slots) and circular D(x) regions in the
-    batch: each replica of a three-replica ensemble (different boxes and
-    seeds) evolves bit for bit as on the single-membrane path (FP32,
-    Philox): positions, species, flags, bonds, event counters and the frame."""
+    529-654) with FRET -- bond mode (intramolecular, or a labelled PIP3
+    acceptor found through the replica's slots) or the proximity biosensor
+    (photophysics.py:592-683, each replica its own target grid) -- and
+    circular D(x) regions in the batch: each replica of a three-replica
+    ensemble (different boxes and seeds) evolves bit for bit as on the
+    single-membrane path (FP32, Philox): positions, species, flags, bonds,
+    event counters and the frame."""
     steps = 60
-    cfgs = {k: workloads.binding_fret(seed=s, side=side, intramolecular=intramolecular)
+    cfgs = {k: workloads.binding_fret(seed=s, side=side, **variant)
             for k, (s, side) in enumerate(((3, 4.0), (4, 5.0), (7, 6.0)))}
     for c in cfgs.values():
         for r in c.reactions:
