@@ -124,6 +124,12 @@ int edm_destroy(edm_membrane* m) {
     frames_free(m);
     if (m->fr_stream) hipStreamDestroy(m->fr_stream);
     if (m->fr_photons) hipEventDestroy(m->fr_photons);
+    if (m->ph_stream) {
+        hipStreamSynchronize(m->ph_stream);
+        hipStreamDestroy(m->ph_stream);
+    }
+    if (m->ev_force) hipEventDestroy(m->ev_force);
+    if (m->ev_photons) hipEventDestroy(m->ev_photons);
     if (m->fr_done) hipEventDestroy(m->fr_done);
     if (m->d_step) hipFree(m->d_step);
     rec_free(m);
@@ -209,6 +215,35 @@ int edm_upload_state(edm_membrane* m, const edm_host_state* s) {
     bool bonds = false;
     for (auto& r : m->rx) bonds = bonds || r.kind == EDM_RX_BIND;
     for (int i = 0; i < n && !bonds; ++i) bonds = bond[i] >= 0;
+    // photons beside the reactions: the species a dye carrier can hold
+    // (its species at upload, closed under catalysis and conversion) are
+    // touched by no recruitment, detachment or binding, and no FRET map
+    // reads positions of other particles (proximity)
+    {
+        uint32_t C = 0;
+        for (int i = 0; i < n; ++i)
+            if (fl[i] & EDM_FLAG_DYE) C |= 1u << sp[i];
+        for (bool grow = true; grow;) {
+            grow = false;
+            for (auto& r : m->rx) {
+                uint32_t T = 0;
+                if (r.kind == EDM_RX_CAT) T = (1u << r.b) | (1u << r.c);
+                else if (r.kind == EDM_RX_CONVERT) T = (1u << r.a) | (1u << r.b);
+                if ((T & C) && (T & ~C)) {
+                    C |= T;
+                    grow = true;
+                }
+            }
+        }
+        bool ok = !bonds;
+        for (auto& r : m->rx)
+            if ((r.kind == EDM_RX_RECRUIT || r.kind == EDM_RX_DETACH) && ((C >> r.a) & 1u))
+                ok = false;
+        if (m->has_pp)
+            for (int j = 0; j < m->pp.n_fret_maps; ++j)
+                if (m->pp.fm_proximity[j] && !m->pp.fm_skip[j]) ok = false;
+        m->ph_overlap_ok = ok;
+    }
     m->keep_slot_of = bonds;
     m->slot_of_valid = true;  // the identity, copied above
     m->alive_at_upload.assign(m->p.n_species, 0);

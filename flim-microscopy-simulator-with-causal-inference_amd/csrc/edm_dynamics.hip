@@ -2140,6 +2140,13 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
                                m->sensor_mask_all, ntile_y, ntiles);
     }
     if (m->time_force) EDM_HIP(hipEventRecord(m->kev[1], st));
+    // the fused photon kernel of this step may start here (ph_fused_image)
+    m->force_mark = false;
+    if (!hook && m->ph_overlap_ok && !m->use_graphs && !m->graph_rec) {
+        if (!m->ev_force) EDM_HIP(hipEventCreateWithFlags(&m->ev_force, hipEventDisableTiming));
+        EDM_HIP(hipEventRecord(m->ev_force, st));
+        m->force_mark = true;
+    }
 
     int64_t off = 2 * (int64_t)n;
     const int64_t npairs = m->n_pairs;

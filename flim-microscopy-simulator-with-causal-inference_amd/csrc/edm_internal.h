@@ -267,6 +267,15 @@ struct edm_membrane {
     uint32_t* flim_alt = nullptr;
     hipStream_t fr_stream = nullptr;
     hipEvent_t fr_photons = nullptr, fr_done = nullptr;
+    // photons beside the reactions: when no reaction after the forces can
+    // change a dye carrier (its alive flag, bond or position) and the FRET
+    // rates come from bonds only, the fused photon kernel of a step waits
+    // only for the step's force kernel (ev_force) on ph_stream, and the main
+    // stream waits for it (ev_photons) before anything queued later
+    bool ph_overlap_ok = false;
+    bool force_mark = false;  // ev_force recorded by this step's forces
+    hipStream_t ph_stream = nullptr;
+    hipEvent_t ev_force = nullptr, ev_photons = nullptr;
     uint32_t *fr_blk = nullptr, *fr_idx = nullptr, *fr_cnt = nullptr;
     unsigned long long* fr_meta = nullptr;
     int fr_nwave = 0;

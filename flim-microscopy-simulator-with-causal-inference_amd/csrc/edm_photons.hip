@@ -1165,10 +1165,10 @@ static const edm_image_params* dev_ip(edm_membrane* m) {
 }
 
 template <typename Real, int kWin>
-static void launch_pss_image(edm_membrane* m, dim3 grid, WlView V, const edm_photo_params* dpp,
-                             const edm_image_params* d_ip, const double* kf, const int32_t* ac,
-                             Rng rng, StepRef step) {
-    hipLaunchKernelGGL((k_pss_image<Real, kWin>), grid, dim3(kB), 0, m->stream, m->n_dye,
+static void launch_pss_image(edm_membrane* m, hipStream_t st, dim3 grid, WlView V,
+                             const edm_photo_params* dpp, const edm_image_params* d_ip,
+                             const double* kf, const int32_t* ac, Rng rng, StepRef step) {
+    hipLaunchKernelGGL((k_pss_image<Real, kWin>), grid, dim3(kB), 0, st, m->n_dye,
                        (const int32_t*)m->dye_ids, V, (const int32_t*)m->a.id, dpp, d_ip,
                        (const Real*)m->a.x, (const Real*)m->a.y, m->a.fl, (const int8_t*)m->a.dye,
                        m->a.ctmc, (const int32_t*)m->a.bond, kf, ac, (const int32_t*)m->a.slot_of,
@@ -1207,12 +1207,29 @@ int ph_fused_image(edm_membrane* m, int64_t* n_emitted, int64_t* n_detected) {
         if (int rc = dyn_ensure_slot_of(m)) return rc;
     const dim3 grid = V.list ? dim3(m->wl_nblk, kWlPer) : dim3(nb(m->n_dye));
     const bool small = psf_small(m->ip.R);
+    // beside the step's reactions when none of them can change what the
+    // photons read (the plan's ph_overlap_ok): after the step's forces only
+    hipStream_t ps = st;
+    const bool beside =
+        m->force_mark && m->ph_overlap_ok && !prox && !m->use_graphs && !m->graph_rec;
+    if (beside) {
+        if (!m->ph_stream) EDM_HIP(hipStreamCreate(&m->ph_stream));
+        if (!m->ev_photons) EDM_HIP(hipEventCreateWithFlags(&m->ev_photons, hipEventDisableTiming));
+        EDM_HIP(hipStreamWaitEvent(m->ph_stream, m->ev_force, 0));
+        ps = m->ph_stream;
+    }
+    m->force_mark = false;
     if (m->fp64) {
-        if (small) launch_pss_image<double, 8>(m, grid, V, dpp, d_ip, kf, ac, rng, step);
-        else launch_pss_image<double, 31>(m, grid, V, dpp, d_ip, kf, ac, rng, step);
+        if (small) launch_pss_image<double, 8>(m, ps, grid, V, dpp, d_ip, kf, ac, rng, step);
+        else launch_pss_image<double, 31>(m, ps, grid, V, dpp, d_ip, kf, ac, rng, step);
     } else {
-        if (small) launch_pss_image<float, 8>(m, grid, V, dpp, d_ip, kf, ac, rng, step);
-        else launch_pss_image<float, 31>(m, grid, V, dpp, d_ip, kf, ac, rng, step);
+        if (small) launch_pss_image<float, 8>(m, ps, grid, V, dpp, d_ip, kf, ac, rng, step);
+        else launch_pss_image<float, 31>(m, ps, grid, V, dpp, d_ip, kf, ac, rng, step);
+    }
+    if (beside) {
+        // everything queued on the main stream from here on follows the photons
+        EDM_HIP(hipEventRecord(m->ev_photons, m->ph_stream));
+        EDM_HIP(hipStreamWaitEvent(st, m->ev_photons, 0));
     }
     EDM_HIP(hipGetLastError());
     if (n_emitted || n_detected) {
