@@ -303,9 +303,11 @@ int edm_ens_add(edm_ensemble* e, const edm_replica* r, const edm_host_state* s, 
         g.ip.pixel = r->img_pixel;
         g.ip.offx = r->img_offx;
         g.ip.offy = r->img_offy;
-        g.flim_elems = (int64_t)g.ip.n_ch * g.ip.nx * g.ip.ny * g.ip.n_bins;
-        EDM_HIP(hipMalloc(&g.flim, (size_t)g.flim_elems * sizeof(uint32_t)));
-        EDM_HIP(hipMemsetAsync(g.flim, 0, (size_t)g.flim_elems * sizeof(uint32_t), e->stream));
+        const int64_t npix = (int64_t)g.ip.n_ch * g.ip.nx * g.ip.ny;
+        g.flim_elems = npix * g.ip.n_bins;
+        g.flim_words = g.flim_elems + (npix + 255) / 256 * 64;  // counts, row flags
+        EDM_HIP(hipMalloc(&g.flim, (size_t)g.flim_words * sizeof(uint32_t)));
+        EDM_HIP(hipMemsetAsync(g.flim, 0, (size_t)g.flim_words * sizeof(uint32_t), e->stream));
     }
     // particles, appended after every pooled one (particle-index order)
     const int64_t beg = e->n_total;

@@ -94,8 +94,12 @@ __global__ __launch_bounds__(256) void k_ens_photons(int n_waves, const int32_t*
         E = emit_pss<Real>(pp, t, (uint32_t)(id[t] - g.kbase), fl, dye, ctmc, bond, g.slot_of,
                            true, kf, aid, rng, gstep - g.step0, emitted, g.kbase);
     }
-    // the wave's emitters are all of replica W.r: its geometry, stack and draws
-    wave_photons<Real, kWin>(E, &g.ip, x, y, g.flim, nullptr, rng, gstep - g.step0, detected);
+    // the wave's emitters are all of replica W.r: its geometry, stack (and
+    // its row flags) and draws
+    wave_photons<Real, kWin>(E, &g.ip, x, y, g.flim,
+                             row_flags(g.flim, (int64_t)g.ip.n_ch * g.ip.nx * g.ip.ny,
+                                       g.ip.n_bins),
+                             rng, gstep - g.step0, detected);
     photon_count(emitted, detected, ctr + 8 * W.r);
 }
 
@@ -161,7 +165,7 @@ int ens_frame_noise(edm_ensemble* e, int slot, int64_t* n_dark, int64_t* n_ap) {
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, e->stream, (int64_t)4,
                        (uint32_t*)e->frame_out, 0u);
     hipLaunchKernelGGL(k_frame_noise, dim3(nb(npix)), dim3(kB), 0, e->stream, npix, ip.n_bins,
-                       g.flim, (uint8_t*)nullptr, ip.dark_lambda, ip.p_ap, rng,
+                       g.flim, row_flags(g.flim, npix, ip.n_bins), ip.dark_lambda, ip.p_ap, rng,
                        e->gstep - g.step0, e->frame_out);
     EDM_HIP(hipGetLastError());
     if (n_dark || n_ap) {
@@ -177,76 +181,161 @@ int ens_frame_noise(edm_ensemble* e, int slot, int64_t* n_dark, int64_t* n_ap) {
 int ens_frame_reset(edm_ensemble* e, int slot) {
     const EnsSeg& g = e->seg[slot];
     if (!g.flim) return 0;
-    hipLaunchKernelGGL(k_fill_u32, dim3(std::min(nb(g.flim_elems), 8192)), dim3(kB), 0, e->stream,
-                       g.flim_elems, g.flim, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(std::min(nb(g.flim_words), 8192)), dim3(kB), 0, e->stream,
+                       g.flim_words, g.flim, 0u);
     EDM_HIP(hipGetLastError());
     return 0;
 }
 
-// Frame end of several replicas at once (blockIdx.y = position in the slot
-// list): detector noise into each listed stack, counted per replica.
-// Compiled for 5 waves per SIMD: latency bound, it runs faster at 5 than at
-// the 3 its registers would allow, despite the spills (1051 against 1135 us
-// for 512 stacks of 64 x 64 x 256).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_ens_frame_noise(const EnsSeg* __restrict__ segs,
-                                                         const int32_t* __restrict__ slots,
-                                                         unsigned long long* noise_ctr,
-                                                         uint32_t gstep) {
-    const int r = slots[blockIdx.y];
-    const EnsSeg& g = segs[r];
-    // the few replica fields the frame end needs, once
-    uint32_t* const flim = g.flim;
-    const int nb = g.ip.n_bins;
-    const int64_t npix = (int64_t)g.ip.n_ch * g.ip.nx * g.ip.ny;
-    const double lam = g.ip.dark_lambda, p_ap = g.ip.p_ap;
-    const Rng rng{g.seed_lo ^ 0x2545F491u, g.seed_hi};
-    const uint32_t frame = gstep - g.step0;
-    if (!flim) return;  // block-uniform
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t end = (npix + 63) / 64 * 64;  // whole wavefronts take part in the counts
-#pragma unroll 1
-    for (int64_t q0 = (int64_t)blockIdx.x * blockDim.x; q0 < end; q0 += stride) {
-        // this wavefront's 64 pixels: totals read coalesced, then one pixel per lane
-        const int64_t qw = q0 + (threadIdx.x & ~63u);
-        const int64_t q = q0 + threadIdx.x;
-        unsigned long long nd = 0, na = 0;
-        if (qw < npix) {
-            const uint32_t tot0 = p_ap > 0.0 ? wave_pixel_totals(flim, qw, npix, nb) : 0u;
-            if (q < npix)
-                (void)frame_noise_pixel(q, nb, flim, nullptr, tot0, lam, p_ap, rng, frame, nd,
-                                        na);
-        }
-        noise_count(nd, na, noise_ctr + 2 * r);
-    }
+// Frame end of several replicas at once, as the single membrane's streamed
+// frame end does it (k_fr_noise_count, k_fr_emit) on each listed stack:
+// blockIdx.y = position q in the list, wavefront w of the block's 4 = the
+// replica's pixel rows 64 w .. 64 w + 63.  The first pass sums the flagged
+// rows, draws every pixel's detector noise (frame_noise_pixel: the draws
+// of the dense frame end) and counts the wavefront's nonzero bins into
+// wcnt[wbase[q] + w]; after an exclusive scan of those counts the second
+// pass writes the flagged rows' nonzero bins as (flat index, count) pairs
+// in index order, replica after replica in list order (oidx nullptr: no
+// output), zeroes the bins it read and clears the flags, which resets the
+// stack.  Only rows that hold counts are read or written.
+struct EnsFrameList {
+    const int32_t* slots;
+    const int64_t* wbase;
+};
+
+__device__ __forceinline__ bool ens_fr_wave(const EnsSeg& g, int64_t& q0, int64_t& npix, int& nb,
+                                            int64_t& w) {
+    if (!g.flim) return false;
+    npix = (int64_t)g.ip.n_ch * g.ip.nx * g.ip.ny;
+    nb = g.ip.n_bins;
+    w = (int64_t)blockIdx.x * 4 + (int64_t)(threadIdx.x >> 6);
+    q0 = w << 6;
+    return q0 < npix;
 }
 
-__global__ void k_ens_frame_reset(const EnsSeg* __restrict__ segs,
-                                  const int32_t* __restrict__ slots) {
-    const EnsSeg& g = segs[slots[blockIdx.y]];
-    if (!g.flim) return;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < g.flim_elems;
-         i += (int64_t)gridDim.x * blockDim.x)
-        g.flim[i] = 0u;
+__global__ __launch_bounds__(256) void k_ens_fr_noise_count(const EnsSeg* __restrict__ segs,
+                                                            EnsFrameList L,
+                                                            unsigned long long* noise_ctr,
+                                                            uint32_t* wcnt, uint32_t gstep) {
+    const int r = L.slots[blockIdx.y];
+    const EnsSeg& g = segs[r];
+    int64_t q0, npix, w;
+    int nb;
+    if (!ens_fr_wave(g, q0, npix, nb, w)) return;  // whole wavefronts
+    uint32_t* flim = g.flim;
+    uint8_t* flags = row_flags(flim, npix, nb);
+    const int lane = lane_id();
+    const int64_t q = q0 + lane;
+    const bool in = q < npix;
+    uint32_t tot0 = 0u, nz = 0u;
+    for (unsigned long long act = __ballot(in && flags[q] != 0u); act; act &= act - 1ull) {
+        const int j = __builtin_ctzll(act);
+        uint32_t s, c;
+        wave_row_sum_nz(flim + (q0 + j) * nb, nb, s, c);
+        if (lane == j) {
+            tot0 = s;
+            nz = c;
+        }
+    }
+    const Rng rng{g.seed_lo ^ 0x2545F491u, g.seed_hi};
+    unsigned long long nd = 0, na = 0;
+    if (in)
+        nz += frame_noise_pixel(q, nb, flim, flags, tot0, g.ip.dark_lambda, g.ip.p_ap, rng,
+                                gstep - g.step0, nd, na);
+    noise_count(nd, na, noise_ctr + 2 * r);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) nz += __shfl_xor(nz, o);
+    if (wcnt && lane == 0) wcnt[L.wbase[blockIdx.y] + w] = nz;
+}
+
+__global__ __launch_bounds__(256) void k_ens_fr_emit(const EnsSeg* __restrict__ segs,
+                                                     EnsFrameList L,
+                                                     const uint32_t* __restrict__ off,
+                                                     uint32_t* oidx, uint32_t* ocnt) {
+    const EnsSeg& g = segs[L.slots[blockIdx.y]];
+    int64_t q0, npix, w;
+    int nb;
+    if (!ens_fr_wave(g, q0, npix, nb, w)) return;  // whole wavefronts
+    uint32_t* flim = g.flim;
+    uint8_t* flags = row_flags(flim, npix, nb);
+    const int lane = lane_id();
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    const int64_t q = q0 + lane;
+    const bool mine = q < npix && flags[q] != 0u;
+    uint32_t base = oidx ? off[L.wbase[blockIdx.y] + w] : 0u;
+    for (unsigned long long act = __ballot(mine); act; act &= act - 1ull) {
+        const int j = __builtin_ctzll(act);
+        uint32_t* row = flim + (q0 + j) * nb;
+        const uint32_t r0 = (uint32_t)((q0 + j) * nb);
+        for (int b0 = 0; b0 < nb; b0 += 64) {
+            const int b = b0 + lane;
+            const uint32_t v = b < nb ? row[b] : 0u;
+            const unsigned long long mk = __ballot(v != 0u);
+            if (v) {
+                if (oidx) {
+                    const uint32_t pos = base + (uint32_t)__popcll(mk & lt);
+                    oidx[pos] = r0 + (uint32_t)b;
+                    ocnt[pos] = v;
+                }
+                row[b] = 0u;
+            }
+            base += (uint32_t)__popcll(mk);
+        }
+    }
+    if (mine) flags[q] = 0u;
+}
+
+// The listed replicas' first wavefront entries (wbase[n] = total
+// wavefronts); returns the grid's x extent (4 wavefronts per block).  On
+// the device the list is n slots, then the n + 1 offsets (8-byte aligned).
+static int64_t ens_frame_list(edm_ensemble* e, const int32_t* slots, int n,
+                              std::vector<int64_t>& wbase) {
+    wbase.assign((size_t)n + 1, 0);
+    int64_t most = 0;
+    for (int q = 0; q < n; ++q) {
+        const EnsSeg& g = e->seg[slots[q]];
+        const int64_t npix =
+            g.flim ? (int64_t)g.ip.n_ch * g.ip.nx * g.ip.ny : 0;
+        const int64_t nw = (npix + 63) / 64;
+        wbase[q + 1] = wbase[q] + nw;
+        most = std::max(most, nw);
+    }
+    return (most + 3) / 4;
+}
+
+static size_t ens_list_bytes(int n) {
+    return ((size_t)n * sizeof(int32_t) + 7) / 8 * 8 + ((size_t)n + 1) * sizeof(int64_t);
 }
 
 int ens_frame_end(edm_ensemble* e, const int32_t* slots, int32_t n) {
     if (n <= 0) return 0;
     if (e->layout_dirty)
         if (int rc = ens_layout(e)) return rc;
-    if ((size_t)n > e->slot_list_cap) {
+    const size_t bytes = ens_list_bytes(n);
+    if (bytes > e->slot_list_cap) {
         hipFree(e->d_slot_list);
-        e->slot_list_cap = (size_t)std::max(n, e->rmax);
-        EDM_HIP(hipMalloc(&e->d_slot_list, e->slot_list_cap * sizeof(int32_t)));
+        e->slot_list_cap = std::max(bytes, ens_list_bytes(e->rmax));
+        EDM_HIP(hipMalloc(&e->d_slot_list, e->slot_list_cap));
     }
-    EDM_HIP(hipMemcpyAsync(e->d_slot_list, slots, (size_t)n * sizeof(int32_t),
-                           hipMemcpyHostToDevice, e->stream));
-    hipLaunchKernelGGL(k_ens_frame_noise, dim3(32, n), dim3(kB), 0, e->stream,
-                       (const EnsSeg*)e->d_seg, (const int32_t*)e->d_slot_list, e->noise_ctr,
-                       e->gstep);
-    hipLaunchKernelGGL(k_ens_frame_reset, dim3(64, n), dim3(kB), 0, e->stream,
-                       (const EnsSeg*)e->d_seg, (const int32_t*)e->d_slot_list);
+    std::vector<int64_t> wbase;
+    const int64_t gx = ens_frame_list(e, slots, n, wbase);
+    std::vector<char> host(bytes);
+    std::memcpy(host.data(), slots, (size_t)n * sizeof(int32_t));
+    const size_t wo = ((size_t)n * sizeof(int32_t) + 7) / 8 * 8;
+    std::memcpy(host.data() + wo, wbase.data(), wbase.size() * sizeof(int64_t));
+    EDM_HIP(hipMemcpyAsync(e->d_slot_list, host.data(), bytes, hipMemcpyHostToDevice, e->stream));
+    const EnsFrameList L{(const int32_t*)e->d_slot_list,
+                         (const int64_t*)((const char*)e->d_slot_list + wo)};
+    if (gx > 0) {
+        hipLaunchKernelGGL(k_ens_fr_noise_count, dim3((unsigned)gx, n), dim3(kB), 0, e->stream,
+                           (const EnsSeg*)e->d_seg, L, e->noise_ctr, (uint32_t*)nullptr,
+                           e->gstep);
+        hipLaunchKernelGGL(k_ens_fr_emit, dim3((unsigned)gx, n), dim3(kB), 0, e->stream,
+                           (const EnsSeg*)e->d_seg, L, (const uint32_t*)nullptr,
+                           (uint32_t*)nullptr, (uint32_t*)nullptr);
+    }
     EDM_HIP(hipGetLastError());
-    // the slot list must stay valid until the kernels have read it
+    // the list must stay valid until the kernels have read it
     EDM_HIP(hipStreamSynchronize(e->stream));
     return 0;
 }
@@ -255,35 +344,8 @@ int ens_frame_end(edm_ensemble* e, const int32_t* slots, int32_t n) {
 // A collect ends the frames of the listed replicas and returns every frame,
 // sparse: after the detector noise, each stack's nonzero bins are written as
 // (flat index, count) pairs, in index order, replica after replica in list
-// order, then the stacks are reset.  Each stack is cut into kSpBlocks equal
-// chunks; a first pass counts the nonzero bins per chunk, an exclusive scan
-// turns the counts into offsets, a second pass writes each chunk's pairs
-// from its offset (ordered within the chunk by a block-wide prefix of wave
-// ballots).
-static constexpr int kSpBlocks = 64;
-
-__device__ __forceinline__ void sp_chunk(const EnsSeg& g, int b, int64_t& c0, int64_t& c1) {
-    const int64_t n = g.flim ? g.flim_elems : 0;
-    const int64_t per = (n + kSpBlocks - 1) / kSpBlocks;
-    c0 = min(n, (int64_t)b * per);
-    c1 = min(n, c0 + per);
-}
-
-__global__ __launch_bounds__(256) void k_sp_count(const EnsSeg* __restrict__ segs,
-                                                  const int32_t* __restrict__ slots,
-                                                  uint32_t* blk) {
-    const EnsSeg& g = segs[slots[blockIdx.y]];
-    int64_t c0, c1;
-    sp_chunk(g, blockIdx.x, c0, c1);
-    uint32_t c = 0;
-    for (int64_t i = c0 + threadIdx.x; i < c1; i += blockDim.x) c += g.flim[i] != 0u;
-    for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o);
-    __shared__ uint32_t ws[4];
-    if (lane_id() == 0) ws[threadIdx.x >> 6] = c;
-    __syncthreads();
-    if (threadIdx.x == 0)
-        blk[(size_t)blockIdx.y * kSpBlocks + blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
-}
+// order, and the stacks are reset (k_ens_fr_noise_count, k_sp_scan over
+// the wavefront counts, k_ens_fr_emit).
 
 // exclusive scan of blk[0 .. n) in place, blk[n] = total (one block)
 __global__ __launch_bounds__(1024) void k_sp_scan(uint32_t* blk, int n) {
@@ -308,35 +370,6 @@ __global__ __launch_bounds__(1024) void k_sp_scan(uint32_t* blk, int n) {
         __syncthreads();
     }
     if (threadIdx.x == 0) blk[n] = carry;
-}
-
-__global__ __launch_bounds__(256) void k_sp_write(const EnsSeg* __restrict__ segs,
-                                                  const int32_t* __restrict__ slots,
-                                                  const uint32_t* __restrict__ off,
-                                                  uint32_t* oidx, uint32_t* ocnt) {
-    const EnsSeg& g = segs[slots[blockIdx.y]];
-    int64_t c0, c1;
-    sp_chunk(g, blockIdx.x, c0, c1);
-    uint32_t base = off[(size_t)blockIdx.y * kSpBlocks + blockIdx.x];
-    __shared__ uint32_t wtot[4];
-    const int w = threadIdx.x >> 6;
-    for (int64_t t0 = c0; t0 < c1; t0 += blockDim.x) {
-        const int64_t i = t0 + threadIdx.x;
-        const uint32_t v = i < c1 ? g.flim[i] : 0u;
-        const unsigned long long m = __ballot(v != 0u);
-        const uint32_t below = (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull));
-        if (lane_id() == 0) wtot[w] = (uint32_t)__popcll(m);
-        __syncthreads();
-        uint32_t before = 0;
-        for (int q = 0; q < w; ++q) before += wtot[q];
-        if (v != 0u) {
-            oidx[base + before + below] = (uint32_t)i;
-            ocnt[base + before + below] = v;
-        }
-        const uint32_t tile = wtot[0] + wtot[1] + wtot[2] + wtot[3];
-        __syncthreads();
-        base += tile;
-    }
 }
 
 // per listed replica: cumulative dark counts, afterpulses and detected photons
@@ -367,25 +400,32 @@ void ens_frames_free(edm_ensemble* e) {
     e->copy_stream = nullptr;
 }
 
-static int sp_grow_slots(edm_ensemble::FrameBuf& b, int64_t n) {
-    if (n <= b.slot_cap) return 0;
-    const int64_t c = std::max<int64_t>(n, 64);
-    if (b.d_slots) EDM_HIP(hipFree(b.d_slots));
-    if (b.d_blk) EDM_HIP(hipFree(b.d_blk));
-    if (b.d_meta) EDM_HIP(hipFree(b.d_meta));
-    if (b.h_blk) EDM_HIP(hipHostFree(b.h_blk));
-    if (b.h_meta) EDM_HIP(hipHostFree(b.h_meta));
-    b.d_slots = nullptr;
-    b.d_blk = nullptr;
-    b.d_meta = nullptr;
-    b.h_blk = nullptr;
-    b.h_meta = nullptr;
-    EDM_HIP(hipMalloc(&b.d_slots, c * sizeof(int32_t)));
-    EDM_HIP(hipMalloc(&b.d_blk, (c * kSpBlocks + 1) * sizeof(uint32_t)));
-    EDM_HIP(hipMalloc(&b.d_meta, c * 3 * sizeof(unsigned long long)));
-    EDM_HIP(hipHostMalloc((void**)&b.h_blk, (c * kSpBlocks + 1) * sizeof(uint32_t)));
-    EDM_HIP(hipHostMalloc((void**)&b.h_meta, c * 3 * sizeof(unsigned long long)));
-    b.slot_cap = c;
+// Buffers of a collect: the list (n slots and n + 1 wavefront offsets), the
+// per-replica metadata, and nw + 1 wavefront counts / offsets.
+static int sp_grow_slots(edm_ensemble::FrameBuf& b, int64_t n, int64_t nw) {
+    if (n > b.slot_cap) {
+        const int64_t c = std::max<int64_t>(n, 64);
+        if (b.d_slots) EDM_HIP(hipFree(b.d_slots));
+        if (b.d_meta) EDM_HIP(hipFree(b.d_meta));
+        if (b.h_meta) EDM_HIP(hipHostFree(b.h_meta));
+        b.d_slots = nullptr;
+        b.d_meta = nullptr;
+        b.h_meta = nullptr;
+        EDM_HIP(hipMalloc(&b.d_slots, ens_list_bytes((int)c)));
+        EDM_HIP(hipMalloc(&b.d_meta, c * 3 * sizeof(unsigned long long)));
+        EDM_HIP(hipHostMalloc((void**)&b.h_meta, c * 3 * sizeof(unsigned long long)));
+        b.slot_cap = c;
+    }
+    if (nw + 1 > b.blk_cap) {
+        const int64_t c = std::max<int64_t>(nw + 1, 1024);
+        if (b.d_blk) EDM_HIP(hipFree(b.d_blk));
+        if (b.h_blk) EDM_HIP(hipHostFree(b.h_blk));
+        b.d_blk = nullptr;
+        b.h_blk = nullptr;
+        EDM_HIP(hipMalloc(&b.d_blk, c * sizeof(uint32_t)));
+        EDM_HIP(hipHostMalloc((void**)&b.h_blk, c * sizeof(uint32_t)));
+        b.blk_cap = c;
+    }
     return 0;
 }
 
@@ -416,36 +456,44 @@ int ens_frames_collect(edm_ensemble* e, const int32_t* slots, int32_t n, int32_t
     if (!e->copy_stream) EDM_HIP(hipStreamCreate(&e->copy_stream));
     if (!b.copied) EDM_HIP(hipEventCreateWithFlags(&b.copied, hipEventDisableTiming));
     if (!b.written) EDM_HIP(hipEventCreateWithFlags(&b.written, hipEventDisableTiming));
-    if (int rc = sp_grow_slots(b, n)) return rc;
-    hipStream_t st = e->stream;
-    EDM_HIP(hipMemcpyAsync(b.d_slots, slots, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice,
-                           st));
-    const int nblk = n * kSpBlocks;
-    if (n > 0) {
-        hipLaunchKernelGGL(k_ens_frame_noise, dim3(32, n), dim3(kB), 0, st,
-                           (const EnsSeg*)e->d_seg, (const int32_t*)b.d_slots, e->noise_ctr,
-                           e->gstep);
-        hipLaunchKernelGGL(k_sp_count, dim3(kSpBlocks, n), dim3(256), 0, st,
-                           (const EnsSeg*)e->d_seg, (const int32_t*)b.d_slots, b.d_blk);
+    std::vector<int64_t> wbase;
+    const int64_t gx = ens_frame_list(e, slots, n, wbase);
+    const int64_t nw = wbase[(size_t)n];
+    if (nw >= (int64_t)INT32_MAX) {
+        set_error("edm_ens_frames_collect: too many pixel rows in one collect");
+        return EDM_EINVAL;
     }
-    hipLaunchKernelGGL(k_sp_scan, dim3(1), dim3(1024), 0, st, b.d_blk, nblk);
-    EDM_HIP(hipMemcpyAsync(b.h_blk, b.d_blk, ((size_t)nblk + 1) * sizeof(uint32_t),
+    if (int rc = sp_grow_slots(b, n, nw)) return rc;
+    hipStream_t st = e->stream;
+    const size_t wo = ((size_t)n * sizeof(int32_t) + 7) / 8 * 8;
+    // the host copy of the list outlives the transfer: it is freed after the
+    // synchronisation below
+    std::vector<char> host(ens_list_bytes(n));
+    std::memcpy(host.data(), slots, (size_t)n * sizeof(int32_t));
+    std::memcpy(host.data() + wo, wbase.data(), wbase.size() * sizeof(int64_t));
+    EDM_HIP(hipMemcpyAsync(b.d_slots, host.data(), host.size(), hipMemcpyHostToDevice, st));
+    const EnsFrameList L{(const int32_t*)b.d_slots, (const int64_t*)((const char*)b.d_slots + wo)};
+    if (n > 0 && gx > 0)
+        hipLaunchKernelGGL(k_ens_fr_noise_count, dim3((unsigned)gx, n), dim3(kB), 0, st,
+                           (const EnsSeg*)e->d_seg, L, e->noise_ctr, b.d_blk, e->gstep);
+    hipLaunchKernelGGL(k_sp_scan, dim3(1), dim3(1024), 0, st, b.d_blk, (int)nw);
+    EDM_HIP(hipMemcpyAsync(b.h_blk, b.d_blk, ((size_t)nw + 1) * sizeof(uint32_t),
                            hipMemcpyDeviceToHost, st));
     // the offsets size the entry buffers: wait for the noise and the counts
     EDM_HIP(hipStreamSynchronize(st));
-    const int64_t total = b.h_blk[nblk];
+    const int64_t total = b.h_blk[nw];
     if (int rc = sp_grow_entries(b, total)) return rc;
     if (n > 0) {
-        hipLaunchKernelGGL(k_sp_write, dim3(kSpBlocks, n), dim3(256), 0, st,
-                           (const EnsSeg*)e->d_seg, (const int32_t*)b.d_slots,
-                           (const uint32_t*)b.d_blk, b.d_idx, b.d_cnt);
-        hipLaunchKernelGGL(k_ens_frame_reset, dim3(64, n), dim3(kB), 0, st,
-                           (const EnsSeg*)e->d_seg, (const int32_t*)b.d_slots);
+        if (gx > 0)
+            hipLaunchKernelGGL(k_ens_fr_emit, dim3((unsigned)gx, n), dim3(kB), 0, st,
+                               (const EnsSeg*)e->d_seg, L, (const uint32_t*)b.d_blk, b.d_idx,
+                               b.d_cnt);
         hipLaunchKernelGGL(k_sp_meta, dim3(nb(n)), dim3(kB), 0, st, (const int32_t*)b.d_slots, n,
                            (const unsigned long long*)e->noise_ctr,
                            (const unsigned long long*)e->ctr, b.d_meta);
     }
     EDM_HIP(hipGetLastError());
+    b.wbase = wbase;
     // the copy stream moves the entries once the compute stream has written
     // them; the compute stream goes on with the next steps meanwhile
     EDM_HIP(hipEventRecord(b.written, st));
@@ -475,7 +523,7 @@ int ens_frames_fetch(edm_ensemble* e, int32_t ticket, int64_t* counts, int64_t* 
     }
     EDM_HIP(hipEventSynchronize(b.copied));
     const int n = (int)b.slots.size();
-    const int64_t tot = b.h_blk[(size_t)n * kSpBlocks];
+    const int64_t tot = b.h_blk[b.wbase[(size_t)n]];
     if (total) *total = tot;
     if (idx || cnt) {
         if (cap < tot) {
@@ -488,8 +536,8 @@ int ens_frames_fetch(edm_ensemble* e, int32_t ticket, int64_t* counts, int64_t* 
     }
     for (int q = 0; q < n; ++q) {
         if (counts)
-            counts[q] = (int64_t)b.h_blk[(size_t)(q + 1) * kSpBlocks] -
-                        (int64_t)b.h_blk[(size_t)q * kSpBlocks];
+            counts[q] = (int64_t)b.h_blk[b.wbase[(size_t)q + 1]] -
+                        (int64_t)b.h_blk[b.wbase[(size_t)q]];
         if (meta)
             for (int k = 0; k < 3; ++k) meta[3 * q + k] = (int64_t)b.h_meta[3 * q + k];
     }

@@ -42,8 +42,9 @@ struct EnsSeg {
     double prob[kEnsMaxRx];
     int32_t maxr[kEnsMaxRx];
     edm_image_params ip;  // the model's imaging constants with this replica's geometry
-    uint32_t* flim;       // this replica's TCSPC stack
-    int64_t flim_elems;
+    uint32_t* flim;       // this replica's TCSPC stack, then one flag byte per pixel
+    int64_t flim_elems;   // row (set by every writer of the row), padded to 256 bytes
+    int64_t flim_words;   // 32-bit words of the allocation
     double* rcdf;         // CDF of density-biased recruitment on this replica's grid, or nullptr
     // work lists: this replica's region of the pass list [0] and of the dye
     // list [1] starts at entry wl_off (a multiple of the wavefront size, so
@@ -212,11 +213,13 @@ struct edm_ensemble {
     // buffer while the compute stream runs on; the fetch waits for that copy.
     struct FrameBuf {
         int32_t ticket = -1;   // the collect it holds, -1: free
-        int32_t* d_slots = nullptr;
-        uint32_t* d_blk = nullptr;  // per (slot, block): nonzero bins, then offsets
+        int32_t* d_slots = nullptr;  // the listed replicas, then their first wavefront entries
+        uint32_t* d_blk = nullptr;   // per (replica, wavefront of 64 pixel rows): nonzero
+                                     // bins, then offsets
         uint32_t *d_idx = nullptr, *d_cnt = nullptr;
         unsigned long long* d_meta = nullptr;  // per slot: dark, afterpulses, detected
-        int64_t slot_cap = 0, cap = 0;
+        int64_t slot_cap = 0, cap = 0, blk_cap = 0;
+        std::vector<int64_t> wbase;  // per listed replica: its first wavefront entry
         uint32_t* h_blk = nullptr;  // pinned
         uint32_t *h_idx = nullptr, *h_cnt = nullptr;
         unsigned long long* h_meta = nullptr;

@@ -415,8 +415,8 @@ __device__ __forceinline__ bool psf_pick(const edm_image_params* ip, const doubl
 // stored right after the stack's n_pix x n_bins counts and set by every
 // kernel that adds counts to a row, so that a clear flag means an empty row
 // and a frame end reads only the flagged rows.  Concurrent writers all
-// store 1.  The ensemble's per-replica stacks keep no flags (nullptr).
-__device__ __forceinline__ uint8_t* row_flags(uint32_t* flim, int64_t npix, int nb) {
+// store 1.  Every stack, a single membrane's and each replica's, has them.
+__host__ __device__ __forceinline__ uint8_t* row_flags(uint32_t* flim, int64_t npix, int nb) {
     return (uint8_t*)(flim + npix * (int64_t)nb);
 }
 
@@ -1420,22 +1420,6 @@ int frame_reset(edm_membrane* m) {
     return 0;
 }
 
-#include "edm_ens_ph.h"
-
-// ----------------------------------------------------------------------
-// Streamed frames of one membrane (microscope.py:1159-1203 for every frame,
-// the reference keeping them all, pipeline.py:645).  edm_frame_collect ends
-// the current frame: the steps that follow write their photons into a
-// second stack, while a second stream gives the ended stack its detector
-// noise (the same draws as edm_frame_noise at this step), compacts it to
-// (flat index, count) pairs in index order, and resets it.  edm_frame_fetch
-// waits for that work and copies the pairs out.  One frame may be pending.
-// ----------------------------------------------------------------------
-// Both passes run one wavefront per 64 pixel rows and touch only the rows
-// whose flag is set (at 1024 x 1024 px and ~3.6e5 counts a frame, ~30 % of
-// them), so a frame end moves about a fifth of the stack instead of reading
-// it whole three times and writing it once.
-
 // Sum and nonzero-bin count of one row, read coalesced by the wavefront
 // (16 B per lane when nb % 4 == 0); wave-uniform arguments, every lane calls.
 __device__ __forceinline__ void wave_row_sum_nz(const uint32_t* row, int nb, uint32_t& sum,
@@ -1464,6 +1448,22 @@ __device__ __forceinline__ void wave_row_sum_nz(const uint32_t* row, int nb, uin
     sum = s;
     nz = c;
 }
+
+#include "edm_ens_ph.h"
+
+// ----------------------------------------------------------------------
+// Streamed frames of one membrane (microscope.py:1159-1203 for every frame,
+// the reference keeping them all, pipeline.py:645).  edm_frame_collect ends
+// the current frame: the steps that follow write their photons into a
+// second stack, while a second stream gives the ended stack its detector
+// noise (the same draws as edm_frame_noise at this step), compacts it to
+// (flat index, count) pairs in index order, and resets it.  edm_frame_fetch
+// waits for that work and copies the pairs out.  One frame may be pending.
+// ----------------------------------------------------------------------
+// Both passes run one wavefront per 64 pixel rows and touch only the rows
+// whose flag is set (at 1024 x 1024 px and ~3.6e5 counts a frame, ~30 % of
+// them), so a frame end moves about a fifth of the stack instead of reading
+// it whole three times and writing it once.
 
 // First pass: the flagged rows' totals (for the afterpulses) and nonzero
 // bins, then every pixel's detector noise drawn as edm_frame_noise draws it
