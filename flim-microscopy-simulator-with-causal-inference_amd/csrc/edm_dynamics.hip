@@ -291,17 +291,21 @@ __global__ void k_scatter(int n, const uint32_t* __restrict__ key, const uint32_
 // One thread per cell: order the cell's particles by particle index (the
 // atomic ranks of k_bin are arbitrary) and clear the cell counter.  Cells of
 // up to 8 particles are sorted in registers (all loads issued together);
-// larger ones by insertion in place.
+// larger ones by the whole wavefront, one member per lane, each member's
+// place its count of smaller indices.
 __global__ void k_sort_cells(int ncell, const uint32_t* __restrict__ start, uint32_t* perm,
                              uint32_t* cnt, const int32_t* __restrict__ id) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
-    cnt[c] = 0;
-    const int lo = (int)start[c], hi = (int)start[c + 1];
+    const bool own = c < ncell;  // the whole wavefront stays for the large cells
+    int lo = 0, hi = 0;
+    if (own) {
+        cnt[c] = 0;
+        lo = (int)start[c];
+        hi = (int)start[c + 1];
+    }
     const int m = hi - lo;
-    if (m < 2) return;
     constexpr int K = 8;
-    if (m <= K) {
+    if (own && m >= 2 && m <= K) {
         uint32_t v[K];
         int32_t k[K];
 #pragma unroll
@@ -325,17 +329,30 @@ __global__ void k_sort_cells(int ncell, const uint32_t* __restrict__ start, uint
 #pragma unroll
         for (int j = 0; j < K; ++j)
             if (j < m) perm[lo + j] = v[j];
-        return;
     }
-    for (int i = lo + 1; i < hi; ++i) {
-        const uint32_t v = perm[i];
-        const int32_t vid = id[v];
-        int j = i - 1;
-        while (j >= lo && id[perm[j]] > vid) {
-            perm[j + 1] = perm[j];
-            --j;
+    const int lane = lane_id();
+    for (uint64_t big = __ballot(own && m > K); big; big &= big - 1ull) {
+        const int L = __builtin_ctzll(big);
+        const int blo = __shfl(lo, L), bm = __shfl(m, L);
+        if (bm <= 64) {
+            const bool in = lane < bm;
+            const uint32_t v = in ? perm[blo + lane] : 0u;
+            const int32_t k = in ? id[v] : 0x7fffffff;
+            int r = 0;
+            for (int j = 0; j < bm; ++j) r += __shfl(k, j) < k;
+            if (in) perm[blo + r] = v;  // every member was read above
+        } else if (lane == L) {
+            for (int i = lo + 1; i < hi; ++i) {
+                const uint32_t v = perm[i];
+                const int32_t vid = id[v];
+                int j = i - 1;
+                while (j >= lo && id[perm[j]] > vid) {
+                    perm[j + 1] = perm[j];
+                    --j;
+                }
+                perm[j + 1] = v;
+            }
         }
-        perm[j + 1] = v;
     }
 }
 
@@ -880,9 +897,13 @@ __global__ void k_sort_cells_fine(int ncell, const uint32_t* __restrict__ start,
                                   uint32_t* cnt, const uint32_t* __restrict__ kid, int nsub,
                                   uint32_t* __restrict__ fstart) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
-    cnt[c] = 0;
-    const int lo = (int)start[c], hi = (int)start[c + 1];
+    const bool own = c < ncell;  // the whole wavefront stays for the large cells
+    int lo = 0, hi = 0;
+    if (own) {
+        cnt[c] = 0;
+        lo = (int)start[c];
+        hi = (int)start[c + 1];
+    }
     const int m = hi - lo;
     // (sub-row << 28 | particle index), written by k_bin
     auto key = [&](uint32_t v) { return (int32_t)kid[v]; };
@@ -890,7 +911,7 @@ __global__ void k_sort_cells_fine(int ncell, const uint32_t* __restrict__ start,
     int nb[kMaxSub + 1];
 #pragma unroll
     for (int q = 0; q <= kMaxSub; ++q) nb[q] = 0;  // members below sub-row q
-    if (m <= K) {
+    if (own && m <= K) {
         uint32_t v[K];
         int32_t k[K];
 #pragma unroll
@@ -923,23 +944,47 @@ __global__ void k_sort_cells_fine(int ncell, const uint32_t* __restrict__ start,
 #pragma unroll
             for (int q = 1; q <= kMaxSub; ++q) nb[q] += b < q;
         }
-    } else {
-        for (int i = lo + 1; i < hi; ++i) {
-            const uint32_t v = perm[i];
-            const int32_t vk = key(v);
-            int j = i - 1;
-            while (j >= lo && key(perm[j]) > vk) {
-                perm[j + 1] = perm[j];
-                --j;
-            }
-            perm[j + 1] = v;
-        }
-        for (int i = lo; i < hi; ++i) {
-            const int b = (int)(kid[perm[i]] >> 28);
+    }
+    // Cells of more than K (about 1 % at a mean of 3 per cell): the
+    // wavefront sorts each in turn with one member per lane, each member's
+    // place its count of smaller keys (keys are distinct), instead of one
+    // lane inserting with a dependent pair of loads per step.
+    const int lane = lane_id();
+    for (uint64_t big = __ballot(own && m > K); big; big &= big - 1ull) {
+        const int L = __builtin_ctzll(big);
+        const int blo = __shfl(lo, L), bm = __shfl(m, L);
+        if (bm <= 64) {
+            const bool in = lane < bm;
+            const uint32_t v = in ? perm[blo + lane] : 0u;
+            const int32_t k = in ? key(v) : 0x7fffffff;
+            int r = 0;
+            for (int j = 0; j < bm; ++j) r += __shfl(k, j) < k;
+            if (in) perm[blo + r] = v;  // every member was read above
+            const int b = in ? (int)((uint32_t)k >> 28) : kMaxSub;
 #pragma unroll
-            for (int q = 1; q <= kMaxSub; ++q) nb[q] += b < q;
+            for (int q = 1; q <= kMaxSub; ++q) {
+                const int below = __popcll(__ballot(b < q));
+                if (lane == L) nb[q] = below;
+            }
+        } else if (lane == L) {
+            for (int i = lo + 1; i < hi; ++i) {
+                const uint32_t v = perm[i];
+                const int32_t vk = key(v);
+                int j = i - 1;
+                while (j >= lo && key(perm[j]) > vk) {
+                    perm[j + 1] = perm[j];
+                    --j;
+                }
+                perm[j + 1] = v;
+            }
+            for (int i = lo; i < hi; ++i) {
+                const int b = (int)(kid[perm[i]] >> 28);
+#pragma unroll
+                for (int q = 1; q <= kMaxSub; ++q) nb[q] += b < q;
+            }
         }
     }
+    if (!own) return;
     // the cell's 8 entries as two 16-byte stores (rows of 32 bytes, so a
     // wavefront writes one contiguous 2 KiB run)
     static_assert(kMaxSub == 8, "fstart rows are two uint4");
@@ -2143,7 +2188,11 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
     // the fused photon kernel of this step may start here (ph_fused_image)
     m->force_mark = false;
     if (!hook && m->ph_overlap_ok && !m->use_graphs && !m->graph_rec) {
-        if (!m->ev_force) EDM_HIP(hipEventCreateWithFlags(&m->ev_force, hipEventDisableTiming));
+        // ordering between two streams of this device only: no system-scope
+        // fence (no write-back for the host) when the event is recorded
+        if (!m->ev_force)
+            EDM_HIP(hipEventCreateWithFlags(&m->ev_force,
+                                            hipEventDisableTiming | hipEventDisableSystemFence));
         EDM_HIP(hipEventRecord(m->ev_force, st));
         m->force_mark = true;
     }

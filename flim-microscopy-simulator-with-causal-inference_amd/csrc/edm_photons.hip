@@ -1214,7 +1214,9 @@ int ph_fused_image(edm_membrane* m, int64_t* n_emitted, int64_t* n_detected) {
         m->force_mark && m->ph_overlap_ok && !prox && !m->use_graphs && !m->graph_rec;
     if (beside) {
         if (!m->ph_stream) EDM_HIP(hipStreamCreate(&m->ph_stream));
-        if (!m->ev_photons) EDM_HIP(hipEventCreateWithFlags(&m->ev_photons, hipEventDisableTiming));
+        if (!m->ev_photons)
+            EDM_HIP(hipEventCreateWithFlags(&m->ev_photons,
+                                            hipEventDisableTiming | hipEventDisableSystemFence));
         EDM_HIP(hipStreamWaitEvent(m->ph_stream, m->ev_force, 0));
         ps = m->ph_stream;
     }
