@@ -266,6 +266,7 @@ int edm_upload_state(edm_membrane* m, const edm_host_state* s) {
     }
     EDM_HIP(hipMemset(m->wl_cnt, 0, (size_t)kWlMaxLists * nb * sizeof(uint32_t)));
     m->wl_valid = false;
+    m->wl_pend_n = 0;  // a pending build would write the old lists
     return 0;
 }
 

@@ -409,27 +409,32 @@ __device__ __forceinline__ bool wl_member(const WlSpec& S, int8_t sp, uint8_t fl
     return ((S.species >> sp) & 1u) && (fl & S.flags) == S.flags;
 }
 
-__global__ __launch_bounds__(256) void k_wl_build(int n, const int8_t* __restrict__ sp,
-                                                  const uint8_t* __restrict__ fl, WlBuild W) {
+// Build block `blk` (256 threads; every thread of the block calls it).
+__device__ __forceinline__ void wl_build_region(int n, const int8_t* __restrict__ sp,
+                                               const uint8_t* __restrict__ fl, const WlBuild& W,
+                                               int blk) {
+    static_assert(kWlPer * kWlMaxLists <= 64, "membership bits of a thread");
     constexpr int kW = 4;  // wavefronts per block
     __shared__ int s_off[kWlMaxLists][kWlPer * kW];  // members per (row, wave), then offsets
-    const int beg = (int)blockIdx.x * kWlSlots, end = min(n, beg + kWlSlots);
+    const int beg = blk * kWlSlots, end = min(n, beg + kWlSlots);
     const int lane = lane_id(), w = (int)(threadIdx.x >> 6);
-    // row k of the block holds slots beg + k * 256 + thread (coalesced loads)
-    int8_t vs[kWlPer];
-    uint8_t vf[kWlPer];
+    // row k of the block holds slots beg + k * 256 + thread (coalesced
+    // loads); bit k * kWlMaxLists + L: that slot is a member of list L
+    uint64_t memb = 0;
 #pragma unroll
     for (int k = 0; k < kWlPer; ++k) {
         const int t = beg + k * 256 + (int)threadIdx.x;
-        vs[k] = t < end ? sp[t] : (int8_t)0;
-        vf[k] = t < end ? fl[t] : (uint8_t)0;
+        if (t < end) {
+            const int8_t vs = sp[t];
+            const uint8_t vf = fl[t];
+            for (int L = 0; L < W.n_lists; ++L)
+                if (wl_member(W.spec[L], vs, vf)) memb |= 1ull << (k * kWlMaxLists + L);
+        }
     }
     for (int L = 0; L < W.n_lists; ++L) {
-        const WlSpec S = W.spec[L];
 #pragma unroll
         for (int k = 0; k < kWlPer; ++k) {
-            const bool live = beg + k * 256 + (int)threadIdx.x < end;
-            const uint64_t b = __ballot(live && wl_member(S, vs[k], vf[k]));
+            const uint64_t b = __ballot((memb >> (k * kWlMaxLists + L)) & 1ull);
             if (lane == 0) s_off[L][k * kW + w] = __popcll(b);
         }
     }
@@ -442,21 +447,24 @@ __global__ __launch_bounds__(256) void k_wl_build(int n, const int8_t* __restric
             s_off[L][i] = run;
             run += c;
         }
-        W.cnt[L][blockIdx.x] = (uint32_t)run;
+        W.cnt[L][blk] = (uint32_t)run;
     }
     __syncthreads();
     const uint64_t below = (1ull << lane) - 1ull;
     for (int L = 0; L < W.n_lists; ++L) {
-        const WlSpec S = W.spec[L];
-        int32_t* out = W.list[L] + (size_t)blockIdx.x * kWlSlots;
+        int32_t* out = W.list[L] + (size_t)blk * kWlSlots;
 #pragma unroll
         for (int k = 0; k < kWlPer; ++k) {
-            const int t = beg + k * 256 + (int)threadIdx.x;
-            const bool mem = t < end && wl_member(S, vs[k], vf[k]);
+            const bool mem = (memb >> (k * kWlMaxLists + L)) & 1ull;
             const uint64_t b = __ballot(mem);
-            if (mem) out[s_off[L][k * kW + w] + __popcll(b & below)] = t;
+            if (mem) out[s_off[L][k * kW + w] + __popcll(b & below)] = beg + k * 256 + (int)threadIdx.x;
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_wl_build(int n, const int8_t* __restrict__ sp,
+                                                  const uint8_t* __restrict__ fl, WlBuild W) {
+    wl_build_region(n, sp, fl, W, (int)blockIdx.x);
 }
 
 // Ensemble list build (one replica's chunk): the chunk's members of both
@@ -1008,10 +1016,19 @@ __global__ __launch_bounds__(256) void k_force_fine(int n, Cells<Real> C, ForceP
                                                     Rng rng, StepRef sref, Grids<Real> G,
                                                     int n_grids, uint32_t keep_mask, int nsub,
                                                     Real inv_h,
-                                                    const uint32_t* __restrict__ fstart) {
+                                                    const uint32_t* __restrict__ fstart,
+                                                    WlBuild W, int wl_nb,
+                                                    const int8_t* __restrict__ wl_sp,
+                                                    const uint8_t* __restrict__ wl_fl) {
+    // the first wl_nb blocks build the step's work lists beside the forces
+    // (they read only species and flags, which this kernel does not write)
+    if ((int)blockIdx.x < wl_nb) {
+        wl_build_region(n, wl_sp, wl_fl, W, (int)blockIdx.x);
+        return;
+    }
     if (cap_gated_off(C)) return;  // uniform across the launch
     const uint32_t step = sref.get();
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = ((int)blockIdx.x - wl_nb) * (int)blockDim.x + (int)threadIdx.x;
     bool take_any = false;
     Real nxp = 0, nyp = 0;
     int si = 0;
@@ -1962,11 +1979,7 @@ static void plan_cat_walks(edm_membrane* m, int& next) {
 }
 
 
-// Builds the listed work lists from the current arrays (sp, fl: the slot
-// order of this step).
-static void wl_build(edm_membrane* m, const int8_t* sp, const uint8_t* fl, const int* lists,
-                     const WlSpec* specs, int nl) {
-    if (nl == 0 || m->n == 0 || !m->wl) return;
+static WlBuild wl_make(edm_membrane* m, const int* lists, const WlSpec* specs, int nl) {
     WlBuild W{};
     W.n_lists = nl;
     for (int i = 0; i < nl; ++i) {
@@ -1975,6 +1988,28 @@ static void wl_build(edm_membrane* m, const int8_t* sp, const uint8_t* fl, const
         W.cnt[i] = m->wl_cnt + (size_t)lists[i] * m->wl_nblk;
         m->wl_built |= 1u << lists[i];
     }
+    return W;
+}
+
+// Launches the build staged by the step's sort (the force launch that
+// follows takes it over when it can).
+int dyn_wl_flush(edm_membrane* m) {
+    if (!m->wl_pend_n) return 0;
+    const WlBuild W = wl_make(m, m->wl_pend_list, m->wl_pend_spec, m->wl_pend_n);
+    m->wl_pend_n = 0;
+    hipLaunchKernelGGL(k_wl_build, dim3(m->wl_nblk), dim3(256), 0, m->stream, m->n,
+                       m->wl_pend_sp, m->wl_pend_fl, W);
+    EDM_HIP(hipGetLastError());
+    return 0;
+}
+
+// Builds the listed work lists from the current arrays (sp, fl: the slot
+// order of this step).
+static void wl_build(edm_membrane* m, const int8_t* sp, const uint8_t* fl, const int* lists,
+                     const WlSpec* specs, int nl) {
+    if (nl == 0 || m->n == 0 || !m->wl) return;
+    dyn_wl_flush(m);
+    const WlBuild W = wl_make(m, lists, specs, nl);
     hipLaunchKernelGGL(k_wl_build, dim3(m->wl_nblk), dim3(256), 0, m->stream, m->n, sp, fl, W);
 }
 
@@ -2052,7 +2087,17 @@ static int begin_t(edm_membrane* m) {
                 specs[nl++] = c.spec;
             }
         m->wl_built = 0;
-        wl_build(m, (const int8_t*)m->a.sp_t, (const uint8_t*)m->a.fl_t, lists, specs, nl);
+        m->wl_pend_n = 0;
+        if (nl > 0 && n > 0) {
+            // staged: built by the force launch of this step, or by
+            // dyn_wl_flush before anything else reads a list
+            std::copy(lists, lists + nl, m->wl_pend_list);
+            std::copy(specs, specs + nl, m->wl_pend_spec);
+            m->wl_pend_sp = (const int8_t*)m->a.sp_t;
+            m->wl_pend_fl = (const uint8_t*)m->a.fl_t;
+            m->wl_pend_n = nl;
+            for (int i = 0; i < nl; ++i) m->wl_built |= 1u << lists[i];
+        }
         m->wl_valid = true;
     }
     std::swap(m->a.x, m->a.xt);
@@ -2145,6 +2190,23 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
                 break;
             }
     const double* noise = hook ? draws : nullptr;
+    // the staged work lists go with k_force_fine; any other force kernel
+    // has them built first
+    const bool fine_launch = !(hook && m->fp64 && p.exact_force_order) && !C.cap_active &&
+                             m->fine_active && !hook;
+    WlBuild Wf{};
+    int wl_nb = 0;
+    const int8_t* wl_sp = nullptr;
+    const uint8_t* wl_fl = nullptr;
+    if (fine_launch && m->wl_pend_n) {
+        Wf = wl_make(m, m->wl_pend_list, m->wl_pend_spec, m->wl_pend_n);
+        wl_nb = m->wl_nblk;
+        wl_sp = m->wl_pend_sp;
+        wl_fl = m->wl_pend_fl;
+        m->wl_pend_n = 0;
+    } else if (int rc = dyn_wl_flush(m)) {
+        return rc;
+    }
     if (m->time_force) EDM_HIP(hipEventRecord(m->kev[0], st));
     if (hook && m->fp64 && p.exact_force_order)
         hipLaunchKernelGGL((k_force_bd<Real, true>), dim3(nblk(n)), dim3(kBlock), 0, st, n, C, F,
@@ -2169,11 +2231,11 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
         const int ntile_y = (p.ncy + m->tile_rows - 1) / m->tile_rows;
         const int ntiles = p.ncx * ntile_y;
         const dim3 grid(8 * ((ntiles + 7) / 8));
-        if (m->fine_active && !hook)
-            hipLaunchKernelGGL((k_force_fine<Real>), dim3(nblk(n)), dim3(256), 0, st, n, Ct, F,
-                               (Real*)m->a.xn, (Real*)m->a.yn, rng, step, G, ng,
+        if (fine_launch)
+            hipLaunchKernelGGL((k_force_fine<Real>), dim3(nblk(n) + wl_nb), dim3(256), 0, st, n,
+                               Ct, F, (Real*)m->a.xn, (Real*)m->a.yn, rng, step, G, ng,
                                m->sensor_mask_all, m->fine_nsub, (Real)m->fine_inv_h,
-                               (const uint32_t*)m->fstart);
+                               (const uint32_t*)m->fstart, Wf, wl_nb, wl_sp, wl_fl);
         else if (m->sub_rows)
             hipLaunchKernelGGL((k_force_tile<Real, true>), grid, dim3(256),
                                force_sub_lds(m->tile_rows, 1), st, Ct, F, (Real*)m->a.xn,

@@ -211,6 +211,14 @@ struct edm_membrane {
     int wl_nblk = 0;
     bool wl_valid = false;
     uint32_t wl_built = 0;  // lists built since the last sort (bit per list)
+    // the lists of the step's sort, built by the step's force launch (or by
+    // dyn_wl_flush before any other reader): n lists, their specs and the
+    // species and flag arrays of this step's slot order
+    int wl_pend_n = 0;
+    int wl_pend_list[kWlMaxLists];
+    WlSpec wl_pend_spec[kWlMaxLists];
+    const int8_t* wl_pend_sp = nullptr;
+    const uint8_t* wl_pend_fl = nullptr;
     std::vector<CatPlan> cat_plan;    // this step's catalysis runs
     std::vector<PassPlan> pass_plan;  // this step's runs of per-particle passes
     std::vector<int> group_start;    // per reaction: group starting here, else -1
@@ -354,7 +362,12 @@ static inline bool loop_pair_offsets(const edm_membrane* m) {
     return m->p.pair_cap > 0 || m->has_autocat;
 }
 
-static inline WlView wl_view(const edm_membrane* m, int L) {
+namespace edm {
+int dyn_wl_flush(edm_membrane* m);
+}
+
+static inline WlView wl_view(edm_membrane* m, int L) {
+    if (m->wl_pend_n) edm::dyn_wl_flush(m);
     if (!m->wl_valid || !((m->wl_built >> L) & 1u)) return WlView{nullptr, nullptr};
     return WlView{m->wl + (size_t)L * m->wl_nblk * kWlSlots,
                   m->wl_cnt + (size_t)L * m->wl_nblk};
