@@ -245,6 +245,9 @@ int edm_upload_state(edm_membrane* m, const edm_host_state* s) {
         m->ph_overlap_ok = ok;
     }
     m->keep_slot_of = bonds;
+    // without bonds the step does not gather the bond arrays: both hold -1
+    if (!bonds && n > 0)
+        EDM_HIP(hipMemcpy(m->a.bond_t, bond.data(), (size_t)n * 4, hipMemcpyHostToDevice));
     m->slot_of_valid = true;  // the identity, copied above
     m->alive_at_upload.assign(m->p.n_species, 0);
     int64_t n_alive = 0;

@@ -358,7 +358,9 @@ __global__ void k_sort_cells(int ncell, const uint32_t* __restrict__ start, uint
 
 // One thread per new slot: gather every field from its old slot (the
 // permutation is close to the identity, so reads stay nearly coalesced).
-template <typename Real>
+// Without bonds (kBond false) both bond arrays hold -1 everywhere and are
+// left alone.
+template <typename Real, bool kBond>
 __global__ void k_gather(int n, const uint32_t* __restrict__ perm, DevArrays a,
                          bool write_slot_of) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -370,7 +372,7 @@ __global__ void k_gather(int n, const uint32_t* __restrict__ perm, DevArrays a,
     a.fl_t[t] = a.fl[s];
     const int32_t pid = a.id[s];
     a.id_t[t] = pid;
-    a.bond_t[t] = a.bond[s];
+    if (kBond) a.bond_t[t] = a.bond[s];
     a.ctmc_t[t] = a.ctmc[s];
     a.dye_t[t] = a.dye[s];
     if (write_slot_of) a.slot_of[pid] = t;
@@ -2059,8 +2061,10 @@ static int begin_t(edm_membrane* m) {
         hipLaunchKernelGGL(k_sort_cells, dim3(nblk(m->ncell)), dim3(kBlock), 0, st, m->ncell,
                            m->cell_start, m->perm, m->cell_cnt, (const int32_t*)m->a.id);
     }
-    hipLaunchKernelGGL(k_gather<Real>, dim3(nblk(n)), dim3(kBlock), 0, st, n,
-                       (const uint32_t*)m->perm, m->a, m->keep_slot_of);
+    // keep_slot_of: bonds can exist
+    hipLaunchKernelGGL((m->keep_slot_of ? k_gather<Real, true> : k_gather<Real, false>),
+                       dim3(nblk(n)), dim3(kBlock), 0, st, n, (const uint32_t*)m->perm, m->a,
+                       m->keep_slot_of);
     m->slot_of_valid = m->keep_slot_of;
     if (m->wl) {
         // work lists of this step's slot order (the gather wrote it into the
