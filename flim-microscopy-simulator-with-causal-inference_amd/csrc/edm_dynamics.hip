@@ -2251,17 +2251,27 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
                                m->sensor_mask_all, ntile_y, ntiles);
     }
     if (m->time_force) EDM_HIP(hipEventRecord(m->kev[1], st));
-    // the fused photon kernel of this step may start here (ph_fused_image)
+    // the fused photon kernel of this step (ph_fused_image) may start once
+    // the forces are done; it is released before the first run of
+    // per-particle passes (those latency-bound launches leave the most room
+    // beside them), or at the end of the reactions
     m->force_mark = false;
+    bool mark_pending = false;
     if (!hook && m->ph_overlap_ok && !m->use_graphs && !m->graph_rec) {
         // ordering between two streams of this device only: no system-scope
         // fence (no write-back for the host) when the event is recorded
         if (!m->ev_force)
             EDM_HIP(hipEventCreateWithFlags(&m->ev_force,
                                             hipEventDisableTiming | hipEventDisableSystemFence));
+        mark_pending = true;
+    }
+    auto release_photons = [&]() -> int {
+        if (!mark_pending) return 0;
         EDM_HIP(hipEventRecord(m->ev_force, st));
         m->force_mark = true;
-    }
+        mark_pending = false;
+        return 0;
+    };
 
     int64_t off = 2 * (int64_t)n;
     const int64_t npairs = m->n_pairs;
@@ -2473,6 +2483,7 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
                                          &pp.list, &pp.spec, 1);
                             V = wl_view(m, pp.list);
                         }
+                    if (int rc = release_photons()) return rc;
                     hipLaunchKernelGGL((local ? k_passes<Real, true> : k_passes<Real, false>),
                                        V.list ? dim3(m->wl_nblk, kWlPer) : dim3(nblk(n)),
                                        dim3(kBlock), 0, st, n, V,
@@ -2496,6 +2507,7 @@ static int finish_t(edm_membrane* m, const double* draws, bool hook) {
             default: break;
         }
     }
+    if (int rc = release_photons()) return rc;
     // post-move positions become the current positions
     std::swap(m->a.x, m->a.xn);
     std::swap(m->a.y, m->a.yn);
