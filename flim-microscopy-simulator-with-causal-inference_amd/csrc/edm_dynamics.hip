@@ -292,11 +292,13 @@ __global__ void k_scatter(int n, const uint32_t* __restrict__ key, const uint32_
 // atomic ranks of k_bin are arbitrary) and clear the cell counter.  Cells of
 // up to 8 particles are sorted in registers (all loads issued together);
 // larger ones by the whole wavefront, one member per lane, each member's
-// place its count of smaller indices.
+// place its count of smaller indices.  Thread 0 also clears cnt[ncell] when
+// clear_next is set (a bucket after the sorted cells).
 __global__ void k_sort_cells(int ncell, const uint32_t* __restrict__ start, uint32_t* perm,
-                             uint32_t* cnt, const int32_t* __restrict__ id) {
+                             uint32_t* cnt, const int32_t* __restrict__ id, int clear_next) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const bool own = c < ncell;  // the whole wavefront stays for the large cells
+    if (clear_next && c == 0) cnt[ncell] = 0;
     int lo = 0, hi = 0;
     if (own) {
         cnt[c] = 0;
@@ -2059,7 +2061,7 @@ static int begin_t(edm_membrane* m) {
                            (const uint32_t*)m->fkey, m->fine_nsub, m->fstart);
     } else {
         hipLaunchKernelGGL(k_sort_cells, dim3(nblk(m->ncell)), dim3(kBlock), 0, st, m->ncell,
-                           m->cell_start, m->perm, m->cell_cnt, (const int32_t*)m->a.id);
+                           m->cell_start, m->perm, m->cell_cnt, (const int32_t*)m->a.id, 0);
     }
     // keep_slot_of: bonds can exist
     hipLaunchKernelGGL((m->keep_slot_of ? k_gather<Real, true> : k_gather<Real, false>),

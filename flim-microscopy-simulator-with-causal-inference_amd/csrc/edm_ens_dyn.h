@@ -230,7 +230,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFp32Waves<
                                                  const EnsSeg* __restrict__ segs, EnsPool<Real> Q,
                                                  const CatArgs<Real>* __restrict__ cats,
                                                  int n_runs, int ci, unsigned long long* ctr,
-                                                 uint32_t gstep) {
+                                                 uint32_t gstep, int32_t* fold_counts,
+                                                 int32_t* fold_delta, int fold_n) {
+    // the previous run's deferred count updates (this run reads no counts)
+    if (fold_n > 0) fold_in_launch(fold_counts, fold_delta, fold_n);
     const int tile = xcd_tile(ntiles);
     if (tile >= ntiles) return;
     const EnsTile T = tiles[tile];
@@ -259,7 +262,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFp32Waves<
                                                       EnsPool<Real> Q,
                                                       const CatArgs<Real>* __restrict__ cats,
                                                       int n_runs, int ci,
-                                                      unsigned long long* ctr, uint32_t gstep) {
+                                                      unsigned long long* ctr, uint32_t gstep,
+                                                      int32_t* fold_counts, int32_t* fold_delta,
+                                                      int fold_n) {
+    // the previous run's deferred count updates (this run reads no counts)
+    if (fold_n > 0) fold_in_launch(fold_counts, fold_delta, fold_n);
     if ((int)blockIdx.x * (int)(blockDim.x >> 6) +
             __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) >= n_waves)
         return;  // whole waves
@@ -974,9 +981,10 @@ static int ens_step_t(edm_ensemble* e) {
         if (e->n_cells > 0)
             hipLaunchKernelGGL(k_sort_cells, dim3(nblk(e->n_cells)), dim3(kBlock), 0, st,
                                (int)e->n_cells, e->cell_start, e->perm, e->cell_cnt,
-                               (const int32_t*)e->a.id);
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, 1,
-                           (unsigned int*)(e->cell_cnt + e->n_cells));
+                               (const int32_t*)e->a.id, 1);
+        else
+            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, 1,
+                               (unsigned int*)(e->cell_cnt + e->n_cells));
         hipLaunchKernelGGL((e->has_bind ? k_ens_gather<Real, true> : k_ens_gather<Real, false>),
                            dim3(nblk(n)), dim3(kBlock), 0, st, n, (const uint32_t*)e->perm, e->a,
                            (const uint16_t*)e->rep, e->rep_t, (const EnsSeg*)e->d_seg);
@@ -1062,6 +1070,14 @@ static int ens_step_t(edm_ensemble* e) {
                            (const EnsTile*)e->d_tiles, ntiles, (const EnsSeg*)e->d_seg, Q, F,
                            e->sensor_mask_all, gstep);
 
+    bool fold_pending = false;  // a catalysis run's deferred updates, not yet folded
+    // a catalysis run that reads no counts (no feedback, not autocatalytic)
+    // can fold the previous run's deferred updates in its own launch
+    auto folds_in = [&](size_t j) {
+        if (j >= e->rx.size()) return false;
+        const edm_reaction& q = e->rx[j];
+        return q.kind == EDM_RX_CAT && q.a != q.c && q.fb1.sensor < 0 && e->cat_run_of[j] >= 0;
+    };
     for (size_t ri = 0; ri < e->rx.size(); ++ri) {
         const edm_reaction& r = e->rx[ri];
         if (r.kind == EDM_RX_BIND) {
@@ -1099,23 +1115,33 @@ static int ens_step_t(edm_ensemble* e) {
                                    (int)e->cat_runs.size(), ci,
                                    (const unsigned long long*)e->ekey, (const int32_t*)e->ea,
                                    (const int32_t*)e->eb, e->ne, e->tconv, e->ctr);
-            } else if (e->cat_walk == 1 && e->cat_list_ok[ci] && e->wl_npass > 0)
-                hipLaunchKernelGGL((k_ens_cat_list<Real>), dim3(nblk(e->wl_npass)), dim3(256), 0,
-                                   st, (int)(e->wl_npass / 64), (const int32_t*)e->wl_pass,
-                                   (const int32_t*)e->wl_wrep_pass, (const uint32_t*)e->wl_rcnt,
-                                   (const EnsSeg*)e->d_seg, Q, (const CatArgs<Real>*)e->d_cats,
-                                   (int)e->cat_runs.size(), ci, e->ctr, gstep);
-            else
-                hipLaunchKernelGGL((k_ens_cat<Real>), tgrid, dim3(256), 0, st,
-                                   (const EnsTile*)e->d_tiles, ntiles, (const EnsSeg*)e->d_seg, Q,
-                                   (const CatArgs<Real>*)e->d_cats, (int)e->cat_runs.size(), ci,
-                                   e->ctr, gstep);
+            } else {
+                const int fold_n = fold_pending ? (int)e->n_counts : 0;
+                fold_pending = false;
+                if (e->cat_walk == 1 && e->cat_list_ok[ci] && e->wl_npass > 0)
+                    hipLaunchKernelGGL((k_ens_cat_list<Real>), dim3(nblk(e->wl_npass)), dim3(256),
+                                       0, st, (int)(e->wl_npass / 64), (const int32_t*)e->wl_pass,
+                                       (const int32_t*)e->wl_wrep_pass,
+                                       (const uint32_t*)e->wl_rcnt, (const EnsSeg*)e->d_seg, Q,
+                                       (const CatArgs<Real>*)e->d_cats, (int)e->cat_runs.size(),
+                                       ci, e->ctr, gstep, e->counts, e->delta, fold_n);
+                else
+                    hipLaunchKernelGGL((k_ens_cat<Real>), tgrid, dim3(256), 0, st,
+                                       (const EnsTile*)e->d_tiles, ntiles,
+                                       (const EnsSeg*)e->d_seg, Q,
+                                       (const CatArgs<Real>*)e->d_cats, (int)e->cat_runs.size(),
+                                       ci, e->ctr, gstep, e->counts, e->delta, fold_n);
+            }
             const edm_reaction& r0 = e->rx[start];
             const bool defer =
                 r0.fb1.sensor >= 0 && (r0.fb1.sensor == r0.b || r0.fb1.sensor == r0.c);
-            if (defer && e->n_counts)
-                hipLaunchKernelGGL(k_fold, dim3(nblk(e->n_counts)), dim3(kBlock), 0, st,
-                                   (int)e->n_counts, e->counts, e->delta);
+            if (defer && e->n_counts) {
+                if (folds_in((size_t)start + (size_t)e->cat_runs[ci].second))
+                    fold_pending = true;
+                else
+                    hipLaunchKernelGGL(k_fold, dim3(nblk(e->n_counts)), dim3(kBlock), 0, st,
+                                       (int)e->n_counts, e->counts, e->delta);
+            }
         } else if (r.kind == EDM_RX_RECRUIT || r.kind == EDM_RX_DETACH ||
                    r.kind == EDM_RX_CONVERT) {
             if (e->group_start[ri] < 0) continue;
@@ -1149,6 +1175,9 @@ static int ens_step_t(edm_ensemble* e) {
                                    (int)e->n_counts, e->counts, e->delta);
         }
     }
+    if (fold_pending && e->n_counts)
+        hipLaunchKernelGGL(k_fold, dim3(nblk(e->n_counts)), dim3(kBlock), 0, st,
+                           (int)e->n_counts, e->counts, e->delta);
     std::swap(e->a.x, e->a.xn);
     std::swap(e->a.y, e->a.yn);
     e->gstep += 1;
