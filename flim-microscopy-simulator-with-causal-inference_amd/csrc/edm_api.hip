@@ -147,7 +147,7 @@ int edm_destroy(edm_membrane* m) {
                     m->sv_val, m->sv_val2, m->asg_px, m->asg_py, m->asg_flag, m->asg_pos,
                     m->asg_t, m->asg_ch, m->sv_px, m->sv_py, m->pix_tot, m->d_scal,
                     m->dye_ids, m->wl, m->wl_cnt, m->fstart, m->fkey, m->tg_cnt, m->tg_fill, m->tg_start,
-                    m->tg_x, m->tg_y, m->tg_u};
+                    m->tg_x, m->tg_y, m->tg_u, m->scan_state, m->scan_ticket};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (auto e : m->kev) hipEventDestroy(e);
@@ -337,6 +337,7 @@ int edm_download_state(edm_membrane* m, edm_host_state* s) {
     if (!m || !s) return invalid("edm_download_state: null argument");
     const int n = m->n;
     if (int rc = dyn_check_overflow(m)) return rc;
+    if (int rc = dyn_check_scan(m)) return rc;
     std::vector<double> x(n), y(n);
     if (m->fp64) {
         EDM_HIP(hipMemcpy(x.data(), m->a.x, n * 8, hipMemcpyDeviceToHost));

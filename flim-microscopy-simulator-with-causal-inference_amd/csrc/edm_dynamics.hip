@@ -242,9 +242,13 @@ __global__ void k_bin(int n, const Real* __restrict__ x, const Real* __restrict_
                       Real csy, int ncx, int ncy, uint32_t* __restrict__ key,
                       uint32_t* __restrict__ rank, uint32_t* __restrict__ cnt, int32_t* counts,
                       int n_counts, const int32_t* __restrict__ id, uint32_t* __restrict__ kid,
-                      int nsub, Real inv_h) {
+                      int nsub, Real inv_h, uint64_t* __restrict__ scan_state, int scan_tiles,
+                      uint32_t* __restrict__ scan_ticket) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < n_counts) counts[s] = 0;  // density counts are rebuilt each step
+    // the cell scan that follows starts from cleared tile states and ticket
+    if (s < scan_tiles) scan_state[s] = 0ull;
+    if (s < 2 && scan_ticket) scan_ticket[s] = 0u;
     const bool live = s < n;          // live lanes are a prefix of the wavefront
     uint32_t c = 0xffffffffu;
     if (live) {
@@ -279,6 +283,114 @@ __global__ void k_bin(int n, const Real* __restrict__ x, const Real* __restrict_
     }
     base = __shfl(base, rs);
     if (live) rank[s] = base + (uint32_t)(lane - rs);
+}
+
+// Exclusive scan of the n cell counts into the cell starts in one launch:
+// tiles of kScanTile counts, each workgroup takes the next tile by ticket
+// (so every earlier tile's workgroup has started), publishes its sum, then
+// one wavefront looks back over the earlier tiles' published sums, 64 at a
+// time, until one carries its inclusive prefix, and publishes its own.  A tile state is (flag << 32 |
+// sum): 0 not yet, 1 the tile's own sum, 2 the inclusive prefix.  The
+// states and the ticket are cleared by k_bin of the same step.  A wait
+// that does not end within kScanSpin reads sets fail[0] and gives up
+// (wrong starts rather than a hang; it does not happen while every
+// started workgroup runs).
+static constexpr int kScanTile = 2048;
+static constexpr uint32_t kScanSpin = 1u << 20;
+
+__global__ __launch_bounds__(256) void k_cell_scan(int n, const uint32_t* __restrict__ cnt,
+                                                   uint32_t* __restrict__ start,
+                                                   uint64_t* state, uint32_t* ticket) {
+    __shared__ int s_tile;
+    __shared__ uint32_t s_wsum[4];
+    __shared__ uint32_t s_prefix;
+    if (threadIdx.x == 0) s_tile = (int)atomicAdd(&ticket[0], 1u);
+    __syncthreads();
+    const int tile = s_tile;
+    const int lane = lane_id(), wave = (int)(threadIdx.x >> 6);
+    const int base = tile * kScanTile + (int)threadIdx.x * 8;
+    uint32_t v[8];
+    if (base + 8 <= n) {
+        const uint4 a = *reinterpret_cast<const uint4*>(cnt + base);
+        const uint4 b = *reinterpret_cast<const uint4*>(cnt + base + 4);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = base + j < n ? cnt[base + j] : 0u;
+    }
+    uint32_t tsum = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) tsum += v[j];
+    uint32_t incl = tsum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) s_wsum[wave] = incl;
+    __syncthreads();
+    uint32_t wexcl = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        const uint32_t ws = s_wsum[w];
+        if (w < wave) wexcl += ws;
+        total += ws;
+    }
+    if (wave == 0) {
+        // look back 64 tiles at a time: lane l reads tile (hi - l); the
+        // nearest tile with its inclusive prefix ends the walk
+        uint32_t prefix = 0;
+        if (tile > 0) {
+            if (lane == 0)
+                __hip_atomic_store(&state[tile], (1ull << 32) | total, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            bool gave_up = false;
+            for (int hi = tile - 1; hi >= 0; hi -= 64) {
+                const int p = hi - lane;
+                uint64_t sv = 2ull << 32;  // before tile 0: an inclusive prefix of 0
+                if (p >= 0) {
+                    uint32_t spins = 0;
+                    do {
+                        sv = __hip_atomic_load(&state[p], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+                    } while ((sv >> 32) == 0u && ++spins <= kScanSpin);
+                    if ((sv >> 32) == 0u) gave_up = true;
+                }
+                const uint64_t inc = __ballot((sv >> 32) == 2u || gave_up);
+                const int stop = inc ? __builtin_ctzll(inc) : 63;
+                uint32_t val = lane <= stop ? (uint32_t)sv : 0u;
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) val += __shfl_xor(val, o);
+                prefix += val;
+                if (inc) break;
+            }
+            if (__ballot(gave_up) && lane == 0) atomicOr(&ticket[1], 1u);
+        }
+        if (lane == 0) {
+            __hip_atomic_store(&state[tile], (2ull << 32) | (prefix + total), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            s_prefix = prefix;
+        }
+    }
+    __syncthreads();
+    uint32_t run = s_prefix + wexcl + incl - tsum;
+    if (base + 8 <= n) {
+        uint32_t o[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            o[j] = run;
+            run += v[j];
+        }
+        *reinterpret_cast<uint4*>(start + base) = make_uint4(o[0], o[1], o[2], o[3]);
+        *reinterpret_cast<uint4*>(start + base + 4) = make_uint4(o[4], o[5], o[6], o[7]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            if (base + j < n) start[base + j] = run;
+            run += v[j];
+        }
+    }
 }
 
 __global__ void k_scatter(int n, const uint32_t* __restrict__ key, const uint32_t* __restrict__ rank,
@@ -1724,6 +1836,10 @@ static int create_t(edm_membrane* m) {
     EDM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b3, dummyi, dummyi, n, m->stream));
     m->cub_tmp_bytes = std::max(b1, std::max(b2, b3)) + 256;
     EDM_HIP(hipMalloc(&m->cub_tmp, m->cub_tmp_bytes));
+    m->scan_tiles = (m->ncell + 1 + kScanTile - 1) / kScanTile;
+    EDM_HIP(hipMalloc(&m->scan_state, (size_t)m->scan_tiles * sizeof(uint64_t)));
+    EDM_HIP(hipMalloc(&m->scan_ticket, 2 * sizeof(uint32_t)));
+    EDM_HIP(hipMemset(m->scan_ticket, 0, 2 * sizeof(uint32_t)));
     if (dalloc(&m->pair_cnt, n) || dalloc(&m->pair_off, n) || dalloc(&m->d_scal, 16)) return EDM_EHIP;
     EDM_HIP(hipMemset(m->d_scal, 0, 16 * sizeof(int64_t)));
     // parameters
@@ -2045,14 +2161,16 @@ static int begin_t(edm_membrane* m) {
         m->fine_nsub = q >= kMaxSub ? kMaxSub : std::max(1, (int)q);
         m->fine_inv_h = (double)m->fine_nsub / m->p.csy;
     }
-    hipLaunchKernelGGL(k_bin<Real>, dim3(nblk(std::max(n, m->n_count))), dim3(kBlock), 0, st, n,
-                       (const Real*)m->a.x, (const Real*)m->a.y, (Real)m->p.csx, (Real)m->p.csy,
-                       m->p.ncx, m->p.ncy, m->key, m->rank, m->cell_cnt, m->counts, m->n_count,
-                       (const int32_t*)m->a.id, m->fine_active ? m->fkey : nullptr,
-                       m->fine_nsub, (Real)m->fine_inv_h);
-    size_t tb = m->cub_tmp_bytes;
-    EDM_HIP(hipcub::DeviceScan::ExclusiveSum(m->cub_tmp, tb, m->cell_cnt, m->cell_start,
-                                             m->ncell + 1, st));
+    hipLaunchKernelGGL(k_bin<Real>, dim3(nblk(std::max({n, m->n_count, m->scan_tiles}))),
+                       dim3(kBlock), 0, st, n, (const Real*)m->a.x, (const Real*)m->a.y,
+                       (Real)m->p.csx, (Real)m->p.csy, m->p.ncx, m->p.ncy, m->key, m->rank,
+                       m->cell_cnt, m->counts, m->n_count, (const int32_t*)m->a.id,
+                       m->fine_active ? m->fkey : nullptr, m->fine_nsub, (Real)m->fine_inv_h,
+                       m->scan_state, m->scan_tiles, m->scan_ticket);
+    hipLaunchKernelGGL(k_cell_scan, dim3(m->scan_tiles), dim3(256), 0, st, m->ncell + 1,
+                       (const uint32_t*)m->cell_cnt, m->cell_start, m->scan_state,
+                       m->scan_ticket);
+    size_t tb = 0;
     hipLaunchKernelGGL(k_scatter, dim3(nblk(n)), dim3(kBlock), 0, st, n, m->key, m->rank,
                        m->cell_start, m->perm);
     if (m->fine_active) {
@@ -2142,8 +2260,22 @@ static int begin_t(edm_membrane* m) {
 
 int dyn_step_begin(edm_membrane* m) { return m->fp64 ? begin_t<double>(m) : begin_t<float>(m); }
 
+int dyn_check_scan(edm_membrane* m) {
+    if (!m->scan_ticket) return 0;
+    unsigned int f = 0;
+    EDM_HIP(hipMemcpyAsync(&f, m->scan_ticket + 1, sizeof(f), hipMemcpyDeviceToHost, m->stream));
+    EDM_HIP(hipStreamSynchronize(m->stream));
+    if (f) {
+        set_error("cell scan: a tile waited too long for an earlier tile's sum; the cell list "
+                  "of that step is wrong");
+        return EDM_EHIP;
+    }
+    return 0;
+}
+
 int dyn_check_overflow(edm_membrane* m) {
     if (!m->n_edges) return 0;
+    if (int rc = dyn_check_scan(m)) return rc;
     unsigned int ne[2];
     EDM_HIP(hipMemcpyAsync(ne, m->n_edges, sizeof(ne), hipMemcpyDeviceToHost, m->stream));
     EDM_HIP(hipStreamSynchronize(m->stream));

@@ -144,6 +144,11 @@ struct edm_membrane {
              *perm = nullptr;
     void* cub_tmp = nullptr;
     size_t cub_tmp_bytes = 0;
+    // the cell scan's tile states (flag << 32 | sum), its tile ticket and a
+    // flag set when a tile gave up waiting; cleared by k_bin every step
+    uint64_t* scan_state = nullptr;
+    uint32_t* scan_ticket = nullptr;  // [0] ticket, [1] gave-up flag
+    int scan_tiles = 0;
 
     // parameters (Real arrays)
     void *sig = nullptr, *eps = nullptr, *kh = nullptr, *Dbase = nullptr;
@@ -380,6 +385,8 @@ int dyn_step_begin(edm_membrane* m);
 int dyn_step_finish(edm_membrane* m, const double* d_draws_dev, bool hook);
 // EDM_ECAPACITY once a binding step overflowed its edge buffer (synchronises)
 int dyn_check_overflow(edm_membrane* m);
+// EDM_EHIP once a cell scan gave up waiting (synchronises)
+int dyn_check_scan(edm_membrane* m);
 int dyn_pairs(edm_membrane* m, int32_t* pairs, int64_t cap, int64_t* n_out);
 int dyn_density(edm_membrane* m, int32_t species, int32_t grid, double* out);
 int dyn_ensure_slot_of(edm_membrane* m);
